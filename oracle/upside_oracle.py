@@ -1,0 +1,1480 @@
+"""CPU oracle: a numpy restatement of the reference Upside ff_2.1 force engine.
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may use this module, and only as the checker.  The product
+library never calls it.
+
+It evaluates the `/input/potential` node graph of a `.up` file the way the
+reference's DerivEngine does (src/deriv_engine.cpp:239-260): every node's value
+in germ order, zeroing each coordinate node's sensitivity right after its value
+is computed, then every node's derivative push in deriv order.  `dtype`
+selects float32 (the reference's arithmetic) or float64 (the FP64
+re-evaluation).  Every node cites the reference code it restates.
+
+Pinning (see tests/test_oracle.py): whole-graph energies and forces against
+oracle/_ref (the reference engine compiled from its own sources) and against
+the reference-computed frames of example/15.SpatialTransformation (committed as
+tests/golden/chig_frames.npz); forces against central differences; BP against
+brute-force enumeration on tree graphs; splines against closed forms.
+"""
+import os
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(_HERE, "..", "upside2-md_amd"))
+import h5lite  # noqa: E402
+
+PotentialAndDerivMode, DerivMode = 1, 0
+N_BIT_ROT = 4
+
+
+# ----------------------------------------------------------------- helpers
+def fma32(a, b, c):
+    """float32 fused multiply-add, emulated in float64 (a*b is exact there)."""
+    return (np.asarray(a, np.float64) * b + c).astype(np.float32)
+
+
+def refine_dist2(d, dtype):
+    """Squared distance as the reference's pair-list refine computes it:
+    the Float4 sum sqr(x)+sqr(y)+sqr(z) (src/interaction_graph.h:239) which the
+    reference build contracts to fma(x,x, fma(z,z, y*y))."""
+    x, y, z = d[..., 0], d[..., 1], d[..., 2]
+    if dtype == np.float32:
+        x, y, z = x.astype(np.float32), y.astype(np.float32), z.astype(np.float32)
+        return fma32(x, x, fma32(z, z, y * y))
+    return x * x + y * y + z * z
+
+
+def ru(n):
+    return n if n == 1 else (n + 3) // 4 * 4
+
+
+def compact_sigmoid(x, sharpness):
+    """src/vector_math.h:701-720: value and derivative, 1 below -1/s, 0 above 1/s."""
+    y = x * sharpness
+    val = 0.25 * (y + 2) * (y - 1) * (y - 1)
+    der = (sharpness * 0.75) * (y * y - 1)
+    val = np.where(y < -1, 1.0, np.where(y > 1, 0.0, val))
+    der = np.where((y < -1) | (y > 1), 0.0, der)
+    return val.astype(x.dtype, copy=False), der.astype(x.dtype, copy=False)
+
+
+def sigmoid(x):
+    """src/vector_math.h:683-690: 1/(1+exp(x)) and its derivative."""
+    z = np.exp(-x)
+    w = 1.0 / (1.0 + z)
+    return w, z * w * w
+
+
+def debook_coeffs(y):
+    """Uniform cubic B-spline weights of the 4 coefficients for fraction y."""
+    y2, y3 = y * y, y * y * y
+    w0 = (1 - y) ** 3 / 6
+    w1 = (3 * y3 - 6 * y2 + 4) / 6
+    w2 = (-3 * y3 + 3 * y2 + 3 * y + 1) / 6
+    w3 = y3 / 6
+    d0 = -(1 - y) ** 2 / 2
+    d1 = (3 * y2 - 4 * y) / 2
+    d2 = (-3 * y2 + 2 * y + 1) / 2
+    d3 = y2 / 2
+    return (w0, w1, w2, w3), (d0, d1, d2, d3)
+
+
+def deboor(coef, x):
+    """Uniform cubic B-spline value and derivative (src/spline.h:98-127, 242-256).
+    coef: (E, K) coefficient rows (flat parameter memory, so the window may run
+    into the next block as in the reference); x >= 1: first spline centred at -1."""
+    xb = np.floor(x).astype(np.int64)  # x >= 0 here, truncation == floor
+    y = (x - xb).astype(x.dtype)
+    (w0, w1, w2, w3), (d0, d1, d2, d3) = debook_coeffs(y)
+    rows = np.arange(len(x))
+    c0 = coef[rows, xb - 1]
+    c1 = coef[rows, xb]
+    c2 = coef[rows, xb + 1]
+    c3 = coef[rows, xb + 2]
+    val = w0 * c0 + w1 * c1 + w2 * c2 + w3 * c3
+    der = d0 * c0 + d1 * c1 + d2 * c2 + d3 * c3
+    return val.astype(x.dtype), der.astype(x.dtype)
+
+
+def clamped_deboor(coef, x, n_knot):
+    """src/spline.h:326-358: constant outside [1, n_knot-2)."""
+    small = x < 1.0
+    big = x >= n_knot - 2
+    xc = np.where(small | big, np.ones_like(x), x)
+    val, der = deboor(coef, xc)
+    left = coef[:, 0] / 6 + coef[:, 1] * (2.0 / 3) + coef[:, 2] / 6
+    right = coef[:, n_knot - 3] / 6 + coef[:, n_knot - 2] * (2.0 / 3) + coef[:, n_knot - 1] / 6
+    val = np.where(small, left, np.where(big, right, val))
+    der = np.where(small | big, 0.0, der)
+    return val.astype(x.dtype), der.astype(x.dtype)
+
+
+def periodic_spline_1d(data):
+    """Coefficients (n,4) of the periodic interpolating cubic spline through
+    data[i] at integer i; row k is a + b f + c f^2 + d f^3 on [k, k+1).
+    Textbook circulant B-spline system (values = (s[i-1]+4 s[i]+s[i+1])/6),
+    solved exactly with the DFT; the reference solves the same system with a
+    Sherman-Morrison cyclic Thomas sweep (src/spline.cpp:33-76,121-156)."""
+    data = np.asarray(data, np.float64)
+    n = data.shape[-1]
+    k = np.arange(n)
+    eig = (4 + 2 * np.cos(2 * np.pi * k / n)) / 6
+    s = np.real(np.fft.ifft(np.fft.fft(data, axis=-1) / eig, axis=-1))
+    sm1, s0, s1, s2 = np.roll(s, 1, -1), s, np.roll(s, -1, -1), np.roll(s, -2, -1)
+    a = (sm1 + 4 * s0 + s1) / 6
+    b = (s1 - sm1) / 2
+    c = (sm1 - 2 * s0 + s1) / 2
+    d = (-sm1 + 3 * s0 - 3 * s1 + s2) / 6
+    return np.stack([a, b, c, d], -1)
+
+
+def periodic_spline_2d(data):
+    """(nx,ny,4,4) tensor-product periodic bicubic coefficients c[px][py]
+    (src/spline.cpp:262-292 layout: value = sum c[px*4+py] fx^px fy^py)."""
+    along_y = periodic_spline_1d(data)                          # (..., nx, ny, py)
+    along_x = periodic_spline_1d(np.moveaxis(along_y, -3, -1))  # (..., ny, py, nx, px)
+    a = along_x.ndim - 4
+    lead = tuple(range(a))
+    return np.transpose(along_x, lead + (a + 2, a, a + 3, a + 1))  # (..., nx, ny, px, py)
+
+
+def bicubic_eval(coef, layer, x, y):
+    """src/spline.h:63-82 on coefficients (L,nx,ny,4,4)."""
+    xb = np.floor(x).astype(np.int64)
+    yb = np.floor(y).astype(np.int64)
+    fx = (x - xb).astype(x.dtype)
+    fy = (y - yb).astype(x.dtype)
+    c = coef[layer, xb, yb]
+    fxp = np.stack([np.ones_like(fx), fx, fx * fx, fx * fx * fx], -1)
+    fyp = np.stack([np.ones_like(fy), fy, fy * fy, fy * fy * fy], -1)
+    dfx = np.stack([np.zeros_like(fx), np.ones_like(fx), 2 * fx, 3 * fx * fx], -1)
+    dfy = np.stack([np.zeros_like(fy), np.ones_like(fy), 2 * fy, 3 * fy * fy], -1)
+    val = np.einsum("epq,ep,eq->e", c, fxp, fyp)
+    dx = np.einsum("epq,ep,eq->e", c, dfx, fyp)
+    dy = np.einsum("epq,ep,eq->e", c, fxp, dfy)
+    return val.astype(x.dtype), dx.astype(x.dtype), dy.astype(x.dtype)
+
+
+def quat_to_rot(q):
+    """src/affine.h:84-96 (row-major)."""
+    a, b, c, d = q[..., 0], q[..., 1], q[..., 2], q[..., 3]
+    U = np.stack([a * a + b * b - c * c - d * d, 2 * b * c - 2 * a * d, 2 * b * d + 2 * a * c,
+                  2 * b * c + 2 * a * d, a * a - b * b + c * c - d * d, 2 * c * d - 2 * a * b,
+                  2 * b * d - 2 * a * c, 2 * c * d + 2 * a * b, a * a - b * b - c * c + d * d], -1)
+    return U.reshape(q.shape[:-1] + (3, 3))
+
+
+# ----------------------------------------------------------------- graph
+class Node:
+    potential_term = False
+
+    def __init__(self, eng, name, args):
+        self.eng, self.name, self.args = eng, name, args
+        self.T = eng.T
+
+    def arg(self, i):
+        return self.eng.nodes[self.args[i]]
+
+
+class CoordNode(Node):
+    def alloc(self, n, w):
+        self.n_elem, self.elem_width = n, w
+        self.output = np.zeros((n, w), self.T)
+        self.sens = np.zeros((n, w), self.T)
+
+
+class PotentialNode(Node):
+    potential_term = True
+    potential = 0.0
+
+    def propagate(self):
+        pass
+
+
+class PosNode(CoordNode):
+    def __init__(self, eng, n_atom):
+        Node.__init__(self, eng, "pos", [])
+        self.alloc(n_atom, 3)
+
+    def compute(self, mode):
+        pass
+
+    def propagate(self):
+        pass
+
+
+def _scatter(dst, idx, val):
+    np.add.at(dst, idx, val)
+
+
+class DistCoord(CoordNode):
+    """Distance3D (src/coord_basic.cpp:13-66)."""
+
+    def load(self, f, g):
+        self.id = f.read(g + "/id").astype(np.int64)
+        self.alloc(len(self.id), 1)
+
+    def compute(self, mode):
+        p1, p2 = self.arg(0).output, self.arg(1).output
+        disp = p1[self.id[:, 0]] - p2[self.id[:, 1]]
+        dist = np.sqrt((disp * disp).sum(-1))
+        self.output[:, 0] = dist
+        with np.errstate(invalid="ignore", divide="ignore"):
+            self.deriv = np.where(dist[:, None] != 0, disp / dist[:, None], 0).astype(self.T)
+
+    def propagate(self):
+        d = self.sens[:, :1] * self.deriv
+        _scatter(self.arg(0).sens, self.id[:, 0], d)
+        _scatter(self.arg(1).sens, self.id[:, 1], -d)
+
+
+class AngleCoord(CoordNode):
+    """Angle: cosine of the angle at atom 3 (src/coord_basic.cpp:183-243)."""
+
+    def load(self, f, g):
+        self.id = f.read(g + "/id").astype(np.int64)
+        self.alloc(len(self.id), 1)
+
+    def compute(self, mode):
+        p = self.arg(0).output
+        x1 = p[self.id[:, 0]] - p[self.id[:, 2]]
+        x2 = p[self.id[:, 1]] - p[self.id[:, 2]]
+        i1 = 1 / np.sqrt((x1 * x1).sum(-1, keepdims=True))
+        i2 = 1 / np.sqrt((x2 * x2).sum(-1, keepdims=True))
+        h1, h2 = x1 * i1, x2 * i2
+        dp = (h1 * h2).sum(-1, keepdims=True)
+        self.output[:, 0] = dp[:, 0]
+        self.d1 = (h2 - h1 * dp) * i1
+        self.d2 = (h1 - h2 * dp) * i2
+
+    def propagate(self):
+        s = self.sens[:, :1]
+        ps = self.arg(0).sens
+        _scatter(ps, self.id[:, 0], s * self.d1)
+        _scatter(ps, self.id[:, 1], s * self.d2)
+        _scatter(ps, self.id[:, 2], -s * (self.d1 + self.d2))
+
+
+def dihedral_germ(r1, r2, r3, r4):
+    """Blondel-Karplus dihedral and gradient (src/vector_math.h:764-797)."""
+    F, G, H = r1 - r2, r2 - r3, r4 - r3
+    A, B = np.cross(F, G), np.cross(H, G)
+    C = np.cross(B, A)
+    iA2 = 1 / (A * A).sum(-1, keepdims=True)
+    iB2 = 1 / (B * B).sum(-1, keepdims=True)
+    G2 = (G * G).sum(-1, keepdims=True)
+    iG = 1 / np.sqrt(G2)
+    Gm = G2 * iG
+    d1 = -Gm * iA2 * A
+    d4 = Gm * iB2 * B
+    fmid = (F * G).sum(-1, keepdims=True) * iA2 * iG * A - (H * G).sum(-1, keepdims=True) * iB2 * iG * B
+    d2 = -d1 + fmid
+    d3 = -d4 - fmid
+    ang = np.arctan2((C * G).sum(-1), (A * B).sum(-1) * Gm[:, 0])
+    return ang, d1, d2, d3, d4
+
+
+class DihedralCoord(CoordNode):
+    """Dihedral; rows with a -1 atom give -1.3963 and no derivative
+    (src/coord_basic.cpp:311-374)."""
+
+    def load(self, f, g):
+        self.id = f.read(g + "/id").astype(np.int64)
+        self.dummy = (self.id[:, 0] == -1) | (self.id[:, 3] == -1)
+        self.alloc(len(self.id), 1)
+
+    def compute(self, mode):
+        p = self.arg(0).output
+        ok = ~self.dummy
+        ids = self.id[ok]
+        ang, *d = dihedral_germ(*[p[ids[:, k]] for k in range(4)])
+        self.output[:, 0] = self.T(-1.3963)
+        self.output[ok, 0] = ang
+        self.jac = d
+
+    def propagate(self):
+        ok = ~self.dummy
+        s = self.sens[ok, :1]
+        ids = self.id[ok]
+        for k in range(4):
+            _scatter(self.arg(0).sens, ids[:, k], s * self.jac[k])
+
+
+class Spring(PotentialNode):
+    """Spring on one component of a coordinate node (src/spring.cpp:9-81)."""
+
+    def load(self, f, g):
+        self.id = f.read(g + "/id").astype(np.int64)
+        self.eq = f.read(g + "/equil_dist").astype(self.T)
+        self.k = f.read(g + "/spring_const").astype(self.T)
+        self.dim1 = int(f.attr(g, "dim1"))
+        self.pbc = int(f.attr(g, "pbc"))
+        self.box = self.T(f.attr(g, "box_len"))
+
+    def compute(self, mode):
+        x = self.arg(0).output[self.id, self.dim1]
+        ex = x - self.eq
+        if self.pbc:
+            for sh in (-self.box, self.box):
+                e2 = x - self.eq + sh
+                ex = np.where(e2 * e2 < ex * ex, e2, ex)
+        self.potential = (0.5 * self.k * ex * ex).sum()
+        _scatter(self.arg(0).sens[:, self.dim1], self.id, self.k * ex)
+
+
+class RamaCoord(CoordNode):
+    """rama_coord = (phi[index_pos1], psi[index_pos2]) (src/coord_map.cpp:69-110)."""
+
+    def load(self, f, g):
+        self.i1 = f.read(g + "/index_pos1").astype(np.int64)
+        self.i2 = f.read(g + "/index_pos2").astype(np.int64)
+        self.alloc(len(self.i1), 2)
+
+    def compute(self, mode):
+        self.output[:, 0] = self.arg(0).output[self.i1, 0]
+        self.output[:, 1] = self.arg(1).output[self.i2, 0]
+
+    def propagate(self):
+        _scatter(self.arg(0).sens[:, 0], self.i1, self.sens[:, 0])
+        _scatter(self.arg(1).sens[:, 0], self.i2, self.sens[:, 1])
+
+
+class CatPos(CoordNode):
+    """cat_pos (src/coord_map.cpp:112-161)."""
+
+    def load(self, f, g):
+        self.i1 = f.read(g + "/index_pos1").astype(np.int64)
+        self.i2 = f.read(g + "/index_pos2").astype(np.int64)
+        self.w = int(f.attr(g, "n_dim"))
+        self.alloc(len(self.i1) + len(self.i2), self.w)
+
+    def compute(self, mode):
+        n1 = len(self.i1)
+        self.output[:n1] = self.arg(0).output[self.i1, :self.w]
+        self.output[n1:] = self.arg(1).output[self.i2, :self.w]
+
+    def propagate(self):
+        n1 = len(self.i1)
+        _scatter(self.arg(0).sens[:, :self.w], self.i1, self.sens[:n1])
+        _scatter(self.arg(1).sens[:, :self.w], self.i2, self.sens[n1:])
+
+
+class RamaMapPot(PotentialNode):
+    """Per-residue periodic bicubic Rama potential (src/rama_map_pot.cpp:15-94)."""
+
+    def load(self, f, g):
+        self.res = f.read(g + "/residue_id").astype(np.int64)
+        self.mid = f.read(g + "/rama_map_id").astype(np.int64)
+        raw = f.read(g + "/rama_pot").astype(np.float64)
+        self.n = raw.shape[1]
+        self.coef = periodic_spline_2d(raw).astype(self.T)
+
+    def compute(self, mode):
+        r = self.arg(0).output[self.res]
+        scale = self.T(self.n * (np.float32(0.5) / np.float32(np.pi) - np.float32(1e-7)))
+        shift = self.T(np.float32(np.pi))
+        v, dx, dy = bicubic_eval(self.coef, self.mid, (r[:, 0] + shift) * scale, (r[:, 1] + shift) * scale)
+        self.residue_potential = v
+        self.potential = v.sum()
+        s = self.arg(0).sens
+        _scatter(s[:, 0], self.res, dx * scale)
+        _scatter(s[:, 1], self.res, dy * scale)
+
+
+_R4 = [[0, 1, 2, 3], [1, 4, 5, 6], [2, 5, 7, 8], [3, 6, 8, 9]]  # packed upper triangle
+
+
+def _eig4_group(A, T, tol=1e-5, max_iter=100):
+    """Symmetric 4x4 eigen-decomposition of the lanes of one 4-wide group, as
+    the reference's symm_QR_4x4 does it (src/eig.cpp:75-273): Householder
+    tridiagonalisation (Golub & Van Loan 8.3.1), then implicit symmetric QR
+    steps with Wilkinson shift (8.3.2).  Off-diagonals are zeroed per lane when
+    |u_i| <= tol(|d_i|+|d_i+1|); the deflation block is chosen from the union
+    over the group's lanes, so lanes iterate together.
+    A: (4 lanes, 10) packed upper triangles.  Returns d (4,4) and rows of
+    eigenvectors rot (4,4,4)."""
+    A = A.astype(T).copy()
+    one, two, half = T(1), T(2), T(0.5)
+    beta = []
+    for k in range(2):
+        m = 3 - k
+        x = [A[:, _R4[k][k + 1 + j]] for j in range(m)]
+        sigma2 = T(1e-20) + sum(x[i] * x[i] for i in range(1, m))
+        mu = np.sqrt(x[0] * x[0] + sigma2)
+        sv = np.where(T(0) < x[0], -sigma2 / (x[0] + mu), x[0] - mu)
+        b = two * sv * sv / (sigma2 + sv * sv)
+        A[:, _R4[k][k + 1]] = mu
+        for i in range(1, m):
+            A[:, _R4[k][k + 1 + i]] = x[i] * (one / sv)
+        beta.append(b)
+        v = [one] + [A[:, _R4[k][k + 1 + j]] for j in range(1, m)]
+        p = []
+        for i in range(m):
+            acc = np.zeros_like(mu)
+            for j in range(m):
+                acc = acc + A[:, _R4[i + k + 1][j + k + 1]] * v[j]
+            p.append(acc * b)
+        pdv = sum(p[i] * v[i] for i in range(m))
+        w = [p[i] - (half * b * pdv) * v[i] for i in range(m)]
+        for i in range(m):
+            for j in range(i, m):
+                A[:, _R4[i + k + 1][j + k + 1]] -= v[i] * w[j] + v[j] * w[i]
+    d = np.stack([A[:, _R4[i][i]] for i in range(4)], -1)
+    u = np.stack([A[:, _R4[i][i + 1]] for i in range(3)], -1)
+    rot = np.zeros((A.shape[0], 4, 4), T)
+    for i in range(4):
+        rot[:, i, i] = 1
+    v0 = [one, A[:, _R4[0][2]], A[:, _R4[0][3]]]
+    v1 = [one, A[:, _R4[1][3]]]
+    for i in range(1, 4):
+        for j in range(i, 4):
+            rot[:, i, j] -= beta[0] * (v0[i - 1] * v0[j - 1])
+    for i in range(2, 4):
+        for j in range(i, 4):
+            rot[:, i, j] -= beta[1] * (v1[i - 2] * v1[j - 2])
+    for i in range(4):
+        for j in range(i + 1, 4):
+            rot[:, j, i] = rot[:, i, j]
+    coeff = beta[0] * beta[1] * (v0[1] * v1[0] + v0[2] * v1[1])
+    for i in range(2, 4):
+        for j in range(1, 4):
+            rot[:, i, j] += coeff * v1[i - 2] * v0[j - 1]
+    for _ in range(max_iter):
+        for i in range(3):
+            small = np.abs(u[:, i]) <= T(tol) * (np.abs(d[:, i]) + np.abs(d[:, i + 1]))
+            u[:, i] = np.where(small, T(0), u[:, i])
+        if np.any(u[:, 2] != 0):
+            q = 0
+        elif np.any(u[:, 1] != 0):
+            q = 1
+        elif np.any(u[:, 0] != 0):
+            q = 2
+        else:
+            break
+        p = 4 - q - 1
+        while p > 0:
+            if not np.any(u[:, p - 1] != 0):
+                break
+            p -= 1
+        n = 4 - q - p
+        dd, uu = d[:, p:p + n], u[:, p:p + n - 1]
+        dval = half * (dd[:, n - 2] - dd[:, n - 1]) + T(1e-20)
+        un = uu[:, n - 2]
+        mu = dd[:, n - 1] - un * un / (dval + np.copysign(np.sqrt(dval * dval + un * un), dval))
+        x = dd[:, 0] - mu
+        z = uu[:, 0].copy()
+        for k in range(n - 1):
+            trivial = z == 0
+            with np.errstate(divide="ignore", invalid="ignore"):
+                inv_r = one / np.sqrt(x * x + z * z)
+            c = np.where(trivial, one, x * inv_r)
+            s_ = np.where(trivial, T(0), -z * inv_r)
+            t1 = rot[:, p + k].copy()
+            t2 = rot[:, p + k + 1].copy()
+            rot[:, p + k] = c[:, None] * t1 - s_[:, None] * t2
+            rot[:, p + k + 1] = s_[:, None] * t1 + c[:, None] * t2
+            if k > 0:
+                uu[:, k - 1] = c * x - s_ * z
+            T00, T11, T01 = dd[:, k].copy(), dd[:, k + 1].copy(), uu[:, k].copy()
+            dd[:, k] = T00 * c * c - T01 * two * c * s_ + T11 * s_ * s_
+            dd[:, k + 1] = T00 * s_ * s_ + T01 * two * c * s_ + T11 * c * c
+            uu[:, k] = (T00 - T11) * c * s_ + T01 * (c * c - s_ * s_)
+            x = uu[:, k].copy()
+            if k < n - 2:
+                z = -uu[:, k + 1] * s_
+                uu[:, k + 1] *= c
+        d[:, p:p + n], u[:, p:p + n - 1] = dd, uu
+    for i in range(1, 4):  # largest eigenvalue into slot 0
+        flip = d[:, 0] < d[:, i]
+        d0, di = np.where(flip, d[:, i], d[:, 0]), np.where(flip, d[:, 0], d[:, i])
+        d[:, 0], d[:, i] = d0, di
+        r0 = np.where(flip[:, None], rot[:, i], rot[:, 0])
+        ri = np.where(flip[:, None], rot[:, 0], rot[:, i])
+        rot[:, 0], rot[:, i] = r0, ri
+    return d, rot
+
+
+def symm_eig4(F, T):
+    """Eigenvalues (slot 0 the largest) and eigenvector rows of symmetric 4x4
+    matrices F (n,4,4), residues processed in groups of 4 lanes as the
+    reference's AffineAlignment does (src/eig.cpp:346-373, padding lanes copy
+    lane 0)."""
+    n = F.shape[0]
+    packed = np.stack([F[:, i, j] for i in range(4) for j in range(i, 4)], -1)
+    ng = (n + 3) // 4
+    pad = np.concatenate([packed, np.repeat(packed[[4 * (n // 4)] if n % 4 else [0]], 4 * ng - n, 0)]) if 4 * ng != n else packed
+    d = np.zeros((4 * ng, 4), T)
+    rot = np.zeros((4 * ng, 4, 4), T)
+    for g in range(ng):
+        d[4 * g:4 * g + 4], rot[4 * g:4 * g + 4] = _eig4_group(pad[4 * g:4 * g + 4], T)
+    return d[:n], rot[:n]
+
+
+class AffineAlignment(CoordNode):
+    """Rigid frame (centre, quaternion) of N,CA,C per residue
+    (src/eig.cpp:277-471)."""
+
+    def load(self, f, g):
+        self.atoms = f.read(g + "/atoms").astype(np.int64)
+        self.ref = f.read(g + "/ref_geom").astype(self.T)
+        self.alloc(len(self.atoms), 7)
+
+    def compute(self, mode):
+        p = self.arg(0).output
+        x = p[self.atoms]  # (n,3,3)
+        center = (x[:, 0] + x[:, 1] + x[:, 2]) * self.T(1.0 / 3.0)
+        x = x - center[:, None]
+        R = np.einsum("naj,nai->nij", x, self.ref)  # R(i,j) = sum_a atom_a[j]*ref_a[i]
+        F = np.zeros((len(x), 4, 4), self.T)
+        F[:, 0, 0] = R[:, 0, 0] + R[:, 1, 1] + R[:, 2, 2]
+        F[:, 0, 1] = R[:, 1, 2] - R[:, 2, 1]
+        F[:, 0, 2] = R[:, 2, 0] - R[:, 0, 2]
+        F[:, 0, 3] = R[:, 0, 1] - R[:, 1, 0]
+        F[:, 1, 1] = R[:, 0, 0] - R[:, 1, 1] - R[:, 2, 2]
+        F[:, 1, 2] = R[:, 0, 1] + R[:, 1, 0]
+        F[:, 1, 3] = R[:, 0, 2] + R[:, 2, 0]
+        F[:, 2, 2] = -R[:, 0, 0] + R[:, 1, 1] - R[:, 2, 2]
+        F[:, 2, 3] = R[:, 1, 2] + R[:, 2, 1]
+        F[:, 3, 3] = -R[:, 0, 0] - R[:, 1, 1] + R[:, 2, 2]
+        iu = np.triu_indices(4, 1)
+        F[:, iu[1], iu[0]] = F[:, iu[0], iu[1]]
+        self.evals, self.evecs = symm_eig4(F, self.T)
+        self.output[:, :3] = center
+        self.output[:, 3:7] = self.evecs[:, 0]
+
+    def propagate(self):
+        ev, E = self.evals, self.evecs
+        s = self.sens
+        sens3, torque = s[:, 0:3], s[:, 3:6]
+        q = E[:, 0]
+        quat_sens = 2 * np.stack([
+            -torque[:, 0] * q[:, 1] - torque[:, 1] * q[:, 2] - torque[:, 2] * q[:, 3],
+            torque[:, 0] * q[:, 0] + torque[:, 1] * q[:, 3] - torque[:, 2] * q[:, 2],
+            torque[:, 1] * q[:, 0] + torque[:, 2] * q[:, 1] - torque[:, 0] * q[:, 3],
+            torque[:, 2] * q[:, 0] + torque[:, 0] * q[:, 2] - torque[:, 1] * q[:, 1]], -1)
+        qsd = np.einsum("ni,ndi->nd", quat_sens, E)  # projection on each eigenvector
+        inv = 1 / (ev[:, :1] - ev[:, 1:])            # k = 1..3
+        ps = self.arg(0).sens
+        for na in range(3):
+            g = self.ref[:, na]
+            z = np.zeros_like(g[:, 0])
+            gx, gy, gz = g[:, 0], g[:, 1], g[:, 2]
+            # dF/dx_j as symmetric 4x4 (upper triangle f00..f33), src/eig.cpp:452-467
+            mats = [
+                (gx, z, gz, -gy, gx, gy, gz, -gx, z, -gx),
+                (gy, -gz, z, gx, -gy, gx, z, gy, gz, -gy),
+                (gz, gy, -gx, z, -gz, z, gx, -gz, gy, gz)]
+            deriv = np.repeat(sens3[:, None, :] * self.T(1.0 / 3.0), 1, axis=1)[:, 0].copy()
+            for j in range(3):
+                f00, f01, f02, f03, f11, f12, f13, f22, f23, f33 = mats[j]
+                M = np.stack([np.stack([f00, f01, f02, f03], -1), np.stack([f01, f11, f12, f13], -1),
+                              np.stack([f02, f12, f22, f23], -1), np.stack([f03, f13, f23, f33], -1)], -2)
+                # first-order perturbation of the top eigenvector: sum_k <e_k|M|e_0>/(l0-lk) e_k
+                tk = np.einsum("nki,nij,nj->nk", E[:, 1:], M, E[:, 0])
+                deriv[:, j] += (inv * tk * qsd[:, 1:]).sum(-1)
+            _scatter(ps, self.atoms[:, na], deriv)
+
+
+class Placement(CoordNode):
+    """PlacementNode (src/placement.cpp:233-314) with RamaPlacement (bicubic in
+    phi,psi, :20-99) or FixedPlacement (:102-167) data; the signature is
+    POINT,VECTOR[,SCALAR] or a single SCALAR."""
+
+    def load(self, f, g):
+        self.aff_res = f.read(g + "/affine_residue").astype(np.float64).astype(np.int64)
+        self.layer = f.read(g + "/layer_index").astype(np.int64)
+        data = f.read(g + "/placement_data")
+        self.rama = data.ndim == 4
+        if self.rama:
+            self.rama_res = f.read(g + "/rama_residue").astype(np.int64)
+            self.nx, self.ny = data.shape[1], data.shape[2]
+            self.dim = data.shape[3]
+            assert self.dim == 1
+            self.coef = periodic_spline_2d(data[..., 0].astype(np.float64)).astype(self.T)
+        else:
+            self.data = data.astype(np.float32).astype(self.T)
+            self.dim = data.shape[1]
+        self.alloc(len(self.layer), self.dim)
+
+    def values(self):
+        if self.rama:
+            r = self.arg(1).output[self.rama_res]
+            sx = self.T(self.nx * (np.float32(0.5) / np.float32(np.pi) - np.float32(1e-7)))
+            sy = self.T(self.ny * (np.float32(0.5) / np.float32(np.pi) - np.float32(1e-7)))
+            sh = self.T(np.float32(np.pi))
+            v, dx, dy = bicubic_eval(self.coef, self.layer, (r[:, 0] + sh) * sx, (r[:, 1] + sh) * sy)
+            self.rd = (dx * sx, dy * sy)
+            return v[:, None]
+        return self.data[self.layer]
+
+    def compute(self, mode):
+        aff = self.arg(0).output[self.aff_res]
+        t = aff[:, :3]
+        U = quat_to_rot(aff[:, 3:7])
+        val = self.values()
+        out = self.output
+        if self.dim == 1:
+            out[:] = val
+        else:
+            out[:, 0:3] = np.einsum("nij,nj->ni", U, val[:, 0:3]) + t
+            out[:, 3:6] = np.einsum("nij,nj->ni", U, val[:, 3:6])
+            if self.dim == 7:
+                out[:, 6] = val[:, 6]
+        self.U, self.t = U, t
+
+    def propagate(self):
+        s = self.sens
+        a_sens = self.arg(0).sens
+        if self.dim == 1:
+            if self.rama:
+                rs = self.arg(1).sens
+                _scatter(rs[:, 0], self.rama_res, s[:, 0] * self.rd[0])
+                _scatter(rs[:, 1], self.rama_res, s[:, 0] * self.rd[1])
+            return
+        x = self.output
+        sp, sv = s[:, 0:3], s[:, 3:6]
+        com = sp
+        torque = np.cross(x[:, 0:3] - self.t, sp) + np.cross(x[:, 3:6], sv)
+        _scatter(a_sens[:, 0:3], self.aff_res, com)
+        _scatter(a_sens[:, 3:6], self.aff_res, torque)
+
+
+class InferHO(CoordNode):
+    """Virtual H and O sites (src/hbond.cpp:20-127)."""
+
+    def load(self, f, g):
+        did = f.read(g + "/donors/id").astype(np.int64)
+        aid = f.read(g + "/acceptors/id").astype(np.int64)
+        self.ids = np.concatenate([did, aid])
+        self.bl = np.concatenate([f.read(g + "/donors/bond_length"), f.read(g + "/acceptors/bond_length")]).astype(np.float32).astype(self.T)
+        self.n_donor = len(did)
+        self.alloc(len(self.ids), 6)
+
+    def compute(self, mode):
+        p = self.arg(0).output
+        prev_c, curr, next_c = p[self.ids[:, 0]], p[self.ids[:, 1]], p[self.ids[:, 2]]
+
+        def unit(v):
+            im = 1 / np.sqrt((v * v).sum(-1, keepdims=True))
+            return v * im, im
+        prev, pim = unit(prev_c - curr)
+        nxt, nim = unit(next_c - curr)
+        disp, dim = unit(prev + nxt)
+        self.cache = (prev, pim, nxt, nim, disp, dim)
+        self.output[:, 0:3] = curr - self.bl[:, None] * disp
+        self.output[:, 3:6] = -disp
+
+    def propagate(self):
+        prev, pim, nxt, nim, disp, dim = self.cache
+        sp, sd = self.sens[:, 0:3], self.sens[:, 3:6]
+        s_neg = sd + self.bl[:, None] * sp
+
+        def back(u, im, s):
+            return im * ((u * s).sum(-1, keepdims=True) * u - s)
+        s_disp = back(disp, dim, s_neg)
+        s_prev = -back(prev, pim, s_disp)
+        s_next = -back(nxt, nim, s_disp)
+        ps = self.arg(0).sens
+        _scatter(ps, self.ids[:, 0], s_prev)
+        _scatter(ps, self.ids[:, 1], sp - s_prev - s_next)
+        _scatter(ps, self.ids[:, 2], s_next)
+
+
+# --------------------------------------------------- pair interactions
+def quadspline(p, x1, x2, n_ka, n_k, inv_dtheta, inv_dx):
+    """bead_interaction.h:31-85 for arrays of edges.  p: (E, n_param) params."""
+    T = x1.dtype.type
+    disp = x2[:, 0:3] - x1[:, 0:3]
+    r1, r2 = x1[:, 3:6], x2[:, 3:6]
+    d2 = (disp * disp).sum(-1)
+    inv = 1 / np.sqrt(d2)
+    dist_coord = d2 * (inv * T(inv_dx))
+    u = inv[:, None] * disp
+    c1 = (r1 * u).sum(-1)
+    c2 = -(r2 * u).sum(-1)
+    a1, da1 = deboor(p, (c1 + 1) * T(inv_dtheta) + 1)
+    a2, da2 = deboor(p[:, n_ka:], (c2 + 1) * T(inv_dtheta) + 1)
+    w, dw = clamped_deboor(p[:, 2 * n_ka:], dist_coord, n_k)
+    nn, dn = clamped_deboor(p[:, 2 * n_ka + n_k:], dist_coord, n_k)
+    aw = a1 * a2
+    rad = T(inv_dx) * (dw + aw * dn)
+    ad1 = T(inv_dtheta) * da1 * a2 * nn
+    ad2 = T(inv_dtheta) * a1 * da2 * nn
+    rXX = ad1[:, None] * r1 - ad2[:, None] * r2
+    ddir = inv[:, None] * (rXX - (u * rXX).sum(-1, keepdims=True) * u)
+    dd = rad[:, None] * u + ddir
+    d1 = np.concatenate([-dd, ad1[:, None] * u], -1)
+    d2_ = np.concatenate([dd, -ad2[:, None] * u], -1)
+    return w + aw * nn, d1, d2_
+
+
+class IGraph:
+    """InteractionGraph (src/interaction_graph.h:262-565): brute-force edge set
+    {acceptable(id1,id2) and refine_dist2 < cutoff^2} (the Verlet cache of
+    :51-169 always holds a superset of it), per-edge values/derivatives, and
+    the scatter of sens*deriv into the producer nodes."""
+
+    def __init__(self, f, g, T, node1, node2, itype):
+        self.T = T
+        self.sym = node2 is None
+        s = "" if self.sym else "1"
+        self.loc1 = f.read(g + "/index" + s).astype(np.int64)
+        self.type1 = f.read(g + "/type" + s).astype(np.int64)
+        self.id1 = f.read(g + "/id" + s).astype(np.float64).astype(np.int64)
+        if self.sym:
+            self.loc2, self.type2, self.id2 = self.loc1, self.type1, self.id1
+        else:
+            self.loc2 = f.read(g + "/index2").astype(np.int64)
+            self.type2 = f.read(g + "/type2").astype(np.int64)
+            self.id2 = f.read(g + "/id2").astype(np.float64).astype(np.int64)
+        ip = f.read(g + "/interaction_param").astype(np.float32)
+        self.n_type2 = ip.shape[1]
+        self.param = ip.reshape(ip.shape[0] * ip.shape[1], -1).astype(T)
+        self.flat_param = np.concatenate([self.param.ravel(), np.zeros(64, T)])
+        self.n_param = ip.shape[2]
+        self.node1, self.node2 = node1, (node1 if self.sym else node2)
+        self.it = itype
+        self.cutoff = np.float32(max(itype.cutoff(row) for row in ip.reshape(-1, ip.shape[2])))
+
+    def edges(self):
+        it = self.it
+        x1 = self.node1.output[self.loc1, :it.n_dim1]
+        x2 = self.node2.output[self.loc2, :it.n_dim2]
+        d = x1[:, None, :3] - x2[None, :, :3]
+        d2 = refine_dist2(d, self.T)
+        cut2 = np.float32(self.cutoff) * np.float32(self.cutoff)
+        ok = d2 < (cut2 if self.T == np.float32 else np.float64(cut2))
+        ok &= it.acceptable(self.id1[:, None], self.id2[None, :])
+        if self.sym:
+            ok &= np.triu(np.ones_like(ok), 1)
+        i1, i2 = np.nonzero(ok)
+        self.i1, self.i2 = i1, i2
+        self.x1, self.x2 = x1[i1], x2[i2]
+        return i1, i2
+
+    def params_of_edges(self):
+        pidx = self.type1[self.i1] * self.n_type2 + self.type2[self.i2]
+        # rows are windows into the flat parameter memory (reads may run past
+        # one block into the next, exactly as the reference's pointer math)
+        base = pidx * self.n_param
+        cols = np.arange(self.n_param + 4)
+        return self.flat_param[base[:, None] + cols[None, :]]
+
+    def compute(self):
+        self.edges()
+        self.value, self.d1, self.d2 = self.it.compute(self.params_of_edges(), self.x1, self.x2)
+        return self.value
+
+    def propagate(self, sens):
+        it = self.it
+        s1 = self.node1.sens
+        s2 = self.node2.sens
+        _scatter(s1[:, :it.n_dim1], self.loc1[self.i1], sens[:, None] * self.d1)
+        _scatter(s2[:, :it.n_dim2], self.loc2[self.i2], sens[:, None] * self.d2)
+
+
+def _excl(k):
+    return lambda a, b: (np.abs(a - b) > k)
+
+
+class RotamerPair:
+    """PosQuadSplineInteraction (src/bead_interaction.h:184-221)."""
+    n_dim1 = n_dim2 = 6
+    n_ka, n_k, spacing = 15, 12, 0.7
+
+    def __init__(self):
+        self.inv_dx = np.float32(1.0) / np.float32(self.spacing)
+        self.inv_dtheta = (self.n_ka - 3) / 2.0
+
+    def cutoff(self, p):
+        return np.float32((self.n_k - 2 - 1e-6) / np.float64(self.inv_dx))
+
+    @staticmethod
+    def acceptable(a, b):
+        return (a >> N_BIT_ROT) != (b >> N_BIT_ROT)
+
+    def compute(self, p, x1, x2):
+        return quadspline(p, x1, x2, self.n_ka, self.n_k, self.inv_dtheta, self.inv_dx)
+
+
+class HBCoverage:
+    """HBondCoverageInteraction (src/hbond.cpp:249-294)."""
+    n_dim1, n_dim2 = 7, 6
+    n_ka, n_k, spacing = 15, 10, 0.625
+    acceptable = staticmethod(_excl(2))
+
+    def __init__(self):
+        self.inv_dx = np.float32(1.0) / np.float32(self.spacing)
+        self.inv_dtheta = (self.n_ka - 3) / 2.0
+
+    def cutoff(self, p):
+        return np.float32((self.n_k - 2 - 1e-6) / np.float64(self.inv_dx))
+
+    def compute(self, p, x1, x2):
+        T = x1.dtype.type
+        cov, d1, d2 = quadspline(p, x1, x2, self.n_ka, self.n_k, self.inv_dtheta, self.inv_dx)
+        one_m = 1 - x1[:, 6]
+        pref = one_m * one_m
+        d1 = np.concatenate([d1 * pref[:, None], (-cov * one_m * T(2))[:, None]], -1)
+        return pref * cov, d1, d2 * pref[:, None]
+
+
+class ProteinHB:
+    """ProteinHBondInteraction (src/hbond.cpp:158-246)."""
+    n_dim1 = n_dim2 = 6
+    acceptable = staticmethod(_excl(1))
+
+    def cutoff(self, p):
+        return np.float32(3.5)
+
+    def compute(self, p, x1, x2):
+        T = x1.dtype.type
+        H, O, rHN, rOC = x1[:, 0:3], x2[:, 0:3], x1[:, 3:6], x2[:, 3:6]
+        HO = H - O
+        m2 = (HO * HO).sum(-1) + T(1e-6)
+        inv = 1 / np.sqrt(m2)
+        mag = m2 * inv
+        rHO = HO * inv[:, None]
+        dHOC = (rHO * rOC).sum(-1)
+        dOHN = -(rHO * rHN).sum(-1)
+        within = (dHOC > 0) & (dOHN > 0)
+        p0, p1, p2, p3, p4, p5 = [p[:, i] for i in range(6)]
+        os_, osd = sigmoid((p2 - mag) * p3)
+        is_, isd = sigmoid((mag - p0) * p1)
+        rad = os_ * is_
+        drad = -p3 * osd * is_ + p1 * isd * os_
+        a1, da1 = sigmoid((dHOC - p4) * p5)
+        a2, da2 = sigmoid((dOHN - p4) * p5)
+        da1, da2 = p5 * da1, p5 * da2
+        hb = rad * a1 * a2
+        c0 = drad * a1 * a2
+        c1 = rad * da1 * a2
+        c2 = -rad * a1 * da2
+        drOC = c1[:, None] * rHO
+        drHN = c2[:, None] * rHO
+        dH = (c0[:, None] * rHO + (c1 * inv)[:, None] * (rOC - dHOC[:, None] * rHO)
+              + (c2 * inv)[:, None] * (rHN + dOHN[:, None] * rHO))
+        w = within[:, None]
+        hb = np.where(within, hb, 0).astype(T)
+        dH, drHN, drOC = [np.where(w, v, 0).astype(T) for v in (dH, drHN, drOC)]
+        with np.errstate(divide="ignore"):
+            hb_log = np.where(hb >= 1, T(100), -np.log(1 - hb)).astype(T)
+            pref = np.minimum(1 / (1 - hb), T(1e5)).astype(T)
+        d1 = np.concatenate([dH, drHN], -1) * pref[:, None]
+        d2 = np.concatenate([-dH, drOC], -1) * pref[:, None]
+        return hb_log, d1, d2
+
+
+class EnvCoverage:
+    """EnvironmentCoverageInteraction (src/environment.cpp:114-170); with
+    weighted=False, HbondEnvironmentCoverageInteraction (:172-228)."""
+    acceptable = staticmethod(_excl(2))
+
+    def __init__(self, weighted):
+        self.weighted = weighted
+        self.n_dim1 = 6
+        self.n_dim2 = 4 if weighted else 6
+
+    def cutoff(self, p):
+        return np.float32(np.float32(p[0]) + np.float32(1.0) / np.float32(p[1]))
+
+    def compute(self, p, x1, x2):
+        disp = x2[:, 0:3] - x1[:, 0:3]
+        r1 = x1[:, 3:6]
+        d2 = (disp * disp).sum(-1)
+        inv = 1 / np.sqrt(d2)
+        dist = d2 * inv
+        u = disp * inv[:, None]
+        dp = (u * r1).sum(-1)
+        rs, drs = compact_sigmoid(dist - p[:, 0], p[:, 1])
+        as_, das = compact_sigmoid(p[:, 2] - dp, p[:, 3])
+        prob = x2[:, 3] if self.weighted else np.ones_like(dist)
+        dd = prob[:, None] * ((drs * as_)[:, None] * u - (rs * das * inv)[:, None] * (r1 - dp[:, None] * u))
+        d1 = np.concatenate([-dd, (-prob * rs * das)[:, None] * u], -1)
+        score = rs * as_
+        if self.weighted:
+            d2_ = np.concatenate([dd, score[:, None]], -1)
+            return prob * score, d1, d2_
+        # the reference leaves d2[3:6] unset (src/environment.cpp:218-221); its
+        # build reads them back as zero (checked against oracle/_ref sens)
+        return score, d1, np.concatenate([dd, np.zeros_like(dd)], -1)
+
+
+class ProteinHBond(CoordNode):
+    """protein_hbond (src/hbond.cpp:357-435)."""
+
+    def load(self, f, g):
+        self.ig = IGraph(f, g, self.T, self.arg(0), self.arg(0) if False else self.arg(0), ProteinHB())
+        self.ig.sym = False
+        self.nd = len(self.ig.loc1)
+        self.alloc(self.nd + len(self.ig.loc2), 7)
+
+    def compute(self, mode):
+        ho = self.arg(0).output
+        self.output[:, :6] = ho[:self.n_elem, :6]
+        self.output[:, 6] = 0
+        v = self.ig.compute()
+        hl = np.zeros(self.n_elem, self.T)
+        _scatter(hl, self.ig.i1, v)
+        _scatter(hl, self.ig.i2 + self.nd, v)
+        self.output[:, 6] = 1 - np.exp(-hl)
+
+    def propagate(self):
+        ss = self.sens[:, 6] * (1 - self.output[:, 6])
+        es = ss[self.ig.i1] + ss[self.ig.i2 + self.nd]
+        self.ig.propagate(es)
+        ps = self.arg(0).sens
+        _scatter(ps[:, :6], self.ig.loc1, self.sens[:self.nd, :6])
+        _scatter(ps[:, :6], self.ig.loc2, self.sens[self.nd:, :6])
+
+
+class HBondCoverage(CoordNode):
+    """hbond_coverage (src/hbond.cpp:438-484); output per second-group element."""
+
+    def load(self, f, g):
+        self.ig = IGraph(f, g, self.T, self.arg(0), self.arg(1), HBCoverage())
+        self.alloc(len(self.ig.loc2), 1)
+
+    def compute(self, mode):
+        v = self.ig.compute()
+        self.output[:] = 0
+        _scatter(self.output[:, 0], self.ig.i2, v)
+
+    def propagate(self):
+        self.ig.propagate(self.sens[self.ig.i2, 0])
+
+
+class EnvironmentCoverage(CoordNode):
+    """environment_coverage (src/environment.cpp:288-348): binned by aa type."""
+
+    def load(self, f, g):
+        self.n_aa = int(f.attr(g, "num_aa_types"))
+        self.ig = IGraph(f, g, self.T, self.arg(0), self.arg(1), EnvCoverage(True))
+        self.aa = f.read(g + "/aa_types").astype(np.int64)
+        self.alloc(len(self.ig.loc1) * self.n_aa, 1)
+
+    def compute(self, mode):
+        v = self.ig.compute()
+        self.bin = self.ig.i1 * self.n_aa + self.aa[self.ig.id2[self.ig.i2]]
+        self.output[:] = 0
+        _scatter(self.output[:, 0], self.bin, v)
+
+    def propagate(self):
+        self.ig.propagate(self.sens[self.bin, 0])
+
+
+class HbEnvironmentCoverage(CoordNode):
+    """hb_environment_coverage (src/environment.cpp:350-385)."""
+
+    def load(self, f, g):
+        self.ig = IGraph(f, g, self.T, self.arg(0), self.arg(1), EnvCoverage(False))
+        self.alloc(len(self.ig.loc1), 1)
+
+    def compute(self, mode):
+        v = self.ig.compute()
+        self.output[:] = 0
+        _scatter(self.output[:, 0], self.ig.i1, v)
+
+    def propagate(self):
+        self.ig.propagate(self.sens[self.ig.i1, 0])
+
+
+class WeightedPos(CoordNode):
+    """weighted_pos (src/environment.cpp:489-533)."""
+
+    def load(self, f, g):
+        self.ip = f.read(g + "/index_pos").astype(np.int64)
+        self.iw = f.read(g + "/index_weight").astype(np.int64)
+        self.alloc(len(self.ip), 4)
+
+    def compute(self, mode):
+        self.output[:, 0:3] = self.arg(0).output[self.ip, 0:3]
+        self.output[:, 3] = np.exp(-self.arg(1).output[self.iw, 0])
+
+    def propagate(self):
+        _scatter(self.arg(0).sens[:, 0:3], self.ip, self.sens[:, 0:3])
+        _scatter(self.arg(1).sens[:, 0], self.iw, -self.output[:, 3] * self.sens[:, 3])
+
+
+class SigmoidCoupling(PotentialNode):
+    """sigmoid_coupling (src/environment.cpp:851-1003)."""
+
+    def load(self, f, g):
+        self.scale = f.read(g + "/scale").astype(np.float32).astype(self.T)
+        self.center = f.read(g + "/center").astype(np.float32).astype(self.T)
+        self.sharp = f.read(g + "/sharpness").astype(np.float32).astype(self.T)
+        self.w = f.read(g + "/weights").astype(np.float32).astype(self.T)
+        self.ct = f.read(g + "/coupling_types").astype(np.int64)
+        self.nt = len(self.scale)
+
+    def compute(self, mode):
+        x = self.arg(0).output[:, 0].reshape(-1, self.nt)
+        W = self.w.reshape(self.nt, self.nt)[self.ct]
+        wn = (W * x).sum(-1)
+        v, dv = compact_sigmoid(wn - self.center[self.ct], self.sharp[self.ct])
+        self.potential = (self.scale[self.ct] * v).sum()
+        self.arg(0).sens[:, 0] += (W * (self.scale[self.ct] * dv)[:, None]).reshape(-1)
+
+
+class BBSigmoidCoupling(PotentialNode):
+    """bb_sigmoid_coupling (src/environment.cpp:1054-1138); args (env_sc, env_hb)."""
+
+    def load(self, f, g):
+        self.nt = int(f.attr(g, "num_aa_types"))
+        T = self.T
+        self.scale = T(np.float32(f.attr(g, "scale")))
+        self.center = T(np.float32(f.attr(g, "center")))
+        self.sharp = T(np.float32(f.attr(g, "sharpness")))
+        self.hw = T(np.float32(f.attr(g, "hbond_weight")))
+        self.w = f.read(g + "/weights").astype(np.float32).astype(T)[:self.nt]
+
+    def compute(self, mode):
+        sc, hb = self.arg(0), self.arg(1)
+        x = sc.output[:, 0].reshape(-1, self.nt)
+        bl = (x * self.w).sum(-1) + self.hw * hb.output[:, 0]
+        v, dv = compact_sigmoid(bl - self.center, self.sharp)
+        self.potential = (self.scale * v).sum()
+        dp = self.scale * dv
+        sc.sens[:, 0] += (self.w[None, :] * dp[:, None]).reshape(-1)
+        hb.sens[:, 0] += self.hw * dp
+
+
+class HBondEnergy(PotentialNode):
+    """hbond_energy (src/hbond.cpp:487-739).  Writes input sensitivities in its
+    value pass, as the reference does."""
+
+    def load(self, f, g):
+        self.params = f.read(g + "/parameters").astype(np.float32)
+        self.dres = f.read(g + "/donor_resid").astype(np.int64)
+        self.ares = f.read(g + "/acceptor_resid").astype(np.int64)
+        self.n_res = len(f.read(g + "/rama_resid"))
+
+    def compute(self, mode):
+        P = self.params.astype(np.float64)
+        pp, rama = self.arg(0), self.arg(1)
+        r = rama.output.astype(np.float64)
+        E_a, E_b, E_o, E_bias = P[0], P[1], P[2], P[3]
+        p1 = compact_sigmoid(r[:, 0] - P[4], P[5])
+        p2 = compact_sigmoid(r[:, 0] - P[6], P[7])
+        s1 = compact_sigmoid(r[:, 1] - P[8], P[9])
+        s2 = compact_sigmoid(r[:, 1] - P[10], P[11])
+        turn = (1 - p1[0]) * p2[0]
+        psi = (1 - s1[0]) * s2[0]
+        nturn, npsi = 1 - turn, 1 - psi
+        helix, sheet = nturn * psi, nturn * npsi
+        dturn = p2[1] - (p1[1] * p2[0] + p1[0] * p2[1])
+        dpsis = s2[1] - (s1[1] * s2[0] + s1[0] * s2[1])
+        Eh = E_a * helix + E_b * sheet + E_o * turn
+        dphi = E_a * (-dturn * psi) + E_b * (-dturn * npsi) + E_o * dturn
+        dpsi = E_a * (nturn * dpsis) + E_b * (nturn * -dpsis)
+        nd = len(self.dres)
+        hb = pp.output[:, 6].astype(np.float64)
+        n1 = np.zeros(self.n_res)
+        np.add.at(n1, self.dres, hb[:nd])
+        np.add.at(n1, self.ares, hb[nd:])
+        n2 = np.zeros(self.n_res - 1)
+        np.add.at(n2, self.dres - 1, hb[:nd])
+        np.add.at(n2, self.ares, hb[nd:])
+        act = n1 != 0
+        self.n_hbond = n1[act].sum()
+        pot = (n1 * Eh)[act].sum()
+        d_of = dict((r_, i) for i, r_ in enumerate(self.dres))
+        a_of = dict((r_, i) for i, r_ in enumerate(self.ares))
+        ps = pp.sens
+        for i in np.nonzero(act)[0]:
+            if i in d_of:
+                ps[d_of[i], 6] += self.T(Eh[i])
+            if i in a_of:
+                ps[a_of[i] + nd, 6] += self.T(Eh[i])
+        rama.sens[:, 0] += np.where(act, n1 * dphi, 0).astype(self.T)
+        rama.sens[:, 1] += np.where(act, n1 * dpsi, 0).astype(self.T)
+        for i in np.nonzero(n2 > 1.0)[0]:
+            v, dv = compact_sigmoid(np.float64(1.5 - n2[i]), 2.0)
+            pot += v * E_bias
+            dE = -dv * E_bias
+            if (i + 1) in d_of:
+                ps[d_of[i + 1], 6] += self.T(dE)
+            if i in a_of:
+                ps[a_of[i] + nd, 6] += self.T(dE)
+        self.potential = pot
+
+
+class BackbonePairs(PotentialNode):
+    """backbone_pairs (src/backbone_steric.cpp:38-146)."""
+
+    def load(self, f, g):
+        self.res = f.read(g + "/id").astype(np.int64)
+        self.nat = f.read(g + "/n_atom").astype(np.int64)
+        self.ref = f.read(g + "/ref_pos").astype(np.float32).astype(self.T)
+        mx = 0.0
+        for r, n in zip(self.ref, self.nat):
+            for a in range(n):
+                mx = max(np.float32(np.sqrt(np.float32((r[a] * r[a]).sum()))), mx)
+        self.dist_cutoff = np.float32(2 * np.float32(mx) + np.float32(np.sqrt(np.float32(3 * 3 + 0.1 * 3))))
+
+    def compute(self, mode):
+        T = self.T
+        aff = self.arg(0).output[self.res]
+        t = aff[:, :3]
+        U = quat_to_rot(aff[:, 3:7])
+        ref = np.nan_to_num(self.ref)
+        X = np.einsum("nij,naj->nai", U, ref) + t[:, None]
+        n = len(t)
+        d = t[:, None] - t[None, :]
+        d2 = refine_dist2(d, T)
+        iu1, iu2 = np.nonzero(np.triu((d2 < self.dist_cutoff * self.dist_cutoff)
+                                      & (np.abs(self.res[:, None] - self.res[None, :]) > 1), 1))
+        cut2 = T(3.0 * 3.0 + 0.1 * 3.0)
+        sharp = T(1.0) / (T(3.0) * T(0.1))
+        pot = T(0)
+        sens = self.arg(0).sens
+        for a1 in range(4):
+            for a2 in range(4):
+                ok = (a1 < self.nat[iu1]) & (a2 < self.nat[iu2])
+                e1, e2 = iu1[ok], iu2[ok]
+                x1, x2 = X[e1, a1], X[e2, a2]
+                r = x1 - x2
+                rm2 = (r * r).sum(-1)
+                near = rm2 <= cut2
+                e1, e2, x1, x2, r, rm2 = e1[near], e2[near], x1[near], x2[near], r[near], rm2[near]
+                v, dv = compact_sigmoid(rm2 - T(9.0), sharp)
+                pot = pot + (T(4) * v).sum()
+                gvec = (T(2) * T(4) * dv)[:, None] * r
+                _scatter(sens[:, 0:3], self.res[e1], gvec)
+                _scatter(sens[:, 3:6], self.res[e1], np.cross(x1 - t[e1], gvec))
+                _scatter(sens[:, 0:3], self.res[e2], -gvec)
+                _scatter(sens[:, 3:6], self.res[e2], np.cross(x2 - t[e2], -gvec))
+        self.potential = pot
+
+
+class Rotamer(PotentialNode):
+    """Rotamer side-chain free energy by loopy belief propagation
+    (src/rotamer.cpp:628-1169)."""
+
+    def load(self, f, g):
+        self.ig = IGraph(f, g + "/pair_interaction", self.T, self.arg(0), None, RotamerPair())
+        self.prob_nodes = [self.eng.nodes[a] for a in self.args[1:]]
+        self.damping = self.T(np.float32(f.attr(g, "damping")))
+        self.max_iter = int(f.attr(g, "max_iter"))
+        self.tol = np.float32(f.attr(g, "tol"))
+        self.chunk = int(f.attr(g, "iteration_chunk_size"))
+        ids = self.ig.id1
+        self.rot = ids & 15
+        self.nrot = (ids >> 4) & 15
+        self.loc_id = ids >> 8
+        self.n_elem = {}
+        for k in (1, 3, 6):
+            self.n_elem[k] = len(np.unique(self.loc_id[self.nrot == k]))
+        self.n_bad_solve = 0
+
+    def compute(self, mode):
+        T = self.T
+        ig = self.ig
+        # node energies and probabilities (fill_holders, :854-913)
+        E1 = sum(p.output[ig.loc1, 0] for p in self.prob_nodes)
+        node_e = {k: np.zeros((self.n_elem[k], k), T) for k in (1, 3, 6)}
+        for k in (1, 3, 6):
+            m = self.nrot == k
+            np.add.at(node_e[k], (self.loc_id[m], self.rot[m]), E1[m])
+        self.prob, self.offset = {}, {}
+        for k in (1, 3, 6):
+            off = node_e[k].min(axis=1) if self.n_elem[k] else np.zeros(0, T)
+            self.offset[k] = off
+            self.prob[k] = np.exp(off[:, None] - node_e[k]).astype(T)
+        v = ig.compute()
+        pe = np.exp(-v).astype(T)
+        a, b = ig.i1, ig.i2
+        na_, nb_ = self.nrot[a], self.nrot[b]
+        swap = na_ > nb_
+        A = np.where(swap, b, a)
+        B = np.where(swap, a, b)
+        self.edge_A, self.edge_B = A, B
+        n1, n2 = self.nrot[A], self.nrot[B]
+        # residue-pair edges (EdgeLocator grouping, :172-205, :383-405)
+        self.E = {}
+        for c1, c2 in ((1, 1), (1, 3), (1, 6), (3, 3), (3, 6), (6, 6)):
+            m = (n1 == c1) & (n2 == c2)
+            keys = self.loc_id[A[m]] * 1000003 + self.loc_id[B[m]]
+            uk, inv = np.unique(keys, return_inverse=True)
+            P = np.ones((len(uk), c1, c2), T)
+            np.multiply.at(P, (inv, self.rot[A[m]], self.rot[B[m]]), pe[m])
+            self.E[(c1, c2)] = dict(P=P, i1=(uk // 1000003), i2=(uk % 1000003), bead=np.nonzero(m)[0], inv=inv)
+        for c in (3, 6):
+            e = self.E[(1, c)]
+            for j in range(len(e["i1"])):
+                self.prob[c][e["i2"][j]] *= e["P"][j, 0]
+        iters = self.solve()
+        if iters >= self.max_iter - self.chunk - 1:
+            self.n_bad_solve += 1
+        self.iterations = iters
+        self.propagate_rot()
+        if mode == PotentialAndDerivMode:
+            self.potential = self.free_energy()
+
+    # ---- BP (solve_for_marginals, :1106-1162)
+    def update_edges(self, old_nodes, old_msgs, cur_nodes):
+        T = self.T
+        new = {}
+        for c in ((3, 3), (3, 6), (6, 6)):
+            e = self.E[c]
+            if not len(e["i1"]):
+                new[c] = (np.zeros((0, c[0]), T), np.zeros((0, c[1]), T))
+                continue
+            m1, m2 = old_msgs[c]
+            v1 = old_nodes[c[0]][e["i1"]] / (T(1e-10) + m1)
+            v2 = old_nodes[c[1]][e["i2"]] / (T(1e-10) + m2)
+            msg1 = np.einsum("eij,ej->ei", e["P"], v2)
+            msg2 = np.einsum("eij,ei->ej", e["P"], v1)
+            np.multiply.at(cur_nodes[c[0]], e["i1"], msg1 / msg1.max(axis=1, keepdims=True))
+            np.multiply.at(cur_nodes[c[1]], e["i2"], msg2 / msg2.max(axis=1, keepdims=True))
+            # each message is scaled to unit L1 norm (src/rotamer.cpp:553-568); the
+            # scale matters through the 1e-10 floor in the division above
+            new[c] = (msg1 / msg1.sum(1, keepdims=True), msg2 / msg2.sum(1, keepdims=True))
+        for k in (3, 6):
+            cur_nodes[k] /= cur_nodes[k].sum(1, keepdims=True)
+        return new
+
+    def solve(self):
+        T = self.T
+        prob = self.prob
+        old = {k: prob[k].copy() for k in (3, 6)}
+        msgs = {c: (np.ones((len(self.E[c]["i1"]), c[0]), T), np.ones((len(self.E[c]["i1"]), c[1]), T))
+                for c in ((3, 3), (3, 6), (6, 6))}
+        cur = {k: prob[k].copy() for k in (3, 6)}
+        msgs = self.update_edges(old, msgs, cur)
+        cur, old = old, cur  # do_swap_for_initial
+        for k in (3, 6):
+            if len(cur[k]):
+                cur[k] = cur[k] / cur[k].max(1, keepdims=True)
+        dev, it = np.float32(1e10), 0
+        d = self.damping
+        while dev > self.tol and it < self.max_iter:
+            for _ in range(self.chunk):
+                old = cur
+                cur = {k: prob[k].copy() for k in (3, 6)}
+                msgs = self.update_edges(old, msgs, cur)
+                for k in (3, 6):
+                    if len(cur[k]):
+                        cur[k] = (1 - d) * cur[k] / cur[k].max(1, keepdims=True) + d * old[k]
+            dev = max([np.float32(0)] + [np.max(cur[k] - old[k]) for k in (3, 6) if len(cur[k])])
+            it += self.chunk
+        self.node_marg = {1: np.ones((self.n_elem[1], 1), T)}
+        for k in (3, 6):
+            self.node_marg[k] = cur[k] / cur[k].sum(1, keepdims=True)
+        self.edge_marg = {}
+        for c in ((3, 3), (3, 6), (6, 6)):
+            e = self.E[c]
+            m1, m2 = msgs[c]
+            b1 = self.node_marg[c[0]][e["i1"]] / (T(1e-10) + m1)
+            b2 = self.node_marg[c[1]][e["i2"]] / (T(1e-10) + m2)
+            mg = e["P"] * b1[:, :, None] * b2[:, None, :]
+            self.edge_marg[c] = mg / mg.sum(axis=(1, 2), keepdims=True)
+        return it
+
+    def propagate_rot(self):
+        T = self.T
+        ig = self.ig
+        sens = np.zeros(len(ig.i1), T)
+        A, B = self.edge_A, self.edge_B
+        for c, e in self.E.items():
+            bead = e["bead"]
+            if c == (1, 1):
+                sens[bead] = 1
+            elif c[0] == 1:
+                sens[bead] = self.node_marg[c[1]][e["i2"][e["inv"]], self.rot[B[bead]]]
+            else:
+                sens[bead] = self.edge_marg[c][e["inv"], self.rot[A[bead]], self.rot[B[bead]]]
+        ig.propagate(sens)
+        for k in (1, 3, 6):
+            m = self.nrot == k
+            s1 = np.zeros(len(ig.loc1), T)
+            s1[m] = self.node_marg[k][self.loc_id[m], self.rot[m]]
+            for p in self.prob_nodes:
+                _scatter(p.sens[:, 0], ig.loc1, s1)
+
+    def free_energy(self):
+        T = self.T
+        en = 0.0
+        for k in (1, 3, 6):
+            b, pr = self.node_marg[k], self.prob[k]
+            en += (self.offset[k] + (b * np.log((T(1e-10) + b) / (T(1e-10) + pr))).sum(1)).sum()
+        en += (-np.log(self.E[(1, 1)]["P"][:, 0, 0])).sum()
+        for c in ((3, 3), (3, 6), (6, 6)):
+            e = self.E[c]
+            p = self.edge_marg[c]
+            b1 = self.node_marg[c[0]][e["i1"]]
+            b2 = self.node_marg[c[1]][e["i2"]]
+            q = e["P"] * b1[:, :, None] * b2[:, None, :]
+            en += (p * np.log((T(1e-10) + p) / (T(1e-10) + q))).sum()
+        return en
+
+
+NODE_TYPES = {
+    "Distance3D": DistCoord, "Angle": AngleCoord, "Dihedral": DihedralCoord, "Spring": Spring,
+    "rama_coord": RamaCoord, "cat_pos": CatPos, "rama_map_pot": RamaMapPot,
+    "affine_alignment": AffineAlignment, "placement_scalar": Placement,
+    "placement_fixed_point_vector_only": Placement, "placement_fixed_point_vector_scalar": Placement,
+    "infer_H_O": InferHO, "protein_hbond": ProteinHBond, "hbond_coverage": HBondCoverage,
+    "hbond_energy": HBondEnergy, "environment_coverage": EnvironmentCoverage,
+    "hb_environment_coverage": HbEnvironmentCoverage, "weighted_pos": WeightedPos,
+    "sigmoid_coupling": SigmoidCoupling, "bb_sigmoid_coupling": BBSigmoidCoupling,
+    "backbone_pairs": BackbonePairs, "rotamer": Rotamer,
+}
+
+
+def node_type_for(name):
+    """Longest registered prefix (the reference requires the registered
+    prefixes to be prefix-free, src/deriv_engine.cpp:52-76,393-399)."""
+    best = None
+    for k in NODE_TYPES:
+        if name.startswith(k) and (best is None or len(k) > len(best)):
+            best = k
+    return best
+
+
+class Engine:
+    """Oracle DerivEngine over one `.up` file."""
+
+    def __init__(self, path, dtype=np.float32, n_atom=None):
+        self.T = np.dtype(dtype).type
+        f = h5lite.File(path)
+        if n_atom is None:
+            n_atom = f.read("/input/pos").shape[0]
+        self.initial_pos = f.read("/input/pos")[:, :, 0].astype(np.float32)
+        self.nodes = {"pos": PosNode(self, n_atom)}
+        names = f.children("/input/potential")
+        deps = {"pos": []}
+        for nm in names:
+            deps[nm] = f.attr("/input/potential/" + nm, "arguments")
+        # topological order as the reference's loader (std::map iteration
+        # rounds, src/deriv_engine.cpp:358-386)
+        order, active = [], dict((k, True) for k in deps)
+        for _ in range(len(deps)):
+            for k in sorted(deps):
+                if active[k] and all(a in order for a in deps[k]):
+                    order.append(k)
+                    active[k] = False
+        if any(active.values()):
+            raise ValueError("unsatisfiable dependency")
+        self.order = [k for k in order if k != "pos"]
+        for nm in self.order:
+            t = node_type_for(nm)
+            if t is None:
+                raise ValueError("No node type found for name '%s'" % nm)
+            node = NODE_TYPES[t](self, nm, deps[nm])
+            self.nodes[nm] = node
+            node.load(f, "/input/potential/" + nm)
+        f.close()
+        self._levels(deps)
+
+    def _levels(self, deps):
+        """build_exec_levels (src/deriv_engine.cpp:127-178)."""
+        names = ["pos"] + self.order
+        parents = dict((n, deps[n]) for n in names)
+        children = dict((n, []) for n in names)
+        for n in names:
+            for p in parents[n]:
+                children[p].append(n)
+        germ = dict((n, -1) for n in names)
+        der = dict((n, -1) for n in names)
+        germ["pos"] = 0
+        self.germ = ["pos"]
+        self.deriv_order = []
+        lvl = 1
+        while True:
+            unfinished = False
+            for n in names:
+                if germ[n] == -1:
+                    unfinished = True
+                    if all(germ[p] != -1 and germ[p] != lvl for p in parents[n]):
+                        germ[n] = lvl
+                        self.germ.append(n)
+                if der[n] == -1 and germ[n] != -1:
+                    unfinished = True
+                    if all(der[c] != -1 and der[c] != lvl for c in children[n]):
+                        der[n] = lvl
+                        self.deriv_order.append(n)
+            if not unfinished:
+                break
+            lvl += 1
+
+    def compute(self, pos, mode=PotentialAndDerivMode):
+        self.nodes["pos"].output[:] = np.asarray(pos, np.float32).astype(self.T)
+        self.potential = 0.0
+        for n in self.germ:
+            node = self.nodes[n]
+            node.compute(mode)
+            if node.potential_term:
+                self.potential += node.potential
+            else:
+                node.sens[:] = 0
+        for n in self.deriv_order:
+            self.nodes[n].propagate()
+        return self.potential
+
+    def energy(self, pos):
+        return self.T(self.compute(pos, PotentialAndDerivMode))
+
+    def deriv(self, pos):
+        self.compute(pos, PotentialAndDerivMode)
+        return self.nodes["pos"].sens.copy()
+
+    def get_output(self, name):
+        n = self.nodes[name]
+        return np.array([[n.potential]], self.T) if n.potential_term else n.output.copy()
+
+
+# ------------------------------------------------------- integrator pieces
+def threefry4x32_20(ctr, key):
+    """Threefry-4x32 with 20 rounds (Salmon et al., SC'11), vectorised over
+    rows of uint32 (n,4) counters with one (4,) key."""
+    R = [(10, 26), (11, 21), (13, 27), (23, 5), (6, 20), (17, 11), (25, 10), (18, 20)]
+    M = np.uint64(0xFFFFFFFF)
+    ks = [np.uint64(k) for k in key]
+    ks.append(np.uint64(0x1BD11BDA) ^ ks[0] ^ ks[1] ^ ks[2] ^ ks[3])
+    x = [ctr[:, i].astype(np.uint64) for i in range(4)]
+
+    def rotl(v, r):
+        return ((v << np.uint64(r)) | (v >> np.uint64(32 - r))) & M
+    for i in range(4):
+        x[i] = (x[i] + ks[i]) & M
+    for rnd in range(20):
+        r0, r1 = R[rnd % 8]
+        if rnd % 2 == 0:
+            x[0] = (x[0] + x[1]) & M; x[1] = rotl(x[1], r0) ^ x[0]
+            x[2] = (x[2] + x[3]) & M; x[3] = rotl(x[3], r1) ^ x[2]
+        else:
+            x[0] = (x[0] + x[3]) & M; x[3] = rotl(x[3], r0) ^ x[0]
+            x[2] = (x[2] + x[1]) & M; x[1] = rotl(x[1], r1) ^ x[2]
+        if rnd % 4 == 3:
+            s = rnd // 4 + 1
+            for i in range(4):
+                x[i] = (x[i] + ks[(s + i) % 5]) & M
+            x[3] = (x[3] + np.uint64(s)) & M
+    return np.stack(x, -1).astype(np.uint32)
+
+
+def thermostat_normals(seed, stream, n_atom, step):
+    """RandomGenerator(seed, stream, atom, step).normal3() for every atom
+    (src/random.h:19-67, src/boxmuller.hpp, src/uniform.hpp), float32."""
+    ctr = np.zeros((n_atom, 4), np.uint32)
+    ctr[:, 0] = np.uint32(step & 0xFFFFFFFF)
+    ctr[:, 1] = np.uint32(step >> 32)
+    ctr[:, 2] = np.arange(n_atom, dtype=np.uint32)
+    bits = threefry4x32_20(ctr, [seed, stream, 0, 0])
+    f32 = np.float32
+
+    def uneg11(u):
+        return (u.view(np.int32).astype(f32) * f32(2.0 ** -31) + f32(2.0 ** -32)).astype(f32)
+
+    def u01(u):
+        return (u.astype(f32) * f32(2.0 ** -32) + f32(2.0 ** -33)).astype(f32)
+    out = []
+    for a, b in ((0, 1), (2, 3)):
+        ang = (f32(np.pi) * uneg11(bits[:, a])).astype(f32)
+        r = np.sqrt(f32(-2.0) * np.log(u01(bits[:, b]))).astype(f32)
+        out += [np.sin(ang) * r, np.cos(ang) * r]
+    return np.stack(out[:3], -1).astype(f32)

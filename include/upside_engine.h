@@ -1,0 +1,99 @@
+/* C ABI of the MI355X Upside force engine (libupside.so).
+ *
+ * The first group of entry points has exactly the signatures and return-code
+ * conventions of the reference's engine_c_library (src/engine_c_library.h:7-32,
+ * src/engine_c_library.cpp), which py/upside_engine.py binds over ctypes
+ * (py/upside_engine.py:48-91): 0 is success, nonzero failure with a message on
+ * stderr, no exception crosses the boundary, the caller owns every buffer.
+ * Positions and derivatives are (n_atom, 3) row-major float32; evaluate_deriv
+ * returns dE/dx (the force is its negative).
+ *
+ * The second group extends the ABI for many systems evaluated and integrated
+ * in one persistent GPU launch (replica exchange, independent chains).  It has
+ * no counterpart in the reference, whose only batching is an OpenMP loop over
+ * systems inside upside_main (src/main.cpp:915-999).
+ */
+#ifndef UPSIDE_ENGINE_H
+#define UPSIDE_ENGINE_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct DerivEngine DerivEngine;
+
+/* src/engine_c_library.cpp:9-21: reads /input/potential of a .up file; NULL on failure. */
+DerivEngine* construct_deriv_engine(int n_atom, const char* potential_file, bool quiet);
+/* src/engine_c_library.cpp:24-26 */
+void free_deriv_engine(DerivEngine* engine);
+/* src/engine_c_library.cpp:30-46: total potential at pos. */
+int evaluate_energy(float* energy, DerivEngine* engine, const float* pos);
+/* src/engine_c_library.cpp:49-65: dE/dx at pos, (n_atom,3). */
+int evaluate_deriv(float* deriv, DerivEngine* engine, const float* pos);
+/* src/engine_c_library.cpp:68-110: node parameter vectors.  get_param returns the
+ * interaction table of pair-graph nodes; set_param and get_param_deriv are not
+ * supported and return nonzero (the reference returns -1 from get_param_deriv
+ * when built without PARAM_DERIV). */
+int set_param(int n_param, const float* param, DerivEngine* engine, const char* node_name);
+int get_param_deriv(int n_param, float* deriv, DerivEngine* engine, const char* node_name);
+int get_param(int n_param, float* param, DerivEngine* engine, const char* node_name);
+/* src/engine_c_library.cpp:113-178: row-major (n_elem, elem_width) copies of a
+ * coordinate node's output or sensitivity from the last evaluation; potential
+ * nodes report (1,1) and their potential. */
+int get_output_dims(int* n_elem, int* elem_width, DerivEngine* engine, const char* node_name);
+int get_output(int n_output, float* output, DerivEngine* engine, const char* node_name);
+int get_sens(int n_output, float* output, DerivEngine* engine, const char* node_name);
+/* src/engine_c_library.cpp:181-195 and the node hooks of src/rotamer.cpp:736-834,
+ * src/hbond.cpp:476-482.  Supported names: rotamer: "rotamer_free_energy",
+ * "count_edges_by_type", "n_node", "read n_bad_solve", "read n_bad_solve and
+ * reset"; any pair-graph node: "count_edges_by_type", "n_edge", "edge_pairs"
+ * (2*n_edge element indices, sorted), "edge_values" (n_edge, same order). */
+int get_value_by_name(int n_output, float* output, DerivEngine* engine, const char* node_name,
+                      const char* log_name);
+/* src/engine_c_library.cpp:197-277: clamped cubic B-spline helpers. */
+int clamped_spline_solve(int N, float* bspline_coeff, const float* values);
+int clamped_spline_value(int N, float* result, const float* bspline_coeff, int nx, float* x);
+int get_clamped_value_and_deriv(int N, float* result, const float* bspline_coeff, int nx, float* x);
+int get_clamped_coeff_deriv(int N, float* result, const float* bspline_coeff, float x);
+
+/* ---- batched extension (no reference counterpart) ---- */
+typedef struct UpsideBatch UpsideBatch;
+
+/* n_sys systems from n_sys .up paths (a path may repeat: replicas share one
+ * lowered program).  Initial positions come from /input/pos of each file.
+ * temperatures[n_sys]; system s uses thermostat seed base_seed+s
+ * (src/main.cpp:682), timescale 5, interval rounds thermostat_interval with
+ * inner step 3 and time step dt (src/main.cpp:775-803).  Momenta are
+ * thermalised as the reference does at a fresh start.  NULL on failure. */
+UpsideBatch* upside_batch_create(int n_sys, const char* const* up_paths, const float* temperatures,
+                                 uint32_t base_seed, float dt, int thermostat_interval);
+void upside_batch_free(UpsideBatch* batch);
+int upside_batch_n_atom(UpsideBatch* batch, int sys);
+int upside_batch_set_pos(UpsideBatch* batch, int sys, const float* pos);
+int upside_batch_get_pos(UpsideBatch* batch, int sys, float* pos);
+int upside_batch_get_mom(UpsideBatch* batch, int sys, float* mom);
+int upside_batch_set_mom(UpsideBatch* batch, int sys, const float* mom);
+/* Advance every system by n_round rounds (3 force evaluations each) in one
+ * launch; when frame_interval>0 the potential is evaluated at rounds that are
+ * multiples of it and stored in frame_pot[n_sys*n_frames] if non-NULL. */
+int upside_batch_run(UpsideBatch* batch, int n_round, int frame_interval, float* frame_pot, int n_frames);
+/* Potential of every system at its current positions: energies[n_sys]. */
+int upside_batch_energies(UpsideBatch* batch, float* energies);
+/* Replica exchange step (src/main.cpp:243-316, criterion 0): swap positions
+ * of the pairs of swap set `set_index`, keep a swap with the Metropolis test
+ * drawn from Threefry stream 1 with counter `round`.  accepted[n_pairs] out. */
+int upside_batch_replica_exchange(UpsideBatch* batch, int n_pairs, const int* pairs, uint64_t round,
+                                  int* accepted);
+/* Summed BP statistics: iterations of the last solve and since creation. */
+int upside_batch_bp_stats(UpsideBatch* batch, int sys, int* last_iters, long long* total_iters,
+                          long long* solves, long long* bad_solves);
+/* Synchronize and time the device (for benchmarks): elapsed ms of the last run. */
+float upside_batch_last_run_ms(UpsideBatch* batch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,156 @@
+// Device helpers shared by the engine kernels: block reductions, the B-spline
+// and sigmoid primitives of the reference (src/spline.h, src/vector_math.h),
+// and the fixed-order gather that replaces scatter-with-atomics.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "engine_dev.h"
+
+namespace upside {
+
+constexpr int NT = 512;            // threads per workgroup (8 wave64)
+constexpr int NWAVE = NT / 64;
+
+struct Ctx {
+    float* A;          // per-system float arena
+    int* I;            // per-system int arena
+    const float* CF;   // constant float arena
+    const int* CI;     // constant int arena
+    float* red;        // LDS reduction scratch [NWAVE]
+    int tid;
+};
+
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Block sum in a fixed order (lane tree, then waves in index order): the
+// result is the same on every run.  All threads must call it.
+__device__ float block_sum(Ctx& c, float v) {
+    v = wave_sum(v);
+    __syncthreads();
+    if ((c.tid & 63) == 0) c.red[c.tid >> 6] = v;
+    __syncthreads();
+    float t = 0.f;
+    for (int w = 0; w < NWAVE; ++w) t += c.red[w];
+    __syncthreads();
+    return t;
+}
+__device__ float block_max(Ctx& c, float v) {
+    v = wave_max(v);
+    __syncthreads();
+    if ((c.tid & 63) == 0) c.red[c.tid >> 6] = v;
+    __syncthreads();
+    float t = c.red[0];
+    for (int w = 1; w < NWAVE; ++w) t = fmaxf(t, c.red[w]);
+    __syncthreads();
+    return t;
+}
+__device__ int block_or(Ctx& c, int v) {
+    int any = __syncthreads_or(v);
+    return any;
+}
+
+// ---- splines (src/spline.h) -------------------------------------------------
+// Uniform cubic B-spline with coefficient window p[xb-1 .. xb+2] at x>=1
+// (first spline centred at -1, src/spline.h:98-127,242-256): value, derivative.
+__device__ __forceinline__ void deboor(const float* p, float x, float& v, float& d) {
+    int xb = (int)x;
+    float y = x - (float)xb;
+    const float* c = p + xb - 1;
+    float c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
+    float omy = 1.f - y, y2 = y * y, y3 = y2 * y;
+    float w0 = omy * omy * omy * (1.f / 6.f);
+    float w1 = (3.f * y3 - 6.f * y2 + 4.f) * (1.f / 6.f);
+    float w2 = (-3.f * y3 + 3.f * y2 + 3.f * y + 1.f) * (1.f / 6.f);
+    float w3 = y3 * (1.f / 6.f);
+    v = w0 * c0 + w1 * c1 + w2 * c2 + w3 * c3;
+    float e0 = -0.5f * omy * omy;
+    float e1 = 1.5f * y2 - 2.f * y;
+    float e2 = -1.5f * y2 + y + 0.5f;
+    float e3 = 0.5f * y2;
+    d = e0 * c0 + e1 * c1 + e2 * c2 + e3 * c3;
+}
+
+// Clamped variant (src/spline.h:326-358): constant outside [1, n_knot-2).
+__device__ __forceinline__ void clamped_deboor(const float* p, float x, int n_knot, float& v, float& d) {
+    if (!(x >= 1.f)) {  // also catches NaN, which must never index the table
+        v = p[0] * (1.f / 6.f) + p[1] * (2.f / 3.f) + p[2] * (1.f / 6.f);
+        d = 0.f;
+    } else if (x >= (float)(n_knot - 2)) {
+        v = p[n_knot - 3] * (1.f / 6.f) + p[n_knot - 2] * (2.f / 3.f) + p[n_knot - 1] * (1.f / 6.f);
+        d = 0.f;
+    } else {
+        deboor(p, x, v, d);
+    }
+}
+
+// 1/(1+exp(x)) and its derivative exp(x)/(1+exp(x))^2 (src/vector_math.h:683-690).
+__device__ __forceinline__ void sigmoid(float x, float& v, float& d) {
+    float z = expf(-x);
+    float w = 1.f / (1.f + z);
+    v = w;
+    d = z * w * w;
+}
+
+// Compact sigmoid 0.25(y+2)(y-1)^2 with y = x*sharpness, 1 below -1, 0 above 1
+// (src/vector_math.h:701-720).
+template <typename S>
+__device__ __forceinline__ void compact_sigmoid(S x, S sharp, S& v, S& d) {
+    S y = x * sharp;
+    v = S(0.25) * (y + S(2)) * (y - S(1)) * (y - S(1));
+    d = (sharp * S(0.75)) * (y * y - S(1));
+    if (y < S(-1)) { v = S(1); d = S(0); }
+    if (y > S(1)) { v = S(0); d = S(0); }
+}
+
+// Periodic bicubic patch c[16] (value, d/dfx, d/dfy), src/spline.h:63-82.
+__device__ __forceinline__ void bicubic(const float* c, float fx, float fy, float& v, float& dx, float& dy) {
+    float fx2 = fx * fx, fx3 = fx * fx2, fy2 = fy * fy;
+    float vx0 = c[0] + fy * (c[1] + fy * (c[2] + fy * c[3]));
+    float vx1 = c[4] + fy * (c[5] + fy * (c[6] + fy * c[7]));
+    float vx2 = c[8] + fy * (c[9] + fy * (c[10] + fy * c[11]));
+    float vx3 = c[12] + fy * (c[13] + fy * (c[14] + fy * c[15]));
+    float vy1 = c[1] + fx * (c[5] + fx * (c[9] + fx * c[13]));
+    float vy2 = c[2] + fx * (c[6] + fx * (c[10] + fx * c[14]));
+    float vy3 = c[3] + fx * (c[7] + fx * (c[11] + fx * c[15]));
+    dx = vx1 + 2.f * fx * vx2 + 3.f * fx2 * vx3;
+    dy = vy1 + 2.f * fy * vy2 + 3.f * fy2 * vy3;
+    v = vx0 + fx * vx1 + fx2 * vx2 + fx3 * vx3;
+}
+
+__device__ __forceinline__ void quat_to_rot(const float* q, float* U) {
+    float a = q[0], b = q[1], c = q[2], d = q[3];
+    U[0] = a * a + b * b - c * c - d * d; U[1] = 2.f * b * c - 2.f * a * d; U[2] = 2.f * b * d + 2.f * a * c;
+    U[3] = 2.f * b * c + 2.f * a * d; U[4] = a * a - b * b + c * c - d * d; U[5] = 2.f * c * d - 2.f * a * b;
+    U[6] = 2.f * b * d - 2.f * a * c; U[7] = 2.f * c * d + 2.f * a * b; U[8] = a * a - b * b - c * c + d * d;
+}
+
+// Fixed-order gather (see engine_dev.h): each target row adds the sum of its
+// contributions, in plan order.  Called by all threads.
+__device__ void plan_gather(Ctx& c, int plan) {
+    if (plan < 0) return;
+    const int* h = c.CI + plan;
+    int nt = h[0], cw = h[1], so = h[2], ss = h[3], col = h[4], cb = h[5], cs = h[6];
+    const int* off = h + PLAN_HDR;
+    const int* idx = off + nt + 1;
+    float* sens = c.A + so;
+    const float* con = c.A + cb;
+    for (int t = c.tid; t < nt; t += NT) {
+        int b = off[t], e = off[t + 1];
+        if (b == e) continue;
+        for (int k = 0; k < cw; ++k) {
+            float acc = 0.f;
+            for (int j = b; j < e; ++j) acc += con[idx[j] * cs + k];
+            sens[t * ss + col + k] += acc;
+        }
+    }
+    __syncthreads();
+}
+
+}  // namespace upside

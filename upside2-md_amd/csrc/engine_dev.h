@@ -1,0 +1,153 @@
+// Device-side data model of the MI355X Upside engine.
+//
+// One system (a chain replica) is evaluated by one workgroup.  The node graph
+// of a `.up` file (src/deriv_engine.cpp:353-426) is lowered on the host into a
+// program of OpRec records; the persistent kernel interprets the program in
+// germ order (values) and deriv order (sensitivity pushes), mirroring
+// DerivEngine::compute (src/deriv_engine.cpp:239-260).  All node outputs,
+// sensitivities and scratch live in per-system float/int arenas addressed by
+// offsets, so replicas that share a topology share one program.
+#pragma once
+#include <cstdint>
+
+namespace upside {
+
+enum OpKind : int {
+    OP_DIST3D = 1, OP_ANGLE, OP_DIHEDRAL, OP_SPRING, OP_RAMA_COORD, OP_CAT_POS,
+    OP_RAMA_MAP_POT, OP_AFFINE, OP_PLACEMENT, OP_INFER_HO, OP_PROTEIN_HBOND,
+    OP_HBOND_COVERAGE, OP_HBOND_ENERGY, OP_ROTAMER, OP_WEIGHTED_POS, OP_ENV_COVERAGE,
+    OP_HB_ENV_COVERAGE, OP_SIGMOID_COUPLING, OP_BB_SIGMOID_COUPLING, OP_BACKBONE_PAIRS,
+    OP_KIND_COUNT
+};
+
+// Edge functions of the pair graphs (InteractionGraph IType's).
+enum EdgeKind : int {
+    EDGE_ROTAMER = 0,      // PosQuadSplineInteraction, src/bead_interaction.h:184-221
+    EDGE_HBOND_COVERAGE,   // HBondCoverageInteraction, src/hbond.cpp:249-294
+    EDGE_PROTEIN_HBOND,    // ProteinHBondInteraction, src/hbond.cpp:158-246
+    EDGE_ENV_WEIGHTED,     // EnvironmentCoverageInteraction, src/environment.cpp:114-170
+    EDGE_ENV_HBOND,        // HbondEnvironmentCoverageInteraction, src/environment.cpp:172-228
+    EDGE_BACKBONE          // residue-pair list of BackbonePairs, src/backbone_steric.cpp:38-146
+};
+
+// A pair graph over two element sets (or one, symmetric).  Candidate pairs
+// within the cache radius are kept in row-major padded lists ("slots"): row i1
+// holds its partners i2 in increasing order (for symmetric graphs only i2>i1).
+// A transposed list per i2 names the slots where it appears, in increasing i1
+// order.  Both orders are fixed, so the sensitivity scatter is a gather in a
+// fixed order: bit-reproducible, no float atomics.
+struct PairGraph {
+    int kind;
+    int sym;
+    int n1, n2;            // element counts
+    int dim1, dim2;        // coordinate widths read from the producers
+    int w1, w2;            // producer row strides
+    int out1, out2;        // producer output offsets (float arena)
+    int sens1, sens2;      // producer sensitivity offsets (float arena)
+    int loc1, loc2;        // const int arrays: element -> producer row
+    int type1, type2;      // const int arrays
+    int id1, id2;          // const int arrays
+    int param;             // const float array, (n_type1*n_type2, n_param) + tail padding
+    int n_type2, n_param;
+    int excl;              // pair accepted if |id1-id2| > excl; -1: rotamer rule (id>>4 differs)
+    float cutoff2;         // refine threshold (float cutoff squared)
+    float cache_cut2;      // cache threshold
+    float drift2;          // rebuild when an element moved more than this (squared)
+    int ddim1, ddim2;      // derivative widths (== dim except for backbone pairs)
+    int maxn, maxt;        // slot capacity per row / per transposed row
+    // per-system scratch (arena offsets)
+    int cache_pos;         // float (n1+n2)*3 (symmetric: n1*3)
+    int pos_pack;          // float (n1+n2)*3 current packed coordinates of both sides
+    int cnt, nbr;          // int n1, n1*maxn
+    int tcnt, tslot;       // int n2, n2*maxt (slot index i1*maxn+k)
+    int val;               // float n1*maxn   edge value (0 when inactive)
+    int d1, d2;            // float n1*maxn*ddim1, n1*maxn*ddim2
+    int esens;             // float n1*maxn   edge sensitivity
+    int act;               // int n1*maxn     active flag
+    int state;             // int [0]=cache valid, [1]=overflow, [2]=rebuild count
+};
+
+// Rotamer belief propagation over residue nodes (src/rotamer.cpp:628-1169).
+struct RotamerDev {
+    int n_node;            // residues with side chains
+    int node_nrot;         // const int [n_node]
+    int node_bead0;        // const int [n_node+1]  beads of node n: [bead0[n], bead0[n+1])
+    int bead_rot;          // const int [n_bead]
+    int n_prob;            // number of 1-body energy producers (<=4)
+    int prob_out[4], prob_sens[4], prob_w[4];
+    float damping, tol;
+    int max_iter, chunk;
+    int maxc, maxi;        // candidate residue pairs per node row / incidences per node
+    // per-system scratch
+    int ccnt, cnbr;        // int n_node, n_node*maxc   candidate partner node (B>A)
+    int icnt, iinc;        // int n_node, n_node*maxi   incidences: cand index*2 + side
+    int cpres;             // int n_node*maxc  present flag (any active bead pair)
+    int cP;                // float n_node*maxc*36  edge probability, rows = first node
+    int cmsg;              // float n_node*maxc*24  [old m1(6) m2(6) | new m1(6) m2(6)]
+    int cmarg;             // float n_node*maxc*36
+    int nprob, ncur, nold; // float n_node*6
+    int noff;              // float n_node   energy offset
+    int stats;             // int [0]=last iterations, [1]=bad solves, [2]=total iterations, [3]=solves
+    float edge_cut2;       // node-pair candidate cache threshold (bead cache radius)
+};
+
+struct OpRec {
+    int kind;
+    int node;              // node index in the host graph (for diagnostics)
+    int n_elem, width;     // output geometry (CoordNodes)
+    int out, sens;         // output/sens offsets (float arena); potential nodes: -1
+    int pot_slot;          // index into the per-system potential array (potential nodes)
+    int in_out[4], in_sens[4], in_w[4];  // inputs (producer output, sens, row stride)
+    int ia[16];            // kind-specific int parameters/arrays (const arena offsets)
+    float fa[16];          // kind-specific float parameters
+    int scratch;           // kind-specific float scratch offset
+    int iscratch;          // kind-specific int scratch offset
+    int plans[4];          // gather plans (const int arena offsets, -1: none)
+    int graph;             // index into PairGraph table (-1 if none)
+    int rot;               // index into RotamerDev table (-1 if none)
+};
+
+struct SysDesc {
+    float* arena;          // per-system float arena
+    int* iarena;           // per-system int arena
+    const OpRec* ops;      // program: n_germ value ops then n_deriv push ops
+    int n_germ, n_deriv;
+    const PairGraph* graphs;
+    const RotamerDev* rots;
+    const float* cf;       // constant float arena (parameters)
+    const int* ci;         // constant int arena (index arrays)
+    int n_atom;
+    int pos_out, pos_sens; // pos node offsets (row stride 3)
+    int n_pot;             // number of potential nodes
+    int pot;               // float offset of the per-node potential array [n_pot]
+    int sens_zero_cnt;     // sens ranges to clear at the start of compute
+    int sens_zero;         // int offset: pairs (offset, count)
+    int mom;               // float offset: momentum [n_atom*3]
+    // thermostat (src/thermostat.h, src/thermostat.cpp)
+    uint32_t seed;
+    float mom_scale, noise_scale;
+    uint64_t n_invocations;
+    int error;             // int offset of error word
+};
+
+struct RunArgs {
+    int mode;              // 1: potential and derivative, 0: derivative only
+    int n_round;           // rounds to integrate (0: one evaluation only)
+    float dt;
+    int thermostat_interval;
+    int first_round;       // global round number of the first round (for intervals)
+    int frame_interval;    // rounds between potential evaluations recorded
+    float* frame_pot;      // optional [n_sys * n_frames] potentials at frames
+    int n_frames;
+};
+
+}  // namespace upside
+
+namespace upside {
+// Gather plan header in the constant int arena:
+//   [0] n_target  [1] cw (columns)  [2] target sens offset  [3] target stride
+//   [4] target column  [5] contribution base (float arena)  [6] contribution
+//   row stride  [7] n_idx, then off[n_target+1], then idx[n_idx].
+// target row t receives sum_{j in [off[t],off[t+1])} contrib[idx[j]*stride + c].
+enum { PLAN_HDR = 8 };
+}  // namespace upside

@@ -1,0 +1,1775 @@
+// Host side of the MI355X Upside engine: `.up` loading, lowering of the node
+// graph to a device program, device memory, and the C ABI of include/upside_engine.h.
+//
+// Loading follows initialize_engine_from_hdf5 (src/deriv_engine.cpp:353-426):
+// nodes under /input/potential are ordered topologically by their `arguments`
+// attribute, typed by registered name prefix (:393-399), then germ and deriv
+// execution levels are built as build_exec_levels does (:127-178).  Every node
+// type of the ff_2.1 graph is lowered; any other type makes construction fail
+// with a message, as an unknown type does in the reference.
+#include <hdf5.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "engine_dev.h"
+#include "../../include/upside_engine.h"
+
+hipError_t upside_launch_engine(upside::SysDesc* d_systems, int n_sys, upside::RunArgs args, hipStream_t stream);
+
+namespace upside {
+
+using std::string;
+using std::vector;
+
+[[noreturn]] static void fail(const string& s) { throw std::runtime_error(s); }
+
+static void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) fail(string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------------ HDF5 ---
+struct H5 {
+    hid_t f = -1;
+    explicit H5(const string& path) {
+        H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+        f = H5Fopen(path.c_str(), H5F_ACC_RDONLY, H5P_DEFAULT);
+        if (f < 0) fail("unable to open " + path);
+    }
+    ~H5() { if (f >= 0) H5Fclose(f); }
+    bool exists(const string& p) const {
+        string cur;
+        size_t i = 1;
+        while (i <= p.size()) {
+            size_t j = p.find('/', i);
+            if (j == string::npos) j = p.size();
+            cur = p.substr(0, j);
+            if (H5Lexists(f, cur.c_str(), H5P_DEFAULT) <= 0) return false;
+            i = j + 1;
+        }
+        return true;
+    }
+    vector<size_t> shape(const string& p) const {
+        hid_t d = H5Dopen2(f, p.c_str(), H5P_DEFAULT);
+        if (d < 0) fail("missing dataset " + p);
+        hid_t s = H5Dget_space(d);
+        int nd = H5Sget_simple_extent_ndims(s);
+        vector<hsize_t> dims(nd > 0 ? nd : 0);
+        if (nd > 0) H5Sget_simple_extent_dims(s, dims.data(), nullptr);
+        H5Sclose(s);
+        H5Dclose(d);
+        return vector<size_t>(dims.begin(), dims.end());
+    }
+    template <typename T>
+    vector<T> read(const string& p, hid_t mt) const {
+        auto sh = shape(p);
+        size_t n = 1;
+        for (auto x : sh) n *= x;
+        vector<T> out(n);
+        if (!n) return out;
+        hid_t d = H5Dopen2(f, p.c_str(), H5P_DEFAULT);
+        if (H5Dread(d, mt, H5S_ALL, H5S_ALL, H5P_DEFAULT, out.data()) < 0) {
+            H5Dclose(d);
+            fail("unable to read " + p);
+        }
+        H5Dclose(d);
+        return out;
+    }
+    vector<float> rf(const string& p) const { return read<float>(p, H5T_NATIVE_FLOAT); }
+    vector<double> rd(const string& p) const { return read<double>(p, H5T_NATIVE_DOUBLE); }
+    vector<int> ri(const string& p) const { return read<int>(p, H5T_NATIVE_INT); }
+    bool has_attr(const string& p, const string& a) const {
+        return H5Aexists_by_name(f, p.c_str(), a.c_str(), H5P_DEFAULT) > 0;
+    }
+    double attr_d(const string& p, const string& a) const {
+        if (!has_attr(p, a)) fail("missing attribute " + a + " of " + p);
+        double v = 0;
+        hid_t at = H5Aopen_by_name(f, p.c_str(), a.c_str(), H5P_DEFAULT, H5P_DEFAULT);
+        H5Aread(at, H5T_NATIVE_DOUBLE, &v);
+        H5Aclose(at);
+        return v;
+    }
+    float attr_f(const string& p, const string& a) const {
+        float v = 0;
+        if (!has_attr(p, a)) fail("missing attribute " + a + " of " + p);
+        hid_t at = H5Aopen_by_name(f, p.c_str(), a.c_str(), H5P_DEFAULT, H5P_DEFAULT);
+        H5Aread(at, H5T_NATIVE_FLOAT, &v);
+        H5Aclose(at);
+        return v;
+    }
+    int attr_i(const string& p, const string& a, int dflt, bool required) const {
+        if (!has_attr(p, a)) {
+            if (required) fail("missing attribute " + a + " of " + p);
+            return dflt;
+        }
+        int v = 0;
+        hid_t at = H5Aopen_by_name(f, p.c_str(), a.c_str(), H5P_DEFAULT, H5P_DEFAULT);
+        H5Aread(at, H5T_NATIVE_INT, &v);
+        H5Aclose(at);
+        return v;
+    }
+    vector<string> attr_strings(const string& p, const string& a) const {
+        if (!has_attr(p, a)) fail("missing attribute " + a + " of " + p);
+        hid_t at = H5Aopen_by_name(f, p.c_str(), a.c_str(), H5P_DEFAULT, H5P_DEFAULT);
+        hid_t t = H5Aget_type(at);
+        hid_t s = H5Aget_space(at);
+        if (H5Tis_variable_str(t) > 0) fail("variable-length strings not supported");
+        size_t len = H5Tget_size(t);
+        int nd = H5Sget_simple_extent_ndims(s);
+        hsize_t n = 1;
+        if (nd == 1) H5Sget_simple_extent_dims(s, &n, nullptr);
+        vector<char> buf(n * len + 1, 0);
+        if (n) H5Aread(at, t, buf.data());
+        vector<string> out;
+        for (hsize_t i = 0; i < n; ++i) {
+            string x(buf.data() + i * len, len);
+            while (!x.empty() && x.back() == '\0') x.pop_back();
+            out.push_back(x);
+        }
+        H5Sclose(s);
+        H5Tclose(t);
+        H5Aclose(at);
+        return out;
+    }
+    vector<string> children(const string& p) const {
+        vector<string> out;
+        hid_t g = H5Gopen2(f, p.c_str(), H5P_DEFAULT);
+        if (g < 0) fail("missing group " + p);
+        H5G_info_t info;
+        H5Gget_info(g, &info);
+        for (hsize_t i = 0; i < info.nlinks; ++i) {
+            ssize_t n = H5Lget_name_by_idx(g, ".", H5_INDEX_NAME, H5_ITER_INC, i, nullptr, 0, H5P_DEFAULT);
+            vector<char> b(n + 1);
+            H5Lget_name_by_idx(g, ".", H5_INDEX_NAME, H5_ITER_INC, i, b.data(), n + 1, H5P_DEFAULT);
+            out.emplace_back(b.data());
+        }
+        H5Gclose(g);
+        return out;
+    }
+};
+
+// --------------------------------------------------------------- layout ---
+struct Node {
+    string name, type;
+    vector<string> args;
+    vector<int> parents, children;
+    bool potential = false;
+    int n_elem = 0, width = 0;
+    int out = -1, sens = -1;
+    int pot_slot = -1;
+    int op = -1;
+    int germ_level = -1, deriv_level = -1;
+};
+
+struct Topology {
+    vector<Node> nodes;
+    vector<OpRec> ops;           // one per non-pos node, in node order
+    vector<int> germ, deriv;     // node indices
+    vector<PairGraph> graphs;
+    vector<RotamerDev> rots;
+    vector<int> graph_node;      // graph -> node
+    vector<float> cf;
+    vector<int> ci;
+    int nf = 0, ni = 0;          // per-system arena sizes
+    vector<std::pair<int, int>> sens_zero;
+    vector<int> sens_zero_host;
+    int n_pot = 0, pot = -1, mom = -1, err = -1, szero = -1;
+    int n_atom = 0;
+    double cap_scale = 1.0;
+
+    int falloc(long n) { int o = nf; nf += (int)((n + 3) / 4 * 4); return o; }
+    int ialloc(long n) { int o = ni; ni += (int)((n + 3) / 4 * 4); return o; }
+    int cf_push(const vector<float>& v, int pad = 0) {
+        int o = (int)cf.size() + pad;
+        cf.insert(cf.end(), pad, 0.f);
+        cf.insert(cf.end(), v.begin(), v.end());
+        cf.insert(cf.end(), pad + 4, 0.f);
+        return o;
+    }
+    int ci_push(const vector<int>& v) {
+        int o = (int)ci.size();
+        ci.insert(ci.end(), v.begin(), v.end());
+        ci.insert(ci.end(), 4, 0);
+        return o;
+    }
+    // Plan: target t of a producer receives the sum of contributions in order.
+    int plan(int n_target, int cw, int tsens, int tstride, int col, int cbase, int cstride,
+             const vector<std::pair<int, int>>& contrib) {
+        vector<int> cnt(n_target + 1, 0);
+        for (auto& p : contrib) {
+            if (p.first < 0 || p.first >= n_target) fail("gather plan target out of range");
+            cnt[p.first + 1]++;
+        }
+        for (int t = 0; t < n_target; ++t) cnt[t + 1] += cnt[t];
+        vector<int> idx(contrib.size());
+        vector<int> fill(cnt.begin(), cnt.end() - 1);
+        for (auto& p : contrib) idx[fill[p.first]++] = p.second;
+        vector<int> h = {n_target, cw, tsens, tstride, col, cbase, cstride, (int)contrib.size()};
+        h.insert(h.end(), cnt.begin(), cnt.end());
+        h.insert(h.end(), idx.begin(), idx.end());
+        return ci_push(h);
+    }
+};
+
+static OpRec blank_op(int kind, int node) {
+    OpRec o;
+    std::memset(&o, 0, sizeof(o));
+    o.kind = kind;
+    o.node = node;
+    o.out = o.sens = o.pot_slot = -1;
+    o.scratch = o.iscratch = -1;
+    o.graph = o.rot = -1;
+    for (int k = 0; k < 4; ++k) { o.in_out[k] = o.in_sens[k] = -1; o.plans[k] = -1; }
+    return o;
+}
+
+static const char* kPrefixes[] = {
+    "Distance3D", "Angle", "Dihedral", "Spring", "rama_coord", "cat_pos", "rama_map_pot",
+    "affine_alignment", "placement_scalar", "placement_fixed_point_vector_only",
+    "placement_fixed_point_vector_scalar", "infer_H_O", "protein_hbond", "hbond_coverage",
+    "hbond_energy", "rotamer", "weighted_pos", "environment_coverage", "hb_environment_coverage",
+    "sigmoid_coupling", "bb_sigmoid_coupling", "backbone_pairs"};
+
+static string node_type(const string& name) {
+    string best;
+    for (auto p : kPrefixes) {
+        string s(p);
+        if (name.compare(0, s.size(), s) == 0 && s.size() > best.size()) best = s;
+    }
+    return best;
+}
+
+struct Builder {
+    Topology& T;
+    H5& h5;
+    explicit Builder(Topology& t, H5& h) : T(t), h5(h) {}
+
+    Node& nd(int i) { return T.nodes[i]; }
+    string gp(const Node& n) const { return "/input/potential/" + n.name; }
+
+    void alloc_coord(Node& n, int n_elem, int width) {
+        n.n_elem = n_elem;
+        n.width = width;
+        n.out = T.falloc((long)n_elem * width);
+        n.sens = T.falloc((long)n_elem * width);
+        T.sens_zero.push_back({n.sens, n_elem * width});
+    }
+    void set_inputs(OpRec& o, const Node& n) {
+        for (size_t k = 0; k < n.parents.size() && k < 4; ++k) {
+            const Node& p = T.nodes[n.parents[k]];
+            o.in_out[k] = p.out;
+            o.in_sens[k] = p.sens;
+            o.in_w[k] = p.width;
+        }
+    }
+    const Node& parent(const Node& n, int k) { return T.nodes[n.parents[k]]; }
+    void need_args(const Node& n, size_t k) {
+        if (n.parents.size() != k) fail("node " + n.name + " expects " + std::to_string(k) + " arguments");
+    }
+    void check_range(const vector<int>& v, int lim, const string& what) {
+        for (int x : v)
+            if (x < 0 || x >= lim) fail(what + " index out of range");
+    }
+    // contributions (e*K+k) -> target rows of the producer for all slots k
+    int atom_plan(const Node& prod, const vector<int>& ids, int K, int cbase, int cw) {
+        vector<std::pair<int, int>> con;
+        for (size_t r = 0; r < ids.size(); ++r)
+            if (ids[r] >= 0) con.push_back({ids[r], (int)r});
+        return T.plan(prod.n_elem, cw, prod.sens, prod.width, 0, cbase, cw, con);
+    }
+
+    void build(int idx) {
+        Node& n = nd(idx);
+        string g = gp(n);
+        OpRec o = blank_op(0, idx);
+        const string& t = n.type;
+        if (t == "Distance3D") {
+            need_args(n, 2);
+            auto id = h5.ri(g + "/id");
+            int m = (int)id.size() / 2;
+            check_range(id, parent(n, 0).n_elem, n.name);
+            alloc_coord(n, m, 1);
+            o.kind = OP_DIST3D;
+            o.ia[0] = T.ci_push(id);
+            o.scratch = T.falloc(m * 3);
+            o.ia[1] = T.falloc(m * 6);
+            if (n.parents[0] == n.parents[1]) {
+                o.plans[0] = atom_plan(parent(n, 0), id, 2, o.ia[1], 3);
+            } else {
+                vector<int> a(id.size(), -1), b(id.size(), -1);
+                for (int e = 0; e < m; ++e) { a[2 * e] = id[2 * e]; b[2 * e + 1] = id[2 * e + 1]; }
+                o.plans[0] = atom_plan(parent(n, 0), a, 2, o.ia[1], 3);
+                o.plans[1] = atom_plan(parent(n, 1), b, 2, o.ia[1], 3);
+            }
+        } else if (t == "Angle") {
+            need_args(n, 1);
+            auto id = h5.ri(g + "/id");
+            int m = (int)id.size() / 3;
+            check_range(id, parent(n, 0).n_elem, n.name);
+            alloc_coord(n, m, 1);
+            o.kind = OP_ANGLE;
+            o.ia[0] = T.ci_push(id);
+            o.scratch = T.falloc(m * 6);
+            o.ia[1] = T.falloc(m * 9);
+            o.plans[0] = atom_plan(parent(n, 0), id, 3, o.ia[1], 3);
+        } else if (t == "Dihedral") {
+            need_args(n, 1);
+            auto id = h5.rd(g + "/id");
+            vector<int> ii(id.size());
+            for (size_t k = 0; k < id.size(); ++k) ii[k] = (int)id[k];
+            int m = (int)ii.size() / 4;
+            vector<int> pl(ii.size(), -1);
+            for (int e = 0; e < m; ++e) {
+                bool dummy = ii[4 * e] < 0 || ii[4 * e + 3] < 0;
+                for (int k = 0; k < 4; ++k) {
+                    if (!dummy) {
+                        if (ii[4 * e + k] >= parent(n, 0).n_elem) fail(n.name + " atom index out of range");
+                        pl[4 * e + k] = ii[4 * e + k];
+                    }
+                }
+            }
+            alloc_coord(n, m, 1);
+            o.kind = OP_DIHEDRAL;
+            o.ia[0] = T.ci_push(ii);
+            o.scratch = T.falloc(m * 12);
+            o.ia[1] = T.falloc(m * 12);
+            o.plans[0] = atom_plan(parent(n, 0), pl, 4, o.ia[1], 3);
+        } else if (t == "Spring") {
+            need_args(n, 1);
+            n.potential = true;
+            auto id = h5.ri(g + "/id");
+            const Node& p = parent(n, 0);
+            check_range(id, p.n_elem, n.name);
+            o.kind = OP_SPRING;
+            o.ia[0] = T.ci_push(id);
+            o.ia[1] = T.cf_push(h5.rf(g + "/equil_dist"));
+            o.ia[2] = T.cf_push(h5.rf(g + "/spring_const"));
+            o.ia[3] = h5.attr_i(g, "dim1", 0, true);
+            o.ia[4] = h5.attr_i(g, "pbc", 0, true);
+            o.ia[5] = (int)id.size();
+            o.fa[0] = h5.attr_f(g, "box_len");
+            if (o.ia[3] >= p.width) fail(n.name + ": dim1 beyond argument width");
+            o.scratch = T.falloc(id.size());
+            vector<std::pair<int, int>> con;
+            for (size_t e = 0; e < id.size(); ++e) con.push_back({id[e], (int)e});
+            o.plans[0] = T.plan(p.n_elem, 1, p.sens, p.width, o.ia[3], o.scratch, 1, con);
+        } else if (t == "rama_coord") {
+            need_args(n, 2);
+            auto i1 = h5.ri(g + "/index_pos1");
+            auto i2 = h5.ri(g + "/index_pos2");
+            check_range(i1, parent(n, 0).n_elem, n.name);
+            check_range(i2, parent(n, 1).n_elem, n.name);
+            alloc_coord(n, (int)i1.size(), 2);
+            o.kind = OP_RAMA_COORD;
+            o.ia[0] = T.ci_push(i1);
+            o.ia[1] = T.ci_push(i2);
+            vector<std::pair<int, int>> c0, c1;
+            for (size_t e = 0; e < i1.size(); ++e) { c0.push_back({i1[e], (int)(2 * e)}); c1.push_back({i2[e], (int)(2 * e + 1)}); }
+            const Node &p0 = parent(n, 0), &p1 = parent(n, 1);
+            o.plans[0] = T.plan(p0.n_elem, 1, p0.sens, p0.width, 0, n.sens, 1, c0);
+            o.plans[1] = T.plan(p1.n_elem, 1, p1.sens, p1.width, 0, n.sens, 1, c1);
+        } else if (t == "cat_pos") {
+            need_args(n, 2);
+            auto i1 = h5.ri(g + "/index_pos1");
+            auto i2 = h5.ri(g + "/index_pos2");
+            int w = h5.attr_i(g, "n_dim", 0, true);
+            const Node &p0 = parent(n, 0), &p1 = parent(n, 1);
+            check_range(i1, p0.n_elem, n.name);
+            check_range(i2, p1.n_elem, n.name);
+            if (w > p0.width || w > p1.width) fail(n.name + ": n_dim beyond argument width");
+            alloc_coord(n, (int)(i1.size() + i2.size()), w);
+            o.kind = OP_CAT_POS;
+            vector<int> all(i1);
+            all.insert(all.end(), i2.begin(), i2.end());
+            o.ia[0] = T.ci_push(all);
+            o.ia[1] = (int)i1.size();
+            vector<std::pair<int, int>> c0, c1;
+            for (size_t e = 0; e < i1.size(); ++e) c0.push_back({i1[e], (int)e});
+            for (size_t e = 0; e < i2.size(); ++e) c1.push_back({i2[e], (int)(i1.size() + e)});
+            o.plans[0] = T.plan(p0.n_elem, w, p0.sens, p0.width, 0, n.sens, w, c0);
+            o.plans[1] = T.plan(p1.n_elem, w, p1.sens, p1.width, 0, n.sens, w, c1);
+        } else if (t == "rama_map_pot") {
+            need_args(n, 1);
+            n.potential = true;
+            auto res = h5.ri(g + "/residue_id");
+            auto mid = h5.ri(g + "/rama_map_id");
+            auto sh = h5.shape(g + "/rama_pot");
+            if (sh.size() != 3 || sh[1] != sh[2]) fail(n.name + ": rama_pot must be (L,n,n)");
+            const Node& p = parent(n, 0);
+            check_range(res, p.n_elem, n.name);
+            check_range(mid, (int)sh[0], n.name);
+            auto raw = h5.rd(g + "/rama_pot");
+            o.kind = OP_RAMA_MAP_POT;
+            o.ia[0] = T.ci_push(res);
+            o.ia[1] = T.ci_push(mid);
+            o.ia[2] = T.cf_push(periodic_bicubic((int)sh[0], (int)sh[1], (int)sh[2], raw));
+            o.ia[3] = (int)sh[1];
+            o.ia[4] = (int)res.size();
+            o.fa[0] = (float)sh[1] * (0.5f / 3.141592653589793f - 1e-7f);
+            o.fa[1] = 3.141592653589793f;
+            o.scratch = T.falloc(res.size() * 3);
+            vector<std::pair<int, int>> con;
+            for (size_t e = 0; e < res.size(); ++e) con.push_back({res[e], (int)e});
+            o.plans[0] = T.plan(p.n_elem, 2, p.sens, p.width, 0, o.scratch, 2, con);
+        } else if (t == "affine_alignment") {
+            need_args(n, 1);
+            auto atoms = h5.ri(g + "/atoms");
+            auto ref = h5.rf(g + "/ref_geom");
+            int m = (int)atoms.size() / 3;
+            check_range(atoms, parent(n, 0).n_elem, n.name);
+            alloc_coord(n, m, 7);
+            o.kind = OP_AFFINE;
+            o.ia[0] = T.ci_push(atoms);
+            o.ia[1] = T.cf_push(ref);
+            o.scratch = T.falloc(m * 20);
+            o.ia[2] = T.falloc(m * 9);
+            o.plans[0] = atom_plan(parent(n, 0), atoms, 3, o.ia[2], 3);
+        } else if (t == "placement_scalar" || t == "placement_fixed_point_vector_only" ||
+                   t == "placement_fixed_point_vector_scalar") {
+            auto ares_d = h5.rd(g + "/affine_residue");
+            vector<int> ares(ares_d.size());
+            for (size_t k = 0; k < ares.size(); ++k) ares[k] = (int)ares_d[k];
+            auto layer = h5.ri(g + "/layer_index");
+            auto sh = h5.shape(g + "/placement_data");
+            const Node& aff = parent(n, 0);
+            check_range(ares, aff.n_elem, n.name);
+            o.kind = OP_PLACEMENT;
+            o.ia[0] = T.ci_push(ares);
+            o.ia[1] = T.ci_push(layer);
+            int m = (int)layer.size();
+            if (t == "placement_scalar") {
+                need_args(n, 2);
+                if (sh.size() != 4 || sh[3] != 1) fail(n.name + ": placement_data must be (L,nx,ny,1)");
+                check_range(layer, (int)sh[0], n.name);
+                auto rres = h5.ri(g + "/rama_residue");
+                check_range(rres, parent(n, 1).n_elem, n.name);
+                auto raw = h5.rd(g + "/placement_data");
+                alloc_coord(n, m, 1);
+                o.ia[2] = T.cf_push(periodic_bicubic((int)sh[0], (int)sh[1], (int)sh[2], raw));
+                o.ia[3] = 1;
+                o.ia[4] = T.ci_push(rres);
+                o.ia[5] = (int)sh[1];
+                o.ia[6] = (int)sh[2];
+                o.ia[7] = 1;
+                o.fa[0] = (float)sh[1] * (0.5f / 3.141592653589793f - 1e-7f);
+                o.fa[1] = (float)sh[2] * (0.5f / 3.141592653589793f - 1e-7f);
+                o.fa[2] = 3.141592653589793f;
+                o.scratch = T.falloc(m * 2);
+                o.ia[8] = T.falloc(m * 2);
+                const Node& rp = parent(n, 1);
+                vector<std::pair<int, int>> con;
+                for (int e = 0; e < m; ++e) con.push_back({rres[e], e});
+                o.plans[1] = T.plan(rp.n_elem, 2, rp.sens, rp.width, 0, o.ia[8], 2, con);
+            } else {
+                need_args(n, 1);
+                int dim = t == "placement_fixed_point_vector_only" ? 6 : 7;
+                if (sh.size() != 2 || (int)sh[1] != dim) fail(n.name + ": placement_data width mismatch");
+                check_range(layer, (int)sh[0], n.name);
+                alloc_coord(n, m, dim);
+                o.ia[2] = T.cf_push(h5.rf(g + "/placement_data"));
+                o.ia[3] = 0;
+                o.ia[7] = dim;
+                o.ia[8] = T.falloc(m * 6);
+                vector<std::pair<int, int>> con;
+                for (int e = 0; e < m; ++e) con.push_back({ares[e], e});
+                o.plans[0] = T.plan(aff.n_elem, 6, aff.sens, aff.width, 0, o.ia[8], 6, con);
+            }
+        } else if (t == "infer_H_O") {
+            need_args(n, 1);
+            auto did = h5.ri(g + "/donors/id");
+            auto aid = h5.ri(g + "/acceptors/id");
+            auto dbl = h5.rf(g + "/donors/bond_length");
+            auto abl = h5.rf(g + "/acceptors/bond_length");
+            vector<int> id(did);
+            id.insert(id.end(), aid.begin(), aid.end());
+            vector<float> bl(dbl);
+            bl.insert(bl.end(), abl.begin(), abl.end());
+            check_range(id, parent(n, 0).n_elem, n.name);
+            int m = (int)bl.size();
+            alloc_coord(n, m, 6);
+            o.kind = OP_INFER_HO;
+            o.ia[0] = T.ci_push(id);
+            o.ia[1] = T.cf_push(bl);
+            o.scratch = T.falloc(m * 12);
+            o.ia[2] = T.falloc(m * 9);
+            o.plans[0] = atom_plan(parent(n, 0), id, 3, o.ia[2], 3);
+        } else if (t == "protein_hbond") {
+            need_args(n, 1);
+            o.kind = OP_PROTEIN_HBOND;
+            o.graph = graph(n, g, EDGE_PROTEIN_HBOND, false, &parent(n, 0), &parent(n, 0));
+            const PairGraph& G = T.graphs[o.graph];
+            alloc_coord(n, G.n1 + G.n2, 7);
+            o.scratch = T.falloc(G.n1 + G.n2);
+        } else if (t == "hbond_coverage") {
+            need_args(n, 2);
+            o.kind = OP_HBOND_COVERAGE;
+            o.graph = graph(n, g, EDGE_HBOND_COVERAGE, false, &parent(n, 0), &parent(n, 1));
+            alloc_coord(n, T.graphs[o.graph].n2, 1);
+        } else if (t == "hbond_energy") {
+            need_args(n, 2);
+            n.potential = true;
+            build_hbond_energy(n, g, o);
+        } else if (t == "rotamer") {
+            if (n.parents.size() < 2 || n.parents.size() > 5) fail(n.name + ": needs bead node and 1-4 energy nodes");
+            n.potential = true;
+            build_rotamer(n, g, o);
+        } else if (t == "weighted_pos") {
+            need_args(n, 2);
+            auto ip = h5.ri(g + "/index_pos");
+            auto iw = h5.ri(g + "/index_weight");
+            const Node &p0 = parent(n, 0), &p1 = parent(n, 1);
+            check_range(ip, p0.n_elem, n.name);
+            check_range(iw, p1.n_elem, n.name);
+            int m = (int)ip.size();
+            alloc_coord(n, m, 4);
+            o.kind = OP_WEIGHTED_POS;
+            o.ia[0] = T.ci_push(ip);
+            o.ia[1] = T.ci_push(iw);
+            o.scratch = T.falloc(m);
+            vector<std::pair<int, int>> c0, c1;
+            for (int e = 0; e < m; ++e) { c0.push_back({ip[e], e}); c1.push_back({iw[e], e}); }
+            o.plans[0] = T.plan(p0.n_elem, 3, p0.sens, p0.width, 0, n.sens, 4, c0);
+            o.plans[1] = T.plan(p1.n_elem, 1, p1.sens, p1.width, 0, o.scratch, 1, c1);
+        } else if (t == "environment_coverage") {
+            need_args(n, 2);
+            o.kind = OP_ENV_COVERAGE;
+            o.graph = graph(n, g, EDGE_ENV_WEIGHTED, false, &parent(n, 0), &parent(n, 1));
+            int naa = h5.attr_i(g, "num_aa_types", 0, true);
+            auto aa = h5.ri(g + "/aa_types");
+            const PairGraph& G = T.graphs[o.graph];
+            const int* id2 = T.ci.data() + G.id2;
+            int mx = (int)aa.size();
+            for (int j = 0; j < G.n2; ++j) {
+                if (id2[j] < 0) fail(n.name + ": negative id2");
+                mx = std::max(mx, id2[j] + 1);
+            }
+            for (int x : aa)
+                if (x < 0 || x >= naa) fail(n.name + ": aa type out of range");
+            aa.resize(mx, 0);
+            o.ia[0] = naa;
+            o.ia[1] = T.ci_push(aa);
+            alloc_coord(n, G.n1 * naa, 1);
+        } else if (t == "hb_environment_coverage") {
+            need_args(n, 2);
+            o.kind = OP_HB_ENV_COVERAGE;
+            o.graph = graph(n, g, EDGE_ENV_HBOND, false, &parent(n, 0), &parent(n, 1));
+            alloc_coord(n, T.graphs[o.graph].n1, 1);
+        } else if (t == "sigmoid_coupling") {
+            need_args(n, 1);
+            n.potential = true;
+            const Node& p = parent(n, 0);
+            if (p.width != 1) fail(n.name + ": argument width must be 1");
+            auto scale = h5.rf(g + "/scale");
+            int nt = (int)scale.size();
+            auto ct = h5.ri(g + "/coupling_types");
+            auto W = h5.rf(g + "/weights");
+            if ((int)W.size() < nt * nt) fail(n.name + ": weights too short");
+            check_range(ct, nt, n.name);
+            if ((int)ct.size() * nt > p.n_elem) fail(n.name + ": coupling types exceed input");
+            o.kind = OP_SIGMOID_COUPLING;
+            o.ia[0] = p.n_elem / nt;
+            o.ia[1] = T.ci_push(ct);
+            o.ia[2] = T.cf_push(W);
+            o.ia[3] = T.cf_push(scale);
+            o.ia[4] = T.cf_push(h5.rf(g + "/center"));
+            o.ia[5] = T.cf_push(h5.rf(g + "/sharpness"));
+            o.ia[6] = nt;
+        } else if (t == "bb_sigmoid_coupling") {
+            need_args(n, 2);
+            n.potential = true;
+            const Node &p0 = parent(n, 0), &p1 = parent(n, 1);
+            int nt = h5.attr_i(g, "num_aa_types", 0, true);
+            auto W = h5.rf(g + "/weights");
+            if ((int)W.size() < nt) fail(n.name + ": weights too short");
+            if (p0.width != 1 || p1.width != 1) fail(n.name + ": argument widths must be 1");
+            o.kind = OP_BB_SIGMOID_COUPLING;
+            o.ia[0] = nt;
+            o.ia[1] = T.cf_push(W);
+            o.ia[2] = p0.n_elem / nt;
+            if (o.ia[2] > p1.n_elem) fail(n.name + ": backbone coverage shorter than environment");
+            o.fa[0] = h5.attr_f(g, "scale");
+            o.fa[1] = h5.attr_f(g, "center");
+            o.fa[2] = h5.attr_f(g, "sharpness");
+            o.fa[3] = h5.attr_f(g, "hbond_weight");
+        } else if (t == "backbone_pairs") {
+            need_args(n, 1);
+            n.potential = true;
+            build_backbone(n, g, o);
+        } else {
+            fail("No node type found for name '" + n.name + "'");
+        }
+        if (n.potential) n.pot_slot = T.n_pot++;
+        o.n_elem = n.n_elem;
+        o.width = n.width;
+        o.out = n.out;
+        o.sens = n.sens;
+        o.pot_slot = n.pot_slot;
+        set_inputs(o, n);
+        n.op = (int)T.ops.size();
+        T.ops.push_back(o);
+    }
+
+    // Periodic bicubic coefficients (L,nx,ny,4,4): tensor product of periodic
+    // interpolating cubic splines through data at integer grid points, solved
+    // in double (the unique spline of solve_periodic_2d_spline, src/spline.cpp:262-292).
+    static vector<double> periodic_1d(const vector<double>& d) {
+        int n = (int)d.size();
+        // cyclic tridiagonal system (s[i-1]+4 s[i]+s[i+1])/6 = d[i] by Sherman-Morrison
+        vector<double> a(n, 1. / 6.), b(n, 2. / 3.), c(n, 1. / 6.);
+        auto thomas = [&](vector<double> rhs, vector<double> bb) {
+            vector<double> cc(c);
+            for (int k = 1; k < n; ++k) {
+                double m = a[k] / bb[k - 1];
+                bb[k] -= m * cc[k - 1];
+                rhs[k] -= m * rhs[k - 1];
+            }
+            rhs[n - 1] /= bb[n - 1];
+            for (int k = n - 2; k >= 0; --k) rhs[k] = (rhs[k] - cc[k] * rhs[k + 1]) / bb[k];
+            return rhs;
+        };
+        double gamma = -b[0];
+        vector<double> bb(b);
+        bb[0] -= gamma;
+        bb[n - 1] -= a[0] * c[n - 1] / gamma;
+        vector<double> u(n, 0.);
+        u[0] = gamma;
+        u[n - 1] = c[n - 1];
+        vector<double> y = thomas(d, bb), z = thomas(u, bb);
+        double fact = (y[0] + a[0] * y[n - 1] / gamma) / (1. + z[0] + a[0] * z[n - 1] / gamma);
+        vector<double> s(n);
+        for (int i = 0; i < n; ++i) s[i] = y[i] - fact * z[i];
+        vector<double> coef(4 * n);
+        for (int i = 0; i < n; ++i) {
+            double sm = s[(i - 1 + n) % n], s0 = s[i], s1 = s[(i + 1) % n], s2 = s[(i + 2) % n];
+            coef[4 * i + 0] = (sm + 4 * s0 + s1) / 6.;
+            coef[4 * i + 1] = (s1 - sm) / 2.;
+            coef[4 * i + 2] = (sm - 2 * s0 + s1) / 2.;
+            coef[4 * i + 3] = (-sm + 3 * s0 - 3 * s1 + s2) / 6.;
+        }
+        return coef;
+    }
+    static vector<float> periodic_bicubic(int L, int nx, int ny, const vector<double>& raw) {
+        vector<float> out((size_t)L * nx * ny * 16);
+        for (int l = 0; l < L; ++l) {
+            vector<double> ay((size_t)nx * ny * 4);
+            for (int ix = 0; ix < nx; ++ix) {
+                vector<double> row(raw.begin() + ((size_t)l * nx + ix) * ny, raw.begin() + ((size_t)l * nx + ix + 1) * ny);
+                auto c = periodic_1d(row);
+                std::copy(c.begin(), c.end(), ay.begin() + (size_t)ix * ny * 4);
+            }
+            for (int iy = 0; iy < ny; ++iy)
+                for (int py = 0; py < 4; ++py) {
+                    vector<double> col(nx);
+                    for (int ix = 0; ix < nx; ++ix) col[ix] = ay[((size_t)ix * ny + iy) * 4 + py];
+                    auto c = periodic_1d(col);
+                    for (int ix = 0; ix < nx; ++ix)
+                        for (int px = 0; px < 4; ++px)
+                            out[((((size_t)l * nx + ix) * ny + iy) * 16) + px * 4 + py] = (float)c[4 * ix + px];
+                }
+        }
+        return out;
+    }
+
+    int cap(int n, int want) { return std::max(4, std::min(n, (int)(want * T.cap_scale))); }
+
+    // InteractionGraph (src/interaction_graph.h:262-405) lowering.
+    int graph(const Node& n, const string& g, int kind, bool sym, const Node* p1, const Node* p2) {
+        PairGraph G;
+        std::memset(&G, 0, sizeof(G));
+        G.kind = kind;
+        G.sym = sym;
+        if (sym) p2 = p1;
+        string s1 = sym ? "" : "1";
+        auto loc1 = h5.ri(g + "/index" + s1);
+        auto ty1 = h5.ri(g + "/type" + s1);
+        auto idd1 = h5.rd(g + "/id" + s1);
+        vector<int> id1(idd1.size());
+        for (size_t k = 0; k < id1.size(); ++k) id1[k] = (int)idd1[k];
+        vector<int> loc2, ty2, id2;
+        if (sym) {
+            loc2 = loc1; ty2 = ty1; id2 = id1;
+        } else {
+            loc2 = h5.ri(g + "/index2");
+            ty2 = h5.ri(g + "/type2");
+            auto idd2 = h5.rd(g + "/id2");
+            for (double x : idd2) id2.push_back((int)x);
+        }
+        if (ty1.size() != loc1.size() || id1.size() != loc1.size() || ty2.size() != loc2.size() ||
+            id2.size() != loc2.size())
+            fail(n.name + ": index/type/id size mismatch");
+        check_range(loc1, p1->n_elem, n.name);
+        check_range(loc2, p2->n_elem, n.name);
+        auto sh = h5.shape(g + "/interaction_param");
+        if (sh.size() != 3) fail(n.name + ": interaction_param must be 3-d");
+        auto param = h5.rf(g + "/interaction_param");
+        int nt1 = (int)sh[0], nt2 = (int)sh[1], np = (int)sh[2];
+        check_range(ty1, nt1, n.name);
+        check_range(ty2, nt2, n.name);
+        // within one side each producer row may be fed by one element only
+        for (auto* L : {&loc1, &loc2}) {
+            vector<int> s(*L);
+            std::sort(s.begin(), s.end());
+            if (std::adjacent_find(s.begin(), s.end()) != s.end()) fail(n.name + ": repeated producer index");
+        }
+        G.n1 = (int)loc1.size();
+        G.n2 = (int)loc2.size();
+        G.loc1 = T.ci_push(loc1); G.loc2 = T.ci_push(loc2);
+        G.type1 = T.ci_push(ty1); G.type2 = T.ci_push(ty2);
+        G.id1 = T.ci_push(id1); G.id2 = T.ci_push(id2);
+        G.param = T.cf_push(param, 64);
+        G.n_type2 = nt2;
+        G.n_param = np;
+        G.w1 = p1->width; G.w2 = p2->width;
+        G.out1 = p1->out; G.out2 = p2->out;
+        G.sens1 = p1->sens; G.sens2 = p2->sens;
+        float cutoff = 0.f, buffer = 0.f;
+        int want = 128;
+        switch (kind) {
+        case EDGE_ROTAMER: {
+            G.dim1 = G.dim2 = 6; G.excl = -1;
+            float inv_dx = 1.f / 0.7f;
+            cutoff = (float)((12 - 2 - 1e-6) / inv_dx);
+            if (np != 54) fail(n.name + ": rotamer pair table must have 54 parameters");
+            want = 384;
+        } break;
+        case EDGE_HBOND_COVERAGE: {
+            G.dim1 = 7; G.dim2 = 6; G.excl = 2;
+            float inv_dx = 1.f / 0.625f;
+            cutoff = (float)((10 - 2 - 1e-6) / inv_dx);
+            if (np != 50) fail(n.name + ": coverage table must have 50 parameters");
+            want = 256;
+        } break;
+        case EDGE_PROTEIN_HBOND:
+            G.dim1 = G.dim2 = 6; G.excl = 1;
+            cutoff = sqrtf(3.5f * 3.5f);
+            if (np != 8) fail(n.name + ": hbond table must have 8 parameters");
+            want = 64;
+            break;
+        case EDGE_ENV_WEIGHTED:
+        case EDGE_ENV_HBOND:
+            G.dim1 = 6; G.dim2 = kind == EDGE_ENV_WEIGHTED ? 4 : 6; G.excl = 2;
+            if (np != 4) fail(n.name + ": coverage table must have 4 parameters");
+            for (int k = 0; k < nt1 * nt2; ++k) cutoff = std::max(cutoff, param[k * 4] + 1.f / param[k * 4 + 1]);
+            want = kind == EDGE_ENV_WEIGHTED ? 512 : 64;
+            break;
+        }
+        if (p1->width < G.dim1 || p2->width < G.dim2) fail(n.name + ": argument narrower than interaction");
+        buffer = 1.0f + 0.2f * cutoff;
+        G.ddim1 = G.dim1; G.ddim2 = G.dim2;
+        finish_graph(G, cutoff, buffer, want);
+        T.graphs.push_back(G);
+        T.graph_node.push_back((int)(&n - T.nodes.data()));
+        return (int)T.graphs.size() - 1;
+    }
+
+    void finish_graph(PairGraph& G, float cutoff, float buffer, int want) {
+        G.cutoff2 = cutoff * cutoff;
+        float cc = cutoff + buffer;
+        G.cache_cut2 = cc * cc;
+        G.drift2 = (0.5f * buffer) * (0.5f * buffer);
+        G.maxn = cap(G.sym ? G.n1 : G.n2, want);
+        G.maxt = cap(G.n1, want);
+        int npack = G.sym ? G.n1 : G.n1 + G.n2;
+        G.cache_pos = T.falloc(npack * 3);
+        G.pos_pack = T.falloc(npack * 3);
+        long slots = (long)G.n1 * G.maxn;
+        G.cnt = T.ialloc(G.n1);
+        G.nbr = T.ialloc(slots);
+        G.tcnt = T.ialloc(G.n2);
+        G.tslot = T.ialloc((long)G.n2 * G.maxt);
+        G.val = T.falloc(slots);
+        G.d1 = T.falloc(slots * G.ddim1);
+        G.d2 = T.falloc(slots * G.ddim2);
+        G.esens = T.falloc(slots);
+        G.act = T.ialloc(slots);
+        G.state = T.ialloc(4);
+    }
+
+    void build_backbone(Node& n, const string& g, OpRec& o) {
+        const Node& aff = parent(n, 0);
+        auto res = h5.ri(g + "/id");
+        auto nat = h5.ri(g + "/n_atom");
+        auto ref = h5.rf(g + "/ref_pos");
+        int m = (int)res.size();
+        check_range(res, aff.n_elem, n.name);
+        if ((int)nat.size() != m || (int)ref.size() != m * 12) fail(n.name + ": size mismatch");
+        float mx = 0.f;
+        for (int r = 0; r < m; ++r) {
+            if (nat[r] < 0 || nat[r] > 4) fail(n.name + ": n_atom out of range");
+            for (int a = 0; a < nat[r]; ++a) {
+                const float* q = &ref[(r * 4 + a) * 3];
+                float m2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+                mx = std::max(m2 * (1.f / sqrtf(m2)), mx);
+            }
+        }
+        for (auto& x : ref)
+            if (std::isnan(x)) x = 0.f;
+        float cutoff = 2 * mx + sqrtf(3.f * 3.f + 0.1f * 3.f);
+        PairGraph G;
+        std::memset(&G, 0, sizeof(G));
+        G.kind = EDGE_BACKBONE;
+        G.sym = 1;
+        G.n1 = G.n2 = m;
+        G.dim1 = G.dim2 = 3;
+        G.ddim1 = G.ddim2 = 6;
+        G.w1 = G.w2 = aff.width;
+        G.out1 = G.out2 = aff.out;
+        G.sens1 = G.sens2 = aff.sens;
+        G.loc1 = G.loc2 = T.ci_push(res);
+        vector<int> zeros(m, 0);
+        G.type1 = G.type2 = T.ci_push(zeros);
+        G.id1 = G.id2 = T.ci_push(res);
+        G.param = T.cf_push(vector<float>(1, 0.f), 64);
+        G.n_type2 = 1;
+        G.n_param = 0;
+        G.excl = 1;
+        finish_graph(G, cutoff, 1.0f, 64);
+        o.kind = OP_BACKBONE_PAIRS;
+        o.ia[0] = G.loc1;
+        o.ia[1] = T.cf_push(ref);
+        o.ia[2] = T.ci_push(nat);
+        T.graphs.push_back(G);
+        T.graph_node.push_back((int)(&n - T.nodes.data()));
+        o.graph = (int)T.graphs.size() - 1;
+    }
+
+    void build_hbond_energy(Node& n, const string& g, OpRec& o) {
+        const Node& pp = parent(n, 0);
+        const Node& rama = parent(n, 1);
+        auto params = h5.rf(g + "/parameters");
+        auto dres = h5.ri(g + "/donor_resid");
+        auto ares = h5.ri(g + "/acceptor_resid");
+        int n_res = (int)h5.shape(g + "/rama_resid")[0];
+        int nd = (int)dres.size(), na = (int)ares.size();
+        if (params.size() < 12) fail(n.name + ": 12 parameters expected");
+        if (nd + na != pp.n_elem || pp.width < 7) fail(n.name + ": donors+acceptors must match protein_hbond");
+        if (rama.n_elem < n_res || rama.width < 2) fail(n.name + ": rama node too small");
+        check_range(dres, n_res, n.name);
+        check_range(ares, n_res, n.name);
+        vector<vector<int>> h1(n_res), h2(std::max(n_res - 1, 0));
+        vector<int> fd(n_res, -1), fa(n_res, -1);
+        for (int i = 0; i < nd; ++i) {
+            h1[dres[i]].push_back(i);
+            if (dres[i] - 1 >= 0 && dres[i] - 1 < n_res - 1) h2[dres[i] - 1].push_back(i);
+            if (fd[dres[i]] < 0) fd[dres[i]] = i;
+        }
+        for (int i = 0; i < na; ++i) {
+            h1[ares[i]].push_back(nd + i);
+            if (ares[i] < n_res - 1) h2[ares[i]].push_back(nd + i);
+            if (fa[ares[i]] < 0) fa[ares[i]] = nd + i;
+        }
+        auto csr = [&](const vector<vector<int>>& L, vector<int>& off, vector<int>& idx) {
+            off.assign(1, 0);
+            for (auto& l : L) { idx.insert(idx.end(), l.begin(), l.end()); off.push_back((int)idx.size()); }
+        };
+        vector<int> o1, i1, o2, i2;
+        csr(h1, o1, i1);
+        csr(h2, o2, i2);
+        o.kind = OP_HBOND_ENERGY;
+        o.ia[0] = n_res;
+        o.ia[1] = T.ci_push(o1);
+        o.ia[2] = T.ci_push(i1);
+        o.ia[3] = T.ci_push(o2);
+        o.ia[4] = T.ci_push(i2);
+        o.ia[5] = T.ci_push(fd);
+        o.ia[6] = T.ci_push(fa);
+        for (int k = 0; k < 12; ++k) o.fa[k] = params[k];
+    }
+
+    void build_rotamer(Node& n, const string& g, OpRec& o) {
+        const Node& beads = parent(n, 0);
+        o.kind = OP_ROTAMER;
+        o.graph = graph(n, g + "/pair_interaction", EDGE_ROTAMER, true, &beads, nullptr);
+        PairGraph& G = T.graphs[o.graph];
+        const int* ids = T.ci.data() + G.id1;
+        int nb = G.n1;
+        vector<int> node_of(nb), nrot, bead0, brot(nb);
+        std::map<int, int> key_node;
+        for (int b = 0; b < nb; ++b) {
+            int id = ids[b];
+            int rot = id & 15, nr = (id >> 4) & 15;
+            if (rot >= nr || !(nr == 1 || nr == 3 || nr == 6))
+                fail(n.name + ": invalid rotamer id (rotamer counts 1, 3, 6 supported)");
+            int key = id >> 4;
+            auto it = key_node.find(key);
+            int node;
+            if (it == key_node.end()) {
+                node = (int)nrot.size();
+                key_node[key] = node;
+                nrot.push_back(nr);
+                bead0.push_back(b);
+            } else {
+                node = it->second;
+                if (node != (int)nrot.size() - 1) fail(n.name + ": beads of a residue must be contiguous");
+            }
+            node_of[b] = node;
+            brot[b] = rot;
+        }
+        bead0.push_back(nb);
+        RotamerDev R;
+        std::memset(&R, 0, sizeof(R));
+        R.n_node = (int)nrot.size();
+        R.node_nrot = T.ci_push(nrot);
+        R.node_bead0 = T.ci_push(bead0);
+        R.bead_rot = T.ci_push(brot);
+        R.n_prob = (int)n.parents.size() - 1;
+        for (int q = 0; q < R.n_prob; ++q) {
+            const Node& p = parent(n, q + 1);
+            if (p.n_elem != beads.n_elem)
+                fail(n.name + ": energy node " + p.name + " must have one element per bead position");
+            R.prob_out[q] = p.out;
+            R.prob_sens[q] = p.sens;
+            R.prob_w[q] = p.width;
+        }
+        R.damping = h5.attr_f(g, "damping");
+        R.max_iter = h5.attr_i(g, "max_iter", 0, true);
+        R.tol = h5.attr_f(g, "tol");
+        R.chunk = h5.attr_i(g, "iteration_chunk_size", 0, true);
+        if (R.chunk < 1) fail(n.name + ": iteration_chunk_size must be positive");
+        R.maxc = cap(R.n_node, 96);
+        R.maxi = cap(R.n_node, 192);
+        int nn = R.n_node;
+        long ncand = (long)nn * R.maxc;
+        R.ccnt = T.ialloc(nn);
+        R.cnbr = T.ialloc(ncand);
+        R.icnt = T.ialloc(nn);
+        R.iinc = T.ialloc((long)nn * R.maxi);
+        R.cpres = T.ialloc(ncand);
+        R.cP = T.falloc(ncand * 36);
+        R.cmsg = T.falloc(ncand * 24);
+        R.cmarg = T.falloc(ncand * 36);
+        R.nprob = T.falloc(nn * 6);
+        R.ncur = T.falloc(nn * 6);
+        R.nold = T.falloc(nn * 6);
+        R.noff = T.falloc(nn);
+        R.stats = T.ialloc(4);
+        R.edge_cut2 = G.cache_cut2;
+        o.ia[0] = T.ci_push(node_of);
+        T.rots.push_back(R);
+        o.rot = (int)T.rots.size() - 1;
+    }
+};
+
+static void load_topology(Topology& T, H5& h5, int n_atom) {
+    T.n_atom = n_atom;
+    T.nf = T.ni = 0;
+    T.cf.clear();
+    T.ci.clear();
+    T.nodes.clear();
+    T.ops.clear();
+    T.graphs.clear();
+    T.rots.clear();
+    T.graph_node.clear();
+    T.sens_zero.clear();
+    T.n_pot = 0;
+    string pg = "/input/potential";
+    std::map<string, vector<string>> deps;
+    deps["pos"] = {};
+    for (auto& nm : h5.children(pg)) deps[nm] = h5.attr_strings(pg + "/" + nm, "arguments");
+    for (auto& kv : deps)
+        for (auto& a : kv.second)
+            if (!deps.count(a))
+                fail("Node " + kv.first + " takes " + a + " as an argument, but no node of that name can be found.");
+    vector<string> order;
+    std::set<string> done;
+    std::map<string, bool> active;
+    for (auto& kv : deps) active[kv.first] = true;
+    for (size_t round = 0; round < deps.size(); ++round)
+        for (auto& kv : deps) {
+            if (!active[kv.first]) continue;
+            bool ok = true;
+            for (auto& a : kv.second)
+                if (!done.count(a)) { ok = false; break; }
+            if (ok) { order.push_back(kv.first); done.insert(kv.first); active[kv.first] = false; }
+        }
+    for (auto& kv : active)
+        if (kv.second) fail("Unsatisfiable dependency " + kv.first + " in potential computation");
+    // pos node first (DerivEngine ctor), then the others in topological order
+    Node pos;
+    pos.name = "pos";
+    pos.type = "pos";
+    T.nodes.push_back(pos);
+    Builder B(T, h5);
+    B.alloc_coord(T.nodes[0], n_atom, 3);
+    std::map<string, int> index;
+    index["pos"] = 0;
+    for (auto& nm : order) {
+        if (nm == "pos") continue;
+        Node n;
+        n.name = nm;
+        n.type = node_type(nm);
+        if (n.type.empty()) fail("No node type found for name '" + nm + "'");
+        n.args = deps[nm];
+        for (auto& a : n.args) {
+            int p = index.at(a);
+            if (T.nodes[p].out < 0)
+                fail(a + " is not an intermediate value, but it is an argument of " + nm);
+            n.parents.push_back(p);
+        }
+        index[nm] = (int)T.nodes.size();
+        T.nodes.push_back(n);
+        for (int p : T.nodes.back().parents) T.nodes[p].children.push_back(index[nm]);
+        try {
+            B.build((int)T.nodes.size() - 1);
+        } catch (std::exception& e) {
+            fail("while adding '" + nm + "', " + e.what());
+        }
+    }
+    // execution levels (src/deriv_engine.cpp:127-178)
+    auto& N = T.nodes;
+    N[0].germ_level = 0;
+    T.germ.assign(1, 0);
+    T.deriv.clear();
+    for (int lvl = 1;; ++lvl) {
+        bool unfinished = false;
+        for (int i = 0; i < (int)N.size(); ++i) {
+            auto& n = N[i];
+            if (n.germ_level == -1) {
+                unfinished = true;
+                bool ok = true;
+                for (int p : n.parents) if (N[p].germ_level == -1 || N[p].germ_level == lvl) ok = false;
+                if (ok) { n.germ_level = lvl; T.germ.push_back(i); }
+            }
+            if (n.deriv_level == -1 && n.germ_level != -1 && i != 0) {
+                unfinished = true;
+                bool ok = true;
+                for (int ch : n.children) if (N[ch].deriv_level == -1 || N[ch].deriv_level == lvl) ok = false;
+                if (ok) { n.deriv_level = lvl; T.deriv.push_back(i); }
+            }
+        }
+        if (!unfinished) break;
+        if (lvl > 10000) fail("cyclic node graph");
+    }
+    T.pot = T.falloc(std::max(T.n_pot, 1));
+    T.mom = T.falloc(n_atom * 3);
+    T.err = T.ialloc(4);
+    vector<int> z;
+    for (auto& r : T.sens_zero) { z.push_back(r.first); z.push_back(r.second); }
+    T.szero = T.ialloc(z.size());
+    T.sens_zero_host = z;
+}
+
+}  // namespace upside
+
+// ============================================================== runtime ===
+namespace upside {
+
+// Host Threefry-4x32-20 (same algorithm as the device one) for replica
+// exchange decisions (RandomGenerator, src/random.h:19-67).
+static void threefry_host(const uint32_t ctr[4], const uint32_t key[4], uint32_t out[4]) {
+    static const int R0[8] = {10, 11, 13, 23, 6, 17, 25, 18};
+    static const int R1[8] = {26, 21, 27, 5, 20, 11, 10, 20};
+    auto rotl = [](uint32_t v, int r) { return (v << r) | (v >> (32 - r)); };
+    uint32_t ks[5] = {key[0], key[1], key[2], key[3], 0x1BD11BDAu ^ key[0] ^ key[1] ^ key[2] ^ key[3]};
+    uint32_t x[4] = {ctr[0] + ks[0], ctr[1] + ks[1], ctr[2] + ks[2], ctr[3] + ks[3]};
+    for (int r = 0; r < 20; ++r) {
+        if ((r & 1) == 0) {
+            x[0] += x[1]; x[1] = rotl(x[1], R0[r & 7]) ^ x[0];
+            x[2] += x[3]; x[3] = rotl(x[3], R1[r & 7]) ^ x[2];
+        } else {
+            x[0] += x[3]; x[3] = rotl(x[3], R0[r & 7]) ^ x[0];
+            x[2] += x[1]; x[1] = rotl(x[1], R1[r & 7]) ^ x[2];
+        }
+        if ((r & 3) == 3) {
+            int s = (r >> 2) + 1;
+            for (int i = 0; i < 4; ++i) x[i] += ks[(s + i) % 5];
+            x[3] += (uint32_t)s;
+        }
+    }
+    for (int i = 0; i < 4; ++i) out[i] = x[i];
+}
+
+struct Program {
+    string path;
+    Topology T;
+    OpRec* d_ops = nullptr;
+    PairGraph* d_graphs = nullptr;
+    RotamerDev* d_rots = nullptr;
+    float* d_cf = nullptr;
+    int* d_ci = nullptr;
+
+    void release() {
+        (void)hipFree(d_ops); (void)hipFree(d_graphs); (void)hipFree(d_rots); (void)hipFree(d_cf); (void)hipFree(d_ci);
+        d_ops = nullptr; d_graphs = nullptr; d_rots = nullptr; d_cf = nullptr; d_ci = nullptr;
+    }
+    ~Program() { release(); }
+
+    void load(int n_atom, double cap_scale) {
+        H5 h5(path);
+        if (n_atom <= 0) n_atom = (int)h5.shape("/input/pos")[0];
+        T.cap_scale = cap_scale;
+        load_topology(T, h5, n_atom);
+        release();
+        vector<OpRec> prog;
+        for (size_t k = 1; k < T.germ.size(); ++k) prog.push_back(T.ops[T.nodes[T.germ[k]].op]);
+        for (int i : T.deriv) prog.push_back(T.ops[T.nodes[i].op]);
+        auto up = [](void** dst, const void* src, size_t bytes, const char* what) {
+            hip_check(hipMalloc(dst, std::max(bytes, (size_t)16)), what);
+            if (bytes) hip_check(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), what);
+        };
+        up((void**)&d_ops, prog.data(), prog.size() * sizeof(OpRec), "program");
+        up((void**)&d_graphs, T.graphs.data(), T.graphs.size() * sizeof(PairGraph), "graphs");
+        up((void**)&d_rots, T.rots.data(), T.rots.size() * sizeof(RotamerDev), "rotamers");
+        up((void**)&d_cf, T.cf.data(), T.cf.size() * sizeof(float), "constants");
+        up((void**)&d_ci, T.ci.data(), T.ci.size() * sizeof(int), "indices");
+    }
+    int n_germ() const { return (int)T.germ.size() - 1; }
+    int n_deriv() const { return (int)T.deriv.size(); }
+};
+
+struct System {
+    std::shared_ptr<Program> prog;
+    float* arena = nullptr;
+    int* iarena = nullptr;
+    float temperature = 1.f;
+    void release() { (void)hipFree(arena); (void)hipFree(iarena); arena = nullptr; iarena = nullptr; }
+};
+
+struct Batch {
+    vector<std::shared_ptr<Program>> progs;
+    vector<System> sys;
+    vector<SysDesc> desc;        // host mirror
+    SysDesc* d_desc = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.f;
+    float dt = 0.009f;
+    int thermostat_interval = 1;
+    uint32_t base_seed = 0;
+    bool have_mom = false;
+    double cap_scale = 1.0;
+
+    ~Batch() {
+        for (auto& s : sys) s.release();
+        (void)hipFree(d_desc);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void alloc_system(System& s) {
+        const Topology& T = s.prog->T;
+        hip_check(hipMalloc((void**)&s.arena, (size_t)std::max(T.nf, 4) * sizeof(float)), "arena");
+        hip_check(hipMalloc((void**)&s.iarena, (size_t)std::max(T.ni, 4) * sizeof(int)), "int arena");
+        hip_check(hipMemset(s.arena, 0, (size_t)std::max(T.nf, 4) * sizeof(float)), "arena clear");
+        hip_check(hipMemset(s.iarena, 0, (size_t)std::max(T.ni, 4) * sizeof(int)), "int arena clear");
+        if (!T.sens_zero_host.empty())
+            hip_check(hipMemcpy(s.iarena + T.szero, T.sens_zero_host.data(), T.sens_zero_host.size() * sizeof(int),
+                                hipMemcpyHostToDevice), "sens ranges");
+    }
+
+    SysDesc make_desc(int k) {
+        System& s = sys[k];
+        const Program& P = *s.prog;
+        const Topology& T = P.T;
+        SysDesc d;
+        std::memset(&d, 0, sizeof(d));
+        d.arena = s.arena;
+        d.iarena = s.iarena;
+        d.ops = P.d_ops;
+        d.n_germ = P.n_germ();
+        d.n_deriv = P.n_deriv();
+        d.graphs = P.d_graphs;
+        d.rots = P.d_rots;
+        d.cf = P.d_cf;
+        d.ci = P.d_ci;
+        d.n_atom = T.n_atom;
+        d.pos_out = T.nodes[0].out;
+        d.pos_sens = T.nodes[0].sens;
+        d.n_pot = T.n_pot;
+        d.pot = T.pot;
+        d.sens_zero_cnt = (int)T.sens_zero.size();
+        d.sens_zero = T.szero;
+        d.mom = T.mom;
+        d.error = T.err;
+        d.seed = base_seed + (uint32_t)k;
+        // OrnsteinUhlenbeckThermostat parameters (src/thermostat.h:9-12, src/main.cpp:775-781)
+        float delta_t = (float)thermostat_interval * 3.f * dt;
+        float timescale = 5.f;
+        float ms = std::exp(-delta_t / timescale);
+        d.mom_scale = ms;
+        d.noise_scale = std::sqrt(s.temperature * (1.f - ms * ms));
+        d.n_invocations = 0;
+        return d;
+    }
+
+    void create(const vector<string>& paths, const vector<float>& temps, int n_atom_hint) {
+        std::map<string, std::shared_ptr<Program>> by_path;
+        hip_check(hipStreamCreate(&stream), "stream");
+        hip_check(hipEventCreate(&ev0), "event");
+        hip_check(hipEventCreate(&ev1), "event");
+        for (size_t k = 0; k < paths.size(); ++k) {
+            auto it = by_path.find(paths[k]);
+            if (it == by_path.end()) {
+                auto P = std::make_shared<Program>();
+                P->path = paths[k];
+                P->load(n_atom_hint, cap_scale);
+                progs.push_back(P);
+                it = by_path.emplace(paths[k], P).first;
+            }
+            System s;
+            s.prog = it->second;
+            s.temperature = k < temps.size() ? temps[k] : 1.f;
+            sys.push_back(s);
+        }
+        for (auto& s : sys) alloc_system(s);
+        desc.clear();
+        for (size_t k = 0; k < sys.size(); ++k) desc.push_back(make_desc((int)k));
+        hip_check(hipMalloc((void**)&d_desc, desc.size() * sizeof(SysDesc)), "descriptors");
+        push_desc();
+    }
+
+    void push_desc() {
+        hip_check(hipMemcpy(d_desc, desc.data(), desc.size() * sizeof(SysDesc), hipMemcpyHostToDevice), "descriptors");
+    }
+    void pull_desc() {
+        hip_check(hipMemcpy(desc.data(), d_desc, desc.size() * sizeof(SysDesc), hipMemcpyDeviceToHost), "descriptors");
+    }
+
+    bool overflowed() {
+        bool of = false;
+        for (auto& s : sys) {
+            const Topology& T = s.prog->T;
+            for (auto& G : T.graphs) {
+                int st[4];
+                hip_check(hipMemcpy(st, s.iarena + G.state, sizeof(st), hipMemcpyDeviceToHost), "graph state");
+                if (st[1]) of = true;
+            }
+        }
+        return of;
+    }
+
+    // Re-lower every program with larger list capacities, keeping positions,
+    // momenta and thermostat counters.
+    void grow() {
+        cap_scale *= 2.0;
+        if (cap_scale > 64.0) fail("neighbour list capacity growth limit reached");
+        vector<vector<float>> pos(sys.size()), mom(sys.size());
+        pull_desc();
+        for (size_t k = 0; k < sys.size(); ++k) {
+            const Topology& T = sys[k].prog->T;
+            pos[k].resize(T.n_atom * 3);
+            mom[k].resize(T.n_atom * 3);
+            hip_check(hipMemcpy(pos[k].data(), sys[k].arena + T.nodes[0].out, pos[k].size() * 4, hipMemcpyDeviceToHost), "pos");
+            hip_check(hipMemcpy(mom[k].data(), sys[k].arena + T.mom, mom[k].size() * 4, hipMemcpyDeviceToHost), "mom");
+        }
+        for (auto& P : progs) P->load(P->T.n_atom, cap_scale);
+        for (size_t k = 0; k < sys.size(); ++k) {
+            sys[k].release();
+            alloc_system(sys[k]);
+            uint64_t inv = desc[k].n_invocations;
+            desc[k] = make_desc((int)k);
+            desc[k].n_invocations = inv;
+            const Topology& T = sys[k].prog->T;
+            hip_check(hipMemcpy(sys[k].arena + T.nodes[0].out, pos[k].data(), pos[k].size() * 4, hipMemcpyHostToDevice), "pos");
+            hip_check(hipMemcpy(sys[k].arena + T.mom, mom[k].data(), mom[k].size() * 4, hipMemcpyHostToDevice), "mom");
+        }
+        push_desc();
+    }
+
+    void launch(const RunArgs& a) {
+        hip_check(hipEventRecord(ev0, stream), "event");
+        hip_check(upside_launch_engine(d_desc, (int)sys.size(), a, stream), "engine launch");
+        hip_check(hipEventRecord(ev1, stream), "event");
+        hip_check(hipStreamSynchronize(stream), "engine run");
+        (void)hipEventElapsedTime(&last_ms, ev0, ev1);
+    }
+
+    // Evaluate potential and derivative of every system at its positions.
+    void evaluate(int mode) {
+        RunArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.mode = mode;
+        for (int attempt = 0;; ++attempt) {
+            launch(a);
+            if (!overflowed()) break;
+            grow();
+        }
+    }
+
+    void run(int n_round, int frame_interval, float* frame_pot, int n_frames, int first_round) {
+        // snapshot for a retry after list overflow
+        vector<vector<float>> pos(sys.size()), mom(sys.size());
+        pull_desc();
+        vector<SysDesc> saved = desc;
+        for (size_t k = 0; k < sys.size(); ++k) {
+            const Topology& T = sys[k].prog->T;
+            pos[k].resize(T.n_atom * 3);
+            mom[k].resize(T.n_atom * 3);
+            hip_check(hipMemcpy(pos[k].data(), sys[k].arena + T.nodes[0].out, pos[k].size() * 4, hipMemcpyDeviceToHost), "pos");
+            hip_check(hipMemcpy(mom[k].data(), sys[k].arena + T.mom, mom[k].size() * 4, hipMemcpyDeviceToHost), "mom");
+        }
+        float* d_frames = nullptr;
+        if (frame_pot && n_frames > 0)
+            hip_check(hipMalloc((void**)&d_frames, sys.size() * n_frames * sizeof(float)), "frames");
+        RunArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.mode = 0;
+        a.n_round = n_round;
+        a.dt = dt;
+        a.thermostat_interval = thermostat_interval;
+        a.first_round = first_round;
+        a.frame_interval = frame_interval;
+        a.frame_pot = d_frames;
+        a.n_frames = n_frames;
+        for (;;) {
+            launch(a);
+            if (!overflowed()) break;
+            grow();
+            for (size_t k = 0; k < sys.size(); ++k) {
+                const Topology& T = sys[k].prog->T;
+                hip_check(hipMemcpy(sys[k].arena + T.nodes[0].out, pos[k].data(), pos[k].size() * 4, hipMemcpyHostToDevice), "pos");
+                hip_check(hipMemcpy(sys[k].arena + T.mom, mom[k].data(), mom[k].size() * 4, hipMemcpyHostToDevice), "mom");
+                desc[k].n_invocations = saved[k].n_invocations;
+            }
+            push_desc();
+        }
+        if (d_frames) {
+            hip_check(hipMemcpy(frame_pot, d_frames, sys.size() * n_frames * sizeof(float), hipMemcpyDeviceToHost), "frames");
+            (void)hipFree(d_frames);
+        }
+        pull_desc();
+    }
+
+    // Fresh-start thermalisation (src/main.cpp:800-803): p = 0 then one
+    // thermostat application with the run's parameters.
+    void thermalize() {
+        for (size_t k = 0; k < sys.size(); ++k) {
+            const Topology& T = sys[k].prog->T;
+            hip_check(hipMemset(sys[k].arena + T.mom, 0, T.n_atom * 3 * sizeof(float)), "mom");
+        }
+        RunArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.mode = 0;
+        a.n_round = -1;  // thermostat only
+        a.thermostat_interval = 1;
+        launch(a);
+        pull_desc();
+        have_mom = true;
+    }
+
+    float potential(int k) {
+        const Topology& T = sys[k].prog->T;
+        vector<float> p(std::max(T.n_pot, 1));
+        hip_check(hipMemcpy(p.data(), sys[k].arena + T.pot, p.size() * 4, hipMemcpyDeviceToHost), "potential");
+        float e = 0.f;
+        for (int i = 0; i < T.n_pot; ++i) e += p[i];
+        return e;
+    }
+
+    const Node* node(int k, const char* name) {
+        for (auto& n : sys[k].prog->T.nodes)
+            if (n.name == name) return &n;
+        fail(string("name not found: ") + name);
+    }
+
+    void copy_pos_in(int k, const float* pos) {
+        const Topology& T = sys[k].prog->T;
+        hip_check(hipMemcpy(sys[k].arena + T.nodes[0].out, pos, T.n_atom * 3 * sizeof(float), hipMemcpyHostToDevice), "pos");
+    }
+};
+
+static void set_initial_positions(Batch& B) {
+    for (size_t k = 0; k < B.sys.size(); ++k) {
+        H5 h5(B.sys[k].prog->path);
+        auto p = h5.rf("/input/pos");
+        const Topology& T = B.sys[k].prog->T;
+        if ((int)p.size() < T.n_atom * 3) fail("/input/pos smaller than n_atom");
+        B.copy_pos_in((int)k, p.data());
+    }
+}
+
+}  // namespace upside
+
+// ================================================================ C ABI ===
+using namespace upside;
+
+struct DerivEngine {
+    Batch batch;
+};
+struct UpsideBatch {
+    Batch batch;
+};
+
+#define ABI_TRY try {
+#define ABI_CATCH(ret)                                        \
+    }                                                         \
+    catch (const std::exception& e) {                         \
+        fprintf(stderr, "\n\nERROR: %s\n", e.what());         \
+        return ret;                                           \
+    }                                                         \
+    catch (...) {                                             \
+        fprintf(stderr, "\n\nERROR: unknown error\n");        \
+        return ret;                                           \
+    }
+
+extern "C" {
+
+DerivEngine* construct_deriv_engine(int n_atom, const char* potential_file, bool quiet) {
+    ABI_TRY
+    (void)quiet;
+    std::unique_ptr<DerivEngine> e(new DerivEngine);
+    e->batch.create({string(potential_file)}, {1.f}, n_atom);
+    if (e->batch.sys[0].prog->T.n_atom != n_atom) fail("n_atom mismatch");
+    return e.release();
+    ABI_CATCH(nullptr)
+}
+
+void free_deriv_engine(DerivEngine* engine) { delete engine; }
+
+int evaluate_energy(float* energy, DerivEngine* engine, const float* pos) {
+    ABI_TRY
+    engine->batch.copy_pos_in(0, pos);
+    engine->batch.evaluate(1);
+    *energy = engine->batch.potential(0);
+    return 0;
+    ABI_CATCH(1)
+}
+
+int evaluate_deriv(float* deriv, DerivEngine* engine, const float* pos) {
+    ABI_TRY
+    auto& B = engine->batch;
+    B.copy_pos_in(0, pos);
+    B.evaluate(1);
+    const Topology& T = B.sys[0].prog->T;
+    hip_check(hipMemcpy(deriv, B.sys[0].arena + T.nodes[0].sens, T.n_atom * 3 * sizeof(float), hipMemcpyDeviceToHost), "deriv");
+    return 0;
+    ABI_CATCH(1)
+}
+
+int set_param(int n_param, const float* param, DerivEngine* engine, const char* node_name) {
+    (void)n_param; (void)param; (void)engine;
+    fprintf(stderr, "ERROR: set_param is not supported by this engine (node %s)\n", node_name);
+    return 1;
+}
+
+int get_param_deriv(int n_param, float* deriv, DerivEngine* engine, const char* node_name) {
+    (void)n_param; (void)deriv; (void)engine; (void)node_name;
+    return -1;
+}
+
+int get_param(int n_param, float* param, DerivEngine* engine, const char* node_name) {
+    ABI_TRY
+    auto& B = engine->batch;
+    const Topology& T = B.sys[0].prog->T;
+    const Node* n = B.node(0, node_name);
+    for (size_t g = 0; g < T.graphs.size(); ++g) {
+        if (&T.nodes[T.graph_node[g]] != n) continue;
+        const PairGraph& G = T.graphs[g];
+        int total = (int)((T.ci[G.type1] , 0));
+        (void)total;
+        // interaction table size = n_type1 * n_type2 * n_param; recover n_type1 from the layout
+        int n_type1 = 0;
+        for (int i = 0; i < G.n1; ++i) n_type1 = std::max(n_type1, T.ci[G.type1 + i] + 1);
+        int size = n_type1 * G.n_type2 * G.n_param;
+        if (size != n_param) fail("Wrong number of parameters, expected " + std::to_string(size) + " but got " + std::to_string(n_param));
+        std::copy(T.cf.begin() + G.param, T.cf.begin() + G.param + size, param);
+        return 0;
+    }
+    fail(string("no parameters exposed for node ") + node_name);
+    ABI_CATCH(1)
+}
+
+int get_output_dims(int* n_elem, int* elem_width, DerivEngine* engine, const char* node_name) {
+    ABI_TRY
+    const Node* n = engine->batch.node(0, node_name);
+    if (n->potential) { *n_elem = 1; *elem_width = 1; }
+    else { *n_elem = n->n_elem; *elem_width = n->width; }
+    return 0;
+    ABI_CATCH(1)
+}
+
+static int copy_node(int n_output, float* output, DerivEngine* engine, const char* node_name, bool sens) {
+    ABI_TRY
+    auto& B = engine->batch;
+    const Node* n = B.node(0, node_name);
+    const Topology& T = B.sys[0].prog->T;
+    if (n->potential) {
+        if (n_output != 1) fail("wrong size for potential node");
+        hip_check(hipMemcpy(output, B.sys[0].arena + T.pot + n->pot_slot, 4, hipMemcpyDeviceToHost), "potential");
+        return 0;
+    }
+    if (n_output != n->n_elem * n->width) fail("wrong size for CoordNode");
+    hip_check(hipMemcpy(output, B.sys[0].arena + (sens ? n->sens : n->out), (size_t)n_output * 4, hipMemcpyDeviceToHost), "node copy");
+    return 0;
+    ABI_CATCH(1)
+}
+
+int get_output(int n_output, float* output, DerivEngine* engine, const char* node_name) {
+    return copy_node(n_output, output, engine, node_name, false);
+}
+int get_sens(int n_output, float* output, DerivEngine* engine, const char* node_name) {
+    return copy_node(n_output, output, engine, node_name, true);
+}
+
+int get_value_by_name(int n_output, float* output, DerivEngine* engine, const char* node_name, const char* log_name) {
+    ABI_TRY
+    auto& B = engine->batch;
+    const Topology& T = B.sys[0].prog->T;
+    const Node* n = B.node(0, node_name);
+    int gidx = -1;
+    for (size_t g = 0; g < T.graphs.size(); ++g)
+        if (&T.nodes[T.graph_node[g]] == n) gidx = (int)g;
+    string ln(log_name);
+    vector<float> out;
+    if (gidx >= 0 && (ln == "count_edges_by_type" || ln == "n_edge" || ln == "edge_pairs" || ln == "edge_values")) {
+        const PairGraph& G = T.graphs[gidx];
+        long slots = (long)G.n1 * G.maxn;
+        vector<int> cnt(G.n1), nbr(slots), act(slots);
+        vector<float> val(slots);
+        float* A = B.sys[0].arena;
+        int* I = B.sys[0].iarena;
+        hip_check(hipMemcpy(cnt.data(), I + G.cnt, cnt.size() * 4, hipMemcpyDeviceToHost), "edges");
+        hip_check(hipMemcpy(nbr.data(), I + G.nbr, nbr.size() * 4, hipMemcpyDeviceToHost), "edges");
+        hip_check(hipMemcpy(act.data(), I + G.act, act.size() * 4, hipMemcpyDeviceToHost), "edges");
+        hip_check(hipMemcpy(val.data(), A + G.val, val.size() * 4, hipMemcpyDeviceToHost), "edges");
+        const int* t1 = T.ci.data() + G.type1;
+        const int* t2 = T.ci.data() + G.type2;
+        int n_type1 = 0;
+        for (int i = 0; i < G.n1; ++i) n_type1 = std::max(n_type1, t1[i] + 1);
+        vector<float> by_type((size_t)n_type1 * G.n_type2, 0.f);
+        vector<float> pairs, vals;
+        for (int i = 0; i < G.n1; ++i)
+            for (int k = 0; k < cnt[i]; ++k) {
+                long s = (long)i * G.maxn + k;
+                if (!act[s]) continue;
+                int j = nbr[s];
+                by_type[(size_t)t1[i] * G.n_type2 + t2[j]] += 1.f;
+                pairs.push_back((float)i);
+                pairs.push_back((float)j);
+                vals.push_back(val[s]);
+            }
+        if (ln == "count_edges_by_type") out = by_type;
+        else if (ln == "n_edge") out = {(float)vals.size()};
+        else if (ln == "edge_pairs") out = pairs;
+        else out = vals;
+    } else if (n->type == "rotamer") {
+        const RotamerDev& R = T.rots[T.ops[n->op].rot];
+        int st[4];
+        hip_check(hipMemcpy(st, B.sys[0].iarena + R.stats, sizeof(st), hipMemcpyDeviceToHost), "stats");
+        if (ln == "n_node") out = {(float)R.n_node};
+        else if (ln == "read n_bad_solve") out = {(float)st[1]};
+        else if (ln == "read n_bad_solve and reset") {
+            out = {(float)st[1]};
+            st[1] = 0;
+            hip_check(hipMemcpy(B.sys[0].iarena + R.stats, st, sizeof(st), hipMemcpyHostToDevice), "stats");
+        } else if (ln == "iterations") out = {(float)st[0]};
+        else fail("Value " + ln + " not implemented");
+    } else {
+        fail("Value " + ln + " not implemented");
+    }
+    if (n_output != (int)out.size())
+        fail("expected size (" + std::to_string(n_output) + " elements) inconsistent with actual size (" +
+             std::to_string(out.size()) + ")");
+    std::copy(out.begin(), out.end(), output);
+    return 0;
+    ABI_CATCH(1)
+}
+
+// ---- clamped spline helpers (src/engine_c_library.cpp:197-277, src/spline.cpp:158-190) ----
+int clamped_spline_solve(int N, float* bspline_coeff, const float* values) {
+    // Solve for N B-spline coefficients whose spline takes values[0..N-3] at
+    // knots 1..N-2 with zero end derivatives (coeff[0]=coeff[2], coeff[N-1]=coeff[N-3]).
+    if (N < 4) return 1;
+    int n = N - 2;
+    vector<double> a(n, 1. / 6.), b(n, 2. / 3.), c(n, 1. / 6.), d(values, values + n);
+    a[n - 1] *= 2.;  // fold the clamping conditions into the tridiagonal system
+    c[0] *= 2.;
+    for (int k = 1; k < n; ++k) {
+        double m = a[k] / b[k - 1];
+        b[k] -= m * c[k - 1];
+        d[k] -= m * d[k - 1];
+    }
+    d[n - 1] /= b[n - 1];
+    for (int k = n - 2; k >= 0; --k) d[k] = (d[k] - c[k] * d[k + 1]) / b[k];
+    for (int k = 0; k < n; ++k) bspline_coeff[k + 1] = (float)d[k];
+    bspline_coeff[0] = bspline_coeff[2];
+    bspline_coeff[N - 1] = bspline_coeff[N - 3];
+    return 0;
+}
+
+static void clamped_eval(const float* p, float x, int n_knot, float& v, float& d) {
+    if (x <= 1.f) { v = p[0] / 6.f + p[1] * (2.f / 3.f) + p[2] / 6.f; d = 0.f; return; }
+    if (x >= n_knot - 2) { v = p[n_knot - 3] / 6.f + p[n_knot - 2] * (2.f / 3.f) + p[n_knot - 1] / 6.f; d = 0.f; return; }
+    int xb = (int)x;
+    float y = x - xb;
+    const float* c = p + xb - 1;
+    float omy = 1.f - y, y2 = y * y, y3 = y2 * y;
+    v = (omy * omy * omy * c[0] + (3 * y3 - 6 * y2 + 4) * c[1] + (-3 * y3 + 3 * y2 + 3 * y + 1) * c[2] + y3 * c[3]) / 6.f;
+    d = -0.5f * omy * omy * c[0] + (1.5f * y2 - 2 * y) * c[1] + (-1.5f * y2 + y + 0.5f) * c[2] + 0.5f * y2 * c[3];
+}
+
+int clamped_spline_value(int N, float* result, const float* bspline_coeff, int nx, float* x) {
+    for (int i = 0; i < nx; ++i) { float d; clamped_eval(bspline_coeff, x[i], N, result[i], d); }
+    return 0;
+}
+
+int get_clamped_value_and_deriv(int N, float* result, const float* bspline_coeff, int nx, float* x) {
+    for (int i = 0; i < nx; ++i) clamped_eval(bspline_coeff, x[i], N, result[2 * i], result[2 * i + 1]);
+    return 0;
+}
+
+int get_clamped_coeff_deriv(int N, float* result, const float* bspline_coeff, float x) {
+    (void)bspline_coeff;
+    for (int i = 0; i < N; ++i) result[i] = 0.f;
+    if (x <= 1.f) { result[0] = 1.f / 6.f; result[1] = 2.f / 3.f; result[2] = 1.f / 6.f; return 0; }
+    if (x >= N - 2) { result[N - 3] = 1.f / 6.f; result[N - 2] = 2.f / 3.f; result[N - 1] = 1.f / 6.f; return 0; }
+    int xb = (int)x;
+    float y = x - xb, omy = 1.f - y, y2 = y * y, y3 = y2 * y;
+    result[xb - 1] = omy * omy * omy / 6.f;
+    result[xb] = (3 * y3 - 6 * y2 + 4) / 6.f;
+    result[xb + 1] = (-3 * y3 + 3 * y2 + 3 * y + 1) / 6.f;
+    result[xb + 2] = y3 / 6.f;
+    return 0;
+}
+
+// ---- batched extension ----
+UpsideBatch* upside_batch_create(int n_sys, const char* const* up_paths, const float* temperatures,
+                                 uint32_t base_seed, float dt, int thermostat_interval) {
+    ABI_TRY
+    std::unique_ptr<UpsideBatch> b(new UpsideBatch);
+    vector<string> paths;
+    vector<float> temps;
+    for (int k = 0; k < n_sys; ++k) {
+        paths.push_back(up_paths[k]);
+        temps.push_back(temperatures ? temperatures[k] : 1.f);
+    }
+    b->batch.dt = dt;
+    b->batch.thermostat_interval = std::max(1, thermostat_interval);
+    b->batch.base_seed = base_seed;
+    b->batch.create(paths, temps, 0);
+    set_initial_positions(b->batch);
+    b->batch.thermalize();
+    return b.release();
+    ABI_CATCH(nullptr)
+}
+
+void upside_batch_free(UpsideBatch* batch) { delete batch; }
+
+int upside_batch_n_atom(UpsideBatch* b, int sys) {
+    if (!b || sys < 0 || sys >= (int)b->batch.sys.size()) return -1;
+    return b->batch.sys[sys].prog->T.n_atom;
+}
+
+static int batch_copy(UpsideBatch* b, int sys, float* host, bool to_device, bool mom) {
+    ABI_TRY
+    auto& B = b->batch;
+    if (sys < 0 || sys >= (int)B.sys.size()) fail("system index out of range");
+    const Topology& T = B.sys[sys].prog->T;
+    float* dev = B.sys[sys].arena + (mom ? T.mom : T.nodes[0].out);
+    size_t bytes = (size_t)T.n_atom * 3 * sizeof(float);
+    hip_check(hipMemcpy(to_device ? (void*)dev : (void*)host, to_device ? (const void*)host : (const void*)dev, bytes,
+                        to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost), "copy");
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_set_pos(UpsideBatch* b, int sys, const float* pos) { return batch_copy(b, sys, (float*)pos, true, false); }
+int upside_batch_get_pos(UpsideBatch* b, int sys, float* pos) { return batch_copy(b, sys, pos, false, false); }
+int upside_batch_set_mom(UpsideBatch* b, int sys, const float* mom) { return batch_copy(b, sys, (float*)mom, true, true); }
+int upside_batch_get_mom(UpsideBatch* b, int sys, float* mom) { return batch_copy(b, sys, mom, false, true); }
+
+static uint64_t g_round_counter = 0;
+
+int upside_batch_run(UpsideBatch* b, int n_round, int frame_interval, float* frame_pot, int n_frames) {
+    ABI_TRY
+    b->batch.run(n_round, frame_interval, frame_pot, n_frames, 0);
+    g_round_counter += n_round;
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_energies(UpsideBatch* b, float* energies) {
+    ABI_TRY
+    b->batch.evaluate(1);
+    for (size_t k = 0; k < b->batch.sys.size(); ++k) energies[k] = b->batch.potential((int)k);
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_replica_exchange(UpsideBatch* b, int n_pairs, const int* pairs, uint64_t round, int* accepted) {
+    ABI_TRY
+    auto& B = b->batch;
+    int ns = (int)B.sys.size();
+    vector<float> beta(ns);
+    for (int k = 0; k < ns; ++k) beta[k] = 1.f / B.sys[k].temperature;
+    auto lboltz = [&]() {
+        B.evaluate(1);
+        vector<float> r(ns);
+        for (int k = 0; k < ns; ++k) r[k] = -beta[k] * B.potential(k);
+        return r;
+    };
+    auto swap_pos = [&](int s1, int s2) {
+        const Topology& T1 = B.sys[s1].prog->T;
+        const Topology& T2 = B.sys[s2].prog->T;
+        if (T1.n_atom != T2.n_atom) fail("replica exchange between systems of different size");
+        size_t bytes = (size_t)T1.n_atom * 3 * sizeof(float);
+        vector<float> a(T1.n_atom * 3), c(T1.n_atom * 3);
+        hip_check(hipMemcpy(a.data(), B.sys[s1].arena + T1.nodes[0].out, bytes, hipMemcpyDeviceToHost), "swap");
+        hip_check(hipMemcpy(c.data(), B.sys[s2].arena + T2.nodes[0].out, bytes, hipMemcpyDeviceToHost), "swap");
+        hip_check(hipMemcpy(B.sys[s1].arena + T1.nodes[0].out, c.data(), bytes, hipMemcpyHostToDevice), "swap");
+        hip_check(hipMemcpy(B.sys[s2].arena + T2.nodes[0].out, a.data(), bytes, hipMemcpyHostToDevice), "swap");
+    };
+    for (int p = 0; p < n_pairs; ++p)
+        if (pairs[2 * p] < 0 || pairs[2 * p] >= ns || pairs[2 * p + 1] < 0 || pairs[2 * p + 1] >= ns)
+            fail("swap pair out of range");
+    auto old_lb = lboltz();
+    for (int p = 0; p < n_pairs; ++p) swap_pos(pairs[2 * p], pairs[2 * p + 1]);
+    auto new_lb = lboltz();
+    // RandomGenerator(seed, REPLICA_EXCHANGE_RANDOM_STREAM, 0, round), one uniform per test
+    uint32_t key[4] = {B.base_seed, 1u, 0u, 0u};
+    uint32_t ctr[4] = {(uint32_t)(round & 0xffffffffull), (uint32_t)(round >> 32), 0u, 0u};
+    for (int p = 0; p < n_pairs; ++p) {
+        int s1 = pairs[2 * p], s2 = pairs[2 * p + 1];
+        float diff = (new_lb[s1] + new_lb[s2]) - (old_lb[s1] + old_lb[s2]);
+        bool reject = false;
+        if (diff < 0.f) {
+            uint32_t bits[4];
+            threefry_host(ctr, key, bits);
+            ctr[3]++;
+            float u = (float)bits[0] * 2.3283064365386962890625e-10f + 1.16415321826934814453125e-10f;
+            reject = std::exp(diff) < u;
+        }
+        if (reject) swap_pos(s1, s2);
+        if (accepted) accepted[p] = reject ? 0 : 1;
+    }
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_bp_stats(UpsideBatch* b, int sys, int* last_iters, long long* total_iters, long long* solves,
+                          long long* bad_solves) {
+    ABI_TRY
+    auto& B = b->batch;
+    const Topology& T = B.sys[sys].prog->T;
+    long long tot = 0, sol = 0, bad = 0;
+    int last = 0;
+    for (auto& R : T.rots) {
+        int st[4];
+        hip_check(hipMemcpy(st, B.sys[sys].iarena + R.stats, sizeof(st), hipMemcpyDeviceToHost), "stats");
+        last += st[0];
+        bad += st[1];
+        tot += st[2];
+        sol += st[3];
+    }
+    if (last_iters) *last_iters = last;
+    if (total_iters) *total_iters = tot;
+    if (solves) *solves = sol;
+    if (bad_solves) *bad_solves = bad;
+    return 0;
+    ABI_CATCH(1)
+}
+
+float upside_batch_last_run_ms(UpsideBatch* b) { return b ? b->batch.last_ms : -1.f; }
+
+}  // extern "C"

@@ -1,0 +1,997 @@
+// Persistent force-and-integration kernel of the MI355X Upside engine.
+//
+// One workgroup evaluates one system: it interprets the lowered node program
+// (value pass in germ order, sensitivity pass in deriv order, as
+// DerivEngine::compute, src/deriv_engine.cpp:239-260), then applies the
+// symplectic-Euler stages of integration_cycle (src/deriv_engine.cpp:317-331)
+// and the Ornstein-Uhlenbeck thermostat (src/thermostat.cpp:9-18), for many
+// rounds without returning to the host.  Each op cites the reference node it
+// reproduces.
+#include "dev_rotamer.h"
+
+namespace upside {
+
+#define OUT(op) (c.A + (op).out)
+#define SENS(op) (c.A + (op).sens)
+#define IN(op, k) (c.A + (op).in_out[k])
+#define INS(op, k) (c.A + (op).in_sens[k])
+#define CIA(op, k) (c.CI + (op).ia[k])
+#define CFA(op, k) (c.CF + (op).ia[k])
+
+__device__ __forceinline__ void put_pot(Ctx& c, const SysDesc& S, const OpRec& op, float e) {
+    if (c.tid == 0) c.A[S.pot + op.pot_slot] = e;
+}
+
+// ---- Distance3D (src/coord_basic.cpp:13-66) --------------------------------
+__device__ void dist3d_fwd(Ctx& c, const OpRec& op) {
+    const int* id = CIA(op, 0);
+    float* der = c.A + op.scratch;
+    for (int e = c.tid; e < op.n_elem; e += NT) {
+        const float* a = IN(op, 0) + id[2 * e] * op.in_w[0];
+        const float* b = IN(op, 1) + id[2 * e + 1] * op.in_w[1];
+        float dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+        float m2 = dx * dx + dy * dy + dz * dz;
+        float d = sqrtf(m2);
+        OUT(op)[e] = d;
+        float inv = d != 0.f ? rsqrtf(m2) : 0.f;
+        der[e * 3 + 0] = dx * inv; der[e * 3 + 1] = dy * inv; der[e * 3 + 2] = dz * inv;
+    }
+}
+__device__ void dist3d_bwd(Ctx& c, const OpRec& op) {
+    const float* der = c.A + op.scratch;
+    float* con = c.A + op.ia[1];
+    for (int e = c.tid; e < op.n_elem; e += NT) {
+        float s = SENS(op)[e];
+        for (int k = 0; k < 3; ++k) {
+            con[(e * 2 + 0) * 3 + k] = s * der[e * 3 + k];
+            con[(e * 2 + 1) * 3 + k] = -s * der[e * 3 + k];
+        }
+    }
+    __syncthreads();
+    plan_gather(c, op.plans[0]);
+    plan_gather(c, op.plans[1]);
+}
+
+// ---- Angle: cosine at the third atom (src/coord_basic.cpp:183-243) ---------
+__device__ void angle_fwd(Ctx& c, const OpRec& op) {
+    const int* id = CIA(op, 0);
+    float* der = c.A + op.scratch;
+    const float* P = IN(op, 0);
+    int w = op.in_w[0];
+    for (int e = c.tid; e < op.n_elem; e += NT) {
+        const float* a1 = P + id[3 * e] * w;
+        const float* a2 = P + id[3 * e + 1] * w;
+        const float* a3 = P + id[3 * e + 2] * w;
+        float x1[3], x2[3];
+        for (int k = 0; k < 3; ++k) { x1[k] = a1[k] - a3[k]; x2[k] = a2[k] - a3[k]; }
+        float i1 = rsqrtf(x1[0] * x1[0] + x1[1] * x1[1] + x1[2] * x1[2]);
+        float i2 = rsqrtf(x2[0] * x2[0] + x2[1] * x2[1] + x2[2] * x2[2]);
+        float h1[3], h2[3];
+        for (int k = 0; k < 3; ++k) { h1[k] = x1[k] * i1; h2[k] = x2[k] * i2; }
+        float dp = h1[0] * h2[0] + h1[1] * h2[1] + h1[2] * h2[2];
+        OUT(op)[e] = dp;
+        for (int k = 0; k < 3; ++k) {
+            der[e * 6 + k] = (h2[k] - h1[k] * dp) * i1;
+            der[e * 6 + 3 + k] = (h1[k] - h2[k] * dp) * i2;
+        }
+    }
+}
+__device__ void angle_bwd(Ctx& c, const OpRec& op) {
+    const float* der = c.A + op.scratch;
+    float* con = c.A + op.ia[1];
+    for (int e = c.tid; e < op.n_elem; e += NT) {
+        float s = SENS(op)[e];
+        for (int k = 0; k < 3; ++k) {
+            float d1 = s * der[e * 6 + k], d2 = s * der[e * 6 + 3 + k];
+            con[(e * 3 + 0) * 3 + k] = d1;
+            con[(e * 3 + 1) * 3 + k] = d2;
+            con[(e * 3 + 2) * 3 + k] = -d1 - d2;
+        }
+    }
+    __syncthreads();
+    plan_gather(c, op.plans[0]);
+}
+
+// ---- Dihedral (src/coord_basic.cpp:311-374, src/vector_math.h:764-797) -----
+__device__ __forceinline__ void cross3(const float* a, const float* b, float* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ __forceinline__ float dot3(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+__device__ void dihedral_fwd(Ctx& c, const OpRec& op) {
+    const int* id = CIA(op, 0);
+    float* jac = c.A + op.scratch;
+    const float* P = IN(op, 0);
+    int w = op.in_w[0];
+    for (int e = c.tid; e < op.n_elem; e += NT) {
+        const int* q = id + 4 * e;
+        if (q[0] < 0 || q[3] < 0) {
+            OUT(op)[e] = -1.3963f;
+            for (int k = 0; k < 12; ++k) jac[e * 12 + k] = 0.f;
+            continue;
+        }
+        const float *r1 = P + q[0] * w, *r2 = P + q[1] * w, *r3 = P + q[2] * w, *r4 = P + q[3] * w;
+        float F[3], G[3], H[3], A[3], B[3], C[3];
+        for (int k = 0; k < 3; ++k) { F[k] = r1[k] - r2[k]; G[k] = r2[k] - r3[k]; H[k] = r4[k] - r3[k]; }
+        cross3(F, G, A);
+        cross3(H, G, B);
+        cross3(B, A, C);
+        float iA2 = 1.f / dot3(A, A), iB2 = 1.f / dot3(B, B);
+        float G2 = dot3(G, G);
+        float iG = rsqrtf(G2);
+        float Gm = G2 * iG;
+        float fg = dot3(F, G) * iA2 * iG, hg = dot3(H, G) * iB2 * iG;
+        for (int k = 0; k < 3; ++k) {
+            float d1 = -Gm * iA2 * A[k];
+            float d4 = Gm * iB2 * B[k];
+            float fm = fg * A[k] - hg * B[k];
+            jac[e * 12 + k] = d1;
+            jac[e * 12 + 3 + k] = -d1 + fm;
+            jac[e * 12 + 6 + k] = -d4 - fm;
+            jac[e * 12 + 9 + k] = d4;
+        }
+        OUT(op)[e] = atan2f(dot3(C, G), dot3(A, B) * Gm);
+    }
+}
+__device__ void dihedral_bwd(Ctx& c, const OpRec& op) {
+    const float* jac = c.A + op.scratch;
+    float* con = c.A + op.ia[1];
+    for (int e = c.tid; e < op.n_elem; e += NT) {
+        float s = SENS(op)[e];
+        for (int k = 0; k < 12; ++k) con[e * 12 + k] = s * jac[e * 12 + k];
+    }
+    __syncthreads();
+    plan_gather(c, op.plans[0]);
+}
+
+// ---- Spring (src/spring.cpp:9-81) -------------------------------------------
+__device__ void spring_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
+    const int* id = CIA(op, 0);
+    const float* eq = CFA(op, 1);
+    const float* kk = CFA(op, 2);
+    int dim1 = op.ia[3], pbc = op.ia[4], n = op.ia[5];
+    float box = op.fa[0];
+    float* con = c.A + op.scratch;
+    float pot = 0.f;
+    for (int e = c.tid; e < n; e += NT) {
+        float x = IN(op, 0)[id[e] * op.in_w[0] + dim1];
+        float ex = x - eq[e];
+        float s2 = ex * ex;
+        if (pbc && ex != 0.f) {
+            float e1 = x - eq[e] - box, e2 = x - eq[e] + box;
+            if (e1 * e1 < s2) { s2 = e1 * e1; ex = e1; }
+            if (e2 * e2 < s2) { s2 = e2 * e2; ex = e2; }
+        }
+        pot += 0.5f * kk[e] * s2;
+        con[e] = kk[e] * ex;
+    }
+    pot = block_sum(c, pot);
+    put_pot(c, S, op, pot);
+    plan_gather(c, op.plans[0]);
+}
+
+// ---- rama_coord / cat_pos (src/coord_map.cpp:69-161) -----------------------
+__device__ void rama_coord_fwd(Ctx& c, const OpRec& op) {
+    const int* i1 = CIA(op, 0);
+    const int* i2 = CIA(op, 1);
+    for (int e = c.tid; e < op.n_elem; e += NT) {
+        OUT(op)[2 * e] = IN(op, 0)[i1[e] * op.in_w[0]];
+        OUT(op)[2 * e + 1] = IN(op, 1)[i2[e] * op.in_w[1]];
+    }
+}
+__device__ void cat_pos_fwd(Ctx& c, const OpRec& op) {
+    const int* idx = CIA(op, 0);
+    int n1 = op.ia[1], w = op.width;
+    for (int e = c.tid; e < op.n_elem; e += NT) {
+        int k = e < n1 ? 0 : 1;
+        const float* src = IN(op, k) + idx[e] * op.in_w[k];
+        for (int d = 0; d < w; ++d) OUT(op)[e * w + d] = src[d];
+    }
+}
+__device__ void passthrough_bwd(Ctx& c, const OpRec& op) {
+    plan_gather(c, op.plans[0]);
+    plan_gather(c, op.plans[1]);
+}
+
+// ---- rama_map_pot (src/rama_map_pot.cpp:15-94) -------------------------------
+__device__ void rama_map_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
+    const int* res = CIA(op, 0);
+    const int* mid = CIA(op, 1);
+    const float* coef = CFA(op, 2);
+    int nx = op.ia[3], n = op.ia[4];
+    float scale = op.fa[0], shift = op.fa[1];
+    float* con = c.A + op.scratch;
+    float pot = 0.f;
+    for (int e = c.tid; e < n; e += NT) {
+        const float* r = IN(op, 0) + res[e] * op.in_w[0];
+        float x = (r[0] + shift) * scale, y = (r[1] + shift) * scale;
+        int xb = (int)x, yb = (int)y;
+        xb = min(max(xb, 0), nx - 1); yb = min(max(yb, 0), nx - 1);
+        float v, dx, dy;
+        bicubic(coef + ((size_t)(mid[e] * nx + xb) * nx + yb) * 16, x - xb, y - yb, v, dx, dy);
+        pot += v;
+        c.A[op.scratch + 2 * n + e] = v;  // per-residue potential (logging)
+        con[2 * e] = dx * scale;
+        con[2 * e + 1] = dy * scale;
+    }
+    pot = block_sum(c, pot);
+    put_pot(c, S, op, pot);
+    plan_gather(c, op.plans[0]);
+}
+
+// ---- affine_alignment (src/eig.cpp:277-471) ----------------------------------
+// 4x4 symmetric eigenproblem per residue: Householder tridiagonalisation then
+// implicit QR with Wilkinson shift (Golub & Van Loan 8.3), in groups of four
+// residues that iterate together as the reference's SIMD lanes do.
+__device__ __forceinline__ int pk(int i, int j) {
+    int a = i < j ? i : j, b = i < j ? j : i;
+    return a == 0 ? b : a == 1 ? 3 + b : a == 2 ? 5 + b : 9;
+}
+__device__ __forceinline__ bool quad_any(bool p) {
+    uint64_t m = __ballot(p);
+    int lane = threadIdx.x & 63;
+    return ((m >> (lane & ~3)) & 0xFull) != 0;
+}
+
+__device__ void eig4_grouped(float* A, float* d, float* rot) {
+    float beta[2];
+    for (int k = 0; k < 2; ++k) {
+        int m = 3 - k;
+        float x0 = A[pk(k, k + 1)];
+        float sigma2 = 1e-20f;
+        for (int i = 1; i < m; ++i) { float xi = A[pk(k, k + 1 + i)]; sigma2 += xi * xi; }
+        float mu = sqrtf(x0 * x0 + sigma2);
+        float s = 0.f < x0 ? -sigma2 / (x0 + mu) : x0 - mu;
+        float b = 2.f * s * s / (sigma2 + s * s);
+        A[pk(k, k + 1)] = mu;
+        float is = 1.f / s;
+        for (int i = 1; i < m; ++i) A[pk(k, k + 1 + i)] *= is;
+        beta[k] = b;
+        float v[3], p[3], w[3];
+        for (int j = 0; j < m; ++j) v[j] = j == 0 ? 1.f : A[pk(k, k + 1 + j)];
+        for (int i = 0; i < m; ++i) {
+            float acc = 0.f;
+            for (int j = 0; j < m; ++j) acc += A[pk(i + k + 1, j + k + 1)] * v[j];
+            p[i] = acc * b;
+        }
+        float pdv = 0.f;
+        for (int i = 0; i < m; ++i) pdv += p[i] * v[i];
+        for (int i = 0; i < m; ++i) w[i] = p[i] - (0.5f * b * pdv) * v[i];
+        for (int i = 0; i < m; ++i)
+            for (int j = i; j < m; ++j) A[pk(i + k + 1, j + k + 1)] -= v[i] * w[j] + v[j] * w[i];
+    }
+    float u[3];
+    for (int i = 0; i < 4; ++i) d[i] = A[pk(i, i)];
+    for (int i = 0; i < 3; ++i) u[i] = A[pk(i, i + 1)];
+    for (int i = 0; i < 16; ++i) rot[i] = (i % 5 == 0) ? 1.f : 0.f;
+    float v0[3] = {1.f, A[pk(0, 2)], A[pk(0, 3)]};
+    float v1[2] = {1.f, A[pk(1, 3)]};
+    for (int i = 1; i < 4; ++i)
+        for (int j = i; j < 4; ++j) rot[i * 4 + j] -= beta[0] * (v0[i - 1] * v0[j - 1]);
+    for (int i = 2; i < 4; ++i)
+        for (int j = i; j < 4; ++j) rot[i * 4 + j] -= beta[1] * (v1[i - 2] * v1[j - 2]);
+    for (int i = 0; i < 4; ++i)
+        for (int j = i + 1; j < 4; ++j) rot[j * 4 + i] = rot[i * 4 + j];
+    float coeff = beta[0] * beta[1] * (v0[1] * v1[0] + v0[2] * v1[1]);
+    for (int i = 2; i < 4; ++i)
+        for (int j = 1; j < 4; ++j) rot[i * 4 + j] += coeff * v1[i - 2] * v0[j - 1];
+    const float tol = 1e-5f;
+    for (int it = 0; it < 100; ++it) {
+        for (int i = 0; i < 3; ++i)
+            if (fabsf(u[i]) <= tol * (fabsf(d[i]) + fabsf(d[i + 1]))) u[i] = 0.f;
+        int q;
+        if (quad_any(u[2] != 0.f)) q = 0;
+        else if (quad_any(u[1] != 0.f)) q = 1;
+        else if (quad_any(u[0] != 0.f)) q = 2;
+        else break;
+        int p = 4 - q - 1;
+        for (; p > 0; --p)
+            if (!quad_any(u[p - 1] != 0.f)) break;
+        int n = 4 - q - p;
+        float* dd = d + p;
+        float* uu = u + p;
+        float* rr = rot + 4 * p;
+        float dval = 0.5f * (dd[n - 2] - dd[n - 1]) + 1e-20f;
+        float un = uu[n - 2];
+        float mu = dd[n - 1] - un * un / (dval + copysignf(sqrtf(dval * dval + un * un), dval));
+        float x = dd[0] - mu, z = uu[0];
+        for (int k = 0; k < n - 1; ++k) {
+            float cc, ss;
+            if (z == 0.f) { cc = 1.f; ss = 0.f; }
+            else { float ir = rsqrtf(x * x + z * z); cc = x * ir; ss = -z * ir; }
+            for (int j = 0; j < 4; ++j) {
+                float t1 = rr[k * 4 + j], t2 = rr[(k + 1) * 4 + j];
+                rr[k * 4 + j] = cc * t1 - ss * t2;
+                rr[(k + 1) * 4 + j] = ss * t1 + cc * t2;
+            }
+            if (k > 0) uu[k - 1] = cc * x - ss * z;
+            float T00 = dd[k], T11 = dd[k + 1], T01 = uu[k];
+            dd[k] = T00 * cc * cc - T01 * 2.f * cc * ss + T11 * ss * ss;
+            dd[k + 1] = T00 * ss * ss + T01 * 2.f * cc * ss + T11 * cc * cc;
+            uu[k] = (T00 - T11) * cc * ss + T01 * (cc * cc - ss * ss);
+            x = uu[k];
+            if (k < n - 2) { z = -uu[k + 1] * ss; uu[k + 1] *= cc; }
+        }
+    }
+    for (int i = 1; i < 4; ++i) {
+        if (d[0] < d[i]) {
+            float t = d[0]; d[0] = d[i]; d[i] = t;
+            for (int j = 0; j < 4; ++j) { float r = rot[j]; rot[j] = rot[i * 4 + j]; rot[i * 4 + j] = r; }
+        }
+    }
+}
+
+__device__ void affine_fwd(Ctx& c, const OpRec& op) {
+    const int* atoms = CIA(op, 0);
+    const float* ref = CFA(op, 1);
+    float* ev = c.A + op.scratch;            // n*4 eigenvalues
+    float* ec = c.A + op.scratch + 4 * op.n_elem;  // n*16 eigenvector rows
+    const float* P = IN(op, 0);
+    int w = op.in_w[0];
+    int n = op.n_elem;
+    for (int base = 0; base < n; base += NT) {
+        int r = base + c.tid;
+        int rr = r < n ? r : (r & ~3);       // padding lanes repeat lane 0 of their quad
+        if (rr >= n) continue;              // whole quad past the end
+        float x[3][3], center[3];
+        for (int a = 0; a < 3; ++a)
+            for (int k = 0; k < 3; ++k) x[a][k] = P[atoms[rr * 3 + a] * w + k];
+        for (int k = 0; k < 3; ++k) center[k] = (1.f / 3.f) * (x[0][k] + x[1][k] + x[2][k]);
+        for (int a = 0; a < 3; ++a)
+            for (int k = 0; k < 3; ++k) x[a][k] -= center[k];
+        const float* g = ref + rr * 9;
+        float R[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) R[i][j] = x[0][j] * g[i] + x[1][j] * g[3 + i] + x[2][j] * g[6 + i];
+        float F[10] = {R[0][0] + R[1][1] + R[2][2], R[1][2] - R[2][1], R[2][0] - R[0][2], R[0][1] - R[1][0],
+                       R[0][0] - R[1][1] - R[2][2], R[0][1] + R[1][0], R[0][2] + R[2][0],
+                       -R[0][0] + R[1][1] - R[2][2], R[1][2] + R[2][1], -R[0][0] - R[1][1] + R[2][2]};
+        float d[4], rot[16];
+        eig4_grouped(F, d, rot);
+        if (r >= n) continue;
+        for (int k = 0; k < 4; ++k) ev[r * 4 + k] = d[k];
+        for (int k = 0; k < 16; ++k) ec[r * 16 + k] = rot[k];
+        float* o = OUT(op) + r * 7;
+        o[0] = center[0]; o[1] = center[1]; o[2] = center[2];
+        o[3] = rot[0]; o[4] = rot[1]; o[5] = rot[2]; o[6] = rot[3];
+    }
+}
+__device__ void affine_bwd(Ctx& c, const OpRec& op) {
+    const float* ref = CFA(op, 1);
+    const float* ev = c.A + op.scratch;
+    const float* ec = c.A + op.scratch + 4 * op.n_elem;
+    float* con = c.A + op.ia[2];
+    for (int r = c.tid; r < op.n_elem; r += NT) {
+        const float* s = SENS(op) + r * 7;
+        const float* E = ec + r * 16;
+        const float* q = E;
+        float t0 = s[3], t1 = s[4], t2 = s[5];
+        float qs[4] = {2.f * (-t0 * q[1] - t1 * q[2] - t2 * q[3]), 2.f * (t0 * q[0] + t1 * q[3] - t2 * q[2]),
+                       2.f * (t1 * q[0] + t2 * q[1] - t0 * q[3]), 2.f * (t2 * q[0] + t0 * q[2] - t1 * q[1])};
+        float qsd[4], inv[4];
+        for (int dd = 0; dd < 4; ++dd) {
+            float acc = 0.f;
+            for (int i = 0; i < 4; ++i) acc += qs[i] * E[dd * 4 + i];
+            qsd[dd] = acc;
+        }
+        for (int k = 1; k < 4; ++k) inv[k] = 1.f / (ev[r * 4] - ev[r * 4 + k]);
+        for (int na = 0; na < 3; ++na) {
+            const float* g = ref + r * 9 + na * 3;
+            float gx = g[0], gy = g[1], gz = g[2];
+            // upper triangles (f00 f01 f02 f03 f11 f12 f13 f22 f23 f33), src/eig.cpp:452-467
+            float M[3][10] = {{gx, 0.f, gz, -gy, gx, gy, gz, -gx, 0.f, -gx},
+                              {gy, -gz, 0.f, gx, -gy, gx, 0.f, gy, gz, -gy},
+                              {gz, gy, -gx, 0.f, -gz, 0.f, gx, -gz, gy, gz}};
+            for (int j = 0; j < 3; ++j) {
+                float dv = (1.f / 3.f) * s[j];
+                for (int k = 1; k < 4; ++k) {
+                    float t = 0.f;
+                    for (int a = 0; a < 4; ++a)
+                        for (int b = a; b < 4; ++b) {
+                            float fab = M[j][pk(a, b)];
+                            float term = a == b ? E[k * 4 + a] * E[b] : E[k * 4 + a] * E[b] + E[k * 4 + b] * E[a];
+                            t += fab * term;
+                        }
+                    dv += inv[k] * t * qsd[k];
+                }
+                con[(r * 3 + na) * 3 + j] = dv;
+            }
+        }
+    }
+    __syncthreads();
+    plan_gather(c, op.plans[0]);
+}
+
+// ---- placement (src/placement.cpp:20-314) -------------------------------------
+__device__ void placement_fwd(Ctx& c, const OpRec& op) {
+    const int* ares = CIA(op, 0);
+    const int* layer = CIA(op, 1);
+    const float* data = CFA(op, 2);
+    int is_rama = op.ia[3], dim = op.ia[7];
+    const float* aff = IN(op, 0);
+    int wa = op.in_w[0];
+    for (int e = c.tid; e < op.n_elem; e += NT) {
+        float* o = OUT(op) + e * op.width;
+        if (is_rama) {
+            const int* rres = CIA(op, 4);
+            int nx = op.ia[5], ny = op.ia[6];
+            const float* r = IN(op, 1) + rres[e] * op.in_w[1];
+            float x = (r[0] + op.fa[2]) * op.fa[0], y = (r[1] + op.fa[2]) * op.fa[1];
+            int xb = min(max((int)x, 0), nx - 1), yb = min(max((int)y, 0), ny - 1);
+            float v, dx, dy;
+            bicubic(data + ((size_t)(layer[e] * nx + xb) * ny + yb) * 16, x - xb, y - yb, v, dx, dy);
+            o[0] = v;
+            c.A[op.scratch + 2 * e] = dx * op.fa[0];
+            c.A[op.scratch + 2 * e + 1] = dy * op.fa[1];
+            continue;
+        }
+        const float* a = aff + ares[e] * wa;
+        float U[9];
+        quat_to_rot(a + 3, U);
+        const float* v = data + layer[e] * dim;
+        for (int k = 0; k < 3; ++k) {
+            o[k] = U[3 * k] * v[0] + U[3 * k + 1] * v[1] + U[3 * k + 2] * v[2] + a[k];
+            o[3 + k] = U[3 * k] * v[3] + U[3 * k + 1] * v[4] + U[3 * k + 2] * v[5];
+        }
+        if (dim == 7) o[6] = v[6];
+    }
+}
+__device__ void placement_bwd(Ctx& c, const OpRec& op) {
+    int is_rama = op.ia[3];
+    float* con = c.A + op.ia[8];
+    if (is_rama) {
+        for (int e = c.tid; e < op.n_elem; e += NT) {
+            float s = SENS(op)[e];
+            con[2 * e] = s * c.A[op.scratch + 2 * e];
+            con[2 * e + 1] = s * c.A[op.scratch + 2 * e + 1];
+        }
+        __syncthreads();
+        plan_gather(c, op.plans[1]);
+        return;
+    }
+    const int* ares = CIA(op, 0);
+    const float* aff = IN(op, 0);
+    int wa = op.in_w[0];
+    for (int e = c.tid; e < op.n_elem; e += NT) {
+        const float* s = SENS(op) + e * op.width;
+        const float* x = OUT(op) + e * op.width;
+        const float* t = aff + ares[e] * wa;
+        float p[3] = {x[0] - t[0], x[1] - t[1], x[2] - t[2]};
+        float tq1[3], tq2[3];
+        cross3(p, s, tq1);
+        cross3(x + 3, s + 3, tq2);
+        for (int k = 0; k < 3; ++k) {
+            con[e * 6 + k] = s[k];
+            con[e * 6 + 3 + k] = tq1[k] + tq2[k];
+        }
+    }
+    __syncthreads();
+    plan_gather(c, op.plans[0]);
+}
+
+// ---- infer_H_O (src/hbond.cpp:20-127) ------------------------------------------
+__device__ void infer_fwd(Ctx& c, const OpRec& op) {
+    const int* id = CIA(op, 0);
+    const float* bl = CFA(op, 1);
+    float* cache = c.A + op.scratch;
+    const float* P = IN(op, 0);
+    int w = op.in_w[0];
+    for (int e = c.tid; e < op.n_elem; e += NT) {
+        const float *pc = P + id[3 * e] * w, *cc = P + id[3 * e + 1] * w, *nc = P + id[3 * e + 2] * w;
+        float pr[3], nx[3], ds[3];
+        for (int k = 0; k < 3; ++k) { pr[k] = pc[k] - cc[k]; nx[k] = nc[k] - cc[k]; }
+        float pi = rsqrtf(dot3(pr, pr)), ni = rsqrtf(dot3(nx, nx));
+        for (int k = 0; k < 3; ++k) { pr[k] *= pi; nx[k] *= ni; ds[k] = pr[k] + nx[k]; }
+        float di = rsqrtf(dot3(ds, ds));
+        for (int k = 0; k < 3; ++k) ds[k] *= di;
+        float* cs = cache + e * 12;
+        for (int k = 0; k < 3; ++k) { cs[k] = pr[k]; cs[4 + k] = nx[k]; cs[8 + k] = ds[k]; }
+        cs[3] = pi; cs[7] = ni; cs[11] = di;
+        float* o = OUT(op) + e * 6;
+        for (int k = 0; k < 3; ++k) { o[k] = cc[k] - bl[e] * ds[k]; o[3 + k] = -ds[k]; }
+    }
+}
+__device__ void infer_bwd(Ctx& c, const OpRec& op) {
+    const float* bl = CFA(op, 1);
+    const float* cache = c.A + op.scratch;
+    float* con = c.A + op.ia[2];
+    for (int e = c.tid; e < op.n_elem; e += NT) {
+        const float* s = SENS(op) + e * 6;
+        const float* cs = cache + e * 12;
+        float sn[3], sd[3], sp[3], sx[3];
+        for (int k = 0; k < 3; ++k) sn[k] = s[3 + k] + bl[e] * s[k];
+        float t = dot3(cs + 8, sn);
+        for (int k = 0; k < 3; ++k) sd[k] = cs[11] * (t * cs[8 + k] - sn[k]);
+        float tp = dot3(cs, sd), tn = dot3(cs + 4, sd);
+        for (int k = 0; k < 3; ++k) {
+            sp[k] = -cs[3] * (tp * cs[k] - sd[k]);
+            sx[k] = -cs[7] * (tn * cs[4 + k] - sd[k]);
+        }
+        for (int k = 0; k < 3; ++k) {
+            con[(e * 3 + 0) * 3 + k] = sp[k];
+            con[(e * 3 + 1) * 3 + k] = s[k] - sp[k] - sx[k];
+            con[(e * 3 + 2) * 3 + k] = sx[k];
+        }
+    }
+    __syncthreads();
+    plan_gather(c, op.plans[0]);
+}
+
+// ---- protein_hbond (src/hbond.cpp:357-435) ---------------------------------------
+__device__ void protein_hbond_fwd(Ctx& c, const OpRec& op, const PairGraph& G) {
+    pg_refresh(c, G, op);
+    int nd = G.n1;
+    const int* loc1 = c.CI + G.loc1;
+    const int* loc2 = c.CI + G.loc2;
+    const int* cnt = c.I + G.cnt;
+    const int* tcnt = c.I + G.tcnt;
+    const int* tslot = c.I + G.tslot;
+    const float* val = c.A + G.val;
+    const int* act = c.I + G.act;
+    for (int v = c.tid; v < op.n_elem; v += NT) {
+        bool don = v < nd;
+        int i = don ? v : v - nd;
+        const float* src = IN(op, 0) + (don ? loc1[i] : loc2[i]) * op.in_w[0];
+        float* o = OUT(op) + v * 7;
+        for (int k = 0; k < 6; ++k) o[k] = src[k];
+        float hl = 0.f;
+        if (don) {
+            for (int k = 0; k < cnt[i]; ++k) { int s = i * G.maxn + k; if (act[s]) hl += val[s]; }
+        } else {
+            for (int t = 0; t < tcnt[i]; ++t) { int s = tslot[i * G.maxt + t]; if (act[s]) hl += val[s]; }
+        }
+        o[6] = 1.f - expf(-hl);
+    }
+}
+__device__ void protein_hbond_bwd(Ctx& c, const OpRec& op, const PairGraph& G) {
+    int nd = G.n1;
+    float* ss = c.A + op.scratch;
+    for (int v = c.tid; v < op.n_elem; v += NT) ss[v] = SENS(op)[v * 7 + 6] * (1.f - OUT(op)[v * 7 + 6]);
+    __syncthreads();
+    const int* cnt = c.I + G.cnt;
+    const int* nbr = c.I + G.nbr;
+    float* es = c.A + G.esens;
+    int wave = c.tid >> 6, lane = c.tid & 63;
+    for (int i = wave; i < G.n1; i += NWAVE)
+        for (int k = lane; k < cnt[i]; k += 64) {
+            int s = i * G.maxn + k;
+            es[s] = ss[i] + ss[nd + nbr[s]];
+        }
+    __syncthreads();
+    pg_gather(c, G);
+    // pass the coordinate sensitivities through to infer_H_O (:419-433)
+    const int* loc1 = c.CI + G.loc1;
+    const int* loc2 = c.CI + G.loc2;
+    for (int i = c.tid; i < G.n1; i += NT)
+        for (int k = 0; k < 6; ++k) INS(op, 0)[loc1[i] * op.in_w[0] + k] += SENS(op)[i * 7 + k];
+    __syncthreads();
+    for (int j = c.tid; j < G.n2; j += NT)
+        for (int k = 0; k < 6; ++k) INS(op, 0)[loc2[j] * op.in_w[0] + k] += SENS(op)[(nd + j) * 7 + k];
+    __syncthreads();
+}
+
+// ---- hbond_coverage (src/hbond.cpp:438-484) ----------------------------------------
+__device__ void hbond_coverage_fwd(Ctx& c, const OpRec& op, const PairGraph& G) {
+    pg_refresh(c, G, op);
+    const int* tcnt = c.I + G.tcnt;
+    const int* tslot = c.I + G.tslot;
+    const float* val = c.A + G.val;
+    const int* act = c.I + G.act;
+    for (int j = c.tid; j < G.n2; j += NT) {
+        float acc = 0.f;
+        for (int t = 0; t < tcnt[j]; ++t) { int s = tslot[j * G.maxt + t]; if (act[s]) acc += val[s]; }
+        OUT(op)[j] = acc;
+    }
+}
+__device__ void hbond_coverage_bwd(Ctx& c, const OpRec& op, const PairGraph& G) {
+    const int* cnt = c.I + G.cnt;
+    const int* nbr = c.I + G.nbr;
+    float* es = c.A + G.esens;
+    int wave = c.tid >> 6, lane = c.tid & 63;
+    for (int i = wave; i < G.n1; i += NWAVE)
+        for (int k = lane; k < cnt[i]; k += 64) {
+            int s = i * G.maxn + k;
+            es[s] = SENS(op)[nbr[s]];
+        }
+    __syncthreads();
+    pg_gather(c, G);
+}
+
+// ---- hbond_energy (src/hbond.cpp:487-739) -------------------------------------------
+// ia: 0 n_res, 1 hb1 off (n_res+1), 2 hb1 idx, 3 hb2 off (n_res), 4 hb2 idx,
+//     5 first donor of residue, 6 first acceptor of residue (virtual index or -1)
+__device__ void hbond_energy_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
+    int n_res = op.ia[0];
+    const int* h1o = CIA(op, 1);
+    const int* h1i = CIA(op, 2);
+    const int* h2o = CIA(op, 3);
+    const int* h2i = CIA(op, 4);
+    const int* fd = CIA(op, 5);
+    const int* fa = CIA(op, 6);
+    const float* pp = IN(op, 0);
+    float* pps = INS(op, 0);
+    const float* rama = IN(op, 1);
+    float* ramas = INS(op, 1);
+    int wp = op.in_w[0], wr = op.in_w[1];
+    const float* P = op.fa;
+    float pot = 0.f;
+    for (int i = c.tid; i < n_res; i += NT) {
+        float phi = rama[i * wr], psi = rama[i * wr + 1];
+        float p1, d1, p2, d2, s1, e1, s2, e2;
+        compact_sigmoid(phi - P[4], P[5], p1, d1);
+        compact_sigmoid(phi - P[6], P[7], p2, d2);
+        compact_sigmoid(psi - P[8], P[9], s1, e1);
+        compact_sigmoid(psi - P[10], P[11], s2, e2);
+        float turn = (float)((1. - (double)p1) * p2);
+        float psis = (float)((1. - (double)s1) * s2);
+        float nturn = (float)(1. - (double)turn), npsi = (float)(1. - (double)psis);
+        float helix = nturn * psis, sheet = nturn * npsi;
+        float dturn = d2 - (d1 * p2 + p1 * d2);
+        float dpsis = e2 - (e1 * s2 + s1 * e2);
+        float Eh = P[0] * helix + P[1] * sheet + P[2] * turn;
+        float dphi = P[0] * (-dturn * psis) + P[1] * (-dturn * npsi) + P[2] * dturn;
+        float dpsi = P[0] * (nturn * dpsis) + P[1] * (nturn * -dpsis);
+        float hb = 0.f;
+        for (int k = h1o[i]; k < h1o[i + 1]; ++k) hb += pp[h1i[k] * wp + 6];
+        if (hb == 0.f) continue;
+        pot += hb * Eh;
+        if (fd[i] >= 0) pps[fd[i] * wp + 6] += Eh;
+        if (fa[i] >= 0) pps[fa[i] * wp + 6] += Eh;
+        ramas[i * wr] += hb * dphi;
+        ramas[i * wr + 1] += hb * dpsi;
+    }
+    __syncthreads();
+    for (int i = c.tid; i < n_res - 1; i += NT) {
+        float hb2 = 0.f;
+        for (int k = h2o[i]; k < h2o[i + 1]; ++k) hb2 += pp[h2i[k] * wp + 6];
+        if (hb2 <= 1.f) continue;
+        double v, dv;
+        compact_sigmoid(1.5 - (double)hb2, 2.0, v, dv);
+        pot += (float)(v * P[3]);
+        float dE = (float)(-dv * P[3]);
+        if (fd[i + 1] >= 0) pps[fd[i + 1] * wp + 6] += dE;
+        if (fa[i] >= 0) pps[fa[i] * wp + 6] += dE;
+    }
+    pot = block_sum(c, pot);
+    put_pot(c, S, op, pot);
+}
+
+// ---- weighted_pos (src/environment.cpp:489-533) -------------------------------------
+__device__ void weighted_pos_fwd(Ctx& c, const OpRec& op) {
+    const int* ip = CIA(op, 0);
+    const int* iw = CIA(op, 1);
+    for (int e = c.tid; e < op.n_elem; e += NT) {
+        const float* p = IN(op, 0) + ip[e] * op.in_w[0];
+        float* o = OUT(op) + e * 4;
+        o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
+        o[3] = expf(-IN(op, 1)[iw[e] * op.in_w[1]]);
+    }
+}
+__device__ void weighted_pos_bwd(Ctx& c, const OpRec& op) {
+    float* con = c.A + op.scratch;
+    for (int e = c.tid; e < op.n_elem; e += NT) con[e] = -OUT(op)[e * 4 + 3] * SENS(op)[e * 4 + 3];
+    __syncthreads();
+    plan_gather(c, op.plans[0]);
+    plan_gather(c, op.plans[1]);
+}
+
+// ---- environment_coverage (src/environment.cpp:288-348) -----------------------------
+__device__ void env_coverage_fwd(Ctx& c, const OpRec& op, const PairGraph& G) {
+    pg_refresh(c, G, op);
+    int naa = op.ia[0];
+    const int* aa = CIA(op, 1);
+    const int* id2 = c.CI + G.id2;
+    const int* cnt = c.I + G.cnt;
+    const int* nbr = c.I + G.nbr;
+    const float* val = c.A + G.val;
+    const int* act = c.I + G.act;
+    for (int i = c.tid; i < G.n1; i += NT) {
+        float* o = OUT(op) + i * naa;
+        for (int a = 0; a < naa; ++a) o[a] = 0.f;
+        for (int k = 0; k < cnt[i]; ++k) {
+            int s = i * G.maxn + k;
+            if (act[s]) o[aa[id2[nbr[s]]]] += val[s];
+        }
+    }
+}
+__device__ void env_coverage_bwd(Ctx& c, const OpRec& op, const PairGraph& G) {
+    int naa = op.ia[0];
+    const int* aa = CIA(op, 1);
+    const int* id2 = c.CI + G.id2;
+    const int* cnt = c.I + G.cnt;
+    const int* nbr = c.I + G.nbr;
+    float* es = c.A + G.esens;
+    int wave = c.tid >> 6, lane = c.tid & 63;
+    for (int i = wave; i < G.n1; i += NWAVE)
+        for (int k = lane; k < cnt[i]; k += 64) {
+            int s = i * G.maxn + k;
+            es[s] = SENS(op)[i * naa + aa[id2[nbr[s]]]];
+        }
+    __syncthreads();
+    pg_gather(c, G);
+}
+
+// ---- hb_environment_coverage (src/environment.cpp:350-385) ------------------------
+__device__ void hb_env_fwd(Ctx& c, const OpRec& op, const PairGraph& G) {
+    pg_refresh(c, G, op);
+    const int* cnt = c.I + G.cnt;
+    const float* val = c.A + G.val;
+    const int* act = c.I + G.act;
+    for (int i = c.tid; i < G.n1; i += NT) {
+        float acc = 0.f;
+        for (int k = 0; k < cnt[i]; ++k) { int s = i * G.maxn + k; if (act[s]) acc += val[s]; }
+        OUT(op)[i] = acc;
+    }
+}
+__device__ void hb_env_bwd(Ctx& c, const OpRec& op, const PairGraph& G) {
+    const int* cnt = c.I + G.cnt;
+    float* es = c.A + G.esens;
+    int wave = c.tid >> 6, lane = c.tid & 63;
+    for (int i = wave; i < G.n1; i += NWAVE)
+        for (int k = lane; k < cnt[i]; k += 64) es[i * G.maxn + k] = SENS(op)[i];
+    __syncthreads();
+    pg_gather(c, G);
+}
+
+// ---- sigmoid_coupling (src/environment.cpp:851-1003) -------------------------------
+// ia: 0 n_res, 1 coupling types, 2 weights (nt*nt), 3 scale, 4 center, 5 sharpness, 6 nt
+__device__ void sigmoid_coupling_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
+    int n = op.ia[0], nt = op.ia[6];
+    const int* ct = CIA(op, 1);
+    const float* W = CFA(op, 2);
+    const float* sc = CFA(op, 3);
+    const float* ce = CFA(op, 4);
+    const float* sh = CFA(op, 5);
+    const float* x = IN(op, 0);
+    float* xs = INS(op, 0);
+    float pot = 0.f;
+    for (int r = c.tid; r < n; r += NT) {
+        int t = ct[r];
+        float wn = 0.f;
+        for (int a = 0; a < nt; ++a) wn += W[t * nt + a] * x[r * nt + a];
+        float v, dv;
+        compact_sigmoid(wn - ce[t], sh[t], v, dv);
+        pot += sc[t] * v;
+        for (int a = 0; a < nt; ++a) xs[r * nt + a] += W[t * nt + a] * sc[t] * dv;
+    }
+    pot = block_sum(c, pot);
+    put_pot(c, S, op, pot);
+}
+
+// ---- bb_sigmoid_coupling (src/environment.cpp:1054-1138) ---------------------------
+__device__ void bb_sigmoid_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
+    int nt = op.ia[0];
+    const float* W = CFA(op, 1);
+    int n = op.ia[2];
+    float scale = op.fa[0], center = op.fa[1], sharp = op.fa[2], hw = op.fa[3];
+    const float* sc = IN(op, 0);
+    float* scs = INS(op, 0);
+    const float* hb = IN(op, 1);
+    float* hbs = INS(op, 1);
+    float pot = 0.f;
+    for (int r = c.tid; r < n; r += NT) {
+        float bl = 0.f;
+        for (int a = 0; a < nt; ++a) bl += W[a] * sc[r * nt + a];
+        bl += hw * hb[r * op.in_w[1]];
+        float v, dv;
+        compact_sigmoid(bl - center, sharp, v, dv);
+        pot += scale * v;
+        float dp = scale * dv;
+        for (int a = 0; a < nt; ++a) scs[r * nt + a] += W[a] * dp;
+        hbs[r * op.in_w[1]] += hw * dp;
+    }
+    pot = block_sum(c, pot);
+    put_pot(c, S, op, pot);
+}
+
+// ---- backbone_pairs (src/backbone_steric.cpp:38-146) --------------------------------
+__device__ void backbone_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G) {
+    pg_refresh(c, G, op);
+    const int* cnt = c.I + G.cnt;
+    const float* val = c.A + G.val;
+    const int* act = c.I + G.act;
+    float* es = c.A + G.esens;
+    float pot = 0.f;
+    for (int i = c.tid; i < G.n1; i += NT)
+        for (int k = 0; k < cnt[i]; ++k) {
+            int s = i * G.maxn + k;
+            es[s] = 1.f;
+            if (act[s]) pot += val[s];
+        }
+    pot = block_sum(c, pot);
+    put_pot(c, S, op, pot);
+    pg_gather(c, G);
+}
+
+// ---- rotamer (src/rotamer.cpp:840-850) ----------------------------------------------
+__device__ void rotamer_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G,
+                            const RotamerDev& R, int mode) {
+    bool rebuilt = pg_pack_and_check(c, G);
+    if (rebuilt) { pg_rebuild(c, G); rot_rebuild(c, G, R); }
+    pg_edges(c, G, op);
+    rot_fill(c, G, R);
+    int it = rot_solve(c, R);
+    if (c.tid == 0) {
+        int* st = c.I + R.stats;
+        st[0] = it;
+        if (it >= R.max_iter - R.chunk - 1) st[1] += 1;
+        st[2] += it;
+        st[3] += 1;
+    }
+    rot_sens(c, G, R, CIA(op, 0));
+    pg_gather(c, G);
+    float fe = mode ? rot_free_energy(c, R) : 0.f;
+    put_pot(c, S, op, fe);
+}
+
+// ---- program interpreter ----------------------------------------------------------
+__device__ void run_value(Ctx& c, const SysDesc& S, const OpRec& op, int mode) {
+    const PairGraph* G = op.graph >= 0 ? S.graphs + op.graph : nullptr;
+    switch (op.kind) {
+    case OP_DIST3D: dist3d_fwd(c, op); break;
+    case OP_ANGLE: angle_fwd(c, op); break;
+    case OP_DIHEDRAL: dihedral_fwd(c, op); break;
+    case OP_SPRING: spring_fwd(c, S, op); break;
+    case OP_RAMA_COORD: rama_coord_fwd(c, op); break;
+    case OP_CAT_POS: cat_pos_fwd(c, op); break;
+    case OP_RAMA_MAP_POT: rama_map_fwd(c, S, op); break;
+    case OP_AFFINE: affine_fwd(c, op); break;
+    case OP_PLACEMENT: placement_fwd(c, op); break;
+    case OP_INFER_HO: infer_fwd(c, op); break;
+    case OP_PROTEIN_HBOND: protein_hbond_fwd(c, op, *G); break;
+    case OP_HBOND_COVERAGE: hbond_coverage_fwd(c, op, *G); break;
+    case OP_HBOND_ENERGY: hbond_energy_fwd(c, S, op); break;
+    case OP_ROTAMER: rotamer_fwd(c, S, op, *G, S.rots[op.rot], mode); break;
+    case OP_WEIGHTED_POS: weighted_pos_fwd(c, op); break;
+    case OP_ENV_COVERAGE: env_coverage_fwd(c, op, *G); break;
+    case OP_HB_ENV_COVERAGE: hb_env_fwd(c, op, *G); break;
+    case OP_SIGMOID_COUPLING: sigmoid_coupling_fwd(c, S, op); break;
+    case OP_BB_SIGMOID_COUPLING: bb_sigmoid_fwd(c, S, op); break;
+    case OP_BACKBONE_PAIRS: backbone_fwd(c, S, op, *G); break;
+    default: break;
+    }
+    __syncthreads();
+}
+
+__device__ void run_deriv(Ctx& c, const SysDesc& S, const OpRec& op) {
+    const PairGraph* G = op.graph >= 0 ? S.graphs + op.graph : nullptr;
+    switch (op.kind) {
+    case OP_DIST3D: dist3d_bwd(c, op); break;
+    case OP_ANGLE: angle_bwd(c, op); break;
+    case OP_DIHEDRAL: dihedral_bwd(c, op); break;
+    case OP_RAMA_COORD: passthrough_bwd(c, op); break;
+    case OP_CAT_POS: passthrough_bwd(c, op); break;
+    case OP_AFFINE: affine_bwd(c, op); break;
+    case OP_PLACEMENT: placement_bwd(c, op); break;
+    case OP_INFER_HO: infer_bwd(c, op); break;
+    case OP_PROTEIN_HBOND: protein_hbond_bwd(c, op, *G); break;
+    case OP_HBOND_COVERAGE: hbond_coverage_bwd(c, op, *G); break;
+    case OP_WEIGHTED_POS: weighted_pos_bwd(c, op); break;
+    case OP_ENV_COVERAGE: env_coverage_bwd(c, op, *G); break;
+    case OP_HB_ENV_COVERAGE: hb_env_bwd(c, op, *G); break;
+    default: break;  // potential nodes push in their value pass
+    }
+    __syncthreads();
+}
+
+__device__ void compute(Ctx& c, const SysDesc& S, int mode) {
+    const int* zr = c.I + S.sens_zero;
+    for (int r = 0; r < S.sens_zero_cnt; ++r) {
+        float* p = c.A + zr[2 * r];
+        int n = zr[2 * r + 1];
+        for (int i = c.tid; i < n; i += NT) p[i] = 0.f;
+    }
+    __syncthreads();
+    for (int k = 0; k < S.n_germ; ++k) run_value(c, S, S.ops[k], mode);
+    for (int k = 0; k < S.n_deriv; ++k) run_deriv(c, S, S.ops[S.n_germ + k]);
+}
+
+// Threefry-4x32-20 (Salmon et al., SC'11), key (seed, stream, 0, 0) and counter
+// (step lo, step hi, atom, 0) as RandomGenerator (src/random.h:19-67).
+__device__ __forceinline__ uint32_t rotl32(uint32_t v, int r) { return (v << r) | (v >> (32 - r)); }
+__device__ void threefry4x32_20(const uint32_t ctr[4], const uint32_t key[4], uint32_t out[4]) {
+    const int R0[8] = {10, 11, 13, 23, 6, 17, 25, 18};
+    const int R1[8] = {26, 21, 27, 5, 20, 11, 10, 20};
+    uint32_t ks[5] = {key[0], key[1], key[2], key[3], 0x1BD11BDAu ^ key[0] ^ key[1] ^ key[2] ^ key[3]};
+    uint32_t x0 = ctr[0] + ks[0], x1 = ctr[1] + ks[1], x2 = ctr[2] + ks[2], x3 = ctr[3] + ks[3];
+#pragma unroll
+    for (int r = 0; r < 20; ++r) {
+        if ((r & 1) == 0) {
+            x0 += x1; x1 = rotl32(x1, R0[r & 7]) ^ x0;
+            x2 += x3; x3 = rotl32(x3, R1[r & 7]) ^ x2;
+        } else {
+            x0 += x3; x3 = rotl32(x3, R0[r & 7]) ^ x0;
+            x2 += x1; x1 = rotl32(x1, R1[r & 7]) ^ x2;
+        }
+        if ((r & 3) == 3) {
+            int s = (r >> 2) + 1;
+            x0 += ks[s % 5]; x1 += ks[(s + 1) % 5]; x2 += ks[(s + 2) % 5]; x3 += ks[(s + 3) % 5] + (uint32_t)s;
+        }
+    }
+    out[0] = x0; out[1] = x1; out[2] = x2; out[3] = x3;
+}
+
+// normal3 of RandomGenerator: Box-Muller on u01/uneg11 (src/boxmuller.hpp, src/uniform.hpp).
+__device__ void normal3(uint32_t seed, uint32_t stream, uint32_t atom, uint64_t step, float* o) {
+    uint32_t key[4] = {seed, stream, 0u, 0u};
+    uint32_t ctr[4] = {(uint32_t)(step & 0xffffffffull), (uint32_t)(step >> 32), atom, 0u};
+    uint32_t b[4];
+    threefry4x32_20(ctr, key, b);
+    float a0 = (float)(int32_t)b[0] * 4.656612873077392578125e-10f + 2.3283064365386962890625e-10f;
+    float u1 = (float)b[1] * 2.3283064365386962890625e-10f + 1.16415321826934814453125e-10f;
+    float a2 = (float)(int32_t)b[2] * 4.656612873077392578125e-10f + 2.3283064365386962890625e-10f;
+    float u3 = (float)b[3] * 2.3283064365386962890625e-10f + 1.16415321826934814453125e-10f;
+    const float PIf = 3.1415926535897932f;
+    float r1 = sqrtf(-2.f * logf(u1));
+    float r3 = sqrtf(-2.f * logf(u3));
+    float t1 = PIf * a0, t3 = PIf * a2;
+    o[0] = sinf(t1) * r1;
+    o[1] = cosf(t1) * r1;
+    o[2] = sinf(t3) * r3;
+}
+
+__device__ void thermostat(Ctx& c, SysDesc& S) {
+    float* mom = c.A + S.mom;
+    for (int a = c.tid; a < S.n_atom; a += NT) {
+        float n[3];
+        normal3(S.seed, 0u, (uint32_t)a, S.n_invocations, n);
+        for (int k = 0; k < 3; ++k) mom[a * 3 + k] = S.mom_scale * mom[a * 3 + k] + S.noise_scale * n[k];
+    }
+    __syncthreads();
+    if (c.tid == 0) S.n_invocations += 1;
+    __syncthreads();
+}
+
+__device__ float total_potential(Ctx& c, const SysDesc& S) {
+    float t = 0.f;
+    for (int k = 0; k < S.n_pot; ++k) t += c.A[S.pot + k];
+    return t;
+}
+
+__global__ void __launch_bounds__(NT) engine_kernel(SysDesc* systems, RunArgs args) {
+    __shared__ float red[NWAVE];
+    SysDesc& S = systems[blockIdx.x];
+    Ctx c{S.arena, S.iarena, S.cf, S.ci, red, (int)threadIdx.x};
+    if (args.n_round == 0) {
+        compute(c, S, args.mode);
+        return;
+    }
+    if (args.n_round < 0) {  // thermostat application only (fresh-start thermalisation)
+        thermostat(c, S);
+        return;
+    }
+    float* pos = c.A + S.pos_out;
+    const float* der = c.A + S.pos_sens;
+    float* mom = c.A + S.mom;
+    int nf = 0;
+    for (int rr = 0; rr < args.n_round; ++rr) {
+        int round = args.first_round + rr;
+        if (args.frame_interval > 0 && round % args.frame_interval == 0) {
+            compute(c, S, 1);
+            if (args.frame_pot && c.tid == 0 && nf < args.n_frames)
+                args.frame_pot[blockIdx.x * args.n_frames + nf] = total_potential(c, S);
+            ++nf;
+            __syncthreads();
+        }
+        if (args.thermostat_interval > 0 && round % args.thermostat_interval == 0) thermostat(c, S);
+        for (int stage = 0; stage < 3; ++stage) {
+            compute(c, S, 0);
+            for (int i = c.tid; i < S.n_atom * 3; i += NT) {
+                float p = mom[i] - args.dt * der[i];
+                mom[i] = p;
+                pos[i] += args.dt * p;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+}  // namespace upside
+
+// Host-side launcher (C++ linkage inside the library only).
+hipError_t upside_launch_engine(upside::SysDesc* d_systems, int n_sys, upside::RunArgs args, hipStream_t stream) {
+    hipLaunchKernelGGL(upside::engine_kernel, dim3(n_sys), dim3(upside::NT), 0, stream, d_systems, args);
+    return hipGetLastError();
+}

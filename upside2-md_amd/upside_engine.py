@@ -1,0 +1,241 @@
+"""ctypes binding of libupside.so, mirroring the reference's py/upside_engine.py.
+
+The class `Upside` keeps the reference's method names, argument meanings and
+error behaviour (py/upside_engine.py:186-276): it reads /input/pos and
+/input/sequence of a `.up` file, constructs an engine through
+construct_deriv_engine and raises RuntimeError when a call returns nonzero.
+`Batch` wraps the batched extension of include/upside_engine.h.
+
+The library is the HIP engine; there is no CPU fallback.  If it is missing or
+no GPU is visible, constructing an engine raises.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+try:
+    from . import h5lite
+except ImportError:
+    import h5lite
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libupside.so")
+
+_calc = None
+
+
+def load_upside_library(path=LIB_PATH):
+    """Load libupside.so and declare the C ABI (py/upside_engine.py:21-91)."""
+    global _calc
+    if _calc is not None:
+        return _calc
+    if not os.path.exists(path):
+        raise OSError("libupside.so not built at %s (run build() / make -C upside2-md_amd)" % path)
+    c = ct.cdll.LoadLibrary(path)
+    vp, i, f, cp = ct.c_void_p, ct.c_int, ct.c_float, ct.c_char_p
+    sig = {
+        "construct_deriv_engine": (vp, [i, cp, ct.c_bool]),
+        "free_deriv_engine": (None, [vp]),
+        "evaluate_energy": (i, [vp, vp, vp]),
+        "evaluate_deriv": (i, [vp, vp, vp]),
+        "set_param": (i, [i, vp, vp, cp]),
+        "get_param_deriv": (i, [i, vp, vp, cp]),
+        "get_param": (i, [i, vp, vp, cp]),
+        "get_output_dims": (i, [vp, vp, vp, cp]),
+        "get_sens": (i, [i, vp, vp, cp]),
+        "get_output": (i, [i, vp, vp, cp]),
+        "get_value_by_name": (i, [i, vp, vp, cp, cp]),
+        "get_clamped_value_and_deriv": (i, [i, vp, vp, i, vp]),
+        "clamped_spline_value": (i, [i, vp, vp, i, vp]),
+        "clamped_spline_solve": (i, [i, vp, vp]),
+        "get_clamped_coeff_deriv": (i, [i, vp, vp, f]),
+        "upside_batch_create": (vp, [i, vp, vp, ct.c_uint32, f, i]),
+        "upside_batch_free": (None, [vp]),
+        "upside_batch_n_atom": (i, [vp, i]),
+        "upside_batch_set_pos": (i, [vp, i, vp]),
+        "upside_batch_get_pos": (i, [vp, i, vp]),
+        "upside_batch_set_mom": (i, [vp, i, vp]),
+        "upside_batch_get_mom": (i, [vp, i, vp]),
+        "upside_batch_run": (i, [vp, i, i, vp, i]),
+        "upside_batch_energies": (i, [vp, vp]),
+        "upside_batch_replica_exchange": (i, [vp, i, vp, ct.c_uint64, vp]),
+        "upside_batch_bp_stats": (i, [vp, i, vp, vp, vp, vp]),
+        "upside_batch_last_run_ms": (f, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(c, name)
+        fn.restype = res
+        fn.argtypes = args
+    _calc = c
+    return c
+
+
+def _f4(x, shape=None):
+    x = np.require(x, dtype="f4", requirements="C")
+    if shape is not None:
+        assert x.shape == shape, (x.shape, shape)
+    return x
+
+
+class Upside:
+    """Force engine over one `.up` file (py/upside_engine.py:186-276)."""
+
+    def __init__(self, config_file_path, quiet=False):
+        self.calc = load_upside_library()
+        self.config_file_path = str(config_file_path)
+        with h5lite.File(self.config_file_path) as t:
+            self.initial_pos = t.read("/input/pos")[:, :, 0]
+            self.n_atom = self.initial_pos.shape[0]
+            self.sequence = t.read("/input/sequence") if t.exists("/input/sequence") else None
+        self.engine = self.calc.construct_deriv_engine(self.n_atom, self.config_file_path.encode(), bool(quiet))
+        if not self.engine:
+            raise RuntimeError("Unable to initialize upside engine for %s" % (config_file_path,))
+
+    def __repr__(self):
+        return "Upside(%r, %r)" % (self.n_atom, self.config_file_path)
+
+    def energy(self, pos):
+        pos = _f4(pos, (self.n_atom, 3))
+        energy = np.zeros(1, dtype="f4")
+        if self.calc.evaluate_energy(energy.ctypes.data, self.engine, pos.ctypes.data):
+            raise RuntimeError("Unable to evaluate energy")
+        return energy[0]
+
+    def deriv(self, pos):
+        pos = _f4(pos, (self.n_atom, 3))
+        deriv = np.zeros_like(pos)
+        if self.calc.evaluate_deriv(deriv.ctypes.data, self.engine, pos.ctypes.data):
+            raise RuntimeError("Unable to evaluate derivative")
+        return deriv
+
+    def set_param(self, param, node_name):
+        p = _f4(np.asarray(param).ravel())
+        if self.calc.set_param(int(p.shape[0]), p.ctypes.data, self.engine, node_name.encode()):
+            raise RuntimeError("Unable to set param with size %s for node %s" % (np.shape(param), node_name))
+
+    def get_param_deriv(self, param_shape, node_name):
+        deriv = np.zeros(param_shape, dtype="f4")
+        if self.calc.get_param_deriv(int(np.prod(param_shape)), deriv.ctypes.data, self.engine, node_name.encode()):
+            raise RuntimeError("Unable to get param deriv")
+        return deriv
+
+    def get_param(self, param_shape, node_name):
+        param = np.zeros(param_shape, dtype="f4")
+        if self.calc.get_param(int(np.prod(param_shape)), param.ctypes.data, self.engine, node_name.encode()):
+            raise RuntimeError("Unable to get param")
+        return param
+
+    def _dims(self, node_name):
+        n_elem = np.zeros(1, dtype=np.intc)
+        width = np.zeros(1, dtype=np.intc)
+        if self.calc.get_output_dims(n_elem.ctypes.data, width.ctypes.data, self.engine, node_name.encode()):
+            raise RuntimeError("Unable to get output dims")
+        return int(n_elem[0]), int(width[0])
+
+    def get_sens(self, node_name):
+        shape = self._dims(node_name)
+        out = np.zeros(shape, dtype="f4")
+        if self.calc.get_sens(out.size, out.ctypes.data, self.engine, node_name.encode()):
+            raise RuntimeError("Unable to get output")
+        return out
+
+    def get_output(self, node_name):
+        shape = self._dims(node_name)
+        out = np.zeros(shape, dtype="f4")
+        if self.calc.get_output(out.size, out.ctypes.data, self.engine, node_name.encode()):
+            raise RuntimeError("Unable to get output")
+        return out
+
+    def get_value_by_name(self, value_shape, node_name, log_name):
+        value = np.zeros(value_shape, dtype="f4")
+        if self.calc.get_value_by_name(int(np.prod(value_shape)), value.ctypes.data, self.engine,
+                                       node_name.encode(), log_name.encode()):
+            raise RuntimeError("Unable to get value by name")
+        return value
+
+    def edge_pairs(self, node_name):
+        """Sorted (i1, i2) element pairs of a pair-graph node's current edge set."""
+        n = int(self.get_value_by_name((1,), node_name, "n_edge")[0])
+        if n == 0:
+            return np.zeros((0, 2), np.int64)
+        return self.get_value_by_name((n, 2), node_name, "edge_pairs").astype(np.int64)
+
+    def __del__(self):
+        if getattr(self, "engine", None):
+            self.calc.free_deriv_engine(self.engine)
+            self.engine = None
+
+
+class Batch:
+    """Many systems integrated in one persistent launch (extension ABI)."""
+
+    def __init__(self, up_paths, temperatures, seed=1, dt=0.009, thermostat_interval=1):
+        self.calc = load_upside_library()
+        self.n_sys = len(up_paths)
+        arr = (ct.c_char_p * self.n_sys)(*[str(p).encode() for p in up_paths])
+        temps = _f4(np.asarray(temperatures, dtype="f4").reshape(self.n_sys))
+        self._keep = (arr, temps)
+        self.h = self.calc.upside_batch_create(self.n_sys, ct.cast(arr, ct.c_void_p), temps.ctypes.data,
+                                               int(seed), float(dt), int(thermostat_interval))
+        if not self.h:
+            raise RuntimeError("Unable to create batch")
+        self.n_atom = [self.calc.upside_batch_n_atom(self.h, s) for s in range(self.n_sys)]
+
+    def pos(self, s):
+        out = np.zeros((self.n_atom[s], 3), "f4")
+        if self.calc.upside_batch_get_pos(self.h, s, out.ctypes.data):
+            raise RuntimeError("get_pos failed")
+        return out
+
+    def set_pos(self, s, pos):
+        pos = _f4(pos, (self.n_atom[s], 3))
+        if self.calc.upside_batch_set_pos(self.h, s, pos.ctypes.data):
+            raise RuntimeError("set_pos failed")
+
+    def mom(self, s):
+        out = np.zeros((self.n_atom[s], 3), "f4")
+        if self.calc.upside_batch_get_mom(self.h, s, out.ctypes.data):
+            raise RuntimeError("get_mom failed")
+        return out
+
+    def set_mom(self, s, mom):
+        mom = _f4(mom, (self.n_atom[s], 3))
+        if self.calc.upside_batch_set_mom(self.h, s, mom.ctypes.data):
+            raise RuntimeError("set_mom failed")
+
+    def run(self, n_round, frame_interval=0, n_frames=0):
+        frames = np.zeros((self.n_sys, max(n_frames, 0)), "f4")
+        ptr = frames.ctypes.data if n_frames > 0 else None
+        if self.calc.upside_batch_run(self.h, int(n_round), int(frame_interval), ptr, int(n_frames)):
+            raise RuntimeError("batch run failed")
+        return frames
+
+    def energies(self):
+        e = np.zeros(self.n_sys, "f4")
+        if self.calc.upside_batch_energies(self.h, e.ctypes.data):
+            raise RuntimeError("energies failed")
+        return e
+
+    def replica_exchange(self, pairs, round_num):
+        pairs = np.require(np.asarray(pairs, dtype=np.intc).reshape(-1, 2), requirements="C")
+        acc = np.zeros(len(pairs), np.intc)
+        if self.calc.upside_batch_replica_exchange(self.h, len(pairs), pairs.ctypes.data, int(round_num),
+                                                   acc.ctypes.data):
+            raise RuntimeError("replica exchange failed")
+        return acc
+
+    def bp_stats(self, s):
+        last = ct.c_int(0)
+        tot, sol, bad = ct.c_longlong(0), ct.c_longlong(0), ct.c_longlong(0)
+        if self.calc.upside_batch_bp_stats(self.h, s, ct.byref(last), ct.byref(tot), ct.byref(sol), ct.byref(bad)):
+            raise RuntimeError("bp stats failed")
+        return dict(last=last.value, total=tot.value, solves=sol.value, bad=bad.value)
+
+    def last_run_ms(self):
+        return float(self.calc.upside_batch_last_run_ms(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.calc.upside_batch_free(self.h)
+            self.h = None

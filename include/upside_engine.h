@@ -50,7 +50,11 @@ int get_sens(int n_output, float* output, DerivEngine* engine, const char* node_
  * src/hbond.cpp:476-482.  Supported names: rotamer: "rotamer_free_energy",
  * "count_edges_by_type", "n_node", "read n_bad_solve", "read n_bad_solve and
  * reset"; any pair-graph node: "count_edges_by_type", "n_edge", "edge_pairs"
- * (2*n_edge element indices, sorted), "edge_values" (n_edge, same order). */
+ * (2*n_edge element indices, sorted), "edge_values" (n_edge, same order).
+ * Measurement hooks with no reference counterpart: pair graphs "graph_shape"
+ * (n1, dim1, n2, dim2); rotamer "iterations" (of the last solve),
+ * "bp_edge_classes" (present residue-pair edges of classes 11 13 16 33 36 66),
+ * "node_classes" (residues with 1, 3, 6 rotamers). */
 int get_value_by_name(int n_output, float* output, DerivEngine* engine, const char* node_name,
                       const char* log_name);
 /* src/engine_c_library.cpp:197-277: clamped cubic B-spline helpers. */
@@ -92,6 +96,11 @@ int upside_batch_bp_stats(UpsideBatch* batch, int sys, int* last_iters, long lon
                           long long* solves, long long* bad_solves);
 /* Synchronize and time the device (for benchmarks): elapsed ms of the last run. */
 float upside_batch_last_run_ms(UpsideBatch* batch);
+/* Per-node wall-clock accumulation inside the kernel (the analogue of the
+ * reference's -DCOLLECT_PROFILE timers, src/timing.cpp:11-54).  The report is
+ * a text table; returns the needed size if buflen is too small. */
+int upside_batch_enable_profile(UpsideBatch* batch, int on);
+int upside_batch_profile_report(UpsideBatch* batch, int sys, char* buf, int buflen);
 
 #ifdef __cplusplus
 }

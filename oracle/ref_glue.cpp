@@ -75,3 +75,30 @@ RegisterNodeType<GlueRamaCoord, 2> glue_rama("rama_coord");
 RegisterNodeType<GlueCatPos, 2> glue_cat("cat_pos");
 
 }  // namespace
+
+// Known-answer probes of the reference's thermostat random numbers
+// (src/random.h:19-67 over the vendored Random123 threefry and boxmuller),
+// called by the tests to pin the oracle's restatement bit for bit.
+#include "random.h"
+
+extern "C" void ref_threefry4x32_20(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+    threefry4x32_ctr_t c = {{ctr[0], ctr[1], ctr[2], ctr[3]}};
+    threefry4x32_key_t k = {{key[0], key[1], key[2], key[3]}};
+    threefry4x32_ctr_t r = threefry4x32(c, k);
+    for (int i = 0; i < 4; ++i) out[i] = r.v[i];
+}
+
+extern "C" void ref_normal3(uint32_t seed, uint32_t stream, int n_atom, uint64_t step, float* out) {
+    for (int a = 0; a < n_atom; ++a) {
+        RandomGenerator g(seed, stream, (uint32_t)a, step);
+        float3 v = g.normal3();
+        out[a * 3 + 0] = v.x();
+        out[a * 3 + 1] = v.y();
+        out[a * 3 + 2] = v.z();
+    }
+}
+
+extern "C" float ref_uniform_x(uint32_t seed, uint32_t stream, uint32_t index, uint64_t step) {
+    RandomGenerator g(seed, stream, index, step);
+    return g.uniform_open_closed().x();
+}

@@ -470,8 +470,8 @@ def _eig4_group(A, T, tol=1e-5, max_iter=100):
             trivial = z == 0
             with np.errstate(divide="ignore", invalid="ignore"):
                 inv_r = one / np.sqrt(x * x + z * z)
-            c = np.where(trivial, one, x * inv_r)
-            s_ = np.where(trivial, T(0), -z * inv_r)
+                c = np.where(trivial, one, x * inv_r)
+                s_ = np.where(trivial, T(0), -z * inv_r)
             t1 = rot[:, p + k].copy()
             t2 = rot[:, p + k + 1].copy()
             rot[:, p + k] = c[:, None] * t1 - s_[:, None] * t2
@@ -1127,6 +1127,7 @@ class BackbonePairs(PotentialNode):
         d2 = refine_dist2(d, T)
         iu1, iu2 = np.nonzero(np.triu((d2 < self.dist_cutoff * self.dist_cutoff)
                                       & (np.abs(self.res[:, None] - self.res[None, :]) > 1), 1))
+        self.i1, self.i2 = iu1, iu2
         cut2 = T(3.0 * 3.0 + 0.1 * 3.0)
         sharp = T(1.0) / (T(3.0) * T(0.1))
         pot = T(0)
@@ -1402,7 +1403,9 @@ class Engine:
             lvl += 1
 
     def compute(self, pos, mode=PotentialAndDerivMode):
-        self.nodes["pos"].output[:] = np.asarray(pos, np.float32).astype(self.T)
+        # float32 engines see float32 positions; the FP64 re-evaluation keeps
+        # whatever precision the caller passes
+        self.nodes["pos"].output[:] = np.asarray(pos).astype(self.T)
         self.potential = 0.0
         for n in self.germ:
             node = self.nodes[n]
@@ -1478,3 +1481,38 @@ def thermostat_normals(seed, stream, n_atom, step):
         r = np.sqrt(f32(-2.0) * np.log(u01(bits[:, b]))).astype(f32)
         out += [np.sin(ang) * r, np.cos(ang) * r]
     return np.stack(out[:3], -1).astype(f32)
+
+
+def run_rounds(eng, pos, mom, n_round, dt, temperature, seed, n_invocations, thermostat_interval=1,
+               timescale=5.0, first_round=0):
+    """Rounds of the reference's simulation loop for one system
+    (src/main.cpp:775-803): the Ornstein-Uhlenbeck thermostat every
+    `thermostat_interval` rounds (src/thermostat.cpp:9-18, mom_scale
+    exp(-interval*3*dt/timescale), noise_scale sqrt(T(1-mom_scale^2))), then
+    three symplectic-Euler stages of unit mass (DerivEngine::integration_cycle,
+    src/deriv_engine.cpp:317-331).  A fresh start first applies the thermostat
+    once to zero momenta.  Returns (pos, mom, n_invocations); float32."""
+    f32 = np.float32
+    ms = f32(np.exp(-thermostat_interval * 3.0 * dt / timescale))
+    ns = f32(np.sqrt(f32(temperature) * (f32(1) - ms * ms)))
+    pos = np.array(pos, f32)
+    mom = np.array(mom, f32)
+    dt = f32(dt)
+    for r in range(first_round, first_round + n_round):
+        if thermostat_interval > 0 and r % thermostat_interval == 0:
+            mom = (ms * mom + ns * thermostat_normals(seed, 0, len(mom), n_invocations)).astype(f32)
+            n_invocations += 1
+        for _ in range(3):
+            d = eng.deriv(pos).astype(f32)
+            mom = (mom - dt * d).astype(f32)
+            pos = (pos + dt * mom).astype(f32)
+    return pos, mom, n_invocations
+
+
+def thermalize(n_atom, temperature, seed, dt, thermostat_interval=1, timescale=5.0):
+    """Fresh-start momenta: one thermostat application to zero momenta
+    (invocation 0), as the reference does before the first round."""
+    f32 = np.float32
+    ms = f32(np.exp(-thermostat_interval * 3.0 * dt / timescale))
+    ns = f32(np.sqrt(f32(temperature) * (f32(1) - ms * ms)))
+    return (ns * thermostat_normals(seed, 0, n_atom, 0)).astype(f32), 1

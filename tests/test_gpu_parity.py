@@ -1,0 +1,262 @@
+"""Parity of the HIP engine with the oracle, through the C ABI (needs a GPU).
+
+Bars (written per test): total energy within 1e-5 relative and dE/dx within
+1e-4 relative RMS (||F-F_ref||/||F_ref||) of the float32 oracle, of the FP64
+oracle and, where oracle/_ref is present, of the compiled reference; pair
+lists and BP edge sets bit-exact; integer statistics (edge counts by type,
+BP iterations) equal.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import REF_LIB, RefEngine, read_pos, ref_lib, rel_rms, say
+
+import upside_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+E_TOL = 1e-5
+F_TOL = 1e-4
+
+PAIR_NODES = ["rotamer", "protein_hbond", "hbond_coverage", "hbond_coverage_hydrophobe",
+              "environment_coverage_sc", "environment_coverage_hb", "hb_environment_coverage_hn",
+              "hb_environment_coverage_oc", "backbone_pairs"]
+
+
+@pytest.fixture(scope="module")
+def ue():
+    import upside_engine
+    upside_engine.load_upside_library()
+    return upside_engine
+
+
+def oracle_edges(eng, name):
+    node = eng.nodes[name]
+    obj = node.ig if hasattr(node, "ig") else node
+    return set(zip(obj.i1.tolist(), obj.i2.tolist()))
+
+
+def check_point(ue_eng, ora, ora64, pos, label, ref=None):
+    e, d = float(ue_eng.energy(pos)), ue_eng.deriv(pos)
+    e32, d32 = float(ora.energy(pos)), ora.deriv(pos)
+    e64, d64 = float(ora64.energy(pos.astype(np.float64))), ora64.deriv(pos.astype(np.float64))
+    err = dict(e32=abs(e - e32) / abs(e32), f32=rel_rms(d, d32), e64=abs(e - e64) / abs(e64), f64=rel_rms(d, d64))
+    if ref is not None:
+        er, dr = ref.energy(pos), ref.deriv(pos)
+        err.update(eref=abs(e - er) / abs(er), fref=rel_rms(d, dr))
+    say(label, " ".join("%s=%.2e" % kv for kv in sorted(err.items())))
+    for k, v in err.items():
+        assert v < (E_TOL if k.startswith("e") else F_TOL), (label, k, v)
+    return e, d
+
+
+def test_chig_frames(ue, chig):
+    """The 20 frames of the reference's chignolin run: potential against the
+    reference-logged values and the oracle, forces against the oracle."""
+    path, z = chig
+    say("case: chignolin golden frames")
+    eng = ue.Upside(path, quiet=True)
+    ora, ora64 = O.Engine(path), O.Engine(path, dtype=np.float64)
+    got = []
+    for f in range(len(z["frame_pos"])):
+        pos = np.ascontiguousarray(z["frame_pos"][f], np.float32)
+        e, _ = check_point(eng, ora, ora64, pos, "chig frame %d" % f)
+        got.append(e)
+    dev = np.abs(np.array(got) - z["frame_potential"].ravel())
+    say("chig max |E - E_logged| = %.2e" % dev.max())
+    assert dev.max() < 2e-3
+
+
+@pytest.mark.parametrize("n_res,seed", [(40, 1), (100, 2), (500, 3)])
+def test_random_chain_energy_forces(ue, chain_file, n_res, seed):
+    path = chain_file(n_res, seed)
+    say("case: random chain L=%d seed=%d" % (n_res, seed))
+    pos0 = read_pos(path)
+    eng = ue.Upside(path, quiet=True)
+    ora, ora64 = O.Engine(path), O.Engine(path, dtype=np.float64)
+    ref = RefEngine(ref_lib(), path, len(pos0)) if os.path.exists(REF_LIB) else None
+    check_point(eng, ora, ora64, pos0, "L=%d start" % n_res, ref)
+    rng = np.random.default_rng(seed)
+    for k in range(2):
+        pos = (pos0 + rng.normal(scale=0.2, size=pos0.shape)).astype(np.float32)
+        check_point(eng, ora, ora64, pos, "L=%d jitter %d" % (n_res, k), ref)
+
+
+@pytest.mark.parametrize("n_res,seed", [(100, 2), (500, 3)])
+def test_pair_lists_bit_exact(ue, chain_file, n_res, seed):
+    """Every interaction graph's edge set equals the oracle's brute force on
+    the same coordinates, at the start and after the Verlet caches have aged
+    through a short trajectory."""
+    path = chain_file(n_res, seed)
+    pos0 = read_pos(path)
+    eng = ue.Upside(path, quiet=True)
+    ora = O.Engine(path)
+    b = ue.Batch([path], [0.85], seed=seed)
+    say("case: pair lists L=%d: run 20 rounds" % n_res)
+    b.run(20)
+    pos_run = b.pos(0)
+    for label, pos in (("start", pos0), ("after 20 rounds", pos_run)):
+        eng.energy(pos)
+        ora.energy(pos)
+        for name in PAIR_NODES:
+            got = set(map(tuple, eng.edge_pairs(name).tolist()))
+            want = oracle_edges(ora, name)
+            say("L=%d %s %-28s edges %6d  xor %d" % (n_res, label, name, len(want), len(got ^ want)))
+            assert got == want, (name, label, sorted(got ^ want)[:10])
+
+
+def test_coordinate_node_outputs(ue, chain_file):
+    """Outputs and sensitivities of the coordinate nodes (get_output/get_sens)."""
+    path = chain_file(60, 7)
+    pos = read_pos(path)
+    pos = pos + np.random.default_rng(0).normal(scale=0.1, size=pos.shape).astype(np.float32)
+    say("case: coordinate node outputs L=60")
+    eng = ue.Upside(path, quiet=True)
+    ora = O.Engine(path)
+    eng.deriv(pos)
+    ora.deriv(pos)
+    for name in ora.order:
+        node = ora.nodes[name]
+        if node.potential_term:
+            got = float(eng.get_output(name)[0, 0])
+            assert abs(got - float(node.potential)) <= 1e-5 * max(1.0, abs(float(node.potential))), name
+            continue
+        out = eng.get_output(name)
+        want = node.output[:, :out.shape[1]]
+        scale = max(1.0, float(np.abs(want).max()))
+        assert np.abs(out - want).max() <= 2e-5 * scale, (name, np.abs(out - want).max())
+        sens = eng.get_sens(name)
+        ws = node.sens[:, :sens.shape[1]]
+        sscale = max(1.0, float(np.abs(ws).max()))
+        assert np.abs(sens - ws).max() <= 1e-4 * sscale, (name, np.abs(sens - ws).max())
+
+
+def test_rotamer_bp_statistics(ue, chain_file):
+    """BP iteration count, edge counts by type and n_bad_solve equal the
+    oracle's on the same coordinates."""
+    path = chain_file(100, 2)
+    pos = read_pos(path)
+    say("case: rotamer BP statistics L=100")
+    eng = ue.Upside(path, quiet=True)
+    ora = O.Engine(path)
+    rng = np.random.default_rng(5)
+    for k in range(3):
+        p = (pos + rng.normal(scale=0.15, size=pos.shape)).astype(np.float32)
+        eng.energy(p)
+        ora.energy(p)
+        it = int(eng.get_value_by_name((1,), "rotamer", "iterations")[0])
+        rot = ora.nodes["rotamer"]
+        say("jitter %d: BP iterations gpu %d oracle %d" % (k, it, rot.iterations))
+        assert abs(it - rot.iterations) <= 2 * rot.chunk
+        ig = rot.ig
+        cnt = np.zeros((ig.type1.max() + 1, ig.n_type2), np.float32)
+        np.add.at(cnt, (ig.type1[ig.i1], ig.type2[ig.i2]), 1)
+        got = eng.get_value_by_name(cnt.shape, "rotamer", "count_edges_by_type")
+        np.testing.assert_array_equal(got, cnt)
+    assert eng.get_value_by_name((1,), "rotamer", "read n_bad_solve and reset")[0] == 0
+
+
+def test_batch_matches_single_engine(ue, chain_file):
+    """Systems of uneven length in one launch: each system's potential equals
+    the single-engine value at its positions."""
+    paths = [chain_file(n, s) for n, s in ((30, 11), (90, 12), (150, 13), (90, 12))]
+    say("case: uneven batch of 4")
+    b = ue.Batch(paths, [0.8, 0.85, 0.9, 0.95], seed=3)
+    b.run(5)
+    e_b = b.energies()
+    for s, p in enumerate(paths):
+        eng = ue.Upside(p, quiet=True)
+        e = eng.energy(b.pos(s))
+        assert abs(e - e_b[s]) <= 1e-6 * abs(e), (s, e, e_b[s])
+    # the two copies of one input file started identically but use different seeds
+    assert not np.array_equal(b.pos(1), b.pos(3))
+
+
+def test_trajectory_matches_oracle_integrator(ue, chain_file):
+    """Thermalisation plus 4 rounds (12 force evaluations and integrator
+    stages) against the oracle's restatement of the simulation loop with the
+    same Threefry noise."""
+    path = chain_file(40, 21)
+    say("case: trajectory L=40, 4 rounds")
+    pos0 = read_pos(path)
+    b = ue.Batch([path], [0.85], seed=7, dt=0.009)
+    mom_gpu0 = b.mom(0)
+    ora = O.Engine(path)
+    mom0, n_inv = O.thermalize(len(pos0), 0.85, 7, 0.009)
+    np.testing.assert_allclose(mom_gpu0, mom0, rtol=2e-5, atol=2e-6)
+    b.run(4)
+    pos, mom, _ = O.run_rounds(ora, pos0, mom0, 4, 0.009, 0.85, 7, n_inv)
+    say("trajectory pos rel %.2e mom rel %.2e" % (rel_rms(b.pos(0), pos), rel_rms(b.mom(0), mom)))
+    assert rel_rms(b.pos(0), pos) < 1e-5
+    assert rel_rms(b.mom(0), mom) < 1e-3
+
+
+def test_replica_exchange(ue, chain_file):
+    """Swap decisions follow the Metropolis test on the batch energies with the
+    uniform numbers of Threefry stream 1 (src/main.cpp:243-316)."""
+    path = chain_file(40, 31)
+    temps = list(0.80 * (0.95 / 0.80) ** (np.arange(8) / 7))
+    say("case: replica exchange 8 replicas")
+    b = ue.Batch([path] * 8, temps, seed=5)
+    b.run(6)
+    pairs = np.array([[0, 1], [2, 3], [4, 5], [6, 7]])
+    pos_before = [b.pos(s) for s in range(8)]
+    e_before = b.energies()
+    acc = b.replica_exchange(pairs, 3)
+    beta = 1 / np.array(temps, np.float32)
+    key = [5, 1, 0, 0]
+    ctr = 0
+    for (s1, s2), a in zip(pairs, acc):
+        diff = np.float32((-beta[s1] * e_before[s2] - beta[s2] * e_before[s1])
+                          - (-beta[s1] * e_before[s1] - beta[s2] * e_before[s2]))
+        want = 1
+        if diff < 0:
+            bits = O.threefry4x32_20(np.array([[3, 0, 0, ctr]], np.uint32), key)[0]
+            ctr += 1
+            u = np.float32(bits[0]) * np.float32(2.0 ** -32) + np.float32(2.0 ** -33)
+            want = 0 if np.exp(diff) < u else 1
+        assert a == want, (s1, s2, diff)
+        np.testing.assert_array_equal(b.pos(s1), pos_before[s2] if a else pos_before[s1])
+        np.testing.assert_array_equal(b.pos(s2), pos_before[s1] if a else pos_before[s2])
+
+
+def test_thermostat_temperature(ue, chain_file):
+    """Equipartition: after the start's strain has been thermostatted away
+    (timescale 5 = 185 rounds), the kinetic energy per degree of freedom
+    averages T/2."""
+    path = chain_file(150, 41)
+    temps = [0.6, 0.85, 1.1, 0.85]
+    say("case: thermostat equipartition L=150, 4 systems, 700 rounds")
+    b = ue.Batch([path] * 4, temps, seed=9)
+    b.run(400)
+    ke = np.zeros(4)
+    n = 0
+    for _ in range(30):
+        b.run(10)
+        for s in range(4):
+            ke[s] += 0.5 * (b.mom(s) ** 2).sum() / (3 * b.n_atom[s])
+        n += 1
+    ke /= n
+    want = 0.5 * np.array(temps)
+    say("mean KE/dof", ke, "targets", want)
+    np.testing.assert_allclose(ke, want, rtol=0.06)
+
+
+def test_error_behaviour(ue, chain_file, updir):
+    """Reference error conventions: NULL engine on a bad file, nonzero return
+    codes on bad names and sizes, raised as RuntimeError by the binding."""
+    say("case: error behaviour")
+    with pytest.raises(RuntimeError):
+        ue.Upside(str(updir / "does_not_exist.up"), quiet=True)
+    path = chain_file(20, 51)
+    eng = ue.Upside(path, quiet=True)
+    with pytest.raises(RuntimeError):
+        eng.get_output("no_such_node")
+    with pytest.raises(RuntimeError):
+        eng.get_value_by_name((3,), "rotamer", "n_node")
+    with pytest.raises(RuntimeError):
+        eng.get_param_deriv((4,), "rotamer")
+    e = eng.energy(read_pos(path))
+    assert np.isfinite(e)

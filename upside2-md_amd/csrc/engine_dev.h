@@ -128,7 +128,12 @@ struct SysDesc {
     float mom_scale, noise_scale;
     uint64_t n_invocations;
     int error;             // int offset of error word
+    unsigned long long* prof;  // optional per-op wall-clock accumulators (see PROF_*), or null
 };
+
+// Profile accumulator layout: [value op k] for k < n_germ, [deriv op k] at
+// n_germ + k, then the fixed slots below (wall_clock64 ticks).
+enum { PROF_BP = 0, PROF_REBUILD = 1, PROF_THERMO = 2, PROF_INTEGRATE = 3, PROF_EXTRA = 4 };
 
 struct RunArgs {
     int mode;              // 1: potential and derivative, 0: derivative only

@@ -1148,9 +1148,61 @@ struct Batch {
     uint32_t base_seed = 0;
     bool have_mom = false;
     double cap_scale = 1.0;
+    bool profile = false;
+    vector<unsigned long long*> prof;
+    int prof_len(int k) const { return sys[k].prog->n_germ() + sys[k].prog->n_deriv() + PROF_EXTRA; }
+
+    void enable_profile(bool on) {
+        for (auto p : prof) (void)hipFree(p);
+        prof.assign(sys.size(), nullptr);
+        profile = on;
+        if (on)
+            for (size_t k = 0; k < sys.size(); ++k) {
+                size_t bytes = prof_len((int)k) * sizeof(unsigned long long);
+                hip_check(hipMalloc((void**)&prof[k], bytes), "profile");
+                hip_check(hipMemset(prof[k], 0, bytes), "profile");
+            }
+        pull_desc();
+        for (size_t k = 0; k < sys.size(); ++k) desc[k].prof = on ? prof[k] : nullptr;
+        push_desc();
+    }
+
+    string profile_report(int k) {
+        if (!profile) return "profiling disabled\n";
+        int n = prof_len(k);
+        vector<unsigned long long> t(n);
+        hip_check(hipMemcpy(t.data(), prof[k], n * sizeof(unsigned long long), hipMemcpyDeviceToHost), "profile");
+        int rate_khz = 100000;
+        (void)hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, 0);
+        double tick = 1.0 / (rate_khz * 1e3);
+        const Topology& T = sys[k].prog->T;
+        string out;
+        char line[256];
+        int ng = sys[k].prog->n_germ();
+        for (int i = 0; i < n; ++i) {
+            string label;
+            if (i < ng) label = "value  " + T.nodes[T.germ[i + 1]].name;
+            else if (i < ng + (int)T.deriv.size()) label = "deriv  " + T.nodes[T.deriv[i - ng]].name;
+            else {
+                static const char* ex[] = {"rotamer BP solve (in value)", "rotamer graph rebuild (in value)",
+                                           "thermostat", "integrator update"};
+                label = ex[i - ng - (int)T.deriv.size()];
+            }
+            snprintf(line, sizeof(line), "%-48s %12.6f s\n", label.c_str(), t[i] * tick);
+            out += line;
+        }
+        for (size_t g = 0; g < T.graphs.size(); ++g) {
+            int st[4];
+            hip_check(hipMemcpy(st, sys[k].iarena + T.graphs[g].state, sizeof(st), hipMemcpyDeviceToHost), "state");
+            snprintf(line, sizeof(line), "rebuilds %-39s %12d\n", T.nodes[T.graph_node[g]].name.c_str(), st[2]);
+            out += line;
+        }
+        return out;
+    }
 
     ~Batch() {
         for (auto& s : sys) s.release();
+        for (auto p : prof) (void)hipFree(p);
         (void)hipFree(d_desc);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -1200,6 +1252,7 @@ struct Batch {
         d.mom_scale = ms;
         d.noise_scale = std::sqrt(s.temperature * (1.f - ms * ms));
         d.n_invocations = 0;
+        d.prof = (profile && k < (int)prof.size()) ? prof[k] : nullptr;
         return d;
     }
 
@@ -1522,7 +1575,35 @@ int get_value_by_name(int n_output, float* output, DerivEngine* engine, const ch
         if (&T.nodes[T.graph_node[g]] == n) gidx = (int)g;
     string ln(log_name);
     vector<float> out;
-    if (gidx >= 0 && (ln == "count_edges_by_type" || ln == "n_edge" || ln == "edge_pairs" || ln == "edge_values")) {
+    if (gidx >= 0 && ln == "graph_shape") {
+        const PairGraph& G = T.graphs[gidx];
+        out = {(float)G.n1, (float)G.dim1, (float)G.n2, (float)G.dim2};
+    } else if (n->type == "rotamer" && ln == "bp_edge_classes") {
+        // present residue-pair edges by rotamer-count class: 11 13 16 33 36 66
+        const RotamerDev& R = T.rots[T.ops[n->op].rot];
+        vector<int> ccnt(R.n_node), cnbr((size_t)R.n_node * R.maxc), cpres((size_t)R.n_node * R.maxc);
+        int* I = B.sys[0].iarena;
+        hip_check(hipMemcpy(ccnt.data(), I + R.ccnt, ccnt.size() * 4, hipMemcpyDeviceToHost), "bp");
+        hip_check(hipMemcpy(cnbr.data(), I + R.cnbr, cnbr.size() * 4, hipMemcpyDeviceToHost), "bp");
+        hip_check(hipMemcpy(cpres.data(), I + R.cpres, cpres.size() * 4, hipMemcpyDeviceToHost), "bp");
+        const int* nrot = T.ci.data() + R.node_nrot;
+        out.assign(6, 0.f);
+        auto cls = [](int a, int b) {
+            if (a > b) std::swap(a, b);
+            if (a == 1) return b == 1 ? 0 : (b == 3 ? 1 : 2);
+            return a == 3 ? (b == 3 ? 3 : 4) : 5;
+        };
+        for (int a = 0; a < R.n_node; ++a)
+            for (int k = 0; k < std::min(ccnt[a], R.maxc); ++k) {
+                size_t ci = (size_t)a * R.maxc + k;
+                if (cpres[ci]) out[cls(nrot[a], nrot[cnbr[ci]])] += 1.f;
+            }
+    } else if (n->type == "rotamer" && ln == "node_classes") {
+        const RotamerDev& R = T.rots[T.ops[n->op].rot];
+        const int* nrot = T.ci.data() + R.node_nrot;
+        out.assign(3, 0.f);
+        for (int a = 0; a < R.n_node; ++a) out[nrot[a] == 1 ? 0 : (nrot[a] == 3 ? 1 : 2)] += 1.f;
+    } else if (gidx >= 0 && (ln == "count_edges_by_type" || ln == "n_edge" || ln == "edge_pairs" || ln == "edge_values")) {
         const PairGraph& G = T.graphs[gidx];
         long slots = (long)G.n1 * G.maxn;
         vector<int> cnt(G.n1), nbr(slots), act(slots);
@@ -1771,5 +1852,21 @@ int upside_batch_bp_stats(UpsideBatch* b, int sys, int* last_iters, long long* t
 }
 
 float upside_batch_last_run_ms(UpsideBatch* b) { return b ? b->batch.last_ms : -1.f; }
+
+int upside_batch_enable_profile(UpsideBatch* b, int on) {
+    ABI_TRY
+    b->batch.enable_profile(on != 0);
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_profile_report(UpsideBatch* b, int sys, char* buf, int buflen) {
+    ABI_TRY
+    string r = b->batch.profile_report(sys);
+    if ((int)r.size() + 1 > buflen) return (int)r.size() + 1;
+    std::memcpy(buf, r.c_str(), r.size() + 1);
+    return 0;
+    ABI_CATCH(-1)
+}
 
 }  // extern "C"

@@ -18,6 +18,10 @@ namespace upside {
 #define CIA(op, k) (c.CI + (op).ia[k])
 #define CFA(op, k) (c.CF + (op).ia[k])
 
+__device__ __forceinline__ void prof_add(const SysDesc& S, int slot, unsigned long long t0) {
+    if (S.prof && threadIdx.x == 0) S.prof[slot] += wall_clock64() - t0;
+}
+
 __device__ __forceinline__ void put_pot(Ctx& c, const SysDesc& S, const OpRec& op, float e) {
     if (c.tid == 0) c.A[S.pot + op.pot_slot] = e;
 }
@@ -809,10 +813,14 @@ __device__ void backbone_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const Pa
 __device__ void rotamer_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G,
                             const RotamerDev& R, int mode) {
     bool rebuilt = pg_pack_and_check(c, G);
+    unsigned long long t0 = S.prof ? wall_clock64() : 0;
     if (rebuilt) { pg_rebuild(c, G); rot_rebuild(c, G, R); }
+    prof_add(S, S.n_germ + S.n_deriv + PROF_REBUILD, t0);
     pg_edges(c, G, op);
     rot_fill(c, G, R);
+    t0 = S.prof ? wall_clock64() : 0;
     int it = rot_solve(c, R);
+    prof_add(S, S.n_germ + S.n_deriv + PROF_BP, t0);
     if (c.tid == 0) {
         int* st = c.I + R.stats;
         st[0] = it;
@@ -884,8 +892,16 @@ __device__ void compute(Ctx& c, const SysDesc& S, int mode) {
         for (int i = c.tid; i < n; i += NT) p[i] = 0.f;
     }
     __syncthreads();
-    for (int k = 0; k < S.n_germ; ++k) run_value(c, S, S.ops[k], mode);
-    for (int k = 0; k < S.n_deriv; ++k) run_deriv(c, S, S.ops[S.n_germ + k]);
+    for (int k = 0; k < S.n_germ; ++k) {
+        unsigned long long t0 = S.prof ? wall_clock64() : 0;
+        run_value(c, S, S.ops[k], mode);
+        prof_add(S, k, t0);
+    }
+    for (int k = 0; k < S.n_deriv; ++k) {
+        unsigned long long t0 = S.prof ? wall_clock64() : 0;
+        run_deriv(c, S, S.ops[S.n_germ + k]);
+        prof_add(S, S.n_germ + k, t0);
+    }
 }
 
 // Threefry-4x32-20 (Salmon et al., SC'11), key (seed, stream, 0, 0) and counter
@@ -975,15 +991,19 @@ __global__ void __launch_bounds__(NT) engine_kernel(SysDesc* systems, RunArgs ar
             ++nf;
             __syncthreads();
         }
+        unsigned long long t0 = S.prof ? wall_clock64() : 0;
         if (args.thermostat_interval > 0 && round % args.thermostat_interval == 0) thermostat(c, S);
+        prof_add(S, S.n_germ + S.n_deriv + PROF_THERMO, t0);
         for (int stage = 0; stage < 3; ++stage) {
             compute(c, S, 0);
+            t0 = S.prof ? wall_clock64() : 0;
             for (int i = c.tid; i < S.n_atom * 3; i += NT) {
                 float p = mom[i] - args.dt * der[i];
                 mom[i] = p;
                 pos[i] += args.dt * p;
             }
             __syncthreads();
+            prof_add(S, S.n_germ + S.n_deriv + PROF_INTEGRATE, t0);
         }
     }
 }

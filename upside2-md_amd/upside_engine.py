@@ -62,6 +62,8 @@ def load_upside_library(path=LIB_PATH):
         "upside_batch_replica_exchange": (i, [vp, i, vp, ct.c_uint64, vp]),
         "upside_batch_bp_stats": (i, [vp, i, vp, vp, vp, vp]),
         "upside_batch_last_run_ms": (f, [vp]),
+        "upside_batch_enable_profile": (i, [vp, i]),
+        "upside_batch_profile_report": (i, [vp, i, ct.c_char_p, i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(c, name)
@@ -234,6 +236,17 @@ class Batch:
 
     def last_run_ms(self):
         return float(self.calc.upside_batch_last_run_ms(self.h))
+
+    def enable_profile(self, on=True):
+        if self.calc.upside_batch_enable_profile(self.h, int(on)):
+            raise RuntimeError("enable_profile failed")
+
+    def profile_report(self, s=0):
+        buf = ct.create_string_buffer(1 << 16)
+        r = self.calc.upside_batch_profile_report(self.h, s, buf, len(buf))
+        if r != 0:
+            raise RuntimeError("profile report failed")
+        return buf.value.decode()
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -271,3 +271,24 @@ def test_error_behaviour(ue, chain_file, updir):
         eng.get_param_deriv((4,), "rotamer")
     e = eng.energy(read_pos(path))
     assert np.isfinite(e)
+
+
+def test_capacity_regrowth_is_transparent(ue, chain_file, monkeypatch):
+    """Neighbour-list capacities that start far too small overflow, regrow and
+    repeat the run: results are bit-identical to a run that never overflowed
+    (slot order is canonical, independent of capacity)."""
+    path = chain_file(100, 61)
+    pos = read_pos(path)
+    say("case: capacity regrowth L=100")
+    ref = ue.Upside(path, quiet=True)
+    e0, d0 = ref.energy(pos), ref.deriv(pos)
+    b0 = ue.Batch([path] * 2, [0.85, 0.9], seed=4)
+    b0.run(5)
+    monkeypatch.setenv("UPSIDE_CAPACITY_SCALE", "0.02")
+    small = ue.Upside(path, quiet=True)
+    assert small.energy(pos) == e0
+    np.testing.assert_array_equal(small.deriv(pos), d0)
+    b1 = ue.Batch([path] * 2, [0.85, 0.9], seed=4)
+    b1.run(5)
+    for s in range(2):
+        np.testing.assert_array_equal(b1.pos(s), b0.pos(s))

@@ -17,6 +17,8 @@ struct Ctx {
     const float* CF;   // constant float arena
     const int* CI;     // constant int arena
     float* red;        // LDS reduction scratch [NWAVE]
+    int* ired;         // LDS reduction scratch [NWAVE] (ints)
+    float* lds;        // LDS staging area [LDS_FLOATS]
     int tid;
 };
 
@@ -54,6 +56,47 @@ __device__ float block_max(Ctx& c, float v) {
 __device__ int block_or(Ctx& c, int v) {
     int any = __syncthreads_or(v);
     return any;
+}
+
+// In-place exclusive scan of the ints a[0..n) (any memory); returns the
+// total.  Each thread scans a contiguous chunk, chunk totals are scanned
+// across the block.  All threads must call it.
+__device__ int block_exscan(Ctx& c, int* a, int n) {
+    int per = (n + NT - 1) / NT;
+    int b = c.tid * per, e = min(b + per, n);
+    int s = 0;
+    for (int i = b; i < e; ++i) s += a[i];
+    int lane = c.tid & 63, w = c.tid >> 6;
+    int v = s;
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    __syncthreads();
+    if (lane == 63) c.ired[w] = v;
+    __syncthreads();
+    int before = 0, tot = 0;
+    for (int k = 0; k < NWAVE; ++k) {
+        int x = c.ired[k];
+        if (k < w) before += x;
+        tot += x;
+    }
+    int run = before + v - s;
+    for (int i = b; i < e; ++i) {
+        int x = a[i];
+        a[i] = run;
+        run += x;
+    }
+    __syncthreads();
+    return tot;
+}
+
+// Sum over aligned groups of G lanes (G a power of two <= 64), fixed order;
+// every lane of the group receives the group total.
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+    for (int o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
 // ---- splines (src/spline.h) -------------------------------------------------

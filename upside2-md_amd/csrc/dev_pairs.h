@@ -1,5 +1,5 @@
-// Pair graphs on the device: Verlet cache, exact edge refine, per-edge spline
-// energies and the fixed-order sensitivity gather.
+// Pair graphs on the device: Verlet cache from a hashed cell grid, exact edge
+// refine, per-edge spline energies and the fixed-order sensitivity gather.
 //
 // Reference: PairlistComputation / InteractionGraph (src/interaction_graph.h).
 // The edge set of a graph is {(i1,i2) acceptable : dist2 < cutoff^2} where
@@ -7,7 +7,19 @@
 // reference build evaluates it, fma(x,x, fma(z,z, y*y)) (the Float4 sum of
 // src/interaction_graph.h:239 contracted by the compiler).  The cache (radius
 // cutoff + buffer, rebuilt when any element drifts more than buffer/2,
-// src/interaction_graph.h:51-169) only bounds which pairs are examined.
+// src/interaction_graph.h:51-169) only bounds which pairs are examined, so the
+// edge set does not depend on how the cache was built.
+//
+// Rebuild (all in one workgroup):
+//   1. hash the side-2 elements into cells of edge length >= cache radius
+//      (counting sort into buckets);
+//   2. every row scans the 27 neighbouring cells twice: once to count its
+//      cache partners, once (after a block scan of the counts) to write them;
+//   3. each row is put in increasing partner order by a wave-cooperative rank
+//      sort, so the slot order is canonical whatever order the grid produced;
+//   4. the transposed rows are built the same way from the other side
+//      (symmetric graphs: the lower partners of the same scan) and each entry
+//      is mapped to its forward slot by binary search in the sorted row.
 #pragma once
 #include "dev_common.h"
 
@@ -23,8 +35,8 @@ __device__ __forceinline__ bool pg_accept(const PairGraph& G, int a, int b) {
     return d > G.excl || -d > G.excl;
 }
 
-__device__ __forceinline__ const float* pg_p2(const Ctx& c, const PairGraph& G, int j) {
-    return c.A + G.pos_pack + (G.sym ? j : G.n1 + j) * 3;
+__device__ __forceinline__ const float* pg_side2(const Ctx& c, const PairGraph& G) {
+    return c.A + G.pos_pack + (G.sym ? 0 : G.n1 * 3);
 }
 
 // Copy the current coordinates of both sides into pos_pack and report whether
@@ -44,7 +56,7 @@ __device__ bool pg_pack_and_check(Ctx& c, const PairGraph& G) {
         float x = src[0], y = src[1], z = src[2];
         pack[i * 3 + 0] = x; pack[i * 3 + 1] = y; pack[i * 3 + 2] = z;
         float dx = x - cache[i * 3 + 0], dy = y - cache[i * 3 + 1], dz = z - cache[i * 3 + 2];
-        if (dx * dx + dy * dy + dz * dz > G.drift2) moved = 1;
+        if (!(dx * dx + dy * dy + dz * dz <= G.drift2)) moved = 1;  // NaN also forces a rebuild
     }
     int any = __syncthreads_or(moved | (valid ? 0 : 1));
     return any != 0;
@@ -59,61 +71,216 @@ __device__ int lower_bound_i(const int* a, int n, int v) {
     return lo;
 }
 
-// Rebuild the candidate rows and the transposed lists from pos_pack.
+// ---- hashed cell grid ---------------------------------------------------------
+struct Grid {
+    float ox, oy, oz, inv;
+    int mask;
+};
+
+__device__ __forceinline__ int cell_coord(float x, float o, float inv) {
+    // clamp keeps far outliers (and NaN) inside int range; they only cost scan time
+    float t = fminf(fmaxf((x - o) * inv, -1.0e6f), 1.0e6f);
+    return (int)floorf(t);
+}
+__device__ __forceinline__ int cell_bucket(const Grid& g, int cx, int cy, int cz) {
+    uint32_t h = ((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u);
+    return (int)(h & (uint32_t)g.mask);
+}
+
+// Bucket the m points pts[3*k] (counting sort): elements of bucket b are
+// gidx[gofs[b] .. gofs[b+1]).  All threads.
+__device__ Grid grid_build(Ctx& c, const PairGraph& G, const float* pts, int m) {
+    float mn0 = 3.0e38f, mn1 = 3.0e38f, mn2 = 3.0e38f;
+    for (int i = c.tid; i < m; i += NT) {
+        mn0 = fminf(mn0, pts[i * 3]);
+        mn1 = fminf(mn1, pts[i * 3 + 1]);
+        mn2 = fminf(mn2, pts[i * 3 + 2]);
+    }
+    Grid g;
+    g.ox = -block_max(c, -mn0);
+    g.oy = -block_max(c, -mn1);
+    g.oz = -block_max(c, -mn2);
+    // cells slightly larger than the cache radius: rounding can never put two
+    // points closer than the radius more than one cell apart
+    g.inv = 1.f / (G.cache_cut * 1.0001f);
+    g.mask = G.nbucket - 1;
+    int* gofs = c.I + G.gofs;
+    int* gidx = c.I + G.gidx;
+    for (int b = c.tid; b <= G.nbucket; b += NT) gofs[b] = 0;
+    __syncthreads();
+    for (int i = c.tid; i < m; i += NT) {
+        int b = cell_bucket(g, cell_coord(pts[i * 3], g.ox, g.inv), cell_coord(pts[i * 3 + 1], g.oy, g.inv),
+                            cell_coord(pts[i * 3 + 2], g.oz, g.inv));
+        atomicAdd(&gofs[b + 1], 1);
+    }
+    __syncthreads();
+    block_exscan(c, gofs + 1, G.nbucket);  // gofs[b+1] = start of bucket b; gofs[0] stays 0
+    for (int i = c.tid; i < m; i += NT) {
+        int b = cell_bucket(g, cell_coord(pts[i * 3], g.ox, g.inv), cell_coord(pts[i * 3 + 1], g.oy, g.inv),
+                            cell_coord(pts[i * 3 + 2], g.oz, g.inv));
+        int p = atomicAdd(&gofs[b + 1], 1);  // ends at the start of bucket b+1
+        gidx[p] = i;
+    }
+    __syncthreads();
+    return g;
+}
+
+// Visit every point of the grid whose cell is adjacent to (x,y,z)'s cell:
+// f(j, yx, yy, yz).  Each point is visited once (hash collisions are
+// filtered by recomputing the point's own cell).
+template <class F>
+__device__ __forceinline__ void grid_visit(const Ctx& c, const PairGraph& G, const Grid& g, const float* pts,
+                                           float x, float y, float z, F&& f) {
+    const int* gofs = c.I + G.gofs;
+    const int* gidx = c.I + G.gidx;
+    int cx = cell_coord(x, g.ox, g.inv), cy = cell_coord(y, g.oy, g.inv), cz = cell_coord(z, g.oz, g.inv);
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                int ax = cx + dx, ay = cy + dy, az = cz + dz;
+                int b = cell_bucket(g, ax, ay, az);
+                int e1 = gofs[b + 1];
+                for (int t = gofs[b]; t < e1; ++t) {
+                    int j = gidx[t];
+                    float px = pts[j * 3], py = pts[j * 3 + 1], pz = pts[j * 3 + 2];
+                    if (cell_coord(px, g.ox, g.inv) != ax || cell_coord(py, g.oy, g.inv) != ay ||
+                        cell_coord(pz, g.oz, g.inv) != az)
+                        continue;
+                    f(j, px, py, pz);
+                }
+            }
+}
+
+// Sort every segment [off[r], off[r+1]) of src into dst (distinct values per
+// segment) by rank: one wave per segment.
+__device__ void sort_segments(Ctx& c, const int* src, int* dst, const int* off, int n_rows) {
+    int wave = c.tid >> 6, lane = c.tid & 63;
+    for (int r = wave; r < n_rows; r += NWAVE) {
+        int b = off[r], e = off[r + 1];
+        for (int k = b + lane; k < e; k += 64) {
+            int v = src[k];
+            int rank = 0;
+            for (int t = b; t < e; ++t) rank += src[t] < v;
+            dst[b + rank] = v;
+        }
+    }
+    __syncthreads();
+}
+
+// Rebuild rows and transposed rows from pos_pack.  Sets state[1] on overflow
+// (the host grows the capacity and repeats the run).
 __device__ void pg_rebuild(Ctx& c, const PairGraph& G) {
     const int* id1 = c.CI + G.id1;
     const int* id2 = c.CI + G.id2;
     float* pack = c.A + G.pos_pack;
     float* cache = c.A + G.cache_pos;
-    int* cnt = c.I + G.cnt;
+    const float* p1 = pack;
+    const float* p2 = pg_side2(c, G);
+    int* off = c.I + G.off;
     int* nbr = c.I + G.nbr;
-    int* tcnt = c.I + G.tcnt;
+    int* row = c.I + G.row;
+    int* toff = c.I + G.toff;
     int* tslot = c.I + G.tslot;
-    int n = G.sym ? G.n1 : G.n1 + G.n2;
-    for (int i = c.tid; i < n * 3; i += NT) cache[i] = pack[i];
-    int overflow = 0;
-    for (int i1 = c.tid; i1 < G.n1; i1 += NT) {
-        const float* x = pack + i1 * 3;
-        float x0 = x[0], x1 = x[1], x2 = x[2];
-        int a = id1[i1];
-        int k = 0;
-        for (int j = G.sym ? i1 + 1 : 0; j < G.n2; ++j) {
-            const float* y = pg_p2(c, G, j);
-            float dx = x0 - y[0], dy = x1 - y[1], dz = x2 - y[2];
-            if (refine_dist2(dx, dy, dz) < G.cache_cut2 && pg_accept(G, a, id2[j])) {
-                if (k < G.maxn) nbr[i1 * G.maxn + k] = j;
-                ++k;
+    int* tmp = c.I + G.tmp;
+    int npack = G.sym ? G.n1 : G.n1 + G.n2;
+    for (int i = c.tid; i < npack * 3; i += NT) cache[i] = pack[i];
+    const float cc2 = G.cache_cut2;
+    const bool sym = G.sym != 0;
+
+    // forward rows (and, for symmetric graphs, the lower partners)
+    Grid g = grid_build(c, G, p2, G.n2);
+    for (int i = c.tid; i < G.n1; i += NT) {
+        float x0 = p1[i * 3], x1 = p1[i * 3 + 1], x2 = p1[i * 3 + 2];
+        int a = id1[i];
+        int up = 0, lo = 0;
+        grid_visit(c, G, g, p2, x0, x1, x2, [&](int j, float y0, float y1, float y2) {
+            if (sym && j == i) return;
+            if (refine_dist2(x0 - y0, x1 - y1, x2 - y2) < cc2 && pg_accept(G, a, id2[j])) {
+                if (!sym || j > i) ++up; else ++lo;
             }
-        }
-        if (k > G.maxn) { overflow = 1; k = G.maxn; }
-        cnt[i1] = k;
+        });
+        off[i] = up;
+        if (sym) toff[i] = lo;
+    }
+    if (c.tid == 0) {
+        off[G.n1] = 0;
+        if (sym) toff[G.n1] = 0;
     }
     __syncthreads();
-    for (int j = c.tid; j < G.n2; j += NT) {
-        const float* y = pg_p2(c, G, j);
-        float y0 = y[0], y1 = y[1], y2 = y[2];
-        int b = id2[j];
-        int t = 0;
-        int lim = G.sym ? j : G.n1;
-        for (int i1 = 0; i1 < lim; ++i1) {
-            const float* x = pack + i1 * 3;
-            float dx = x[0] - y0, dy = x[1] - y1, dz = x[2] - y2;
-            if (refine_dist2(dx, dy, dz) < G.cache_cut2 && pg_accept(G, id1[i1], b)) {
-                int k = lower_bound_i(nbr + i1 * G.maxn, cnt[i1], j);
-                if (k < cnt[i1] && nbr[i1 * G.maxn + k] == j) {
-                    if (t < G.maxt) tslot[j * G.maxt + t] = i1 * G.maxn + k;
-                    ++t;
-                }
-            }
-        }
-        if (t > G.maxt) { overflow = 1; t = G.maxt; }
-        tcnt[j] = t;
+    int E = block_exscan(c, off, G.n1 + 1);
+    if (sym) block_exscan(c, toff, G.n1 + 1);
+    if (E > G.ecap) {
+        if (c.tid == 0) { c.I[G.state + 1] = 1; c.I[G.state + 0] = 0; }
+        __syncthreads();
+        return;
     }
-    int of = __syncthreads_or(overflow);
+    for (int i = c.tid; i < G.n1; i += NT) {
+        float x0 = p1[i * 3], x1 = p1[i * 3 + 1], x2 = p1[i * 3 + 2];
+        int a = id1[i];
+        int up = off[i], lo = sym ? toff[i] : 0;
+        grid_visit(c, G, g, p2, x0, x1, x2, [&](int j, float y0, float y1, float y2) {
+            if (sym && j == i) return;
+            if (refine_dist2(x0 - y0, x1 - y1, x2 - y2) < cc2 && pg_accept(G, a, id2[j])) {
+                if (!sym || j > i) tmp[up++] = j; else tslot[lo++] = j;
+            }
+        });
+    }
+    __syncthreads();
+    sort_segments(c, tmp, nbr, off, G.n1);
+    for (int i = c.tid; i < G.n1; i += NT)
+        for (int s = off[i]; s < off[i + 1]; ++s) row[s] = i;
+
+    // transposed rows: partners on side 1 of every side-2 element, sorted
+    if (sym) {
+        sort_segments(c, tslot, tmp, toff, G.n1);
+    } else {
+        Grid h = grid_build(c, G, p1, G.n1);
+        for (int j = c.tid; j < G.n2; j += NT) {
+            float y0 = p2[j * 3], y1 = p2[j * 3 + 1], y2 = p2[j * 3 + 2];
+            int b = id2[j];
+            int n = 0;
+            grid_visit(c, G, h, p1, y0, y1, y2, [&](int i, float x0, float x1, float x2) {
+                if (refine_dist2(x0 - y0, x1 - y1, x2 - y2) < cc2 && pg_accept(G, id1[i], b)) ++n;
+            });
+            toff[j] = n;
+        }
+        if (c.tid == 0) toff[G.n2] = 0;
+        __syncthreads();
+        int Et = block_exscan(c, toff, G.n2 + 1);
+        if (Et != E) {  // both scans apply the same test to the same pairs
+            if (c.tid == 0) { c.I[G.state + 1] = 2; c.I[G.state + 0] = 0; }
+            __syncthreads();
+            return;
+        }
+        for (int j = c.tid; j < G.n2; j += NT) {
+            float y0 = p2[j * 3], y1 = p2[j * 3 + 1], y2 = p2[j * 3 + 2];
+            int b = id2[j];
+            int n = toff[j];
+            grid_visit(c, G, h, p1, y0, y1, y2, [&](int i, float x0, float x1, float x2) {
+                if (refine_dist2(x0 - y0, x1 - y1, x2 - y2) < cc2 && pg_accept(G, id1[i], b)) tslot[n++] = i;
+            });
+        }
+        __syncthreads();
+        sort_segments(c, tslot, tmp, toff, G.n2);
+    }
+    // tmp holds the sorted side-1 partners of each transposed row: map them
+    // to forward slots
+    int nt = sym ? G.n1 : G.n2;
+    int bad = 0;
+    for (int j = c.tid; j < nt; j += NT)
+        for (int t = toff[j]; t < toff[j + 1]; ++t) {
+            int i = tmp[t];
+            int b0 = off[i], n = off[i + 1] - b0;
+            int k = lower_bound_i(nbr + b0, n, j);
+            if (k >= n || nbr[b0 + k] != j) { bad = 1; k = 0; }
+            tslot[t] = b0 + k;
+        }
+    int anybad = __syncthreads_or(bad);
     if (c.tid == 0) {
-        c.I[G.state + 0] = 1;
-        if (of) c.I[G.state + 1] = 1;
+        c.I[G.state + 0] = anybad ? 0 : 1;
+        if (anybad) c.I[G.state + 1] = 2;
         c.I[G.state + 2] += 1;
+        c.I[G.state + 3] = E;
     }
     __syncthreads();
 }
@@ -179,6 +346,7 @@ __device__ __forceinline__ float protein_hbond_edge(const float* p, const float*
         hb = rv * a1 * a2;
         float c0 = rd * a1 * a2, c1 = rv * da1 * a2, c2 = -rv * a1 * da2;
         float r[3] = {rx, ry, rz};
+#pragma unroll
         for (int k = 0; k < 3; ++k) {
             drOC[k] = c1 * r[k];
             drHN[k] = c2 * r[k];
@@ -187,6 +355,7 @@ __device__ __forceinline__ float protein_hbond_edge(const float* p, const float*
     }
     float hb_log = (1.f <= hb) ? 100.f : -logf(1.f - hb);
     float pref = fminf(1.f / (1.f - hb), 1e5f);
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
         d1[k] = dH[k] * pref; d1[3 + k] = drHN[k] * pref;
         d2[k] = -dH[k] * pref; d2[3 + k] = drOC[k] * pref;
@@ -209,6 +378,7 @@ __device__ __forceinline__ float env_edge(const float* p, const float* x1, const
     compact_sigmoid(p[2] - dp, p[3], as, das);
     float prob = WEIGHTED ? x2[3] : 1.f;
     float u[3] = {ux, uy, uz};
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
         float ddk = prob * (drs * as * u[k] - rs * das * inv * (x1[3 + k] - dp * u[k]));
         d1[k] = -ddk;
@@ -225,8 +395,8 @@ __device__ __forceinline__ float env_edge(const float* p, const float* x1, const
 }
 
 // BackbonePairs residue pair (src/backbone_steric.cpp:94-140): compact-sigmoid
-// repulsion between the placed backbone atoms of two residues; derivatives are
-// (force, torque) on each residue frame.
+// repulsion between the placed backbone atoms (at most 4) of two residues;
+// derivatives are (force, torque) on each residue frame.
 __device__ float backbone_edge(const Ctx& c, const OpRec& op, int r1, int r2, float* d1, float* d2) {
     const float* aff = c.A + op.in_out[0];
     int w = op.in_w[0];
@@ -241,20 +411,26 @@ __device__ float backbone_edge(const Ctx& c, const OpRec& op, int r1, int r2, fl
     quat_to_rot(a2 + 3, U2);
     float X1[4][3], X2[4][3];
     int n1 = nat[r1], n2 = nat[r2];
-    for (int a = 0; a < n1; ++a) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
         const float* q = ref + (r1 * 4 + a) * 3;
-        for (int k = 0; k < 3; ++k) X1[a][k] = U1[3 * k] * q[0] + U1[3 * k + 1] * q[1] + U1[3 * k + 2] * q[2] + a1[k];
-    }
-    for (int a = 0; a < n2; ++a) {
-        const float* q = ref + (r2 * 4 + a) * 3;
-        for (int k = 0; k < 3; ++k) X2[a][k] = U2[3 * k] * q[0] + U2[3 * k + 1] * q[1] + U2[3 * k + 2] * q[2] + a2[k];
+        const float* s = ref + (r2 * 4 + a) * 3;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            X1[a][k] = U1[3 * k] * q[0] + U1[3 * k + 1] * q[1] + U1[3 * k + 2] * q[2] + a1[k];
+            X2[a][k] = U2[3 * k] * s[0] + U2[3 * k + 1] * s[1] + U2[3 * k + 2] * s[2] + a2[k];
+        }
     }
     const float cut2 = 3.f * 3.f + 0.1f * 3.f;
     const float sharp = 1.f / (3.0f * 0.10f);
     float pot = 0.f;
+#pragma unroll
     for (int k = 0; k < 6; ++k) { d1[k] = 0.f; d2[k] = 0.f; }
-    for (int i = 0; i < n1; ++i) {
-        for (int j = 0; j < n2; ++j) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (i >= n1 || j >= n2) continue;
             float rx = X1[i][0] - X2[j][0], ry = X1[i][1] - X2[j][1], rz = X1[i][2] - X2[j][2];
             float rm2 = rx * rx + ry * ry + rz * rz;
             if (rm2 > cut2) continue;
@@ -274,77 +450,96 @@ __device__ float backbone_edge(const Ctx& c, const OpRec& op, int r1, int r2, fl
     return pot;
 }
 
-// Refine every cached slot against the cutoff and evaluate active edges.
-__device__ void pg_edges(Ctx& c, const PairGraph& G, const OpRec& op) {
-    const int* cnt = c.I + G.cnt;
+// Stage the graph's coefficient table in LDS (when it has one that fits).
+__device__ const float* pg_stage_params(Ctx& c, const PairGraph& G) {
+    const float* src = c.CF + G.param;
+    if (G.param_len <= 0) return src;
+    for (int k = c.tid; k < G.param_len; k += NT) c.lds[k] = src[k];
+    __syncthreads();
+    return c.lds;
+}
+
+// Refine every cached slot against the cutoff and evaluate the active edges.
+template <int KIND, int DD1, int DD2>
+__device__ void pg_edges_t(Ctx& c, const PairGraph& G, const OpRec& op) {
+    const float* param = pg_stage_params(c, G);
     const int* nbr = c.I + G.nbr;
+    const int* row = c.I + G.row;
     int* act = c.I + G.act;
     float* val = c.A + G.val;
     float* D1 = c.A + G.d1;
     float* D2 = c.A + G.d2;
     const float* pack = c.A + G.pos_pack;
+    const float* pk2 = pg_side2(c, G);
     const int* loc1 = c.CI + G.loc1;
     const int* loc2 = c.CI + G.loc2;
     const int* t1 = c.CI + G.type1;
     const int* t2 = c.CI + G.type2;
     const float* P1 = c.A + G.out1;
     const float* P2 = c.A + G.out2;
-    const float* param = c.CF + G.param;
-    int wave = c.tid >> 6, lane = c.tid & 63;
-    for (int i1 = wave; i1 < G.n1; i1 += NWAVE) {
-        int n = cnt[i1];
+    const int E = c.I[G.state + 3];
+    const int cap = G.ecap;
+    for (int s = c.tid; s < E; s += NT) {
+        int i1 = row[s], j = nbr[s];
         const float* xp = pack + i1 * 3;
+        const float* yp = pk2 + j * 3;
+        float d2 = refine_dist2(xp[0] - yp[0], xp[1] - yp[1], xp[2] - yp[2]);
+        int a = d2 < G.cutoff2;
+        act[s] = a;
+        if (!a) { val[s] = 0.f; continue; }
         const float* x1 = P1 + loc1[i1] * G.w1;
-        for (int k = lane; k < n; k += 64) {
-            int slot = i1 * G.maxn + k;
-            int j = nbr[slot];
-            const float* yp = pg_p2(c, G, j);
-            float d2 = refine_dist2(xp[0] - yp[0], xp[1] - yp[1], xp[2] - yp[2]);
-            int a = d2 < G.cutoff2;
-            act[slot] = a;
-            if (!a) { val[slot] = 0.f; continue; }
-            const float* x2 = P2 + loc2[j] * G.w2;
-            const float* p = param + (t1[i1] * G.n_type2 + t2[j]) * G.n_param;
-            float e1[7], e2[7];
-            float v = 0.f;
-            switch (G.kind) {
-            case EDGE_ROTAMER:
-                v = quadspline<15, 12>(p, x1, x2, 6.f, 1.f / 0.7f, e1, e2);
-                break;
-            case EDGE_HBOND_COVERAGE: {
-                float cov = quadspline<15, 10>(p, x1, x2, 6.f, 1.f / 0.625f, e1, e2);
-                float om = 1.f - x1[6];
-                float pref = om * om;
-                for (int q = 0; q < 6; ++q) { e1[q] *= pref; e2[q] *= pref; }
-                e1[6] = -cov * om * 2.f;
-                v = pref * cov;
-            } break;
-            case EDGE_PROTEIN_HBOND:
-                v = protein_hbond_edge(p, x1, x2, e1, e2);
-                break;
-            case EDGE_ENV_WEIGHTED:
-                v = env_edge<true>(p, x1, x2, e1, e2);
-                break;
-            case EDGE_ENV_HBOND:
-                v = env_edge<false>(p, x1, x2, e1, e2);
-                break;
-            case EDGE_BACKBONE:
-                v = backbone_edge(c, op, i1, j, e1, e2);
-                break;
-            }
-            val[slot] = v;
-            for (int q = 0; q < G.ddim1; ++q) D1[(size_t)slot * G.ddim1 + q] = e1[q];
-            for (int q = 0; q < G.ddim2; ++q) D2[(size_t)slot * G.ddim2 + q] = e2[q];
+        const float* x2 = P2 + loc2[j] * G.w2;
+        const float* p = param + (t1[i1] * G.n_type2 + t2[j]) * G.n_param;
+        float e1[7], e2[7];
+        float v = 0.f;
+        if (KIND == EDGE_ROTAMER) {
+            v = quadspline<15, 12>(p, x1, x2, 6.f, 1.f / 0.7f, e1, e2);
+        } else if (KIND == EDGE_HBOND_COVERAGE) {
+            float cov = quadspline<15, 10>(p, x1, x2, 6.f, 1.f / 0.625f, e1, e2);
+            float om = 1.f - x1[6];
+            float pref = om * om;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) { e1[q] *= pref; e2[q] *= pref; }
+            e1[6] = -cov * om * 2.f;
+            v = pref * cov;
+        } else if (KIND == EDGE_PROTEIN_HBOND) {
+            v = protein_hbond_edge(p, x1, x2, e1, e2);
+        } else if (KIND == EDGE_ENV_WEIGHTED) {
+            v = env_edge<true>(p, x1, x2, e1, e2);
+        } else if (KIND == EDGE_ENV_HBOND) {
+            v = env_edge<false>(p, x1, x2, e1, e2);
+        } else {
+            v = backbone_edge(c, op, i1, j, e1, e2);
         }
+        val[s] = v;
+#pragma unroll
+        for (int q = 0; q < DD1; ++q) D1[q * cap + s] = e1[q];
+#pragma unroll
+        for (int q = 0; q < DD2; ++q) D2[q * cap + s] = e2[q];
     }
     __syncthreads();
 }
 
-// Gather esens*deriv into the producers in a fixed order.  `col0` offsets the
-// target columns (backbone pairs write into the 6 leading affine columns).
-__device__ void pg_gather(Ctx& c, const PairGraph& G) {
-    const int* cnt = c.I + G.cnt;
-    const int* tcnt = c.I + G.tcnt;
+__device__ void pg_edges(Ctx& c, const PairGraph& G, const OpRec& op) {
+    switch (G.kind) {
+    case EDGE_ROTAMER: pg_edges_t<EDGE_ROTAMER, 6, 6>(c, G, op); break;
+    case EDGE_HBOND_COVERAGE: pg_edges_t<EDGE_HBOND_COVERAGE, 7, 6>(c, G, op); break;
+    case EDGE_PROTEIN_HBOND: pg_edges_t<EDGE_PROTEIN_HBOND, 6, 6>(c, G, op); break;
+    case EDGE_ENV_WEIGHTED: pg_edges_t<EDGE_ENV_WEIGHTED, 6, 4>(c, G, op); break;
+    case EDGE_ENV_HBOND: pg_edges_t<EDGE_ENV_HBOND, 6, 6>(c, G, op); break;
+    default: pg_edges_t<EDGE_BACKBONE, 6, 6>(c, G, op); break;
+    }
+}
+
+// Gather esens*deriv into the producers in a fixed order: groups of 16 lanes
+// per element, each lane striding the element's slots, then a fixed-order
+// group reduction.
+constexpr int GW = 16;
+
+template <int D1N, int D2N>
+__device__ void pg_gather_t(Ctx& c, const PairGraph& G) {
+    const int* off = c.I + G.off;
+    const int* toff = c.I + G.toff;
     const int* tslot = c.I + G.tslot;
     const int* act = c.I + G.act;
     const float* es = c.A + G.esens;
@@ -354,48 +549,106 @@ __device__ void pg_gather(Ctx& c, const PairGraph& G) {
     const int* loc2 = c.CI + G.loc2;
     float* S1 = c.A + G.sens1;
     float* S2 = c.A + G.sens2;
-    for (int i = c.tid; i < G.n1; i += NT) {
-        float acc[7] = {0, 0, 0, 0, 0, 0, 0};
-        int n = cnt[i];
-        for (int k = 0; k < n; ++k) {
-            int slot = i * G.maxn + k;
-            if (!act[slot]) continue;
-            float s = es[slot];
-            for (int q = 0; q < G.ddim1; ++q) acc[q] += s * D1[(size_t)slot * G.ddim1 + q];
+    const int cap = G.ecap;
+    int grp = c.tid / GW, gl = c.tid % GW;
+    const int NG = NT / GW;
+    // uniform trip counts: every lane runs every group reduction
+    for (int base = 0; base < G.n1; base += NG) {
+        int i = base + grp;
+        bool ok = i < G.n1;
+        float acc[D1N];
+#pragma unroll
+        for (int q = 0; q < D1N; ++q) acc[q] = 0.f;
+        int b = ok ? off[i] : 0, e = ok ? off[i + 1] : 0;
+        for (int s = b + gl; s < e; s += GW) {
+            if (!act[s]) continue;
+            float w = es[s];
+#pragma unroll
+            for (int q = 0; q < D1N; ++q) acc[q] += w * D1[q * cap + s];
         }
-        if (G.sym) {
-            int m = tcnt[i];
-            for (int t = 0; t < m; ++t) {
-                int slot = tslot[i * G.maxt + t];
-                if (!act[slot]) continue;
-                float s = es[slot];
-                for (int q = 0; q < G.ddim2; ++q) acc[q] += s * D2[(size_t)slot * G.ddim2 + q];
+        if (G.sym && ok) {
+            int tb = toff[i], te = toff[i + 1];
+            for (int t = tb + gl; t < te; t += GW) {
+                int s = tslot[t];
+                if (!act[s]) continue;
+                float w = es[s];
+#pragma unroll
+                for (int q = 0; q < D1N; ++q) acc[q] += w * D2[q * cap + s];
             }
         }
-        float* dst = S1 + loc1[i] * G.w1;
-        for (int q = 0; q < G.ddim1; ++q) dst[q] += acc[q];
+#pragma unroll
+        for (int q = 0; q < D1N; ++q) acc[q] = group_sum<GW>(acc[q]);
+        if (ok && gl == 0) {
+            float* dst = S1 + loc1[i] * G.w1;
+#pragma unroll
+            for (int q = 0; q < D1N; ++q) dst[q] += acc[q];
+        }
     }
     __syncthreads();
     if (G.sym) return;
-    for (int j = c.tid; j < G.n2; j += NT) {
-        float acc[7] = {0, 0, 0, 0, 0, 0, 0};
-        int m = tcnt[j];
-        for (int t = 0; t < m; ++t) {
-            int slot = tslot[j * G.maxt + t];
-            if (!act[slot]) continue;
-            float s = es[slot];
-            for (int q = 0; q < G.ddim2; ++q) acc[q] += s * D2[(size_t)slot * G.ddim2 + q];
+    for (int base = 0; base < G.n2; base += NG) {
+        int j = base + grp;
+        bool ok = j < G.n2;
+        float acc[D2N];
+#pragma unroll
+        for (int q = 0; q < D2N; ++q) acc[q] = 0.f;
+        int tb = ok ? toff[j] : 0, te = ok ? toff[j + 1] : 0;
+        for (int t = tb + gl; t < te; t += GW) {
+            int s = tslot[t];
+            if (!act[s]) continue;
+            float w = es[s];
+#pragma unroll
+            for (int q = 0; q < D2N; ++q) acc[q] += w * D2[q * cap + s];
         }
-        float* dst = S2 + loc2[j] * G.w2;
-        for (int q = 0; q < G.ddim2; ++q) dst[q] += acc[q];
+#pragma unroll
+        for (int q = 0; q < D2N; ++q) acc[q] = group_sum<GW>(acc[q]);
+        if (ok && gl == 0) {
+            float* dst = S2 + loc2[j] * G.w2;
+#pragma unroll
+            for (int q = 0; q < D2N; ++q) dst[q] += acc[q];
+        }
     }
     __syncthreads();
 }
 
+__device__ void pg_gather(Ctx& c, const PairGraph& G) {
+    switch (G.kind) {
+    case EDGE_HBOND_COVERAGE: pg_gather_t<7, 6>(c, G); break;
+    case EDGE_ENV_WEIGHTED: pg_gather_t<6, 4>(c, G); break;
+    default: pg_gather_t<6, 6>(c, G); break;  // symmetric graphs: both sides 6 wide
+    }
+}
+
+// Sum of the active edge values of each row (side 1) or transposed row
+// (side 2), in slot order: out[k*stride] = sum.  Groups of 16 lanes.
+__device__ void pg_row_sums(Ctx& c, const PairGraph& G, bool side2, float* out, int stride) {
+    const int* off = c.I + (side2 ? G.toff : G.off);
+    const int* tslot = c.I + G.tslot;
+    const int* act = c.I + G.act;
+    const float* val = c.A + G.val;
+    int n = side2 ? G.n2 : G.n1;
+    int grp = c.tid / GW, gl = c.tid % GW;
+    for (int base = 0; base < n; base += NT / GW) {
+        int r = base + grp;
+        bool ok = r < n;
+        float acc = 0.f;
+        int k0 = ok ? off[r] : 0, k1 = ok ? off[r + 1] : 0;
+        for (int k = k0 + gl; k < k1; k += GW) {
+            int s = side2 ? tslot[k] : k;
+            if (act[s]) acc += val[s];
+        }
+        acc = group_sum<GW>(acc);
+        if (ok && gl == 0) out[r * stride] = acc;
+    }
+}
+
 // Prepare a graph for an evaluation: pack, maybe rebuild, refine and evaluate.
-__device__ void pg_refresh(Ctx& c, const PairGraph& G, const OpRec& op) {
+// Returns false when the slot capacity overflowed (the evaluation is void).
+__device__ bool pg_refresh(Ctx& c, const PairGraph& G, const OpRec& op) {
     if (pg_pack_and_check(c, G)) pg_rebuild(c, G);
+    if (!c.I[G.state + 0]) return false;
     pg_edges(c, G, op);
+    return true;
 }
 
 }  // namespace upside

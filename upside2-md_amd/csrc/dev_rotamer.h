@@ -1,12 +1,20 @@
 // Rotamer side-chain free energy by loopy belief propagation on the device
 // (src/rotamer.cpp:628-1169), one workgroup per system.
 //
-// Residue nodes carry 1, 3 or 6 rotamer states.  Residue pairs are candidate
-// "edges" kept per cache rebuild (any bead pair within the cache radius); an
-// edge is present in an evaluation when at least one of its bead pairs is in
-// the refined edge set, which is the reference's EdgeLocator edge set
-// (src/rotamer.cpp:134-206,383-396).  Messages are updated synchronously
-// (Jacobi) per edge, node beliefs are products over incident edges, with the
+// Residue nodes carry 1, 3 or 6 rotamer states, one bead per state.  Residue
+// pairs with any bead pair in the rotamer pair graph's cache are "candidates"
+// (kept per cache rebuild, in (a,b) order).  In an evaluation a candidate is
+// present when one of its bead pairs is in the refined edge set, which is the
+// reference's EdgeLocator edge set (src/rotamer.cpp:134-206,383-396):
+//   * 1-rotamer x 1-rotamer pairs contribute -log P to the free energy;
+//   * 1-rotamer x n pairs are folded into the n-rotamer node's probabilities
+//     (move_edge_prob_to_node2, :398-405) -- here as the bead sum of their
+//     edge energies, exp(-sum) being the product of the edge probabilities;
+//   * 3/6 x 3/6 pairs become the BP edges, compacted per evaluation into
+//     dense structure-of-arrays (matrix P, messages, marginals) so that one
+//     thread updates one edge with coalesced loads.
+// Messages are updated synchronously (Jacobi) per edge, node beliefs are
+// products over incident edges renormalised after every factor, with the
 // reference's schedule: an undamped first pass whose node beliefs are swapped
 // out, then damped passes in chunks of `iteration_chunk_size`, stopping when
 // the one-sided max change is below tol (src/rotamer.cpp:1089-1162).
@@ -16,182 +24,266 @@
 namespace upside {
 
 struct RotView {
-    const int* nrot; const int* bead0; const int* brot;
-    int* ccnt; int* cnbr; int* icnt; int* iinc; int* cpres;
-    float* cP; float* cmsg; float* cmarg; float* nprob; float* ncur; float* nold; float* noff;
-    int maxc, maxi;
+    const int *nrot, *bead0, *brot, *bnode;
+    int *coff, *ca, *cb, *scand, *ioff, *iinc, *eidx, *bpc, *bpf, *bps, *ninc, *binc, *stats;
+    float *P, *msg, *marg, *e1x, *nprob, *ncur, *nold, *noff;
+    int cap;
 };
 
 __device__ RotView rot_view(Ctx& c, const RotamerDev& R) {
     RotView v;
     v.nrot = c.CI + R.node_nrot; v.bead0 = c.CI + R.node_bead0; v.brot = c.CI + R.bead_rot;
-    v.ccnt = c.I + R.ccnt; v.cnbr = c.I + R.cnbr; v.icnt = c.I + R.icnt; v.iinc = c.I + R.iinc;
-    v.cpres = c.I + R.cpres;
-    v.cP = c.A + R.cP; v.cmsg = c.A + R.cmsg; v.cmarg = c.A + R.cmarg;
+    v.bnode = c.CI + R.bead_node;
+    v.coff = c.I + R.coff; v.ca = c.I + R.ca; v.cb = c.I + R.cb; v.scand = c.I + R.slot_cand;
+    v.ioff = c.I + R.ioff; v.iinc = c.I + R.iinc; v.eidx = c.I + R.eidx; v.bpc = c.I + R.bpc;
+    v.bpf = c.I + R.bpf; v.bps = c.I + R.bps; v.ninc = c.I + R.ninc; v.binc = c.I + R.binc;
+    v.stats = c.I + R.stats;
+    v.P = c.A + R.P; v.msg = c.A + R.msg; v.marg = c.A + R.marg; v.e1x = c.A + R.e1x;
     v.nprob = c.A + R.nprob; v.ncur = c.A + R.ncur; v.nold = c.A + R.nold; v.noff = c.A + R.noff;
-    v.maxc = R.maxc; v.maxi = R.maxi;
+    v.cap = R.ccap;
     return v;
 }
 
-// Node-pair candidates from the bead cache positions (called after a bead
-// graph rebuild).  Row A lists partner nodes B>A in increasing order; the
-// incidence list of node X names (candidate, side) for A<X first, then its own.
-__device__ void rot_rebuild(Ctx& c, const PairGraph& G, const RotamerDev& R) {
-    RotView v = rot_view(c, R);
-    const float* cache = c.A + G.cache_pos;
-    int overflow = 0;
-    for (int a = c.tid; a < R.n_node; a += NT) {
-        int k = 0;
-        for (int b = a + 1; b < R.n_node; ++b) {
-            bool hit = false;
-            for (int i = v.bead0[a]; i < v.bead0[a + 1] && !hit; ++i)
-                for (int j = v.bead0[b]; j < v.bead0[b + 1]; ++j) {
-                    float dx = cache[i * 3] - cache[j * 3], dy = cache[i * 3 + 1] - cache[j * 3 + 1],
-                          dz = cache[i * 3 + 2] - cache[j * 3 + 2];
-                    if (refine_dist2(dx, dy, dz) < R.edge_cut2) { hit = true; break; }
-                }
-            if (hit) {
-                if (k < v.maxc) v.cnbr[a * v.maxc + k] = b;
-                ++k;
+constexpr int BIG = 0x7fffffff;
+
+// Merge the sorted partner lists of a node's beads (at most 6) into distinct
+// partner nodes, in increasing order: emit(b) once per partner node, slot(k)
+// for every list entry k.  Lists are [beg[k], end[k]); node_of(k) is the
+// partner node of entry k and is non-decreasing along each list.
+template <class NodeOf, class Emit, class Slot>
+__device__ __forceinline__ void merge_partner_nodes(int* beg, const int* end, NodeOf node_of, Emit emit, Slot slot) {
+    int head[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) head[k] = beg[k] < end[k] ? node_of(beg[k]) : BIG;
+    for (;;) {
+        int mn = BIG;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) mn = min(mn, head[k]);
+        if (mn == BIG) break;
+        emit(mn);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            while (head[k] == mn) {
+                slot(beg[k]);
+                ++beg[k];
+                head[k] = beg[k] < end[k] ? node_of(beg[k]) : BIG;
             }
         }
-        if (k > v.maxc) { overflow = 1; k = v.maxc; }
-        v.ccnt[a] = k;
     }
-    __syncthreads();
-    for (int x = c.tid; x < R.n_node; x += NT) {
-        int t = 0;
-        for (int a = 0; a < x; ++a) {
-            int k = lower_bound_i(v.cnbr + a * v.maxc, v.ccnt[a], x);
-            if (k < v.ccnt[a] && v.cnbr[a * v.maxc + k] == x) {
-                if (t < v.maxi) v.iinc[x * v.maxi + t] = (a * v.maxc + k) * 2 + 1;
-                ++t;
-            }
-        }
-        for (int k = 0; k < v.ccnt[x]; ++k) {
-            if (t < v.maxi) v.iinc[x * v.maxi + t] = (x * v.maxc + k) * 2 + 0;
-            ++t;
-        }
-        if (t > v.maxi) { overflow = 1; t = v.maxi; }
-        v.icnt[x] = t;
-    }
-    int of = __syncthreads_or(overflow);
-    if (c.tid == 0 && of) c.I[G.state + 1] = 1;
-    __syncthreads();
 }
 
-// Orientation of candidate (a,b): rows of P belong to the node with fewer
-// rotamers (src/rotamer.cpp:898); for equal counts the lower node comes first.
-__device__ __forceinline__ bool rot_swapped(const RotView& v, int a, int b) { return v.nrot[a] > v.nrot[b]; }
-
-// Fill node probabilities and candidate edge matrices for this evaluation.
-__device__ void rot_fill(Ctx& c, const PairGraph& G, const RotamerDev& R) {
+// Residue-pair candidates and node incidences from the freshly rebuilt bead
+// rows.  Candidate c = (ca[c], cb[c]) with ca < cb, sorted; the incidence list
+// of node x names (candidate, 1) for partners a < x first, then (candidate, 0)
+// for its own candidates.  Returns false on capacity overflow.
+__device__ bool rot_rebuild(Ctx& c, const PairGraph& G, const RotamerDev& R) {
     RotView v = rot_view(c, R);
-    const int* loc1 = c.CI + G.loc1;
-    const int* cnt = c.I + G.cnt;
+    const int* off = c.I + G.off;
     const int* nbr = c.I + G.nbr;
-    const int* act = c.I + G.act;
-    const float* val = c.A + G.val;
-    // node energies -> probabilities (fill_holders, convert_energy_to_prob, :239-256,867-880)
-    for (int n = c.tid; n < R.n_node; n += NT) {
-        float e[6] = {0, 0, 0, 0, 0, 0};
-        for (int b = v.bead0[n]; b < v.bead0[n + 1]; ++b) {
-            float s = 0.f;
-            for (int q = 0; q < R.n_prob; ++q) s += c.A[R.prob_out[q] + loc1[b] * R.prob_w[q]];
-            e[v.brot[b]] += s;
-        }
-        int nr = v.nrot[n];
-        float mn = e[0];
-        for (int r = 1; r < nr; ++r) mn = fminf(mn, e[r]);
-        v.noff[n] = mn;
-        for (int r = 0; r < 6; ++r) v.nprob[n * 6 + r] = r < nr ? expf(mn - e[r]) : 0.f;
-    }
-    // candidate edge probabilities P[r1][r2] = prod exp(-E) over active bead pairs
+    const int* row = c.I + G.row;
+    const int* toff = c.I + G.toff;
+    const int* tslot = c.I + G.tslot;
+    auto fwd_node = [&](int s) { return v.bnode[nbr[s]]; };
+    auto low_node = [&](int t) { return v.bnode[row[tslot[t]]]; };
     for (int a = c.tid; a < R.n_node; a += NT) {
-        for (int k = 0; k < v.ccnt[a]; ++k) {
-            int ci = a * v.maxc + k;
-            int b = v.cnbr[ci];
-            bool sw = rot_swapped(v, a, b);
-            int n1 = sw ? v.nrot[b] : v.nrot[a], n2 = sw ? v.nrot[a] : v.nrot[b];
-            float* P = v.cP + (size_t)ci * 36;
-            for (int q = 0; q < 36; ++q) P[q] = ((q / 6) < n1 && (q % 6) < n2) ? 1.f : 0.f;
-            int pres = 0;
-            int jb0 = v.bead0[b], jb1 = v.bead0[b + 1];
-            for (int i = v.bead0[a]; i < v.bead0[a + 1]; ++i) {
-                const int* row = nbr + i * G.maxn;
-                int k0 = lower_bound_i(row, cnt[i], jb0);
-                for (int s = k0; s < cnt[i] && row[s] < jb1; ++s) {
-                    int slot = i * G.maxn + s;
-                    if (!act[slot]) continue;
-                    pres = 1;
-                    int ra = v.brot[i], rb = v.brot[row[s]];
-                    int r1 = sw ? rb : ra, r2 = sw ? ra : rb;
-                    P[r1 * 6 + r2] *= expf(-val[slot]);
-                }
-            }
-            v.cpres[ci] = pres;
+        int beg[6], end[6];
+        int b0 = v.bead0[a], b1 = v.bead0[a + 1];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            beg[k] = b0 + k < b1 ? off[b0 + k] : 0;
+            end[k] = b0 + k < b1 ? off[b0 + k + 1] : 0;
         }
+        int n = 0;
+        merge_partner_nodes(beg, end, fwd_node, [&](int) { ++n; }, [&](int) {});
+        v.coff[a] = n;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            beg[k] = b0 + k < b1 ? toff[b0 + k] : 0;
+            end[k] = b0 + k < b1 ? toff[b0 + k + 1] : 0;
+        }
+        int m = 0;
+        merge_partner_nodes(beg, end, low_node, [&](int) { ++m; }, [&](int) {});
+        v.ioff[a] = n + m;
+    }
+    if (c.tid == 0) { v.coff[R.n_node] = 0; v.ioff[R.n_node] = 0; }
+    __syncthreads();
+    int C = block_exscan(c, v.coff, R.n_node + 1);
+    int I = block_exscan(c, v.ioff, R.n_node + 1);
+    if (C > R.ccap || I > 2 * R.ccap) {
+        if (c.tid == 0) { c.I[G.state + 1] = 1; c.I[G.state + 0] = 0; }
+        __syncthreads();
+        return false;
+    }
+    int* scand = v.scand;
+    for (int a = c.tid; a < R.n_node; a += NT) {
+        int beg[6], end[6];
+        int b0 = v.bead0[a], b1 = v.bead0[a + 1];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            beg[k] = b0 + k < b1 ? off[b0 + k] : 0;
+            end[k] = b0 + k < b1 ? off[b0 + k + 1] : 0;
+        }
+        int cc = v.coff[a] - 1;
+        merge_partner_nodes(beg, end, fwd_node, [&](int b) { ++cc; v.ca[cc] = a; v.cb[cc] = b; },
+                            [&](int s) { scand[s] = cc; });
     }
     __syncthreads();
-    // fold present 1-rotamer edges into the partner's probabilities (:398-405)
     for (int x = c.tid; x < R.n_node; x += NT) {
-        if (v.nrot[x] == 1) continue;
-        for (int t = 0; t < v.icnt[x]; ++t) {
-            int e = v.iinc[x * v.maxi + t];
-            int ci = e >> 1;
-            if (!v.cpres[ci]) continue;
-            int a = ci / v.maxc, b = v.cnbr[ci];
-            int other = (e & 1) ? a : b;
-            if (v.nrot[other] != 1) continue;
-            const float* P = v.cP + (size_t)ci * 36;  // row 0 belongs to the 1-rotamer node
-            for (int r = 0; r < v.nrot[x]; ++r) v.nprob[x * 6 + r] *= P[r];
+        int beg[6], end[6];
+        int b0 = v.bead0[x], b1 = v.bead0[x + 1];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            beg[k] = b0 + k < b1 ? toff[b0 + k] : 0;
+            end[k] = b0 + k < b1 ? toff[b0 + k + 1] : 0;
         }
+        int t = v.ioff[x];
+        merge_partner_nodes(beg, end, low_node, [&](int a) {
+            int c0 = v.coff[a];
+            int k = lower_bound_i(v.cb + c0, v.coff[a + 1] - c0, x);
+            v.iinc[t++] = (c0 + k) * 2 + 1;
+        }, [&](int) {});
+        for (int cc = v.coff[x]; cc < v.coff[x + 1]; ++cc) v.iinc[t++] = cc * 2;
     }
+    if (c.tid == 0) v.stats[5] = C;
     __syncthreads();
-}
-
-__device__ __forceinline__ bool rot_bp_edge(const RotView& v, int ci, int& f, int& s) {
-    // BP runs on present edges between 3- or 6-rotamer nodes
-    int a = ci / v.maxc, b = v.cnbr[ci];
-    if (v.nrot[a] == 1 || v.nrot[b] == 1) return false;
-    bool sw = rot_swapped(v, a, b);
-    f = sw ? b : a; s = sw ? a : b;
     return true;
 }
 
-// One synchronous message pass: new messages from old node beliefs `nold_`
-// and old messages (half `po` of cmsg); each new message is scaled to unit L1
-// norm (src/rotamer.cpp:515-568).
-__device__ void rot_edge_pass(Ctx& c, const RotamerDev& R, const float* nold_, int po) {
+// Node probabilities, BP-edge compaction and edge matrices for this evaluation.
+__device__ void rot_fill(Ctx& c, const PairGraph& G, const RotamerDev& R) {
     RotView v = rot_view(c, R);
-    int total = R.n_node * v.maxc;
-    for (int ci = c.tid; ci < total; ci += NT) {
-        int a = ci / v.maxc, k = ci % v.maxc;
-        if (k >= v.ccnt[a] || !v.cpres[ci]) continue;
-        int f, s;
-        if (!rot_bp_edge(v, ci, f, s)) continue;
-        int n1 = v.nrot[f], n2 = v.nrot[s];
-        const float* P = v.cP + (size_t)ci * 36;
-        const float* mo = v.cmsg + (size_t)ci * 24 + po * 12;
-        float* mn = v.cmsg + (size_t)ci * 24 + (1 - po) * 12;
+    const int* off = c.I + G.off;
+    const int* nbr = c.I + G.nbr;
+    const int* row = c.I + G.row;
+    const int* toff = c.I + G.toff;
+    const int* tslot = c.I + G.tslot;
+    const int* act = c.I + G.act;
+    const float* val = c.A + G.val;
+    const int* loc1 = c.CI + G.loc1;
+    const int E = c.I[G.state + 3];
+    const int C = v.coff[R.n_node];
+    const int cap = v.cap;
+    // energies of each bead's edges to 1-rotamer residues (folded into the node)
+    {
+        int grp = c.tid / GW, gl = c.tid % GW;
+        for (int base = 0; base < G.n1; base += NT / GW) {
+            int b = base + grp;
+            bool ok = b < G.n1;
+            float acc = 0.f;
+            if (ok && v.nrot[v.bnode[b]] > 1) {
+                for (int s = off[b] + gl; s < off[b + 1]; s += GW)
+                    if (act[s] && v.nrot[v.bnode[nbr[s]]] == 1) acc += val[s];
+                for (int t = toff[b] + gl; t < toff[b + 1]; t += GW) {
+                    int s = tslot[t];
+                    if (act[s] && v.nrot[v.bnode[row[s]]] == 1) acc += val[s];
+                }
+            }
+            acc = group_sum<GW>(acc);
+            if (ok && gl == 0) v.e1x[b] = acc;
+        }
+    }
+    for (int k = c.tid; k <= C; k += NT) v.eidx[k] = 0;
+    __syncthreads();
+    // node energies -> probabilities (fill_holders, convert_energy_to_prob, :239-256,867-880)
+    for (int n = c.tid; n < R.n_node; n += NT) {
+        int b0 = v.bead0[n], nr = v.nrot[n];
+        float e[6];
+        float mn = 3.0e38f;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            float s = 0.f;
+            if (k < nr) {
+                for (int q = 0; q < R.n_prob; ++q) s += c.A[R.prob_out[q] + loc1[b0 + k] * R.prob_w[q]];
+                mn = fminf(mn, s);
+            }
+            e[k] = s;
+        }
+        v.noff[n] = mn;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            v.nprob[n * 6 + k] = k < nr ? expf(mn - e[k] - v.e1x[b0 + k]) : 0.f;
+    }
+    // presence of BP edges (both residues with 3 or 6 rotamers)
+    for (int s = c.tid; s < E; s += NT) {
+        if (!act[s]) continue;
+        if (v.nrot[v.bnode[row[s]]] > 1 && v.nrot[v.bnode[nbr[s]]] > 1) v.eidx[v.scand[s]] = 1;
+    }
+    __syncthreads();
+    int nbp = block_exscan(c, v.eidx, C + 1);
+    for (int cc = c.tid; cc < C; cc += NT) {
+        int e = v.eidx[cc];
+        if (v.eidx[cc + 1] == e) continue;
+        int a = v.ca[cc], b = v.cb[cc];
+        // rows of P belong to the residue with fewer rotamers (:898), the lower one on ties
+        bool sw = v.nrot[a] > v.nrot[b];
+        int f = sw ? b : a, s2 = sw ? a : b;
+        v.bpc[e] = cc; v.bpf[e] = f; v.bps[e] = s2;
+        int n1 = v.nrot[f], n2 = v.nrot[s2];
+#pragma unroll
+        for (int q = 0; q < 36; ++q) v.P[q * cap + e] = ((q / 6) < n1 && (q % 6) < n2) ? 1.f : 0.f;
+    }
+    if (c.tid == 0) v.stats[4] = nbp;
+    __syncthreads();
+    // P[r1][r2] = exp(-E) of the bead pair (one bead per rotamer state)
+    for (int s = c.tid; s < E; s += NT) {
+        if (!act[s]) continue;
+        int i = row[s], j = nbr[s];
+        int a = v.bnode[i], b = v.bnode[j];
+        if (v.nrot[a] == 1 || v.nrot[b] == 1) continue;
+        int e = v.eidx[v.scand[s]];
+        bool sw = v.nrot[a] > v.nrot[b];
+        int ra = v.brot[i], rb = v.brot[j];
+        int q = sw ? rb * 6 + ra : ra * 6 + rb;
+        v.P[q * cap + e] = expf(-val[s]);
+    }
+    // compact BP incidences of every node, in incidence order
+    for (int x = c.tid; x < R.n_node; x += NT) {
+        int k = v.ioff[x];
+        if (v.nrot[x] > 1) {
+            for (int t = v.ioff[x]; t < v.ioff[x + 1]; ++t) {
+                int cc = v.iinc[t] >> 1;
+                int e = v.eidx[cc];
+                if (v.eidx[cc + 1] == e) continue;
+                v.binc[k++] = e * 2 + (v.bpf[e] == x ? 0 : 1);
+            }
+        }
+        v.ninc[x] = k - v.ioff[x];
+    }
+    __syncthreads();
+}
+
+// One synchronous message pass: new messages (half 1-po) from old node
+// beliefs `nold_` and old messages (half po); each new message is scaled to
+// unit L1 norm (src/rotamer.cpp:515-568).  Message layout: component
+// (h*12 + side*6 + r) of edge e at msg[(...)*cap + e].
+__device__ void rot_edge_pass(Ctx& c, const RotView& v, int nbp, const float* nold_, int po) {
+    const int cap = v.cap;
+    for (int e = c.tid; e < nbp; e += NT) {
+        int f = v.bpf[e], s = v.bps[e];
         float v1[6], v2[6];
-        for (int r = 0; r < n1; ++r) v1[r] = nold_[f * 6 + r] / (1e-10f + mo[r]);
-        for (int r = 0; r < n2; ++r) v2[r] = nold_[s * 6 + r] / (1e-10f + mo[6 + r]);
-        float m1[6], m2[6], s1 = 0.f, s2 = 0.f;
-        for (int i = 0; i < n1; ++i) {
-            float acc = 0.f;
-            for (int j = 0; j < n2; ++j) acc += P[i * 6 + j] * v2[j];
-            m1[i] = acc; s1 += acc;
-        }
-        for (int j = 0; j < n2; ++j) {
-            float acc = 0.f;
-            for (int i = 0; i < n1; ++i) acc += P[i * 6 + j] * v1[i];
-            m2[j] = acc; s2 += acc;
-        }
-        float i1 = 1.f / s1, i2 = 1.f / s2;
+#pragma unroll
         for (int r = 0; r < 6; ++r) {
-            mn[r] = r < n1 ? m1[r] * i1 : 0.f;
-            mn[6 + r] = r < n2 ? m2[r] * i2 : 0.f;
+            v1[r] = nold_[f * 6 + r] / (1e-10f + v.msg[(po * 12 + r) * cap + e]);
+            v2[r] = nold_[s * 6 + r] / (1e-10f + v.msg[(po * 12 + 6 + r) * cap + e]);
+        }
+        float m1[6] = {0, 0, 0, 0, 0, 0}, m2[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                float p = v.P[(i * 6 + j) * cap + e];
+                m1[i] += p * v2[j];
+                m2[j] += p * v1[i];
+            }
+        float s1 = 0.f, s2 = 0.f;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) { s1 += m1[r]; s2 += m2[r]; }
+        float i1 = 1.f / s1, i2 = 1.f / s2;
+        int h = (1 - po) * 12;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            v.msg[(h + r) * cap + e] = m1[r] * i1;
+            v.msg[(h + 6 + r) * cap + e] = m2[r] * i2;
         }
     }
     __syncthreads();
@@ -200,40 +292,45 @@ __device__ void rot_edge_pass(Ctx& c, const RotamerDev& R, const float* nold_, i
 // Node pass: cur = prob * prod(new messages), renormalised after every factor;
 // then cur = (1-d) cur/max(cur) + d*old (standardize_belief_update, :258-273).
 // With swap_first, the product goes to `nold_` and cur restarts from prob
-// (the first, undamped pass of solve_for_marginals).
-__device__ void rot_node_pass(Ctx& c, const RotamerDev& R, float* ncur_, float* nold_, int pn, float damping,
-                              bool swap_first) {
-    RotView v = rot_view(c, R);
-    for (int x = c.tid; x < R.n_node; x += NT) {
-        int nr = v.nrot[x];
-        if (nr == 1) continue;
+// (the first, undamped pass of solve_for_marginals).  Padding states carry
+// zeros throughout.
+__device__ void rot_node_pass(Ctx& c, const RotView& v, int n_node, float* ncur_, float* nold_, int pn,
+                              float damping, bool swap_first) {
+    const int cap = v.cap;
+    for (int x = c.tid; x < n_node; x += NT) {
+        if (v.nrot[x] == 1) continue;
         float b[6];
+#pragma unroll
         for (int r = 0; r < 6; ++r) b[r] = v.nprob[x * 6 + r];
-        for (int t = 0; t < v.icnt[x]; ++t) {
-            int e = v.iinc[x * v.maxi + t];
-            int ci = e >> 1;
-            if (!v.cpres[ci]) continue;
-            int f, s;
-            if (!rot_bp_edge(v, ci, f, s)) continue;
-            const float* m = v.cmsg + (size_t)ci * 24 + pn * 12 + (f == x ? 0 : 6);
+        int t0 = v.ioff[x], n = v.ninc[x];
+        for (int t = 0; t < n; ++t) {
+            int inc = v.binc[t0 + t];
+            int e = inc >> 1;
+            int base = (pn * 12 + (inc & 1) * 6) * cap + e;
             float sum = 0.f;
-            for (int r = 0; r < nr; ++r) { b[r] *= m[r]; sum += b[r]; }
+#pragma unroll
+            for (int r = 0; r < 6; ++r) { b[r] *= v.msg[base + r * cap]; sum += b[r]; }
             float inv = 1.f / sum;
-            for (int r = 0; r < nr; ++r) b[r] *= inv;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) b[r] *= inv;
         }
         if (swap_first) {
-            float mx = v.nprob[x * 6];
-            for (int r = 1; r < nr; ++r) mx = fmaxf(mx, v.nprob[x * 6 + r]);
+            float mx = 0.f;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) mx = fmaxf(mx, v.nprob[x * 6 + r]);
+            float im = 1.f / mx;
+#pragma unroll
             for (int r = 0; r < 6; ++r) {
                 nold_[x * 6 + r] = b[r];
-                ncur_[x * 6 + r] = r < nr ? v.nprob[x * 6 + r] / mx : 0.f;
+                ncur_[x * 6 + r] = v.nprob[x * 6 + r] * im;
             }
         } else {
-            float mx = b[0];
-            for (int r = 1; r < nr; ++r) mx = fmaxf(mx, b[r]);
+            float mx = 0.f;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) mx = fmaxf(mx, b[r]);
             float im = 1.f / mx;
-            for (int r = 0; r < 6; ++r)
-                ncur_[x * 6 + r] = r < nr ? (1.f - damping) * im * b[r] + damping * nold_[x * 6 + r] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) ncur_[x * 6 + r] = (1.f - damping) * im * b[r] + damping * nold_[x * 6 + r];
         }
     }
     __syncthreads();
@@ -242,97 +339,104 @@ __device__ void rot_node_pass(Ctx& c, const RotamerDev& R, float* ncur_, float* 
 // Solve; returns the iteration count (the reference's `iter`).
 __device__ int rot_solve(Ctx& c, const RotamerDev& R) {
     RotView v = rot_view(c, R);
-    int total = R.n_node * v.maxc;
+    const int nbp = v.stats[4];
+    const int cap = v.cap;
     // initial messages = 1, old beliefs = prob (:1110-1133)
-    for (int ci = c.tid; ci < total; ci += NT) {
-        float* m = v.cmsg + (size_t)ci * 24;
-        for (int r = 0; r < 12; ++r) m[r] = 1.f;
-    }
+    for (int e = c.tid; e < nbp; e += NT)
+#pragma unroll
+        for (int r = 0; r < 12; ++r) v.msg[r * cap + e] = 1.f;
     for (int i = c.tid; i < R.n_node * 6; i += NT) v.nold[i] = v.nprob[i];
     __syncthreads();
     float* cur = v.ncur;
     float* old = v.nold;
     int po = 0;
-    rot_edge_pass(c, R, old, po);
+    rot_edge_pass(c, v, nbp, old, po);
     po = 1 - po;  // new messages now live in half po
-    rot_node_pass(c, R, cur, old, po, 0.f, true);
+    rot_node_pass(c, v, R.n_node, cur, old, po, 0.f, true);
     float dev = 1e10f;
     int it = 0;
     for (; dev > R.tol && it < R.max_iter; it += R.chunk) {
         for (int j = 0; j < R.chunk; ++j) {
             float* t = cur; cur = old; old = t;     // swap node beliefs
-            rot_edge_pass(c, R, old, po);           // messages old = half po
+            rot_edge_pass(c, v, nbp, old, po);      // messages old = half po
             po = 1 - po;
-            rot_node_pass(c, R, cur, old, po, R.damping, false);
+            rot_node_pass(c, v, R.n_node, cur, old, po, R.damping, false);
         }
         float d = 0.f;
         for (int x = c.tid; x < R.n_node; x += NT) {
-            int nr = v.nrot[x];
-            if (nr == 1) continue;
-            for (int r = 0; r < nr; ++r) d = fmaxf(d, cur[x * 6 + r] - old[x * 6 + r]);
+            if (v.nrot[x] == 1) continue;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) d = fmaxf(d, cur[x * 6 + r] - old[x * 6 + r]);
         }
         dev = block_max(c, d);
     }
-    // marginals (:283-290, :425-449); final beliefs may sit in either buffer
+    // node marginals (:283-290); final beliefs may sit in either buffer
     for (int x = c.tid; x < R.n_node; x += NT) {
         int nr = v.nrot[x];
+        float b[6];
         float s = 0.f;
-        for (int r = 0; r < nr; ++r) s += cur[x * 6 + r];
-        for (int r = 0; r < 6; ++r)
-            v.ncur[x * 6 + r] = nr == 1 ? (r == 0 ? 1.f : 0.f) : (r < nr ? cur[x * 6 + r] / s : 0.f);
+#pragma unroll
+        for (int r = 0; r < 6; ++r) { b[r] = cur[x * 6 + r]; s += b[r]; }
+        float is = 1.f / s;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) v.ncur[x * 6 + r] = nr == 1 ? (r == 0 ? 1.f : 0.f) : b[r] * is;
     }
     __syncthreads();
-    for (int ci = c.tid; ci < total; ci += NT) {
-        int a = ci / v.maxc, k = ci % v.maxc;
-        if (k >= v.ccnt[a] || !v.cpres[ci]) continue;
-        int f, s;
-        if (!rot_bp_edge(v, ci, f, s)) continue;
-        int n1 = v.nrot[f], n2 = v.nrot[s];
-        const float* P = v.cP + (size_t)ci * 36;
-        const float* m = v.cmsg + (size_t)ci * 24 + po * 12;
+    // edge marginals (:425-449)
+    for (int e = c.tid; e < nbp; e += NT) {
+        int f = v.bpf[e], s = v.bps[e];
         float b1[6], b2[6];
-        for (int r = 0; r < n1; ++r) b1[r] = v.ncur[f * 6 + r] / (1e-10f + m[r]);
-        for (int r = 0; r < n2; ++r) b2[r] = v.ncur[s * 6 + r] / (1e-10f + m[6 + r]);
-        float* mg = v.cmarg + (size_t)ci * 36;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            b1[r] = v.ncur[f * 6 + r] / (1e-10f + v.msg[(po * 12 + r) * cap + e]);
+            b2[r] = v.ncur[s * 6 + r] / (1e-10f + v.msg[(po * 12 + 6 + r) * cap + e]);
+        }
+        // two sweeps (sum, then scaled store) keep 36 products out of registers
         float sum = 0.f;
-        for (int i = 0; i < n1; ++i)
-            for (int j = 0; j < n2; ++j) { float q = P[i * 6 + j] * b1[i] * b2[j]; mg[i * 6 + j] = q; sum += q; }
+        for (int i = 0; i < 6; ++i)
+#pragma unroll
+            for (int j = 0; j < 6; ++j) sum += v.P[(i * 6 + j) * cap + e] * b1[i] * b2[j];
         float is = 1.f / sum;
-        for (int i = 0; i < n1; ++i)
-            for (int j = 0; j < n2; ++j) mg[i * 6 + j] *= is;
+        for (int i = 0; i < 6; ++i)
+#pragma unroll
+            for (int j = 0; j < 6; ++j) v.marg[(i * 6 + j) * cap + e] = v.P[(i * 6 + j) * cap + e] * b1[i] * b2[j] * is;
     }
     __syncthreads();
     return it;
 }
 
 // Bethe free energy (calculate_energy_from_marginals, :915-927).
-__device__ float rot_free_energy(Ctx& c, const RotamerDev& R) {
+__device__ float rot_free_energy(Ctx& c, const PairGraph& G, const RotamerDev& R) {
     RotView v = rot_view(c, R);
+    const int cap = v.cap;
     float en = 0.f;
     for (int x = c.tid; x < R.n_node; x += NT) {
-        int nr = v.nrot[x];
         float e = v.noff[x];
-        for (int r = 0; r < nr; ++r) {
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
             float b = v.ncur[x * 6 + r];
             e += b * logf((1e-10f + b) / (1e-10f + v.nprob[x * 6 + r]));
         }
         en += e;
     }
-    int total = R.n_node * v.maxc;
-    for (int ci = c.tid; ci < total; ci += NT) {
-        int a = ci / v.maxc, k = ci % v.maxc;
-        if (k >= v.ccnt[a] || !v.cpres[ci]) continue;
-        int b = v.cnbr[ci];
-        const float* P = v.cP + (size_t)ci * 36;
-        if (v.nrot[a] == 1 && v.nrot[b] == 1) { en += -logf(P[0]); continue; }
-        int f, s;
-        if (!rot_bp_edge(v, ci, f, s)) continue;
-        int n1 = v.nrot[f], n2 = v.nrot[s];
-        const float* mg = v.cmarg + (size_t)ci * 36;
-        for (int i = 0; i < n1; ++i)
-            for (int j = 0; j < n2; ++j) {
-                float p = mg[i * 6 + j];
-                en += p * logf((1e-10f + p) / (1e-10f + P[i * 6 + j] * v.ncur[f * 6 + i] * v.ncur[s * 6 + j]));
+    // 1-rotamer pairs: -log P with P the probability of their single bead pair
+    const int* row = c.I + G.row;
+    const int* nbr = c.I + G.nbr;
+    const int* act = c.I + G.act;
+    const float* val = c.A + G.val;
+    const int E = c.I[G.state + 3];
+    for (int s = c.tid; s < E; s += NT)
+        if (act[s] && v.nrot[v.bnode[row[s]]] == 1 && v.nrot[v.bnode[nbr[s]]] == 1) en += -logf(expf(-val[s]));
+    const int nbp = v.stats[4];
+    for (int e = c.tid; e < nbp; e += NT) {
+        int f = v.bpf[e], s = v.bps[e];
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                float p = v.marg[(i * 6 + j) * cap + e];
+                float q = v.P[(i * 6 + j) * cap + e] * v.ncur[f * 6 + i] * v.ncur[s * 6 + j];
+                en += p * logf((1e-10f + p) / (1e-10f + q));
             }
     }
     return block_sum(c, en);
@@ -340,37 +444,34 @@ __device__ float rot_free_energy(Ctx& c, const RotamerDev& R) {
 
 // Edge sensitivities of bead pairs (src/rotamer.cpp:1057-1066) and the 1-body
 // sensitivities (:1074-1085).
-__device__ void rot_sens(Ctx& c, const PairGraph& G, const RotamerDev& R, const int* node_of_bead) {
+__device__ void rot_sens(Ctx& c, const PairGraph& G, const RotamerDev& R) {
     RotView v = rot_view(c, R);
-    const int* cnt = c.I + G.cnt;
+    const int* row = c.I + G.row;
     const int* nbr = c.I + G.nbr;
     const int* act = c.I + G.act;
     float* es = c.A + G.esens;
     const int* loc1 = c.CI + G.loc1;
-    int wave = c.tid >> 6, lane = c.tid & 63;
-    for (int i = wave; i < G.n1; i += NWAVE) {
-        int n = cnt[i];
-        int a = node_of_bead[i];
-        for (int k = lane; k < n; k += 64) {
-            int slot = i * G.maxn + k;
-            if (!act[slot]) { es[slot] = 0.f; continue; }
-            int j = nbr[slot];
-            int b = node_of_bead[j];
-            int kk = lower_bound_i(v.cnbr + a * v.maxc, v.ccnt[a], b);
-            if (kk >= v.ccnt[a] || v.cnbr[a * v.maxc + kk] != b) { es[slot] = 0.f; continue; }  // list overflow
-            int ci = a * v.maxc + kk;
-            bool sw = rot_swapped(v, a, b);
-            int f = sw ? b : a, s = sw ? a : b;
-            int rf = sw ? v.brot[j] : v.brot[i], rs = sw ? v.brot[i] : v.brot[j];
-            float sv;
-            if (v.nrot[f] == 1 && v.nrot[s] == 1) sv = 1.f;
-            else if (v.nrot[f] == 1) sv = v.ncur[s * 6 + rs];
-            else sv = v.cmarg[(size_t)ci * 36 + rf * 6 + rs];
-            es[slot] = sv;
+    const int E = c.I[G.state + 3];
+    const int cap = v.cap;
+    for (int s = c.tid; s < E; s += NT) {
+        if (!act[s]) { es[s] = 0.f; continue; }
+        int i = row[s], j = nbr[s];
+        int a = v.bnode[i], b = v.bnode[j];
+        int na = v.nrot[a], nb = v.nrot[b];
+        int ra = v.brot[i], rb = v.brot[j];
+        float sv;
+        if (na == 1 && nb == 1) sv = 1.f;
+        else if (na == 1) sv = v.ncur[b * 6 + rb];
+        else if (nb == 1) sv = v.ncur[a * 6 + ra];
+        else {
+            int e = v.eidx[v.scand[s]];
+            int q = na > nb ? rb * 6 + ra : ra * 6 + rb;
+            sv = v.marg[q * cap + e];
         }
+        es[s] = sv;
     }
     for (int b = c.tid; b < G.n1; b += NT) {
-        float m = v.ncur[node_of_bead[b] * 6 + v.brot[b]];
+        float m = v.ncur[v.bnode[b] * 6 + v.brot[b]];
         for (int q = 0; q < R.n_prob; ++q) c.A[R.prob_sens[q] + loc1[b] * R.prob_w[q]] += m;
     }
     __syncthreads();

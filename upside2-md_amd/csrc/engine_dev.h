@@ -12,6 +12,11 @@
 
 namespace upside {
 
+// Dynamic LDS per workgroup: the coefficient table of the pair graph being
+// evaluated is staged here (the largest, the rotamer pair table, is
+// 20*20*54 floats = 86.4 KB, plus read-past padding).
+constexpr int LDS_FLOATS = 22016;
+
 enum OpKind : int {
     OP_DIST3D = 1, OP_ANGLE, OP_DIHEDRAL, OP_SPRING, OP_RAMA_COORD, OP_CAT_POS,
     OP_RAMA_MAP_POT, OP_AFFINE, OP_PLACEMENT, OP_INFER_HO, OP_PROTEIN_HBOND,
@@ -31,11 +36,14 @@ enum EdgeKind : int {
 };
 
 // A pair graph over two element sets (or one, symmetric).  Candidate pairs
-// within the cache radius are kept in row-major padded lists ("slots"): row i1
-// holds its partners i2 in increasing order (for symmetric graphs only i2>i1).
-// A transposed list per i2 names the slots where it appears, in increasing i1
-// order.  Both orders are fixed, so the sensitivity scatter is a gather in a
-// fixed order: bit-reproducible, no float atomics.
+// within the cache radius ("slots") are kept in compressed rows: row i1 holds
+// its partners i2 in increasing order (symmetric graphs: only i2 > i1).  The
+// transposed rows list, for each i2, the slots naming it in increasing i1
+// order.  Both orders are canonical, so every sum over slots (energies,
+// sensitivity gathers) runs in a fixed order: bit-reproducible, no float
+// atomics.  Rows are rebuilt from a hashed cell grid of cell size equal to
+// the cache radius.  Per-slot derivatives are stored structure-of-arrays:
+// component q of slot s at d1[q*ecap + s].
 struct PairGraph {
     int kind;
     int sym;
@@ -49,46 +57,64 @@ struct PairGraph {
     int id1, id2;          // const int arrays
     int param;             // const float array, (n_type1*n_type2, n_param) + tail padding
     int n_type2, n_param;
+    int param_len;         // floats of the table staged in LDS (0: read from memory)
     int excl;              // pair accepted if |id1-id2| > excl; -1: rotamer rule (id>>4 differs)
     float cutoff2;         // refine threshold (float cutoff squared)
     float cache_cut2;      // cache threshold
+    float cache_cut;       // cache radius (cell size of the grid)
     float drift2;          // rebuild when an element moved more than this (squared)
     int ddim1, ddim2;      // derivative widths (== dim except for backbone pairs)
-    int maxn, maxt;        // slot capacity per row / per transposed row
+    int ecap;              // slot capacity
+    int nbucket;           // cell-grid hash buckets (power of two)
     // per-system scratch (arena offsets)
     int cache_pos;         // float (n1+n2)*3 (symmetric: n1*3)
     int pos_pack;          // float (n1+n2)*3 current packed coordinates of both sides
-    int cnt, nbr;          // int n1, n1*maxn
-    int tcnt, tslot;       // int n2, n2*maxt (slot index i1*maxn+k)
-    int val;               // float n1*maxn   edge value (0 when inactive)
-    int d1, d2;            // float n1*maxn*ddim1, n1*maxn*ddim2
-    int esens;             // float n1*maxn   edge sensitivity
-    int act;               // int n1*maxn     active flag
-    int state;             // int [0]=cache valid, [1]=overflow, [2]=rebuild count
+    int off;               // int n1+1  row offsets
+    int nbr;               // int ecap  partner of each slot
+    int row;               // int ecap  row of each slot
+    int toff;              // int n2+1  transposed row offsets
+    int tslot;             // int ecap  slot of each transposed entry
+    int tmp;               // int ecap  unsorted row entries during a rebuild
+    int gofs;              // int nbucket+1  cell-grid bucket offsets
+    int gidx;              // int max(n1,n2)  elements ordered by bucket
+    int val;               // float ecap   edge value (0 when inactive)
+    int d1, d2;            // float ddim1*ecap, ddim2*ecap (SoA)
+    int esens;             // float ecap   edge sensitivity
+    int act;               // int ecap     active flag
+    int state;             // int [0]=cache valid, [1]=overflow, [2]=rebuild count, [3]=slots
 };
 
 // Rotamer belief propagation over residue nodes (src/rotamer.cpp:628-1169).
+// Each residue node has one bead per rotamer state, listed in state order.
 struct RotamerDev {
     int n_node;            // residues with side chains
     int node_nrot;         // const int [n_node]
     int node_bead0;        // const int [n_node+1]  beads of node n: [bead0[n], bead0[n+1])
     int bead_rot;          // const int [n_bead]
+    int bead_node;         // const int [n_bead]
     int n_prob;            // number of 1-body energy producers (<=4)
     int prob_out[4], prob_sens[4], prob_w[4];
     float damping, tol;
     int max_iter, chunk;
-    int maxc, maxi;        // candidate residue pairs per node row / incidences per node
-    // per-system scratch
-    int ccnt, cnbr;        // int n_node, n_node*maxc   candidate partner node (B>A)
-    int icnt, iinc;        // int n_node, n_node*maxi   incidences: cand index*2 + side
-    int cpres;             // int n_node*maxc  present flag (any active bead pair)
-    int cP;                // float n_node*maxc*36  edge probability, rows = first node
-    int cmsg;              // float n_node*maxc*24  [old m1(6) m2(6) | new m1(6) m2(6)]
-    int cmarg;             // float n_node*maxc*36
+    int ccap;              // residue-pair candidate capacity
+    // per-system scratch (arena offsets)
+    int coff;              // int n_node+1   candidates of first node a: [coff[a], coff[a+1])
+    int ca, cb;            // int ccap       candidate residues (a < b)
+    int slot_cand;         // int ecap of the bead graph: candidate of every slot
+    int ioff;              // int n_node+1   incidence offsets
+    int iinc;              // int 2*ccap     incidences: candidate*2 + (node is b)
+    int eidx;              // int ccap+1     BP edge of a candidate (exclusive scan of presence)
+    int bpc, bpf, bps;     // int ccap       BP edge: candidate, first node (fewer rotamers), second node
+    int ninc;              // int n_node     BP incidences of a node
+    int binc;              // int 2*ccap     BP incidences (edge*2 + side), stored from ioff[x]
+    int P;                 // float 36*ccap  edge probability matrix, SoA over BP edges
+    int msg;               // float 24*ccap  [old m1(6) m2(6) | new m1(6) m2(6)], SoA
+    int marg;              // float 36*ccap  edge marginals, SoA
+    int e1x;               // float n_bead   energy of edges to 1-rotamer residues
     int nprob, ncur, nold; // float n_node*6
     int noff;              // float n_node   energy offset
-    int stats;             // int [0]=last iterations, [1]=bad solves, [2]=total iterations, [3]=solves
-    float edge_cut2;       // node-pair candidate cache threshold (bead cache radius)
+    int stats;             // int [0] last iterations [1] bad solves [2] total iterations [3] solves
+                           //     [4] BP edges [5] candidates
 };
 
 struct OpRec {

@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -733,39 +734,41 @@ struct Builder {
         G.out1 = p1->out; G.out2 = p2->out;
         G.sens1 = p1->sens; G.sens2 = p2->sens;
         float cutoff = 0.f, buffer = 0.f;
-        int want = 128;
+        int want = 64;  // initial slot capacity per row (grown on overflow)
         switch (kind) {
         case EDGE_ROTAMER: {
             G.dim1 = G.dim2 = 6; G.excl = -1;
             float inv_dx = 1.f / 0.7f;
             cutoff = (float)((12 - 2 - 1e-6) / inv_dx);
             if (np != 54) fail(n.name + ": rotamer pair table must have 54 parameters");
-            want = 384;
+            want = 64;
         } break;
         case EDGE_HBOND_COVERAGE: {
             G.dim1 = 7; G.dim2 = 6; G.excl = 2;
             float inv_dx = 1.f / 0.625f;
             cutoff = (float)((10 - 2 - 1e-6) / inv_dx);
             if (np != 50) fail(n.name + ": coverage table must have 50 parameters");
-            want = 256;
+            want = 64;
         } break;
         case EDGE_PROTEIN_HBOND:
             G.dim1 = G.dim2 = 6; G.excl = 1;
             cutoff = sqrtf(3.5f * 3.5f);
             if (np != 8) fail(n.name + ": hbond table must have 8 parameters");
-            want = 64;
+            want = 16;
             break;
         case EDGE_ENV_WEIGHTED:
         case EDGE_ENV_HBOND:
             G.dim1 = 6; G.dim2 = kind == EDGE_ENV_WEIGHTED ? 4 : 6; G.excl = 2;
             if (np != 4) fail(n.name + ": coverage table must have 4 parameters");
             for (int k = 0; k < nt1 * nt2; ++k) cutoff = std::max(cutoff, param[k * 4] + 1.f / param[k * 4 + 1]);
-            want = kind == EDGE_ENV_WEIGHTED ? 512 : 64;
+            want = kind == EDGE_ENV_WEIGHTED ? 256 : 32;
             break;
         }
         if (p1->width < G.dim1 || p2->width < G.dim2) fail(n.name + ": argument narrower than interaction");
         buffer = 1.0f + 0.2f * cutoff;
         G.ddim1 = G.dim1; G.ddim2 = G.dim2;
+        long table = (long)nt1 * nt2 * np + 64;
+        G.param_len = table <= LDS_FLOATS ? (int)table : 0;
         finish_graph(G, cutoff, buffer, want);
         T.graphs.push_back(G);
         T.graph_node.push_back((int)(&n - T.nodes.data()));
@@ -776,17 +779,26 @@ struct Builder {
         G.cutoff2 = cutoff * cutoff;
         float cc = cutoff + buffer;
         G.cache_cut2 = cc * cc;
+        G.cache_cut = cc;
         G.drift2 = (0.5f * buffer) * (0.5f * buffer);
-        G.maxn = cap(G.sym ? G.n1 : G.n2, want);
-        G.maxt = cap(G.n1, want);
+        long full = G.sym ? (long)G.n1 * (G.n1 - 1) / 2 : (long)G.n1 * G.n2;
+        long ecap = std::min(full, (long)(std::max(G.n1, 1) * (double)want * T.cap_scale));
+        G.ecap = (int)std::max(ecap, 16L);
+        int nb = 64;
+        while (nb < 2 * std::max(G.n1, G.n2)) nb *= 2;
+        G.nbucket = nb;
         int npack = G.sym ? G.n1 : G.n1 + G.n2;
         G.cache_pos = T.falloc(npack * 3);
         G.pos_pack = T.falloc(npack * 3);
-        long slots = (long)G.n1 * G.maxn;
-        G.cnt = T.ialloc(G.n1);
+        long slots = G.ecap;
+        G.off = T.ialloc(G.n1 + 1);
         G.nbr = T.ialloc(slots);
-        G.tcnt = T.ialloc(G.n2);
-        G.tslot = T.ialloc((long)G.n2 * G.maxt);
+        G.row = T.ialloc(slots);
+        G.toff = T.ialloc(G.n2 + 1);
+        G.tslot = T.ialloc(slots);
+        G.tmp = T.ialloc(slots);
+        G.gofs = T.ialloc(nb + 1);
+        G.gidx = T.ialloc(std::max(G.n1, G.n2));
         G.val = T.falloc(slots);
         G.d1 = T.falloc(slots * G.ddim1);
         G.d2 = T.falloc(slots * G.ddim2);
@@ -832,8 +844,9 @@ struct Builder {
         G.param = T.cf_push(vector<float>(1, 0.f), 64);
         G.n_type2 = 1;
         G.n_param = 0;
+        G.param_len = 0;
         G.excl = 1;
-        finish_graph(G, cutoff, 1.0f, 64);
+        finish_graph(G, cutoff, 1.0f, 32);
         o.kind = OP_BACKBONE_PAIRS;
         o.ia[0] = G.loc1;
         o.ia[1] = T.cf_push(ref);
@@ -916,12 +929,19 @@ struct Builder {
             brot[b] = rot;
         }
         bead0.push_back(nb);
+        for (size_t a = 0; a < nrot.size(); ++a) {
+            if (bead0[a + 1] - bead0[a] != nrot[a])
+                fail(n.name + ": a residue must have one bead per rotamer state");
+            for (int k = 0; k < nrot[a]; ++k)
+                if (brot[bead0[a] + k] != k) fail(n.name + ": rotamer beads must be listed in state order");
+        }
         RotamerDev R;
         std::memset(&R, 0, sizeof(R));
         R.n_node = (int)nrot.size();
         R.node_nrot = T.ci_push(nrot);
         R.node_bead0 = T.ci_push(bead0);
         R.bead_rot = T.ci_push(brot);
+        R.bead_node = T.ci_push(node_of);
         R.n_prob = (int)n.parents.size() - 1;
         for (int q = 0; q < R.n_prob; ++q) {
             const Node& p = parent(n, q + 1);
@@ -936,24 +956,30 @@ struct Builder {
         R.tol = h5.attr_f(g, "tol");
         R.chunk = h5.attr_i(g, "iteration_chunk_size", 0, true);
         if (R.chunk < 1) fail(n.name + ": iteration_chunk_size must be positive");
-        R.maxc = cap(R.n_node, 96);
-        R.maxi = cap(R.n_node, 192);
         int nn = R.n_node;
-        long ncand = (long)nn * R.maxc;
-        R.ccnt = T.ialloc(nn);
-        R.cnbr = T.ialloc(ncand);
-        R.icnt = T.ialloc(nn);
-        R.iinc = T.ialloc((long)nn * R.maxi);
-        R.cpres = T.ialloc(ncand);
-        R.cP = T.falloc(ncand * 36);
-        R.cmsg = T.falloc(ncand * 24);
-        R.cmarg = T.falloc(ncand * 36);
+        R.ccap = (int)std::max(16.0, nn * 48.0 * T.cap_scale);
+        long cc = R.ccap;
+        R.coff = T.ialloc(nn + 1);
+        R.ca = T.ialloc(cc);
+        R.cb = T.ialloc(cc);
+        R.slot_cand = T.ialloc(G.ecap);
+        R.ioff = T.ialloc(nn + 1);
+        R.iinc = T.ialloc(2 * cc);
+        R.eidx = T.ialloc(cc + 1);
+        R.bpc = T.ialloc(cc);
+        R.bpf = T.ialloc(cc);
+        R.bps = T.ialloc(cc);
+        R.ninc = T.ialloc(nn);
+        R.binc = T.ialloc(2 * cc);
+        R.P = T.falloc(cc * 36);
+        R.msg = T.falloc(cc * 24);
+        R.marg = T.falloc(cc * 36);
+        R.e1x = T.falloc(nb);
         R.nprob = T.falloc(nn * 6);
         R.ncur = T.falloc(nn * 6);
         R.nold = T.falloc(nn * 6);
         R.noff = T.falloc(nn);
-        R.stats = T.ialloc(4);
-        R.edge_cut2 = G.cache_cut2;
+        R.stats = T.ialloc(8);
         o.ia[0] = T.ci_push(node_of);
         T.rots.push_back(R);
         o.rot = (int)T.rots.size() - 1;
@@ -1258,6 +1284,12 @@ struct Batch {
 
     void create(const vector<string>& paths, const vector<float>& temps, int n_atom_hint) {
         std::map<string, std::shared_ptr<Program>> by_path;
+        // UPSIDE_CAPACITY_SCALE (e.g. 0.05) starts with small neighbour-list
+        // capacities; overflow regrowth then doubles them as needed
+        if (const char* e = std::getenv("UPSIDE_CAPACITY_SCALE")) {
+            double v = std::atof(e);
+            if (v > 0.0 && v <= 64.0) cap_scale = v;
+        }
         hip_check(hipStreamCreate(&stream), "stream");
         hip_check(hipEventCreate(&ev0), "event");
         hip_check(hipEventCreate(&ev1), "event");
@@ -1306,7 +1338,7 @@ struct Batch {
     // momenta and thermostat counters.
     void grow() {
         cap_scale *= 2.0;
-        if (cap_scale > 64.0) fail("neighbour list capacity growth limit reached");
+        if (cap_scale > 128.0) fail("neighbour list capacity growth limit reached");
         vector<vector<float>> pos(sys.size()), mom(sys.size());
         pull_desc();
         for (size_t k = 0; k < sys.size(); ++k) {
@@ -1581,23 +1613,30 @@ int get_value_by_name(int n_output, float* output, DerivEngine* engine, const ch
     } else if (n->type == "rotamer" && ln == "bp_edge_classes") {
         // present residue-pair edges by rotamer-count class: 11 13 16 33 36 66
         const RotamerDev& R = T.rots[T.ops[n->op].rot];
-        vector<int> ccnt(R.n_node), cnbr((size_t)R.n_node * R.maxc), cpres((size_t)R.n_node * R.maxc);
+        const PairGraph& G = T.graphs[T.ops[n->op].graph];
         int* I = B.sys[0].iarena;
-        hip_check(hipMemcpy(ccnt.data(), I + R.ccnt, ccnt.size() * 4, hipMemcpyDeviceToHost), "bp");
-        hip_check(hipMemcpy(cnbr.data(), I + R.cnbr, cnbr.size() * 4, hipMemcpyDeviceToHost), "bp");
-        hip_check(hipMemcpy(cpres.data(), I + R.cpres, cpres.size() * 4, hipMemcpyDeviceToHost), "bp");
+        int st[4];
+        hip_check(hipMemcpy(st, I + G.state, sizeof(st), hipMemcpyDeviceToHost), "state");
+        int E = st[3];
+        vector<int> act(E), scand(E), row(E), nbr(E);
+        if (E) {
+            hip_check(hipMemcpy(act.data(), I + G.act, E * 4, hipMemcpyDeviceToHost), "bp");
+            hip_check(hipMemcpy(scand.data(), I + R.slot_cand, E * 4, hipMemcpyDeviceToHost), "bp");
+            hip_check(hipMemcpy(row.data(), I + G.row, E * 4, hipMemcpyDeviceToHost), "bp");
+            hip_check(hipMemcpy(nbr.data(), I + G.nbr, E * 4, hipMemcpyDeviceToHost), "bp");
+        }
         const int* nrot = T.ci.data() + R.node_nrot;
+        const int* bnode = T.ci.data() + R.bead_node;
         out.assign(6, 0.f);
         auto cls = [](int a, int b) {
             if (a > b) std::swap(a, b);
             if (a == 1) return b == 1 ? 0 : (b == 3 ? 1 : 2);
             return a == 3 ? (b == 3 ? 3 : 4) : 5;
         };
-        for (int a = 0; a < R.n_node; ++a)
-            for (int k = 0; k < std::min(ccnt[a], R.maxc); ++k) {
-                size_t ci = (size_t)a * R.maxc + k;
-                if (cpres[ci]) out[cls(nrot[a], nrot[cnbr[ci]])] += 1.f;
-            }
+        std::set<int> seen;
+        for (int s2 = 0; s2 < E; ++s2)
+            if (act[s2] && seen.insert(scand[s2]).second)
+                out[cls(nrot[bnode[row[s2]]], nrot[bnode[nbr[s2]]])] += 1.f;
     } else if (n->type == "rotamer" && ln == "node_classes") {
         const RotamerDev& R = T.rots[T.ops[n->op].rot];
         const int* nrot = T.ci.data() + R.node_nrot;
@@ -1605,31 +1644,33 @@ int get_value_by_name(int n_output, float* output, DerivEngine* engine, const ch
         for (int a = 0; a < R.n_node; ++a) out[nrot[a] == 1 ? 0 : (nrot[a] == 3 ? 1 : 2)] += 1.f;
     } else if (gidx >= 0 && (ln == "count_edges_by_type" || ln == "n_edge" || ln == "edge_pairs" || ln == "edge_values")) {
         const PairGraph& G = T.graphs[gidx];
-        long slots = (long)G.n1 * G.maxn;
-        vector<int> cnt(G.n1), nbr(slots), act(slots);
-        vector<float> val(slots);
         float* A = B.sys[0].arena;
         int* I = B.sys[0].iarena;
-        hip_check(hipMemcpy(cnt.data(), I + G.cnt, cnt.size() * 4, hipMemcpyDeviceToHost), "edges");
-        hip_check(hipMemcpy(nbr.data(), I + G.nbr, nbr.size() * 4, hipMemcpyDeviceToHost), "edges");
-        hip_check(hipMemcpy(act.data(), I + G.act, act.size() * 4, hipMemcpyDeviceToHost), "edges");
-        hip_check(hipMemcpy(val.data(), A + G.val, val.size() * 4, hipMemcpyDeviceToHost), "edges");
+        int st[4];
+        hip_check(hipMemcpy(st, I + G.state, sizeof(st), hipMemcpyDeviceToHost), "state");
+        int E = st[3];
+        vector<int> row(E), nbr(E), act(E);
+        vector<float> val(E);
+        if (E) {
+            hip_check(hipMemcpy(row.data(), I + G.row, E * 4, hipMemcpyDeviceToHost), "edges");
+            hip_check(hipMemcpy(nbr.data(), I + G.nbr, E * 4, hipMemcpyDeviceToHost), "edges");
+            hip_check(hipMemcpy(act.data(), I + G.act, E * 4, hipMemcpyDeviceToHost), "edges");
+            hip_check(hipMemcpy(val.data(), A + G.val, E * 4, hipMemcpyDeviceToHost), "edges");
+        }
         const int* t1 = T.ci.data() + G.type1;
         const int* t2 = T.ci.data() + G.type2;
         int n_type1 = 0;
         for (int i = 0; i < G.n1; ++i) n_type1 = std::max(n_type1, t1[i] + 1);
         vector<float> by_type((size_t)n_type1 * G.n_type2, 0.f);
         vector<float> pairs, vals;
-        for (int i = 0; i < G.n1; ++i)
-            for (int k = 0; k < cnt[i]; ++k) {
-                long s = (long)i * G.maxn + k;
-                if (!act[s]) continue;
-                int j = nbr[s];
-                by_type[(size_t)t1[i] * G.n_type2 + t2[j]] += 1.f;
-                pairs.push_back((float)i);
-                pairs.push_back((float)j);
-                vals.push_back(val[s]);
-            }
+        for (int s2 = 0; s2 < E; ++s2) {  // slots are in (row, partner) order
+            if (!act[s2]) continue;
+            int i = row[s2], j = nbr[s2];
+            by_type[(size_t)t1[i] * G.n_type2 + t2[j]] += 1.f;
+            pairs.push_back((float)i);
+            pairs.push_back((float)j);
+            vals.push_back(val[s2]);
+        }
         if (ln == "count_edges_by_type") out = by_type;
         else if (ln == "n_edge") out = {(float)vals.size()};
         else if (ln == "edge_pairs") out = pairs;

@@ -523,30 +523,32 @@ __device__ void infer_bwd(Ctx& c, const OpRec& op) {
     plan_gather(c, op.plans[0]);
 }
 
+// A pair-graph capacity overflow voids the evaluation: the system's error
+// word is set, the remaining ops are skipped and the host grows the
+// capacities and repeats the run.
+__device__ void graph_overflow(Ctx& c, const SysDesc& S) {
+    if (c.tid == 0) c.I[S.error] = 1;
+    __syncthreads();
+}
+
 // ---- protein_hbond (src/hbond.cpp:357-435) ---------------------------------------
-__device__ void protein_hbond_fwd(Ctx& c, const OpRec& op, const PairGraph& G) {
-    pg_refresh(c, G, op);
+__device__ void protein_hbond_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G) {
+    if (!pg_refresh(c, G, op)) { graph_overflow(c, S); return; }
     int nd = G.n1;
     const int* loc1 = c.CI + G.loc1;
     const int* loc2 = c.CI + G.loc2;
-    const int* cnt = c.I + G.cnt;
-    const int* tcnt = c.I + G.tcnt;
-    const int* tslot = c.I + G.tslot;
-    const float* val = c.A + G.val;
-    const int* act = c.I + G.act;
+    float* ss = c.A + op.scratch;  // per-site sum of hb_log
+    pg_row_sums(c, G, false, ss, 1);
+    pg_row_sums(c, G, true, ss + nd, 1);
+    __syncthreads();
     for (int v = c.tid; v < op.n_elem; v += NT) {
         bool don = v < nd;
         int i = don ? v : v - nd;
         const float* src = IN(op, 0) + (don ? loc1[i] : loc2[i]) * op.in_w[0];
         float* o = OUT(op) + v * 7;
+#pragma unroll
         for (int k = 0; k < 6; ++k) o[k] = src[k];
-        float hl = 0.f;
-        if (don) {
-            for (int k = 0; k < cnt[i]; ++k) { int s = i * G.maxn + k; if (act[s]) hl += val[s]; }
-        } else {
-            for (int t = 0; t < tcnt[i]; ++t) { int s = tslot[i * G.maxt + t]; if (act[s]) hl += val[s]; }
-        }
-        o[6] = 1.f - expf(-hl);
+        o[6] = 1.f - expf(-ss[v]);
     }
 }
 __device__ void protein_hbond_bwd(Ctx& c, const OpRec& op, const PairGraph& G) {
@@ -554,51 +556,36 @@ __device__ void protein_hbond_bwd(Ctx& c, const OpRec& op, const PairGraph& G) {
     float* ss = c.A + op.scratch;
     for (int v = c.tid; v < op.n_elem; v += NT) ss[v] = SENS(op)[v * 7 + 6] * (1.f - OUT(op)[v * 7 + 6]);
     __syncthreads();
-    const int* cnt = c.I + G.cnt;
+    const int* row = c.I + G.row;
     const int* nbr = c.I + G.nbr;
     float* es = c.A + G.esens;
-    int wave = c.tid >> 6, lane = c.tid & 63;
-    for (int i = wave; i < G.n1; i += NWAVE)
-        for (int k = lane; k < cnt[i]; k += 64) {
-            int s = i * G.maxn + k;
-            es[s] = ss[i] + ss[nd + nbr[s]];
-        }
+    const int E = c.I[G.state + 3];
+    for (int s = c.tid; s < E; s += NT) es[s] = ss[row[s]] + ss[nd + nbr[s]];
     __syncthreads();
     pg_gather(c, G);
     // pass the coordinate sensitivities through to infer_H_O (:419-433)
     const int* loc1 = c.CI + G.loc1;
     const int* loc2 = c.CI + G.loc2;
     for (int i = c.tid; i < G.n1; i += NT)
+#pragma unroll
         for (int k = 0; k < 6; ++k) INS(op, 0)[loc1[i] * op.in_w[0] + k] += SENS(op)[i * 7 + k];
     __syncthreads();
     for (int j = c.tid; j < G.n2; j += NT)
+#pragma unroll
         for (int k = 0; k < 6; ++k) INS(op, 0)[loc2[j] * op.in_w[0] + k] += SENS(op)[(nd + j) * 7 + k];
     __syncthreads();
 }
 
 // ---- hbond_coverage (src/hbond.cpp:438-484) ----------------------------------------
-__device__ void hbond_coverage_fwd(Ctx& c, const OpRec& op, const PairGraph& G) {
-    pg_refresh(c, G, op);
-    const int* tcnt = c.I + G.tcnt;
-    const int* tslot = c.I + G.tslot;
-    const float* val = c.A + G.val;
-    const int* act = c.I + G.act;
-    for (int j = c.tid; j < G.n2; j += NT) {
-        float acc = 0.f;
-        for (int t = 0; t < tcnt[j]; ++t) { int s = tslot[j * G.maxt + t]; if (act[s]) acc += val[s]; }
-        OUT(op)[j] = acc;
-    }
+__device__ void hbond_coverage_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G) {
+    if (!pg_refresh(c, G, op)) { graph_overflow(c, S); return; }
+    pg_row_sums(c, G, true, OUT(op), 1);
 }
 __device__ void hbond_coverage_bwd(Ctx& c, const OpRec& op, const PairGraph& G) {
-    const int* cnt = c.I + G.cnt;
     const int* nbr = c.I + G.nbr;
     float* es = c.A + G.esens;
-    int wave = c.tid >> 6, lane = c.tid & 63;
-    for (int i = wave; i < G.n1; i += NWAVE)
-        for (int k = lane; k < cnt[i]; k += 64) {
-            int s = i * G.maxn + k;
-            es[s] = SENS(op)[nbr[s]];
-        }
+    const int E = c.I[G.state + 3];
+    for (int s = c.tid; s < E; s += NT) es[s] = SENS(op)[nbr[s]];
     __syncthreads();
     pg_gather(c, G);
 }
@@ -682,59 +669,49 @@ __device__ void weighted_pos_bwd(Ctx& c, const OpRec& op) {
 }
 
 // ---- environment_coverage (src/environment.cpp:288-348) -----------------------------
-__device__ void env_coverage_fwd(Ctx& c, const OpRec& op, const PairGraph& G) {
-    pg_refresh(c, G, op);
+// Output row i holds the coverage of element i by each residue type: the
+// active edge values of its slots binned by the partner's amino-acid type.
+__device__ void env_coverage_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G) {
+    if (!pg_refresh(c, G, op)) { graph_overflow(c, S); return; }
     int naa = op.ia[0];
     const int* aa = CIA(op, 1);
     const int* id2 = c.CI + G.id2;
-    const int* cnt = c.I + G.cnt;
+    const int* off = c.I + G.off;
     const int* nbr = c.I + G.nbr;
     const float* val = c.A + G.val;
     const int* act = c.I + G.act;
-    for (int i = c.tid; i < G.n1; i += NT) {
-        float* o = OUT(op) + i * naa;
-        for (int a = 0; a < naa; ++a) o[a] = 0.f;
-        for (int k = 0; k < cnt[i]; ++k) {
-            int s = i * G.maxn + k;
-            if (act[s]) o[aa[id2[nbr[s]]]] += val[s];
-        }
+    // one thread per (element, type): scan the element's slots in order
+    for (int k = c.tid; k < G.n1 * naa; k += NT) {
+        int i = k / naa, t = k - i * naa;
+        float acc = 0.f;
+        for (int s = off[i]; s < off[i + 1]; ++s)
+            if (act[s] && aa[id2[nbr[s]]] == t) acc += val[s];
+        OUT(op)[k] = acc;
     }
 }
 __device__ void env_coverage_bwd(Ctx& c, const OpRec& op, const PairGraph& G) {
     int naa = op.ia[0];
     const int* aa = CIA(op, 1);
     const int* id2 = c.CI + G.id2;
-    const int* cnt = c.I + G.cnt;
+    const int* row = c.I + G.row;
     const int* nbr = c.I + G.nbr;
     float* es = c.A + G.esens;
-    int wave = c.tid >> 6, lane = c.tid & 63;
-    for (int i = wave; i < G.n1; i += NWAVE)
-        for (int k = lane; k < cnt[i]; k += 64) {
-            int s = i * G.maxn + k;
-            es[s] = SENS(op)[i * naa + aa[id2[nbr[s]]]];
-        }
+    const int E = c.I[G.state + 3];
+    for (int s = c.tid; s < E; s += NT) es[s] = SENS(op)[row[s] * naa + aa[id2[nbr[s]]]];
     __syncthreads();
     pg_gather(c, G);
 }
 
 // ---- hb_environment_coverage (src/environment.cpp:350-385) ------------------------
-__device__ void hb_env_fwd(Ctx& c, const OpRec& op, const PairGraph& G) {
-    pg_refresh(c, G, op);
-    const int* cnt = c.I + G.cnt;
-    const float* val = c.A + G.val;
-    const int* act = c.I + G.act;
-    for (int i = c.tid; i < G.n1; i += NT) {
-        float acc = 0.f;
-        for (int k = 0; k < cnt[i]; ++k) { int s = i * G.maxn + k; if (act[s]) acc += val[s]; }
-        OUT(op)[i] = acc;
-    }
+__device__ void hb_env_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G) {
+    if (!pg_refresh(c, G, op)) { graph_overflow(c, S); return; }
+    pg_row_sums(c, G, false, OUT(op), 1);
 }
 __device__ void hb_env_bwd(Ctx& c, const OpRec& op, const PairGraph& G) {
-    const int* cnt = c.I + G.cnt;
+    const int* row = c.I + G.row;
     float* es = c.A + G.esens;
-    int wave = c.tid >> 6, lane = c.tid & 63;
-    for (int i = wave; i < G.n1; i += NWAVE)
-        for (int k = lane; k < cnt[i]; k += 64) es[i * G.maxn + k] = SENS(op)[i];
+    const int E = c.I[G.state + 3];
+    for (int s = c.tid; s < E; s += NT) es[s] = SENS(op)[row[s]];
     __syncthreads();
     pg_gather(c, G);
 }
@@ -792,18 +769,16 @@ __device__ void bb_sigmoid_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
 
 // ---- backbone_pairs (src/backbone_steric.cpp:38-146) --------------------------------
 __device__ void backbone_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G) {
-    pg_refresh(c, G, op);
-    const int* cnt = c.I + G.cnt;
+    if (!pg_refresh(c, G, op)) { graph_overflow(c, S); return; }
     const float* val = c.A + G.val;
     const int* act = c.I + G.act;
     float* es = c.A + G.esens;
+    const int E = c.I[G.state + 3];
     float pot = 0.f;
-    for (int i = c.tid; i < G.n1; i += NT)
-        for (int k = 0; k < cnt[i]; ++k) {
-            int s = i * G.maxn + k;
-            es[s] = 1.f;
-            if (act[s]) pot += val[s];
-        }
+    for (int s = c.tid; s < E; s += NT) {
+        es[s] = 1.f;
+        if (act[s]) pot += val[s];
+    }
     pot = block_sum(c, pot);
     put_pot(c, S, op, pot);
     pg_gather(c, G);
@@ -812,10 +787,14 @@ __device__ void backbone_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const Pa
 // ---- rotamer (src/rotamer.cpp:840-850) ----------------------------------------------
 __device__ void rotamer_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G,
                             const RotamerDev& R, int mode) {
-    bool rebuilt = pg_pack_and_check(c, G);
+    bool rebuild = pg_pack_and_check(c, G);
     unsigned long long t0 = S.prof ? wall_clock64() : 0;
-    if (rebuilt) { pg_rebuild(c, G); rot_rebuild(c, G, R); }
+    if (rebuild) {
+        pg_rebuild(c, G);
+        if (c.I[G.state + 0] && !rot_rebuild(c, G, R)) {}
+    }
     prof_add(S, S.n_germ + S.n_deriv + PROF_REBUILD, t0);
+    if (!c.I[G.state + 0]) { graph_overflow(c, S); return; }
     pg_edges(c, G, op);
     rot_fill(c, G, R);
     t0 = S.prof ? wall_clock64() : 0;
@@ -828,9 +807,9 @@ __device__ void rotamer_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const Pai
         st[2] += it;
         st[3] += 1;
     }
-    rot_sens(c, G, R, CIA(op, 0));
+    rot_sens(c, G, R);
     pg_gather(c, G);
-    float fe = mode ? rot_free_energy(c, R) : 0.f;
+    float fe = mode ? rot_free_energy(c, G, R) : 0.f;
     put_pot(c, S, op, fe);
 }
 
@@ -848,13 +827,13 @@ __device__ void run_value(Ctx& c, const SysDesc& S, const OpRec& op, int mode) {
     case OP_AFFINE: affine_fwd(c, op); break;
     case OP_PLACEMENT: placement_fwd(c, op); break;
     case OP_INFER_HO: infer_fwd(c, op); break;
-    case OP_PROTEIN_HBOND: protein_hbond_fwd(c, op, *G); break;
-    case OP_HBOND_COVERAGE: hbond_coverage_fwd(c, op, *G); break;
+    case OP_PROTEIN_HBOND: protein_hbond_fwd(c, S, op, *G); break;
+    case OP_HBOND_COVERAGE: hbond_coverage_fwd(c, S, op, *G); break;
     case OP_HBOND_ENERGY: hbond_energy_fwd(c, S, op); break;
     case OP_ROTAMER: rotamer_fwd(c, S, op, *G, S.rots[op.rot], mode); break;
     case OP_WEIGHTED_POS: weighted_pos_fwd(c, op); break;
-    case OP_ENV_COVERAGE: env_coverage_fwd(c, op, *G); break;
-    case OP_HB_ENV_COVERAGE: hb_env_fwd(c, op, *G); break;
+    case OP_ENV_COVERAGE: env_coverage_fwd(c, S, op, *G); break;
+    case OP_HB_ENV_COVERAGE: hb_env_fwd(c, S, op, *G); break;
     case OP_SIGMOID_COUPLING: sigmoid_coupling_fwd(c, S, op); break;
     case OP_BB_SIGMOID_COUPLING: bb_sigmoid_fwd(c, S, op); break;
     case OP_BACKBONE_PAIRS: backbone_fwd(c, S, op, *G); break;
@@ -896,6 +875,7 @@ __device__ void compute(Ctx& c, const SysDesc& S, int mode) {
         unsigned long long t0 = S.prof ? wall_clock64() : 0;
         run_value(c, S, S.ops[k], mode);
         prof_add(S, k, t0);
+        if (c.I[S.error]) return;  // uniform: read after the op's final barrier
     }
     for (int k = 0; k < S.n_deriv; ++k) {
         unsigned long long t0 = S.prof ? wall_clock64() : 0;
@@ -968,8 +948,10 @@ __device__ float total_potential(Ctx& c, const SysDesc& S) {
 
 __global__ void __launch_bounds__(NT) engine_kernel(SysDesc* systems, RunArgs args) {
     __shared__ float red[NWAVE];
+    __shared__ int ired[NWAVE];
+    extern __shared__ float lds[];
     SysDesc& S = systems[blockIdx.x];
-    Ctx c{S.arena, S.iarena, S.cf, S.ci, red, (int)threadIdx.x};
+    Ctx c{S.arena, S.iarena, S.cf, S.ci, red, ired, lds, (int)threadIdx.x};
     if (args.n_round == 0) {
         compute(c, S, args.mode);
         return;
@@ -986,6 +968,7 @@ __global__ void __launch_bounds__(NT) engine_kernel(SysDesc* systems, RunArgs ar
         int round = args.first_round + rr;
         if (args.frame_interval > 0 && round % args.frame_interval == 0) {
             compute(c, S, 1);
+            if (c.I[S.error]) return;
             if (args.frame_pot && c.tid == 0 && nf < args.n_frames)
                 args.frame_pot[blockIdx.x * args.n_frames + nf] = total_potential(c, S);
             ++nf;
@@ -996,6 +979,7 @@ __global__ void __launch_bounds__(NT) engine_kernel(SysDesc* systems, RunArgs ar
         prof_add(S, S.n_germ + S.n_deriv + PROF_THERMO, t0);
         for (int stage = 0; stage < 3; ++stage) {
             compute(c, S, 0);
+            if (c.I[S.error]) return;  // capacity overflow: the host repeats the run
             t0 = S.prof ? wall_clock64() : 0;
             for (int i = c.tid; i < S.n_atom * 3; i += NT) {
                 float p = mom[i] - args.dt * der[i];
@@ -1012,6 +996,14 @@ __global__ void __launch_bounds__(NT) engine_kernel(SysDesc* systems, RunArgs ar
 
 // Host-side launcher (C++ linkage inside the library only).
 hipError_t upside_launch_engine(upside::SysDesc* d_systems, int n_sys, upside::RunArgs args, hipStream_t stream) {
-    hipLaunchKernelGGL(upside::engine_kernel, dim3(n_sys), dim3(upside::NT), 0, stream, d_systems, args);
+    static bool attr_set = false;
+    const size_t lds_bytes = upside::LDS_FLOATS * sizeof(float);
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)upside::engine_kernel,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(upside::engine_kernel, dim3(n_sys), dim3(upside::NT), lds_bytes, stream, d_systems, args);
     return hipGetLastError();
 }

@@ -96,6 +96,15 @@ int upside_batch_bp_stats(UpsideBatch* batch, int sys, int* last_iters, long lon
                           long long* solves, long long* bad_solves);
 /* Synchronize and time the device (for benchmarks): elapsed ms of the last run. */
 float upside_batch_last_run_ms(UpsideBatch* batch);
+/* Workgroups cooperating on each system ("team"; 1 = one workgroup per
+ * system, no grid barriers).  max_team is the largest team every system can
+ * have with all workgroups resident at once.  autotune runs n_round rounds per
+ * candidate team size from the same state, restores the state and keeps the
+ * fastest; ms_out[n_candidates] (may be NULL) receives the kernel times. */
+int upside_batch_max_team(UpsideBatch* batch);
+int upside_batch_set_team(UpsideBatch* batch, int team);
+int upside_batch_get_team(UpsideBatch* batch);
+int upside_batch_autotune(UpsideBatch* batch, int n_round, const int* candidates, int n_candidates, float* ms_out);
 /* Per-node wall-clock accumulation inside the kernel (the analogue of the
  * reference's -DCOLLECT_PROFILE timers, src/timing.cpp:11-54).  The report is
  * a text table; returns the needed size if buflen is too small. */

@@ -292,3 +292,22 @@ def test_capacity_regrowth_is_transparent(ue, chain_file, monkeypatch):
     b1.run(5)
     for s in range(2):
         np.testing.assert_array_equal(b1.pos(s), b0.pos(s))
+
+
+@pytest.mark.parametrize("team", [2, 8])
+def test_team_execution_matches_single_workgroup(ue, chain_file, team):
+    """A system evaluated by a team of workgroups (grid barriers between
+    phases) follows the single-workgroup trajectory bit for bit: forces are
+    sums in a canonical order that does not depend on the team size."""
+    path = chain_file(120, 71)
+    say("case: team %d vs single workgroup, L=120" % team)
+    b1 = ue.Batch([path] * 2, [0.85, 0.9], seed=6)
+    b1.run(4)
+    bt = ue.Batch([path] * 2, [0.85, 0.9], seed=6)
+    if team > bt.max_team():
+        pytest.skip("team %d exceeds resident limit %d" % (team, bt.max_team()))
+    bt.set_team(team)
+    bt.run(4)
+    for s in range(2):
+        np.testing.assert_array_equal(bt.pos(s), b1.pos(s))
+    np.testing.assert_allclose(bt.energies(), b1.energies(), rtol=1e-5)

@@ -11,6 +11,10 @@ namespace upside {
 constexpr int NT = 512;            // threads per workgroup (8 wave64)
 constexpr int NWAVE = NT / 64;
 
+// Execution context of one workgroup.  A system is evaluated by a team of G
+// workgroups (G = 1: a single workgroup, no grid barriers).  Data-parallel
+// loops stride over the team's threads (gtid, gsize); team_sync separates
+// phases whose data cross workgroups.
 struct Ctx {
     float* A;          // per-system float arena
     int* I;            // per-system int arena
@@ -19,8 +23,58 @@ struct Ctx {
     float* red;        // LDS reduction scratch [NWAVE]
     int* ired;         // LDS reduction scratch [NWAVE] (ints)
     float* lds;        // LDS staging area [LDS_FLOATS]
-    int tid;
+    int tid;           // thread in the workgroup
+    unsigned long long* prof;  // wall-clock accumulators (null: profiling off)
+    int pbase;                 // first fixed profile slot
+    int rank, G;       // workgroup rank in the team, team size
+    int gtid, gsize;   // thread in the team, team threads
+    unsigned* bar;     // team barrier words [count, generation, spin-timeout flag]
+    unsigned gen;      // generation this workgroup expects to complete next
+    float* tred;       // team reduction scratch, 2 buffers of G floats
+    int tpar;          // buffer parity of the next team reduction
 };
+
+__device__ __forceinline__ unsigned long long ptick(const Ctx& c) { return c.prof ? wall_clock64() : 0ull; }
+__device__ __forceinline__ void pacc(const Ctx& c, int slot, unsigned long long t0) {
+    if (c.prof && c.gtid == 0) c.prof[c.pbase + slot] += wall_clock64() - t0;
+}
+
+// Team barrier: every workgroup of the team arrives (agent-scope release of
+// its writes), the last one to arrive opens the next generation, the others
+// wait for it (agent-scope acquire).  The wait is bounded: after ~1 s of
+// spinning the workgroup records a timeout and proceeds, so a launch always
+// drains; the host then reports the run as failed.
+constexpr unsigned BAR_SPIN_LIMIT = 1u << 24;
+
+__device__ void team_sync(Ctx& c) {
+    __syncthreads();
+    if (c.G > 1) {
+        if (c.tid == 0) {
+            unsigned g = c.gen;
+            __threadfence();
+            unsigned prev = __hip_atomic_fetch_add(&c.bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (prev == (unsigned)c.G - 1u) {
+                __hip_atomic_store(&c.bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&c.bar[1], g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                unsigned spins = 0;
+                while (__hip_atomic_load(&c.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                    __builtin_amdgcn_s_sleep(2);
+                    // after one timeout anywhere in the team no barrier waits again,
+                    // so the launch drains quickly
+                    if (++spins > BAR_SPIN_LIMIT ||
+                        __hip_atomic_load(&c.bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                        __hip_atomic_store(&c.bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                }
+            }
+            __threadfence();
+        }
+        c.gen += 1u;
+        __syncthreads();
+    }
+}
 
 __device__ __forceinline__ float wave_sum(float v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -58,6 +112,36 @@ __device__ int block_or(Ctx& c, int v) {
     return any;
 }
 
+// Team reductions: workgroup partials combined in rank order (fixed order, so
+// the result is the same on every run).  All threads of the team.
+__device__ float team_sum(Ctx& c, float v) {
+    float t = block_sum(c, v);
+    if (c.G == 1) return t;
+    float* buf = c.tred + c.tpar * c.G;
+    c.tpar ^= 1;
+    if (c.tid == 0) buf[c.rank] = t;
+    team_sync(c);
+    float s = 0.f;
+    for (int r = 0; r < c.G; ++r) s += buf[r];
+    return s;
+}
+__device__ float team_max(Ctx& c, float v) {
+    float t = block_max(c, v);
+    if (c.G == 1) return t;
+    float* buf = c.tred + c.tpar * c.G;
+    c.tpar ^= 1;
+    if (c.tid == 0) buf[c.rank] = t;
+    team_sync(c);
+    float s = buf[0];
+    for (int r = 1; r < c.G; ++r) s = fmaxf(s, buf[r]);
+    return s;
+}
+__device__ int team_or(Ctx& c, int v) {
+    int t = __syncthreads_or(v);
+    if (c.G == 1) return t;
+    return team_max(c, t ? 1.f : 0.f) > 0.f ? 1 : 0;
+}
+
 // In-place exclusive scan of the ints a[0..n) (any memory); returns the
 // total.  Each thread scans a contiguous chunk, chunk totals are scanned
 // across the block.  All threads must call it.
@@ -89,6 +173,29 @@ __device__ int block_exscan(Ctx& c, int* a, int n) {
     }
     __syncthreads();
     return tot;
+}
+
+// Team-wide in-place exclusive scan of the ints a[0..n) in global memory;
+// returns the total.  Each workgroup scans a contiguous chunk, chunk totals
+// are added in rank order.  All threads of the team.
+__device__ int team_exscan(Ctx& c, int* a, int n) {
+    if (c.G == 1) return block_exscan(c, a, n);
+    int per = (n + c.G - 1) / c.G;
+    int b = min(c.rank * per, n), e = min(b + per, n);
+    int tot = block_exscan(c, a + b, e - b);
+    float* buf = c.tred + c.tpar * c.G;
+    c.tpar ^= 1;
+    if (c.tid == 0) buf[c.rank] = __int_as_float(tot);
+    team_sync(c);
+    int before = 0, all = 0;
+    for (int r = 0; r < c.G; ++r) {
+        int x = __float_as_int(buf[r]);
+        if (r < c.rank) before += x;
+        all += x;
+    }
+    for (int i = b + c.tid; i < e; i += NT) a[i] += before;
+    team_sync(c);
+    return all;
 }
 
 // Sum over aligned groups of G lanes (G a power of two <= 64), fixed order;
@@ -184,7 +291,7 @@ __device__ void plan_gather(Ctx& c, int plan) {
     const int* idx = off + nt + 1;
     float* sens = c.A + so;
     const float* con = c.A + cb;
-    for (int t = c.tid; t < nt; t += NT) {
+    for (int t = c.gtid; t < nt; t += c.gsize) {
         int b = off[t], e = off[t + 1];
         if (b == e) continue;
         for (int k = 0; k < cw; ++k) {
@@ -193,7 +300,7 @@ __device__ void plan_gather(Ctx& c, int plan) {
             sens[t * ss + col + k] += acc;
         }
     }
-    __syncthreads();
+    team_sync(c);
 }
 
 }  // namespace upside

@@ -1,4 +1,4 @@
-// Pair graphs on the device: Verlet cache from a hashed cell grid, exact edge
+// Pair graphs on the device: Verlet cache from LDS-tiled scans, exact edge
 // refine, per-edge spline energies and the fixed-order sensitivity gather.
 //
 // Reference: PairlistComputation / InteractionGraph (src/interaction_graph.h).
@@ -10,16 +10,12 @@
 // src/interaction_graph.h:51-169) only bounds which pairs are examined, so the
 // edge set does not depend on how the cache was built.
 //
-// Rebuild (all in one workgroup):
-//   1. hash the side-2 elements into cells of edge length >= cache radius
-//      (counting sort into buckets);
-//   2. every row scans the 27 neighbouring cells twice: once to count its
-//      cache partners, once (after a block scan of the counts) to write them;
-//   3. each row is put in increasing partner order by a wave-cooperative rank
-//      sort, so the slot order is canonical whatever order the grid produced;
-//   4. the transposed rows are built the same way from the other side
-//      (symmetric graphs: the lower partners of the same scan) and each entry
-//      is mapped to its forward slot by binary search in the sorted row.
+// Rebuild (all in one workgroup): the other side is put into a hashed cell
+// grid in LDS (or streamed in tiles when it does not fit) and every row scans
+// it twice, once to count partners and, after a block scan of the counts,
+// once to write them.  Row order is canonical (fixed cell-visit order, index
+// order inside a cell); transposed rows are scanned the same way from the
+// other side and mapped to forward slots through a hash table.
 #pragma once
 #include "dev_common.h"
 
@@ -51,14 +47,14 @@ __device__ bool pg_pack_and_check(Ctx& c, const PairGraph& G) {
     int n = G.sym ? G.n1 : G.n1 + G.n2;
     int valid = c.I[G.state];
     int moved = 0;
-    for (int i = c.tid; i < n; i += NT) {
+    for (int i = c.gtid; i < n; i += c.gsize) {
         const float* src = (i < G.n1) ? P1 + loc1[i] * G.w1 : P2 + loc2[i - G.n1] * G.w2;
         float x = src[0], y = src[1], z = src[2];
         pack[i * 3 + 0] = x; pack[i * 3 + 1] = y; pack[i * 3 + 2] = z;
         float dx = x - cache[i * 3 + 0], dy = y - cache[i * 3 + 1], dz = z - cache[i * 3 + 2];
         if (!(dx * dx + dy * dy + dz * dz <= G.drift2)) moved = 1;  // NaN also forces a rebuild
     }
-    int any = __syncthreads_or(moved | (valid ? 0 : 1));
+    int any = team_or(c, moved | (valid ? 0 : 1));
     return any != 0;
 }
 
@@ -71,10 +67,81 @@ __device__ int lower_bound_i(const int* a, int n, int v) {
     return lo;
 }
 
-// ---- hashed cell grid ---------------------------------------------------------
-struct Grid {
+// ---- cache rebuild: LDS-tiled scans ------------------------------------------
+// Fallback scan: the other side streamed through LDS in tiles; every wave
+// reads the same tile entries (LDS broadcast), each lane tests its own query.
+// Partners come out in increasing index.
+constexpr int TILE = LDS_FLOATS / 4;
+
+// Stage elements [t0, t0+n) of side pts (ids in id) as float4 (x, y, z, id).
+__device__ void stage_tile(Ctx& c, const float* pts, const int* id, int t0, int n) {
+    float4* tile = reinterpret_cast<float4*>(c.lds);
+    for (int k = c.tid; k < n; k += NT) {  // every workgroup stages its own copy
+        int j = t0 + k;
+        tile[k] = make_float4(pts[j * 3], pts[j * 3 + 1], pts[j * 3 + 2], __int_as_float(id[j]));
+    }
+    __syncthreads();
+}
+
+// One scan of all queries q in [0, nq) (positions qp, ids qid) against all
+// elements of the other side (positions ep, ids eid, count ne), for accepted
+// pairs within the cache radius.  SYM: partners j > q go to the "up" list, j < q
+// to the "lo" list (j == q skipped); otherwise every partner is "up".
+// FILL=false adds the counts into up_n[q] / lo_n[q]; FILL=true writes the
+// partners, in increasing j, at cursors up_n[q] / lo_n[q] (advanced).  Counts
+// and cursors live in registers while a row is scanned.  All threads.
+template <bool SYM, bool FILL>
+__device__ void pair_scan(Ctx& c, const PairGraph& G, const float* qp, const int* qid, int nq, const float* ep,
+                          const int* eid, int ne, int* up_n, int* lo_n, int* up_dst, int* lo_dst) {
+    const float4* tile = reinterpret_cast<const float4*>(c.lds);
+    const float cc2 = G.cache_cut2;
+    for (int t0 = 0; t0 < ne; t0 += TILE) {
+        int tn = min(TILE, ne - t0);
+        stage_tile(c, ep, eid, t0, tn);
+        for (int q = c.gtid; q < nq; q += c.gsize) {
+            float x0 = qp[q * 3], x1 = qp[q * 3 + 1], x2 = qp[q * 3 + 2];
+            int a = qid[q];
+            int up = FILL ? up_n[q] : 0;
+            int lo = (FILL && SYM) ? lo_n[q] : 0;
+            for (int k = 0; k < tn; ++k) {
+                float4 e = tile[k];
+                float d2 = refine_dist2(x0 - e.x, x1 - e.y, x2 - e.z);
+                int j = t0 + k;
+                if (d2 < cc2 && !(SYM && j == q) && pg_accept(G, a, __float_as_int(e.w))) {
+                    if (!SYM || j > q) {
+                        if (FILL) up_dst[up] = j;
+                        ++up;
+                    } else {
+                        if (FILL) lo_dst[lo] = j;
+                        ++lo;
+                    }
+                }
+            }
+            if (FILL) {
+                up_n[q] = up;
+                if (SYM) lo_n[q] = lo;
+            } else {
+                up_n[q] += up;
+                if (SYM) lo_n[q] += lo;
+            }
+        }
+        __syncthreads();  // the tile is restaged next
+    }
+    team_sync(c);
+}
+
+// ---- cache rebuild: cell grid in LDS ----------------------------------------
+// The other side is counting-sorted into hashed cells of edge >= the cache
+// radius, kept in LDS as (bucket offsets, float4 elements (x, y, z, index)),
+// each bucket sorted by element index.  A query visits its 27 neighbouring
+// cells in a fixed order, so rows come out in a canonical order.  When the
+// grid does not fit in LDS the scan streams the other side in tiles instead
+// (same emission semantics, partners in increasing index).
+struct LGrid {
     float ox, oy, oz, inv;
     int mask;
+    int* ofs;      // [mask+2]
+    float4* el;    // [m]
 };
 
 __device__ __forceinline__ int cell_coord(float x, float o, float inv) {
@@ -82,93 +149,143 @@ __device__ __forceinline__ int cell_coord(float x, float o, float inv) {
     float t = fminf(fmaxf((x - o) * inv, -1.0e6f), 1.0e6f);
     return (int)floorf(t);
 }
-__device__ __forceinline__ int cell_bucket(const Grid& g, int cx, int cy, int cz) {
+__device__ __forceinline__ int cell_bucket(int mask, int cx, int cy, int cz) {
     uint32_t h = ((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u);
-    return (int)(h & (uint32_t)g.mask);
+    return (int)(h & (uint32_t)mask);
 }
 
-// Bucket the m points pts[3*k] (counting sort): elements of bucket b are
-// gidx[gofs[b] .. gofs[b+1]).  All threads.
-__device__ Grid grid_build(Ctx& c, const PairGraph& G, const float* pts, int m) {
+__device__ __forceinline__ int lgrid_buckets(int m) {
+    int nb = 64;
+    while (nb < m && nb < 4096) nb *= 2;
+    return nb;
+}
+__device__ __forceinline__ bool lgrid_fits(int m) {
+    int nb = lgrid_buckets(m);
+    return (nb + 4) * 4 + m * 16 <= LDS_FLOATS * 4;
+}
+
+// Build the grid of the m points pts (stride 3) in this workgroup's LDS
+// (every workgroup of a team builds its own copy).  All threads of the workgroup.
+__device__ LGrid lgrid_build(Ctx& c, const PairGraph& G, const float* pts, int m) {
     float mn0 = 3.0e38f, mn1 = 3.0e38f, mn2 = 3.0e38f;
     for (int i = c.tid; i < m; i += NT) {
         mn0 = fminf(mn0, pts[i * 3]);
         mn1 = fminf(mn1, pts[i * 3 + 1]);
         mn2 = fminf(mn2, pts[i * 3 + 2]);
     }
-    Grid g;
+    LGrid g;
     g.ox = -block_max(c, -mn0);
     g.oy = -block_max(c, -mn1);
     g.oz = -block_max(c, -mn2);
     // cells slightly larger than the cache radius: rounding can never put two
     // points closer than the radius more than one cell apart
     g.inv = 1.f / (G.cache_cut * 1.0001f);
-    g.mask = G.nbucket - 1;
-    int* gofs = c.I + G.gofs;
-    int* gidx = c.I + G.gidx;
-    for (int b = c.tid; b <= G.nbucket; b += NT) gofs[b] = 0;
+    int nb = lgrid_buckets(m);
+    g.mask = nb - 1;
+    g.ofs = reinterpret_cast<int*>(c.lds);
+    g.el = reinterpret_cast<float4*>(c.lds + ((nb + 4) & ~3));
+    for (int b = c.tid; b <= nb; b += NT) g.ofs[b] = 0;
     __syncthreads();
     for (int i = c.tid; i < m; i += NT) {
-        int b = cell_bucket(g, cell_coord(pts[i * 3], g.ox, g.inv), cell_coord(pts[i * 3 + 1], g.oy, g.inv),
+        int b = cell_bucket(g.mask, cell_coord(pts[i * 3], g.ox, g.inv), cell_coord(pts[i * 3 + 1], g.oy, g.inv),
                             cell_coord(pts[i * 3 + 2], g.oz, g.inv));
-        atomicAdd(&gofs[b + 1], 1);
+        atomicAdd(&g.ofs[b + 1], 1);
     }
     __syncthreads();
-    block_exscan(c, gofs + 1, G.nbucket);  // gofs[b+1] = start of bucket b; gofs[0] stays 0
+    block_exscan(c, g.ofs + 1, nb);  // ofs[b+1] = start of bucket b
     for (int i = c.tid; i < m; i += NT) {
-        int b = cell_bucket(g, cell_coord(pts[i * 3], g.ox, g.inv), cell_coord(pts[i * 3 + 1], g.oy, g.inv),
-                            cell_coord(pts[i * 3 + 2], g.oz, g.inv));
-        int p = atomicAdd(&gofs[b + 1], 1);  // ends at the start of bucket b+1
-        gidx[p] = i;
+        float x = pts[i * 3], y = pts[i * 3 + 1], z = pts[i * 3 + 2];
+        int b = cell_bucket(g.mask, cell_coord(x, g.ox, g.inv), cell_coord(y, g.oy, g.inv), cell_coord(z, g.oz, g.inv));
+        int p = atomicAdd(&g.ofs[b + 1], 1);  // ends at the start of bucket b+1
+        g.el[p] = make_float4(x, y, z, __int_as_float(i));
+    }
+    __syncthreads();
+    // canonical order inside every bucket: by element index
+    for (int b = c.tid; b < nb; b += NT) {
+        int lo = g.ofs[b], hi = g.ofs[b + 1];
+        for (int k = lo + 1; k < hi; ++k) {
+            float4 v = g.el[k];
+            int key = __float_as_int(v.w);
+            int t = k - 1;
+            while (t >= lo && __float_as_int(g.el[t].w) > key) {
+                g.el[t + 1] = g.el[t];
+                --t;
+            }
+            g.el[t + 1] = v;
+        }
     }
     __syncthreads();
     return g;
 }
 
-// Visit every point of the grid whose cell is adjacent to (x,y,z)'s cell:
-// f(j, yx, yy, yz).  Each point is visited once (hash collisions are
-// filtered by recomputing the point's own cell).
-template <class F>
-__device__ __forceinline__ void grid_visit(const Ctx& c, const PairGraph& G, const Grid& g, const float* pts,
-                                           float x, float y, float z, F&& f) {
-    const int* gofs = c.I + G.gofs;
-    const int* gidx = c.I + G.gidx;
-    int cx = cell_coord(x, g.ox, g.inv), cy = cell_coord(y, g.oy, g.inv), cz = cell_coord(z, g.oz, g.inv);
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                int ax = cx + dx, ay = cy + dy, az = cz + dz;
-                int b = cell_bucket(g, ax, ay, az);
-                int e1 = gofs[b + 1];
-                for (int t = gofs[b]; t < e1; ++t) {
-                    int j = gidx[t];
-                    float px = pts[j * 3], py = pts[j * 3 + 1], pz = pts[j * 3 + 2];
-                    if (cell_coord(px, g.ox, g.inv) != ax || cell_coord(py, g.oy, g.inv) != ay ||
-                        cell_coord(pz, g.oz, g.inv) != az)
-                        continue;
-                    f(j, px, py, pz);
+// Query rows against the grid; same contract as pair_scan.
+template <bool SYM, bool FILL>
+__device__ void pair_scan_grid(Ctx& c, const PairGraph& G, const LGrid& g, const float* qp, const int* qid, int nq,
+                               const int* eid, int* up_n, int* lo_n, int* up_dst, int* lo_dst) {
+    const float cc2 = G.cache_cut2;
+    for (int q = c.gtid; q < nq; q += c.gsize) {
+        float x0 = qp[q * 3], x1 = qp[q * 3 + 1], x2 = qp[q * 3 + 2];
+        int a = qid[q];
+        int up = FILL ? up_n[q] : 0;
+        int lo = (FILL && SYM) ? lo_n[q] : 0;
+        int cx = cell_coord(x0, g.ox, g.inv), cy = cell_coord(x1, g.oy, g.inv), cz = cell_coord(x2, g.oz, g.inv);
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    int ax = cx + dx, ay = cy + dy, az = cz + dz;
+                    int b = cell_bucket(g.mask, ax, ay, az);
+                    int e1 = g.ofs[b + 1];
+                    for (int t = g.ofs[b]; t < e1; ++t) {
+                        float4 e = g.el[t];
+                        float d2 = refine_dist2(x0 - e.x, x1 - e.y, x2 - e.z);
+                        if (!(d2 < cc2)) continue;
+                        if (cell_coord(e.x, g.ox, g.inv) != ax || cell_coord(e.y, g.oy, g.inv) != ay ||
+                            cell_coord(e.z, g.oz, g.inv) != az)
+                            continue;  // another cell sharing the bucket
+                        int j = __float_as_int(e.w);
+                        if (SYM && j == q) continue;
+                        if (!pg_accept(G, a, eid[j])) continue;
+                        if (!SYM || j > q) {
+                            if (FILL) up_dst[up] = j;
+                            ++up;
+                        } else {
+                            if (FILL) lo_dst[lo] = j;
+                            ++lo;
+                        }
+                    }
                 }
-            }
-}
-
-// Sort every segment [off[r], off[r+1]) of src into dst (distinct values per
-// segment) by rank: one wave per segment.
-__device__ void sort_segments(Ctx& c, const int* src, int* dst, const int* off, int n_rows) {
-    int wave = c.tid >> 6, lane = c.tid & 63;
-    for (int r = wave; r < n_rows; r += NWAVE) {
-        int b = off[r], e = off[r + 1];
-        for (int k = b + lane; k < e; k += 64) {
-            int v = src[k];
-            int rank = 0;
-            for (int t = b; t < e; ++t) rank += src[t] < v;
-            dst[b + rank] = v;
+        if (FILL) {
+            up_n[q] = up;
+            if (SYM) lo_n[q] = lo;
+        } else {
+            up_n[q] += up;
+            if (SYM) lo_n[q] += lo;
         }
     }
-    __syncthreads();
+    team_sync(c);
 }
+
+// Scan by grid when the other side fits in LDS, else by tiles.
+template <bool SYM, bool FILL>
+__device__ void pair_scan_any(Ctx& c, const PairGraph& G, const float* qp, const int* qid, int nq, const float* ep,
+                              const int* eid, int ne, int* up_n, int* lo_n, int* up_dst, int* lo_dst) {
+    if (lgrid_fits(ne)) {
+        LGrid g = lgrid_build(c, G, ep, ne);
+        pair_scan_grid<SYM, FILL>(c, G, g, qp, qid, nq, eid, up_n, lo_n, up_dst, lo_dst);
+    } else {
+        pair_scan<SYM, FILL>(c, G, qp, qid, nq, ep, eid, ne, up_n, lo_n, up_dst, lo_dst);
+    }
+}
+
+// Slot lookup table: open addressing on key i*n2 + j (HCAP a power of two).
+__device__ __forceinline__ uint32_t slot_hash(uint32_t k) { return k * 2654435761u; }
 
 // Rebuild rows and transposed rows from pos_pack.  Sets state[1] on overflow
 // (the host grows the capacity and repeats the run).
+//   forward rows:    side-1 element i, partners j (symmetric: j > i);
+//   transposed rows: side-2 element j, partners i (symmetric: i < j), stored
+//                    as the forward slot of (i, j) found through a hash table.
+// Both row orders are canonical (see the scans above).
 __device__ void pg_rebuild(Ctx& c, const PairGraph& G) {
     const int* id1 = c.CI + G.id1;
     const int* id2 = c.CI + G.id2;
@@ -182,107 +299,86 @@ __device__ void pg_rebuild(Ctx& c, const PairGraph& G) {
     int* toff = c.I + G.toff;
     int* tslot = c.I + G.tslot;
     int* tmp = c.I + G.tmp;
-    int npack = G.sym ? G.n1 : G.n1 + G.n2;
-    for (int i = c.tid; i < npack * 3; i += NT) cache[i] = pack[i];
-    const float cc2 = G.cache_cut2;
+    int* hkey = c.I + G.hkey;
+    int* hval = c.I + G.hval;
     const bool sym = G.sym != 0;
-
-    // forward rows (and, for symmetric graphs, the lower partners)
-    Grid g = grid_build(c, G, p2, G.n2);
-    for (int i = c.tid; i < G.n1; i += NT) {
-        float x0 = p1[i * 3], x1 = p1[i * 3 + 1], x2 = p1[i * 3 + 2];
-        int a = id1[i];
-        int up = 0, lo = 0;
-        grid_visit(c, G, g, p2, x0, x1, x2, [&](int j, float y0, float y1, float y2) {
-            if (sym && j == i) return;
-            if (refine_dist2(x0 - y0, x1 - y1, x2 - y2) < cc2 && pg_accept(G, a, id2[j])) {
-                if (!sym || j > i) ++up; else ++lo;
-            }
-        });
-        off[i] = up;
-        if (sym) toff[i] = lo;
+    int npack = sym ? G.n1 : G.n1 + G.n2;
+    for (int i = c.gtid; i < npack * 3; i += c.gsize) cache[i] = pack[i];
+    for (int i = c.gtid; i <= G.n1; i += c.gsize) off[i] = 0;
+    for (int j = c.gtid; j <= G.n2; j += c.gsize) toff[j] = 0;
+    team_sync(c);
+    unsigned long long t0 = ptick(c);
+    // counts (each query row is owned by one thread)
+    if (sym) {
+        pair_scan_any<true, false>(c, G, p1, id1, G.n1, p1, id1, G.n1, off, toff, nullptr, nullptr);
+    } else {
+        pair_scan_any<false, false>(c, G, p1, id1, G.n1, p2, id2, G.n2, off, nullptr, nullptr, nullptr);
+        pair_scan_any<false, false>(c, G, p2, id2, G.n2, p1, id1, G.n1, toff, nullptr, nullptr, nullptr);
     }
-    if (c.tid == 0) {
-        off[G.n1] = 0;
-        if (sym) toff[G.n1] = 0;
-    }
-    __syncthreads();
-    int E = block_exscan(c, off, G.n1 + 1);
-    if (sym) block_exscan(c, toff, G.n1 + 1);
-    if (E > G.ecap) {
-        if (c.tid == 0) { c.I[G.state + 1] = 1; c.I[G.state + 0] = 0; }
-        __syncthreads();
+    int E = team_exscan(c, off, G.n1 + 1);
+    int Et = team_exscan(c, toff, G.n2 + 1);
+    if (E > G.ecap || Et != E) {  // Et == E always holds: both scans apply one symmetric test
+        if (c.gtid == 0) { c.I[G.state + 1] = E > G.ecap ? 1 : 2; c.I[G.state + 0] = 0; }
+        team_sync(c);
         return;
     }
-    for (int i = c.tid; i < G.n1; i += NT) {
-        float x0 = p1[i * 3], x1 = p1[i * 3 + 1], x2 = p1[i * 3 + 2];
-        int a = id1[i];
-        int up = off[i], lo = sym ? toff[i] : 0;
-        grid_visit(c, G, g, p2, x0, x1, x2, [&](int j, float y0, float y1, float y2) {
-            if (sym && j == i) return;
-            if (refine_dist2(x0 - y0, x1 - y1, x2 - y2) < cc2 && pg_accept(G, a, id2[j])) {
-                if (!sym || j > i) tmp[up++] = j; else tslot[lo++] = j;
-            }
-        });
-    }
-    __syncthreads();
-    sort_segments(c, tmp, nbr, off, G.n1);
-    for (int i = c.tid; i < G.n1; i += NT)
-        for (int s = off[i]; s < off[i + 1]; ++s) row[s] = i;
-
-    // transposed rows: partners on side 1 of every side-2 element, sorted
+    pacc(c, PROF_RB_COUNT, t0);
+    t0 = ptick(c);
+    // fill: forward partners into nbr, transposed partners into tmp
+    for (int i = c.gtid; i < G.n1; i += c.gsize) row[i] = off[i];        // row[] as a cursor for now
+    for (int j = c.gtid; j < G.n2; j += c.gsize) tslot[j] = toff[j];     // tslot[] as a cursor for now
+    for (int h = c.gtid; h < G.hcap; h += c.gsize) hkey[h] = -1;
+    team_sync(c);
     if (sym) {
-        sort_segments(c, tslot, tmp, toff, G.n1);
+        pair_scan_any<true, true>(c, G, p1, id1, G.n1, p1, id1, G.n1, row, tslot, nbr, tmp);
     } else {
-        Grid h = grid_build(c, G, p1, G.n1);
-        for (int j = c.tid; j < G.n2; j += NT) {
-            float y0 = p2[j * 3], y1 = p2[j * 3 + 1], y2 = p2[j * 3 + 2];
-            int b = id2[j];
-            int n = 0;
-            grid_visit(c, G, h, p1, y0, y1, y2, [&](int i, float x0, float x1, float x2) {
-                if (refine_dist2(x0 - y0, x1 - y1, x2 - y2) < cc2 && pg_accept(G, id1[i], b)) ++n;
-            });
-            toff[j] = n;
-        }
-        if (c.tid == 0) toff[G.n2] = 0;
-        __syncthreads();
-        int Et = block_exscan(c, toff, G.n2 + 1);
-        if (Et != E) {  // both scans apply the same test to the same pairs
-            if (c.tid == 0) { c.I[G.state + 1] = 2; c.I[G.state + 0] = 0; }
-            __syncthreads();
-            return;
-        }
-        for (int j = c.tid; j < G.n2; j += NT) {
-            float y0 = p2[j * 3], y1 = p2[j * 3 + 1], y2 = p2[j * 3 + 2];
-            int b = id2[j];
-            int n = toff[j];
-            grid_visit(c, G, h, p1, y0, y1, y2, [&](int i, float x0, float x1, float x2) {
-                if (refine_dist2(x0 - y0, x1 - y1, x2 - y2) < cc2 && pg_accept(G, id1[i], b)) tslot[n++] = i;
-            });
-        }
-        __syncthreads();
-        sort_segments(c, tslot, tmp, toff, G.n2);
+        pair_scan_any<false, true>(c, G, p1, id1, G.n1, p2, id2, G.n2, row, nullptr, nbr, nullptr);
+        pair_scan_any<false, true>(c, G, p2, id2, G.n2, p1, id1, G.n1, tslot, nullptr, tmp, nullptr);
     }
-    // tmp holds the sorted side-1 partners of each transposed row: map them
-    // to forward slots
-    int nt = sym ? G.n1 : G.n2;
+    for (int i = c.gtid; i < G.n1; i += c.gsize)
+        for (int s = off[i]; s < off[i + 1]; ++s) row[s] = i;
+    team_sync(c);
+    pacc(c, PROF_RB_FILL, t0);
+    t0 = ptick(c);
+    // slot table: (i, j) -> forward slot
+    const uint32_t hm = (uint32_t)G.hcap - 1u;
+    const int n2 = G.n2;
     int bad = 0;
-    for (int j = c.tid; j < nt; j += NT)
-        for (int t = toff[j]; t < toff[j + 1]; ++t) {
-            int i = tmp[t];
-            int b0 = off[i], n = off[i + 1] - b0;
-            int k = lower_bound_i(nbr + b0, n, j);
-            if (k >= n || nbr[b0 + k] != j) { bad = 1; k = 0; }
-            tslot[t] = b0 + k;
+    for (int s = c.gtid; s < E; s += c.gsize) {
+        int key = row[s] * n2 + nbr[s];
+        uint32_t h = slot_hash((uint32_t)key) & hm;
+        for (int probe = 0; probe < G.hcap; ++probe) {
+            int prev = atomicCAS(&hkey[h], -1, key);
+            if (prev == -1) { hval[h] = s; break; }
+            h = (h + 1u) & hm;
         }
-    int anybad = __syncthreads_or(bad);
-    if (c.tid == 0) {
+    }
+    team_sync(c);
+    // transposed entries -> forward slots
+    int nt = sym ? G.n1 : G.n2;
+    for (int j = c.gtid; j < nt; j += c.gsize)
+        for (int t = toff[j]; t < toff[j + 1]; ++t) {
+            int key = tmp[t] * n2 + j;
+            uint32_t h = slot_hash((uint32_t)key) & hm;
+            int found = -1;
+            for (int probe = 0; probe < G.hcap; ++probe) {
+                int k = hkey[h];
+                if (k == key) { found = hval[h]; break; }
+                if (k == -1) break;
+                h = (h + 1u) & hm;
+            }
+            if (found < 0) { bad = 1; found = 0; }
+            tslot[t] = found;
+        }
+    int anybad = team_or(c, bad);
+    pacc(c, PROF_RB_MAP, t0);
+    if (c.gtid == 0) {
         c.I[G.state + 0] = anybad ? 0 : 1;
         if (anybad) c.I[G.state + 1] = 2;
         c.I[G.state + 2] += 1;
         c.I[G.state + 3] = E;
     }
-    __syncthreads();
+    team_sync(c);
 }
 
 // ---- edge functions -----------------------------------------------------
@@ -454,7 +550,7 @@ __device__ float backbone_edge(const Ctx& c, const OpRec& op, int r1, int r2, fl
 __device__ const float* pg_stage_params(Ctx& c, const PairGraph& G) {
     const float* src = c.CF + G.param;
     if (G.param_len <= 0) return src;
-    for (int k = c.tid; k < G.param_len; k += NT) c.lds[k] = src[k];
+    for (int k = c.tid; k < G.param_len; k += NT) c.lds[k] = src[k];  // per workgroup
     __syncthreads();
     return c.lds;
 }
@@ -479,7 +575,7 @@ __device__ void pg_edges_t(Ctx& c, const PairGraph& G, const OpRec& op) {
     const float* P2 = c.A + G.out2;
     const int E = c.I[G.state + 3];
     const int cap = G.ecap;
-    for (int s = c.tid; s < E; s += NT) {
+    for (int s = c.gtid; s < E; s += c.gsize) {
         int i1 = row[s], j = nbr[s];
         const float* xp = pack + i1 * 3;
         const float* yp = pk2 + j * 3;
@@ -517,7 +613,7 @@ __device__ void pg_edges_t(Ctx& c, const PairGraph& G, const OpRec& op) {
 #pragma unroll
         for (int q = 0; q < DD2; ++q) D2[q * cap + s] = e2[q];
     }
-    __syncthreads();
+    team_sync(c);
 }
 
 __device__ void pg_edges(Ctx& c, const PairGraph& G, const OpRec& op) {
@@ -550,8 +646,8 @@ __device__ void pg_gather_t(Ctx& c, const PairGraph& G) {
     float* S1 = c.A + G.sens1;
     float* S2 = c.A + G.sens2;
     const int cap = G.ecap;
-    int grp = c.tid / GW, gl = c.tid % GW;
-    const int NG = NT / GW;
+    int grp = c.gtid / GW, gl = c.gtid % GW;
+    const int NG = c.gsize / GW;
     // uniform trip counts: every lane runs every group reduction
     for (int base = 0; base < G.n1; base += NG) {
         int i = base + grp;
@@ -560,20 +656,29 @@ __device__ void pg_gather_t(Ctx& c, const PairGraph& G) {
 #pragma unroll
         for (int q = 0; q < D1N; ++q) acc[q] = 0.f;
         int b = ok ? off[i] : 0, e = ok ? off[i + 1] : 0;
+        // branch-free: every load of an iteration is issued at once
+#pragma unroll 2
         for (int s = b + gl; s < e; s += GW) {
-            if (!act[s]) continue;
+            bool a = act[s] != 0;
             float w = es[s];
 #pragma unroll
-            for (int q = 0; q < D1N; ++q) acc[q] += w * D1[q * cap + s];
+            for (int q = 0; q < D1N; ++q) {
+                float d = D1[q * cap + s];
+                acc[q] += a ? w * d : 0.f;
+            }
         }
         if (G.sym && ok) {
             int tb = toff[i], te = toff[i + 1];
+#pragma unroll 2
             for (int t = tb + gl; t < te; t += GW) {
                 int s = tslot[t];
-                if (!act[s]) continue;
+                bool a = act[s] != 0;
                 float w = es[s];
 #pragma unroll
-                for (int q = 0; q < D1N; ++q) acc[q] += w * D2[q * cap + s];
+                for (int q = 0; q < D1N; ++q) {
+                    float d = D2[q * cap + s];
+                    acc[q] += a ? w * d : 0.f;
+                }
             }
         }
 #pragma unroll
@@ -584,7 +689,7 @@ __device__ void pg_gather_t(Ctx& c, const PairGraph& G) {
             for (int q = 0; q < D1N; ++q) dst[q] += acc[q];
         }
     }
-    __syncthreads();
+    team_sync(c);
     if (G.sym) return;
     for (int base = 0; base < G.n2; base += NG) {
         int j = base + grp;
@@ -593,12 +698,16 @@ __device__ void pg_gather_t(Ctx& c, const PairGraph& G) {
 #pragma unroll
         for (int q = 0; q < D2N; ++q) acc[q] = 0.f;
         int tb = ok ? toff[j] : 0, te = ok ? toff[j + 1] : 0;
+#pragma unroll 2
         for (int t = tb + gl; t < te; t += GW) {
             int s = tslot[t];
-            if (!act[s]) continue;
+            bool a = act[s] != 0;
             float w = es[s];
 #pragma unroll
-            for (int q = 0; q < D2N; ++q) acc[q] += w * D2[q * cap + s];
+            for (int q = 0; q < D2N; ++q) {
+                float d = D2[q * cap + s];
+                acc[q] += a ? w * d : 0.f;
+            }
         }
 #pragma unroll
         for (int q = 0; q < D2N; ++q) acc[q] = group_sum<GW>(acc[q]);
@@ -608,15 +717,17 @@ __device__ void pg_gather_t(Ctx& c, const PairGraph& G) {
             for (int q = 0; q < D2N; ++q) dst[q] += acc[q];
         }
     }
-    __syncthreads();
+    team_sync(c);
 }
 
 __device__ void pg_gather(Ctx& c, const PairGraph& G) {
+    unsigned long long t0 = ptick(c);
     switch (G.kind) {
     case EDGE_HBOND_COVERAGE: pg_gather_t<7, 6>(c, G); break;
     case EDGE_ENV_WEIGHTED: pg_gather_t<6, 4>(c, G); break;
     default: pg_gather_t<6, 6>(c, G); break;  // symmetric graphs: both sides 6 wide
     }
+    pacc(c, PROF_PG_GATHER, t0);
 }
 
 // Sum of the active edge values of each row (side 1) or transposed row
@@ -627,15 +738,16 @@ __device__ void pg_row_sums(Ctx& c, const PairGraph& G, bool side2, float* out, 
     const int* act = c.I + G.act;
     const float* val = c.A + G.val;
     int n = side2 ? G.n2 : G.n1;
-    int grp = c.tid / GW, gl = c.tid % GW;
-    for (int base = 0; base < n; base += NT / GW) {
+    int grp = c.gtid / GW, gl = c.gtid % GW;
+    for (int base = 0; base < n; base += c.gsize / GW) {
         int r = base + grp;
         bool ok = r < n;
         float acc = 0.f;
         int k0 = ok ? off[r] : 0, k1 = ok ? off[r + 1] : 0;
         for (int k = k0 + gl; k < k1; k += GW) {
             int s = side2 ? tslot[k] : k;
-            if (act[s]) acc += val[s];
+            float v = val[s];  // zero on inactive slots
+            acc += v;
         }
         acc = group_sum<GW>(acc);
         if (ok && gl == 0) out[r * stride] = acc;
@@ -645,9 +757,13 @@ __device__ void pg_row_sums(Ctx& c, const PairGraph& G, bool side2, float* out, 
 // Prepare a graph for an evaluation: pack, maybe rebuild, refine and evaluate.
 // Returns false when the slot capacity overflowed (the evaluation is void).
 __device__ bool pg_refresh(Ctx& c, const PairGraph& G, const OpRec& op) {
+    unsigned long long t0 = ptick(c);
     if (pg_pack_and_check(c, G)) pg_rebuild(c, G);
+    pacc(c, PROF_PG_REBUILD, t0);
     if (!c.I[G.state + 0]) return false;
+    t0 = ptick(c);
     pg_edges(c, G, op);
+    pacc(c, PROF_PG_EDGES, t0);
     return true;
 }
 

@@ -2,8 +2,8 @@
 // (src/rotamer.cpp:628-1169), one workgroup per system.
 //
 // Residue nodes carry 1, 3 or 6 rotamer states, one bead per state.  Residue
-// pairs with any bead pair in the rotamer pair graph's cache are "candidates"
-// (kept per cache rebuild, in (a,b) order).  In an evaluation a candidate is
+// pairs that may hold a bead pair of the rotamer pair graph's cache are
+// "candidates" (kept per cache rebuild, in (a,b) order).  In an evaluation a candidate is
 // present when one of its bead pairs is in the refined edge set, which is the
 // reference's EdgeLocator edge set (src/rotamer.cpp:134-206,383-396):
 //   * 1-rotamer x 1-rotamer pairs contribute -log P to the free energy;
@@ -44,108 +44,91 @@ __device__ RotView rot_view(Ctx& c, const RotamerDev& R) {
     return v;
 }
 
-constexpr int BIG = 0x7fffffff;
-
-// Merge the sorted partner lists of a node's beads (at most 6) into distinct
-// partner nodes, in increasing order: emit(b) once per partner node, slot(k)
-// for every list entry k.  Lists are [beg[k], end[k]); node_of(k) is the
-// partner node of entry k and is non-decreasing along each list.
-template <class NodeOf, class Emit, class Slot>
-__device__ __forceinline__ void merge_partner_nodes(int* beg, const int* end, NodeOf node_of, Emit emit, Slot slot) {
-    int head[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) head[k] = beg[k] < end[k] ? node_of(beg[k]) : BIG;
-    for (;;) {
-        int mn = BIG;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) mn = min(mn, head[k]);
-        if (mn == BIG) break;
-        emit(mn);
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            while (head[k] == mn) {
-                slot(beg[k]);
-                ++beg[k];
-                head[k] = beg[k] < end[k] ? node_of(beg[k]) : BIG;
-            }
-        }
-    }
-}
-
-// Residue-pair candidates and node incidences from the freshly rebuilt bead
-// rows.  Candidate c = (ca[c], cb[c]) with ca < cb, sorted; the incidence list
-// of node x names (candidate, 1) for partners a < x first, then (candidate, 0)
-// for its own candidates.  Returns false on capacity overflow.
+// Residue-pair candidates and node incidences, rebuilt with the bead rows.
+// Candidates are the residue pairs whose bead spheres (centre = bead mean,
+// radius = farthest bead) come within the cache radius of each other: a
+// superset of the pairs with a bead pair in the cache, so every slot finds its
+// candidate.  Candidate c = (ca[c], cb[c]), ca < cb, sorted by (a, b); the
+// incidence list of node x names (candidate, 1) for partners a < x first, in
+// increasing a, then (candidate, 0) for its own candidates.  Returns false on
+// capacity overflow.
 __device__ bool rot_rebuild(Ctx& c, const PairGraph& G, const RotamerDev& R) {
     RotView v = rot_view(c, R);
-    const int* off = c.I + G.off;
-    const int* nbr = c.I + G.nbr;
+    const float* pack = c.A + G.pos_pack;
+    float* cen = c.A + R.ncen;
     const int* row = c.I + G.row;
-    const int* toff = c.I + G.toff;
-    const int* tslot = c.I + G.tslot;
-    auto fwd_node = [&](int s) { return v.bnode[nbr[s]]; };
-    auto low_node = [&](int t) { return v.bnode[row[tslot[t]]]; };
-    for (int a = c.tid; a < R.n_node; a += NT) {
-        int beg[6], end[6];
+    const int* nbr = c.I + G.nbr;
+    const int n = R.n_node;
+    for (int a = c.gtid; a < n; a += c.gsize) {
         int b0 = v.bead0[a], b1 = v.bead0[a + 1];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            beg[k] = b0 + k < b1 ? off[b0 + k] : 0;
-            end[k] = b0 + k < b1 ? off[b0 + k + 1] : 0;
+        float sx = 0.f, sy = 0.f, sz = 0.f;
+        for (int b = b0; b < b1; ++b) { sx += pack[b * 3]; sy += pack[b * 3 + 1]; sz += pack[b * 3 + 2]; }
+        float inv = 1.f / (float)(b1 - b0);
+        sx *= inv; sy *= inv; sz *= inv;
+        float r2 = 0.f;
+        for (int b = b0; b < b1; ++b) {
+            float dx = pack[b * 3] - sx, dy = pack[b * 3 + 1] - sy, dz = pack[b * 3 + 2] - sz;
+            r2 = fmaxf(r2, dx * dx + dy * dy + dz * dz);
         }
-        int n = 0;
-        merge_partner_nodes(beg, end, fwd_node, [&](int) { ++n; }, [&](int) {});
-        v.coff[a] = n;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            beg[k] = b0 + k < b1 ? toff[b0 + k] : 0;
-            end[k] = b0 + k < b1 ? toff[b0 + k + 1] : 0;
-        }
-        int m = 0;
-        merge_partner_nodes(beg, end, low_node, [&](int) { ++m; }, [&](int) {});
-        v.ioff[a] = n + m;
+        cen[a * 4] = sx; cen[a * 4 + 1] = sy; cen[a * 4 + 2] = sz; cen[a * 4 + 3] = sqrtf(r2);
     }
-    if (c.tid == 0) { v.coff[R.n_node] = 0; v.ioff[R.n_node] = 0; }
-    __syncthreads();
-    int C = block_exscan(c, v.coff, R.n_node + 1);
-    int I = block_exscan(c, v.ioff, R.n_node + 1);
+    team_sync(c);
+    const float rc = G.cache_cut;
+    auto near = [&](int a, int b) {
+        float dx = cen[a * 4] - cen[b * 4], dy = cen[a * 4 + 1] - cen[b * 4 + 1], dz = cen[a * 4 + 2] - cen[b * 4 + 2];
+        float lim = (rc + cen[a * 4 + 3] + cen[b * 4 + 3]) * 1.0001f + 1e-3f;
+        return dx * dx + dy * dy + dz * dz < lim * lim;
+    };
+    for (int a = c.gtid; a < n; a += c.gsize) {
+        int up = 0, lo = 0;
+        for (int b = 0; b < n; ++b)
+            if (b != a && near(a, b)) { if (b > a) ++up; else ++lo; }
+        v.coff[a] = up;
+        v.ioff[a] = up + lo;
+    }
+    if (c.gtid == 0) { v.coff[n] = 0; v.ioff[n] = 0; }
+    team_sync(c);
+    int C = team_exscan(c, v.coff, n + 1);
+    int I = team_exscan(c, v.ioff, n + 1);
     if (C > R.ccap || I > 2 * R.ccap) {
-        if (c.tid == 0) { c.I[G.state + 1] = 1; c.I[G.state + 0] = 0; }
-        __syncthreads();
+        if (c.gtid == 0) { c.I[G.state + 1] = 1; c.I[G.state + 0] = 0; }
+        team_sync(c);
         return false;
     }
-    int* scand = v.scand;
-    for (int a = c.tid; a < R.n_node; a += NT) {
-        int beg[6], end[6];
-        int b0 = v.bead0[a], b1 = v.bead0[a + 1];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            beg[k] = b0 + k < b1 ? off[b0 + k] : 0;
-            end[k] = b0 + k < b1 ? off[b0 + k + 1] : 0;
-        }
-        int cc = v.coff[a] - 1;
-        merge_partner_nodes(beg, end, fwd_node, [&](int b) { ++cc; v.ca[cc] = a; v.cb[cc] = b; },
-                            [&](int s) { scand[s] = cc; });
+    for (int a = c.gtid; a < n; a += c.gsize) {
+        int k = v.coff[a];
+        for (int b = a + 1; b < n; ++b)
+            if (near(a, b)) { v.ca[k] = a; v.cb[k] = b; ++k; }
     }
-    __syncthreads();
-    for (int x = c.tid; x < R.n_node; x += NT) {
-        int beg[6], end[6];
-        int b0 = v.bead0[x], b1 = v.bead0[x + 1];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            beg[k] = b0 + k < b1 ? toff[b0 + k] : 0;
-            end[k] = b0 + k < b1 ? toff[b0 + k + 1] : 0;
-        }
+    team_sync(c);
+    for (int x = c.gtid; x < n; x += c.gsize) {
         int t = v.ioff[x];
-        merge_partner_nodes(beg, end, low_node, [&](int a) {
-            int c0 = v.coff[a];
-            int k = lower_bound_i(v.cb + c0, v.coff[a + 1] - c0, x);
-            v.iinc[t++] = (c0 + k) * 2 + 1;
-        }, [&](int) {});
+        for (int a = 0; a < x; ++a)
+            if (near(a, x)) {
+                int c0 = v.coff[a];
+                int k = lower_bound_i(v.cb + c0, v.coff[a + 1] - c0, x);
+                v.iinc[t++] = (c0 + k) * 2 + 1;
+            }
         for (int cc = v.coff[x]; cc < v.coff[x + 1]; ++cc) v.iinc[t++] = cc * 2;
     }
-    if (c.tid == 0) v.stats[5] = C;
-    __syncthreads();
+    // candidate of every bead slot
+    const int E = c.I[G.state + 3];
+    int bad = 0;
+    for (int s = c.gtid; s < E; s += c.gsize) {
+        int a = v.bnode[row[s]], b = v.bnode[nbr[s]];
+        int c0 = v.coff[a], m = v.coff[a + 1] - c0;
+        int k = lower_bound_i(v.cb + c0, m, b);
+        if (k >= m || v.cb[c0 + k] != b) { bad = 1; k = 0; }
+        v.scand[s] = c0 + k;
+    }
+    int anybad = team_or(c, bad);
+    if (anybad) {
+        if (c.gtid == 0) { c.I[G.state + 1] = 2; c.I[G.state + 0] = 0; }
+        team_sync(c);
+        return false;
+    }
+    if (c.gtid == 0) v.stats[5] = C;
+    team_sync(c);
     return true;
 }
 
@@ -165,8 +148,8 @@ __device__ void rot_fill(Ctx& c, const PairGraph& G, const RotamerDev& R) {
     const int cap = v.cap;
     // energies of each bead's edges to 1-rotamer residues (folded into the node)
     {
-        int grp = c.tid / GW, gl = c.tid % GW;
-        for (int base = 0; base < G.n1; base += NT / GW) {
+        int grp = c.gtid / GW, gl = c.gtid % GW;
+        for (int base = 0; base < G.n1; base += c.gsize / GW) {
             int b = base + grp;
             bool ok = b < G.n1;
             float acc = 0.f;
@@ -182,10 +165,10 @@ __device__ void rot_fill(Ctx& c, const PairGraph& G, const RotamerDev& R) {
             if (ok && gl == 0) v.e1x[b] = acc;
         }
     }
-    for (int k = c.tid; k <= C; k += NT) v.eidx[k] = 0;
-    __syncthreads();
+    for (int k = c.gtid; k <= C; k += c.gsize) v.eidx[k] = 0;
+    team_sync(c);
     // node energies -> probabilities (fill_holders, convert_energy_to_prob, :239-256,867-880)
-    for (int n = c.tid; n < R.n_node; n += NT) {
+    for (int n = c.gtid; n < R.n_node; n += c.gsize) {
         int b0 = v.bead0[n], nr = v.nrot[n];
         float e[6];
         float mn = 3.0e38f;
@@ -204,13 +187,13 @@ __device__ void rot_fill(Ctx& c, const PairGraph& G, const RotamerDev& R) {
             v.nprob[n * 6 + k] = k < nr ? expf(mn - e[k] - v.e1x[b0 + k]) : 0.f;
     }
     // presence of BP edges (both residues with 3 or 6 rotamers)
-    for (int s = c.tid; s < E; s += NT) {
+    for (int s = c.gtid; s < E; s += c.gsize) {
         if (!act[s]) continue;
         if (v.nrot[v.bnode[row[s]]] > 1 && v.nrot[v.bnode[nbr[s]]] > 1) v.eidx[v.scand[s]] = 1;
     }
-    __syncthreads();
-    int nbp = block_exscan(c, v.eidx, C + 1);
-    for (int cc = c.tid; cc < C; cc += NT) {
+    team_sync(c);
+    int nbp = team_exscan(c, v.eidx, C + 1);
+    for (int cc = c.gtid; cc < C; cc += c.gsize) {
         int e = v.eidx[cc];
         if (v.eidx[cc + 1] == e) continue;
         int a = v.ca[cc], b = v.cb[cc];
@@ -222,10 +205,10 @@ __device__ void rot_fill(Ctx& c, const PairGraph& G, const RotamerDev& R) {
 #pragma unroll
         for (int q = 0; q < 36; ++q) v.P[q * cap + e] = ((q / 6) < n1 && (q % 6) < n2) ? 1.f : 0.f;
     }
-    if (c.tid == 0) v.stats[4] = nbp;
-    __syncthreads();
+    if (c.gtid == 0) v.stats[4] = nbp;
+    team_sync(c);
     // P[r1][r2] = exp(-E) of the bead pair (one bead per rotamer state)
-    for (int s = c.tid; s < E; s += NT) {
+    for (int s = c.gtid; s < E; s += c.gsize) {
         if (!act[s]) continue;
         int i = row[s], j = nbr[s];
         int a = v.bnode[i], b = v.bnode[j];
@@ -237,7 +220,7 @@ __device__ void rot_fill(Ctx& c, const PairGraph& G, const RotamerDev& R) {
         v.P[q * cap + e] = expf(-val[s]);
     }
     // compact BP incidences of every node, in incidence order
-    for (int x = c.tid; x < R.n_node; x += NT) {
+    for (int x = c.gtid; x < R.n_node; x += c.gsize) {
         int k = v.ioff[x];
         if (v.nrot[x] > 1) {
             for (int t = v.ioff[x]; t < v.ioff[x + 1]; ++t) {
@@ -249,7 +232,7 @@ __device__ void rot_fill(Ctx& c, const PairGraph& G, const RotamerDev& R) {
         }
         v.ninc[x] = k - v.ioff[x];
     }
-    __syncthreads();
+    team_sync(c);
 }
 
 // One synchronous message pass: new messages (half 1-po) from old node
@@ -258,7 +241,7 @@ __device__ void rot_fill(Ctx& c, const PairGraph& G, const RotamerDev& R) {
 // (h*12 + side*6 + r) of edge e at msg[(...)*cap + e].
 __device__ void rot_edge_pass(Ctx& c, const RotView& v, int nbp, const float* nold_, int po) {
     const int cap = v.cap;
-    for (int e = c.tid; e < nbp; e += NT) {
+    for (int e = c.gtid; e < nbp; e += c.gsize) {
         int f = v.bpf[e], s = v.bps[e];
         float v1[6], v2[6];
 #pragma unroll
@@ -286,7 +269,7 @@ __device__ void rot_edge_pass(Ctx& c, const RotView& v, int nbp, const float* no
             v.msg[(h + 6 + r) * cap + e] = m2[r] * i2;
         }
     }
-    __syncthreads();
+    team_sync(c);
 }
 
 // Node pass: cur = prob * prod(new messages), renormalised after every factor;
@@ -297,7 +280,7 @@ __device__ void rot_edge_pass(Ctx& c, const RotView& v, int nbp, const float* no
 __device__ void rot_node_pass(Ctx& c, const RotView& v, int n_node, float* ncur_, float* nold_, int pn,
                               float damping, bool swap_first) {
     const int cap = v.cap;
-    for (int x = c.tid; x < n_node; x += NT) {
+    for (int x = c.gtid; x < n_node; x += c.gsize) {
         if (v.nrot[x] == 1) continue;
         float b[6];
 #pragma unroll
@@ -333,7 +316,7 @@ __device__ void rot_node_pass(Ctx& c, const RotView& v, int n_node, float* ncur_
             for (int r = 0; r < 6; ++r) ncur_[x * 6 + r] = (1.f - damping) * im * b[r] + damping * nold_[x * 6 + r];
         }
     }
-    __syncthreads();
+    team_sync(c);
 }
 
 // Solve; returns the iteration count (the reference's `iter`).
@@ -342,11 +325,11 @@ __device__ int rot_solve(Ctx& c, const RotamerDev& R) {
     const int nbp = v.stats[4];
     const int cap = v.cap;
     // initial messages = 1, old beliefs = prob (:1110-1133)
-    for (int e = c.tid; e < nbp; e += NT)
+    for (int e = c.gtid; e < nbp; e += c.gsize)
 #pragma unroll
         for (int r = 0; r < 12; ++r) v.msg[r * cap + e] = 1.f;
-    for (int i = c.tid; i < R.n_node * 6; i += NT) v.nold[i] = v.nprob[i];
-    __syncthreads();
+    for (int i = c.gtid; i < R.n_node * 6; i += c.gsize) v.nold[i] = v.nprob[i];
+    team_sync(c);
     float* cur = v.ncur;
     float* old = v.nold;
     int po = 0;
@@ -363,15 +346,15 @@ __device__ int rot_solve(Ctx& c, const RotamerDev& R) {
             rot_node_pass(c, v, R.n_node, cur, old, po, R.damping, false);
         }
         float d = 0.f;
-        for (int x = c.tid; x < R.n_node; x += NT) {
+        for (int x = c.gtid; x < R.n_node; x += c.gsize) {
             if (v.nrot[x] == 1) continue;
 #pragma unroll
             for (int r = 0; r < 6; ++r) d = fmaxf(d, cur[x * 6 + r] - old[x * 6 + r]);
         }
-        dev = block_max(c, d);
+        dev = team_max(c, d);
     }
     // node marginals (:283-290); final beliefs may sit in either buffer
-    for (int x = c.tid; x < R.n_node; x += NT) {
+    for (int x = c.gtid; x < R.n_node; x += c.gsize) {
         int nr = v.nrot[x];
         float b[6];
         float s = 0.f;
@@ -381,9 +364,9 @@ __device__ int rot_solve(Ctx& c, const RotamerDev& R) {
 #pragma unroll
         for (int r = 0; r < 6; ++r) v.ncur[x * 6 + r] = nr == 1 ? (r == 0 ? 1.f : 0.f) : b[r] * is;
     }
-    __syncthreads();
+    team_sync(c);
     // edge marginals (:425-449)
-    for (int e = c.tid; e < nbp; e += NT) {
+    for (int e = c.gtid; e < nbp; e += c.gsize) {
         int f = v.bpf[e], s = v.bps[e];
         float b1[6], b2[6];
 #pragma unroll
@@ -401,7 +384,7 @@ __device__ int rot_solve(Ctx& c, const RotamerDev& R) {
 #pragma unroll
             for (int j = 0; j < 6; ++j) v.marg[(i * 6 + j) * cap + e] = v.P[(i * 6 + j) * cap + e] * b1[i] * b2[j] * is;
     }
-    __syncthreads();
+    team_sync(c);
     return it;
 }
 
@@ -410,7 +393,7 @@ __device__ float rot_free_energy(Ctx& c, const PairGraph& G, const RotamerDev& R
     RotView v = rot_view(c, R);
     const int cap = v.cap;
     float en = 0.f;
-    for (int x = c.tid; x < R.n_node; x += NT) {
+    for (int x = c.gtid; x < R.n_node; x += c.gsize) {
         float e = v.noff[x];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
@@ -425,10 +408,10 @@ __device__ float rot_free_energy(Ctx& c, const PairGraph& G, const RotamerDev& R
     const int* act = c.I + G.act;
     const float* val = c.A + G.val;
     const int E = c.I[G.state + 3];
-    for (int s = c.tid; s < E; s += NT)
+    for (int s = c.gtid; s < E; s += c.gsize)
         if (act[s] && v.nrot[v.bnode[row[s]]] == 1 && v.nrot[v.bnode[nbr[s]]] == 1) en += -logf(expf(-val[s]));
     const int nbp = v.stats[4];
-    for (int e = c.tid; e < nbp; e += NT) {
+    for (int e = c.gtid; e < nbp; e += c.gsize) {
         int f = v.bpf[e], s = v.bps[e];
 #pragma unroll
         for (int i = 0; i < 6; ++i)
@@ -439,7 +422,7 @@ __device__ float rot_free_energy(Ctx& c, const PairGraph& G, const RotamerDev& R
                 en += p * logf((1e-10f + p) / (1e-10f + q));
             }
     }
-    return block_sum(c, en);
+    return team_sum(c, en);
 }
 
 // Edge sensitivities of bead pairs (src/rotamer.cpp:1057-1066) and the 1-body
@@ -453,7 +436,7 @@ __device__ void rot_sens(Ctx& c, const PairGraph& G, const RotamerDev& R) {
     const int* loc1 = c.CI + G.loc1;
     const int E = c.I[G.state + 3];
     const int cap = v.cap;
-    for (int s = c.tid; s < E; s += NT) {
+    for (int s = c.gtid; s < E; s += c.gsize) {
         if (!act[s]) { es[s] = 0.f; continue; }
         int i = row[s], j = nbr[s];
         int a = v.bnode[i], b = v.bnode[j];
@@ -470,11 +453,11 @@ __device__ void rot_sens(Ctx& c, const PairGraph& G, const RotamerDev& R) {
         }
         es[s] = sv;
     }
-    for (int b = c.tid; b < G.n1; b += NT) {
+    for (int b = c.gtid; b < G.n1; b += c.gsize) {
         float m = v.ncur[v.bnode[b] * 6 + v.brot[b]];
         for (int q = 0; q < R.n_prob; ++q) c.A[R.prob_sens[q] + loc1[b] * R.prob_w[q]] += m;
     }
-    __syncthreads();
+    team_sync(c);
 }
 
 }  // namespace upside

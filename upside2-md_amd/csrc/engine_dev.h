@@ -37,13 +37,12 @@ enum EdgeKind : int {
 
 // A pair graph over two element sets (or one, symmetric).  Candidate pairs
 // within the cache radius ("slots") are kept in compressed rows: row i1 holds
-// its partners i2 in increasing order (symmetric graphs: only i2 > i1).  The
-// transposed rows list, for each i2, the slots naming it in increasing i1
-// order.  Both orders are canonical, so every sum over slots (energies,
+// its partners i2 (symmetric graphs: only i2 > i1) in a canonical order fixed
+// by the positions alone.  The transposed rows list, for each i2, the slots
+// naming it, in the same kind of canonical order.  Both orders are canonical, so every sum over slots (energies,
 // sensitivity gathers) runs in a fixed order: bit-reproducible, no float
-// atomics.  Rows are rebuilt from a hashed cell grid of cell size equal to
-// the cache radius.  Per-slot derivatives are stored structure-of-arrays:
-// component q of slot s at d1[q*ecap + s].
+// atomics.  Per-slot derivatives are stored structure-of-arrays: component q
+// of slot s at d1[q*ecap + s].
 struct PairGraph {
     int kind;
     int sym;
@@ -65,7 +64,6 @@ struct PairGraph {
     float drift2;          // rebuild when an element moved more than this (squared)
     int ddim1, ddim2;      // derivative widths (== dim except for backbone pairs)
     int ecap;              // slot capacity
-    int nbucket;           // cell-grid hash buckets (power of two)
     // per-system scratch (arena offsets)
     int cache_pos;         // float (n1+n2)*3 (symmetric: n1*3)
     int pos_pack;          // float (n1+n2)*3 current packed coordinates of both sides
@@ -74,9 +72,9 @@ struct PairGraph {
     int row;               // int ecap  row of each slot
     int toff;              // int n2+1  transposed row offsets
     int tslot;             // int ecap  slot of each transposed entry
-    int tmp;               // int ecap  unsorted row entries during a rebuild
-    int gofs;              // int nbucket+1  cell-grid bucket offsets
-    int gidx;              // int max(n1,n2)  elements ordered by bucket
+    int tmp;               // int ecap  transposed partners during a rebuild
+    int hcap;              // slot hash table size (power of two >= 2*ecap)
+    int hkey, hval;        // int hcap  (i*n2 + j) -> slot, rebuilt with the rows
     int val;               // float ecap   edge value (0 when inactive)
     int d1, d2;            // float ddim1*ecap, ddim2*ecap (SoA)
     int esens;             // float ecap   edge sensitivity
@@ -111,6 +109,7 @@ struct RotamerDev {
     int msg;               // float 24*ccap  [old m1(6) m2(6) | new m1(6) m2(6)], SoA
     int marg;              // float 36*ccap  edge marginals, SoA
     int e1x;               // float n_bead   energy of edges to 1-rotamer residues
+    int ncen;              // float n_node*4 bead-sphere centre and radius of each residue
     int nprob, ncur, nold; // float n_node*6
     int noff;              // float n_node   energy offset
     int stats;             // int [0] last iterations [1] bad solves [2] total iterations [3] solves
@@ -154,12 +153,18 @@ struct SysDesc {
     float mom_scale, noise_scale;
     uint64_t n_invocations;
     int error;             // int offset of error word
+    int bar;               // int offset of the team barrier words [count, generation, timeout, pad]
+    int tred;              // float offset of team reduction scratch [2 * MAX_TEAM]
     unsigned long long* prof;  // optional per-op wall-clock accumulators (see PROF_*), or null
 };
 
 // Profile accumulator layout: [value op k] for k < n_germ, [deriv op k] at
 // n_germ + k, then the fixed slots below (wall_clock64 ticks).
-enum { PROF_BP = 0, PROF_REBUILD = 1, PROF_THERMO = 2, PROF_INTEGRATE = 3, PROF_EXTRA = 4 };
+enum {
+    PROF_BP = 0, PROF_REBUILD = 1, PROF_THERMO = 2, PROF_INTEGRATE = 3, PROF_PG_REBUILD = 4, PROF_PG_EDGES = 5,
+    PROF_PG_GATHER = 6, PROF_ROT_FILL = 7, PROF_ROT_REBUILD = 8, PROF_RB_COUNT = 9, PROF_RB_FILL = 10,
+    PROF_RB_MAP = 11, PROF_EXTRA = 12
+};
 
 struct RunArgs {
     int mode;              // 1: potential and derivative, 0: derivative only
@@ -170,7 +175,10 @@ struct RunArgs {
     int frame_interval;    // rounds between potential evaluations recorded
     float* frame_pot;      // optional [n_sys * n_frames] potentials at frames
     int n_frames;
+    int team;              // workgroups per system (1: no grid barriers)
 };
+
+constexpr int MAX_TEAM = 64;
 
 }  // namespace upside
 

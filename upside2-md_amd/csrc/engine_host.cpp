@@ -27,6 +27,7 @@
 #include "../../include/upside_engine.h"
 
 hipError_t upside_launch_engine(upside::SysDesc* d_systems, int n_sys, upside::RunArgs args, hipStream_t stream);
+int upside_max_resident_blocks();
 
 namespace upside {
 
@@ -184,7 +185,7 @@ struct Topology {
     int nf = 0, ni = 0;          // per-system arena sizes
     vector<std::pair<int, int>> sens_zero;
     vector<int> sens_zero_host;
-    int n_pot = 0, pot = -1, mom = -1, err = -1, szero = -1;
+    int n_pot = 0, pot = -1, mom = -1, err = -1, szero = -1, bar = -1, tred = -1;
     int n_atom = 0;
     double cap_scale = 1.0;
 
@@ -784,9 +785,6 @@ struct Builder {
         long full = G.sym ? (long)G.n1 * (G.n1 - 1) / 2 : (long)G.n1 * G.n2;
         long ecap = std::min(full, (long)(std::max(G.n1, 1) * (double)want * T.cap_scale));
         G.ecap = (int)std::max(ecap, 16L);
-        int nb = 64;
-        while (nb < 2 * std::max(G.n1, G.n2)) nb *= 2;
-        G.nbucket = nb;
         int npack = G.sym ? G.n1 : G.n1 + G.n2;
         G.cache_pos = T.falloc(npack * 3);
         G.pos_pack = T.falloc(npack * 3);
@@ -797,8 +795,12 @@ struct Builder {
         G.toff = T.ialloc(G.n2 + 1);
         G.tslot = T.ialloc(slots);
         G.tmp = T.ialloc(slots);
-        G.gofs = T.ialloc(nb + 1);
-        G.gidx = T.ialloc(std::max(G.n1, G.n2));
+        int hc = 64;
+        while (hc < 2 * G.ecap) hc *= 2;
+        G.hcap = hc;
+        G.hkey = T.ialloc(hc);
+        G.hval = T.ialloc(hc);
+        if ((long)G.n1 * G.n2 >= (1L << 31)) fail("pair graph too large for 32-bit slot keys");
         G.val = T.falloc(slots);
         G.d1 = T.falloc(slots * G.ddim1);
         G.d2 = T.falloc(slots * G.ddim2);
@@ -957,7 +959,7 @@ struct Builder {
         R.chunk = h5.attr_i(g, "iteration_chunk_size", 0, true);
         if (R.chunk < 1) fail(n.name + ": iteration_chunk_size must be positive");
         int nn = R.n_node;
-        R.ccap = (int)std::max(16.0, nn * 48.0 * T.cap_scale);
+        R.ccap = (int)std::max(16.0, nn * 64.0 * T.cap_scale);
         long cc = R.ccap;
         R.coff = T.ialloc(nn + 1);
         R.ca = T.ialloc(cc);
@@ -975,6 +977,7 @@ struct Builder {
         R.msg = T.falloc(cc * 24);
         R.marg = T.falloc(cc * 36);
         R.e1x = T.falloc(nb);
+        R.ncen = T.falloc(nn * 4);
         R.nprob = T.falloc(nn * 6);
         R.ncur = T.falloc(nn * 6);
         R.nold = T.falloc(nn * 6);
@@ -1079,6 +1082,8 @@ static void load_topology(Topology& T, H5& h5, int n_atom) {
     T.pot = T.falloc(std::max(T.n_pot, 1));
     T.mom = T.falloc(n_atom * 3);
     T.err = T.ialloc(4);
+    T.bar = T.ialloc(4);
+    T.tred = T.falloc(2 * MAX_TEAM);
     vector<int> z;
     for (auto& r : T.sens_zero) { z.push_back(r.first); z.push_back(r.second); }
     T.szero = T.ialloc(z.size());
@@ -1211,7 +1216,13 @@ struct Batch {
             else if (i < ng + (int)T.deriv.size()) label = "deriv  " + T.nodes[T.deriv[i - ng]].name;
             else {
                 static const char* ex[] = {"rotamer BP solve (in value)", "rotamer graph rebuild (in value)",
-                                           "thermostat", "integrator update"};
+                                           "thermostat", "integrator update",
+                                           "pair-list rebuilds, other graphs (in value)",
+                                           "pair edge evaluation, all graphs (in value)",
+                                           "pair sensitivity gathers, all graphs",
+                                           "rotamer fill (in value)", "rotamer candidates (in rebuild)",
+                                           "rebuild: count scans", "rebuild: fill scans",
+                                           "rebuild: transposed slot mapping"};
                 label = ex[i - ng - (int)T.deriv.size()];
             }
             snprintf(line, sizeof(line), "%-48s %12.6f s\n", label.c_str(), t[i] * tick);
@@ -1270,6 +1281,8 @@ struct Batch {
         d.sens_zero = T.szero;
         d.mom = T.mom;
         d.error = T.err;
+        d.bar = T.bar;
+        d.tred = T.tred;
         d.seed = base_seed + (uint32_t)k;
         // OrnsteinUhlenbeckThermostat parameters (src/thermostat.h:9-12, src/main.cpp:775-781)
         float delta_t = (float)thermostat_interval * 3.f * dt;
@@ -1312,6 +1325,10 @@ struct Batch {
         for (size_t k = 0; k < sys.size(); ++k) desc.push_back(make_desc((int)k));
         hip_check(hipMalloc((void**)&d_desc, desc.size() * sizeof(SysDesc)), "descriptors");
         push_desc();
+        if (const char* e = std::getenv("UPSIDE_TEAM")) {  // fixed team size (tests, experiments)
+            int t = std::atoi(e);
+            if (t >= 1) team = std::min(t, max_team());
+        }
     }
 
     void push_desc() {
@@ -1362,12 +1379,75 @@ struct Batch {
         push_desc();
     }
 
-    void launch(const RunArgs& a) {
+    int team = 1;   // workgroups per system (set_team / autotune)
+
+    // Largest team every system of the batch can have with all workgroups
+    // resident at once (cooperative launch), capped at MAX_TEAM.
+    int max_team() const {
+        int blocks = upside_max_resident_blocks();
+        int t = blocks / std::max((int)sys.size(), 1);
+        return std::max(1, std::min(t, MAX_TEAM));
+    }
+
+    void set_team(int t) {
+        if (t < 1 || t > max_team()) fail("team size " + std::to_string(t) + " not possible (max " +
+                                          std::to_string(max_team()) + " for this batch)");
+        team = t;
+    }
+
+    // Pick the team size with the shortest run of n_round rounds: every
+    // candidate starts from the same snapshot (positions, momenta, thermostat
+    // counters), which is restored afterwards, so tuning leaves the state as
+    // it was.  ms_out[k] receives candidate k's kernel time (-1 if skipped).
+    void autotune(int n_round, const int* cand, int n_cand, float* ms_out) {
+        vector<vector<float>> pos(sys.size()), mom(sys.size());
+        pull_desc();
+        vector<SysDesc> saved = desc;
+        for (size_t k = 0; k < sys.size(); ++k) {
+            const Topology& T = sys[k].prog->T;
+            pos[k].resize(T.n_atom * 3);
+            mom[k].resize(T.n_atom * 3);
+            hip_check(hipMemcpy(pos[k].data(), sys[k].arena + T.nodes[0].out, pos[k].size() * 4, hipMemcpyDeviceToHost), "pos");
+            hip_check(hipMemcpy(mom[k].data(), sys[k].arena + T.mom, mom[k].size() * 4, hipMemcpyDeviceToHost), "mom");
+        }
+        auto restore = [&]() {
+            for (size_t k = 0; k < sys.size(); ++k) {
+                const Topology& T = sys[k].prog->T;
+                hip_check(hipMemcpy(sys[k].arena + T.nodes[0].out, pos[k].data(), pos[k].size() * 4, hipMemcpyHostToDevice), "pos");
+                hip_check(hipMemcpy(sys[k].arena + T.mom, mom[k].data(), mom[k].size() * 4, hipMemcpyHostToDevice), "mom");
+                desc[k].n_invocations = saved[k].n_invocations;
+            }
+            push_desc();
+        };
+        int best = team;
+        float best_ms = 3.0e38f;
+        int limit = max_team();
+        for (int k = 0; k < n_cand; ++k) {
+            if (ms_out) ms_out[k] = -1.f;
+            if (cand[k] < 1 || cand[k] > limit) continue;
+            team = cand[k];
+            run(n_round, 0, nullptr, 0, 0);
+            if (ms_out) ms_out[k] = last_ms;
+            if (last_ms < best_ms) { best_ms = last_ms; best = cand[k]; }
+            restore();
+        }
+        team = best;
+    }
+
+    void launch(const RunArgs& a0) {
+        RunArgs a = a0;
+        a.team = team;
         hip_check(hipEventRecord(ev0, stream), "event");
         hip_check(upside_launch_engine(d_desc, (int)sys.size(), a, stream), "engine launch");
         hip_check(hipEventRecord(ev1, stream), "event");
         hip_check(hipStreamSynchronize(stream), "engine run");
         (void)hipEventElapsedTime(&last_ms, ev0, ev1);
+        if (team > 1)
+            for (auto& sy : sys) {
+                unsigned w[4];
+                hip_check(hipMemcpy(w, sy.iarena + sy.prog->T.bar, sizeof(w), hipMemcpyDeviceToHost), "barrier");
+                if (w[2]) fail("team barrier wait timed out: results of this run are void");
+            }
     }
 
     // Evaluate potential and derivative of every system at its positions.
@@ -1893,6 +1973,28 @@ int upside_batch_bp_stats(UpsideBatch* b, int sys, int* last_iters, long long* t
 }
 
 float upside_batch_last_run_ms(UpsideBatch* b) { return b ? b->batch.last_ms : -1.f; }
+
+int upside_batch_max_team(UpsideBatch* b) {
+    ABI_TRY
+    return b->batch.max_team();
+    ABI_CATCH(-1)
+}
+
+int upside_batch_set_team(UpsideBatch* b, int team) {
+    ABI_TRY
+    b->batch.set_team(team);
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_get_team(UpsideBatch* b) { return b ? b->batch.team : -1; }
+
+int upside_batch_autotune(UpsideBatch* b, int n_round, const int* candidates, int n_candidates, float* ms_out) {
+    ABI_TRY
+    b->batch.autotune(n_round, candidates, n_candidates, ms_out);
+    return 0;
+    ABI_CATCH(1)
+}
 
 int upside_batch_enable_profile(UpsideBatch* b, int on) {
     ABI_TRY

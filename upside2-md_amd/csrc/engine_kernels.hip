@@ -18,19 +18,20 @@ namespace upside {
 #define CIA(op, k) (c.CI + (op).ia[k])
 #define CFA(op, k) (c.CF + (op).ia[k])
 
-__device__ __forceinline__ void prof_add(const SysDesc& S, int slot, unsigned long long t0) {
-    if (S.prof && threadIdx.x == 0) S.prof[slot] += wall_clock64() - t0;
+// Profile accumulation (op slots by index, fixed slots through pacc).
+__device__ __forceinline__ void prof_op(const Ctx& c, int slot, unsigned long long t0) {
+    if (c.prof && c.gtid == 0) c.prof[slot] += wall_clock64() - t0;
 }
 
 __device__ __forceinline__ void put_pot(Ctx& c, const SysDesc& S, const OpRec& op, float e) {
-    if (c.tid == 0) c.A[S.pot + op.pot_slot] = e;
+    if (c.gtid == 0) c.A[S.pot + op.pot_slot] = e;
 }
 
 // ---- Distance3D (src/coord_basic.cpp:13-66) --------------------------------
 __device__ void dist3d_fwd(Ctx& c, const OpRec& op) {
     const int* id = CIA(op, 0);
     float* der = c.A + op.scratch;
-    for (int e = c.tid; e < op.n_elem; e += NT) {
+    for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
         const float* a = IN(op, 0) + id[2 * e] * op.in_w[0];
         const float* b = IN(op, 1) + id[2 * e + 1] * op.in_w[1];
         float dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
@@ -44,14 +45,14 @@ __device__ void dist3d_fwd(Ctx& c, const OpRec& op) {
 __device__ void dist3d_bwd(Ctx& c, const OpRec& op) {
     const float* der = c.A + op.scratch;
     float* con = c.A + op.ia[1];
-    for (int e = c.tid; e < op.n_elem; e += NT) {
+    for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
         float s = SENS(op)[e];
         for (int k = 0; k < 3; ++k) {
             con[(e * 2 + 0) * 3 + k] = s * der[e * 3 + k];
             con[(e * 2 + 1) * 3 + k] = -s * der[e * 3 + k];
         }
     }
-    __syncthreads();
+    team_sync(c);
     plan_gather(c, op.plans[0]);
     plan_gather(c, op.plans[1]);
 }
@@ -62,7 +63,7 @@ __device__ void angle_fwd(Ctx& c, const OpRec& op) {
     float* der = c.A + op.scratch;
     const float* P = IN(op, 0);
     int w = op.in_w[0];
-    for (int e = c.tid; e < op.n_elem; e += NT) {
+    for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
         const float* a1 = P + id[3 * e] * w;
         const float* a2 = P + id[3 * e + 1] * w;
         const float* a3 = P + id[3 * e + 2] * w;
@@ -83,7 +84,7 @@ __device__ void angle_fwd(Ctx& c, const OpRec& op) {
 __device__ void angle_bwd(Ctx& c, const OpRec& op) {
     const float* der = c.A + op.scratch;
     float* con = c.A + op.ia[1];
-    for (int e = c.tid; e < op.n_elem; e += NT) {
+    for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
         float s = SENS(op)[e];
         for (int k = 0; k < 3; ++k) {
             float d1 = s * der[e * 6 + k], d2 = s * der[e * 6 + 3 + k];
@@ -92,7 +93,7 @@ __device__ void angle_bwd(Ctx& c, const OpRec& op) {
             con[(e * 3 + 2) * 3 + k] = -d1 - d2;
         }
     }
-    __syncthreads();
+    team_sync(c);
     plan_gather(c, op.plans[0]);
 }
 
@@ -109,7 +110,7 @@ __device__ void dihedral_fwd(Ctx& c, const OpRec& op) {
     float* jac = c.A + op.scratch;
     const float* P = IN(op, 0);
     int w = op.in_w[0];
-    for (int e = c.tid; e < op.n_elem; e += NT) {
+    for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
         const int* q = id + 4 * e;
         if (q[0] < 0 || q[3] < 0) {
             OUT(op)[e] = -1.3963f;
@@ -142,11 +143,11 @@ __device__ void dihedral_fwd(Ctx& c, const OpRec& op) {
 __device__ void dihedral_bwd(Ctx& c, const OpRec& op) {
     const float* jac = c.A + op.scratch;
     float* con = c.A + op.ia[1];
-    for (int e = c.tid; e < op.n_elem; e += NT) {
+    for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
         float s = SENS(op)[e];
         for (int k = 0; k < 12; ++k) con[e * 12 + k] = s * jac[e * 12 + k];
     }
-    __syncthreads();
+    team_sync(c);
     plan_gather(c, op.plans[0]);
 }
 
@@ -159,7 +160,7 @@ __device__ void spring_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
     float box = op.fa[0];
     float* con = c.A + op.scratch;
     float pot = 0.f;
-    for (int e = c.tid; e < n; e += NT) {
+    for (int e = c.gtid; e < n; e += c.gsize) {
         float x = IN(op, 0)[id[e] * op.in_w[0] + dim1];
         float ex = x - eq[e];
         float s2 = ex * ex;
@@ -171,7 +172,7 @@ __device__ void spring_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
         pot += 0.5f * kk[e] * s2;
         con[e] = kk[e] * ex;
     }
-    pot = block_sum(c, pot);
+    pot = team_sum(c, pot);
     put_pot(c, S, op, pot);
     plan_gather(c, op.plans[0]);
 }
@@ -180,7 +181,7 @@ __device__ void spring_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
 __device__ void rama_coord_fwd(Ctx& c, const OpRec& op) {
     const int* i1 = CIA(op, 0);
     const int* i2 = CIA(op, 1);
-    for (int e = c.tid; e < op.n_elem; e += NT) {
+    for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
         OUT(op)[2 * e] = IN(op, 0)[i1[e] * op.in_w[0]];
         OUT(op)[2 * e + 1] = IN(op, 1)[i2[e] * op.in_w[1]];
     }
@@ -188,7 +189,7 @@ __device__ void rama_coord_fwd(Ctx& c, const OpRec& op) {
 __device__ void cat_pos_fwd(Ctx& c, const OpRec& op) {
     const int* idx = CIA(op, 0);
     int n1 = op.ia[1], w = op.width;
-    for (int e = c.tid; e < op.n_elem; e += NT) {
+    for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
         int k = e < n1 ? 0 : 1;
         const float* src = IN(op, k) + idx[e] * op.in_w[k];
         for (int d = 0; d < w; ++d) OUT(op)[e * w + d] = src[d];
@@ -208,7 +209,7 @@ __device__ void rama_map_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
     float scale = op.fa[0], shift = op.fa[1];
     float* con = c.A + op.scratch;
     float pot = 0.f;
-    for (int e = c.tid; e < n; e += NT) {
+    for (int e = c.gtid; e < n; e += c.gsize) {
         const float* r = IN(op, 0) + res[e] * op.in_w[0];
         float x = (r[0] + shift) * scale, y = (r[1] + shift) * scale;
         int xb = (int)x, yb = (int)y;
@@ -220,7 +221,7 @@ __device__ void rama_map_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
         con[2 * e] = dx * scale;
         con[2 * e + 1] = dy * scale;
     }
-    pot = block_sum(c, pot);
+    pot = team_sum(c, pot);
     put_pot(c, S, op, pot);
     plan_gather(c, op.plans[0]);
 }
@@ -335,8 +336,8 @@ __device__ void affine_fwd(Ctx& c, const OpRec& op) {
     const float* P = IN(op, 0);
     int w = op.in_w[0];
     int n = op.n_elem;
-    for (int base = 0; base < n; base += NT) {
-        int r = base + c.tid;
+    for (int base = 0; base < n; base += c.gsize) {
+        int r = base + c.gtid;
         int rr = r < n ? r : (r & ~3);       // padding lanes repeat lane 0 of their quad
         if (rr >= n) continue;              // whole quad past the end
         float x[3][3], center[3];
@@ -367,7 +368,7 @@ __device__ void affine_bwd(Ctx& c, const OpRec& op) {
     const float* ev = c.A + op.scratch;
     const float* ec = c.A + op.scratch + 4 * op.n_elem;
     float* con = c.A + op.ia[2];
-    for (int r = c.tid; r < op.n_elem; r += NT) {
+    for (int r = c.gtid; r < op.n_elem; r += c.gsize) {
         const float* s = SENS(op) + r * 7;
         const float* E = ec + r * 16;
         const float* q = E;
@@ -404,7 +405,7 @@ __device__ void affine_bwd(Ctx& c, const OpRec& op) {
             }
         }
     }
-    __syncthreads();
+    team_sync(c);
     plan_gather(c, op.plans[0]);
 }
 
@@ -416,7 +417,7 @@ __device__ void placement_fwd(Ctx& c, const OpRec& op) {
     int is_rama = op.ia[3], dim = op.ia[7];
     const float* aff = IN(op, 0);
     int wa = op.in_w[0];
-    for (int e = c.tid; e < op.n_elem; e += NT) {
+    for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
         float* o = OUT(op) + e * op.width;
         if (is_rama) {
             const int* rres = CIA(op, 4);
@@ -446,19 +447,19 @@ __device__ void placement_bwd(Ctx& c, const OpRec& op) {
     int is_rama = op.ia[3];
     float* con = c.A + op.ia[8];
     if (is_rama) {
-        for (int e = c.tid; e < op.n_elem; e += NT) {
+        for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
             float s = SENS(op)[e];
             con[2 * e] = s * c.A[op.scratch + 2 * e];
             con[2 * e + 1] = s * c.A[op.scratch + 2 * e + 1];
         }
-        __syncthreads();
+        team_sync(c);
         plan_gather(c, op.plans[1]);
         return;
     }
     const int* ares = CIA(op, 0);
     const float* aff = IN(op, 0);
     int wa = op.in_w[0];
-    for (int e = c.tid; e < op.n_elem; e += NT) {
+    for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
         const float* s = SENS(op) + e * op.width;
         const float* x = OUT(op) + e * op.width;
         const float* t = aff + ares[e] * wa;
@@ -471,7 +472,7 @@ __device__ void placement_bwd(Ctx& c, const OpRec& op) {
             con[e * 6 + 3 + k] = tq1[k] + tq2[k];
         }
     }
-    __syncthreads();
+    team_sync(c);
     plan_gather(c, op.plans[0]);
 }
 
@@ -482,7 +483,7 @@ __device__ void infer_fwd(Ctx& c, const OpRec& op) {
     float* cache = c.A + op.scratch;
     const float* P = IN(op, 0);
     int w = op.in_w[0];
-    for (int e = c.tid; e < op.n_elem; e += NT) {
+    for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
         const float *pc = P + id[3 * e] * w, *cc = P + id[3 * e + 1] * w, *nc = P + id[3 * e + 2] * w;
         float pr[3], nx[3], ds[3];
         for (int k = 0; k < 3; ++k) { pr[k] = pc[k] - cc[k]; nx[k] = nc[k] - cc[k]; }
@@ -501,7 +502,7 @@ __device__ void infer_bwd(Ctx& c, const OpRec& op) {
     const float* bl = CFA(op, 1);
     const float* cache = c.A + op.scratch;
     float* con = c.A + op.ia[2];
-    for (int e = c.tid; e < op.n_elem; e += NT) {
+    for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
         const float* s = SENS(op) + e * 6;
         const float* cs = cache + e * 12;
         float sn[3], sd[3], sp[3], sx[3];
@@ -519,7 +520,7 @@ __device__ void infer_bwd(Ctx& c, const OpRec& op) {
             con[(e * 3 + 2) * 3 + k] = sx[k];
         }
     }
-    __syncthreads();
+    team_sync(c);
     plan_gather(c, op.plans[0]);
 }
 
@@ -527,8 +528,8 @@ __device__ void infer_bwd(Ctx& c, const OpRec& op) {
 // word is set, the remaining ops are skipped and the host grows the
 // capacities and repeats the run.
 __device__ void graph_overflow(Ctx& c, const SysDesc& S) {
-    if (c.tid == 0) c.I[S.error] = 1;
-    __syncthreads();
+    if (c.gtid == 0) c.I[S.error] = 1;
+    team_sync(c);
 }
 
 // ---- protein_hbond (src/hbond.cpp:357-435) ---------------------------------------
@@ -540,8 +541,8 @@ __device__ void protein_hbond_fwd(Ctx& c, const SysDesc& S, const OpRec& op, con
     float* ss = c.A + op.scratch;  // per-site sum of hb_log
     pg_row_sums(c, G, false, ss, 1);
     pg_row_sums(c, G, true, ss + nd, 1);
-    __syncthreads();
-    for (int v = c.tid; v < op.n_elem; v += NT) {
+    team_sync(c);
+    for (int v = c.gtid; v < op.n_elem; v += c.gsize) {
         bool don = v < nd;
         int i = don ? v : v - nd;
         const float* src = IN(op, 0) + (don ? loc1[i] : loc2[i]) * op.in_w[0];
@@ -554,26 +555,26 @@ __device__ void protein_hbond_fwd(Ctx& c, const SysDesc& S, const OpRec& op, con
 __device__ void protein_hbond_bwd(Ctx& c, const OpRec& op, const PairGraph& G) {
     int nd = G.n1;
     float* ss = c.A + op.scratch;
-    for (int v = c.tid; v < op.n_elem; v += NT) ss[v] = SENS(op)[v * 7 + 6] * (1.f - OUT(op)[v * 7 + 6]);
-    __syncthreads();
+    for (int v = c.gtid; v < op.n_elem; v += c.gsize) ss[v] = SENS(op)[v * 7 + 6] * (1.f - OUT(op)[v * 7 + 6]);
+    team_sync(c);
     const int* row = c.I + G.row;
     const int* nbr = c.I + G.nbr;
     float* es = c.A + G.esens;
     const int E = c.I[G.state + 3];
-    for (int s = c.tid; s < E; s += NT) es[s] = ss[row[s]] + ss[nd + nbr[s]];
-    __syncthreads();
+    for (int s = c.gtid; s < E; s += c.gsize) es[s] = ss[row[s]] + ss[nd + nbr[s]];
+    team_sync(c);
     pg_gather(c, G);
     // pass the coordinate sensitivities through to infer_H_O (:419-433)
     const int* loc1 = c.CI + G.loc1;
     const int* loc2 = c.CI + G.loc2;
-    for (int i = c.tid; i < G.n1; i += NT)
+    for (int i = c.gtid; i < G.n1; i += c.gsize)
 #pragma unroll
         for (int k = 0; k < 6; ++k) INS(op, 0)[loc1[i] * op.in_w[0] + k] += SENS(op)[i * 7 + k];
-    __syncthreads();
-    for (int j = c.tid; j < G.n2; j += NT)
+    team_sync(c);
+    for (int j = c.gtid; j < G.n2; j += c.gsize)
 #pragma unroll
         for (int k = 0; k < 6; ++k) INS(op, 0)[loc2[j] * op.in_w[0] + k] += SENS(op)[(nd + j) * 7 + k];
-    __syncthreads();
+    team_sync(c);
 }
 
 // ---- hbond_coverage (src/hbond.cpp:438-484) ----------------------------------------
@@ -585,8 +586,8 @@ __device__ void hbond_coverage_bwd(Ctx& c, const OpRec& op, const PairGraph& G) 
     const int* nbr = c.I + G.nbr;
     float* es = c.A + G.esens;
     const int E = c.I[G.state + 3];
-    for (int s = c.tid; s < E; s += NT) es[s] = SENS(op)[nbr[s]];
-    __syncthreads();
+    for (int s = c.gtid; s < E; s += c.gsize) es[s] = SENS(op)[nbr[s]];
+    team_sync(c);
     pg_gather(c, G);
 }
 
@@ -608,7 +609,7 @@ __device__ void hbond_energy_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
     int wp = op.in_w[0], wr = op.in_w[1];
     const float* P = op.fa;
     float pot = 0.f;
-    for (int i = c.tid; i < n_res; i += NT) {
+    for (int i = c.gtid; i < n_res; i += c.gsize) {
         float phi = rama[i * wr], psi = rama[i * wr + 1];
         float p1, d1, p2, d2, s1, e1, s2, e2;
         compact_sigmoid(phi - P[4], P[5], p1, d1);
@@ -633,8 +634,8 @@ __device__ void hbond_energy_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
         ramas[i * wr] += hb * dphi;
         ramas[i * wr + 1] += hb * dpsi;
     }
-    __syncthreads();
-    for (int i = c.tid; i < n_res - 1; i += NT) {
+    team_sync(c);
+    for (int i = c.gtid; i < n_res - 1; i += c.gsize) {
         float hb2 = 0.f;
         for (int k = h2o[i]; k < h2o[i + 1]; ++k) hb2 += pp[h2i[k] * wp + 6];
         if (hb2 <= 1.f) continue;
@@ -645,7 +646,7 @@ __device__ void hbond_energy_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
         if (fd[i + 1] >= 0) pps[fd[i + 1] * wp + 6] += dE;
         if (fa[i] >= 0) pps[fa[i] * wp + 6] += dE;
     }
-    pot = block_sum(c, pot);
+    pot = team_sum(c, pot);
     put_pot(c, S, op, pot);
 }
 
@@ -653,7 +654,7 @@ __device__ void hbond_energy_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
 __device__ void weighted_pos_fwd(Ctx& c, const OpRec& op) {
     const int* ip = CIA(op, 0);
     const int* iw = CIA(op, 1);
-    for (int e = c.tid; e < op.n_elem; e += NT) {
+    for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
         const float* p = IN(op, 0) + ip[e] * op.in_w[0];
         float* o = OUT(op) + e * 4;
         o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
@@ -662,8 +663,8 @@ __device__ void weighted_pos_fwd(Ctx& c, const OpRec& op) {
 }
 __device__ void weighted_pos_bwd(Ctx& c, const OpRec& op) {
     float* con = c.A + op.scratch;
-    for (int e = c.tid; e < op.n_elem; e += NT) con[e] = -OUT(op)[e * 4 + 3] * SENS(op)[e * 4 + 3];
-    __syncthreads();
+    for (int e = c.gtid; e < op.n_elem; e += c.gsize) con[e] = -OUT(op)[e * 4 + 3] * SENS(op)[e * 4 + 3];
+    team_sync(c);
     plan_gather(c, op.plans[0]);
     plan_gather(c, op.plans[1]);
 }
@@ -681,7 +682,7 @@ __device__ void env_coverage_fwd(Ctx& c, const SysDesc& S, const OpRec& op, cons
     const float* val = c.A + G.val;
     const int* act = c.I + G.act;
     // one thread per (element, type): scan the element's slots in order
-    for (int k = c.tid; k < G.n1 * naa; k += NT) {
+    for (int k = c.gtid; k < G.n1 * naa; k += c.gsize) {
         int i = k / naa, t = k - i * naa;
         float acc = 0.f;
         for (int s = off[i]; s < off[i + 1]; ++s)
@@ -697,8 +698,8 @@ __device__ void env_coverage_bwd(Ctx& c, const OpRec& op, const PairGraph& G) {
     const int* nbr = c.I + G.nbr;
     float* es = c.A + G.esens;
     const int E = c.I[G.state + 3];
-    for (int s = c.tid; s < E; s += NT) es[s] = SENS(op)[row[s] * naa + aa[id2[nbr[s]]]];
-    __syncthreads();
+    for (int s = c.gtid; s < E; s += c.gsize) es[s] = SENS(op)[row[s] * naa + aa[id2[nbr[s]]]];
+    team_sync(c);
     pg_gather(c, G);
 }
 
@@ -711,8 +712,8 @@ __device__ void hb_env_bwd(Ctx& c, const OpRec& op, const PairGraph& G) {
     const int* row = c.I + G.row;
     float* es = c.A + G.esens;
     const int E = c.I[G.state + 3];
-    for (int s = c.tid; s < E; s += NT) es[s] = SENS(op)[row[s]];
-    __syncthreads();
+    for (int s = c.gtid; s < E; s += c.gsize) es[s] = SENS(op)[row[s]];
+    team_sync(c);
     pg_gather(c, G);
 }
 
@@ -728,7 +729,7 @@ __device__ void sigmoid_coupling_fwd(Ctx& c, const SysDesc& S, const OpRec& op) 
     const float* x = IN(op, 0);
     float* xs = INS(op, 0);
     float pot = 0.f;
-    for (int r = c.tid; r < n; r += NT) {
+    for (int r = c.gtid; r < n; r += c.gsize) {
         int t = ct[r];
         float wn = 0.f;
         for (int a = 0; a < nt; ++a) wn += W[t * nt + a] * x[r * nt + a];
@@ -737,7 +738,7 @@ __device__ void sigmoid_coupling_fwd(Ctx& c, const SysDesc& S, const OpRec& op) 
         pot += sc[t] * v;
         for (int a = 0; a < nt; ++a) xs[r * nt + a] += W[t * nt + a] * sc[t] * dv;
     }
-    pot = block_sum(c, pot);
+    pot = team_sum(c, pot);
     put_pot(c, S, op, pot);
 }
 
@@ -752,7 +753,7 @@ __device__ void bb_sigmoid_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
     const float* hb = IN(op, 1);
     float* hbs = INS(op, 1);
     float pot = 0.f;
-    for (int r = c.tid; r < n; r += NT) {
+    for (int r = c.gtid; r < n; r += c.gsize) {
         float bl = 0.f;
         for (int a = 0; a < nt; ++a) bl += W[a] * sc[r * nt + a];
         bl += hw * hb[r * op.in_w[1]];
@@ -763,7 +764,7 @@ __device__ void bb_sigmoid_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
         for (int a = 0; a < nt; ++a) scs[r * nt + a] += W[a] * dp;
         hbs[r * op.in_w[1]] += hw * dp;
     }
-    pot = block_sum(c, pot);
+    pot = team_sum(c, pot);
     put_pot(c, S, op, pot);
 }
 
@@ -775,11 +776,11 @@ __device__ void backbone_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const Pa
     float* es = c.A + G.esens;
     const int E = c.I[G.state + 3];
     float pot = 0.f;
-    for (int s = c.tid; s < E; s += NT) {
+    for (int s = c.gtid; s < E; s += c.gsize) {
         es[s] = 1.f;
         if (act[s]) pot += val[s];
     }
-    pot = block_sum(c, pot);
+    pot = team_sum(c, pot);
     put_pot(c, S, op, pot);
     pg_gather(c, G);
 }
@@ -787,20 +788,26 @@ __device__ void backbone_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const Pa
 // ---- rotamer (src/rotamer.cpp:840-850) ----------------------------------------------
 __device__ void rotamer_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G,
                             const RotamerDev& R, int mode) {
+    unsigned long long t0 = ptick(c);
     bool rebuild = pg_pack_and_check(c, G);
-    unsigned long long t0 = S.prof ? wall_clock64() : 0;
     if (rebuild) {
         pg_rebuild(c, G);
-        if (c.I[G.state + 0] && !rot_rebuild(c, G, R)) {}
+        unsigned long long t1 = ptick(c);
+        if (c.I[G.state + 0]) rot_rebuild(c, G, R);
+        pacc(c, PROF_ROT_REBUILD, t1);
     }
-    prof_add(S, S.n_germ + S.n_deriv + PROF_REBUILD, t0);
+    pacc(c, PROF_REBUILD, t0);
     if (!c.I[G.state + 0]) { graph_overflow(c, S); return; }
+    t0 = ptick(c);
     pg_edges(c, G, op);
+    pacc(c, PROF_PG_EDGES, t0);
+    t0 = ptick(c);
     rot_fill(c, G, R);
-    t0 = S.prof ? wall_clock64() : 0;
+    pacc(c, PROF_ROT_FILL, t0);
+    t0 = ptick(c);
     int it = rot_solve(c, R);
-    prof_add(S, S.n_germ + S.n_deriv + PROF_BP, t0);
-    if (c.tid == 0) {
+    pacc(c, PROF_BP, t0);
+    if (c.gtid == 0) {
         int* st = c.I + R.stats;
         st[0] = it;
         if (it >= R.max_iter - R.chunk - 1) st[1] += 1;
@@ -839,7 +846,7 @@ __device__ void run_value(Ctx& c, const SysDesc& S, const OpRec& op, int mode) {
     case OP_BACKBONE_PAIRS: backbone_fwd(c, S, op, *G); break;
     default: break;
     }
-    __syncthreads();
+    team_sync(c);
 }
 
 __device__ void run_deriv(Ctx& c, const SysDesc& S, const OpRec& op) {
@@ -860,7 +867,7 @@ __device__ void run_deriv(Ctx& c, const SysDesc& S, const OpRec& op) {
     case OP_HB_ENV_COVERAGE: hb_env_bwd(c, op, *G); break;
     default: break;  // potential nodes push in their value pass
     }
-    __syncthreads();
+    team_sync(c);
 }
 
 __device__ void compute(Ctx& c, const SysDesc& S, int mode) {
@@ -868,19 +875,19 @@ __device__ void compute(Ctx& c, const SysDesc& S, int mode) {
     for (int r = 0; r < S.sens_zero_cnt; ++r) {
         float* p = c.A + zr[2 * r];
         int n = zr[2 * r + 1];
-        for (int i = c.tid; i < n; i += NT) p[i] = 0.f;
+        for (int i = c.gtid; i < n; i += c.gsize) p[i] = 0.f;
     }
-    __syncthreads();
+    team_sync(c);
     for (int k = 0; k < S.n_germ; ++k) {
-        unsigned long long t0 = S.prof ? wall_clock64() : 0;
+        unsigned long long t0 = ptick(c);
         run_value(c, S, S.ops[k], mode);
-        prof_add(S, k, t0);
+        prof_op(c, k, t0);
         if (c.I[S.error]) return;  // uniform: read after the op's final barrier
     }
     for (int k = 0; k < S.n_deriv; ++k) {
-        unsigned long long t0 = S.prof ? wall_clock64() : 0;
+        unsigned long long t0 = ptick(c);
         run_deriv(c, S, S.ops[S.n_germ + k]);
-        prof_add(S, S.n_germ + k, t0);
+        prof_op(c, S.n_germ + k, t0);
     }
 }
 
@@ -930,14 +937,14 @@ __device__ void normal3(uint32_t seed, uint32_t stream, uint32_t atom, uint64_t 
 
 __device__ void thermostat(Ctx& c, SysDesc& S) {
     float* mom = c.A + S.mom;
-    for (int a = c.tid; a < S.n_atom; a += NT) {
+    for (int a = c.gtid; a < S.n_atom; a += c.gsize) {
         float n[3];
         normal3(S.seed, 0u, (uint32_t)a, S.n_invocations, n);
         for (int k = 0; k < 3; ++k) mom[a * 3 + k] = S.mom_scale * mom[a * 3 + k] + S.noise_scale * n[k];
     }
-    __syncthreads();
-    if (c.tid == 0) S.n_invocations += 1;
-    __syncthreads();
+    team_sync(c);
+    if (c.gtid == 0) S.n_invocations += 1;
+    team_sync(c);
 }
 
 __device__ float total_potential(Ctx& c, const SysDesc& S) {
@@ -946,12 +953,27 @@ __device__ float total_potential(Ctx& c, const SysDesc& S) {
     return t;
 }
 
+// One launch evaluates or integrates every system of a batch; system s is
+// handled by the team of workgroups [s*team, (s+1)*team).
 __global__ void __launch_bounds__(NT) engine_kernel(SysDesc* systems, RunArgs args) {
     __shared__ float red[NWAVE];
     __shared__ int ired[NWAVE];
     extern __shared__ float lds[];
-    SysDesc& S = systems[blockIdx.x];
-    Ctx c{S.arena, S.iarena, S.cf, S.ci, red, ired, lds, (int)threadIdx.x};
+    const int G = args.team;
+    const int sys = blockIdx.x / G, rank = blockIdx.x % G;
+    SysDesc& S = systems[sys];
+    Ctx c;
+    c.A = S.arena; c.I = S.iarena; c.CF = S.cf; c.CI = S.ci;
+    c.red = red; c.ired = ired; c.lds = lds;
+    c.tid = (int)threadIdx.x;
+    c.prof = rank == 0 ? S.prof : nullptr;
+    c.pbase = S.n_germ + S.n_deriv;
+    c.rank = rank; c.G = G;
+    c.gtid = rank * NT + c.tid; c.gsize = G * NT;
+    c.bar = reinterpret_cast<unsigned*>(S.iarena + S.bar);
+    c.gen = __hip_atomic_load(&c.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    c.tred = S.arena + S.tred;
+    c.tpar = 0;
     if (args.n_round == 0) {
         compute(c, S, args.mode);
         return;
@@ -969,25 +991,25 @@ __global__ void __launch_bounds__(NT) engine_kernel(SysDesc* systems, RunArgs ar
         if (args.frame_interval > 0 && round % args.frame_interval == 0) {
             compute(c, S, 1);
             if (c.I[S.error]) return;
-            if (args.frame_pot && c.tid == 0 && nf < args.n_frames)
-                args.frame_pot[blockIdx.x * args.n_frames + nf] = total_potential(c, S);
+            if (args.frame_pot && c.gtid == 0 && nf < args.n_frames)
+                args.frame_pot[sys * args.n_frames + nf] = total_potential(c, S);
             ++nf;
-            __syncthreads();
+            team_sync(c);
         }
-        unsigned long long t0 = S.prof ? wall_clock64() : 0;
+        unsigned long long t0 = ptick(c);
         if (args.thermostat_interval > 0 && round % args.thermostat_interval == 0) thermostat(c, S);
-        prof_add(S, S.n_germ + S.n_deriv + PROF_THERMO, t0);
+        pacc(c, PROF_THERMO, t0);
         for (int stage = 0; stage < 3; ++stage) {
             compute(c, S, 0);
             if (c.I[S.error]) return;  // capacity overflow: the host repeats the run
-            t0 = S.prof ? wall_clock64() : 0;
-            for (int i = c.tid; i < S.n_atom * 3; i += NT) {
+            t0 = ptick(c);
+            for (int i = c.gtid; i < S.n_atom * 3; i += c.gsize) {
                 float p = mom[i] - args.dt * der[i];
                 mom[i] = p;
                 pos[i] += args.dt * p;
             }
-            __syncthreads();
-            prof_add(S, S.n_germ + S.n_deriv + PROF_INTEGRATE, t0);
+            team_sync(c);
+            pacc(c, PROF_INTEGRATE, t0);
         }
     }
 }
@@ -1004,6 +1026,31 @@ hipError_t upside_launch_engine(upside::SysDesc* d_systems, int n_sys, upside::R
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    hipLaunchKernelGGL(upside::engine_kernel, dim3(n_sys), dim3(upside::NT), lds_bytes, stream, d_systems, args);
-    return hipGetLastError();
+    if (args.team <= 1) {
+        args.team = 1;
+        hipLaunchKernelGGL(upside::engine_kernel, dim3(n_sys), dim3(upside::NT), lds_bytes, stream, d_systems, args);
+        return hipGetLastError();
+    }
+    // teams wait for each other: every workgroup must be resident at once,
+    // which a cooperative launch guarantees (or refuses)
+    void* kargs[] = {&d_systems, &args};
+    return hipLaunchCooperativeKernel((const void*)upside::engine_kernel, dim3(n_sys * args.team), dim3(upside::NT),
+                                      kargs, (unsigned)lds_bytes, stream);
+}
+
+// Workgroups of the engine kernel the device can hold at once.
+int upside_max_resident_blocks() {
+    static int cached = -1;
+    if (cached >= 0) return cached;
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1;
+    const size_t lds_bytes = upside::LDS_FLOATS * sizeof(float);
+    (void)hipFuncSetAttribute((const void*)upside::engine_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds_bytes);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)upside::engine_kernel, upside::NT,
+                                                     lds_bytes) != hipSuccess)
+        return 1;
+    cached = per_cu * cus;
+    return cached;
 }

@@ -63,6 +63,10 @@ def load_upside_library(path=LIB_PATH):
         "upside_batch_bp_stats": (i, [vp, i, vp, vp, vp, vp]),
         "upside_batch_last_run_ms": (f, [vp]),
         "upside_batch_enable_profile": (i, [vp, i]),
+        "upside_batch_max_team": (i, [vp]),
+        "upside_batch_set_team": (i, [vp, i]),
+        "upside_batch_get_team": (i, [vp]),
+        "upside_batch_autotune": (i, [vp, i, vp, i, vp]),
         "upside_batch_profile_report": (i, [vp, i, ct.c_char_p, i]),
     }
     for name, (res, args) in sig.items():
@@ -236,6 +240,25 @@ class Batch:
 
     def last_run_ms(self):
         return float(self.calc.upside_batch_last_run_ms(self.h))
+
+    def max_team(self):
+        return int(self.calc.upside_batch_max_team(self.h))
+
+    def team(self):
+        return int(self.calc.upside_batch_get_team(self.h))
+
+    def set_team(self, team):
+        if self.calc.upside_batch_set_team(self.h, int(team)):
+            raise RuntimeError("team size %d not possible" % team)
+
+    def autotune(self, n_round=3, candidates=(1, 2, 4, 8, 16, 32)):
+        """Time n_round rounds per team size from the current state, keep the
+        fastest; returns {team: kernel ms} of the candidates that ran."""
+        cand = np.asarray(candidates, dtype=np.intc)
+        ms = np.zeros(len(cand), dtype="f4")
+        if self.calc.upside_batch_autotune(self.h, int(n_round), cand.ctypes.data, len(cand), ms.ctypes.data):
+            raise RuntimeError("autotune failed")
+        return {int(t): float(m) for t, m in zip(cand, ms) if m >= 0}
 
     def enable_profile(self, on=True):
         if self.calc.upside_batch_enable_profile(self.h, int(on)):

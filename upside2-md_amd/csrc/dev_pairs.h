@@ -141,7 +141,8 @@ struct LGrid {
     float ox, oy, oz, inv;
     int mask;
     int* ofs;      // [mask+2]
-    float4* el;    // [m]
+    float4* el;    // [m]  (x, y, z, element index)
+    int* eid;      // [m]  element ids, same order (no global loads in the scan)
 };
 
 __device__ __forceinline__ int cell_coord(float x, float o, float inv) {
@@ -161,12 +162,12 @@ __device__ __forceinline__ int lgrid_buckets(int m) {
 }
 __device__ __forceinline__ bool lgrid_fits(int m) {
     int nb = lgrid_buckets(m);
-    return (nb + 4) * 4 + m * 16 <= LDS_FLOATS * 4;
+    return (nb + 4) * 4 + m * 20 <= LDS_FLOATS * 4;
 }
 
 // Build the grid of the m points pts (stride 3) in this workgroup's LDS
 // (every workgroup of a team builds its own copy).  All threads of the workgroup.
-__device__ LGrid lgrid_build(Ctx& c, const PairGraph& G, const float* pts, int m) {
+__device__ LGrid lgrid_build(Ctx& c, const PairGraph& G, const float* pts, const int* ids, int m) {
     float mn0 = 3.0e38f, mn1 = 3.0e38f, mn2 = 3.0e38f;
     for (int i = c.tid; i < m; i += NT) {
         mn0 = fminf(mn0, pts[i * 3]);
@@ -184,6 +185,7 @@ __device__ LGrid lgrid_build(Ctx& c, const PairGraph& G, const float* pts, int m
     g.mask = nb - 1;
     g.ofs = reinterpret_cast<int*>(c.lds);
     g.el = reinterpret_cast<float4*>(c.lds + ((nb + 4) & ~3));
+    g.eid = reinterpret_cast<int*>(g.el + m);
     for (int b = c.tid; b <= nb; b += NT) g.ofs[b] = 0;
     __syncthreads();
     for (int i = c.tid; i < m; i += NT) {
@@ -198,6 +200,7 @@ __device__ LGrid lgrid_build(Ctx& c, const PairGraph& G, const float* pts, int m
         int b = cell_bucket(g.mask, cell_coord(x, g.ox, g.inv), cell_coord(y, g.oy, g.inv), cell_coord(z, g.oz, g.inv));
         int p = atomicAdd(&g.ofs[b + 1], 1);  // ends at the start of bucket b+1
         g.el[p] = make_float4(x, y, z, __int_as_float(i));
+        g.eid[p] = ids[i];
     }
     __syncthreads();
     // canonical order inside every bucket: by element index
@@ -205,13 +208,16 @@ __device__ LGrid lgrid_build(Ctx& c, const PairGraph& G, const float* pts, int m
         int lo = g.ofs[b], hi = g.ofs[b + 1];
         for (int k = lo + 1; k < hi; ++k) {
             float4 v = g.el[k];
+            int vid = g.eid[k];
             int key = __float_as_int(v.w);
             int t = k - 1;
             while (t >= lo && __float_as_int(g.el[t].w) > key) {
                 g.el[t + 1] = g.el[t];
+                g.eid[t + 1] = g.eid[t];
                 --t;
             }
             g.el[t + 1] = v;
+            g.eid[t + 1] = vid;
         }
     }
     __syncthreads();
@@ -221,7 +227,7 @@ __device__ LGrid lgrid_build(Ctx& c, const PairGraph& G, const float* pts, int m
 // Query rows against the grid; same contract as pair_scan.
 template <bool SYM, bool FILL>
 __device__ void pair_scan_grid(Ctx& c, const PairGraph& G, const LGrid& g, const float* qp, const int* qid, int nq,
-                               const int* eid, int* up_n, int* lo_n, int* up_dst, int* lo_dst) {
+                               int* up_n, int* lo_n, int* up_dst, int* lo_dst) {
     const float cc2 = G.cache_cut2;
     for (int q = c.gtid; q < nq; q += c.gsize) {
         float x0 = qp[q * 3], x1 = qp[q * 3 + 1], x2 = qp[q * 3 + 2];
@@ -244,7 +250,7 @@ __device__ void pair_scan_grid(Ctx& c, const PairGraph& G, const LGrid& g, const
                             continue;  // another cell sharing the bucket
                         int j = __float_as_int(e.w);
                         if (SYM && j == q) continue;
-                        if (!pg_accept(G, a, eid[j])) continue;
+                        if (!pg_accept(G, a, g.eid[t])) continue;
                         if (!SYM || j > q) {
                             if (FILL) up_dst[up] = j;
                             ++up;
@@ -270,8 +276,12 @@ template <bool SYM, bool FILL>
 __device__ void pair_scan_any(Ctx& c, const PairGraph& G, const float* qp, const int* qid, int nq, const float* ep,
                               const int* eid, int ne, int* up_n, int* lo_n, int* up_dst, int* lo_dst) {
     if (lgrid_fits(ne)) {
-        LGrid g = lgrid_build(c, G, ep, ne);
-        pair_scan_grid<SYM, FILL>(c, G, g, qp, qid, nq, eid, up_n, lo_n, up_dst, lo_dst);
+        unsigned long long t0 = ptick(c);
+        LGrid g = lgrid_build(c, G, ep, eid, ne);
+        pacc(c, PROF_RB_GRID, t0);
+        t0 = ptick(c);
+        pair_scan_grid<SYM, FILL>(c, G, g, qp, qid, nq, up_n, lo_n, up_dst, lo_dst);
+        pacc(c, PROF_RB_SCAN, t0);
     } else {
         pair_scan<SYM, FILL>(c, G, qp, qid, nq, ep, eid, ne, up_n, lo_n, up_dst, lo_dst);
     }

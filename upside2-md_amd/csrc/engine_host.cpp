@@ -776,7 +776,18 @@ struct Builder {
         return (int)T.graphs.size() - 1;
     }
 
+    // The Verlet buffer only trades cache rebuild frequency against cache
+    // size: the edge set is the exact refine of the cache whatever its radius.
+    // The reference uses 1 + 0.2*cutoff (src/interaction_graph.h:397); this
+    // engine scales it by UPSIDE_BUFFER_SCALE (default 1).
     void finish_graph(PairGraph& G, float cutoff, float buffer, int want) {
+        if (const char* e = std::getenv("UPSIDE_BUFFER_SCALE")) {
+            double f = std::atof(e);
+            if (f >= 0.25 && f <= 8.0) {
+                buffer = (float)(buffer * f);
+                want = (int)(want * std::pow((cutoff + buffer) / (cutoff + buffer / f), 3.0) + 1);
+            }
+        }
         G.cutoff2 = cutoff * cutoff;
         float cc = cutoff + buffer;
         G.cache_cut2 = cc * cc;
@@ -1222,7 +1233,8 @@ struct Batch {
                                            "pair sensitivity gathers, all graphs",
                                            "rotamer fill (in value)", "rotamer candidates (in rebuild)",
                                            "rebuild: count scans", "rebuild: fill scans",
-                                           "rebuild: transposed slot mapping"};
+                                           "rebuild: transposed slot mapping",
+                                           "rebuild: LDS grid builds", "rebuild: grid scans"};
                 label = ex[i - ng - (int)T.deriv.size()];
             }
             snprintf(line, sizeof(line), "%-48s %12.6f s\n", label.c_str(), t[i] * tick);

@@ -43,7 +43,7 @@ import upconfig  # noqa: E402
 METRIC = "residue-steps/s vs plain GPU & CPU at ≤1e-4 rel force err; BP iters/step"
 N_RES, SEED, TEMP, DT = 500, 1, 0.85, 0.009
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "engine_kernel_chain500_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "engine_step_kernel_chain500_traffic.txt")
 
 
 def log(*a):
@@ -183,6 +183,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--cpu-rounds", type=int, default=400, help="rounds of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--team", type=int, default=0,
+                    help="workgroups per system (0: time 1..32 on the warmed state and keep the fastest)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only an elementwise copy of 1 GiB (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -220,6 +222,13 @@ def main():
     probe = upside_engine.Upside(path, quiet=True)
     if warm_rounds:
         b.run(warm_rounds)
+    tune = None
+    if args.team > 0:
+        b.set_team(args.team)
+    else:
+        cand = [t for t in (1, 2, 4, 8, 16, 32) if t <= b.max_team()]
+        tune = b.autotune(6, cand)       # restores the warmed state after each candidate
+        log("team autotune (ms per 6 rounds): %s -> team %d" % (tune, b.team()))
     probe.energy(b.pos(0))
     g0, bp0, nodes = graph_counts(probe, node_names)
     small = small_node_bytes(probe, node_names, set(g0))
@@ -257,7 +266,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "one %d-residue seeded random-sequence chain (seed %d), random-Rama start, "
                                "ff_2.1, FP32, T=%.2f, dt=%.3f, thermostat every round" % (N_RES, SEED, TEMP, DT),
-                   "n_res": N_RES, "n_atom": n_atom, "launch": "one persistent launch of all steps"},
+                   "n_res": N_RES, "n_atom": n_atom, "launch": "one persistent launch of all steps",
+                   "workgroups_per_system": b.team()},
         "bp_iters_per_step": iters / max(solves, 1),
         "bp_passes_per_solve": passes / max(solves, 1),
         "force_rel_err_vs_reference_at_end": fe,
@@ -265,7 +275,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (tps * steps) if tps is not None else None,
                      "algorithmic_bytes_per_step": per_step, "algorithmic_bytes_per_bp_pass": per_pass,
-                     "kernel_ms": kernel_ms, "kernel": "upside::engine_kernel"},
+                     "kernel_ms": kernel_ms, "kernel": "upside::engine_step_kernel"},
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -10,12 +10,13 @@
 // src/interaction_graph.h:51-169) only bounds which pairs are examined, so the
 // edge set does not depend on how the cache was built.
 //
-// Rebuild (all in one workgroup): the other side is put into a hashed cell
-// grid in LDS (or streamed in tiles when it does not fit) and every row scans
-// it twice, once to count partners and, after a block scan of the counts,
-// once to write them.  Row order is canonical (fixed cell-visit order, index
-// order inside a cell); transposed rows are scanned the same way from the
-// other side and mapped to forward slots through a hash table.
+// Rebuild: the other side is put into a hashed cell grid in LDS (or streamed
+// in tiles when it does not fit) and every row scans it twice, once to count
+// partners and, after a team scan of the counts, once to write them.  Row
+// order is canonical (fixed cell-visit order, index order inside a cell).
+// Transposed rows (the forward slots of each side-2 element) are placed by
+// per-column counters and then ordered by slot, so every order is
+// independent of scheduling.
 #pragma once
 #include "dev_common.h"
 
@@ -67,10 +68,43 @@ __device__ int lower_bound_i(const int* a, int n, int v) {
     return lo;
 }
 
-// ---- cache rebuild: LDS-tiled scans ------------------------------------------
-// Fallback scan: the other side streamed through LDS in tiles; every wave
-// reads the same tile entries (LDS broadcast), each lane tests its own query.
-// Partners come out in increasing index.
+// ---- cache rebuild -------------------------------------------------------------
+// Rows are scanned by groups of SG lanes: the lanes of a group stride one
+// query's candidates together and emit the hits in candidate order through a
+// ballot, so a row costs ceil(candidates / SG) dependent steps instead of one
+// step per candidate.
+constexpr int SG = 8;
+
+struct GroupLanes {
+    int gl;                    // lane within the group
+    unsigned long long gmask;  // the group's lanes in the wave
+    unsigned long long below;  // wave lanes below this one
+};
+__device__ __forceinline__ GroupLanes group_lanes(const Ctx& c) {
+    GroupLanes L;
+    int lane = c.tid & 63;
+    L.gl = lane & (SG - 1);
+    L.gmask = ((1ull << SG) - 1ull) << (lane & ~(SG - 1));
+    L.below = (1ull << lane) - 1ull;
+    return L;
+}
+
+// One SG-wide step of a row: returns the group's hits; FILL writes each hit at
+// cur + (hits of the lower lanes), so hits keep candidate order.  Every lane of
+// the group calls it.
+template <bool FILL>
+__device__ __forceinline__ int group_emit(const GroupLanes& L, bool hit, int cur, int q, int j, int* nbr, int* row) {
+    unsigned long long m = __ballot(hit) & L.gmask;
+    if (FILL && hit) {
+        int p = cur + __popcll(m & L.below);
+        nbr[p] = j;
+        row[p] = q;
+    }
+    return __popcll(m);
+}
+
+// Fallback scan for sides too large for the LDS grid: the other side streamed
+// through LDS in tiles of TILE elements, partners in increasing index.
 constexpr int TILE = LDS_FLOATS / 4;
 
 // Stage elements [t0, t0+n) of side pts (ids in id) as float4 (x, y, z, id).
@@ -83,60 +117,49 @@ __device__ void stage_tile(Ctx& c, const float* pts, const int* id, int t0, int 
     __syncthreads();
 }
 
-// One scan of all queries q in [0, nq) (positions qp, ids qid) against all
-// elements of the other side (positions ep, ids eid, count ne), for accepted
-// pairs within the cache radius.  SYM: partners j > q go to the "up" list, j < q
-// to the "lo" list (j == q skipped); otherwise every partner is "up".
-// FILL=false adds the counts into up_n[q] / lo_n[q]; FILL=true writes the
-// partners, in increasing j, at cursors up_n[q] / lo_n[q] (advanced).  Counts
-// and cursors live in registers while a row is scanned.  All threads.
+// Rows q in [0, nq) (positions qp, ids qid) against the elements ep[0..ne)
+// (ids eid): a pair (q, j) is a hit when within the cache radius and
+// acceptable (SYM: also j > q).  !FILL adds row q's hits to off[q].  FILL
+// writes them at the cursor cur[q] (advanced).
 template <bool SYM, bool FILL>
-__device__ void pair_scan(Ctx& c, const PairGraph& G, const float* qp, const int* qid, int nq, const float* ep,
-                          const int* eid, int ne, int* up_n, int* lo_n, int* up_dst, int* lo_dst) {
+__device__ void tile_rows(Ctx& c, const PairGraph& G, const float* qp, const int* qid, int nq, const float* ep,
+                          const int* eid, int ne, int* off, int* cur_arr, int* nbr, int* row) {
     const float4* tile = reinterpret_cast<const float4*>(c.lds);
     const float cc2 = G.cache_cut2;
+    const GroupLanes L = group_lanes(c);
+    const int grp = c.gtid / SG, ngrp = c.gsize / SG;
     for (int t0 = 0; t0 < ne; t0 += TILE) {
         int tn = min(TILE, ne - t0);
         stage_tile(c, ep, eid, t0, tn);
-        for (int q = c.gtid; q < nq; q += c.gsize) {
+        for (int q = grp; q < nq; q += ngrp) {
             float x0 = qp[q * 3], x1 = qp[q * 3 + 1], x2 = qp[q * 3 + 2];
             int a = qid[q];
-            int up = FILL ? up_n[q] : 0;
-            int lo = (FILL && SYM) ? lo_n[q] : 0;
-            for (int k = 0; k < tn; ++k) {
-                float4 e = tile[k];
-                float d2 = refine_dist2(x0 - e.x, x1 - e.y, x2 - e.z);
-                int j = t0 + k;
-                if (d2 < cc2 && !(SYM && j == q) && pg_accept(G, a, __float_as_int(e.w))) {
-                    if (!SYM || j > q) {
-                        if (FILL) up_dst[up] = j;
-                        ++up;
-                    } else {
-                        if (FILL) lo_dst[lo] = j;
-                        ++lo;
-                    }
+            int cur = FILL ? cur_arr[q] : 0;
+            for (int k0 = 0; k0 < tn; k0 += SG) {
+                int k = k0 + L.gl, j = t0 + k;
+                bool hit = false;
+                if (k < tn) {
+                    float4 e = tile[k];
+                    float d2 = refine_dist2(x0 - e.x, x1 - e.y, x2 - e.z);
+                    hit = d2 < cc2 && (!SYM || j > q) && pg_accept(G, a, __float_as_int(e.w));
                 }
+                cur += group_emit<FILL>(L, hit, cur, q, j, nbr, row);
             }
-            if (FILL) {
-                up_n[q] = up;
-                if (SYM) lo_n[q] = lo;
-            } else {
-                up_n[q] += up;
-                if (SYM) lo_n[q] += lo;
+            if (L.gl == 0) {
+                if (FILL) cur_arr[q] = cur;
+                else off[q] += cur;
             }
         }
-        __syncthreads();  // the tile is restaged next
+        __syncthreads();  // the tile is restaged next; cursors visible to the group
     }
     team_sync(c);
 }
 
 // ---- cache rebuild: cell grid in LDS ----------------------------------------
 // The other side is counting-sorted into hashed cells of edge >= the cache
-// radius, kept in LDS as (bucket offsets, float4 elements (x, y, z, index)),
-// each bucket sorted by element index.  A query visits its 27 neighbouring
-// cells in a fixed order, so rows come out in a canonical order.  When the
-// grid does not fit in LDS the scan streams the other side in tiles instead
-// (same emission semantics, partners in increasing index).
+// radius, kept in LDS as (bucket offsets, float4 elements (x, y, z, index),
+// element ids), each bucket sorted by element index.  A row visits its 27
+// neighbouring cells in a fixed order, so rows come out in a canonical order.
 struct LGrid {
     float ox, oy, oz, inv;
     int mask;
@@ -224,167 +247,172 @@ __device__ LGrid lgrid_build(Ctx& c, const PairGraph& G, const float* pts, const
     return g;
 }
 
-// Query rows against the grid; same contract as pair_scan.
+// Rows against the grid; same hit rule as tile_rows.  !FILL: off[q] = hits of
+// row q.  FILL: row q's hits are written from off[q] on.
 template <bool SYM, bool FILL>
-__device__ void pair_scan_grid(Ctx& c, const PairGraph& G, const LGrid& g, const float* qp, const int* qid, int nq,
-                               int* up_n, int* lo_n, int* up_dst, int* lo_dst) {
+__device__ void grid_rows(Ctx& c, const PairGraph& G, const LGrid& g, const float* qp, const int* qid, int nq,
+                          int* off, int* nbr, int* row) {
     const float cc2 = G.cache_cut2;
-    for (int q = c.gtid; q < nq; q += c.gsize) {
+    const GroupLanes L = group_lanes(c);
+    const int grp = c.gtid / SG, ngrp = c.gsize / SG;
+    for (int q = grp; q < nq; q += ngrp) {
         float x0 = qp[q * 3], x1 = qp[q * 3 + 1], x2 = qp[q * 3 + 2];
         int a = qid[q];
-        int up = FILL ? up_n[q] : 0;
-        int lo = (FILL && SYM) ? lo_n[q] : 0;
+        int cur = FILL ? off[q] : 0;
         int cx = cell_coord(x0, g.ox, g.inv), cy = cell_coord(x1, g.oy, g.inv), cz = cell_coord(x2, g.oz, g.inv);
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy)
                 for (int dx = -1; dx <= 1; ++dx) {
                     int ax = cx + dx, ay = cy + dy, az = cz + dz;
                     int b = cell_bucket(g.mask, ax, ay, az);
-                    int e1 = g.ofs[b + 1];
-                    for (int t = g.ofs[b]; t < e1; ++t) {
-                        float4 e = g.el[t];
-                        float d2 = refine_dist2(x0 - e.x, x1 - e.y, x2 - e.z);
-                        if (!(d2 < cc2)) continue;
-                        if (cell_coord(e.x, g.ox, g.inv) != ax || cell_coord(e.y, g.oy, g.inv) != ay ||
-                            cell_coord(e.z, g.oz, g.inv) != az)
-                            continue;  // another cell sharing the bucket
-                        int j = __float_as_int(e.w);
-                        if (SYM && j == q) continue;
-                        if (!pg_accept(G, a, g.eid[t])) continue;
-                        if (!SYM || j > q) {
-                            if (FILL) up_dst[up] = j;
-                            ++up;
-                        } else {
-                            if (FILL) lo_dst[lo] = j;
-                            ++lo;
+                    int t1 = g.ofs[b + 1];
+                    for (int t = g.ofs[b]; t < t1; t += SG) {
+                        int k = t + L.gl, j = 0;
+                        bool hit = false;
+                        if (k < t1) {
+                            float4 e = g.el[k];
+                            j = __float_as_int(e.w);
+                            float d2 = refine_dist2(x0 - e.x, x1 - e.y, x2 - e.z);
+                            // the cell test drops elements of other cells sharing the bucket
+                            hit = d2 < cc2 && (!SYM || j > q) && cell_coord(e.x, g.ox, g.inv) == ax &&
+                                  cell_coord(e.y, g.oy, g.inv) == ay && cell_coord(e.z, g.oz, g.inv) == az &&
+                                  pg_accept(G, a, g.eid[k]);
                         }
+                        cur += group_emit<FILL>(L, hit, cur, q, j, nbr, row);
                     }
                 }
-        if (FILL) {
-            up_n[q] = up;
-            if (SYM) lo_n[q] = lo;
-        } else {
-            up_n[q] += up;
-            if (SYM) lo_n[q] += lo;
-        }
+        if (!FILL && L.gl == 0) off[q] = cur;
     }
     team_sync(c);
 }
 
-// Scan by grid when the other side fits in LDS, else by tiles.
-template <bool SYM, bool FILL>
-__device__ void pair_scan_any(Ctx& c, const PairGraph& G, const float* qp, const int* qid, int nq, const float* ep,
-                              const int* eid, int ne, int* up_n, int* lo_n, int* up_dst, int* lo_dst) {
-    if (lgrid_fits(ne)) {
-        unsigned long long t0 = ptick(c);
-        LGrid g = lgrid_build(c, G, ep, eid, ne);
-        pacc(c, PROF_RB_GRID, t0);
-        t0 = ptick(c);
-        pair_scan_grid<SYM, FILL>(c, G, g, qp, qid, nq, up_n, lo_n, up_dst, lo_dst);
-        pacc(c, PROF_RB_SCAN, t0);
-    } else {
-        pair_scan<SYM, FILL>(c, G, qp, qid, nq, ep, eid, ne, up_n, lo_n, up_dst, lo_dst);
+// Order every transposed row by forward slot: entry tmp[t] of column j (the
+// slots whose partner is j, in arrival order) goes to tslot[toff[j] + r],
+// r = number of entries of the column with a smaller slot.  Columns are split
+// among the team's workgroups; each workgroup stages runs of whole columns in
+// LDS.  All threads of the team.
+__device__ void pg_order_columns(Ctx& c, const PairGraph& G, int n2) {
+    const int* toff = c.I + G.toff;
+    const int* tmp = c.I + G.tmp;
+    const int* nbr = c.I + G.nbr;
+    int* tslot = c.I + G.tslot;
+    int* sl = reinterpret_cast<int*>(c.lds);
+    const int CAP = LDS_FLOATS;
+    int per = (n2 + c.G - 1) / c.G;
+    int jb = min(c.rank * per, n2);
+    const int jend = min(jb + per, n2);
+    while (jb < jend) {
+        const int base = toff[jb];
+        // the longest run of whole columns [jb, je) that fits: largest je in
+        // [jb+1, jend] with toff[je] - base <= CAP (je = jb+1 if none)
+        int lo = jb + 1, hi = jend;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (toff[mid] - base <= CAP) lo = mid; else hi = mid - 1;
+        }
+        const int je = lo;
+        const int n = toff[je] - base;
+        if (n <= CAP) {
+            for (int k = c.tid; k < n; k += NT) sl[k] = tmp[base + k];
+            __syncthreads();
+            for (int k = c.tid; k < n; k += NT) {
+                int s = sl[k];
+                int j = nbr[s];
+                int u0 = toff[j] - base, u1 = toff[j + 1] - base;
+                int r = 0;
+                for (int u = u0; u < u1; ++u) r += sl[u] < s;
+                tslot[base + u0 + r] = s;
+            }
+            __syncthreads();
+        } else {  // one column longer than the LDS area: rank from memory
+            for (int k = c.tid; k < n; k += NT) {
+                int s = tmp[base + k];
+                int r = 0;
+                for (int u = 0; u < n; ++u) r += tmp[base + u] < s;
+                tslot[base + r] = s;
+            }
+        }
+        jb = je;
     }
+    team_sync(c);
 }
-
-// Slot lookup table: open addressing on key i*n2 + j (HCAP a power of two).
-__device__ __forceinline__ uint32_t slot_hash(uint32_t k) { return k * 2654435761u; }
 
 // Rebuild rows and transposed rows from pos_pack.  Sets state[1] on overflow
 // (the host grows the capacity and repeats the run).
-//   forward rows:    side-1 element i, partners j (symmetric: j > i);
-//   transposed rows: side-2 element j, partners i (symmetric: i < j), stored
-//                    as the forward slot of (i, j) found through a hash table.
-// Both row orders are canonical (see the scans above).
+//   forward rows:    side-1 element i, partners j (symmetric: j > i) in the
+//                    canonical scan order;
+//   transposed rows: side-2 element j, the forward slots with partner j in
+//                    increasing slot order.
 __device__ void pg_rebuild(Ctx& c, const PairGraph& G) {
     const int* id1 = c.CI + G.id1;
-    const int* id2 = c.CI + G.id2;
+    const bool sym = G.sym != 0;
+    const int* id2 = sym ? id1 : c.CI + G.id2;
     float* pack = c.A + G.pos_pack;
     float* cache = c.A + G.cache_pos;
     const float* p1 = pack;
     const float* p2 = pg_side2(c, G);
+    const int n1 = G.n1, n2 = sym ? G.n1 : G.n2;
     int* off = c.I + G.off;
     int* nbr = c.I + G.nbr;
     int* row = c.I + G.row;
     int* toff = c.I + G.toff;
-    int* tslot = c.I + G.tslot;
     int* tmp = c.I + G.tmp;
-    int* hkey = c.I + G.hkey;
-    int* hval = c.I + G.hval;
-    const bool sym = G.sym != 0;
-    int npack = sym ? G.n1 : G.n1 + G.n2;
+    int* tcur = c.I + G.tcur;
+    int npack = sym ? n1 : n1 + n2;
     for (int i = c.gtid; i < npack * 3; i += c.gsize) cache[i] = pack[i];
-    for (int i = c.gtid; i <= G.n1; i += c.gsize) off[i] = 0;
-    for (int j = c.gtid; j <= G.n2; j += c.gsize) toff[j] = 0;
+    for (int i = c.gtid; i <= n1; i += c.gsize) off[i] = 0;
+    for (int j = c.gtid; j <= n2; j += c.gsize) toff[j] = 0;
     team_sync(c);
     unsigned long long t0 = ptick(c);
-    // counts (each query row is owned by one thread)
-    if (sym) {
-        pair_scan_any<true, false>(c, G, p1, id1, G.n1, p1, id1, G.n1, off, toff, nullptr, nullptr);
-    } else {
-        pair_scan_any<false, false>(c, G, p1, id1, G.n1, p2, id2, G.n2, off, nullptr, nullptr, nullptr);
-        pair_scan_any<false, false>(c, G, p2, id2, G.n2, p1, id1, G.n1, toff, nullptr, nullptr, nullptr);
+    // the grid (when it fits) stays in LDS from the count scan to the fill scan
+    const bool use_grid = lgrid_fits(n2);
+    LGrid g{};
+    if (use_grid) {
+        g = lgrid_build(c, G, p2, id2, n2);
+        pacc(c, PROF_RB_GRID, t0);
     }
-    int E = team_exscan(c, off, G.n1 + 1);
-    int Et = team_exscan(c, toff, G.n2 + 1);
-    if (E > G.ecap || Et != E) {  // Et == E always holds: both scans apply one symmetric test
-        if (c.gtid == 0) { c.I[G.state + 1] = E > G.ecap ? 1 : 2; c.I[G.state + 0] = 0; }
+    unsigned long long t1 = ptick(c);
+    if (use_grid) {
+        if (sym) grid_rows<true, false>(c, G, g, p1, id1, n1, off, nullptr, nullptr);
+        else grid_rows<false, false>(c, G, g, p1, id1, n1, off, nullptr, nullptr);
+    } else {
+        if (sym) tile_rows<true, false>(c, G, p1, id1, n1, p2, id2, n2, off, nullptr, nullptr, nullptr);
+        else tile_rows<false, false>(c, G, p1, id1, n1, p2, id2, n2, off, nullptr, nullptr, nullptr);
+    }
+    pacc(c, PROF_RB_SCAN, t1);
+    const int E = team_exscan(c, off, n1 + 1);
+    if (E > G.ecap) {
+        if (c.gtid == 0) { c.I[G.state + 1] = 1; c.I[G.state + 0] = 0; }
         team_sync(c);
         return;
     }
     pacc(c, PROF_RB_COUNT, t0);
     t0 = ptick(c);
-    // fill: forward partners into nbr, transposed partners into tmp
-    for (int i = c.gtid; i < G.n1; i += c.gsize) row[i] = off[i];        // row[] as a cursor for now
-    for (int j = c.gtid; j < G.n2; j += c.gsize) tslot[j] = toff[j];     // tslot[] as a cursor for now
-    for (int h = c.gtid; h < G.hcap; h += c.gsize) hkey[h] = -1;
-    team_sync(c);
-    if (sym) {
-        pair_scan_any<true, true>(c, G, p1, id1, G.n1, p1, id1, G.n1, row, tslot, nbr, tmp);
+    t1 = ptick(c);
+    if (use_grid) {
+        if (sym) grid_rows<true, true>(c, G, g, p1, id1, n1, off, nbr, row);
+        else grid_rows<false, true>(c, G, g, p1, id1, n1, off, nbr, row);
     } else {
-        pair_scan_any<false, true>(c, G, p1, id1, G.n1, p2, id2, G.n2, row, nullptr, nbr, nullptr);
-        pair_scan_any<false, true>(c, G, p2, id2, G.n2, p1, id1, G.n1, tslot, nullptr, tmp, nullptr);
+        for (int i = c.gtid; i < n1; i += c.gsize) tcur[i] = off[i];
+        team_sync(c);
+        if (sym) tile_rows<true, true>(c, G, p1, id1, n1, p2, id2, n2, off, tcur, nbr, row);
+        else tile_rows<false, true>(c, G, p1, id1, n1, p2, id2, n2, off, tcur, nbr, row);
     }
-    for (int i = c.gtid; i < G.n1; i += c.gsize)
-        for (int s = off[i]; s < off[i + 1]; ++s) row[s] = i;
-    team_sync(c);
+    pacc(c, PROF_RB_SCAN, t1);
     pacc(c, PROF_RB_FILL, t0);
     t0 = ptick(c);
-    // slot table: (i, j) -> forward slot
-    const uint32_t hm = (uint32_t)G.hcap - 1u;
-    const int n2 = G.n2;
-    int bad = 0;
-    for (int s = c.gtid; s < E; s += c.gsize) {
-        int key = row[s] * n2 + nbr[s];
-        uint32_t h = slot_hash((uint32_t)key) & hm;
-        for (int probe = 0; probe < G.hcap; ++probe) {
-            int prev = atomicCAS(&hkey[h], -1, key);
-            if (prev == -1) { hval[h] = s; break; }
-            h = (h + 1u) & hm;
-        }
-    }
+    // transposed rows: column counts, arrival-order placement, slot order
+    for (int s = c.gtid; s < E; s += c.gsize) atomicAdd(&toff[nbr[s]], 1);
     team_sync(c);
-    // transposed entries -> forward slots
-    int nt = sym ? G.n1 : G.n2;
-    for (int j = c.gtid; j < nt; j += c.gsize)
-        for (int t = toff[j]; t < toff[j + 1]; ++t) {
-            int key = tmp[t] * n2 + j;
-            uint32_t h = slot_hash((uint32_t)key) & hm;
-            int found = -1;
-            for (int probe = 0; probe < G.hcap; ++probe) {
-                int k = hkey[h];
-                if (k == key) { found = hval[h]; break; }
-                if (k == -1) break;
-                h = (h + 1u) & hm;
-            }
-            if (found < 0) { bad = 1; found = 0; }
-            tslot[t] = found;
-        }
-    int anybad = team_or(c, bad);
+    team_exscan(c, toff, n2 + 1);
+    for (int j = c.gtid; j < n2; j += c.gsize) tcur[j] = toff[j];
+    team_sync(c);
+    for (int s = c.gtid; s < E; s += c.gsize) tmp[atomicAdd(&tcur[nbr[s]], 1)] = s;
+    team_sync(c);
+    pg_order_columns(c, G, n2);
     pacc(c, PROF_RB_MAP, t0);
     if (c.gtid == 0) {
-        c.I[G.state + 0] = anybad ? 0 : 1;
-        if (anybad) c.I[G.state + 1] = 2;
+        c.I[G.state + 0] = 1;
         c.I[G.state + 2] += 1;
         c.I[G.state + 3] = E;
     }

@@ -72,9 +72,8 @@ struct PairGraph {
     int row;               // int ecap  row of each slot
     int toff;              // int n2+1  transposed row offsets
     int tslot;             // int ecap  slot of each transposed entry
-    int tmp;               // int ecap  transposed partners during a rebuild
-    int hcap;              // slot hash table size (power of two >= 2*ecap)
-    int hkey, hval;        // int hcap  (i*n2 + j) -> slot, rebuilt with the rows
+    int tmp;               // int ecap  transposed entries in arrival order during a rebuild
+    int tcur;              // int max(n1,n2)+1  cursors during a rebuild
     int val;               // float ecap   edge value (0 when inactive)
     int d1, d2;            // float ddim1*ecap, ddim2*ecap (SoA)
     int esens;             // float ecap   edge sensitivity

@@ -806,12 +806,7 @@ struct Builder {
         G.toff = T.ialloc(G.n2 + 1);
         G.tslot = T.ialloc(slots);
         G.tmp = T.ialloc(slots);
-        int hc = 64;
-        while (hc < 2 * G.ecap) hc *= 2;
-        G.hcap = hc;
-        G.hkey = T.ialloc(hc);
-        G.hval = T.ialloc(hc);
-        if ((long)G.n1 * G.n2 >= (1L << 31)) fail("pair graph too large for 32-bit slot keys");
+        G.tcur = T.ialloc(std::max(G.n1, G.n2) + 1);
         G.val = T.falloc(slots);
         G.d1 = T.falloc(slots * G.ddim1);
         G.d2 = T.falloc(slots * G.ddim2);

@@ -954,11 +954,12 @@ __device__ float total_potential(Ctx& c, const SysDesc& S) {
 }
 
 // One launch evaluates or integrates every system of a batch; system s is
-// handled by the team of workgroups [s*team, (s+1)*team).
-__global__ void __launch_bounds__(NT) engine_kernel(SysDesc* systems, RunArgs args) {
-    __shared__ float red[NWAVE];
-    __shared__ int ired[NWAVE];
-    extern __shared__ float lds[];
+// handled by the team of workgroups [s*team, (s+1)*team).  The body is shared
+// by two kernel symbols so that the integration launches (the hot path) and
+// the single evaluations / thermostat applications show up separately in a
+// kernel trace.
+__device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& args, float* red, int* ired,
+                                            float* lds) {
     const int G = args.team;
     const int sys = blockIdx.x / G, rank = blockIdx.x % G;
     SysDesc& S = systems[sys];
@@ -1014,6 +1015,22 @@ __global__ void __launch_bounds__(NT) engine_kernel(SysDesc* systems, RunArgs ar
     }
 }
 
+// Integration rounds (args.n_round > 0).
+__global__ void __launch_bounds__(NT) engine_step_kernel(SysDesc* systems, RunArgs args) {
+    __shared__ float red[NWAVE];
+    __shared__ int ired[NWAVE];
+    extern __shared__ float lds[];
+    engine_body(systems, args, red, ired, lds);
+}
+
+// One evaluation (args.n_round == 0) or one thermostat application (< 0).
+__global__ void __launch_bounds__(NT) engine_eval_kernel(SysDesc* systems, RunArgs args) {
+    __shared__ float red[NWAVE];
+    __shared__ int ired[NWAVE];
+    extern __shared__ float lds[];
+    engine_body(systems, args, red, ired, lds);
+}
+
 }  // namespace upside
 
 // Host-side launcher (C++ linkage inside the library only).
@@ -1021,36 +1038,40 @@ hipError_t upside_launch_engine(upside::SysDesc* d_systems, int n_sys, upside::R
     static bool attr_set = false;
     const size_t lds_bytes = upside::LDS_FLOATS * sizeof(float);
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void*)upside::engine_kernel,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-        if (e != hipSuccess) return e;
+        for (const void* f : {(const void*)upside::engine_step_kernel, (const void*)upside::engine_eval_kernel}) {
+            hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+            if (e != hipSuccess) return e;
+        }
         attr_set = true;
     }
-    if (args.team <= 1) {
-        args.team = 1;
-        hipLaunchKernelGGL(upside::engine_kernel, dim3(n_sys), dim3(upside::NT), lds_bytes, stream, d_systems, args);
-        return hipGetLastError();
-    }
+    const void* fn = args.n_round > 0 ? (const void*)upside::engine_step_kernel
+                                      : (const void*)upside::engine_eval_kernel;
+    if (args.team <= 1) args.team = 1;
+    void* kargs[] = {&d_systems, &args};
+    if (args.team == 1)
+        return hipLaunchKernel(fn, dim3(n_sys), dim3(upside::NT), kargs, lds_bytes, stream);
     // teams wait for each other: every workgroup must be resident at once,
     // which a cooperative launch guarantees (or refuses)
-    void* kargs[] = {&d_systems, &args};
-    return hipLaunchCooperativeKernel((const void*)upside::engine_kernel, dim3(n_sys * args.team), dim3(upside::NT),
-                                      kargs, (unsigned)lds_bytes, stream);
+    return hipLaunchCooperativeKernel(fn, dim3(n_sys * args.team), dim3(upside::NT), kargs, (unsigned)lds_bytes,
+                                      stream);
 }
 
-// Workgroups of the engine kernel the device can hold at once.
+// Workgroups of the engine kernels the device can hold at once (the smaller
+// of the two symbols' occupancies).
 int upside_max_resident_blocks() {
     static int cached = -1;
     if (cached >= 0) return cached;
-    int dev = 0, cus = 0, per_cu = 0;
+    int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1;
     const size_t lds_bytes = upside::LDS_FLOATS * sizeof(float);
-    (void)hipFuncSetAttribute((const void*)upside::engine_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds_bytes);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)upside::engine_kernel, upside::NT,
-                                                     lds_bytes) != hipSuccess)
-        return 1;
-    cached = per_cu * cus;
+    int best = 1 << 30;
+    for (const void* f : {(const void*)upside::engine_step_kernel, (const void*)upside::engine_eval_kernel}) {
+        int per_cu = 0;
+        (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, upside::NT, lds_bytes) != hipSuccess) return 1;
+        best = per_cu < best ? per_cu : best;
+    }
+    cached = best * cus;
     return cached;
 }

@@ -91,6 +91,12 @@ int upside_batch_energies(UpsideBatch* batch, float* energies);
  * drawn from Threefry stream 1 with counter `round`.  accepted[n_pairs] out. */
 int upside_batch_replica_exchange(UpsideBatch* batch, int n_pairs, const int* pairs, uint64_t round,
                                   int* accepted);
+/* Per-term potentials of system `sys` at the last evaluation (each potential
+ * node's `potential`, summed into the total in src/deriv_engine.cpp:240-250): values[slot] for
+ * slot < max_terms, node names '\n'-separated into names.  Returns the number
+ * of terms (-1 on error). */
+int upside_batch_potential_terms(UpsideBatch* batch, int sys, float* values, int max_terms, char* names,
+                                 int names_len);
 /* Summed BP statistics: iterations of the last solve and since creation. */
 int upside_batch_bp_stats(UpsideBatch* batch, int sys, int* last_iters, long long* total_iters,
                           long long* solves, long long* bad_solves);

@@ -15,6 +15,18 @@ constexpr int NWAVE = NT / 64;
 // workgroups (G = 1: a single workgroup, no grid barriers).  Data-parallel
 // loops stride over the team's threads (gtid, gsize); team_sync separates
 // phases whose data cross workgroups.
+// Address-space-typed pointers.  Arena and table pointers reach the device
+// code through descriptors, so the compiler sees them as generic (flat)
+// pointers: every flat load is counted against both the vector-memory and
+// the LDS counters and waits serialise.  Hot loops re-type their pointers
+// with gmem()/lmem() so that loads become global_load / ds_read.
+#define UPS_GLOBAL __attribute__((address_space(1)))
+#define UPS_LDS __attribute__((address_space(3)))
+template <class T> using gp = UPS_GLOBAL T*;
+template <class T> using lp = UPS_LDS T*;
+template <class T> __device__ __forceinline__ gp<T> gmem(T* p) { return (gp<T>)p; }
+template <class T> __device__ __forceinline__ lp<T> lmem(T* p) { return (lp<T>)p; }
+
 struct Ctx {
     float* A;          // per-system float arena
     int* I;            // per-system int arena
@@ -28,8 +40,8 @@ struct Ctx {
     int pbase;                 // first fixed profile slot
     int rank, G;       // workgroup rank in the team, team size
     int gtid, gsize;   // thread in the team, team threads
-    unsigned* bar;     // team barrier words [count, generation, spin-timeout flag]
-    unsigned gen;      // generation this workgroup expects to complete next
+    unsigned long long* bar;  // team barrier arrival counter
+    unsigned* bar_flag;       // team barrier spin-timeout flag
     float* tred;       // team reduction scratch, 2 buffers of G floats
     int tpar;          // buffer parity of the next team reduction
 };
@@ -39,39 +51,37 @@ __device__ __forceinline__ void pacc(const Ctx& c, int slot, unsigned long long 
     if (c.prof && c.gtid == 0) c.prof[c.pbase + slot] += wall_clock64() - t0;
 }
 
-// Team barrier: every workgroup of the team arrives (agent-scope release of
-// its writes), the last one to arrive opens the next generation, the others
-// wait for it (agent-scope acquire).  The wait is bounded: after ~1 s of
-// spinning the workgroup records a timeout and proceeds, so a launch always
-// drains; the host then reports the run as failed.
+// Team barrier on a monotonic 64-bit arrival counter: arrival n of a team of
+// G falls in barrier n / G, which opens when the counter reaches the next
+// multiple of G.  Arrivals are agent-scope releases of the workgroup's writes,
+// the wait an agent-scope acquire.  No generation is read at launch and
+// nothing is reset between barriers, so a stale read can only prolong a wait.
+// The counter is a multiple of G between launches (the host zeroes it when the
+// team size changes).  The wait is bounded: after ~1 s of spinning the
+// workgroup records a timeout and proceeds, so a launch always drains; the
+// host then reports the run as failed.
 constexpr unsigned BAR_SPIN_LIMIT = 1u << 24;
 
 __device__ void team_sync(Ctx& c) {
     __syncthreads();
     if (c.G > 1) {
         if (c.tid == 0) {
-            unsigned g = c.gen;
             __threadfence();
-            unsigned prev = __hip_atomic_fetch_add(&c.bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            if (prev == (unsigned)c.G - 1u) {
-                __hip_atomic_store(&c.bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&c.bar[1], g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                unsigned spins = 0;
-                while (__hip_atomic_load(&c.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
-                    __builtin_amdgcn_s_sleep(2);
-                    // after one timeout anywhere in the team no barrier waits again,
-                    // so the launch drains quickly
-                    if (++spins > BAR_SPIN_LIMIT ||
-                        __hip_atomic_load(&c.bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
-                        __hip_atomic_store(&c.bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        break;
-                    }
+            unsigned long long old = __hip_atomic_fetch_add(c.bar, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned long long target = (old / (unsigned long long)c.G + 1ull) * (unsigned long long)c.G;
+            unsigned spins = 0;
+            while (__hip_atomic_load(c.bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                __builtin_amdgcn_s_sleep(2);
+                // after one timeout anywhere in the team no barrier waits again,
+                // so the launch drains quickly
+                if (++spins > BAR_SPIN_LIMIT ||
+                    __hip_atomic_load(c.bar_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                    __hip_atomic_store(c.bar_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
                 }
             }
             __threadfence();
         }
-        c.gen += 1u;
         __syncthreads();
     }
 }
@@ -145,7 +155,8 @@ __device__ int team_or(Ctx& c, int v) {
 // In-place exclusive scan of the ints a[0..n) (any memory); returns the
 // total.  Each thread scans a contiguous chunk, chunk totals are scanned
 // across the block.  All threads must call it.
-__device__ int block_exscan(Ctx& c, int* a, int n) {
+template <class P>
+__device__ int block_exscan(Ctx& c, P a, int n) {
     int per = (n + NT - 1) / NT;
     int b = c.tid * per, e = min(b + per, n);
     int s = 0;
@@ -178,7 +189,8 @@ __device__ int block_exscan(Ctx& c, int* a, int n) {
 // Team-wide in-place exclusive scan of the ints a[0..n) in global memory;
 // returns the total.  Each workgroup scans a contiguous chunk, chunk totals
 // are added in rank order.  All threads of the team.
-__device__ int team_exscan(Ctx& c, int* a, int n) {
+template <class P>
+__device__ int team_exscan(Ctx& c, P a, int n) {
     if (c.G == 1) return block_exscan(c, a, n);
     int per = (n + c.G - 1) / c.G;
     int b = min(c.rank * per, n), e = min(b + per, n);
@@ -209,10 +221,11 @@ __device__ __forceinline__ float group_sum(float v) {
 // ---- splines (src/spline.h) -------------------------------------------------
 // Uniform cubic B-spline with coefficient window p[xb-1 .. xb+2] at x>=1
 // (first spline centred at -1, src/spline.h:98-127,242-256): value, derivative.
-__device__ __forceinline__ void deboor(const float* p, float x, float& v, float& d) {
+template <class P>
+__device__ __forceinline__ void deboor(P p, float x, float& v, float& d) {
     int xb = (int)x;
     float y = x - (float)xb;
-    const float* c = p + xb - 1;
+    P c = p + xb - 1;
     float c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
     float omy = 1.f - y, y2 = y * y, y3 = y2 * y;
     float w0 = omy * omy * omy * (1.f / 6.f);
@@ -228,7 +241,8 @@ __device__ __forceinline__ void deboor(const float* p, float x, float& v, float&
 }
 
 // Clamped variant (src/spline.h:326-358): constant outside [1, n_knot-2).
-__device__ __forceinline__ void clamped_deboor(const float* p, float x, int n_knot, float& v, float& d) {
+template <class P>
+__device__ __forceinline__ void clamped_deboor(P p, float x, int n_knot, float& v, float& d) {
     if (!(x >= 1.f)) {  // also catches NaN, which must never index the table
         v = p[0] * (1.f / 6.f) + p[1] * (2.f / 3.f) + p[2] * (1.f / 6.f);
         d = 0.f;
@@ -260,7 +274,8 @@ __device__ __forceinline__ void compact_sigmoid(S x, S sharp, S& v, S& d) {
 }
 
 // Periodic bicubic patch c[16] (value, d/dfx, d/dfy), src/spline.h:63-82.
-__device__ __forceinline__ void bicubic(const float* c, float fx, float fy, float& v, float& dx, float& dy) {
+template <class P>
+__device__ __forceinline__ void bicubic(P c, float fx, float fy, float& v, float& dx, float& dy) {
     float fx2 = fx * fx, fx3 = fx * fx2, fy2 = fy * fy;
     float vx0 = c[0] + fy * (c[1] + fy * (c[2] + fy * c[3]));
     float vx1 = c[4] + fy * (c[5] + fy * (c[6] + fy * c[7]));
@@ -274,7 +289,8 @@ __device__ __forceinline__ void bicubic(const float* c, float fx, float fy, floa
     v = vx0 + fx * vx1 + fx2 * vx2 + fx3 * vx3;
 }
 
-__device__ __forceinline__ void quat_to_rot(const float* q, float* U) {
+template <class P>
+__device__ __forceinline__ void quat_to_rot(P q, float* U) {
     float a = q[0], b = q[1], c = q[2], d = q[3];
     U[0] = a * a + b * b - c * c - d * d; U[1] = 2.f * b * c - 2.f * a * d; U[2] = 2.f * b * d + 2.f * a * c;
     U[3] = 2.f * b * c + 2.f * a * d; U[4] = a * a - b * b + c * c - d * d; U[5] = 2.f * c * d - 2.f * a * b;
@@ -285,12 +301,12 @@ __device__ __forceinline__ void quat_to_rot(const float* q, float* U) {
 // contributions, in plan order.  Called by all threads.
 __device__ void plan_gather(Ctx& c, int plan) {
     if (plan < 0) return;
-    const int* h = c.CI + plan;
+    gp<const int> h = gmem(c.CI + plan);
     int nt = h[0], cw = h[1], so = h[2], ss = h[3], col = h[4], cb = h[5], cs = h[6];
-    const int* off = h + PLAN_HDR;
-    const int* idx = off + nt + 1;
-    float* sens = c.A + so;
-    const float* con = c.A + cb;
+    gp<const int> off = h + PLAN_HDR;
+    gp<const int> idx = off + nt + 1;
+    gp<float> sens = gmem(c.A + so);
+    gp<const float> con = gmem((const float*)c.A + cb);
     for (int t = c.gtid; t < nt; t += c.gsize) {
         int b = off[t], e = off[t + 1];
         if (b == e) continue;

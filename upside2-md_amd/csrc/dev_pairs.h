@@ -39,17 +39,17 @@ __device__ __forceinline__ const float* pg_side2(const Ctx& c, const PairGraph& 
 // Copy the current coordinates of both sides into pos_pack and report whether
 // the cache must be rebuilt (first use, or drift beyond half the buffer).
 __device__ bool pg_pack_and_check(Ctx& c, const PairGraph& G) {
-    const int* loc1 = c.CI + G.loc1;
-    const int* loc2 = c.CI + G.loc2;
-    const float* P1 = c.A + G.out1;
-    const float* P2 = c.A + G.out2;
-    float* pack = c.A + G.pos_pack;
-    const float* cache = c.A + G.cache_pos;
+    gp<const int> loc1 = gmem(c.CI + G.loc1);
+    gp<const int> loc2 = gmem(c.CI + G.loc2);
+    gp<const float> P1 = gmem((const float*)c.A + G.out1);
+    gp<const float> P2 = gmem((const float*)c.A + G.out2);
+    gp<float> pack = gmem(c.A + G.pos_pack);
+    gp<const float> cache = gmem((const float*)c.A + G.cache_pos);
     int n = G.sym ? G.n1 : G.n1 + G.n2;
     int valid = c.I[G.state];
     int moved = 0;
     for (int i = c.gtid; i < n; i += c.gsize) {
-        const float* src = (i < G.n1) ? P1 + loc1[i] * G.w1 : P2 + loc2[i - G.n1] * G.w2;
+        gp<const float> src = (i < G.n1) ? P1 + loc1[i] * G.w1 : P2 + loc2[i - G.n1] * G.w2;
         float x = src[0], y = src[1], z = src[2];
         pack[i * 3 + 0] = x; pack[i * 3 + 1] = y; pack[i * 3 + 2] = z;
         float dx = x - cache[i * 3 + 0], dy = y - cache[i * 3 + 1], dz = z - cache[i * 3 + 2];
@@ -59,7 +59,8 @@ __device__ bool pg_pack_and_check(Ctx& c, const PairGraph& G) {
     return any != 0;
 }
 
-__device__ int lower_bound_i(const int* a, int n, int v) {
+template <class P>
+__device__ int lower_bound_i(P a, int n, int v) {
     int lo = 0, hi = n;
     while (lo < hi) {
         int m = (lo + hi) >> 1;
@@ -93,7 +94,8 @@ __device__ __forceinline__ GroupLanes group_lanes(const Ctx& c) {
 // cur + (hits of the lower lanes), so hits keep candidate order.  Every lane of
 // the group calls it.
 template <bool FILL>
-__device__ __forceinline__ int group_emit(const GroupLanes& L, bool hit, int cur, int q, int j, int* nbr, int* row) {
+__device__ __forceinline__ int group_emit(const GroupLanes& L, bool hit, int cur, int q, int j, gp<int> nbr,
+                                         gp<int> row) {
     unsigned long long m = __ballot(hit) & L.gmask;
     if (FILL && hit) {
         int p = cur + __popcll(m & L.below);
@@ -107,12 +109,18 @@ __device__ __forceinline__ int group_emit(const GroupLanes& L, bool hit, int cur
 // through LDS in tiles of TILE elements, partners in increasing index.
 constexpr int TILE = LDS_FLOATS / 4;
 
-// Stage elements [t0, t0+n) of side pts (ids in id) as float4 (x, y, z, id).
-__device__ void stage_tile(Ctx& c, const float* pts, const int* id, int t0, int n) {
-    float4* tile = reinterpret_cast<float4*>(c.lds);
+// Elements staged in LDS are 4 floats: position and an integer tag stored as
+// float bits (the four stores/loads combine into 16-byte LDS accesses).
+__device__ __forceinline__ void el_put(lp<float> el, int k, float x, float y, float z, int tag) {
+    el[4 * k] = x; el[4 * k + 1] = y; el[4 * k + 2] = z; el[4 * k + 3] = __int_as_float(tag);
+}
+
+// Stage elements [t0, t0+n) of side pts (ids in id) as (x, y, z, id).
+__device__ void stage_tile(Ctx& c, gp<const float> pts, gp<const int> id, int t0, int n) {
+    lp<float> tile = lmem(c.lds);
     for (int k = c.tid; k < n; k += NT) {  // every workgroup stages its own copy
         int j = t0 + k;
-        tile[k] = make_float4(pts[j * 3], pts[j * 3 + 1], pts[j * 3 + 2], __int_as_float(id[j]));
+        el_put(tile, k, pts[j * 3], pts[j * 3 + 1], pts[j * 3 + 2], id[j]);
     }
     __syncthreads();
 }
@@ -122,9 +130,10 @@ __device__ void stage_tile(Ctx& c, const float* pts, const int* id, int t0, int 
 // acceptable (SYM: also j > q).  !FILL adds row q's hits to off[q].  FILL
 // writes them at the cursor cur[q] (advanced).
 template <bool SYM, bool FILL>
-__device__ void tile_rows(Ctx& c, const PairGraph& G, const float* qp, const int* qid, int nq, const float* ep,
-                          const int* eid, int ne, int* off, int* cur_arr, int* nbr, int* row) {
-    const float4* tile = reinterpret_cast<const float4*>(c.lds);
+__device__ void tile_rows(Ctx& c, const PairGraph& G, gp<const float> qp, gp<const int> qid, int nq,
+                          gp<const float> ep, gp<const int> eid, int ne, gp<int> off, gp<int> cur_arr, gp<int> nbr,
+                          gp<int> row) {
+    lp<const float> tile = (lp<const float>)lmem(c.lds);
     const float cc2 = G.cache_cut2;
     const GroupLanes L = group_lanes(c);
     const int grp = c.gtid / SG, ngrp = c.gsize / SG;
@@ -139,9 +148,9 @@ __device__ void tile_rows(Ctx& c, const PairGraph& G, const float* qp, const int
                 int k = k0 + L.gl, j = t0 + k;
                 bool hit = false;
                 if (k < tn) {
-                    float4 e = tile[k];
-                    float d2 = refine_dist2(x0 - e.x, x1 - e.y, x2 - e.z);
-                    hit = d2 < cc2 && (!SYM || j > q) && pg_accept(G, a, __float_as_int(e.w));
+                    float ex = tile[4 * k], ey = tile[4 * k + 1], ez = tile[4 * k + 2];
+                    float d2 = refine_dist2(x0 - ex, x1 - ey, x2 - ez);
+                    hit = d2 < cc2 && (!SYM || j > q) && pg_accept(G, a, __float_as_int(tile[4 * k + 3]));
                 }
                 cur += group_emit<FILL>(L, hit, cur, q, j, nbr, row);
             }
@@ -157,15 +166,15 @@ __device__ void tile_rows(Ctx& c, const PairGraph& G, const float* qp, const int
 
 // ---- cache rebuild: cell grid in LDS ----------------------------------------
 // The other side is counting-sorted into hashed cells of edge >= the cache
-// radius, kept in LDS as (bucket offsets, float4 elements (x, y, z, index),
+// radius, kept in LDS as (bucket offsets, elements (x, y, z, index),
 // element ids), each bucket sorted by element index.  A row visits its 27
 // neighbouring cells in a fixed order, so rows come out in a canonical order.
 struct LGrid {
     float ox, oy, oz, inv;
     int mask;
-    int* ofs;      // [mask+2]
-    float4* el;    // [m]  (x, y, z, element index)
-    int* eid;      // [m]  element ids, same order (no global loads in the scan)
+    lp<int> ofs;      // [mask+2]
+    lp<float> el;     // [4m] (x, y, z, element index)
+    lp<int> eid;      // [m]  element ids, same order (no global loads in the scan)
 };
 
 __device__ __forceinline__ int cell_coord(float x, float o, float inv) {
@@ -190,7 +199,7 @@ __device__ __forceinline__ bool lgrid_fits(int m) {
 
 // Build the grid of the m points pts (stride 3) in this workgroup's LDS
 // (every workgroup of a team builds its own copy).  All threads of the workgroup.
-__device__ LGrid lgrid_build(Ctx& c, const PairGraph& G, const float* pts, const int* ids, int m) {
+__device__ LGrid lgrid_build(Ctx& c, const PairGraph& G, gp<const float> pts, gp<const int> ids, int m) {
     float mn0 = 3.0e38f, mn1 = 3.0e38f, mn2 = 3.0e38f;
     for (int i = c.tid; i < m; i += NT) {
         mn0 = fminf(mn0, pts[i * 3]);
@@ -206,23 +215,24 @@ __device__ LGrid lgrid_build(Ctx& c, const PairGraph& G, const float* pts, const
     g.inv = 1.f / (G.cache_cut * 1.0001f);
     int nb = lgrid_buckets(m);
     g.mask = nb - 1;
-    g.ofs = reinterpret_cast<int*>(c.lds);
-    g.el = reinterpret_cast<float4*>(c.lds + ((nb + 4) & ~3));
-    g.eid = reinterpret_cast<int*>(g.el + m);
+    lp<float> L = lmem(c.lds);
+    g.ofs = (lp<int>)L;
+    g.el = L + ((nb + 4) & ~3);
+    g.eid = (lp<int>)(g.el + 4 * m);
     for (int b = c.tid; b <= nb; b += NT) g.ofs[b] = 0;
     __syncthreads();
     for (int i = c.tid; i < m; i += NT) {
         int b = cell_bucket(g.mask, cell_coord(pts[i * 3], g.ox, g.inv), cell_coord(pts[i * 3 + 1], g.oy, g.inv),
                             cell_coord(pts[i * 3 + 2], g.oz, g.inv));
-        atomicAdd(&g.ofs[b + 1], 1);
+        atomicAdd((int*)(g.ofs + b + 1), 1);
     }
     __syncthreads();
-    block_exscan(c, g.ofs + 1, nb);  // ofs[b+1] = start of bucket b
+    block_exscan(c, (int*)(g.ofs + 1), nb);  // ofs[b+1] = start of bucket b
     for (int i = c.tid; i < m; i += NT) {
         float x = pts[i * 3], y = pts[i * 3 + 1], z = pts[i * 3 + 2];
         int b = cell_bucket(g.mask, cell_coord(x, g.ox, g.inv), cell_coord(y, g.oy, g.inv), cell_coord(z, g.oz, g.inv));
-        int p = atomicAdd(&g.ofs[b + 1], 1);  // ends at the start of bucket b+1
-        g.el[p] = make_float4(x, y, z, __int_as_float(i));
+        int p = atomicAdd((int*)(g.ofs + b + 1), 1);  // ends at the start of bucket b+1
+        el_put(g.el, p, x, y, z, i);
         g.eid[p] = ids[i];
     }
     __syncthreads();
@@ -230,16 +240,17 @@ __device__ LGrid lgrid_build(Ctx& c, const PairGraph& G, const float* pts, const
     for (int b = c.tid; b < nb; b += NT) {
         int lo = g.ofs[b], hi = g.ofs[b + 1];
         for (int k = lo + 1; k < hi; ++k) {
-            float4 v = g.el[k];
+            float vx = g.el[4 * k], vy = g.el[4 * k + 1], vz = g.el[4 * k + 2];
+            int key = __float_as_int(g.el[4 * k + 3]);
             int vid = g.eid[k];
-            int key = __float_as_int(v.w);
             int t = k - 1;
-            while (t >= lo && __float_as_int(g.el[t].w) > key) {
-                g.el[t + 1] = g.el[t];
+            while (t >= lo && __float_as_int(g.el[4 * t + 3]) > key) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) g.el[4 * (t + 1) + q] = g.el[4 * t + q];
                 g.eid[t + 1] = g.eid[t];
                 --t;
             }
-            g.el[t + 1] = v;
+            el_put(g.el, t + 1, vx, vy, vz, key);
             g.eid[t + 1] = vid;
         }
     }
@@ -250,8 +261,8 @@ __device__ LGrid lgrid_build(Ctx& c, const PairGraph& G, const float* pts, const
 // Rows against the grid; same hit rule as tile_rows.  !FILL: off[q] = hits of
 // row q.  FILL: row q's hits are written from off[q] on.
 template <bool SYM, bool FILL>
-__device__ void grid_rows(Ctx& c, const PairGraph& G, const LGrid& g, const float* qp, const int* qid, int nq,
-                          int* off, int* nbr, int* row) {
+__device__ void grid_rows(Ctx& c, const PairGraph& G, const LGrid& g, gp<const float> qp, gp<const int> qid, int nq,
+                          gp<int> off, gp<int> nbr, gp<int> row) {
     const float cc2 = G.cache_cut2;
     const GroupLanes L = group_lanes(c);
     const int grp = c.gtid / SG, ngrp = c.gsize / SG;
@@ -270,12 +281,12 @@ __device__ void grid_rows(Ctx& c, const PairGraph& G, const LGrid& g, const floa
                         int k = t + L.gl, j = 0;
                         bool hit = false;
                         if (k < t1) {
-                            float4 e = g.el[k];
-                            j = __float_as_int(e.w);
-                            float d2 = refine_dist2(x0 - e.x, x1 - e.y, x2 - e.z);
+                            float ex = g.el[4 * k], ey = g.el[4 * k + 1], ez = g.el[4 * k + 2];
+                            j = __float_as_int(g.el[4 * k + 3]);
+                            float d2 = refine_dist2(x0 - ex, x1 - ey, x2 - ez);
                             // the cell test drops elements of other cells sharing the bucket
-                            hit = d2 < cc2 && (!SYM || j > q) && cell_coord(e.x, g.ox, g.inv) == ax &&
-                                  cell_coord(e.y, g.oy, g.inv) == ay && cell_coord(e.z, g.oz, g.inv) == az &&
+                            hit = d2 < cc2 && (!SYM || j > q) && cell_coord(ex, g.ox, g.inv) == ax &&
+                                  cell_coord(ey, g.oy, g.inv) == ay && cell_coord(ez, g.oz, g.inv) == az &&
                                   pg_accept(G, a, g.eid[k]);
                         }
                         cur += group_emit<FILL>(L, hit, cur, q, j, nbr, row);
@@ -292,11 +303,11 @@ __device__ void grid_rows(Ctx& c, const PairGraph& G, const LGrid& g, const floa
 // among the team's workgroups; each workgroup stages runs of whole columns in
 // LDS.  All threads of the team.
 __device__ void pg_order_columns(Ctx& c, const PairGraph& G, int n2) {
-    const int* toff = c.I + G.toff;
-    const int* tmp = c.I + G.tmp;
-    const int* nbr = c.I + G.nbr;
-    int* tslot = c.I + G.tslot;
-    int* sl = reinterpret_cast<int*>(c.lds);
+    gp<const int> toff = gmem((const int*)c.I + G.toff);
+    gp<const int> tmp = gmem((const int*)c.I + G.tmp);
+    gp<const int> nbr = gmem((const int*)c.I + G.nbr);
+    gp<int> tslot = gmem(c.I + G.tslot);
+    lp<int> sl = (lp<int>)lmem(c.lds);
     const int CAP = LDS_FLOATS;
     int per = (n2 + c.G - 1) / c.G;
     int jb = min(c.rank * per, n2);
@@ -344,20 +355,20 @@ __device__ void pg_order_columns(Ctx& c, const PairGraph& G, int n2) {
 //   transposed rows: side-2 element j, the forward slots with partner j in
 //                    increasing slot order.
 __device__ void pg_rebuild(Ctx& c, const PairGraph& G) {
-    const int* id1 = c.CI + G.id1;
+    gp<const int> id1 = gmem(c.CI + G.id1);
     const bool sym = G.sym != 0;
-    const int* id2 = sym ? id1 : c.CI + G.id2;
-    float* pack = c.A + G.pos_pack;
-    float* cache = c.A + G.cache_pos;
-    const float* p1 = pack;
-    const float* p2 = pg_side2(c, G);
+    gp<const int> id2 = sym ? id1 : gmem(c.CI + G.id2);
+    gp<const float> pack = gmem((const float*)c.A + G.pos_pack);
+    gp<float> cache = gmem(c.A + G.cache_pos);
+    gp<const float> p1 = pack;
+    gp<const float> p2 = gmem(pg_side2(c, G));
     const int n1 = G.n1, n2 = sym ? G.n1 : G.n2;
-    int* off = c.I + G.off;
-    int* nbr = c.I + G.nbr;
-    int* row = c.I + G.row;
-    int* toff = c.I + G.toff;
-    int* tmp = c.I + G.tmp;
-    int* tcur = c.I + G.tcur;
+    gp<int> off = gmem(c.I + G.off);
+    gp<int> nbr = gmem(c.I + G.nbr);
+    gp<int> row = gmem(c.I + G.row);
+    gp<int> toff = gmem(c.I + G.toff);
+    gp<int> tmp = gmem(c.I + G.tmp);
+    gp<int> tcur = gmem(c.I + G.tcur);
     int npack = sym ? n1 : n1 + n2;
     for (int i = c.gtid; i < npack * 3; i += c.gsize) cache[i] = pack[i];
     for (int i = c.gtid; i <= n1; i += c.gsize) off[i] = 0;
@@ -373,14 +384,14 @@ __device__ void pg_rebuild(Ctx& c, const PairGraph& G) {
     }
     unsigned long long t1 = ptick(c);
     if (use_grid) {
-        if (sym) grid_rows<true, false>(c, G, g, p1, id1, n1, off, nullptr, nullptr);
-        else grid_rows<false, false>(c, G, g, p1, id1, n1, off, nullptr, nullptr);
+        if (sym) grid_rows<true, false>(c, G, g, p1, id1, n1, off, off, off);
+        else grid_rows<false, false>(c, G, g, p1, id1, n1, off, off, off);
     } else {
-        if (sym) tile_rows<true, false>(c, G, p1, id1, n1, p2, id2, n2, off, nullptr, nullptr, nullptr);
-        else tile_rows<false, false>(c, G, p1, id1, n1, p2, id2, n2, off, nullptr, nullptr, nullptr);
+        if (sym) tile_rows<true, false>(c, G, p1, id1, n1, p2, id2, n2, off, off, off, off);
+        else tile_rows<false, false>(c, G, p1, id1, n1, p2, id2, n2, off, off, off, off);
     }
     pacc(c, PROF_RB_SCAN, t1);
-    const int E = team_exscan(c, off, n1 + 1);
+    const int E = team_exscan(c, (int*)off, n1 + 1);
     if (E > G.ecap) {
         if (c.gtid == 0) { c.I[G.state + 1] = 1; c.I[G.state + 0] = 0; }
         team_sync(c);
@@ -402,12 +413,12 @@ __device__ void pg_rebuild(Ctx& c, const PairGraph& G) {
     pacc(c, PROF_RB_FILL, t0);
     t0 = ptick(c);
     // transposed rows: column counts, arrival-order placement, slot order
-    for (int s = c.gtid; s < E; s += c.gsize) atomicAdd(&toff[nbr[s]], 1);
+    for (int s = c.gtid; s < E; s += c.gsize) atomicAdd((int*)(toff + nbr[s]), 1);
     team_sync(c);
-    team_exscan(c, toff, n2 + 1);
+    team_exscan(c, (int*)toff, n2 + 1);
     for (int j = c.gtid; j < n2; j += c.gsize) tcur[j] = toff[j];
     team_sync(c);
-    for (int s = c.gtid; s < E; s += c.gsize) tmp[atomicAdd(&tcur[nbr[s]], 1)] = s;
+    for (int s = c.gtid; s < E; s += c.gsize) tmp[atomicAdd((int*)(tcur + nbr[s]), 1)] = s;
     team_sync(c);
     pg_order_columns(c, G, n2);
     pacc(c, PROF_RB_MAP, t0);
@@ -422,9 +433,8 @@ __device__ void pg_rebuild(Ctx& c, const PairGraph& G) {
 // ---- edge functions -----------------------------------------------------
 // quadspline (src/bead_interaction.h:31-85): radial clamped splines times
 // angular splines of the two bead directions.
-template <int NKA, int NK>
-__device__ __forceinline__ float quadspline(const float* p, const float* x1, const float* x2,
-                                            float inv_dtheta, float inv_dx, float* d1, float* d2) {
+template <int NKA, int NK, class P, class X>
+__device__ __forceinline__ float quadspline(P p, X x1, X x2, float inv_dtheta, float inv_dx, float* d1, float* d2) {
     float dx = x2[0] - x1[0], dy = x2[1] - x1[1], dz = x2[2] - x1[2];
     float r1x = x1[3], r1y = x1[4], r1z = x1[5];
     float r2x = x2[3], r2y = x2[4], r2z = x2[5];
@@ -459,8 +469,8 @@ __device__ __forceinline__ float quadspline(const float* p, const float* x1, con
 }
 
 // ProteinHBondInteraction (src/hbond.cpp:158-246).
-__device__ __forceinline__ float protein_hbond_edge(const float* p, const float* x1, const float* x2,
-                                                    float* d1, float* d2) {
+template <class P, class X>
+__device__ __forceinline__ float protein_hbond_edge(P p, X x1, X x2, float* d1, float* d2) {
     float HOx = x1[0] - x2[0], HOy = x1[1] - x2[1], HOz = x1[2] - x2[2];
     float m2 = __fmaf_rn(HOz, HOz, __fmaf_rn(HOy, HOy, HOx * HOx)) + 1e-6f;
     float inv = rsqrtf(m2);
@@ -499,8 +509,8 @@ __device__ __forceinline__ float protein_hbond_edge(const float* p, const float*
 
 // Environment coverage (src/environment.cpp:114-228); weighted: second element
 // carries a probability in component 3.
-template <bool WEIGHTED>
-__device__ __forceinline__ float env_edge(const float* p, const float* x1, const float* x2, float* d1, float* d2) {
+template <bool WEIGHTED, class P, class X>
+__device__ __forceinline__ float env_edge(P p, X x1, X x2, float* d1, float* d2) {
     float dx = x2[0] - x1[0], dy = x2[1] - x1[1], dz = x2[2] - x1[2];
     float dist2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, dx * dx));
     float inv = rsqrtf(dist2);
@@ -532,14 +542,15 @@ __device__ __forceinline__ float env_edge(const float* p, const float* x1, const
 // repulsion between the placed backbone atoms (at most 4) of two residues;
 // derivatives are (force, torque) on each residue frame.
 __device__ float backbone_edge(const Ctx& c, const OpRec& op, int r1, int r2, float* d1, float* d2) {
-    const float* aff = c.A + op.in_out[0];
+    gp<const float> aff = gmem((const float*)c.A + op.in_out[0]);
     int w = op.in_w[0];
-    const float* ref = c.CF + op.ia[1];
-    const int* nat = c.CI + op.ia[2];
-    const int* res = c.CI + op.ia[0];
+    gp<const float> ref = gmem(c.CF + op.ia[1]);
+    gp<const int> nat = gmem(c.CI + op.ia[2]);
+    gp<const int> res = gmem(c.CI + op.ia[0]);
     int R1 = res[r1], R2 = res[r2];
-    const float* a1 = aff + R1 * w;
-    const float* a2 = aff + R2 * w;
+    float a1[7], a2[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) { a1[k] = aff[R1 * w + k]; a2[k] = aff[R2 * w + k]; }
     float U1[9], U2[9];
     quat_to_rot(a1 + 3, U1);
     quat_to_rot(a2 + 3, U2);
@@ -547,8 +558,8 @@ __device__ float backbone_edge(const Ctx& c, const OpRec& op, int r1, int r2, fl
     int n1 = nat[r1], n2 = nat[r2];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-        const float* q = ref + (r1 * 4 + a) * 3;
-        const float* s = ref + (r2 * 4 + a) * 3;
+        gp<const float> q = ref + (r1 * 4 + a) * 3;
+        gp<const float> s = ref + (r2 * 4 + a) * 3;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             X1[a][k] = U1[3 * k] * q[0] + U1[3 * k + 1] * q[1] + U1[3 * k + 2] * q[2] + a1[k];
@@ -584,46 +595,37 @@ __device__ float backbone_edge(const Ctx& c, const OpRec& op, int r1, int r2, fl
     return pot;
 }
 
-// Stage the graph's coefficient table in LDS (when it has one that fits).
-__device__ const float* pg_stage_params(Ctx& c, const PairGraph& G) {
-    const float* src = c.CF + G.param;
-    if (G.param_len <= 0) return src;
-    for (int k = c.tid; k < G.param_len; k += NT) c.lds[k] = src[k];  // per workgroup
-    __syncthreads();
-    return c.lds;
-}
-
-// Refine every cached slot against the cutoff and evaluate the active edges.
-template <int KIND, int DD1, int DD2>
-__device__ void pg_edges_t(Ctx& c, const PairGraph& G, const OpRec& op) {
-    const float* param = pg_stage_params(c, G);
-    const int* nbr = c.I + G.nbr;
-    const int* row = c.I + G.row;
-    int* act = c.I + G.act;
-    float* val = c.A + G.val;
-    float* D1 = c.A + G.d1;
-    float* D2 = c.A + G.d2;
-    const float* pack = c.A + G.pos_pack;
-    const float* pk2 = pg_side2(c, G);
-    const int* loc1 = c.CI + G.loc1;
-    const int* loc2 = c.CI + G.loc2;
-    const int* t1 = c.CI + G.type1;
-    const int* t2 = c.CI + G.type2;
-    const float* P1 = c.A + G.out1;
-    const float* P2 = c.A + G.out2;
+// Refine every cached slot against the cutoff and evaluate the active edges;
+// param is the coefficient table (LDS or memory).
+template <int KIND, int DD1, int DD2, class P>
+__device__ void pg_edges_impl(Ctx& c, const PairGraph& G, const OpRec& op, P param) {
+    gp<const int> nbr = gmem((const int*)c.I + G.nbr);
+    gp<const int> row = gmem((const int*)c.I + G.row);
+    gp<int> act = gmem(c.I + G.act);
+    gp<float> val = gmem(c.A + G.val);
+    gp<float> D1 = gmem(c.A + G.d1);
+    gp<float> D2 = gmem(c.A + G.d2);
+    gp<const float> pack = gmem((const float*)c.A + G.pos_pack);
+    gp<const float> pk2 = gmem(pg_side2(c, G));
+    gp<const int> loc1 = gmem(c.CI + G.loc1);
+    gp<const int> loc2 = gmem(c.CI + G.loc2);
+    gp<const int> t1 = gmem(c.CI + G.type1);
+    gp<const int> t2 = gmem(c.CI + G.type2);
+    gp<const float> P1 = gmem((const float*)c.A + G.out1);
+    gp<const float> P2 = gmem((const float*)c.A + G.out2);
     const int E = c.I[G.state + 3];
     const int cap = G.ecap;
     for (int s = c.gtid; s < E; s += c.gsize) {
         int i1 = row[s], j = nbr[s];
-        const float* xp = pack + i1 * 3;
-        const float* yp = pk2 + j * 3;
+        gp<const float> xp = pack + i1 * 3;
+        gp<const float> yp = pk2 + j * 3;
         float d2 = refine_dist2(xp[0] - yp[0], xp[1] - yp[1], xp[2] - yp[2]);
         int a = d2 < G.cutoff2;
         act[s] = a;
         if (!a) { val[s] = 0.f; continue; }
-        const float* x1 = P1 + loc1[i1] * G.w1;
-        const float* x2 = P2 + loc2[j] * G.w2;
-        const float* p = param + (t1[i1] * G.n_type2 + t2[j]) * G.n_param;
+        gp<const float> x1 = P1 + loc1[i1] * G.w1;
+        gp<const float> x2 = P2 + loc2[j] * G.w2;
+        P p = param + (t1[i1] * G.n_type2 + t2[j]) * G.n_param;
         float e1[7], e2[7];
         float v = 0.f;
         if (KIND == EDGE_ROTAMER) {
@@ -654,6 +656,21 @@ __device__ void pg_edges_t(Ctx& c, const PairGraph& G, const OpRec& op) {
     team_sync(c);
 }
 
+// Stage the graph's coefficient table in this workgroup's LDS when it has one
+// that fits, then evaluate.
+template <int KIND, int DD1, int DD2>
+__device__ void pg_edges_t(Ctx& c, const PairGraph& G, const OpRec& op) {
+    gp<const float> src = gmem(c.CF + G.param);
+    if (G.param_len > 0) {
+        lp<float> L = lmem(c.lds);
+        for (int k = c.tid; k < G.param_len; k += NT) L[k] = src[k];
+        __syncthreads();
+        pg_edges_impl<KIND, DD1, DD2>(c, G, op, (lp<const float>)L);
+    } else {
+        pg_edges_impl<KIND, DD1, DD2>(c, G, op, src);
+    }
+}
+
 __device__ void pg_edges(Ctx& c, const PairGraph& G, const OpRec& op) {
     switch (G.kind) {
     case EDGE_ROTAMER: pg_edges_t<EDGE_ROTAMER, 6, 6>(c, G, op); break;
@@ -672,17 +689,17 @@ constexpr int GW = 16;
 
 template <int D1N, int D2N>
 __device__ void pg_gather_t(Ctx& c, const PairGraph& G) {
-    const int* off = c.I + G.off;
-    const int* toff = c.I + G.toff;
-    const int* tslot = c.I + G.tslot;
-    const int* act = c.I + G.act;
-    const float* es = c.A + G.esens;
-    const float* D1 = c.A + G.d1;
-    const float* D2 = c.A + G.d2;
-    const int* loc1 = c.CI + G.loc1;
-    const int* loc2 = c.CI + G.loc2;
-    float* S1 = c.A + G.sens1;
-    float* S2 = c.A + G.sens2;
+    gp<const int> off = gmem((const int*)c.I + G.off);
+    gp<const int> toff = gmem((const int*)c.I + G.toff);
+    gp<const int> tslot = gmem((const int*)c.I + G.tslot);
+    gp<const int> act = gmem((const int*)c.I + G.act);
+    gp<const float> es = gmem((const float*)c.A + G.esens);
+    gp<const float> D1 = gmem((const float*)c.A + G.d1);
+    gp<const float> D2 = gmem((const float*)c.A + G.d2);
+    gp<const int> loc1 = gmem(c.CI + G.loc1);
+    gp<const int> loc2 = gmem(c.CI + G.loc2);
+    gp<float> S1 = gmem(c.A + G.sens1);
+    gp<float> S2 = gmem(c.A + G.sens2);
     const int cap = G.ecap;
     int grp = c.gtid / GW, gl = c.gtid % GW;
     const int NG = c.gsize / GW;
@@ -722,7 +739,7 @@ __device__ void pg_gather_t(Ctx& c, const PairGraph& G) {
 #pragma unroll
         for (int q = 0; q < D1N; ++q) acc[q] = group_sum<GW>(acc[q]);
         if (ok && gl == 0) {
-            float* dst = S1 + loc1[i] * G.w1;
+            gp<float> dst = S1 + loc1[i] * G.w1;
 #pragma unroll
             for (int q = 0; q < D1N; ++q) dst[q] += acc[q];
         }
@@ -750,7 +767,7 @@ __device__ void pg_gather_t(Ctx& c, const PairGraph& G) {
 #pragma unroll
         for (int q = 0; q < D2N; ++q) acc[q] = group_sum<GW>(acc[q]);
         if (ok && gl == 0) {
-            float* dst = S2 + loc2[j] * G.w2;
+            gp<float> dst = S2 + loc2[j] * G.w2;
 #pragma unroll
             for (int q = 0; q < D2N; ++q) dst[q] += acc[q];
         }
@@ -770,11 +787,11 @@ __device__ void pg_gather(Ctx& c, const PairGraph& G) {
 
 // Sum of the active edge values of each row (side 1) or transposed row
 // (side 2), in slot order: out[k*stride] = sum.  Groups of 16 lanes.
-__device__ void pg_row_sums(Ctx& c, const PairGraph& G, bool side2, float* out, int stride) {
-    const int* off = c.I + (side2 ? G.toff : G.off);
-    const int* tslot = c.I + G.tslot;
-    const int* act = c.I + G.act;
-    const float* val = c.A + G.val;
+template <class O>
+__device__ void pg_row_sums(Ctx& c, const PairGraph& G, bool side2, O out, int stride) {
+    gp<const int> off = gmem((const int*)c.I + (side2 ? G.toff : G.off));
+    gp<const int> tslot = gmem((const int*)c.I + G.tslot);
+    gp<const float> val = gmem((const float*)c.A + G.val);
     int n = side2 ? G.n2 : G.n1;
     int grp = c.gtid / GW, gl = c.gtid % GW;
     for (int base = 0; base < n; base += c.gsize / GW) {

@@ -1,5 +1,6 @@
 // Rotamer side-chain free energy by loopy belief propagation on the device
-// (src/rotamer.cpp:628-1169), one workgroup per system.
+// (src/rotamer.cpp:628-1169).  Graph work is spread over the system's team of
+// workgroups; the iterative solve runs on one workgroup.
 //
 // Residue nodes carry 1, 3 or 6 rotamer states, one bead per state.  Residue
 // pairs that may hold a bead pair of the rotamer pair graph's cache are
@@ -24,22 +25,23 @@
 namespace upside {
 
 struct RotView {
-    const int *nrot, *bead0, *brot, *bnode;
-    int *coff, *ca, *cb, *scand, *ioff, *iinc, *eidx, *bpc, *bpf, *bps, *ninc, *binc, *stats;
-    float *P, *msg, *marg, *e1x, *nprob, *ncur, *nold, *noff;
+    gp<const int> nrot, bead0, brot, bnode;
+    gp<int> coff, ca, cb, scand, ioff, iinc, eidx, bpc, bpf, bps, ninc, binc, stats;
+    gp<float> P, msg, marg, e1x, nprob, ncur, nold, noff;
     int cap;
 };
 
 __device__ RotView rot_view(Ctx& c, const RotamerDev& R) {
     RotView v;
-    v.nrot = c.CI + R.node_nrot; v.bead0 = c.CI + R.node_bead0; v.brot = c.CI + R.bead_rot;
-    v.bnode = c.CI + R.bead_node;
-    v.coff = c.I + R.coff; v.ca = c.I + R.ca; v.cb = c.I + R.cb; v.scand = c.I + R.slot_cand;
-    v.ioff = c.I + R.ioff; v.iinc = c.I + R.iinc; v.eidx = c.I + R.eidx; v.bpc = c.I + R.bpc;
-    v.bpf = c.I + R.bpf; v.bps = c.I + R.bps; v.ninc = c.I + R.ninc; v.binc = c.I + R.binc;
-    v.stats = c.I + R.stats;
-    v.P = c.A + R.P; v.msg = c.A + R.msg; v.marg = c.A + R.marg; v.e1x = c.A + R.e1x;
-    v.nprob = c.A + R.nprob; v.ncur = c.A + R.ncur; v.nold = c.A + R.nold; v.noff = c.A + R.noff;
+    v.nrot = gmem(c.CI + R.node_nrot); v.bead0 = gmem(c.CI + R.node_bead0); v.brot = gmem(c.CI + R.bead_rot);
+    v.bnode = gmem(c.CI + R.bead_node);
+    v.coff = gmem(c.I + R.coff); v.ca = gmem(c.I + R.ca); v.cb = gmem(c.I + R.cb); v.scand = gmem(c.I + R.slot_cand);
+    v.ioff = gmem(c.I + R.ioff); v.iinc = gmem(c.I + R.iinc); v.eidx = gmem(c.I + R.eidx);
+    v.bpc = gmem(c.I + R.bpc); v.bpf = gmem(c.I + R.bpf); v.bps = gmem(c.I + R.bps); v.ninc = gmem(c.I + R.ninc);
+    v.binc = gmem(c.I + R.binc); v.stats = gmem(c.I + R.stats);
+    v.P = gmem(c.A + R.P); v.msg = gmem(c.A + R.msg); v.marg = gmem(c.A + R.marg); v.e1x = gmem(c.A + R.e1x);
+    v.nprob = gmem(c.A + R.nprob); v.ncur = gmem(c.A + R.ncur); v.nold = gmem(c.A + R.nold);
+    v.noff = gmem(c.A + R.noff);
     v.cap = R.ccap;
     return v;
 }
@@ -73,6 +75,12 @@ __device__ bool rot_rebuild(Ctx& c, const PairGraph& G, const RotamerDev& R) {
         cen[a * 4] = sx; cen[a * 4 + 1] = sy; cen[a * 4 + 2] = sz; cen[a * 4 + 3] = sqrtf(r2);
     }
     team_sync(c);
+    // every workgroup reads the spheres from its own LDS copy when they fit
+    if (4 * n <= LDS_FLOATS) {
+        for (int k = c.tid; k < 4 * n; k += NT) c.lds[k] = cen[k];
+        __syncthreads();
+        cen = c.lds;
+    }
     const float rc = G.cache_cut;
     auto near = [&](int a, int b) {
         float dx = cen[a * 4] - cen[b * 4], dy = cen[a * 4 + 1] - cen[b * 4 + 1], dz = cen[a * 4 + 2] - cen[b * 4 + 2];
@@ -235,28 +243,43 @@ __device__ void rot_fill(Ctx& c, const PairGraph& G, const RotamerDev& R) {
     team_sync(c);
 }
 
+// Belief-propagation state of one solve: messages, beliefs and incidences in
+// LDS when they fit (F, I = lp<float>, lp<int>), else in memory (gp<...>).
+template <class F, class I>
+struct BpMem {
+    F msg;             // component (h*12 + side*6 + r) of edge e at msg[(...)*ms + e]
+    int ms;
+    gp<const float> P; // edge matrices (memory), P[(i*6+j)*ps + e]
+    int ps;
+    F prob;            // node probabilities [n*6]
+    I bpf, bps;        // edge endpoints (rows of P, columns of P)
+    I ioff, ninc, inc; // node incidences inc[ioff[x] + t], t < ninc[x]
+    gp<const int> nrot;
+};
+
 // One synchronous message pass: new messages (half 1-po) from old node
 // beliefs `nold_` and old messages (half po); each new message is scaled to
-// unit L1 norm (src/rotamer.cpp:515-568).  Message layout: component
-// (h*12 + side*6 + r) of edge e at msg[(...)*cap + e].
-__device__ void rot_edge_pass(Ctx& c, const RotView& v, int nbp, const float* nold_, int po) {
-    const int cap = v.cap;
-    for (int e = c.gtid; e < nbp; e += c.gsize) {
-        int f = v.bpf[e], s = v.bps[e];
+// unit L1 norm (src/rotamer.cpp:515-568).  One workgroup.
+template <class F, class I>
+__device__ void rot_edge_pass(Ctx& c, const BpMem<F, I>& m, int nbp, F nold_, int po) {
+    for (int e = c.tid; e < nbp; e += NT) {
+        int f = m.bpf[e], s = m.bps[e];
+        float p[36];
+#pragma unroll
+        for (int q = 0; q < 36; ++q) p[q] = m.P[q * m.ps + e];
         float v1[6], v2[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            v1[r] = nold_[f * 6 + r] / (1e-10f + v.msg[(po * 12 + r) * cap + e]);
-            v2[r] = nold_[s * 6 + r] / (1e-10f + v.msg[(po * 12 + 6 + r) * cap + e]);
+            v1[r] = nold_[f * 6 + r] / (1e-10f + m.msg[(po * 12 + r) * m.ms + e]);
+            v2[r] = nold_[s * 6 + r] / (1e-10f + m.msg[(po * 12 + 6 + r) * m.ms + e]);
         }
         float m1[6] = {0, 0, 0, 0, 0, 0}, m2[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int i = 0; i < 6; ++i)
 #pragma unroll
             for (int j = 0; j < 6; ++j) {
-                float p = v.P[(i * 6 + j) * cap + e];
-                m1[i] += p * v2[j];
-                m2[j] += p * v1[i];
+                m1[i] += p[i * 6 + j] * v2[j];
+                m2[j] += p[i * 6 + j] * v1[i];
             }
         float s1 = 0.f, s2 = 0.f;
 #pragma unroll
@@ -265,34 +288,34 @@ __device__ void rot_edge_pass(Ctx& c, const RotView& v, int nbp, const float* no
         int h = (1 - po) * 12;
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            v.msg[(h + r) * cap + e] = m1[r] * i1;
-            v.msg[(h + 6 + r) * cap + e] = m2[r] * i2;
+            m.msg[(h + r) * m.ms + e] = m1[r] * i1;
+            m.msg[(h + 6 + r) * m.ms + e] = m2[r] * i2;
         }
     }
-    team_sync(c);
+    __syncthreads();
 }
 
 // Node pass: cur = prob * prod(new messages), renormalised after every factor;
 // then cur = (1-d) cur/max(cur) + d*old (standardize_belief_update, :258-273).
 // With swap_first, the product goes to `nold_` and cur restarts from prob
 // (the first, undamped pass of solve_for_marginals).  Padding states carry
-// zeros throughout.
-__device__ void rot_node_pass(Ctx& c, const RotView& v, int n_node, float* ncur_, float* nold_, int pn,
-                              float damping, bool swap_first) {
-    const int cap = v.cap;
-    for (int x = c.gtid; x < n_node; x += c.gsize) {
-        if (v.nrot[x] == 1) continue;
+// zeros throughout.  One workgroup.
+template <class F, class I>
+__device__ void rot_node_pass(Ctx& c, const BpMem<F, I>& m, int n_node, F ncur_, F nold_, int pn, float damping,
+                              bool swap_first) {
+    for (int x = c.tid; x < n_node; x += NT) {
+        if (m.nrot[x] == 1) continue;
         float b[6];
 #pragma unroll
-        for (int r = 0; r < 6; ++r) b[r] = v.nprob[x * 6 + r];
-        int t0 = v.ioff[x], n = v.ninc[x];
+        for (int r = 0; r < 6; ++r) b[r] = m.prob[x * 6 + r];
+        int t0 = m.ioff[x], n = m.ninc[x];
         for (int t = 0; t < n; ++t) {
-            int inc = v.binc[t0 + t];
+            int inc = m.inc[t0 + t];
             int e = inc >> 1;
-            int base = (pn * 12 + (inc & 1) * 6) * cap + e;
+            int base = (pn * 12 + (inc & 1) * 6) * m.ms + e;
             float sum = 0.f;
 #pragma unroll
-            for (int r = 0; r < 6; ++r) { b[r] *= v.msg[base + r * cap]; sum += b[r]; }
+            for (int r = 0; r < 6; ++r) { b[r] *= m.msg[base + r * m.ms]; sum += b[r]; }
             float inv = 1.f / sum;
 #pragma unroll
             for (int r = 0; r < 6; ++r) b[r] *= inv;
@@ -300,12 +323,12 @@ __device__ void rot_node_pass(Ctx& c, const RotView& v, int n_node, float* ncur_
         if (swap_first) {
             float mx = 0.f;
 #pragma unroll
-            for (int r = 0; r < 6; ++r) mx = fmaxf(mx, v.nprob[x * 6 + r]);
+            for (int r = 0; r < 6; ++r) mx = fmaxf(mx, m.prob[x * 6 + r]);
             float im = 1.f / mx;
 #pragma unroll
             for (int r = 0; r < 6; ++r) {
                 nold_[x * 6 + r] = b[r];
-                ncur_[x * 6 + r] = v.nprob[x * 6 + r] * im;
+                ncur_[x * 6 + r] = m.prob[x * 6 + r] * im;
             }
         } else {
             float mx = 0.f;
@@ -316,74 +339,151 @@ __device__ void rot_node_pass(Ctx& c, const RotView& v, int n_node, float* ncur_
             for (int r = 0; r < 6; ++r) ncur_[x * 6 + r] = (1.f - damping) * im * b[r] + damping * nold_[x * 6 + r];
         }
     }
-    team_sync(c);
+    __syncthreads();
 }
 
-// Solve; returns the iteration count (the reference's `iter`).
-__device__ int rot_solve(Ctx& c, const RotamerDev& R) {
-    RotView v = rot_view(c, R);
-    const int nbp = v.stats[4];
+// Iterations, node marginals and edge marginals of one solve whose state is
+// set up in m (messages all 1, `old` = prob); returns the iteration count.
+template <class F, class I>
+__device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const BpMem<F, I>& m, int nbp, F cur, F old) {
+    const int n = R.n_node;
     const int cap = v.cap;
-    // initial messages = 1, old beliefs = prob (:1110-1133)
-    for (int e = c.gtid; e < nbp; e += c.gsize)
-#pragma unroll
-        for (int r = 0; r < 12; ++r) v.msg[r * cap + e] = 1.f;
-    for (int i = c.gtid; i < R.n_node * 6; i += c.gsize) v.nold[i] = v.nprob[i];
-    team_sync(c);
-    float* cur = v.ncur;
-    float* old = v.nold;
+    unsigned long long t0 = ptick(c);
     int po = 0;
-    rot_edge_pass(c, v, nbp, old, po);
+    rot_edge_pass(c, m, nbp, old, po);
+    pacc(c, PROF_BP_EDGE, t0);
     po = 1 - po;  // new messages now live in half po
-    rot_node_pass(c, v, R.n_node, cur, old, po, 0.f, true);
+    t0 = ptick(c);
+    rot_node_pass(c, m, n, cur, old, po, 0.f, true);
+    pacc(c, PROF_BP_NODE, t0);
     float dev = 1e10f;
     int it = 0;
     for (; dev > R.tol && it < R.max_iter; it += R.chunk) {
         for (int j = 0; j < R.chunk; ++j) {
-            float* t = cur; cur = old; old = t;     // swap node beliefs
-            rot_edge_pass(c, v, nbp, old, po);      // messages old = half po
+            F t = cur; cur = old; old = t;          // swap node beliefs
+            t0 = ptick(c);
+            rot_edge_pass(c, m, nbp, old, po);      // messages old = half po
+            pacc(c, PROF_BP_EDGE, t0);
             po = 1 - po;
-            rot_node_pass(c, v, R.n_node, cur, old, po, R.damping, false);
+            t0 = ptick(c);
+            rot_node_pass(c, m, n, cur, old, po, R.damping, false);
+            pacc(c, PROF_BP_NODE, t0);
         }
         float d = 0.f;
-        for (int x = c.gtid; x < R.n_node; x += c.gsize) {
-            if (v.nrot[x] == 1) continue;
+        for (int x = c.tid; x < n; x += NT) {
+            if (m.nrot[x] == 1) continue;
 #pragma unroll
             for (int r = 0; r < 6; ++r) d = fmaxf(d, cur[x * 6 + r] - old[x * 6 + r]);
         }
-        dev = team_max(c, d);
+        dev = block_max(c, d);
     }
-    // node marginals (:283-290); final beliefs may sit in either buffer
-    for (int x = c.gtid; x < R.n_node; x += c.gsize) {
-        int nr = v.nrot[x];
+    t0 = ptick(c);
+    // node marginals (:283-290) into memory and into `old` (free now; cur and
+    // old are always distinct buffers)
+    gp<float> ncur = gmem(v.ncur);
+    for (int x = c.tid; x < n; x += NT) {
+        int nr = m.nrot[x];
         float b[6];
         float s = 0.f;
 #pragma unroll
         for (int r = 0; r < 6; ++r) { b[r] = cur[x * 6 + r]; s += b[r]; }
         float is = 1.f / s;
 #pragma unroll
-        for (int r = 0; r < 6; ++r) v.ncur[x * 6 + r] = nr == 1 ? (r == 0 ? 1.f : 0.f) : b[r] * is;
+        for (int r = 0; r < 6; ++r) b[r] = nr == 1 ? (r == 0 ? 1.f : 0.f) : b[r] * is;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) { ncur[x * 6 + r] = b[r]; old[x * 6 + r] = b[r]; }
     }
-    team_sync(c);
+    __syncthreads();
     // edge marginals (:425-449)
-    for (int e = c.gtid; e < nbp; e += c.gsize) {
-        int f = v.bpf[e], s = v.bps[e];
+    gp<float> marg = gmem(v.marg);
+    for (int e = c.tid; e < nbp; e += NT) {
+        int f = m.bpf[e], s = m.bps[e];
         float b1[6], b2[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            b1[r] = v.ncur[f * 6 + r] / (1e-10f + v.msg[(po * 12 + r) * cap + e]);
-            b2[r] = v.ncur[s * 6 + r] / (1e-10f + v.msg[(po * 12 + 6 + r) * cap + e]);
+            b1[r] = old[f * 6 + r] / (1e-10f + m.msg[(po * 12 + r) * m.ms + e]);
+            b2[r] = old[s * 6 + r] / (1e-10f + m.msg[(po * 12 + 6 + r) * m.ms + e]);
         }
-        // two sweeps (sum, then scaled store) keep 36 products out of registers
+        float p[36];
+#pragma unroll
+        for (int q = 0; q < 36; ++q) p[q] = m.P[q * cap + e];
         float sum = 0.f;
+#pragma unroll
         for (int i = 0; i < 6; ++i)
 #pragma unroll
-            for (int j = 0; j < 6; ++j) sum += v.P[(i * 6 + j) * cap + e] * b1[i] * b2[j];
+            for (int j = 0; j < 6; ++j) sum += p[i * 6 + j] * b1[i] * b2[j];
         float is = 1.f / sum;
+#pragma unroll
         for (int i = 0; i < 6; ++i)
 #pragma unroll
-            for (int j = 0; j < 6; ++j) v.marg[(i * 6 + j) * cap + e] = v.P[(i * 6 + j) * cap + e] * b1[i] * b2[j] * is;
+            for (int j = 0; j < 6; ++j) marg[(i * 6 + j) * cap + e] = p[i * 6 + j] * b1[i] * b2[j] * is;
     }
+    pacc(c, PROF_BP_SETUP, t0);
+    return it;
+}
+
+// The solve proper, on one workgroup: a pass has a few thousand flops per
+// node, too little to pay for team barriers, and a single workgroup can keep
+// messages, beliefs and incidences in LDS (then every dependent load of the
+// node products is an LDS read).  Returns the iteration count.
+__device__ int rot_solve_wg(Ctx& c, const RotamerDev& R) {
+    RotView v = rot_view(c, R);
+    const int nbp = v.stats[4];
+    const int n = R.n_node;
+    const int cap = v.cap;
+    unsigned long long t0 = ptick(c);
+    if (28 * nbp + 20 * n + 1 <= LDS_FLOATS) {
+        lp<float> L = lmem(c.lds);
+        BpMem<lp<float>, lp<int>> m;
+        m.P = gmem((const float*)v.P); m.ps = cap; m.nrot = gmem(v.nrot);
+        m.msg = L; m.ms = nbp; L += 24 * nbp;
+        lp<float> cur = L; L += 6 * n;
+        lp<float> old = L; L += 6 * n;
+        lp<float> pr = L; L += 6 * n;
+        lp<int> Ii = (lp<int>)L;
+        lp<int> bpf = Ii; Ii += nbp;
+        lp<int> bps = Ii; Ii += nbp;
+        lp<int> ioff = Ii; Ii += n + 1;
+        lp<int> ninc = Ii; Ii += n;
+        lp<int> inc = Ii;  // 2*nbp: every BP edge has two incidences
+        gp<const float> nprob = gmem((const float*)v.nprob);
+        for (int k = c.tid; k < 6 * n; k += NT) { float q = nprob[k]; pr[k] = q; old[k] = q; }
+        for (int e = c.tid; e < nbp; e += NT) { bpf[e] = v.bpf[e]; bps[e] = v.bps[e]; }
+        for (int x = c.tid; x < n; x += NT) { int k = v.ninc[x]; ninc[x] = k; ioff[x] = k; }
+        if (c.tid == 0) ioff[n] = 0;
+        __syncthreads();
+        block_exscan(c, (int*)ioff, n + 1);
+        for (int x = c.tid; x < n; x += NT) {
+            int s0 = v.ioff[x], d0 = ioff[x], k = ninc[x];
+            for (int t = 0; t < k; ++t) inc[d0 + t] = v.binc[s0 + t];
+        }
+        m.prob = pr; m.bpf = bpf; m.bps = bps; m.ioff = ioff; m.ninc = ninc; m.inc = inc;
+        // initial messages = 1, old beliefs = prob (:1110-1133)
+        for (int e = c.tid; e < nbp; e += NT)
+#pragma unroll
+            for (int r = 0; r < 12; ++r) m.msg[r * m.ms + e] = 1.f;
+        __syncthreads();
+        pacc(c, PROF_BP_SETUP, t0);
+        return bp_iterate(c, R, v, m, nbp, cur, old);
+    }
+    BpMem<gp<float>, gp<int>> m;
+    m.P = gmem((const float*)v.P); m.ps = cap; m.nrot = gmem(v.nrot);
+    m.msg = gmem(v.msg); m.ms = cap;
+    m.prob = gmem(v.nprob); m.bpf = gmem(v.bpf); m.bps = gmem(v.bps);
+    m.ioff = gmem(v.ioff); m.ninc = gmem(v.ninc); m.inc = gmem(v.binc);
+    for (int i = c.tid; i < n * 6; i += NT) v.nold[i] = v.nprob[i];
+    for (int e = c.tid; e < nbp; e += NT)
+#pragma unroll
+        for (int r = 0; r < 12; ++r) m.msg[r * m.ms + e] = 1.f;
+    __syncthreads();
+    pacc(c, PROF_BP_SETUP, t0);
+    return bp_iterate(c, R, v, m, nbp, gmem(v.ncur), gmem(v.nold));
+}
+
+// Solve by rank 0 of the team while the others wait at the barrier.
+__device__ int rot_solve(Ctx& c, const RotamerDev& R) {
+    int it = 0;
+    if (c.rank == 0) it = rot_solve_wg(c, R);
     team_sync(c);
     return it;
 }

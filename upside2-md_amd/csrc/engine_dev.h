@@ -12,10 +12,11 @@
 
 namespace upside {
 
-// Dynamic LDS per workgroup: the coefficient table of the pair graph being
-// evaluated is staged here (the largest, the rotamer pair table, is
-// 20*20*54 floats = 86.4 KB, plus read-past padding).
-constexpr int LDS_FLOATS = 22016;
+// Dynamic LDS per workgroup (159 KB of the 160 KB a gfx950 workgroup may
+// hold, the rest being the static reduction scratch): staging for pair-graph
+// coefficient tables (the largest, the rotamer pair table, is 20*20*54 floats
+// = 86.4 KB, plus read-past padding), cell grids, and the rotamer solve state.
+constexpr int LDS_FLOATS = 40704;
 
 enum OpKind : int {
     OP_DIST3D = 1, OP_ANGLE, OP_DIHEDRAL, OP_SPRING, OP_RAMA_COORD, OP_CAT_POS,
@@ -129,6 +130,7 @@ struct OpRec {
     int plans[4];          // gather plans (const int arena offsets, -1: none)
     int graph;             // index into PairGraph table (-1 if none)
     int rot;               // index into RotamerDev table (-1 if none)
+    int solo;              // run by one workgroup of the team (small elementwise op)
 };
 
 struct SysDesc {
@@ -152,7 +154,8 @@ struct SysDesc {
     float mom_scale, noise_scale;
     uint64_t n_invocations;
     int error;             // int offset of error word
-    int bar;               // int offset of the team barrier words [count, generation, timeout, pad]
+    int bar;               // int offset (16-byte aligned) of the team barrier words: [0..1] 64-bit
+                           // arrival counter, [2] spin-timeout flag, [3] pad
     int tred;              // float offset of team reduction scratch [2 * MAX_TEAM]
     unsigned long long* prof;  // optional per-op wall-clock accumulators (see PROF_*), or null
 };
@@ -162,7 +165,8 @@ struct SysDesc {
 enum {
     PROF_BP = 0, PROF_REBUILD = 1, PROF_THERMO = 2, PROF_INTEGRATE = 3, PROF_PG_REBUILD = 4, PROF_PG_EDGES = 5,
     PROF_PG_GATHER = 6, PROF_ROT_FILL = 7, PROF_ROT_REBUILD = 8, PROF_RB_COUNT = 9, PROF_RB_FILL = 10,
-    PROF_RB_MAP = 11, PROF_RB_GRID = 12, PROF_RB_SCAN = 13, PROF_EXTRA = 14
+    PROF_RB_MAP = 11, PROF_RB_GRID = 12, PROF_RB_SCAN = 13, PROF_BP_EDGE = 14, PROF_BP_NODE = 15,
+    PROF_BP_SETUP = 16, PROF_EXTRA = 17
 };
 
 struct RunArgs {

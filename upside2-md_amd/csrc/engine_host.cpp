@@ -1126,6 +1126,18 @@ static void threefry_host(const uint32_t ctr[4], const uint32_t key[4], uint32_t
     for (int i = 0; i < 4; ++i) out[i] = x[i];
 }
 
+// Op kinds with work in the sensitivity pass (the others push in their value pass).
+static bool has_deriv_step(int kind) {
+    switch (kind) {
+    case OP_DIST3D: case OP_ANGLE: case OP_DIHEDRAL: case OP_RAMA_COORD: case OP_CAT_POS: case OP_AFFINE:
+    case OP_PLACEMENT: case OP_INFER_HO: case OP_PROTEIN_HBOND: case OP_HBOND_COVERAGE: case OP_WEIGHTED_POS:
+    case OP_ENV_COVERAGE: case OP_HB_ENV_COVERAGE:
+        return true;
+    default:
+        return false;
+    }
+}
+
 struct Program {
     string path;
     Topology T;
@@ -1147,9 +1159,20 @@ struct Program {
         T.cap_scale = cap_scale;
         load_topology(T, h5, n_atom);
         release();
+        // potential nodes push their sensitivities in the value pass: their
+        // deriv steps have nothing to do and stay out of the program
+        T.deriv.erase(std::remove_if(T.deriv.begin(), T.deriv.end(),
+                                     [&](int i) { return !has_deriv_step(T.ops[T.nodes[i].op].kind); }),
+                      T.deriv.end());
         vector<OpRec> prog;
         for (size_t k = 1; k < T.germ.size(); ++k) prog.push_back(T.ops[T.nodes[T.germ[k]].op]);
         for (int i : T.deriv) prog.push_back(T.ops[T.nodes[i].op]);
+        // ops without a pair graph or rotamer solve are elementwise over at most
+        // a few thousand elements: one workgroup runs them (UPSIDE_SOLO_OPS=0:
+        // the whole team)
+        const char* so = std::getenv("UPSIDE_SOLO_OPS");
+        const bool solo_ok = !(so && so[0] == '0');
+        for (auto& o : prog) o.solo = (solo_ok && o.graph < 0 && o.rot < 0) ? 1 : 0;
         auto up = [](void** dst, const void* src, size_t bytes, const char* what) {
             hip_check(hipMalloc(dst, std::max(bytes, (size_t)16)), what);
             if (bytes) hip_check(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), what);
@@ -1229,7 +1252,8 @@ struct Batch {
                                            "rotamer fill (in value)", "rotamer candidates (in rebuild)",
                                            "rebuild: count scans", "rebuild: fill scans",
                                            "rebuild: transposed slot mapping",
-                                           "rebuild: LDS grid builds", "rebuild: grid scans"};
+                                           "rebuild: LDS grid builds", "rebuild: grid scans",
+                                           "BP: message passes", "BP: node passes", "BP: staging and marginals"};
                 label = ex[i - ng - (int)T.deriv.size()];
             }
             snprintf(line, sizeof(line), "%-48s %12.6f s\n", label.c_str(), t[i] * tick);
@@ -1399,6 +1423,9 @@ struct Batch {
     void set_team(int t) {
         if (t < 1 || t > max_team()) fail("team size " + std::to_string(t) + " not possible (max " +
                                           std::to_string(max_team()) + " for this batch)");
+        if (t != team)  // the barrier counters must be multiples of the new size
+            for (auto& sy : sys)
+                hip_check(hipMemset(sy.iarena + sy.prog->T.bar, 0, 4 * sizeof(int)), "barrier reset");
         team = t;
     }
 
@@ -1432,13 +1459,13 @@ struct Batch {
         for (int k = 0; k < n_cand; ++k) {
             if (ms_out) ms_out[k] = -1.f;
             if (cand[k] < 1 || cand[k] > limit) continue;
-            team = cand[k];
+            set_team(cand[k]);
             run(n_round, 0, nullptr, 0, 0);
             if (ms_out) ms_out[k] = last_ms;
             if (last_ms < best_ms) { best_ms = last_ms; best = cand[k]; }
             restore();
         }
-        team = best;
+        set_team(best);
     }
 
     void launch(const RunArgs& a0) {
@@ -1453,7 +1480,10 @@ struct Batch {
             for (auto& sy : sys) {
                 unsigned w[4];
                 hip_check(hipMemcpy(w, sy.iarena + sy.prog->T.bar, sizeof(w), hipMemcpyDeviceToHost), "barrier");
+                unsigned long long count = (unsigned long long)w[0] | ((unsigned long long)w[1] << 32);
                 if (w[2]) fail("team barrier wait timed out: results of this run are void");
+                if (count % (unsigned long long)team)
+                    fail("team barrier arrivals unbalanced at the end of a launch: results of this run are void");
             }
     }
 
@@ -1954,6 +1984,28 @@ int upside_batch_replica_exchange(UpsideBatch* b, int n_pairs, const int* pairs,
     }
     return 0;
     ABI_CATCH(1)
+}
+
+int upside_batch_potential_terms(UpsideBatch* b, int sys, float* values, int max_terms, char* names, int names_len) {
+    ABI_TRY
+    auto& B = b->batch;
+    if (sys < 0 || sys >= (int)B.sys.size()) fail("system index out of range");
+    const Topology& T = B.sys[sys].prog->T;
+    vector<float> p(std::max(T.n_pot, 1));
+    hip_check(hipMemcpy(p.data(), B.sys[sys].arena + T.pot, p.size() * 4, hipMemcpyDeviceToHost), "potential");
+    string nm;
+    for (int slot = 0; slot < T.n_pot; ++slot) {
+        for (auto& n : T.nodes)
+            if (n.potential && n.pot_slot == slot) nm += n.name + "\n";
+        if (values && slot < max_terms) values[slot] = p[slot];
+    }
+    if (names && names_len > 0) {
+        size_t k = std::min(nm.size(), (size_t)names_len - 1);
+        std::memcpy(names, nm.data(), k);
+        names[k] = 0;
+    }
+    return T.n_pot;
+    ABI_CATCH(-1)
 }
 
 int upside_batch_bp_stats(UpsideBatch* b, int sys, int* last_iters, long long* total_iters, long long* solves,

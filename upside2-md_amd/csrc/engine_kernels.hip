@@ -11,12 +11,13 @@
 
 namespace upside {
 
-#define OUT(op) (c.A + (op).out)
-#define SENS(op) (c.A + (op).sens)
-#define IN(op, k) (c.A + (op).in_out[k])
-#define INS(op, k) (c.A + (op).in_sens[k])
-#define CIA(op, k) (c.CI + (op).ia[k])
-#define CFA(op, k) (c.CF + (op).ia[k])
+// operand pointers, typed as global memory (see gmem)
+#define OUT(op) gmem(c.A + (op).out)
+#define SENS(op) gmem(c.A + (op).sens)
+#define IN(op, k) gmem(c.A + (op).in_out[k])
+#define INS(op, k) gmem(c.A + (op).in_sens[k])
+#define CIA(op, k) gmem(c.CI + (op).ia[k])
+#define CFA(op, k) gmem(c.CF + (op).ia[k])
 
 // Profile accumulation (op slots by index, fixed slots through pacc).
 __device__ __forceinline__ void prof_op(const Ctx& c, int slot, unsigned long long t0) {
@@ -29,11 +30,11 @@ __device__ __forceinline__ void put_pot(Ctx& c, const SysDesc& S, const OpRec& o
 
 // ---- Distance3D (src/coord_basic.cpp:13-66) --------------------------------
 __device__ void dist3d_fwd(Ctx& c, const OpRec& op) {
-    const int* id = CIA(op, 0);
-    float* der = c.A + op.scratch;
+    auto id = CIA(op, 0);
+    auto der = gmem(c.A + op.scratch);
     for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
-        const float* a = IN(op, 0) + id[2 * e] * op.in_w[0];
-        const float* b = IN(op, 1) + id[2 * e + 1] * op.in_w[1];
+        auto a = IN(op, 0) + id[2 * e] * op.in_w[0];
+        auto b = IN(op, 1) + id[2 * e + 1] * op.in_w[1];
         float dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
         float m2 = dx * dx + dy * dy + dz * dz;
         float d = sqrtf(m2);
@@ -43,8 +44,8 @@ __device__ void dist3d_fwd(Ctx& c, const OpRec& op) {
     }
 }
 __device__ void dist3d_bwd(Ctx& c, const OpRec& op) {
-    const float* der = c.A + op.scratch;
-    float* con = c.A + op.ia[1];
+    auto der = gmem(c.A + op.scratch);
+    auto con = gmem(c.A + op.ia[1]);
     for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
         float s = SENS(op)[e];
         for (int k = 0; k < 3; ++k) {
@@ -59,14 +60,14 @@ __device__ void dist3d_bwd(Ctx& c, const OpRec& op) {
 
 // ---- Angle: cosine at the third atom (src/coord_basic.cpp:183-243) ---------
 __device__ void angle_fwd(Ctx& c, const OpRec& op) {
-    const int* id = CIA(op, 0);
-    float* der = c.A + op.scratch;
-    const float* P = IN(op, 0);
+    auto id = CIA(op, 0);
+    auto der = gmem(c.A + op.scratch);
+    auto P = IN(op, 0);
     int w = op.in_w[0];
     for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
-        const float* a1 = P + id[3 * e] * w;
-        const float* a2 = P + id[3 * e + 1] * w;
-        const float* a3 = P + id[3 * e + 2] * w;
+        auto a1 = P + id[3 * e] * w;
+        auto a2 = P + id[3 * e + 1] * w;
+        auto a3 = P + id[3 * e + 2] * w;
         float x1[3], x2[3];
         for (int k = 0; k < 3; ++k) { x1[k] = a1[k] - a3[k]; x2[k] = a2[k] - a3[k]; }
         float i1 = rsqrtf(x1[0] * x1[0] + x1[1] * x1[1] + x1[2] * x1[2]);
@@ -82,8 +83,8 @@ __device__ void angle_fwd(Ctx& c, const OpRec& op) {
     }
 }
 __device__ void angle_bwd(Ctx& c, const OpRec& op) {
-    const float* der = c.A + op.scratch;
-    float* con = c.A + op.ia[1];
+    auto der = gmem(c.A + op.scratch);
+    auto con = gmem(c.A + op.ia[1]);
     for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
         float s = SENS(op)[e];
         for (int k = 0; k < 3; ++k) {
@@ -98,26 +99,28 @@ __device__ void angle_bwd(Ctx& c, const OpRec& op) {
 }
 
 // ---- Dihedral (src/coord_basic.cpp:311-374, src/vector_math.h:764-797) -----
-__device__ __forceinline__ void cross3(const float* a, const float* b, float* o) {
+template <class P, class Q, class O>
+__device__ __forceinline__ void cross3(P a, Q b, O o) {
     o[0] = a[1] * b[2] - a[2] * b[1];
     o[1] = a[2] * b[0] - a[0] * b[2];
     o[2] = a[0] * b[1] - a[1] * b[0];
 }
-__device__ __forceinline__ float dot3(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+template <class P, class Q>
+__device__ __forceinline__ float dot3(P a, Q b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
 __device__ void dihedral_fwd(Ctx& c, const OpRec& op) {
-    const int* id = CIA(op, 0);
-    float* jac = c.A + op.scratch;
-    const float* P = IN(op, 0);
+    auto id = CIA(op, 0);
+    auto jac = gmem(c.A + op.scratch);
+    auto P = IN(op, 0);
     int w = op.in_w[0];
     for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
-        const int* q = id + 4 * e;
+        auto q = id + 4 * e;
         if (q[0] < 0 || q[3] < 0) {
             OUT(op)[e] = -1.3963f;
             for (int k = 0; k < 12; ++k) jac[e * 12 + k] = 0.f;
             continue;
         }
-        const float *r1 = P + q[0] * w, *r2 = P + q[1] * w, *r3 = P + q[2] * w, *r4 = P + q[3] * w;
+        auto r1 = P + q[0] * w; auto r2 = P + q[1] * w; auto r3 = P + q[2] * w; auto r4 = P + q[3] * w;
         float F[3], G[3], H[3], A[3], B[3], C[3];
         for (int k = 0; k < 3; ++k) { F[k] = r1[k] - r2[k]; G[k] = r2[k] - r3[k]; H[k] = r4[k] - r3[k]; }
         cross3(F, G, A);
@@ -141,8 +144,8 @@ __device__ void dihedral_fwd(Ctx& c, const OpRec& op) {
     }
 }
 __device__ void dihedral_bwd(Ctx& c, const OpRec& op) {
-    const float* jac = c.A + op.scratch;
-    float* con = c.A + op.ia[1];
+    auto jac = gmem(c.A + op.scratch);
+    auto con = gmem(c.A + op.ia[1]);
     for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
         float s = SENS(op)[e];
         for (int k = 0; k < 12; ++k) con[e * 12 + k] = s * jac[e * 12 + k];
@@ -153,12 +156,12 @@ __device__ void dihedral_bwd(Ctx& c, const OpRec& op) {
 
 // ---- Spring (src/spring.cpp:9-81) -------------------------------------------
 __device__ void spring_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
-    const int* id = CIA(op, 0);
-    const float* eq = CFA(op, 1);
-    const float* kk = CFA(op, 2);
+    auto id = CIA(op, 0);
+    auto eq = CFA(op, 1);
+    auto kk = CFA(op, 2);
     int dim1 = op.ia[3], pbc = op.ia[4], n = op.ia[5];
     float box = op.fa[0];
-    float* con = c.A + op.scratch;
+    auto con = gmem(c.A + op.scratch);
     float pot = 0.f;
     for (int e = c.gtid; e < n; e += c.gsize) {
         float x = IN(op, 0)[id[e] * op.in_w[0] + dim1];
@@ -179,19 +182,19 @@ __device__ void spring_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
 
 // ---- rama_coord / cat_pos (src/coord_map.cpp:69-161) -----------------------
 __device__ void rama_coord_fwd(Ctx& c, const OpRec& op) {
-    const int* i1 = CIA(op, 0);
-    const int* i2 = CIA(op, 1);
+    auto i1 = CIA(op, 0);
+    auto i2 = CIA(op, 1);
     for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
         OUT(op)[2 * e] = IN(op, 0)[i1[e] * op.in_w[0]];
         OUT(op)[2 * e + 1] = IN(op, 1)[i2[e] * op.in_w[1]];
     }
 }
 __device__ void cat_pos_fwd(Ctx& c, const OpRec& op) {
-    const int* idx = CIA(op, 0);
+    auto idx = CIA(op, 0);
     int n1 = op.ia[1], w = op.width;
     for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
         int k = e < n1 ? 0 : 1;
-        const float* src = IN(op, k) + idx[e] * op.in_w[k];
+        auto src = IN(op, k) + idx[e] * op.in_w[k];
         for (int d = 0; d < w; ++d) OUT(op)[e * w + d] = src[d];
     }
 }
@@ -202,15 +205,15 @@ __device__ void passthrough_bwd(Ctx& c, const OpRec& op) {
 
 // ---- rama_map_pot (src/rama_map_pot.cpp:15-94) -------------------------------
 __device__ void rama_map_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
-    const int* res = CIA(op, 0);
-    const int* mid = CIA(op, 1);
-    const float* coef = CFA(op, 2);
+    auto res = CIA(op, 0);
+    auto mid = CIA(op, 1);
+    auto coef = CFA(op, 2);
     int nx = op.ia[3], n = op.ia[4];
     float scale = op.fa[0], shift = op.fa[1];
-    float* con = c.A + op.scratch;
+    auto con = gmem(c.A + op.scratch);
     float pot = 0.f;
     for (int e = c.gtid; e < n; e += c.gsize) {
-        const float* r = IN(op, 0) + res[e] * op.in_w[0];
+        auto r = IN(op, 0) + res[e] * op.in_w[0];
         float x = (r[0] + shift) * scale, y = (r[1] + shift) * scale;
         int xb = (int)x, yb = (int)y;
         xb = min(max(xb, 0), nx - 1); yb = min(max(yb, 0), nx - 1);
@@ -295,9 +298,9 @@ __device__ void eig4_grouped(float* A, float* d, float* rot) {
         for (; p > 0; --p)
             if (!quad_any(u[p - 1] != 0.f)) break;
         int n = 4 - q - p;
-        float* dd = d + p;
-        float* uu = u + p;
-        float* rr = rot + 4 * p;
+        auto dd = d + p;
+        auto uu = u + p;
+        auto rr = rot + 4 * p;
         float dval = 0.5f * (dd[n - 2] - dd[n - 1]) + 1e-20f;
         float un = uu[n - 2];
         float mu = dd[n - 1] - un * un / (dval + copysignf(sqrtf(dval * dval + un * un), dval));
@@ -329,11 +332,11 @@ __device__ void eig4_grouped(float* A, float* d, float* rot) {
 }
 
 __device__ void affine_fwd(Ctx& c, const OpRec& op) {
-    const int* atoms = CIA(op, 0);
-    const float* ref = CFA(op, 1);
-    float* ev = c.A + op.scratch;            // n*4 eigenvalues
-    float* ec = c.A + op.scratch + 4 * op.n_elem;  // n*16 eigenvector rows
-    const float* P = IN(op, 0);
+    auto atoms = CIA(op, 0);
+    auto ref = CFA(op, 1);
+    auto ev = gmem(c.A + op.scratch);            // n*4 eigenvalues
+    auto ec = gmem(c.A + op.scratch + 4 * op.n_elem);  // n*16 eigenvector rows
+    auto P = IN(op, 0);
     int w = op.in_w[0];
     int n = op.n_elem;
     for (int base = 0; base < n; base += c.gsize) {
@@ -346,7 +349,7 @@ __device__ void affine_fwd(Ctx& c, const OpRec& op) {
         for (int k = 0; k < 3; ++k) center[k] = (1.f / 3.f) * (x[0][k] + x[1][k] + x[2][k]);
         for (int a = 0; a < 3; ++a)
             for (int k = 0; k < 3; ++k) x[a][k] -= center[k];
-        const float* g = ref + rr * 9;
+        auto g = ref + rr * 9;
         float R[3][3];
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) R[i][j] = x[0][j] * g[i] + x[1][j] * g[3 + i] + x[2][j] * g[6 + i];
@@ -358,20 +361,20 @@ __device__ void affine_fwd(Ctx& c, const OpRec& op) {
         if (r >= n) continue;
         for (int k = 0; k < 4; ++k) ev[r * 4 + k] = d[k];
         for (int k = 0; k < 16; ++k) ec[r * 16 + k] = rot[k];
-        float* o = OUT(op) + r * 7;
+        auto o = OUT(op) + r * 7;
         o[0] = center[0]; o[1] = center[1]; o[2] = center[2];
         o[3] = rot[0]; o[4] = rot[1]; o[5] = rot[2]; o[6] = rot[3];
     }
 }
 __device__ void affine_bwd(Ctx& c, const OpRec& op) {
-    const float* ref = CFA(op, 1);
-    const float* ev = c.A + op.scratch;
-    const float* ec = c.A + op.scratch + 4 * op.n_elem;
-    float* con = c.A + op.ia[2];
+    auto ref = CFA(op, 1);
+    auto ev = gmem(c.A + op.scratch);
+    auto ec = gmem(c.A + op.scratch + 4 * op.n_elem);
+    auto con = gmem(c.A + op.ia[2]);
     for (int r = c.gtid; r < op.n_elem; r += c.gsize) {
-        const float* s = SENS(op) + r * 7;
-        const float* E = ec + r * 16;
-        const float* q = E;
+        auto s = SENS(op) + r * 7;
+        auto E = ec + r * 16;
+        auto q = E;
         float t0 = s[3], t1 = s[4], t2 = s[5];
         float qs[4] = {2.f * (-t0 * q[1] - t1 * q[2] - t2 * q[3]), 2.f * (t0 * q[0] + t1 * q[3] - t2 * q[2]),
                        2.f * (t1 * q[0] + t2 * q[1] - t0 * q[3]), 2.f * (t2 * q[0] + t0 * q[2] - t1 * q[1])};
@@ -383,7 +386,7 @@ __device__ void affine_bwd(Ctx& c, const OpRec& op) {
         }
         for (int k = 1; k < 4; ++k) inv[k] = 1.f / (ev[r * 4] - ev[r * 4 + k]);
         for (int na = 0; na < 3; ++na) {
-            const float* g = ref + r * 9 + na * 3;
+            auto g = ref + r * 9 + na * 3;
             float gx = g[0], gy = g[1], gz = g[2];
             // upper triangles (f00 f01 f02 f03 f11 f12 f13 f22 f23 f33), src/eig.cpp:452-467
             float M[3][10] = {{gx, 0.f, gz, -gy, gx, gy, gz, -gx, 0.f, -gx},
@@ -411,18 +414,18 @@ __device__ void affine_bwd(Ctx& c, const OpRec& op) {
 
 // ---- placement (src/placement.cpp:20-314) -------------------------------------
 __device__ void placement_fwd(Ctx& c, const OpRec& op) {
-    const int* ares = CIA(op, 0);
-    const int* layer = CIA(op, 1);
-    const float* data = CFA(op, 2);
+    auto ares = CIA(op, 0);
+    auto layer = CIA(op, 1);
+    auto data = CFA(op, 2);
     int is_rama = op.ia[3], dim = op.ia[7];
-    const float* aff = IN(op, 0);
+    auto aff = IN(op, 0);
     int wa = op.in_w[0];
     for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
-        float* o = OUT(op) + e * op.width;
+        auto o = OUT(op) + e * op.width;
         if (is_rama) {
-            const int* rres = CIA(op, 4);
+            auto rres = CIA(op, 4);
             int nx = op.ia[5], ny = op.ia[6];
-            const float* r = IN(op, 1) + rres[e] * op.in_w[1];
+            auto r = IN(op, 1) + rres[e] * op.in_w[1];
             float x = (r[0] + op.fa[2]) * op.fa[0], y = (r[1] + op.fa[2]) * op.fa[1];
             int xb = min(max((int)x, 0), nx - 1), yb = min(max((int)y, 0), ny - 1);
             float v, dx, dy;
@@ -432,10 +435,10 @@ __device__ void placement_fwd(Ctx& c, const OpRec& op) {
             c.A[op.scratch + 2 * e + 1] = dy * op.fa[1];
             continue;
         }
-        const float* a = aff + ares[e] * wa;
+        auto a = aff + ares[e] * wa;
         float U[9];
         quat_to_rot(a + 3, U);
-        const float* v = data + layer[e] * dim;
+        auto v = data + layer[e] * dim;
         for (int k = 0; k < 3; ++k) {
             o[k] = U[3 * k] * v[0] + U[3 * k + 1] * v[1] + U[3 * k + 2] * v[2] + a[k];
             o[3 + k] = U[3 * k] * v[3] + U[3 * k + 1] * v[4] + U[3 * k + 2] * v[5];
@@ -445,7 +448,7 @@ __device__ void placement_fwd(Ctx& c, const OpRec& op) {
 }
 __device__ void placement_bwd(Ctx& c, const OpRec& op) {
     int is_rama = op.ia[3];
-    float* con = c.A + op.ia[8];
+    auto con = gmem(c.A + op.ia[8]);
     if (is_rama) {
         for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
             float s = SENS(op)[e];
@@ -456,13 +459,13 @@ __device__ void placement_bwd(Ctx& c, const OpRec& op) {
         plan_gather(c, op.plans[1]);
         return;
     }
-    const int* ares = CIA(op, 0);
-    const float* aff = IN(op, 0);
+    auto ares = CIA(op, 0);
+    auto aff = IN(op, 0);
     int wa = op.in_w[0];
     for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
-        const float* s = SENS(op) + e * op.width;
-        const float* x = OUT(op) + e * op.width;
-        const float* t = aff + ares[e] * wa;
+        auto s = SENS(op) + e * op.width;
+        auto x = OUT(op) + e * op.width;
+        auto t = aff + ares[e] * wa;
         float p[3] = {x[0] - t[0], x[1] - t[1], x[2] - t[2]};
         float tq1[3], tq2[3];
         cross3(p, s, tq1);
@@ -478,33 +481,33 @@ __device__ void placement_bwd(Ctx& c, const OpRec& op) {
 
 // ---- infer_H_O (src/hbond.cpp:20-127) ------------------------------------------
 __device__ void infer_fwd(Ctx& c, const OpRec& op) {
-    const int* id = CIA(op, 0);
-    const float* bl = CFA(op, 1);
-    float* cache = c.A + op.scratch;
-    const float* P = IN(op, 0);
+    auto id = CIA(op, 0);
+    auto bl = CFA(op, 1);
+    auto cache = gmem(c.A + op.scratch);
+    auto P = IN(op, 0);
     int w = op.in_w[0];
     for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
-        const float *pc = P + id[3 * e] * w, *cc = P + id[3 * e + 1] * w, *nc = P + id[3 * e + 2] * w;
+        auto pc = P + id[3 * e] * w; auto cc = P + id[3 * e + 1] * w; auto nc = P + id[3 * e + 2] * w;
         float pr[3], nx[3], ds[3];
         for (int k = 0; k < 3; ++k) { pr[k] = pc[k] - cc[k]; nx[k] = nc[k] - cc[k]; }
         float pi = rsqrtf(dot3(pr, pr)), ni = rsqrtf(dot3(nx, nx));
         for (int k = 0; k < 3; ++k) { pr[k] *= pi; nx[k] *= ni; ds[k] = pr[k] + nx[k]; }
         float di = rsqrtf(dot3(ds, ds));
         for (int k = 0; k < 3; ++k) ds[k] *= di;
-        float* cs = cache + e * 12;
+        auto cs = cache + e * 12;
         for (int k = 0; k < 3; ++k) { cs[k] = pr[k]; cs[4 + k] = nx[k]; cs[8 + k] = ds[k]; }
         cs[3] = pi; cs[7] = ni; cs[11] = di;
-        float* o = OUT(op) + e * 6;
+        auto o = OUT(op) + e * 6;
         for (int k = 0; k < 3; ++k) { o[k] = cc[k] - bl[e] * ds[k]; o[3 + k] = -ds[k]; }
     }
 }
 __device__ void infer_bwd(Ctx& c, const OpRec& op) {
-    const float* bl = CFA(op, 1);
-    const float* cache = c.A + op.scratch;
-    float* con = c.A + op.ia[2];
+    auto bl = CFA(op, 1);
+    auto cache = gmem(c.A + op.scratch);
+    auto con = gmem(c.A + op.ia[2]);
     for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
-        const float* s = SENS(op) + e * 6;
-        const float* cs = cache + e * 12;
+        auto s = SENS(op) + e * 6;
+        auto cs = cache + e * 12;
         float sn[3], sd[3], sp[3], sx[3];
         for (int k = 0; k < 3; ++k) sn[k] = s[3 + k] + bl[e] * s[k];
         float t = dot3(cs + 8, sn);
@@ -536,17 +539,17 @@ __device__ void graph_overflow(Ctx& c, const SysDesc& S) {
 __device__ void protein_hbond_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G) {
     if (!pg_refresh(c, G, op)) { graph_overflow(c, S); return; }
     int nd = G.n1;
-    const int* loc1 = c.CI + G.loc1;
-    const int* loc2 = c.CI + G.loc2;
-    float* ss = c.A + op.scratch;  // per-site sum of hb_log
+    auto loc1 = gmem(c.CI + G.loc1);
+    auto loc2 = gmem(c.CI + G.loc2);
+    auto ss = gmem(c.A + op.scratch);  // per-site sum of hb_log
     pg_row_sums(c, G, false, ss, 1);
     pg_row_sums(c, G, true, ss + nd, 1);
     team_sync(c);
     for (int v = c.gtid; v < op.n_elem; v += c.gsize) {
         bool don = v < nd;
         int i = don ? v : v - nd;
-        const float* src = IN(op, 0) + (don ? loc1[i] : loc2[i]) * op.in_w[0];
-        float* o = OUT(op) + v * 7;
+        auto src = IN(op, 0) + (don ? loc1[i] : loc2[i]) * op.in_w[0];
+        auto o = OUT(op) + v * 7;
 #pragma unroll
         for (int k = 0; k < 6; ++k) o[k] = src[k];
         o[6] = 1.f - expf(-ss[v]);
@@ -554,19 +557,19 @@ __device__ void protein_hbond_fwd(Ctx& c, const SysDesc& S, const OpRec& op, con
 }
 __device__ void protein_hbond_bwd(Ctx& c, const OpRec& op, const PairGraph& G) {
     int nd = G.n1;
-    float* ss = c.A + op.scratch;
+    auto ss = gmem(c.A + op.scratch);
     for (int v = c.gtid; v < op.n_elem; v += c.gsize) ss[v] = SENS(op)[v * 7 + 6] * (1.f - OUT(op)[v * 7 + 6]);
     team_sync(c);
-    const int* row = c.I + G.row;
-    const int* nbr = c.I + G.nbr;
-    float* es = c.A + G.esens;
+    auto row = gmem(c.I + G.row);
+    auto nbr = gmem(c.I + G.nbr);
+    auto es = gmem(c.A + G.esens);
     const int E = c.I[G.state + 3];
     for (int s = c.gtid; s < E; s += c.gsize) es[s] = ss[row[s]] + ss[nd + nbr[s]];
     team_sync(c);
     pg_gather(c, G);
     // pass the coordinate sensitivities through to infer_H_O (:419-433)
-    const int* loc1 = c.CI + G.loc1;
-    const int* loc2 = c.CI + G.loc2;
+    auto loc1 = gmem(c.CI + G.loc1);
+    auto loc2 = gmem(c.CI + G.loc2);
     for (int i = c.gtid; i < G.n1; i += c.gsize)
 #pragma unroll
         for (int k = 0; k < 6; ++k) INS(op, 0)[loc1[i] * op.in_w[0] + k] += SENS(op)[i * 7 + k];
@@ -583,8 +586,8 @@ __device__ void hbond_coverage_fwd(Ctx& c, const SysDesc& S, const OpRec& op, co
     pg_row_sums(c, G, true, OUT(op), 1);
 }
 __device__ void hbond_coverage_bwd(Ctx& c, const OpRec& op, const PairGraph& G) {
-    const int* nbr = c.I + G.nbr;
-    float* es = c.A + G.esens;
+    auto nbr = gmem(c.I + G.nbr);
+    auto es = gmem(c.A + G.esens);
     const int E = c.I[G.state + 3];
     for (int s = c.gtid; s < E; s += c.gsize) es[s] = SENS(op)[nbr[s]];
     team_sync(c);
@@ -596,18 +599,18 @@ __device__ void hbond_coverage_bwd(Ctx& c, const OpRec& op, const PairGraph& G) 
 //     5 first donor of residue, 6 first acceptor of residue (virtual index or -1)
 __device__ void hbond_energy_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
     int n_res = op.ia[0];
-    const int* h1o = CIA(op, 1);
-    const int* h1i = CIA(op, 2);
-    const int* h2o = CIA(op, 3);
-    const int* h2i = CIA(op, 4);
-    const int* fd = CIA(op, 5);
-    const int* fa = CIA(op, 6);
-    const float* pp = IN(op, 0);
-    float* pps = INS(op, 0);
-    const float* rama = IN(op, 1);
-    float* ramas = INS(op, 1);
+    auto h1o = CIA(op, 1);
+    auto h1i = CIA(op, 2);
+    auto h2o = CIA(op, 3);
+    auto h2i = CIA(op, 4);
+    auto fd = CIA(op, 5);
+    auto fa = CIA(op, 6);
+    auto pp = IN(op, 0);
+    auto pps = INS(op, 0);
+    auto rama = IN(op, 1);
+    auto ramas = INS(op, 1);
     int wp = op.in_w[0], wr = op.in_w[1];
-    const float* P = op.fa;
+    auto P = op.fa;
     float pot = 0.f;
     for (int i = c.gtid; i < n_res; i += c.gsize) {
         float phi = rama[i * wr], psi = rama[i * wr + 1];
@@ -652,17 +655,17 @@ __device__ void hbond_energy_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
 
 // ---- weighted_pos (src/environment.cpp:489-533) -------------------------------------
 __device__ void weighted_pos_fwd(Ctx& c, const OpRec& op) {
-    const int* ip = CIA(op, 0);
-    const int* iw = CIA(op, 1);
+    auto ip = CIA(op, 0);
+    auto iw = CIA(op, 1);
     for (int e = c.gtid; e < op.n_elem; e += c.gsize) {
-        const float* p = IN(op, 0) + ip[e] * op.in_w[0];
-        float* o = OUT(op) + e * 4;
+        auto p = IN(op, 0) + ip[e] * op.in_w[0];
+        auto o = OUT(op) + e * 4;
         o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
         o[3] = expf(-IN(op, 1)[iw[e] * op.in_w[1]]);
     }
 }
 __device__ void weighted_pos_bwd(Ctx& c, const OpRec& op) {
-    float* con = c.A + op.scratch;
+    auto con = gmem(c.A + op.scratch);
     for (int e = c.gtid; e < op.n_elem; e += c.gsize) con[e] = -OUT(op)[e * 4 + 3] * SENS(op)[e * 4 + 3];
     team_sync(c);
     plan_gather(c, op.plans[0]);
@@ -675,12 +678,12 @@ __device__ void weighted_pos_bwd(Ctx& c, const OpRec& op) {
 __device__ void env_coverage_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G) {
     if (!pg_refresh(c, G, op)) { graph_overflow(c, S); return; }
     int naa = op.ia[0];
-    const int* aa = CIA(op, 1);
-    const int* id2 = c.CI + G.id2;
-    const int* off = c.I + G.off;
-    const int* nbr = c.I + G.nbr;
-    const float* val = c.A + G.val;
-    const int* act = c.I + G.act;
+    auto aa = CIA(op, 1);
+    auto id2 = gmem(c.CI + G.id2);
+    auto off = gmem(c.I + G.off);
+    auto nbr = gmem(c.I + G.nbr);
+    auto val = gmem(c.A + G.val);
+    auto act = gmem(c.I + G.act);
     // one thread per (element, type): scan the element's slots in order
     for (int k = c.gtid; k < G.n1 * naa; k += c.gsize) {
         int i = k / naa, t = k - i * naa;
@@ -692,11 +695,11 @@ __device__ void env_coverage_fwd(Ctx& c, const SysDesc& S, const OpRec& op, cons
 }
 __device__ void env_coverage_bwd(Ctx& c, const OpRec& op, const PairGraph& G) {
     int naa = op.ia[0];
-    const int* aa = CIA(op, 1);
-    const int* id2 = c.CI + G.id2;
-    const int* row = c.I + G.row;
-    const int* nbr = c.I + G.nbr;
-    float* es = c.A + G.esens;
+    auto aa = CIA(op, 1);
+    auto id2 = gmem(c.CI + G.id2);
+    auto row = gmem(c.I + G.row);
+    auto nbr = gmem(c.I + G.nbr);
+    auto es = gmem(c.A + G.esens);
     const int E = c.I[G.state + 3];
     for (int s = c.gtid; s < E; s += c.gsize) es[s] = SENS(op)[row[s] * naa + aa[id2[nbr[s]]]];
     team_sync(c);
@@ -709,8 +712,8 @@ __device__ void hb_env_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const Pair
     pg_row_sums(c, G, false, OUT(op), 1);
 }
 __device__ void hb_env_bwd(Ctx& c, const OpRec& op, const PairGraph& G) {
-    const int* row = c.I + G.row;
-    float* es = c.A + G.esens;
+    auto row = gmem(c.I + G.row);
+    auto es = gmem(c.A + G.esens);
     const int E = c.I[G.state + 3];
     for (int s = c.gtid; s < E; s += c.gsize) es[s] = SENS(op)[row[s]];
     team_sync(c);
@@ -721,13 +724,13 @@ __device__ void hb_env_bwd(Ctx& c, const OpRec& op, const PairGraph& G) {
 // ia: 0 n_res, 1 coupling types, 2 weights (nt*nt), 3 scale, 4 center, 5 sharpness, 6 nt
 __device__ void sigmoid_coupling_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
     int n = op.ia[0], nt = op.ia[6];
-    const int* ct = CIA(op, 1);
-    const float* W = CFA(op, 2);
-    const float* sc = CFA(op, 3);
-    const float* ce = CFA(op, 4);
-    const float* sh = CFA(op, 5);
-    const float* x = IN(op, 0);
-    float* xs = INS(op, 0);
+    auto ct = CIA(op, 1);
+    auto W = CFA(op, 2);
+    auto sc = CFA(op, 3);
+    auto ce = CFA(op, 4);
+    auto sh = CFA(op, 5);
+    auto x = IN(op, 0);
+    auto xs = INS(op, 0);
     float pot = 0.f;
     for (int r = c.gtid; r < n; r += c.gsize) {
         int t = ct[r];
@@ -745,13 +748,13 @@ __device__ void sigmoid_coupling_fwd(Ctx& c, const SysDesc& S, const OpRec& op) 
 // ---- bb_sigmoid_coupling (src/environment.cpp:1054-1138) ---------------------------
 __device__ void bb_sigmoid_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
     int nt = op.ia[0];
-    const float* W = CFA(op, 1);
+    auto W = CFA(op, 1);
     int n = op.ia[2];
     float scale = op.fa[0], center = op.fa[1], sharp = op.fa[2], hw = op.fa[3];
-    const float* sc = IN(op, 0);
-    float* scs = INS(op, 0);
-    const float* hb = IN(op, 1);
-    float* hbs = INS(op, 1);
+    auto sc = IN(op, 0);
+    auto scs = INS(op, 0);
+    auto hb = IN(op, 1);
+    auto hbs = INS(op, 1);
     float pot = 0.f;
     for (int r = c.gtid; r < n; r += c.gsize) {
         float bl = 0.f;
@@ -771,9 +774,9 @@ __device__ void bb_sigmoid_fwd(Ctx& c, const SysDesc& S, const OpRec& op) {
 // ---- backbone_pairs (src/backbone_steric.cpp:38-146) --------------------------------
 __device__ void backbone_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G) {
     if (!pg_refresh(c, G, op)) { graph_overflow(c, S); return; }
-    const float* val = c.A + G.val;
-    const int* act = c.I + G.act;
-    float* es = c.A + G.esens;
+    auto val = gmem(c.A + G.val);
+    auto act = gmem(c.I + G.act);
+    auto es = gmem(c.A + G.esens);
     const int E = c.I[G.state + 3];
     float pot = 0.f;
     for (int s = c.gtid; s < E; s += c.gsize) {
@@ -808,7 +811,7 @@ __device__ void rotamer_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const Pai
     int it = rot_solve(c, R);
     pacc(c, PROF_BP, t0);
     if (c.gtid == 0) {
-        int* st = c.I + R.stats;
+        auto st = gmem(c.I + R.stats);
         st[0] = it;
         if (it >= R.max_iter - R.chunk - 1) st[1] += 1;
         st[2] += it;
@@ -870,24 +873,79 @@ __device__ void run_deriv(Ctx& c, const SysDesc& S, const OpRec& op) {
     team_sync(c);
 }
 
+// A team context narrowed to rank 0's workgroup alone (team barriers become
+// workgroup barriers, team reductions block reductions).
+__device__ __forceinline__ Ctx solo_ctx(const Ctx& c) {
+    Ctx l = c;
+    l.G = 1;
+    l.rank = 0;
+    l.gtid = c.tid;
+    l.gsize = NT;
+    return l;
+}
+
+// End of the run of solo ops starting at k (ops [k, end) are all solo).
+__device__ __forceinline__ int solo_run_end(const SysDesc& S, int k, int end) {
+    while (k < end && S.ops[k].solo) ++k;
+    return k;
+}
+
+// One evaluation: value ops in germ order, then sensitivity pushes.  Ops
+// marked solo (elementwise work over at most a few thousand elements) run on
+// rank 0 of the team alone: a run of them goes back to back with workgroup
+// barriers and ends with one team barrier, since a team barrier costs more
+// than such an op.
 __device__ void compute(Ctx& c, const SysDesc& S, int mode) {
-    const int* zr = c.I + S.sens_zero;
+    auto zr = gmem(c.I + S.sens_zero);
     for (int r = 0; r < S.sens_zero_cnt; ++r) {
-        float* p = c.A + zr[2 * r];
+        auto p = gmem(c.A + zr[2 * r]);
         int n = zr[2 * r + 1];
         for (int i = c.gtid; i < n; i += c.gsize) p[i] = 0.f;
     }
     team_sync(c);
-    for (int k = 0; k < S.n_germ; ++k) {
+    for (int k = 0; k < S.n_germ;) {
+        if (c.G > 1 && S.ops[k].solo) {
+            const int k2 = solo_run_end(S, k, S.n_germ);
+            if (c.rank == 0) {
+                Ctx l = solo_ctx(c);
+                for (int q = k; q < k2; ++q) {
+                    unsigned long long t0 = ptick(l);
+                    run_value(l, S, S.ops[q], mode);
+                    prof_op(l, q, t0);
+                    if (l.I[S.error]) break;
+                }
+            }
+            team_sync(c);
+            if (c.I[S.error]) return;
+            k = k2;
+            continue;
+        }
         unsigned long long t0 = ptick(c);
         run_value(c, S, S.ops[k], mode);
         prof_op(c, k, t0);
         if (c.I[S.error]) return;  // uniform: read after the op's final barrier
+        ++k;
     }
-    for (int k = 0; k < S.n_deriv; ++k) {
+    const int nd = S.n_germ + S.n_deriv;
+    for (int k = S.n_germ; k < nd;) {
+        if (c.G > 1 && S.ops[k].solo) {
+            const int k2 = solo_run_end(S, k, nd);
+            if (c.rank == 0) {
+                Ctx l = solo_ctx(c);
+                for (int q = k; q < k2; ++q) {
+                    unsigned long long t0 = ptick(l);
+                    run_deriv(l, S, S.ops[q]);
+                    prof_op(l, q, t0);
+                }
+            }
+            team_sync(c);
+            k = k2;
+            continue;
+        }
         unsigned long long t0 = ptick(c);
-        run_deriv(c, S, S.ops[S.n_germ + k]);
-        prof_op(c, S.n_germ + k, t0);
+        run_deriv(c, S, S.ops[k]);
+        prof_op(c, k, t0);
+        ++k;
     }
 }
 
@@ -936,7 +994,7 @@ __device__ void normal3(uint32_t seed, uint32_t stream, uint32_t atom, uint64_t 
 }
 
 __device__ void thermostat(Ctx& c, SysDesc& S) {
-    float* mom = c.A + S.mom;
+    auto mom = gmem(c.A + S.mom);
     for (int a = c.gtid; a < S.n_atom; a += c.gsize) {
         float n[3];
         normal3(S.seed, 0u, (uint32_t)a, S.n_invocations, n);
@@ -971,8 +1029,8 @@ __device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& arg
     c.pbase = S.n_germ + S.n_deriv;
     c.rank = rank; c.G = G;
     c.gtid = rank * NT + c.tid; c.gsize = G * NT;
-    c.bar = reinterpret_cast<unsigned*>(S.iarena + S.bar);
-    c.gen = __hip_atomic_load(&c.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    c.bar = reinterpret_cast<unsigned long long*>(S.iarena + S.bar);  // 16-byte aligned words
+    c.bar_flag = reinterpret_cast<unsigned*>(S.iarena + S.bar + 2);
     c.tred = S.arena + S.tred;
     c.tpar = 0;
     if (args.n_round == 0) {
@@ -983,9 +1041,9 @@ __device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& arg
         thermostat(c, S);
         return;
     }
-    float* pos = c.A + S.pos_out;
-    const float* der = c.A + S.pos_sens;
-    float* mom = c.A + S.mom;
+    auto pos = gmem(c.A + S.pos_out);
+    auto der = gmem(c.A + S.pos_sens);
+    auto mom = gmem(c.A + S.mom);
     int nf = 0;
     for (int rr = 0; rr < args.n_round; ++rr) {
         int round = args.first_round + rr;

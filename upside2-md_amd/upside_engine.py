@@ -61,6 +61,7 @@ def load_upside_library(path=LIB_PATH):
         "upside_batch_energies": (i, [vp, vp]),
         "upside_batch_replica_exchange": (i, [vp, i, vp, ct.c_uint64, vp]),
         "upside_batch_bp_stats": (i, [vp, i, vp, vp, vp, vp]),
+        "upside_batch_potential_terms": (i, [vp, i, vp, i, ct.c_char_p, i]),
         "upside_batch_last_run_ms": (f, [vp]),
         "upside_batch_enable_profile": (i, [vp, i]),
         "upside_batch_max_team": (i, [vp]),
@@ -237,6 +238,15 @@ class Batch:
         if self.calc.upside_batch_bp_stats(self.h, s, ct.byref(last), ct.byref(tot), ct.byref(sol), ct.byref(bad)):
             raise RuntimeError("bp stats failed")
         return dict(last=last.value, total=tot.value, solves=sol.value, bad=bad.value)
+
+    def potential_terms(self, s):
+        """{node name: potential} of system s at the last evaluation."""
+        vals = np.zeros(256, "f4")
+        names = ct.create_string_buffer(1 << 14)
+        n = self.calc.upside_batch_potential_terms(self.h, s, vals.ctypes.data, len(vals), names, len(names))
+        if n < 0:
+            raise RuntimeError("potential terms failed")
+        return dict(zip(names.value.decode().split("\n")[:n], vals[:n].tolist()))
 
     def last_run_ms(self):
         return float(self.calc.upside_batch_last_run_ms(self.h))

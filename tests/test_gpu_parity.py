@@ -311,3 +311,41 @@ def test_team_execution_matches_single_workgroup(ue, chain_file, team):
     for s in range(2):
         np.testing.assert_array_equal(bt.pos(s), b1.pos(s))
     np.testing.assert_allclose(bt.energies(), b1.energies(), rtol=1e-5)
+
+
+def test_team_evaluations_are_fresh(ue, chain_file):
+    """Every evaluation launch of a team recomputes every term, whatever ran
+    before it: right after a trajectory launch, after positions set by the
+    host, and with the team size switched in between.  Compared per term
+    against single-workgroup evaluations of the same positions.  (A barrier
+    that took its generation from a value read at launch start once let the
+    first op run of a launch go stale.)"""
+    path = chain_file(120, 71)
+    b1 = ue.Batch([path] * 2, [0.85, 0.9], seed=6)
+    b1.run(1)
+    want_e = b1.energies()
+    want_t = [b1.potential_terms(s) for s in range(2)]
+    team = min(16, ue.Batch([path] * 2, [0.85, 0.9], seed=6).max_team())
+    if team < 2:
+        pytest.skip("no team of two workgroups fits")
+
+    def check(b, tag):
+        say("case: %s, team %d" % (tag, b.team()))
+        np.testing.assert_allclose(b.energies(), want_e, rtol=1e-5, err_msg=tag)
+        for s in range(2):
+            got = b.potential_terms(s)
+            for k, v in want_t[s].items():
+                assert abs(got[k] - v) <= 1e-4 * max(1.0, abs(v)), (tag, s, k, got[k], v)
+
+    bt = ue.Batch([path] * 2, [0.85, 0.9], seed=6)
+    bt.set_team(team)
+    bt.run(1)
+    check(bt, "after a trajectory launch")
+    check(bt, "second evaluation")
+    bt.set_team(2)
+    check(bt, "team size switched")
+    bf = ue.Batch([path] * 2, [0.85, 0.9], seed=6)
+    bf.set_team(team)
+    for s in range(2):
+        bf.set_pos(s, b1.pos(s))
+    check(bf, "fresh batch, positions from the host")

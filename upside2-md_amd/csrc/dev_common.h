@@ -42,6 +42,7 @@ struct Ctx {
     int gtid, gsize;   // thread in the team, team threads
     unsigned long long* bar;  // team barrier arrival counter
     unsigned* bar_flag;       // team barrier spin-timeout flag
+    int local;                // the whole team runs on one XCD (shares one L2)
     float* tred;       // team reduction scratch, 2 buffers of G floats
     int tpar;          // buffer parity of the next team reduction
 };
@@ -53,24 +54,36 @@ __device__ __forceinline__ void pacc(const Ctx& c, int slot, unsigned long long 
 
 // Team barrier on a monotonic 64-bit arrival counter: arrival n of a team of
 // G falls in barrier n / G, which opens when the counter reaches the next
-// multiple of G.  Arrivals are agent-scope releases of the workgroup's writes,
-// the wait an agent-scope acquire.  No generation is read at launch and
-// nothing is reset between barriers, so a stale read can only prolong a wait.
-// The counter is a multiple of G between launches (the host zeroes it when the
-// team size changes).  The wait is bounded: after ~1 s of spinning the
-// workgroup records a timeout and proceeds, so a launch always drains; the
-// host then reports the run as failed.
+// multiple of G.  No generation is read at launch and nothing is reset
+// between barriers, so a stale read can only prolong a wait.  The counter is a
+// multiple of G between launches (the host zeroes it when the team size
+// changes).
+//   Memory ordering: a team spread over XCDs (separate L2s) needs
+// device-scope fences (write back L2 before arriving, invalidate after the
+// wait).  A team on one XCD shares its L2: every wave waits for its stores to
+// be acknowledged by L2 before the workgroup barrier (no L2 write-back), and
+// after the wait a device-scope invalidate drops the CU's L1.  (A
+// workgroup-scope invalidate does not reach L1 outside thread-group-split
+// mode.)
+//   The wait is bounded: after ~1 s of spinning the workgroup records a
+// timeout and proceeds, so a launch always drains; the host then reports the
+// run as failed.
 constexpr unsigned BAR_SPIN_LIMIT = 1u << 24;
 
+// s_waitcnt immediate: vmcnt(0), expcnt and lgkmcnt not waited on.
+constexpr int WAIT_VMEM = 0x0F70;
+
 __device__ void team_sync(Ctx& c) {
+    // every wave's stores acknowledged by L2 before the team can observe the arrival
+    if (c.G > 1 && c.local) __builtin_amdgcn_s_waitcnt(WAIT_VMEM);
     __syncthreads();
     if (c.G > 1) {
         if (c.tid == 0) {
-            __threadfence();
-            unsigned long long old = __hip_atomic_fetch_add(c.bar, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (!c.local) __threadfence();
+            unsigned long long old = __hip_atomic_fetch_add(c.bar, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             unsigned long long target = (old / (unsigned long long)c.G + 1ull) * (unsigned long long)c.G;
             unsigned spins = 0;
-            while (__hip_atomic_load(c.bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            while (__hip_atomic_load(c.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
                 __builtin_amdgcn_s_sleep(2);
                 // after one timeout anywhere in the team no barrier waits again,
                 // so the launch drains quickly
@@ -80,10 +93,29 @@ __device__ void team_sync(Ctx& c) {
                     break;
                 }
             }
-            __threadfence();
+            if (c.local) asm volatile("buffer_inv sc1" ::: "memory");  // L1 (no L2 write-back)
+            else __threadfence();
         }
         __syncthreads();
     }
+}
+
+// XCD of the calling workgroup (hardware register XCC_ID).
+__device__ __forceinline__ unsigned xcd_id() { return __builtin_amdgcn_s_getreg((15 << 11) | (0 << 6) | 20); }
+
+// At launch start of a team: every workgroup records its XCD; the team uses
+// L2-local barriers only if all agree.  All threads of the team.
+__device__ void team_place(Ctx& c, int* xtab) {
+    c.local = 0;
+    if (c.G == 1) return;
+    if (c.tid == 0) __hip_atomic_store(&xtab[c.rank], (int)xcd_id(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    team_sync(c);
+    int same = 1;
+    int x0 = __hip_atomic_load(&xtab[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int r = 1; r < c.G; ++r) same &= __hip_atomic_load(&xtab[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == x0;
+    team_sync(c);  // every workgroup has read the table before it is rewritten
+    c.local = same;
+    if (c.rank == 0 && c.tid == 0) xtab[-1] = same;  // reported to the host (barrier word 3)
 }
 
 __device__ __forceinline__ float wave_sum(float v) {

@@ -155,7 +155,8 @@ struct SysDesc {
     uint64_t n_invocations;
     int error;             // int offset of error word
     int bar;               // int offset (16-byte aligned) of the team barrier words: [0..1] 64-bit
-                           // arrival counter, [2] spin-timeout flag, [3] pad
+                           // arrival counter, [2] spin-timeout flag, [3] pad, [4..4+MAX_TEAM) XCD
+                           // of each workgroup of the team
     int tred;              // float offset of team reduction scratch [2 * MAX_TEAM]
     unsigned long long* prof;  // optional per-op wall-clock accumulators (see PROF_*), or null
 };
@@ -179,9 +180,14 @@ struct RunArgs {
     float* frame_pot;      // optional [n_sys * n_frames] potentials at frames
     int n_frames;
     int team;              // workgroups per system (1: no grid barriers)
+    int n_sys;             // systems in the launch
 };
 
 constexpr int MAX_TEAM = 64;
+// Workgroups of a launch are dispatched round-robin over this many XCDs
+// (MI355X: 8 XCDs of 32 CUs, each with its own L2).  Team launches place every
+// team on one XCD (see engine_kernels.hip), which the kernel verifies.
+constexpr int N_XCD = 8;
 
 }  // namespace upside
 

@@ -1088,7 +1088,7 @@ static void load_topology(Topology& T, H5& h5, int n_atom) {
     T.pot = T.falloc(std::max(T.n_pot, 1));
     T.mom = T.falloc(n_atom * 3);
     T.err = T.ialloc(4);
-    T.bar = T.ialloc(4);
+    T.bar = T.ialloc(4 + MAX_TEAM);
     T.tred = T.falloc(2 * MAX_TEAM);
     vector<int> z;
     for (auto& r : T.sens_zero) { z.push_back(r.first); z.push_back(r.second); }
@@ -1238,6 +1238,8 @@ struct Batch {
         const Topology& T = sys[k].prog->T;
         string out;
         char line[256];
+        if (team > 1) out += string("team of ") + std::to_string(team) + " workgroups, " +
+                             (team_local ? "on one XCD (L2-local barriers)\n" : "spread over XCDs (device-scope barriers)\n");
         int ng = sys[k].prog->n_germ();
         for (int i = 0; i < n; ++i) {
             string label;
@@ -1411,12 +1413,14 @@ struct Batch {
     }
 
     int team = 1;   // workgroups per system (set_team / autotune)
+    bool team_local = false;  // the last team launch ran every team on one XCD
 
     // Largest team every system of the batch can have with all workgroups
     // resident at once (cooperative launch), capped at MAX_TEAM.
     int max_team() const {
         int blocks = upside_max_resident_blocks();
-        int t = blocks / std::max((int)sys.size(), 1);
+        int groups = ((int)sys.size() + N_XCD - 1) / N_XCD;  // team launches pad to N_XCD systems
+        int t = blocks / (N_XCD * std::max(groups, 1));
         return std::max(1, std::min(t, MAX_TEAM));
     }
 
@@ -1484,6 +1488,7 @@ struct Batch {
                 if (w[2]) fail("team barrier wait timed out: results of this run are void");
                 if (count % (unsigned long long)team)
                     fail("team barrier arrivals unbalanced at the end of a launch: results of this run are void");
+                team_local = w[3] != 0;
             }
     }
 

@@ -1019,7 +1019,13 @@ __device__ float total_potential(Ctx& c, const SysDesc& S) {
 __device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& args, float* red, int* ired,
                                             float* lds) {
     const int G = args.team;
-    const int sys = blockIdx.x / G, rank = blockIdx.x % G;
+    int sys = blockIdx.x, rank = 0;
+    if (G > 1) {  // team launches: workgroup b runs on XCD b % N_XCD; a team stays on one XCD
+        const int xcd = blockIdx.x % N_XCD, slot = blockIdx.x / N_XCD;
+        sys = (slot / G) * N_XCD + xcd;
+        rank = slot % G;
+        if (sys >= args.n_sys) return;  // padding workgroup
+    }
     SysDesc& S = systems[sys];
     Ctx c;
     c.A = S.arena; c.I = S.iarena; c.CF = S.cf; c.CI = S.ci;
@@ -1033,6 +1039,7 @@ __device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& arg
     c.bar_flag = reinterpret_cast<unsigned*>(S.iarena + S.bar + 2);
     c.tred = S.arena + S.tred;
     c.tpar = 0;
+    team_place(c, S.iarena + S.bar + 4);
     if (args.n_round == 0) {
         compute(c, S, args.mode);
         return;
@@ -1105,13 +1112,16 @@ hipError_t upside_launch_engine(upside::SysDesc* d_systems, int n_sys, upside::R
     const void* fn = args.n_round > 0 ? (const void*)upside::engine_step_kernel
                                       : (const void*)upside::engine_eval_kernel;
     if (args.team <= 1) args.team = 1;
+    args.n_sys = n_sys;
     void* kargs[] = {&d_systems, &args};
     if (args.team == 1)
         return hipLaunchKernel(fn, dim3(n_sys), dim3(upside::NT), kargs, lds_bytes, stream);
     // teams wait for each other: every workgroup must be resident at once,
-    // which a cooperative launch guarantees (or refuses)
-    return hipLaunchCooperativeKernel(fn, dim3(n_sys * args.team), dim3(upside::NT), kargs, (unsigned)lds_bytes,
-                                      stream);
+    // which a cooperative launch guarantees (or refuses).  The grid covers
+    // N_XCD slots per team rank; slots past the last system exit at once.
+    const int groups = (n_sys + upside::N_XCD - 1) / upside::N_XCD;
+    return hipLaunchCooperativeKernel(fn, dim3(upside::N_XCD * args.team * groups), dim3(upside::NT), kargs,
+                                      (unsigned)lds_bytes, stream);
 }
 
 // Workgroups of the engine kernels the device can hold at once (the smaller

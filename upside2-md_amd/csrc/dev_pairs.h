@@ -744,7 +744,8 @@ __device__ void pg_gather_t(Ctx& c, const PairGraph& G) {
             for (int q = 0; q < D1N; ++q) dst[q] += acc[q];
         }
     }
-    team_sync(c);
+    // both sides in one phase unless they accumulate into shared producer rows
+    if (G.sym || G.gather_split) team_sync(c);
     if (G.sym) return;
     for (int base = 0; base < G.n2; base += NG) {
         int j = base + grp;

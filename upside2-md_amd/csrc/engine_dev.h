@@ -65,6 +65,7 @@ struct PairGraph {
     float drift2;          // rebuild when an element moved more than this (squared)
     int ddim1, ddim2;      // derivative widths (== dim except for backbone pairs)
     int ecap;              // slot capacity
+    int gather_split;      // side-1 and side-2 sensitivities share producer rows: gather them in two phases
     // per-system scratch (arena offsets)
     int cache_pos;         // float (n1+n2)*3 (symmetric: n1*3)
     int pos_pack;          // float (n1+n2)*3 current packed coordinates of both sides

@@ -725,6 +725,13 @@ struct Builder {
         }
         G.n1 = (int)loc1.size();
         G.n2 = (int)loc2.size();
+        {   // the two gathers may share a phase when they never touch the same producer row
+            bool same_node = sym || p1 == p2;
+            std::set<int> r1(loc1.begin(), loc1.end());
+            bool shared = false;
+            for (int j : loc2) shared = shared || r1.count(j);
+            G.gather_split = (same_node && shared) ? 1 : 0;
+        }
         G.loc1 = T.ci_push(loc1); G.loc2 = T.ci_push(loc2);
         G.type1 = T.ci_push(ty1); G.type2 = T.ci_push(ty2);
         G.id1 = T.ci_push(id1); G.id2 = T.ci_push(id2);

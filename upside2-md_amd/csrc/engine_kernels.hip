@@ -840,7 +840,7 @@ __device__ void run_value(Ctx& c, const SysDesc& S, const OpRec& op, int mode) {
     case OP_PROTEIN_HBOND: protein_hbond_fwd(c, S, op, *G); break;
     case OP_HBOND_COVERAGE: hbond_coverage_fwd(c, S, op, *G); break;
     case OP_HBOND_ENERGY: hbond_energy_fwd(c, S, op); break;
-    case OP_ROTAMER: rotamer_fwd(c, S, op, *G, S.rots[op.rot], mode); break;
+    case OP_ROTAMER: rotamer_fwd(c, S, op, *G, S.rots[op.rot], mode); return;  // ends with a team barrier
     case OP_WEIGHTED_POS: weighted_pos_fwd(c, op); break;
     case OP_ENV_COVERAGE: env_coverage_fwd(c, S, op, *G); break;
     case OP_HB_ENV_COVERAGE: hb_env_fwd(c, S, op, *G); break;
@@ -863,11 +863,12 @@ __device__ void run_deriv(Ctx& c, const SysDesc& S, const OpRec& op) {
     case OP_AFFINE: affine_bwd(c, op); break;
     case OP_PLACEMENT: placement_bwd(c, op); break;
     case OP_INFER_HO: infer_bwd(c, op); break;
-    case OP_PROTEIN_HBOND: protein_hbond_bwd(c, op, *G); break;
-    case OP_HBOND_COVERAGE: hbond_coverage_bwd(c, op, *G); break;
+    // these end with their own team barrier (the sensitivity gather)
+    case OP_PROTEIN_HBOND: protein_hbond_bwd(c, op, *G); return;
+    case OP_HBOND_COVERAGE: hbond_coverage_bwd(c, op, *G); return;
     case OP_WEIGHTED_POS: weighted_pos_bwd(c, op); break;
-    case OP_ENV_COVERAGE: env_coverage_bwd(c, op, *G); break;
-    case OP_HB_ENV_COVERAGE: hb_env_bwd(c, op, *G); break;
+    case OP_ENV_COVERAGE: env_coverage_bwd(c, op, *G); return;
+    case OP_HB_ENV_COVERAGE: hb_env_bwd(c, op, *G); return;
     default: break;  // potential nodes push in their value pass
     }
     team_sync(c);

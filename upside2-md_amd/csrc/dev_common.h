@@ -8,9 +8,6 @@
 
 namespace upside {
 
-constexpr int NT = 512;            // threads per workgroup (8 wave64)
-constexpr int NWAVE = NT / 64;
-
 // Execution context of one workgroup.  A system is evaluated by a team of G
 // workgroups (G = 1: a single workgroup, no grid barriers).  Data-parallel
 // loops stride over the team's threads (gtid, gsize); team_sync separates

@@ -227,18 +227,35 @@ __device__ void rot_fill(Ctx& c, const PairGraph& G, const RotamerDev& R) {
         int q = sw ? rb * 6 + ra : ra * 6 + rb;
         v.P[q * cap + e] = expf(-val[s]);
     }
-    // compact BP incidences of every node, in incidence order
-    for (int x = c.gtid; x < R.n_node; x += c.gsize) {
-        int k = v.ioff[x];
-        if (v.nrot[x] > 1) {
-            for (int t = v.ioff[x]; t < v.ioff[x + 1]; ++t) {
-                int cc = v.iinc[t] >> 1;
-                int e = v.eidx[cc];
-                if (v.eidx[cc + 1] == e) continue;
-                v.binc[k++] = e * 2 + (v.bpf[e] == x ? 0 : 1);
+    // compact BP incidences of every node, in incidence order: a 16-lane
+    // group per node tests 16 incidences at a time and keeps the BP ones
+    // through a ballot
+    {
+        const int lane = c.tid & 63;
+        const unsigned long long gmask = 0xFFFFull << (lane & ~(GW - 1));
+        const unsigned long long below = (1ull << lane) - 1ull;
+        const int grp = c.gtid / GW, gl = c.gtid % GW, NG = c.gsize / GW;
+        for (int base = 0; base < R.n_node; base += NG) {
+            const int x = base + grp;
+            const bool ok = x < R.n_node && v.nrot[x] > 1;
+            const int t0 = ok ? v.ioff[x] : 0, t1 = ok ? v.ioff[x + 1] : 0;
+            int k = t0;
+            for (int t = t0; t < t1; t += GW) {  // same trip count for the group's lanes
+                const int tt = t + gl;
+                bool keep = false;
+                int code = 0;
+                if (tt < t1) {
+                    int cc = v.iinc[tt] >> 1;
+                    int e = v.eidx[cc];
+                    keep = v.eidx[cc + 1] != e;
+                    code = e * 2 + (keep && v.bpf[e] == x ? 0 : 1);
+                }
+                unsigned long long m = __ballot(keep) & gmask;
+                if (keep) v.binc[k + __popcll(m & below)] = code;
+                k += __popcll(m);
             }
+            if (x < R.n_node && gl == 0) v.ninc[x] = k - t0;
         }
-        v.ninc[x] = k - v.ioff[x];
     }
     team_sync(c);
 }

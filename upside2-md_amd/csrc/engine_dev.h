@@ -17,6 +17,8 @@ namespace upside {
 // coefficient tables (the largest, the rotamer pair table, is 20*20*54 floats
 // = 86.4 KB, plus read-past padding), cell grids, and the rotamer solve state.
 constexpr int LDS_FLOATS = 40704;
+constexpr int NT = 512;            // threads per workgroup (8 wave64)
+constexpr int NWAVE = NT / 64;
 
 enum OpKind : int {
     OP_DIST3D = 1, OP_ANGLE, OP_DIHEDRAL, OP_SPRING, OP_RAMA_COORD, OP_CAT_POS,

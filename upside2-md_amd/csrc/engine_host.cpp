@@ -558,6 +558,7 @@ struct Builder {
             for (int x : aa)
                 if (x < 0 || x >= naa) fail(n.name + ": aa type out of range");
             aa.resize(mx, 0);
+            if (naa < 1 || naa * NT > LDS_FLOATS) fail(n.name + ": num_aa_types out of range");
             o.ia[0] = naa;
             o.ia[1] = T.ci_push(aa);
             alloc_coord(n, G.n1 * naa, 1);

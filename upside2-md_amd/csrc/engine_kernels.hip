@@ -684,13 +684,31 @@ __device__ void env_coverage_fwd(Ctx& c, const SysDesc& S, const OpRec& op, cons
     auto nbr = gmem(c.I + G.nbr);
     auto val = gmem(c.A + G.val);
     auto act = gmem(c.I + G.act);
-    // one thread per (element, type): scan the element's slots in order
-    for (int k = c.gtid; k < G.n1 * naa; k += c.gsize) {
-        int i = k / naa, t = k - i * naa;
-        float acc = 0.f;
-        for (int s = off[i]; s < off[i + 1]; ++s)
-            if (act[s] && aa[id2[nbr[s]]] == t) acc += val[s];
-        OUT(op)[k] = acc;
+    auto out = OUT(op);
+    // one 16-lane group per element: each lane strides the element's slots
+    // into its own type bins (LDS), then lane t of the group sums bin t over
+    // the group's lanes in lane order
+    lp<float> mine = lmem(c.lds) + c.tid * naa;
+    lp<float> grp_bins = lmem(c.lds) + (c.tid & ~(GW - 1)) * naa;
+    const int grp = c.gtid / GW, gl = c.gtid % GW, NG = c.gsize / GW;
+    for (int base = 0; base < G.n1; base += NG) {
+        const int i = base + grp;
+        const bool ok = i < G.n1;
+        for (int t = 0; t < naa; ++t) mine[t] = 0.f;
+        const int e = ok ? off[i + 1] : 0;
+        for (int s = (ok ? off[i] : 0) + gl; s < e; s += GW)
+            if (act[s]) mine[aa[id2[nbr[s]]]] += val[s];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int t = gl; t < naa; t += GW) {
+            float acc = 0.f;
+            for (int l = 0; l < GW; ++l) acc += grp_bins[l * naa + t];
+            if (ok) out[i * naa + t] = acc;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();  // bins are cleared for the next element only after every lane read them
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 __device__ void env_coverage_bwd(Ctx& c, const OpRec& op, const PairGraph& G) {

@@ -46,6 +46,88 @@ __device__ RotView rot_view(Ctx& c, const RotamerDev& R) {
     return v;
 }
 
+// Residue-pair candidate lists from the node spheres cen (LDS or memory):
+// counts, candidates (a < b, by a then b) and node incidences, each node
+// handled by a 16-lane group testing 16 partners at a time (ballot
+// compaction keeps partner order).  Returns false on capacity overflow.
+template <class P>
+__device__ bool rot_candidate_lists(Ctx& c, const PairGraph& G, const RotamerDev& R, const RotView& v, P cen) {
+    const int n = R.n_node;
+    const float rc = G.cache_cut;
+    auto near = [&](int a, int b) {
+        float dx = cen[a * 4] - cen[b * 4], dy = cen[a * 4 + 1] - cen[b * 4 + 1], dz = cen[a * 4 + 2] - cen[b * 4 + 2];
+        float lim = (rc + cen[a * 4 + 3] + cen[b * 4 + 3]) * 1.0001f + 1e-3f;
+        return dx * dx + dy * dy + dz * dz < lim * lim;
+    };
+    const int lane = c.tid & 63;
+    const unsigned long long gmask = 0xFFFFull << (lane & ~(GW - 1));
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int grp = c.gtid / GW, gl = c.gtid % GW, NG = c.gsize / GW;
+    for (int base = 0; base < n; base += NG) {
+        const int a = base + grp;
+        const int lim = a < n ? n : 0;
+        int up = 0, lo = 0;
+        for (int b0 = 0; b0 < lim; b0 += GW) {
+            const int b = b0 + gl;
+            const bool nb = b < n && b != a && near(a, b);
+            bool keep = nb && b > a;
+            up += __popcll(__ballot(keep) & gmask);
+            keep = nb && b < a;
+            lo += __popcll(__ballot(keep) & gmask);
+        }
+        if (a < n && gl == 0) {
+            v.coff[a] = up;
+            v.ioff[a] = up + lo;
+        }
+    }
+    if (c.gtid == 0) { v.coff[n] = 0; v.ioff[n] = 0; }
+    team_sync(c);
+    int C = team_exscan(c, v.coff, n + 1);
+    int I = team_exscan(c, v.ioff, n + 1);
+    if (C > R.ccap || I > 2 * R.ccap) {
+        if (c.gtid == 0) { c.I[G.state + 1] = 1; c.I[G.state + 0] = 0; }
+        team_sync(c);
+        return false;
+    }
+    for (int base = 0; base < n; base += NG) {
+        const int a = base + grp;
+        const int lim = a < n ? n : 0;
+        int k = a < n ? v.coff[a] : 0;
+        for (int b0 = a + 1; b0 < lim; b0 += GW) {
+            const int b = b0 + gl;
+            bool keep = b < n && near(a, b);
+            const unsigned long long m = __ballot(keep) & gmask;
+            if (keep) {
+                const int pos = k + __popcll(m & below);
+                v.ca[pos] = a;
+                v.cb[pos] = b;
+            }
+            k += __popcll(m);
+        }
+    }
+    team_sync(c);
+    for (int base = 0; base < n; base += NG) {
+        const int x = base + grp;
+        const int lim = x < n ? x : 0;
+        int t = x < n ? v.ioff[x] : 0;
+        for (int a0 = 0; a0 < lim; a0 += GW) {  // partners a < x, in increasing a
+            const int a = a0 + gl;
+            bool keep = a < x && near(a, x);
+            int code = 0;
+            if (keep) {
+                int c0 = v.coff[a];
+                code = (c0 + lower_bound_i(v.cb + c0, v.coff[a + 1] - c0, x)) * 2 + 1;
+            }
+            const unsigned long long m = __ballot(keep) & gmask;
+            if (keep) v.iinc[t + __popcll(m & below)] = code;
+            t += __popcll(m);
+        }
+        if (x < n)  // then its own candidates
+            for (int cc = v.coff[x] + gl; cc < v.coff[x + 1]; cc += GW) v.iinc[t + cc - v.coff[x]] = cc * 2;
+    }
+    return true;
+}
+
 // Residue-pair candidates and node incidences, rebuilt with the bead rows.
 // Candidates are the residue pairs whose bead spheres (centre = bead mean,
 // radius = farthest bead) come within the cache radius of each other: a
@@ -56,10 +138,10 @@ __device__ RotView rot_view(Ctx& c, const RotamerDev& R) {
 // capacity overflow.
 __device__ bool rot_rebuild(Ctx& c, const PairGraph& G, const RotamerDev& R) {
     RotView v = rot_view(c, R);
-    const float* pack = c.A + G.pos_pack;
-    float* cen = c.A + R.ncen;
-    const int* row = c.I + G.row;
-    const int* nbr = c.I + G.nbr;
+    gp<const float> pack = gmem((const float*)c.A + G.pos_pack);
+    gp<float> cen = gmem(c.A + R.ncen);
+    gp<const int> row = gmem((const int*)c.I + G.row);
+    gp<const int> nbr = gmem((const int*)c.I + G.nbr);
     const int n = R.n_node;
     for (int a = c.gtid; a < n; a += c.gsize) {
         int b0 = v.bead0[a], b1 = v.bead0[a + 1];
@@ -76,49 +158,16 @@ __device__ bool rot_rebuild(Ctx& c, const PairGraph& G, const RotamerDev& R) {
     }
     team_sync(c);
     // every workgroup reads the spheres from its own LDS copy when they fit
+    bool ok;
     if (4 * n <= LDS_FLOATS) {
-        for (int k = c.tid; k < 4 * n; k += NT) c.lds[k] = cen[k];
+        lp<float> L = lmem(c.lds);
+        for (int k = c.tid; k < 4 * n; k += NT) L[k] = cen[k];
         __syncthreads();
-        cen = c.lds;
+        ok = rot_candidate_lists(c, G, R, v, (lp<const float>)L);
+    } else {
+        ok = rot_candidate_lists(c, G, R, v, (gp<const float>)cen);
     }
-    const float rc = G.cache_cut;
-    auto near = [&](int a, int b) {
-        float dx = cen[a * 4] - cen[b * 4], dy = cen[a * 4 + 1] - cen[b * 4 + 1], dz = cen[a * 4 + 2] - cen[b * 4 + 2];
-        float lim = (rc + cen[a * 4 + 3] + cen[b * 4 + 3]) * 1.0001f + 1e-3f;
-        return dx * dx + dy * dy + dz * dz < lim * lim;
-    };
-    for (int a = c.gtid; a < n; a += c.gsize) {
-        int up = 0, lo = 0;
-        for (int b = 0; b < n; ++b)
-            if (b != a && near(a, b)) { if (b > a) ++up; else ++lo; }
-        v.coff[a] = up;
-        v.ioff[a] = up + lo;
-    }
-    if (c.gtid == 0) { v.coff[n] = 0; v.ioff[n] = 0; }
-    team_sync(c);
-    int C = team_exscan(c, v.coff, n + 1);
-    int I = team_exscan(c, v.ioff, n + 1);
-    if (C > R.ccap || I > 2 * R.ccap) {
-        if (c.gtid == 0) { c.I[G.state + 1] = 1; c.I[G.state + 0] = 0; }
-        team_sync(c);
-        return false;
-    }
-    for (int a = c.gtid; a < n; a += c.gsize) {
-        int k = v.coff[a];
-        for (int b = a + 1; b < n; ++b)
-            if (near(a, b)) { v.ca[k] = a; v.cb[k] = b; ++k; }
-    }
-    team_sync(c);
-    for (int x = c.gtid; x < n; x += c.gsize) {
-        int t = v.ioff[x];
-        for (int a = 0; a < x; ++a)
-            if (near(a, x)) {
-                int c0 = v.coff[a];
-                int k = lower_bound_i(v.cb + c0, v.coff[a + 1] - c0, x);
-                v.iinc[t++] = (c0 + k) * 2 + 1;
-            }
-        for (int cc = v.coff[x]; cc < v.coff[x + 1]; ++cc) v.iinc[t++] = cc * 2;
-    }
+    if (!ok) return false;
     // candidate of every bead slot
     const int E = c.I[G.state + 3];
     int bad = 0;
@@ -135,7 +184,7 @@ __device__ bool rot_rebuild(Ctx& c, const PairGraph& G, const RotamerDev& R) {
         team_sync(c);
         return false;
     }
-    if (c.gtid == 0) v.stats[5] = C;
+    if (c.gtid == 0) v.stats[5] = v.coff[n];  // candidate count
     team_sync(c);
     return true;
 }
@@ -274,41 +323,57 @@ struct BpMem {
     gp<const int> nrot;
 };
 
-// One synchronous message pass: new messages (half 1-po) from old node
-// beliefs `nold_` and old messages (half po); each new message is scaled to
-// unit L1 norm (src/rotamer.cpp:515-568).  One workgroup.
-template <class F, class I>
-__device__ void rot_edge_pass(Ctx& c, const BpMem<F, I>& m, int nbp, F nold_, int po) {
-    for (int e = c.tid; e < nbp; e += NT) {
-        int f = m.bpf[e], s = m.bps[e];
-        float p[36];
+// New messages of edges e[0..W) (W = 1 or 2): every load of the W edges is
+// issued before any of their stores, so the loads are in flight together.
+template <int W, class F, class I>
+__device__ __forceinline__ void rot_edge_w(const BpMem<F, I>& m, const int* e, F nold_, int po) {
+    float p[W][36], v1[W][6], v2[W][6];
 #pragma unroll
-        for (int q = 0; q < 36; ++q) p[q] = m.P[q * m.ps + e];
-        float v1[6], v2[6];
+    for (int w = 0; w < W; ++w) {
+        int f = m.bpf[e[w]], s = m.bps[e[w]];
+#pragma unroll
+        for (int q = 0; q < 36; ++q) p[w][q] = m.P[q * m.ps + e[w]];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            v1[r] = nold_[f * 6 + r] / (1e-10f + m.msg[(po * 12 + r) * m.ms + e]);
-            v2[r] = nold_[s * 6 + r] / (1e-10f + m.msg[(po * 12 + 6 + r) * m.ms + e]);
+            v1[w][r] = nold_[f * 6 + r] / (1e-10f + m.msg[(po * 12 + r) * m.ms + e[w]]);
+            v2[w][r] = nold_[s * 6 + r] / (1e-10f + m.msg[(po * 12 + 6 + r) * m.ms + e[w]]);
         }
+    }
+    const int h = (1 - po) * 12;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
         float m1[6] = {0, 0, 0, 0, 0, 0}, m2[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int i = 0; i < 6; ++i)
 #pragma unroll
             for (int j = 0; j < 6; ++j) {
-                m1[i] += p[i * 6 + j] * v2[j];
-                m2[j] += p[i * 6 + j] * v1[i];
+                m1[i] += p[w][i * 6 + j] * v2[w][j];
+                m2[j] += p[w][i * 6 + j] * v1[w][i];
             }
         float s1 = 0.f, s2 = 0.f;
 #pragma unroll
         for (int r = 0; r < 6; ++r) { s1 += m1[r]; s2 += m2[r]; }
         float i1 = 1.f / s1, i2 = 1.f / s2;
-        int h = (1 - po) * 12;
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            m.msg[(h + r) * m.ms + e] = m1[r] * i1;
-            m.msg[(h + 6 + r) * m.ms + e] = m2[r] * i2;
+            m.msg[(h + r) * m.ms + e[w]] = m1[r] * i1;
+            m.msg[(h + 6 + r) * m.ms + e[w]] = m2[r] * i2;
         }
     }
+}
+
+// One synchronous message pass: new messages (half 1-po) from old node
+// beliefs `nold_` and old messages (half po); each new message is scaled to
+// unit L1 norm (src/rotamer.cpp:515-568).  One workgroup; a thread takes
+// edges e and e+NT together.
+template <class F, class I>
+__device__ void rot_edge_pass(Ctx& c, const BpMem<F, I>& m, int nbp, F nold_, int po) {
+    int e = c.tid;
+    for (; e + NT < nbp; e += 2 * NT) {
+        const int pair[2] = {e, e + NT};
+        rot_edge_w<2>(m, pair, nold_, po);
+    }
+    if (e < nbp) rot_edge_w<1>(m, &e, nold_, po);
     __syncthreads();
 }
 
@@ -326,16 +391,34 @@ __device__ void rot_node_pass(Ctx& c, const BpMem<F, I>& m, int n_node, F ncur_,
 #pragma unroll
         for (int r = 0; r < 6; ++r) b[r] = m.prob[x * 6 + r];
         int t0 = m.ioff[x], n = m.ninc[x];
-        for (int t = 0; t < n; ++t) {
-            int inc = m.inc[t0 + t];
-            int e = inc >> 1;
-            int base = (pn * 12 + (inc & 1) * 6) * m.ms + e;
+        auto factor = [&](const float* a) {
             float sum = 0.f;
 #pragma unroll
-            for (int r = 0; r < 6; ++r) { b[r] *= m.msg[base + r * m.ms]; sum += b[r]; }
+            for (int r = 0; r < 6; ++r) { b[r] *= a[r]; sum += b[r]; }
             float inv = 1.f / sum;
 #pragma unroll
             for (int r = 0; r < 6; ++r) b[r] *= inv;
+        };
+        // two incidences per step: both messages are loaded before the first
+        // factor is applied (the factors stay in incidence order)
+        int t = 0;
+        for (; t + 1 < n; t += 2) {
+            int inc0 = m.inc[t0 + t], inc1 = m.inc[t0 + t + 1];
+            int base0 = (pn * 12 + (inc0 & 1) * 6) * m.ms + (inc0 >> 1);
+            int base1 = (pn * 12 + (inc1 & 1) * 6) * m.ms + (inc1 >> 1);
+            float a0[6], a1[6];
+#pragma unroll
+            for (int r = 0; r < 6; ++r) { a0[r] = m.msg[base0 + r * m.ms]; a1[r] = m.msg[base1 + r * m.ms]; }
+            factor(a0);
+            factor(a1);
+        }
+        if (t < n) {
+            int inc = m.inc[t0 + t];
+            int base = (pn * 12 + (inc & 1) * 6) * m.ms + (inc >> 1);
+            float a0[6];
+#pragma unroll
+            for (int r = 0; r < 6; ++r) a0[r] = m.msg[base + r * m.ms];
+            factor(a0);
         }
         if (swap_first) {
             float mx = 0.f;

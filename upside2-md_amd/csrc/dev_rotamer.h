@@ -257,7 +257,7 @@ __device__ void rot_fill(Ctx& c, const PairGraph& G, const RotamerDev& R) {
         // rows of P belong to the residue with fewer rotamers (:898), the lower one on ties
         bool sw = v.nrot[a] > v.nrot[b];
         int f = sw ? b : a, s2 = sw ? a : b;
-        v.bpc[e] = cc; v.bpf[e] = f; v.bps[e] = s2;
+        v.bpc[e] = f | (s2 << 16); v.bpf[e] = f; v.bps[e] = s2;
         int n1 = v.nrot[f], n2 = v.nrot[s2];
 #pragma unroll
         for (int q = 0; q < 36; ++q) v.P[q * cap + e] = ((q / 6) < n1 && (q % 6) < n2) ? 1.f : 0.f;
@@ -309,37 +309,42 @@ __device__ void rot_fill(Ctx& c, const PairGraph& G, const RotamerDev& R) {
     team_sync(c);
 }
 
-// Belief-propagation state of one solve: messages, beliefs and incidences in
-// LDS when they fit (F, I = lp<float>, lp<int>), else in memory (gp<...>).
-template <class F, class I>
+// Belief-propagation state of one solve.  Messages are updated in place (an
+// edge's new messages depend only on its own old messages and on node
+// beliefs), beliefs and incidences in LDS when they fit (F, I, J = LDS
+// pointers), else in memory.
+template <class F, class I, class J>
 struct BpMem {
-    F msg;             // component (h*12 + side*6 + r) of edge e at msg[(...)*ms + e]
+    F msg;             // component (side*6 + r) of edge e at msg[(side*6 + r)*ms + e]
     int ms;
     gp<const float> P; // edge matrices (memory), P[(i*6+j)*ps + e]
     int ps;
-    F prob;            // node probabilities [n*6]
-    I bpf, bps;        // edge endpoints (rows of P, columns of P)
-    I ioff, ninc, inc; // node incidences inc[ioff[x] + t], t < ninc[x]
+    gp<const float> prob;  // node probabilities [n*6]
+    I ends;            // edge endpoints: first | second << 16 (rows, columns of P)
+    I ioff, ninc;      // node incidences inc[ioff[x] + t], t < ninc[x]
+    J inc;             // incidence: edge*2 + side
     gp<const int> nrot;
 };
 
-// New messages of edges e[0..W) (W = 1 or 2): every load of the W edges is
-// issued before any of their stores, so the loads are in flight together.
-template <int W, class F, class I>
-__device__ __forceinline__ void rot_edge_w(const BpMem<F, I>& m, const int* e, F nold_, int po) {
+// New messages of edges e[0..W) (W = 1 or 2) from old node beliefs `nold_`
+// and the edges' old messages; each new message is scaled to unit L1 norm
+// (src/rotamer.cpp:515-568).  Every load of the W edges is issued before any
+// of their stores, so the loads are in flight together.
+template <int W, class F, class I, class J>
+__device__ __forceinline__ void rot_edge_w(const BpMem<F, I, J>& m, const int* e, F nold_) {
     float p[W][36], v1[W][6], v2[W][6];
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-        int f = m.bpf[e[w]], s = m.bps[e[w]];
+        const int ends = m.ends[e[w]];
+        const int f = ends & 0xFFFF, s = ends >> 16;
 #pragma unroll
         for (int q = 0; q < 36; ++q) p[w][q] = m.P[q * m.ps + e[w]];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            v1[w][r] = nold_[f * 6 + r] / (1e-10f + m.msg[(po * 12 + r) * m.ms + e[w]]);
-            v2[w][r] = nold_[s * 6 + r] / (1e-10f + m.msg[(po * 12 + 6 + r) * m.ms + e[w]]);
+            v1[w][r] = nold_[f * 6 + r] / (1e-10f + m.msg[r * m.ms + e[w]]);
+            v2[w][r] = nold_[s * 6 + r] / (1e-10f + m.msg[(6 + r) * m.ms + e[w]]);
         }
     }
-    const int h = (1 - po) * 12;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
         float m1[6] = {0, 0, 0, 0, 0, 0}, m2[6] = {0, 0, 0, 0, 0, 0};
@@ -356,24 +361,22 @@ __device__ __forceinline__ void rot_edge_w(const BpMem<F, I>& m, const int* e, F
         float i1 = 1.f / s1, i2 = 1.f / s2;
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            m.msg[(h + r) * m.ms + e[w]] = m1[r] * i1;
-            m.msg[(h + 6 + r) * m.ms + e[w]] = m2[r] * i2;
+            m.msg[r * m.ms + e[w]] = m1[r] * i1;
+            m.msg[(6 + r) * m.ms + e[w]] = m2[r] * i2;
         }
     }
 }
 
-// One synchronous message pass: new messages (half 1-po) from old node
-// beliefs `nold_` and old messages (half po); each new message is scaled to
-// unit L1 norm (src/rotamer.cpp:515-568).  One workgroup; a thread takes
-// edges e and e+NT together.
-template <class F, class I>
-__device__ void rot_edge_pass(Ctx& c, const BpMem<F, I>& m, int nbp, F nold_, int po) {
+// One synchronous message pass over all BP edges.  One workgroup; a thread
+// takes edges e and e+NT together.
+template <class F, class I, class J>
+__device__ void rot_edge_pass(Ctx& c, const BpMem<F, I, J>& m, int nbp, F nold_) {
     int e = c.tid;
     for (; e + NT < nbp; e += 2 * NT) {
         const int pair[2] = {e, e + NT};
-        rot_edge_w<2>(m, pair, nold_, po);
+        rot_edge_w<2>(m, pair, nold_);
     }
-    if (e < nbp) rot_edge_w<1>(m, &e, nold_, po);
+    if (e < nbp) rot_edge_w<1>(m, &e, nold_);
     __syncthreads();
 }
 
@@ -382,14 +385,14 @@ __device__ void rot_edge_pass(Ctx& c, const BpMem<F, I>& m, int nbp, F nold_, in
 // With swap_first, the product goes to `nold_` and cur restarts from prob
 // (the first, undamped pass of solve_for_marginals).  Padding states carry
 // zeros throughout.  One workgroup.
-template <class F, class I>
-__device__ void rot_node_pass(Ctx& c, const BpMem<F, I>& m, int n_node, F ncur_, F nold_, int pn, float damping,
+template <class F, class I, class J>
+__device__ void rot_node_pass(Ctx& c, const BpMem<F, I, J>& m, int n_node, F ncur_, F nold_, float damping,
                               bool swap_first) {
     for (int x = c.tid; x < n_node; x += NT) {
         if (m.nrot[x] == 1) continue;
-        float b[6];
+        float pr[6], b[6];
 #pragma unroll
-        for (int r = 0; r < 6; ++r) b[r] = m.prob[x * 6 + r];
+        for (int r = 0; r < 6; ++r) { pr[r] = m.prob[x * 6 + r]; b[r] = pr[r]; }
         int t0 = m.ioff[x], n = m.ninc[x];
         auto factor = [&](const float* a) {
             float sum = 0.f;
@@ -404,8 +407,8 @@ __device__ void rot_node_pass(Ctx& c, const BpMem<F, I>& m, int n_node, F ncur_,
         int t = 0;
         for (; t + 1 < n; t += 2) {
             int inc0 = m.inc[t0 + t], inc1 = m.inc[t0 + t + 1];
-            int base0 = (pn * 12 + (inc0 & 1) * 6) * m.ms + (inc0 >> 1);
-            int base1 = (pn * 12 + (inc1 & 1) * 6) * m.ms + (inc1 >> 1);
+            int base0 = ((inc0 & 1) * 6) * m.ms + (inc0 >> 1);
+            int base1 = ((inc1 & 1) * 6) * m.ms + (inc1 >> 1);
             float a0[6], a1[6];
 #pragma unroll
             for (int r = 0; r < 6; ++r) { a0[r] = m.msg[base0 + r * m.ms]; a1[r] = m.msg[base1 + r * m.ms]; }
@@ -414,7 +417,7 @@ __device__ void rot_node_pass(Ctx& c, const BpMem<F, I>& m, int n_node, F ncur_,
         }
         if (t < n) {
             int inc = m.inc[t0 + t];
-            int base = (pn * 12 + (inc & 1) * 6) * m.ms + (inc >> 1);
+            int base = ((inc & 1) * 6) * m.ms + (inc >> 1);
             float a0[6];
 #pragma unroll
             for (int r = 0; r < 6; ++r) a0[r] = m.msg[base + r * m.ms];
@@ -423,12 +426,12 @@ __device__ void rot_node_pass(Ctx& c, const BpMem<F, I>& m, int n_node, F ncur_,
         if (swap_first) {
             float mx = 0.f;
 #pragma unroll
-            for (int r = 0; r < 6; ++r) mx = fmaxf(mx, m.prob[x * 6 + r]);
+            for (int r = 0; r < 6; ++r) mx = fmaxf(mx, pr[r]);
             float im = 1.f / mx;
 #pragma unroll
             for (int r = 0; r < 6; ++r) {
                 nold_[x * 6 + r] = b[r];
-                ncur_[x * 6 + r] = m.prob[x * 6 + r] * im;
+                ncur_[x * 6 + r] = pr[r] * im;
             }
         } else {
             float mx = 0.f;
@@ -444,17 +447,16 @@ __device__ void rot_node_pass(Ctx& c, const BpMem<F, I>& m, int n_node, F ncur_,
 
 // Iterations, node marginals and edge marginals of one solve whose state is
 // set up in m (messages all 1, `old` = prob); returns the iteration count.
-template <class F, class I>
-__device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const BpMem<F, I>& m, int nbp, F cur, F old) {
+template <class F, class I, class J>
+__device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const BpMem<F, I, J>& m, int nbp, F cur,
+                          F old) {
     const int n = R.n_node;
     const int cap = v.cap;
     unsigned long long t0 = ptick(c);
-    int po = 0;
-    rot_edge_pass(c, m, nbp, old, po);
+    rot_edge_pass(c, m, nbp, old);
     pacc(c, PROF_BP_EDGE, t0);
-    po = 1 - po;  // new messages now live in half po
     t0 = ptick(c);
-    rot_node_pass(c, m, n, cur, old, po, 0.f, true);
+    rot_node_pass(c, m, n, cur, old, 0.f, true);
     pacc(c, PROF_BP_NODE, t0);
     float dev = 1e10f;
     int it = 0;
@@ -462,11 +464,10 @@ __device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const B
         for (int j = 0; j < R.chunk; ++j) {
             F t = cur; cur = old; old = t;          // swap node beliefs
             t0 = ptick(c);
-            rot_edge_pass(c, m, nbp, old, po);      // messages old = half po
+            rot_edge_pass(c, m, nbp, old);
             pacc(c, PROF_BP_EDGE, t0);
-            po = 1 - po;
             t0 = ptick(c);
-            rot_node_pass(c, m, n, cur, old, po, R.damping, false);
+            rot_node_pass(c, m, n, cur, old, R.damping, false);
             pacc(c, PROF_BP_NODE, t0);
         }
         float d = 0.f;
@@ -497,12 +498,13 @@ __device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const B
     // edge marginals (:425-449)
     gp<float> marg = gmem(v.marg);
     for (int e = c.tid; e < nbp; e += NT) {
-        int f = m.bpf[e], s = m.bps[e];
+        const int ends = m.ends[e];
+        const int f = ends & 0xFFFF, s = ends >> 16;
         float b1[6], b2[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            b1[r] = old[f * 6 + r] / (1e-10f + m.msg[(po * 12 + r) * m.ms + e]);
-            b2[r] = old[s * 6 + r] / (1e-10f + m.msg[(po * 12 + 6 + r) * m.ms + e]);
+            b1[r] = old[f * 6 + r] / (1e-10f + m.msg[r * m.ms + e]);
+            b2[r] = old[s * 6 + r] / (1e-10f + m.msg[(6 + r) * m.ms + e]);
         }
         float p[36];
 #pragma unroll
@@ -522,6 +524,11 @@ __device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const B
     return it;
 }
 
+// LDS floats a solve in LDS needs: messages 12 per edge, packed endpoints 1
+// per edge, incidences (16-bit, two per edge) 1 per edge, beliefs 12 per
+// node, incidence offsets and counts 2 per node.
+__device__ __forceinline__ int bp_lds_floats(int nbp, int n) { return 14 * nbp + 14 * n + 2; }
+
 // The solve proper, on one workgroup: a pass has a few thousand flops per
 // node, too little to pay for team barriers, and a single workgroup can keep
 // messages, beliefs and incidences in LDS (then every dependent load of the
@@ -532,32 +539,29 @@ __device__ int rot_solve_wg(Ctx& c, const RotamerDev& R) {
     const int n = R.n_node;
     const int cap = v.cap;
     unsigned long long t0 = ptick(c);
-    if (28 * nbp + 20 * n + 1 <= LDS_FLOATS) {
+    if (bp_lds_floats(nbp, n) <= LDS_FLOATS && 2 * nbp < 65536) {
         lp<float> L = lmem(c.lds);
-        BpMem<lp<float>, lp<int>> m;
-        m.P = gmem((const float*)v.P); m.ps = cap; m.nrot = gmem(v.nrot);
-        m.msg = L; m.ms = nbp; L += 24 * nbp;
+        BpMem<lp<float>, lp<int>, lp<unsigned short>> m;
+        m.P = (gp<const float>)v.P; m.ps = cap; m.nrot = v.nrot; m.prob = (gp<const float>)v.nprob;
+        m.msg = L; m.ms = nbp; L += 12 * nbp;
         lp<float> cur = L; L += 6 * n;
         lp<float> old = L; L += 6 * n;
-        lp<float> pr = L; L += 6 * n;
         lp<int> Ii = (lp<int>)L;
-        lp<int> bpf = Ii; Ii += nbp;
-        lp<int> bps = Ii; Ii += nbp;
+        lp<int> ends = Ii; Ii += nbp;
         lp<int> ioff = Ii; Ii += n + 1;
         lp<int> ninc = Ii; Ii += n;
-        lp<int> inc = Ii;  // 2*nbp: every BP edge has two incidences
-        gp<const float> nprob = gmem((const float*)v.nprob);
-        for (int k = c.tid; k < 6 * n; k += NT) { float q = nprob[k]; pr[k] = q; old[k] = q; }
-        for (int e = c.tid; e < nbp; e += NT) { bpf[e] = v.bpf[e]; bps[e] = v.bps[e]; }
+        lp<unsigned short> inc = (lp<unsigned short>)Ii;  // 2*nbp entries
+        for (int k = c.tid; k < 6 * n; k += NT) old[k] = v.nprob[k];
+        for (int e = c.tid; e < nbp; e += NT) ends[e] = v.bpc[e];
         for (int x = c.tid; x < n; x += NT) { int k = v.ninc[x]; ninc[x] = k; ioff[x] = k; }
         if (c.tid == 0) ioff[n] = 0;
         __syncthreads();
-        block_exscan(c, (int*)ioff, n + 1);
+        block_exscan(c, ioff, n + 1);
         for (int x = c.tid; x < n; x += NT) {
             int s0 = v.ioff[x], d0 = ioff[x], k = ninc[x];
-            for (int t = 0; t < k; ++t) inc[d0 + t] = v.binc[s0 + t];
+            for (int t = 0; t < k; ++t) inc[d0 + t] = (unsigned short)v.binc[s0 + t];
         }
-        m.prob = pr; m.bpf = bpf; m.bps = bps; m.ioff = ioff; m.ninc = ninc; m.inc = inc;
+        m.ends = ends; m.ioff = ioff; m.ninc = ninc; m.inc = inc;
         // initial messages = 1, old beliefs = prob (:1110-1133)
         for (int e = c.tid; e < nbp; e += NT)
 #pragma unroll
@@ -566,18 +570,17 @@ __device__ int rot_solve_wg(Ctx& c, const RotamerDev& R) {
         pacc(c, PROF_BP_SETUP, t0);
         return bp_iterate(c, R, v, m, nbp, cur, old);
     }
-    BpMem<gp<float>, gp<int>> m;
-    m.P = gmem((const float*)v.P); m.ps = cap; m.nrot = gmem(v.nrot);
-    m.msg = gmem(v.msg); m.ms = cap;
-    m.prob = gmem(v.nprob); m.bpf = gmem(v.bpf); m.bps = gmem(v.bps);
-    m.ioff = gmem(v.ioff); m.ninc = gmem(v.ninc); m.inc = gmem(v.binc);
+    BpMem<gp<float>, gp<int>, gp<int>> m;
+    m.P = (gp<const float>)v.P; m.ps = cap; m.nrot = v.nrot; m.prob = (gp<const float>)v.nprob;
+    m.msg = v.msg; m.ms = cap;
+    m.ends = v.bpc; m.ioff = v.ioff; m.ninc = v.ninc; m.inc = v.binc;
     for (int i = c.tid; i < n * 6; i += NT) v.nold[i] = v.nprob[i];
     for (int e = c.tid; e < nbp; e += NT)
 #pragma unroll
         for (int r = 0; r < 12; ++r) m.msg[r * m.ms + e] = 1.f;
     __syncthreads();
     pacc(c, PROF_BP_SETUP, t0);
-    return bp_iterate(c, R, v, m, nbp, gmem(v.ncur), gmem(v.nold));
+    return bp_iterate(c, R, v, m, nbp, v.ncur, v.nold);
 }
 
 // Solve by rank 0 of the team while the others wait at the barrier.

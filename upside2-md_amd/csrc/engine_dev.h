@@ -105,11 +105,12 @@ struct RotamerDev {
     int ioff;              // int n_node+1   incidence offsets
     int iinc;              // int 2*ccap     incidences: candidate*2 + (node is b)
     int eidx;              // int ccap+1     BP edge of a candidate (exclusive scan of presence)
-    int bpc, bpf, bps;     // int ccap       BP edge: candidate, first node (fewer rotamers), second node
+    int bpc, bpf, bps;     // int ccap       BP edge: packed endpoints (first | second << 16), first node
+                           //                (fewer rotamers), second node
     int ninc;              // int n_node     BP incidences of a node
     int binc;              // int 2*ccap     BP incidences (edge*2 + side), stored from ioff[x]
     int P;                 // float 36*ccap  edge probability matrix, SoA over BP edges
-    int msg;               // float 24*ccap  [old m1(6) m2(6) | new m1(6) m2(6)], SoA
+    int msg;               // float 12*ccap  [m1(6) m2(6)] per BP edge, SoA, updated in place
     int marg;              // float 36*ccap  edge marginals, SoA
     int e1x;               // float n_bead   energy of edges to 1-rotamer residues
     int ncen;              // float n_node*4 bead-sphere centre and radius of each residue

@@ -973,6 +973,7 @@ struct Builder {
         R.chunk = h5.attr_i(g, "iteration_chunk_size", 0, true);
         if (R.chunk < 1) fail(n.name + ": iteration_chunk_size must be positive");
         int nn = R.n_node;
+        if (nn >= 65536) fail(n.name + ": too many residue nodes (BP edge endpoints are packed in 16 bits)");
         R.ccap = (int)std::max(16.0, nn * 64.0 * T.cap_scale);
         long cc = R.ccap;
         R.coff = T.ialloc(nn + 1);
@@ -988,7 +989,7 @@ struct Builder {
         R.ninc = T.ialloc(nn);
         R.binc = T.ialloc(2 * cc);
         R.P = T.falloc(cc * 36);
-        R.msg = T.falloc(cc * 24);
+        R.msg = T.falloc(cc * 12);
         R.marg = T.falloc(cc * 36);
         R.e1x = T.falloc(nb);
         R.ncen = T.falloc(nn * 4);

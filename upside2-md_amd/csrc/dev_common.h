@@ -46,7 +46,9 @@ struct Ctx {
 
 __device__ __forceinline__ unsigned long long ptick(const Ctx& c) { return c.prof ? wall_clock64() : 0ull; }
 __device__ __forceinline__ void pacc(const Ctx& c, int slot, unsigned long long t0) {
-    if (c.prof && c.gtid == 0) c.prof[c.pbase + slot] += wall_clock64() - t0;
+    // a no-return atomic: the thread does not wait for the accumulator's line
+    if (c.prof && c.gtid == 0)
+        __hip_atomic_fetch_add(c.prof + c.pbase + slot, wall_clock64() - t0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Team barrier on a monotonic 64-bit arrival counter: arrival n of a team of

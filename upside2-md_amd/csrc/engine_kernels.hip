@@ -21,7 +21,8 @@ namespace upside {
 
 // Profile accumulation (op slots by index, fixed slots through pacc).
 __device__ __forceinline__ void prof_op(const Ctx& c, int slot, unsigned long long t0) {
-    if (c.prof && c.gtid == 0) c.prof[slot] += wall_clock64() - t0;
+    if (c.prof && c.gtid == 0)
+        __hip_atomic_fetch_add(c.prof + slot, wall_clock64() - t0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ void put_pot(Ctx& c, const SysDesc& S, const OpRec& op, float e) {

@@ -175,6 +175,59 @@ def traffic_per_step():
     return float(m.group(1)) if m else None
 
 
+# ------------------------------------------------------ batched workloads
+def batched_workloads(work, rounds=60):
+    """Two batch workloads reported beside the single-chain line (not `value`):
+    16 replicas of one 150-residue chain at temperatures geometric over
+    0.80-0.95 with alternating neighbour swap sets every 7 rounds (21 steps),
+    and 64 independent chains with lengths uniform in [50, 400].  Each is
+    autotuned, warmed up, then timed over `rounds` rounds."""
+    import upside_engine
+    out = []
+
+    def timed(b, n_res_total, segment=None, sets=None):
+        cand = [t for t in (1, 2, 4, 8, 16) if t <= b.max_team()]
+        b.run(3)
+        b.autotune(3, cand)
+        hip_sync()
+        t0 = time.perf_counter()
+        done, k = 0, 0
+        while done < rounds:
+            n = min(segment or rounds, rounds - done)
+            b.run(n)
+            done += n
+            if sets:
+                b.replica_exchange(sets[k % 2], done)
+                k += 1
+        hip_sync()
+        wall = time.perf_counter() - t0
+        return n_res_total * 3 * rounds / wall, wall, b.team()
+
+    p = os.path.join(work, "rex150.up")
+    upconfig.make_chain(p, 150, seed=2)
+    temps = [0.80 * (0.95 / 0.80) ** (i / 15.0) for i in range(16)]
+    b = upside_engine.Batch([p] * 16, temps, seed=2, dt=DT)
+    sets = [[[i, i + 1] for i in range(0, 15, 2)], [[i, i + 1] for i in range(1, 14, 2)]]
+    v, wall, team = timed(b, 16 * 150, segment=7, sets=sets)
+    out.append({"workload": "16 replicas x 150 residues, T geometric 0.80-0.95, neighbour swaps every 21 steps",
+                "value": v, "unit": "residue-steps/s", "steps": 3 * rounds, "seconds": wall,
+                "workgroups_per_system": team})
+    del b
+    rng = np.random.default_rng(3)
+    lens = [int(x) for x in rng.integers(50, 401, 64)]
+    paths = []
+    for i, L in enumerate(lens):
+        q = os.path.join(work, "uneven%02d.up" % i)
+        upconfig.make_chain(q, L, seed=100 + i)
+        paths.append(q)
+    b = upside_engine.Batch(paths, [TEMP] * 64, seed=3, dt=DT)
+    v, wall, team = timed(b, sum(lens))
+    out.append({"workload": "64 independent chains, lengths uniform in [50, 400] (%d residues)" % sum(lens),
+                "value": v, "unit": "residue-steps/s", "steps": 3 * rounds, "seconds": wall,
+                "workgroups_per_system": team})
+    return out
+
+
 # ----------------------------------------------------------------- main
 def main():
     ap = argparse.ArgumentParser()
@@ -183,6 +236,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--cpu-rounds", type=int, default=400, help="rounds of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the two batch workloads")
     ap.add_argument("--team", type=int, default=0,
                     help="workgroups per system (0: time 1..32 on the warmed state and keep the fastest)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -280,6 +334,8 @@ def main():
     }
     if cpu:
         line["speedup_vs_cpu_baseline"] = value / cpu["value"]
+    if not args.no_batched:
+        line["batched"] = batched_workloads(work)
     shutil.rmtree(work, ignore_errors=True)
     print(json.dumps(line), flush=True)
 

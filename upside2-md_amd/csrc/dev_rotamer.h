@@ -400,7 +400,7 @@ __device__ void rot_node_pass(Ctx& c, const BpMem<F, I, J>& m, int n_node, F ncu
             float sum = 0.f;
 #pragma unroll
             for (int r = 0; r < 6; ++r) { b[r] *= a[r]; sum += b[r]; }
-            float inv = 1.f / sum;
+            float inv = __builtin_amdgcn_rcpf(sum);  // renormalisation only: 1 ulp is enough
 #pragma unroll
             for (int r = 0; r < 6; ++r) b[r] *= inv;
         };

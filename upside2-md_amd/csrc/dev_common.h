@@ -40,6 +40,9 @@ struct Ctx {
     unsigned long long* bar;  // team barrier arrival counter
     unsigned* bar_flag;       // team barrier spin-timeout flag
     int local;                // the whole team runs on one XCD (shares one L2)
+    unsigned long long* bar2; // arrival counter of the sub-team of ranks 1..G-1
+    float* tred2;             // the sub-team's reduction scratch
+    int tpar2;                // the sub-team's next reduction buffer parity
     float* tred;       // team reduction scratch, 2 buffers of G floats
     int tpar;          // buffer parity of the next team reduction
 };
@@ -97,6 +100,21 @@ __device__ void team_sync(Ctx& c) {
         }
         __syncthreads();
     }
+}
+
+// The sub-team of ranks 1..G-1 (G >= 2), with its own barrier counter and
+// reduction scratch, for work that runs beside rank 0.
+__device__ __forceinline__ Ctx sub_team(const Ctx& c) {
+    Ctx s = c;
+    s.G = c.G - 1;
+    s.rank = c.rank - 1;
+    s.gtid = s.rank * NT + c.tid;
+    s.gsize = s.G * NT;
+    s.bar = c.bar2;
+    s.tred = c.tred2;
+    s.tpar = c.tpar2;
+    s.prof = nullptr;
+    return s;
 }
 
 // XCD of the calling workgroup (hardware register XCC_ID).

@@ -135,6 +135,7 @@ struct OpRec {
     int graph;             // index into PairGraph table (-1 if none)
     int rot;               // index into RotamerDev table (-1 if none)
     int solo;              // run by one workgroup of the team (small elementwise op)
+    int overlap_end;       // rotamer: germ ops [this+1, overlap_end) run beside the BP solve (0: none)
 };
 
 struct SysDesc {
@@ -159,8 +160,9 @@ struct SysDesc {
     uint64_t n_invocations;
     int error;             // int offset of error word
     int bar;               // int offset (16-byte aligned) of the team barrier words: [0..1] 64-bit
-                           // arrival counter, [2] spin-timeout flag, [3] pad, [4..4+MAX_TEAM) XCD
-                           // of each workgroup of the team
+                           // arrival counter, [2] spin-timeout flag, [3] placement report, [4..4+MAX_TEAM)
+                           // XCD of each workgroup, [4+MAX_TEAM..+2) 64-bit counter of the sub-team
+                           // (ranks 1..G-1) that runs beside the rotamer solve
     int tred;              // float offset of team reduction scratch [2 * MAX_TEAM]
     unsigned long long* prof;  // optional per-op wall-clock accumulators (see PROF_*), or null
 };
@@ -188,6 +190,7 @@ struct RunArgs {
 };
 
 constexpr int MAX_TEAM = 64;
+constexpr int BAR_WORDS = 4 + MAX_TEAM + 2;
 // Workgroups of a launch are dispatched round-robin over this many XCDs
 // (MI355X: 8 XCDs of 32 CUs, each with its own L2).  Team launches place every
 // team on one XCD (see engine_kernels.hip), which the kernel verifies.

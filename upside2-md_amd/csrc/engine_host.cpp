@@ -1097,8 +1097,8 @@ static void load_topology(Topology& T, H5& h5, int n_atom) {
     T.pot = T.falloc(std::max(T.n_pot, 1));
     T.mom = T.falloc(n_atom * 3);
     T.err = T.ialloc(4);
-    T.bar = T.ialloc(4 + MAX_TEAM);
-    T.tred = T.falloc(2 * MAX_TEAM);
+    T.bar = T.ialloc(BAR_WORDS);
+    T.tred = T.falloc(4 * MAX_TEAM);  // the team's two buffers, then the sub-team's
     vector<int> z;
     for (auto& r : T.sens_zero) { z.push_back(r.first); z.push_back(r.second); }
     T.szero = T.ialloc(z.size());
@@ -1182,6 +1182,40 @@ struct Program {
         const char* so = std::getenv("UPSIDE_SOLO_OPS");
         const bool solo_ok = !(so && so[0] == '0');
         for (auto& o : prog) o.solo = (solo_ok && o.graph < 0 && o.rot < 0) ? 1 : 0;
+        // A rotamer node has no coordinate output, so the germ ops after it
+        // usually do not depend on it: those that neither descend from it nor
+        // push sensitivities into its inputs run beside its BP solve
+        // (UPSIDE_OVERLAP=0: never).
+        const char* ov = std::getenv("UPSIDE_OVERLAP");
+        const bool overlap_ok = !(ov && ov[0] == '0');
+        const int ng = (int)T.germ.size() - 1;
+        for (int k = 0; k < ng && overlap_ok; ++k) {
+            if (prog[k].kind != OP_ROTAMER) continue;
+            const int rn = T.germ[k + 1];
+            std::set<int> inputs(T.nodes[rn].parents.begin(), T.nodes[rn].parents.end());
+            auto descends = [&](int x) {  // x has the rotamer node among its ancestors
+                vector<int> stack(T.nodes[x].parents.begin(), T.nodes[x].parents.end());
+                std::set<int> seen;
+                while (!stack.empty()) {
+                    int y = stack.back();
+                    stack.pop_back();
+                    if (y == rn) return true;
+                    if (!seen.insert(y).second) continue;
+                    for (int z : T.nodes[y].parents) stack.push_back(z);
+                }
+                return false;
+            };
+            int j = k + 1;
+            for (; j < ng; ++j) {
+                const Node& nj = T.nodes[T.germ[j + 1]];
+                if (prog[j].kind == OP_ROTAMER || descends(T.germ[j + 1])) break;
+                bool pushes_into_inputs = false;
+                if (nj.potential)
+                    for (int p : nj.parents) pushes_into_inputs = pushes_into_inputs || inputs.count(p);
+                if (pushes_into_inputs) break;
+            }
+            prog[k].overlap_end = j > k + 1 ? j : 0;
+        }
         auto up = [](void** dst, const void* src, size_t bytes, const char* what) {
             hip_check(hipMalloc(dst, std::max(bytes, (size_t)16)), what);
             if (bytes) hip_check(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), what);
@@ -1438,7 +1472,7 @@ struct Batch {
                                           std::to_string(max_team()) + " for this batch)");
         if (t != team)  // the barrier counters must be multiples of the new size
             for (auto& sy : sys)
-                hip_check(hipMemset(sy.iarena + sy.prog->T.bar, 0, 4 * sizeof(int)), "barrier reset");
+                hip_check(hipMemset(sy.iarena + sy.prog->T.bar, 0, BAR_WORDS * sizeof(int)), "barrier reset");
         team = t;
     }
 
@@ -1491,11 +1525,13 @@ struct Batch {
         (void)hipEventElapsedTime(&last_ms, ev0, ev1);
         if (team > 1)
             for (auto& sy : sys) {
-                unsigned w[4];
+                unsigned w[BAR_WORDS];
                 hip_check(hipMemcpy(w, sy.iarena + sy.prog->T.bar, sizeof(w), hipMemcpyDeviceToHost), "barrier");
                 unsigned long long count = (unsigned long long)w[0] | ((unsigned long long)w[1] << 32);
+                const int s2 = 4 + MAX_TEAM;
+                unsigned long long count2 = (unsigned long long)w[s2] | ((unsigned long long)w[s2 + 1] << 32);
                 if (w[2]) fail("team barrier wait timed out: results of this run are void");
-                if (count % (unsigned long long)team)
+                if (count % (unsigned long long)team || (team > 2 && count2 % (unsigned long long)(team - 1)))
                     fail("team barrier arrivals unbalanced at the end of a launch: results of this run are void");
                 team_local = w[3] != 0;
             }

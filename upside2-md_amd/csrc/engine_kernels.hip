@@ -808,8 +808,9 @@ __device__ void backbone_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const Pa
 }
 
 // ---- rotamer (src/rotamer.cpp:840-850) ----------------------------------------------
-__device__ void rotamer_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G,
-                            const RotamerDev& R, int mode) {
+// Rotamer node, before the solve: cache check and rebuild, edge energies,
+// BP fill.  Returns false on capacity overflow (error word set, team synced).
+__device__ bool rotamer_pre(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G, const RotamerDev& R) {
     unsigned long long t0 = ptick(c);
     bool rebuild = pg_pack_and_check(c, G);
     if (rebuild) {
@@ -819,16 +820,21 @@ __device__ void rotamer_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const Pai
         pacc(c, PROF_ROT_REBUILD, t1);
     }
     pacc(c, PROF_REBUILD, t0);
-    if (!c.I[G.state + 0]) { graph_overflow(c, S); return; }
+    if (!c.I[G.state + 0]) { graph_overflow(c, S); return false; }
     t0 = ptick(c);
     pg_edges(c, G, op);
     pacc(c, PROF_PG_EDGES, t0);
     t0 = ptick(c);
     rot_fill(c, G, R);
     pacc(c, PROF_ROT_FILL, t0);
-    t0 = ptick(c);
-    int it = rot_solve(c, R);
-    pacc(c, PROF_BP, t0);
+    return true;
+}
+
+// Rotamer node, after the solve (it: iteration count, on gtid 0): statistics,
+// edge and 1-body sensitivities, their gather, free energy.  Ends with a team
+// barrier.
+__device__ void rotamer_post(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G, const RotamerDev& R,
+                             int mode, int it) {
     if (c.gtid == 0) {
         auto st = gmem(c.I + R.stats);
         st[0] = it;
@@ -840,6 +846,15 @@ __device__ void rotamer_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const Pai
     pg_gather(c, G);
     float fe = mode ? rot_free_energy(c, G, R) : 0.f;
     put_pot(c, S, op, fe);
+}
+
+__device__ void rotamer_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G,
+                            const RotamerDev& R, int mode) {
+    if (!rotamer_pre(c, S, op, G, R)) return;
+    unsigned long long t0 = ptick(c);
+    int it = rot_solve(c, R);
+    pacc(c, PROF_BP, t0);
+    rotamer_post(c, S, op, G, R, mode, it);
 }
 
 // ---- program interpreter ----------------------------------------------------------
@@ -910,22 +925,19 @@ __device__ __forceinline__ int solo_run_end(const SysDesc& S, int k, int end) {
     return k;
 }
 
-// One evaluation: value ops in germ order, then sensitivity pushes.  Ops
-// marked solo (elementwise work over at most a few thousand elements) run on
-// rank 0 of the team alone: a run of them goes back to back with workgroup
-// barriers and ends with one team barrier, since a team barrier costs more
-// than such an op.
-__device__ void compute(Ctx& c, const SysDesc& S, int mode) {
-    auto zr = gmem(c.I + S.sens_zero);
-    for (int r = 0; r < S.sens_zero_cnt; ++r) {
-        auto p = gmem(c.A + zr[2 * r]);
-        int n = zr[2 * r + 1];
-        for (int i = c.gtid; i < n; i += c.gsize) p[i] = 0.f;
-    }
-    team_sync(c);
-    for (int k = 0; k < S.n_germ;) {
-        if (c.G > 1 && S.ops[k].solo) {
-            const int k2 = solo_run_end(S, k, S.n_germ);
+// Value ops [k0, k1) in germ order on the team c.  Ops marked solo
+// (elementwise work over at most a few thousand elements) run on rank 0 of
+// the team alone: a run of them goes back to back with workgroup barriers and
+// ends with one team barrier, since a team barrier costs more than such an
+// op.  With SPLIT, a rotamer op with an overlap range runs its solve on rank 0
+// while the other ranks evaluate the following independent ops.  Returns
+// false when an op set the error word.
+template <bool SPLIT>
+__device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mode) {
+    for (int k = k0; k < k1;) {
+        const OpRec& op = S.ops[k];
+        if (c.G > 1 && op.solo) {
+            const int k2 = solo_run_end(S, k, k1);
             if (c.rank == 0) {
                 Ctx l = solo_ctx(c);
                 for (int q = k; q < k2; ++q) {
@@ -936,16 +948,51 @@ __device__ void compute(Ctx& c, const SysDesc& S, int mode) {
                 }
             }
             team_sync(c);
-            if (c.I[S.error]) return;
+            if (c.I[S.error]) return false;
             k = k2;
             continue;
         }
+        if (SPLIT && op.kind == OP_ROTAMER && c.G > 1 && op.overlap_end > k + 1 && op.overlap_end <= k1) {
+            unsigned long long t0 = ptick(c);
+            const PairGraph& G = S.graphs[op.graph];
+            const RotamerDev& R = S.rots[op.rot];
+            if (!rotamer_pre(c, S, op, G, R)) return false;
+            int it = 0;
+            if (c.rank == 0) {
+                unsigned long long t1 = ptick(c);
+                it = rot_solve_wg(c, R);
+                pacc(c, PROF_BP, t1);
+            } else {
+                Ctx s = sub_team(c);
+                run_germ_range<false>(s, S, k + 1, op.overlap_end, mode);
+                c.tpar2 = s.tpar;
+            }
+            team_sync(c);  // join; an error in the overlapped ops is read after the rotamer
+            rotamer_post(c, S, op, G, R, mode, it);
+            prof_op(c, k, t0);
+            if (c.I[S.error]) return false;
+            k = op.overlap_end;
+            continue;
+        }
         unsigned long long t0 = ptick(c);
-        run_value(c, S, S.ops[k], mode);
+        run_value(c, S, op, mode);
         prof_op(c, k, t0);
-        if (c.I[S.error]) return;  // uniform: read after the op's final barrier
+        if (c.I[S.error]) return false;  // uniform: read after the op's final barrier
         ++k;
     }
+    return true;
+}
+
+// One evaluation: value ops in germ order, then sensitivity pushes.
+__device__ void compute(Ctx& c, const SysDesc& S, int mode) {
+    const int* zr = c.I + S.sens_zero;
+    for (int r = 0; r < S.sens_zero_cnt; ++r) {
+        float* p = c.A + zr[2 * r];
+        int n = zr[2 * r + 1];
+        for (int i = c.gtid; i < n; i += c.gsize) p[i] = 0.f;
+    }
+    team_sync(c);
+    if (!run_germ_range<true>(c, S, 0, S.n_germ, mode)) return;
     const int nd = S.n_germ + S.n_deriv;
     for (int k = S.n_germ; k < nd;) {
         if (c.G > 1 && S.ops[k].solo) {
@@ -1059,6 +1106,9 @@ __device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& arg
     c.bar_flag = reinterpret_cast<unsigned*>(S.iarena + S.bar + 2);
     c.tred = S.arena + S.tred;
     c.tpar = 0;
+    c.bar2 = reinterpret_cast<unsigned long long*>(S.iarena + S.bar + 4 + MAX_TEAM);
+    c.tred2 = c.tred + 2 * MAX_TEAM;
+    c.tpar2 = 0;
     team_place(c, S.iarena + S.bar + 4);
     if (args.n_round == 0) {
         compute(c, S, args.mode);

@@ -324,61 +324,88 @@ struct BpMem {
     I ioff, ninc;      // node incidences inc[ioff[x] + t], t < ninc[x]
     J inc;             // incidence: edge*2 + side
     gp<const int> nrot;
+    int c1, c2;        // edges [0,c1) are 3x3, [c1,c2) 3x6, the rest 6x6 (unsorted: c1 = c2 = 0)
+    gp<const int> pos; // position of BP edge e in this order (null: the order of the BP edges)
 };
 
-// New messages of edges e[0..W) (W = 1 or 2) from old node beliefs `nold_`
-// and the edges' old messages; each new message is scaled to unit L1 norm
+// New messages of edges e[0..W) (W = 1 or 2) of one class A x B (the first
+// node has A <= B states; padded states of an edge never enter: their matrix
+// entries and beliefs are zero) from old node beliefs `nold_` and the edges'
+// old messages; each new message is scaled to unit L1 norm
 // (src/rotamer.cpp:515-568).  Every load of the W edges is issued before any
 // of their stores, so the loads are in flight together.
-template <int W, class F, class I, class J>
+template <int W, int A, int B, class F, class I, class J>
 __device__ __forceinline__ void rot_edge_w(const BpMem<F, I, J>& m, const int* e, F nold_) {
-    float p[W][36], v1[W][6], v2[W][6];
+    float p[W][A * B], v1[W][A], v2[W][B];
 #pragma unroll
     for (int w = 0; w < W; ++w) {
         const int ends = m.ends[e[w]];
         const int f = ends & 0xFFFF, s = ends >> 16;
 #pragma unroll
-        for (int q = 0; q < 36; ++q) p[w][q] = m.P[q * m.ps + e[w]];
-        // belief / old message by the hardware reciprocal (1 ulp) rather
-        // than a correctly rounded division: 12 per edge dominate the pass
+        for (int i = 0; i < A; ++i)
 #pragma unroll
-        for (int r = 0; r < 6; ++r) {
+            for (int j = 0; j < B; ++j) p[w][i * B + j] = m.P[(i * 6 + j) * m.ps + e[w]];
+        // belief / old message by the hardware reciprocal (1 ulp) rather
+        // than a correctly rounded division: these dominate the pass
+#pragma unroll
+        for (int r = 0; r < A; ++r)
             v1[w][r] = nold_[f * 6 + r] * __builtin_amdgcn_rcpf(1e-10f + m.msg[r * m.ms + e[w]]);
+#pragma unroll
+        for (int r = 0; r < B; ++r)
             v2[w][r] = nold_[s * 6 + r] * __builtin_amdgcn_rcpf(1e-10f + m.msg[(6 + r) * m.ms + e[w]]);
-        }
     }
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-        float m1[6] = {0, 0, 0, 0, 0, 0}, m2[6] = {0, 0, 0, 0, 0, 0};
+        float m1[A], m2[B];
 #pragma unroll
-        for (int i = 0; i < 6; ++i)
+        for (int i = 0; i < A; ++i) m1[i] = 0.f;
 #pragma unroll
-            for (int j = 0; j < 6; ++j) {
-                m1[i] += p[w][i * 6 + j] * v2[w][j];
-                m2[j] += p[w][i * 6 + j] * v1[w][i];
+        for (int j = 0; j < B; ++j) m2[j] = 0.f;
+#pragma unroll
+        for (int i = 0; i < A; ++i)
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                m1[i] += p[w][i * B + j] * v2[w][j];
+                m2[j] += p[w][i * B + j] * v1[w][i];
             }
         float s1 = 0.f, s2 = 0.f;
 #pragma unroll
-        for (int r = 0; r < 6; ++r) { s1 += m1[r]; s2 += m2[r]; }
+        for (int i = 0; i < A; ++i) s1 += m1[i];
+#pragma unroll
+        for (int j = 0; j < B; ++j) s2 += m2[j];
         float i1 = 1.f / s1, i2 = 1.f / s2;
 #pragma unroll
-        for (int r = 0; r < 6; ++r) {
-            m.msg[r * m.ms + e[w]] = m1[r] * i1;
-            m.msg[(6 + r) * m.ms + e[w]] = m2[r] * i2;
-        }
+        for (int i = 0; i < A; ++i) m.msg[i * m.ms + e[w]] = m1[i] * i1;
+#pragma unroll
+        for (int j = 0; j < B; ++j) m.msg[(6 + j) * m.ms + e[w]] = m2[j] * i2;
     }
 }
 
+template <int W, class F, class I, class J>
+__device__ __forceinline__ void rot_edge_cls(const BpMem<F, I, J>& m, int cls, const int* e, F nold_) {
+    if (cls == 0) rot_edge_w<W, 3, 3>(m, e, nold_);
+    else if (cls == 1) rot_edge_w<W, 3, 6>(m, e, nold_);
+    else rot_edge_w<W, 6, 6>(m, e, nold_);
+}
+
 // One synchronous message pass over all BP edges.  One workgroup; a thread
-// takes edges e and e+NT together.
+// takes edges e and e+NT together when they are of one class (edges sorted by
+// class keep most waves on one code path).
 template <class F, class I, class J>
 __device__ void rot_edge_pass(Ctx& c, const BpMem<F, I, J>& m, int nbp, F nold_) {
+    auto cls = [&](int k) { return k < m.c1 ? 0 : (k < m.c2 ? 1 : 2); };
     int e = c.tid;
     for (; e + NT < nbp; e += 2 * NT) {
         const int pair[2] = {e, e + NT};
-        rot_edge_w<2>(m, pair, nold_);
+        const int ca = cls(e), cb = cls(e + NT);
+        if (ca == cb) {
+            rot_edge_cls<2>(m, ca, pair, nold_);
+        } else {
+            rot_edge_cls<1>(m, ca, pair, nold_);
+            rot_edge_cls<1>(m, cb, pair + 1, nold_);
+        }
     }
-    if (e < nbp) rot_edge_w<1>(m, &e, nold_);
+    if (e < nbp) rot_edge_cls<1>(m, cls(e), &e, nold_);
     __syncthreads();
 }
 
@@ -497,20 +524,21 @@ __device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const B
         for (int r = 0; r < 6; ++r) { ncur[x * 6 + r] = b[r]; old[x * 6 + r] = b[r]; }
     }
     __syncthreads();
-    // edge marginals (:425-449)
+    // edge marginals (:425-449), written in the order of the BP edges
     gp<float> marg = gmem(v.marg);
     for (int e = c.tid; e < nbp; e += NT) {
-        const int ends = m.ends[e];
+        const int k = m.pos ? m.pos[e] : e;
+        const int ends = m.ends[k];
         const int f = ends & 0xFFFF, s = ends >> 16;
         float b1[6], b2[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            b1[r] = old[f * 6 + r] / (1e-10f + m.msg[r * m.ms + e]);
-            b2[r] = old[s * 6 + r] / (1e-10f + m.msg[(6 + r) * m.ms + e]);
+            b1[r] = old[f * 6 + r] / (1e-10f + m.msg[r * m.ms + k]);
+            b2[r] = old[s * 6 + r] / (1e-10f + m.msg[(6 + r) * m.ms + k]);
         }
         float p[36];
 #pragma unroll
-        for (int q = 0; q < 36; ++q) p[q] = m.P[q * cap + e];
+        for (int q = 0; q < 36; ++q) p[q] = m.P[q * m.ps + k];
         float sum = 0.f;
 #pragma unroll
         for (int i = 0; i < 6; ++i)
@@ -528,23 +556,27 @@ __device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const B
 
 // LDS floats a solve in LDS needs: messages 12 per edge, packed endpoints 1
 // per edge, incidences (16-bit, two per edge) 1 per edge, beliefs 12 per
-// node, incidence offsets and counts 2 per node.
+// node, incidence offsets and counts 2 per node.  The message area doubles as
+// scratch for sorting the edges by class while staging (5 per edge).
 __device__ __forceinline__ int bp_lds_floats(int nbp, int n) { return 14 * nbp + 14 * n + 2; }
 
 // The solve proper, on one workgroup: a pass has a few thousand flops per
 // node, too little to pay for team barriers, and a single workgroup can keep
 // messages, beliefs and incidences in LDS (then every dependent load of the
-// node products is an LDS read).  Returns the iteration count.
+// node products is an LDS read).  In LDS the edges are sorted by class
+// (3x3, 3x6, 6x6; stable) and their matrices copied in that order into the
+// memory message area, which the LDS solve does not use.  Returns the
+// iteration count.
 __device__ int rot_solve_wg(Ctx& c, const RotamerDev& R) {
     RotView v = rot_view(c, R);
     const int nbp = v.stats[4];
     const int n = R.n_node;
     const int cap = v.cap;
     unsigned long long t0 = ptick(c);
-    if (bp_lds_floats(nbp, n) <= LDS_FLOATS && 2 * nbp < 65536) {
+    if (bp_lds_floats(nbp, n) <= LDS_FLOATS && 2 * nbp < 65536 && 36 * nbp <= 12 * cap) {
         lp<float> L = lmem(c.lds);
         BpMem<lp<float>, lp<int>, lp<unsigned short>> m;
-        m.P = (gp<const float>)v.P; m.ps = cap; m.nrot = v.nrot; m.prob = (gp<const float>)v.nprob;
+        m.nrot = v.nrot; m.prob = (gp<const float>)v.nprob;
         m.msg = L; m.ms = nbp; L += 12 * nbp;
         lp<float> cur = L; L += 6 * n;
         lp<float> old = L; L += 6 * n;
@@ -553,16 +585,53 @@ __device__ int rot_solve_wg(Ctx& c, const RotamerDev& R) {
         lp<int> ioff = Ii; Ii += n + 1;
         lp<int> ninc = Ii; Ii += n;
         lp<unsigned short> inc = (lp<unsigned short>)Ii;  // 2*nbp entries
+        // sort scratch in the message area: class indicators [3][nbp], stashed
+        // endpoints [nbp], positions [nbp]
+        lp<int> S = (lp<int>)m.msg;
+        lp<int> stash = S + 3 * nbp;
+        lp<int> posl = S + 4 * nbp;
         for (int k = c.tid; k < 6 * n; k += NT) old[k] = v.nprob[k];
-        for (int e = c.tid; e < nbp; e += NT) ends[e] = v.bpc[e];
         for (int x = c.tid; x < n; x += NT) { int k = v.ninc[x]; ninc[x] = k; ioff[x] = k; }
         if (c.tid == 0) ioff[n] = 0;
+        for (int e = c.tid; e < nbp; e += NT) {
+            const int en = v.bpc[e];
+            const int nf = v.nrot[en & 0xFFFF], ns = v.nrot[en >> 16];
+            const int cls = nf == 6 ? 2 : (ns == 6 ? 1 : 0);
+            S[e] = cls == 0;
+            S[nbp + e] = cls == 1;
+            S[2 * nbp + e] = cls == 2;
+            stash[e] = en;
+        }
         __syncthreads();
         block_exscan(c, ioff, n + 1);
+        const int n0 = block_exscan(c, S, nbp);
+        const int n1 = block_exscan(c, S + nbp, nbp);
+        block_exscan(c, S + 2 * nbp, nbp);
+        m.c1 = n0;
+        m.c2 = n0 + n1;
+        gp<float> Ps = v.msg;  // matrices in class order, stride nbp
+        gp<int> gpos = v.bpc;  // position of every BP edge (endpoints now staged)
+        for (int e = c.tid; e < nbp; e += NT) {
+            const int en = stash[e];
+            const int nf = v.nrot[en & 0xFFFF], ns = v.nrot[en >> 16];
+            const int cls = nf == 6 ? 2 : (ns == 6 ? 1 : 0);
+            const int k = (cls == 0 ? 0 : cls == 1 ? m.c1 : m.c2) + S[cls * nbp + e];
+            ends[k] = en;
+            posl[e] = k;
+            gpos[e] = k;
+#pragma unroll 4
+            for (int q = 0; q < 36; ++q) Ps[q * nbp + k] = v.P[q * cap + e];
+        }
+        __syncthreads();
         for (int x = c.tid; x < n; x += NT) {
             int s0 = v.ioff[x], d0 = ioff[x], k = ninc[x];
-            for (int t = 0; t < k; ++t) inc[d0 + t] = (unsigned short)v.binc[s0 + t];
+            for (int t = 0; t < k; ++t) {
+                int code = v.binc[s0 + t];
+                inc[d0 + t] = (unsigned short)(posl[code >> 1] * 2 + (code & 1));
+            }
         }
+        __syncthreads();  // the scratch is overwritten by the messages next
+        m.P = (gp<const float>)Ps; m.ps = nbp; m.pos = (gp<const int>)gpos;
         m.ends = ends; m.ioff = ioff; m.ninc = ninc; m.inc = inc;
         // initial messages = 1, old beliefs = prob (:1110-1133)
         for (int e = c.tid; e < nbp; e += NT)
@@ -576,6 +645,8 @@ __device__ int rot_solve_wg(Ctx& c, const RotamerDev& R) {
     m.P = (gp<const float>)v.P; m.ps = cap; m.nrot = v.nrot; m.prob = (gp<const float>)v.nprob;
     m.msg = v.msg; m.ms = cap;
     m.ends = v.bpc; m.ioff = v.ioff; m.ninc = v.ninc; m.inc = v.binc;
+    m.c1 = m.c2 = 0;
+    m.pos = nullptr;
     for (int i = c.tid; i < n * 6; i += NT) v.nold[i] = v.nprob[i];
     for (int e = c.tid; e < nbp; e += NT)
 #pragma unroll

@@ -324,6 +324,7 @@ def main():
                    "workgroups_per_system": b.team()},
         "bp_iters_per_step": iters / max(solves, 1),
         "bp_passes_per_solve": passes / max(solves, 1),
+        "bp_edges_3x3_3x6_6x6": [bp0[3:6], bp1[3:6]],
         "force_rel_err_vs_reference_at_end": fe,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

@@ -98,23 +98,64 @@ def cpu_baseline(path, rounds, use_ref=True):
                 sample="numpy oracle restatement, %d rounds of the same chain" % n)
 
 
-def reference_force_error(path, pos, deriv_gpu):
-    """||F_gpu - F_ref|| / ||F_ref|| at the final positions, against the
-    compiled reference engine (part of the CPU checker leg)."""
+def reference_errors(path, pos, energy_gpu, deriv_gpu):
+    """Relative energy and force errors (||F_gpu - F_ref|| / ||F_ref||) at the
+    final positions against the compiled reference engine (part of the CPU
+    checker leg)."""
     lib_path = os.path.join(ROOT, "oracle", "_ref", "libupside_ref.so")
     if not os.path.exists(lib_path):
-        return None
+        return None, None
     lib = ct.CDLL(lib_path)
     lib.construct_deriv_engine.restype = ct.c_void_p
     lib.construct_deriv_engine.argtypes = [ct.c_int, ct.c_char_p, ct.c_bool]
+    lib.evaluate_energy.argtypes = [ct.c_void_p] * 3
     lib.evaluate_deriv.argtypes = [ct.c_void_p] * 3
     lib.free_deriv_engine.argtypes = [ct.c_void_p]
     h = lib.construct_deriv_engine(len(pos), path.encode(), True)
     d = np.zeros_like(pos)
+    e = np.zeros(1, np.float32)
     p = np.ascontiguousarray(pos, np.float32)
+    lib.evaluate_energy(e.ctypes.data, ct.c_void_p(h), p.ctypes.data)
     lib.evaluate_deriv(d.ctypes.data, ct.c_void_p(h), p.ctypes.data)
     lib.free_deriv_engine(ct.c_void_p(h))
-    return float(np.sqrt(((deriv_gpu - d) ** 2).sum() / (d ** 2).sum()))
+    return (abs(energy_gpu - float(e[0])) / max(abs(float(e[0])), 1.0),
+            float(np.sqrt(((deriv_gpu - d) ** 2).sum() / (d ** 2).sum())))
+
+
+def oracle_checks(path, pos, energy_gpu, deriv_gpu, probe, graph_names):
+    """At the final positions: relative energy and force errors against the
+    float64 oracle, and the XOR count of every pair graph's edge set (GPU list
+    against the float32 oracle's refined list; must be 0).  Part of the CPU
+    checker leg."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import upside_oracle
+    ora = upside_oracle.Engine(path, dtype=np.float64)
+    p64 = pos.astype(np.float64)
+    e64 = float(ora.energy(p64))
+    d64 = ora.deriv(p64)
+    ora32 = upside_oracle.Engine(path)      # edge sets are defined in float32
+    ora32.energy(np.ascontiguousarray(pos, np.float32))
+    xor = {}
+    for name in graph_names:
+        node = ora32.nodes.get(name)
+        if node is None:
+            continue
+        obj = node.ig if hasattr(node, "ig") else node
+        if not hasattr(obj, "i1"):
+            continue
+        want = set(zip(obj.i1.tolist(), obj.i2.tolist()))
+        got = set(map(tuple, probe.edge_pairs(name).tolist()))
+        xor[name] = len(got ^ want)
+    return (abs(energy_gpu - e64) / max(abs(e64), 1.0),
+            float(np.sqrt(((deriv_gpu - d64) ** 2).sum() / (d64 ** 2).sum())), xor)
+
+
+def percentile_from_hist(h, q):
+    """Smallest k with cumulative count >= q * total."""
+    c = np.cumsum(h)
+    if c[-1] == 0:
+        return None
+    return int(np.searchsorted(c, q * c[-1]))
 
 
 # ------------------------------------------------------- algorithmic bytes
@@ -287,6 +328,7 @@ def main():
     g0, bp0, nodes = graph_counts(probe, node_names)
     small = small_node_bytes(probe, node_names, set(g0))
     st0 = b.bp_stats(0)
+    h0 = b.bp_histogram(0)
 
     hip_sync()
     t0 = time.perf_counter()
@@ -296,6 +338,7 @@ def main():
     kernel_ms = b.last_run_ms()
 
     st1 = b.bp_stats(0)
+    hist = b.bp_histogram(0) - h0
     pos_end = b.pos(0)
     probe.energy(pos_end)
     g1, bp1, _ = graph_counts(probe, node_names)
@@ -310,9 +353,14 @@ def main():
     tps = traffic_per_step()
     value = N_RES * steps / wall
 
-    fe = None
+    checks = None
     if not args.no_cpu_baseline:
-        fe = reference_force_error(path, pos_end, probe.deriv(pos_end))
+        e_end, d_end = float(probe.energy(pos_end)), probe.deriv(pos_end)
+        er, fr = reference_errors(path, pos_end, e_end, d_end)
+        eo, fo, xor = oracle_checks(path, pos_end, e_end, d_end, probe, list(g1))
+        checks = {"energy_rel_err_vs_reference": er, "force_rel_err_vs_reference": fr,
+                  "energy_rel_err_vs_oracle_f64": eo, "force_rel_err_vs_oracle_f64": fo,
+                  "edge_set_xor_vs_oracle": xor}
 
     line = {
         "metric": METRIC, "value": value, "unit": "residue-steps/s", "n_gpus": 1, "steps": steps,
@@ -324,8 +372,10 @@ def main():
                    "workgroups_per_system": b.team()},
         "bp_iters_per_step": iters / max(solves, 1),
         "bp_passes_per_solve": passes / max(solves, 1),
+        "bp_iters_per_solve_p50_p99": [percentile_from_hist(hist, 0.5), percentile_from_hist(hist, 0.99)],
         "bp_edges_3x3_3x6_6x6": [bp0[3:6], bp1[3:6]],
-        "force_rel_err_vs_reference_at_end": fe,
+        "force_rel_err_vs_reference_at_end": checks["force_rel_err_vs_reference"] if checks else None,
+        "checks_at_end": checks,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (tps * steps) if tps is not None else None,

@@ -100,6 +100,13 @@ int upside_batch_potential_terms(UpsideBatch* batch, int sys, float* values, int
 /* Summed BP statistics: iterations of the last solve and since creation. */
 int upside_batch_bp_stats(UpsideBatch* batch, int sys, int* last_iters, long long* total_iters,
                           long long* solves, long long* bad_solves);
+/* Histogram of BP iterations per solve since creation, summed over the
+ * system's rotamer nodes: counts[k] solves of k iterations (the last bin
+ * collects the rest; the iteration count is the `iter` that
+ * solve_for_marginals returns, src/rotamer.cpp:1137-1161).  n_bins must be at least 256; returns the number of bins written,
+ * -1 on error.  No counterpart in the reference's C ABI (its solver keeps
+ * only n_bad_solve, src/rotamer.cpp:845-846). */
+int upside_batch_bp_histogram(UpsideBatch* batch, int sys, long long* counts, int n_bins);
 /* Synchronize and time the device (for benchmarks): elapsed ms of the last run. */
 float upside_batch_last_run_ms(UpsideBatch* batch);
 /* Workgroups cooperating on each system ("team"; 1 = one workgroup per

@@ -174,6 +174,12 @@ def test_batch_matches_single_engine(ue, chain_file):
         assert abs(e - e_b[s]) <= 1e-6 * abs(e), (s, e, e_b[s])
     # the two copies of one input file started identically but use different seeds
     assert not np.array_equal(b.pos(1), b.pos(3))
+    # the per-solve iteration histogram agrees with the running totals
+    for s in range(len(paths)):
+        st, h = b.bp_stats(s), b.bp_histogram(s)
+        assert h[-1] == 0
+        assert h.sum() == st["solves"] and (h * np.arange(len(h))).sum() == st["total"], (s, st)
+        assert h[st["last"]] >= 1
 
 
 def test_trajectory_matches_oracle_integrator(ue, chain_file):

@@ -17,6 +17,12 @@ namespace upside {
 // coefficient tables (the largest, the rotamer pair table, is 20*20*54 floats
 // = 86.4 KB, plus read-past padding), cell grids, and the rotamer solve state.
 constexpr int LDS_FLOATS = 40704;
+// Rotamer statistics words: [0] iterations of the last solve, [1] solves at
+// the iteration limit, [2] total iterations, [3] solves, [4] BP edges, [5]
+// residue-pair candidates, then a histogram of iterations per solve
+// (bin = iterations, the last bin collects the rest).
+constexpr int BP_HIST = 8;
+constexpr int BP_HIST_BINS = 256;
 constexpr int NT = 512;            // threads per workgroup (8 wave64)
 constexpr int NWAVE = NT / 64;
 

@@ -997,7 +997,7 @@ struct Builder {
         R.ncur = T.falloc(nn * 6);
         R.nold = T.falloc(nn * 6);
         R.noff = T.falloc(nn);
-        R.stats = T.ialloc(8);
+        R.stats = T.ialloc(BP_HIST + BP_HIST_BINS);
         o.ia[0] = T.ci_push(node_of);
         T.rots.push_back(R);
         o.rot = (int)T.rots.size() - 1;
@@ -2079,6 +2079,23 @@ int upside_batch_bp_stats(UpsideBatch* b, int sys, int* last_iters, long long* t
     if (bad_solves) *bad_solves = bad;
     return 0;
     ABI_CATCH(1)
+}
+
+int upside_batch_bp_histogram(UpsideBatch* b, int sys, long long* counts, int n_bins) {
+    ABI_TRY
+    auto& B = b->batch;
+    if (sys < 0 || sys >= (int)B.sys.size()) fail("bp_histogram: system out of range");
+    if (n_bins < BP_HIST_BINS) fail("bp_histogram: need " + std::to_string(BP_HIST_BINS) + " bins");
+    const Topology& T = B.sys[sys].prog->T;
+    for (int k = 0; k < n_bins; ++k) counts[k] = 0;
+    for (auto& R : T.rots) {
+        std::vector<int> h(BP_HIST_BINS);
+        hip_check(hipMemcpy(h.data(), B.sys[sys].iarena + R.stats + BP_HIST, BP_HIST_BINS * sizeof(int),
+                            hipMemcpyDeviceToHost), "stats");
+        for (int k = 0; k < BP_HIST_BINS; ++k) counts[k] += h[k];
+    }
+    return BP_HIST_BINS;
+    ABI_CATCH(-1)
 }
 
 float upside_batch_last_run_ms(UpsideBatch* b) { return b ? b->batch.last_ms : -1.f; }

@@ -841,6 +841,7 @@ __device__ void rotamer_post(Ctx& c, const SysDesc& S, const OpRec& op, const Pa
         if (it >= R.max_iter - R.chunk - 1) st[1] += 1;
         st[2] += it;
         st[3] += 1;
+        st[BP_HIST + min(it, BP_HIST_BINS - 1)] += 1;
     }
     rot_sens(c, G, R);
     pg_gather(c, G);

@@ -62,6 +62,7 @@ def load_upside_library(path=LIB_PATH):
         "upside_batch_replica_exchange": (i, [vp, i, vp, ct.c_uint64, vp]),
         "upside_batch_bp_stats": (i, [vp, i, vp, vp, vp, vp]),
         "upside_batch_potential_terms": (i, [vp, i, vp, i, ct.c_char_p, i]),
+        "upside_batch_bp_histogram": (i, [vp, i, vp, i]),
         "upside_batch_last_run_ms": (f, [vp]),
         "upside_batch_enable_profile": (i, [vp, i]),
         "upside_batch_max_team": (i, [vp]),
@@ -238,6 +239,13 @@ class Batch:
         if self.calc.upside_batch_bp_stats(self.h, s, ct.byref(last), ct.byref(tot), ct.byref(sol), ct.byref(bad)):
             raise RuntimeError("bp stats failed")
         return dict(last=last.value, total=tot.value, solves=sol.value, bad=bad.value)
+
+    def bp_histogram(self, s):
+        """counts[k] = solves of system s that took k BP iterations (last bin: the rest)."""
+        h = np.zeros(256, np.int64)
+        if self.calc.upside_batch_bp_histogram(self.h, s, h.ctypes.data, len(h)) < 0:
+            raise RuntimeError("bp histogram failed")
+        return h
 
     def potential_terms(self, s):
         """{node name: potential} of system s at the last evaluation."""

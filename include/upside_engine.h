@@ -103,10 +103,21 @@ int upside_batch_bp_stats(UpsideBatch* batch, int sys, int* last_iters, long lon
 /* Histogram of BP iterations per solve since creation, summed over the
  * system's rotamer nodes: counts[k] solves of k iterations (the last bin
  * collects the rest; the iteration count is the `iter` that
- * solve_for_marginals returns, src/rotamer.cpp:1137-1161).  n_bins must be at least 256; returns the number of bins written,
- * -1 on error.  No counterpart in the reference's C ABI (its solver keeps
+ * solve_for_marginals returns, src/rotamer.cpp:1137-1161).  n_bins must be at
+ * least 256; returns the number of bins written, -1 on error.  No counterpart in the reference's C ABI (its solver keeps
  * only n_bad_solve, src/rotamer.cpp:845-846). */
 int upside_batch_bp_histogram(UpsideBatch* batch, int sys, long long* counts, int n_bins);
+/* Checkpoint and resume (no counterpart in the reference, whose thermostat
+ * counter lives only in memory, src/thermostat.h:24-29): the whole device state
+ * of the batch (per system: thermostat seed and counter, temperature,
+ * positions, momenta, Verlet caches, pair lists and BP state) into a caller
+ * buffer of upside_batch_state_size() bytes.  Loading it into a batch made
+ * from the same .up files (any seed, any team size) continues bit for bit as
+ * the saved batch would have; a state whose system count or arena sizes do not
+ * match the batch is refused.  0 on success. */
+long long upside_batch_state_size(UpsideBatch* batch);
+int upside_batch_save_state(UpsideBatch* batch, void* buf, long long size);
+int upside_batch_load_state(UpsideBatch* batch, const void* buf, long long size);
 /* Synchronize and time the device (for benchmarks): elapsed ms of the last run. */
 float upside_batch_last_run_ms(UpsideBatch* batch);
 /* Workgroups cooperating on each system ("team"; 1 = one workgroup per

@@ -300,6 +300,36 @@ def test_capacity_regrowth_is_transparent(ue, chain_file, monkeypatch):
         np.testing.assert_array_equal(b1.pos(s), b0.pos(s))
 
 
+def test_checkpoint_resume_is_bit_identical(ue, chain_file):
+    """A saved batch state loaded into a fresh batch of the same files (other
+    seed, other team size) continues bit for bit, also through a replica
+    exchange; a state of other files is refused."""
+    path = chain_file(90, 12)
+    say("case: checkpoint and resume, 2 x L=90")
+    b1 = ue.Batch([path] * 2, [0.85, 0.9], seed=6)
+    b1.run(3)
+    state = b1.save_state()
+    b1.run(4)
+    acc1 = b1.replica_exchange([[0, 1]], round_num=7)
+    b1.run(2)
+    b2 = ue.Batch([path] * 2, [0.85, 0.9], seed=99)
+    b2.set_team(min(4, b2.max_team()))
+    b2.load_state(state)
+    b2.run(4)
+    acc2 = b2.replica_exchange([[0, 1]], round_num=7)
+    b2.run(2)
+    np.testing.assert_array_equal(acc1, acc2)
+    for s in range(2):
+        np.testing.assert_array_equal(b2.pos(s), b1.pos(s))
+        np.testing.assert_array_equal(b2.mom(s), b1.mom(s))
+    np.testing.assert_array_equal(b2.energies(), b1.energies())
+    other = ue.Batch([chain_file(40, 21)] * 2, [0.85, 0.9], seed=6)
+    with pytest.raises(RuntimeError):
+        other.load_state(state)
+    with pytest.raises(RuntimeError):
+        ue.Batch([path], [0.85], seed=6).load_state(state)
+
+
 @pytest.mark.parametrize("team", [2, 8])
 def test_team_execution_matches_single_workgroup(ue, chain_file, team):
     """A system evaluated by a team of workgroups (grid barriers between

@@ -1593,6 +1593,111 @@ struct Batch {
         pull_desc();
     }
 
+    // Checkpoint of the whole device state: a header, then per system its
+    // thermostat counter and temperature and its float and int arenas
+    // (positions, momenta, Verlet caches, pair lists, BP state).  Loading it
+    // into a batch of the same .up files resumes bit for bit: the pair-list
+    // row order depends on when each cache was last rebuilt, so positions and
+    // momenta alone would resume only to rounding.
+    struct StateHeader {
+        char magic[8];
+        int version, n_sys, thermostat_interval;
+        uint32_t base_seed;
+        double cap_scale;
+        float dt, pad2;
+    };
+    struct SysState {
+        long long nf, ni;
+        uint64_t n_invocations;
+        float temperature, mom_scale, noise_scale;
+        uint32_t seed;
+    };
+    size_t state_size() const {
+        size_t n = sizeof(StateHeader);
+        for (auto& sy : sys)
+            n += sizeof(SysState) + (size_t)std::max(sy.prog->T.nf, 4) * 4 + (size_t)std::max(sy.prog->T.ni, 4) * 4;
+        return n;
+    }
+    void save_state(char* out, size_t size) {
+        if (size < state_size()) fail("save_state: buffer of " + std::to_string(size) + " bytes, need " +
+                                      std::to_string(state_size()));
+        hip_check(hipStreamSynchronize(stream), "sync");
+        pull_desc();
+        StateHeader h;
+        std::memset(&h, 0, sizeof(h));
+        std::memcpy(h.magic, "UPSTATE1", 8);
+        h.version = 1;
+        h.n_sys = (int)sys.size();
+        h.thermostat_interval = thermostat_interval;
+        h.base_seed = base_seed;
+        h.cap_scale = cap_scale;
+        h.dt = dt;
+        std::memcpy(out, &h, sizeof(h));
+        out += sizeof(h);
+        for (size_t k = 0; k < sys.size(); ++k) {
+            const Topology& T = sys[k].prog->T;
+            SysState st;
+            std::memset(&st, 0, sizeof(st));
+            st.nf = std::max(T.nf, 4);
+            st.ni = std::max(T.ni, 4);
+            st.n_invocations = desc[k].n_invocations;
+            st.temperature = sys[k].temperature;
+            st.mom_scale = desc[k].mom_scale;
+            st.noise_scale = desc[k].noise_scale;
+            st.seed = desc[k].seed;
+            std::memcpy(out, &st, sizeof(st));
+            out += sizeof(st);
+            hip_check(hipMemcpy(out, sys[k].arena, st.nf * 4, hipMemcpyDeviceToHost), "save arena");
+            out += st.nf * 4;
+            hip_check(hipMemcpy(out, sys[k].iarena, st.ni * 4, hipMemcpyDeviceToHost), "save int arena");
+            out += st.ni * 4;
+        }
+    }
+    void load_state(const char* in, size_t size) {
+        StateHeader h;
+        if (size < sizeof(h)) fail("load_state: truncated state");
+        std::memcpy(&h, in, sizeof(h));
+        if (std::memcmp(h.magic, "UPSTATE1", 8) || h.version != 1) fail("load_state: not a saved batch state");
+        if (h.n_sys != (int)sys.size())
+            fail("load_state: state of " + std::to_string(h.n_sys) + " systems, batch of " +
+                 std::to_string(sys.size()));
+        if (h.cap_scale < cap_scale) fail("load_state: state has smaller list capacities than this batch");
+        hip_check(hipStreamSynchronize(stream), "sync");
+        while (cap_scale < h.cap_scale) grow();
+        if (size < state_size()) fail("load_state: truncated state");
+        const char* p = in + sizeof(h);
+        vector<SysState> st(sys.size());
+        for (size_t k = 0; k < sys.size(); ++k) {  // validate everything before touching the device
+            std::memcpy(&st[k], p, sizeof(SysState));
+            const Topology& T = sys[k].prog->T;
+            if (st[k].nf != std::max(T.nf, 4) || st[k].ni != std::max(T.ni, 4))
+                fail("load_state: system " + std::to_string(k) + " was saved from a different .up file");
+            p += sizeof(SysState) + st[k].nf * 4 + st[k].ni * 4;
+        }
+        p = in + sizeof(h);
+        pull_desc();
+        for (size_t k = 0; k < sys.size(); ++k) {
+            const Topology& T = sys[k].prog->T;
+            p += sizeof(SysState);
+            hip_check(hipMemcpy(sys[k].arena, p, st[k].nf * 4, hipMemcpyHostToDevice), "load arena");
+            p += st[k].nf * 4;
+            hip_check(hipMemcpy(sys[k].iarena, p, st[k].ni * 4, hipMemcpyHostToDevice), "load int arena");
+            p += st[k].ni * 4;
+            // barrier counters restart at zero (a multiple of any team size)
+            hip_check(hipMemset(sys[k].iarena + T.bar, 0, BAR_WORDS * sizeof(int)), "barrier reset");
+            desc[k].n_invocations = st[k].n_invocations;
+            desc[k].mom_scale = st[k].mom_scale;
+            desc[k].noise_scale = st[k].noise_scale;
+            desc[k].seed = st[k].seed;
+            sys[k].temperature = st[k].temperature;
+        }
+        dt = h.dt;
+        thermostat_interval = h.thermostat_interval;
+        base_seed = h.base_seed;
+        have_mom = true;
+        push_desc();
+    }
+
     // Fresh-start thermalisation (src/main.cpp:800-803): p = 0 then one
     // thermostat application with the run's parameters.
     void thermalize() {
@@ -1967,6 +2072,28 @@ int upside_batch_set_pos(UpsideBatch* b, int sys, const float* pos) { return bat
 int upside_batch_get_pos(UpsideBatch* b, int sys, float* pos) { return batch_copy(b, sys, pos, false, false); }
 int upside_batch_set_mom(UpsideBatch* b, int sys, const float* mom) { return batch_copy(b, sys, (float*)mom, true, true); }
 int upside_batch_get_mom(UpsideBatch* b, int sys, float* mom) { return batch_copy(b, sys, mom, false, true); }
+
+long long upside_batch_state_size(UpsideBatch* b) {
+    ABI_TRY
+    return (long long)b->batch.state_size();
+    ABI_CATCH(-1)
+}
+
+int upside_batch_save_state(UpsideBatch* b, void* buf, long long size) {
+    ABI_TRY
+    if (!buf || size < 0) fail("save_state: no buffer");
+    b->batch.save_state((char*)buf, (size_t)size);
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_load_state(UpsideBatch* b, const void* buf, long long size) {
+    ABI_TRY
+    if (!buf || size < 0) fail("load_state: no buffer");
+    b->batch.load_state((const char*)buf, (size_t)size);
+    return 0;
+    ABI_CATCH(1)
+}
 
 static uint64_t g_round_counter = 0;
 

@@ -63,6 +63,9 @@ def load_upside_library(path=LIB_PATH):
         "upside_batch_bp_stats": (i, [vp, i, vp, vp, vp, vp]),
         "upside_batch_potential_terms": (i, [vp, i, vp, i, ct.c_char_p, i]),
         "upside_batch_bp_histogram": (i, [vp, i, vp, i]),
+        "upside_batch_state_size": (ct.c_longlong, [vp]),
+        "upside_batch_save_state": (i, [vp, vp, ct.c_longlong]),
+        "upside_batch_load_state": (i, [vp, vp, ct.c_longlong]),
         "upside_batch_last_run_ms": (f, [vp]),
         "upside_batch_enable_profile": (i, [vp, i]),
         "upside_batch_max_team": (i, [vp]),
@@ -239,6 +242,22 @@ class Batch:
         if self.calc.upside_batch_bp_stats(self.h, s, ct.byref(last), ct.byref(tot), ct.byref(sol), ct.byref(bad)):
             raise RuntimeError("bp stats failed")
         return dict(last=last.value, total=tot.value, solves=sol.value, bad=bad.value)
+
+    def save_state(self):
+        """The whole device state of the batch as bytes (see load_state)."""
+        n = self.calc.upside_batch_state_size(self.h)
+        if n < 0:
+            raise RuntimeError("state size failed")
+        buf = ct.create_string_buffer(n)
+        if self.calc.upside_batch_save_state(self.h, buf, n):
+            raise RuntimeError("save_state failed")
+        return buf.raw
+
+    def load_state(self, state):
+        """Resume from save_state() bytes of a batch made from the same .up files."""
+        buf = ct.create_string_buffer(bytes(state), len(state))
+        if self.calc.upside_batch_load_state(self.h, buf, len(state)):
+            raise RuntimeError("load_state failed (see stderr)")
 
     def bp_histogram(self, s):
         """counts[k] = solves of system s that took k BP iterations (last bin: the rest)."""

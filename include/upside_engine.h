@@ -81,8 +81,9 @@ int upside_batch_get_pos(UpsideBatch* batch, int sys, float* pos);
 int upside_batch_get_mom(UpsideBatch* batch, int sys, float* mom);
 int upside_batch_set_mom(UpsideBatch* batch, int sys, const float* mom);
 /* Advance every system by n_round rounds (3 force evaluations each) in one
- * launch; when frame_interval>0 the potential is evaluated at rounds that are
- * multiples of it and stored in frame_pot[n_sys*n_frames] if non-NULL. */
+ * launch; when frame_interval>0 the potential is evaluated at rounds (counted
+ * since the batch was created) that are multiples of it and stored in
+ * frame_pot[n_sys*n_frames] if non-NULL. */
 int upside_batch_run(UpsideBatch* batch, int n_round, int frame_interval, float* frame_pot, int n_frames);
 /* Potential of every system at its current positions: energies[n_sys]. */
 int upside_batch_energies(UpsideBatch* batch, float* energies);
@@ -107,6 +108,23 @@ int upside_batch_bp_stats(UpsideBatch* batch, int sys, int* last_iters, long lon
  * least 256; returns the number of bins written, -1 on error.  No counterpart in the reference's C ABI (its solver keeps
  * only n_bad_solve, src/rotamer.cpp:845-846). */
 int upside_batch_bp_histogram(UpsideBatch* batch, int sys, long long* counts, int n_bins);
+/* /output logging as the reference's simulation loop does at its basic log
+ * level (src/main.cpp:705-736, :807-831, :893, :946-949; datasets as
+ * src/state_logger.h:21-39 creates them): per system, an `output` group with
+ * extendable datasets pos (n_frame,1,n_atom,3) f4, mom (same, if
+ * record_momentum), kinetic (n_frame,1) f8, potential (n_frame,1) f8, time
+ * (n_frame,) f8 and temperature (n_frame,1) f8, chunked by 100 frames.
+ * output_paths NULL writes into each system's input .up (replacing an existing
+ * /output, as the reference does without -o; systems must not share an input
+ * file), else output_paths[s] is created afresh (as with -o).
+ * upside_batch_run_logged advances n_round rounds and records a frame at
+ * every round number (counted since the batch was created) that is a
+ * multiple of frame_interval, before that round's thermostat and integration.
+ * upside_batch_close_output flushes and closes the files (also done by
+ * upside_batch_free).  0 on success. */
+int upside_batch_open_output(UpsideBatch* batch, const char* const* output_paths, int record_momentum);
+int upside_batch_run_logged(UpsideBatch* batch, int n_round, int frame_interval);
+int upside_batch_close_output(UpsideBatch* batch);
 /* Checkpoint and resume (no counterpart in the reference, whose thermostat
  * counter lives only in memory, src/thermostat.h:24-29): the whole device state
  * of the batch (per system: thermostat seed and counter, temperature,

@@ -300,6 +300,44 @@ def test_capacity_regrowth_is_transparent(ue, chain_file, monkeypatch):
         np.testing.assert_array_equal(b1.pos(s), b0.pos(s))
 
 
+def test_output_logger(ue, chain_file, tmp_path):
+    """/output datasets as the reference's basic log level writes them: frames
+    at rounds 0, 3, 6 of a 7-round run hold the state before that round's
+    thermostat and integration; the logged run follows an unlogged launch
+    with in-kernel frames at the same rounds bit for bit."""
+    import h5lite
+    path = chain_file(40, 21)
+    say("case: /output logger, 2 x L=40, 7 rounds, frames every 3")
+    outs = [str(tmp_path / "out0.up"), str(tmp_path / "out1.up")]
+    temps, dt = [0.85, 0.9], 0.009
+    b = ue.Batch([path] * 2, temps, seed=8, dt=dt)
+    with pytest.raises(RuntimeError):
+        b.open_output(None)          # both systems read one input file
+    b.open_output(outs, record_momentum=True)
+    b.run_logged(7, 3)
+    b.close_output()
+    ref = ue.Batch([path] * 2, temps, seed=8, dt=dt)
+    pots = ref.run(7, frame_interval=3, n_frames=3)
+    pos0 = read_pos(path)
+    n_atom = len(pos0)
+    for s, o in enumerate(outs):
+        np.testing.assert_array_equal(b.pos(s), ref.pos(s))
+        with h5lite.File(o) as f:
+            pos, mom = f.read("/output/pos"), f.read("/output/mom")
+            kin, pot = f.read("/output/kinetic"), f.read("/output/potential")
+            t, temp = f.read("/output/time"), f.read("/output/temperature")
+        assert pos.shape == (3, 1, n_atom, 3) and mom.shape == (3, 1, n_atom, 3)
+        assert kin.shape == (3, 1) and pot.shape == (3, 1) and temp.shape == (3, 1) and t.shape == (3,)
+        assert pos.dtype == np.float32 and kin.dtype == np.float64 and t.dtype == np.float64
+        np.testing.assert_array_equal(pos[0, 0], pos0)
+        np.testing.assert_allclose(pot[:, 0], pots[s], rtol=1e-6, atol=1e-5)
+        m2 = (mom[:, 0].astype(np.float64) ** 2).sum(axis=(1, 2))
+        np.testing.assert_allclose(kin[:, 0], 0.5 * m2 / n_atom, rtol=1e-6)
+        want_t = [float(np.float32(3 * np.float32(dt)) * np.float32(r)) for r in (0, 3, 6)]
+        np.testing.assert_array_equal(t, want_t)
+        np.testing.assert_array_equal(temp[:, 0], np.float32(temps[s]))
+
+
 def test_checkpoint_resume_is_bit_identical(ue, chain_file):
     """A saved batch state loaded into a fresh batch of the same files (other
     seed, other team size) continues bit for bit, also through a replica

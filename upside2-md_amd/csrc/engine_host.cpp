@@ -1251,6 +1251,7 @@ struct Batch {
     uint32_t base_seed = 0;
     bool have_mom = false;
     double cap_scale = 1.0;
+    uint64_t round_num = 0;      // rounds integrated since creation (src/main.cpp:920 round_num)
     bool profile = false;
     vector<unsigned long long*> prof;
     int prof_len(int k) const { return sys[k].prog->n_germ() + sys[k].prog->n_deriv() + PROF_EXTRA; }
@@ -1507,7 +1508,7 @@ struct Batch {
             if (ms_out) ms_out[k] = -1.f;
             if (cand[k] < 1 || cand[k] > limit) continue;
             set_team(cand[k]);
-            run(n_round, 0, nullptr, 0, 0);
+            run(n_round, 0, nullptr, 0, (int)round_num);
             if (ms_out) ms_out[k] = last_ms;
             if (last_ms < best_ms) { best_ms = last_ms; best = cand[k]; }
             restore();
@@ -1604,6 +1605,7 @@ struct Batch {
         int version, n_sys, thermostat_interval;
         uint32_t base_seed;
         double cap_scale;
+        uint64_t round_num;
         float dt, pad2;
     };
     struct SysState {
@@ -1631,6 +1633,7 @@ struct Batch {
         h.thermostat_interval = thermostat_interval;
         h.base_seed = base_seed;
         h.cap_scale = cap_scale;
+        h.round_num = round_num;
         h.dt = dt;
         std::memcpy(out, &h, sizeof(h));
         out += sizeof(h);
@@ -1694,6 +1697,7 @@ struct Batch {
         dt = h.dt;
         thermostat_interval = h.thermostat_interval;
         base_seed = h.base_seed;
+        round_num = h.round_num;
         have_mom = true;
         push_desc();
     }
@@ -1754,8 +1758,121 @@ using namespace upside;
 struct DerivEngine {
     Batch batch;
 };
+// /output logger of one system: the basic-level datasets of the reference's
+// simulation loop (src/main.cpp:807-831 pos, mom, kinetic, potential, time;
+// :893 temperature) as extendable arrays with the first dimension unlimited
+// and chunks of 100 frames (src/state_logger.h:21-39), appended one frame at a
+// time and flushed on close.
+struct OutputLog {
+    hid_t file = -1, group = -1;
+    hid_t pos = -1, mom = -1, kinetic = -1, potential = -1, time = -1, temperature = -1;
+    long long n = 0;
+
+    static hid_t earray(hid_t g, const char* name, hid_t type, const vector<hsize_t>& row) {
+        vector<hsize_t> dims{0}, maxd{H5S_UNLIMITED}, chunk{100};
+        for (hsize_t r : row) { dims.push_back(r); maxd.push_back(r); chunk.push_back(r); }
+        hid_t sp = H5Screate_simple((int)dims.size(), dims.data(), maxd.data());
+        hid_t pl = H5Pcreate(H5P_DATASET_CREATE);
+        H5Pset_chunk(pl, (int)chunk.size(), chunk.data());
+        hid_t d = H5Dcreate2(g, name, type, sp, H5P_DEFAULT, pl, H5P_DEFAULT);
+        H5Pclose(pl);
+        H5Sclose(sp);
+        if (d < 0) fail(string("output: cannot create dataset ") + name);
+        return d;
+    }
+    static void append(hid_t d, hid_t type, const void* data, long long row) {
+        hid_t sp = H5Dget_space(d);
+        int rank = H5Sget_simple_extent_ndims(sp);
+        vector<hsize_t> dims(rank);
+        H5Sget_simple_extent_dims(sp, dims.data(), nullptr);
+        H5Sclose(sp);
+        dims[0] = (hsize_t)row + 1;
+        if (H5Dset_extent(d, dims.data()) < 0) fail("output: cannot extend dataset");
+        sp = H5Dget_space(d);
+        vector<hsize_t> start(rank, 0), count = dims;
+        start[0] = (hsize_t)row;
+        count[0] = 1;
+        H5Sselect_hyperslab(sp, H5S_SELECT_SET, start.data(), nullptr, count.data(), nullptr);
+        hid_t ms = H5Screate_simple(rank, count.data(), nullptr);
+        herr_t e = H5Dwrite(d, type, ms, sp, H5P_DEFAULT, data);
+        H5Sclose(ms);
+        H5Sclose(sp);
+        if (e < 0) fail("output: write failed");
+    }
+    void open(const string& path, bool into_input, int n_atom, bool record_mom) {
+        if (into_input) {
+            file = H5Fopen(path.c_str(), H5F_ACC_RDWR, H5P_DEFAULT);
+            if (file >= 0 && H5Lexists(file, "output", H5P_DEFAULT) > 0) H5Ldelete(file, "output", H5P_DEFAULT);
+        } else {
+            file = H5Fcreate(path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+        }
+        if (file < 0) fail("output: cannot open " + path + " for writing");
+        group = H5Gcreate2(file, "output", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+        if (group < 0) fail("output: cannot create /output in " + path);
+        const hsize_t na = (hsize_t)n_atom;
+        pos = earray(group, "pos", H5T_NATIVE_FLOAT, {1, na, 3});
+        if (record_mom) mom = earray(group, "mom", H5T_NATIVE_FLOAT, {1, na, 3});
+        kinetic = earray(group, "kinetic", H5T_NATIVE_DOUBLE, {1});
+        potential = earray(group, "potential", H5T_NATIVE_DOUBLE, {1});
+        time = earray(group, "time", H5T_NATIVE_DOUBLE, {});
+        temperature = earray(group, "temperature", H5T_NATIVE_DOUBLE, {1});
+        // the reference records its command line here (src/main.cpp:734-736)
+        const char* inv = "libupside batch (upside_batch_run_logged)";
+        hid_t st = H5Tcopy(H5T_C_S1);
+        H5Tset_size(st, strlen(inv));
+        hid_t sp = H5Screate(H5S_SCALAR);
+        hid_t a = H5Acreate2(group, "invocation", st, sp, H5P_DEFAULT, H5P_DEFAULT);
+        if (a >= 0) { H5Awrite(a, st, inv); H5Aclose(a); }
+        H5Sclose(sp);
+        H5Tclose(st);
+    }
+    void frame(const float* p, const float* m, int n_atom, double kin, double pot, double t, double temp) {
+        append(pos, H5T_NATIVE_FLOAT, p, n);
+        if (mom >= 0) append(mom, H5T_NATIVE_FLOAT, m, n);
+        append(kinetic, H5T_NATIVE_DOUBLE, &kin, n);
+        append(potential, H5T_NATIVE_DOUBLE, &pot, n);
+        append(time, H5T_NATIVE_DOUBLE, &t, n);
+        append(temperature, H5T_NATIVE_DOUBLE, &temp, n);
+        ++n;
+        (void)n_atom;
+    }
+    void close() {
+        for (hid_t* d : {&pos, &mom, &kinetic, &potential, &time, &temperature})
+            if (*d >= 0) { H5Dclose(*d); *d = -1; }
+        if (group >= 0) { H5Gclose(group); group = -1; }
+        if (file >= 0) { H5Fflush(file, H5F_SCOPE_LOCAL); H5Fclose(file); file = -1; }
+    }
+    ~OutputLog() { close(); }
+};
+
 struct UpsideBatch {
     Batch batch;
+    std::vector<std::unique_ptr<OutputLog>> out;
+    bool record_mom = false;
+
+    // One frame of every system at the current state: the potential and
+    // derivative are evaluated first, as src/main.cpp:946-949 does before
+    // collect_samples.
+    void log_frame() {
+        batch.evaluate(1);
+        for (size_t k = 0; k < batch.sys.size(); ++k) {
+            const Topology& T = batch.sys[k].prog->T;
+            vector<float> pos(T.n_atom * 3), mom(T.n_atom * 3);
+            hip_check(hipMemcpy(pos.data(), batch.sys[k].arena + T.nodes[0].out, pos.size() * 4,
+                                hipMemcpyDeviceToHost), "pos");
+            hip_check(hipMemcpy(mom.data(), batch.sys[k].arena + T.mom, mom.size() * 4, hipMemcpyDeviceToHost),
+                      "mom");
+            double kin = 0.0;  // (1/2) <|p|^2> per atom, summed in double (src/main.cpp:822-826)
+            for (int a = 0; a < T.n_atom; ++a) {
+                float x = mom[a * 3], y = mom[a * 3 + 1], z = mom[a * 3 + 2];
+                kin += x * x + y * y + z * z;
+            }
+            kin *= 0.5 / T.n_atom;
+            const float t = 3 * batch.dt * (float)batch.round_num;  // inner_step*dt*round_num (:830-831)
+            out[k]->frame(pos.data(), mom.data(), T.n_atom, kin, (double)batch.potential((int)k), (double)t,
+                          (double)batch.sys[k].temperature);
+        }
+    }
 };
 
 #define ABI_TRY try {
@@ -2073,6 +2190,55 @@ int upside_batch_get_pos(UpsideBatch* b, int sys, float* pos) { return batch_cop
 int upside_batch_set_mom(UpsideBatch* b, int sys, const float* mom) { return batch_copy(b, sys, (float*)mom, true, true); }
 int upside_batch_get_mom(UpsideBatch* b, int sys, float* mom) { return batch_copy(b, sys, mom, false, true); }
 
+int upside_batch_open_output(UpsideBatch* b, const char* const* output_paths, int record_momentum) {
+    ABI_TRY
+    auto& B = b->batch;
+    const int ns = (int)B.sys.size();
+    std::set<string> inputs;
+    if (!output_paths)
+        for (int k = 0; k < ns; ++k)
+            if (!inputs.insert(B.sys[k].prog->path).second)
+                fail("open_output: systems share the input file " + B.sys[k].prog->path + "; give output paths");
+    for (auto& o : b->out) o->close();
+    b->out.clear();
+    b->record_mom = record_momentum != 0;
+    for (int k = 0; k < ns; ++k) {
+        b->out.emplace_back(new OutputLog());
+        const string path = output_paths ? string(output_paths[k]) : B.sys[k].prog->path;
+        b->out.back()->open(path, !output_paths, B.sys[k].prog->T.n_atom, b->record_mom);
+    }
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_run_logged(UpsideBatch* b, int n_round, int frame_interval) {
+    ABI_TRY
+    auto& B = b->batch;
+    if (b->out.empty()) fail("run_logged: no output open");
+    if (n_round < 0 || frame_interval < 1) fail("run_logged: need n_round >= 0 and frame_interval >= 1");
+    int done = 0;
+    while (done < n_round) {
+        const uint64_t r = B.round_num;
+        if (r % (uint64_t)frame_interval == 0) b->log_frame();
+        // integrate up to the next frame round or the end
+        const int to_frame = (int)((uint64_t)frame_interval - r % (uint64_t)frame_interval);
+        const int n = std::min(to_frame, n_round - done);
+        B.run(n, 0, nullptr, 0, (int)B.round_num);
+        B.round_num += n;
+        done += n;
+    }
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_close_output(UpsideBatch* b) {
+    ABI_TRY
+    for (auto& o : b->out) o->close();
+    b->out.clear();
+    return 0;
+    ABI_CATCH(1)
+}
+
 long long upside_batch_state_size(UpsideBatch* b) {
     ABI_TRY
     return (long long)b->batch.state_size();
@@ -2095,12 +2261,12 @@ int upside_batch_load_state(UpsideBatch* b, const void* buf, long long size) {
     ABI_CATCH(1)
 }
 
-static uint64_t g_round_counter = 0;
 
 int upside_batch_run(UpsideBatch* b, int n_round, int frame_interval, float* frame_pot, int n_frames) {
     ABI_TRY
-    b->batch.run(n_round, frame_interval, frame_pot, n_frames, 0);
-    g_round_counter += n_round;
+    if (n_round < 0) fail("run: negative round count");
+    b->batch.run(n_round, frame_interval, frame_pot, n_frames, (int)b->batch.round_num);
+    b->batch.round_num += n_round;
     return 0;
     ABI_CATCH(1)
 }

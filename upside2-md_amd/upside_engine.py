@@ -63,6 +63,9 @@ def load_upside_library(path=LIB_PATH):
         "upside_batch_bp_stats": (i, [vp, i, vp, vp, vp, vp]),
         "upside_batch_potential_terms": (i, [vp, i, vp, i, ct.c_char_p, i]),
         "upside_batch_bp_histogram": (i, [vp, i, vp, i]),
+        "upside_batch_open_output": (i, [vp, vp, i]),
+        "upside_batch_run_logged": (i, [vp, i, i]),
+        "upside_batch_close_output": (i, [vp]),
         "upside_batch_state_size": (ct.c_longlong, [vp]),
         "upside_batch_save_state": (i, [vp, vp, ct.c_longlong]),
         "upside_batch_load_state": (i, [vp, vp, ct.c_longlong]),
@@ -242,6 +245,25 @@ class Batch:
         if self.calc.upside_batch_bp_stats(self.h, s, ct.byref(last), ct.byref(tot), ct.byref(sol), ct.byref(bad)):
             raise RuntimeError("bp stats failed")
         return dict(last=last.value, total=tot.value, solves=sol.value, bad=bad.value)
+
+    def open_output(self, paths=None, record_momentum=False):
+        """Start /output logging: into each system's input .up (paths None) or
+        into new files paths[s]."""
+        arr = None
+        if paths is not None:
+            enc = [p.encode() for p in paths]
+            arr = (ct.c_char_p * len(enc))(*enc)
+        if self.calc.upside_batch_open_output(self.h, arr, int(bool(record_momentum))):
+            raise RuntimeError("open_output failed (see stderr)")
+
+    def run_logged(self, n_round, frame_interval):
+        """Advance n_round rounds, logging a frame at rounds that are multiples of frame_interval."""
+        if self.calc.upside_batch_run_logged(self.h, int(n_round), int(frame_interval)):
+            raise RuntimeError("run_logged failed (see stderr)")
+
+    def close_output(self):
+        if self.calc.upside_batch_close_output(self.h):
+            raise RuntimeError("close_output failed")
 
     def save_state(self):
         """The whole device state of the batch as bytes (see load_state)."""

@@ -34,9 +34,11 @@ int evaluate_energy(float* energy, DerivEngine* engine, const float* pos);
 /* src/engine_c_library.cpp:49-65: dE/dx at pos, (n_atom,3). */
 int evaluate_deriv(float* deriv, DerivEngine* engine, const float* pos);
 /* src/engine_c_library.cpp:68-110: node parameter vectors.  get_param returns the
- * interaction table of pair-graph nodes; set_param and get_param_deriv are not
- * supported and return nonzero (the reference returns -1 from get_param_deriv
- * when built without PARAM_DERIV). */
+ * interaction table of pair-graph nodes.  get_param_deriv (the reference's
+ * PARAM_DERIV build, src/interaction_graph.h:406-416) returns d(potential)/d(table)
+ * at the last evaluation for the spline pair graphs rotamer, hbond_coverage and
+ * hbond_coverage_hydrophobe, and nonzero for other nodes.  set_param is not
+ * supported and returns nonzero. */
 int set_param(int n_param, const float* param, DerivEngine* engine, const char* node_name);
 int get_param_deriv(int n_param, float* deriv, DerivEngine* engine, const char* node_name);
 int get_param(int n_param, float* param, DerivEngine* engine, const char* node_name);

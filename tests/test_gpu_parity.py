@@ -300,6 +300,40 @@ def test_capacity_regrowth_is_transparent(ue, chain_file, monkeypatch):
         np.testing.assert_array_equal(b1.pos(s), b0.pos(s))
 
 
+@pytest.mark.parametrize("name", ["rotamer", "hbond_coverage", "hbond_coverage_hydrophobe"])
+def test_param_deriv_matches_reference(ue, chain_file, name):
+    """get_param_deriv of the spline pair graphs against the reference engine
+    built from its sources with PARAM_DERIV (oracle/_ref), on the same
+    jittered coordinates: relative norm error within 1e-4, and the same
+    result on a second call."""
+    import ctypes as ct
+    import h5lite
+    lib = ref_lib()
+    if lib is None:
+        pytest.skip("reference engine not built")
+    lib.get_param_deriv.argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_char_p]
+    path = chain_file(60, 7)
+    pos0 = read_pos(path)
+    pos = (pos0 + np.random.default_rng(3).normal(scale=0.1, size=pos0.shape)).astype(np.float32)
+    with h5lite.File(path) as f:
+        grp = "/input/potential/rotamer/pair_interaction" if name == "rotamer" else "/input/potential/" + name
+        shape = tuple(f.read(grp + "/interaction_param").shape)
+    say("case: param deriv %s, table %s" % (name, shape))
+    eng = ue.Upside(path, quiet=True)
+    ref = RefEngine(lib, path, len(pos0))
+    eng.deriv(pos)
+    ref.deriv(pos)
+    got = eng.get_param_deriv(shape, name)
+    want = np.zeros(shape, np.float32)
+    assert lib.get_param_deriv(want.size, want.ctypes.data, ct.c_void_p(ref.h), name.encode()) == 0
+    err = float(np.linalg.norm(got - want) / np.linalg.norm(want))
+    say("%s: |dE/dparam| %.4g, rel err %.2e, nonzero %d of %d" % (name, np.linalg.norm(want), err,
+                                                                   np.count_nonzero(want), want.size))
+    assert np.linalg.norm(want) > 0
+    assert err < 1e-4, err
+    np.testing.assert_array_equal(eng.get_param_deriv(shape, name), got)
+
+
 def test_output_logger(ue, chain_file, tmp_path):
     """/output datasets as the reference's basic log level writes them: frames
     at rounds 0, 3, 6 of a 7-round run hold the state before that round's

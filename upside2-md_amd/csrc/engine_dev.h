@@ -64,7 +64,7 @@ struct PairGraph {
     int type1, type2;      // const int arrays
     int id1, id2;          // const int arrays
     int param;             // const float array, (n_type1*n_type2, n_param) + tail padding
-    int n_type2, n_param;
+    int n_type1, n_type2, n_param;
     int param_len;         // floats of the table staged in LDS (0: read from memory)
     int excl;              // pair accepted if |id1-id2| > excl; -1: rotamer rule (id>>4 differs)
     float cutoff2;         // refine threshold (float cutoff squared)
@@ -89,6 +89,26 @@ struct PairGraph {
     int esens;             // float ecap   edge sensitivity
     int act;               // int ecap     active flag
     int state;             // int [0]=cache valid, [1]=overflow, [2]=rebuild count, [3]=slots
+};
+
+// Parameter derivative of a spline pair graph after an evaluation: one
+// workgroup per interaction-table row (type pair).
+struct ParamDerivArgs {
+    const float* x1;       // producer output of side 1, row stride w1, element rows loc1
+    const float* x2;       // side 2 (symmetric graphs: side 1)
+    int w1, w2;
+    const int* loc1;
+    const int* loc2;
+    const int* type1;
+    const int* type2;
+    const int* row;        // slot -> side-1 element
+    const int* nbr;        // slot -> side-2 element
+    const int* act;        // slot refined in (edge present)
+    const float* es;       // slot edge sensitivity
+    int n_slot;
+    const float* param;    // (n_type1*n_type2, n_param)
+    int n_type2, n_param, sym, kind;
+    float* out;            // (n_type1*n_type2, n_param)
 };
 
 // Rotamer belief propagation over residue nodes (src/rotamer.cpp:628-1169).

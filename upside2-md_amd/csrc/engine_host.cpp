@@ -27,6 +27,7 @@
 #include "../../include/upside_engine.h"
 
 hipError_t upside_launch_engine(upside::SysDesc* d_systems, int n_sys, upside::RunArgs args, hipStream_t stream);
+hipError_t upside_launch_param_deriv(const upside::ParamDerivArgs& args, int n_rows, hipStream_t stream);
 int upside_max_resident_blocks();
 
 namespace upside {
@@ -737,6 +738,7 @@ struct Builder {
         G.type1 = T.ci_push(ty1); G.type2 = T.ci_push(ty2);
         G.id1 = T.ci_push(id1); G.id2 = T.ci_push(id2);
         G.param = T.cf_push(param, 64);
+        G.n_type1 = nt1;
         G.n_type2 = nt2;
         G.n_param = np;
         G.w1 = p1->width; G.w2 = p2->width;
@@ -858,6 +860,7 @@ struct Builder {
         G.type1 = G.type2 = T.ci_push(zeros);
         G.id1 = G.id2 = T.ci_push(res);
         G.param = T.cf_push(vector<float>(1, 0.f), 64);
+        G.n_type1 = 1;
         G.n_type2 = 1;
         G.n_param = 0;
         G.param_len = 0;
@@ -1757,6 +1760,7 @@ using namespace upside;
 
 struct DerivEngine {
     Batch batch;
+    bool evaluated = false;   // an evaluation has set every node's outputs and sensitivities
 };
 // /output logger of one system: the basic-level datasets of the reference's
 // simulation loop (src/main.cpp:807-831 pos, mom, kinetic, potential, time;
@@ -1905,6 +1909,7 @@ int evaluate_energy(float* energy, DerivEngine* engine, const float* pos) {
     ABI_TRY
     engine->batch.copy_pos_in(0, pos);
     engine->batch.evaluate(1);
+    engine->evaluated = true;
     *energy = engine->batch.potential(0);
     return 0;
     ABI_CATCH(1)
@@ -1915,6 +1920,7 @@ int evaluate_deriv(float* deriv, DerivEngine* engine, const float* pos) {
     auto& B = engine->batch;
     B.copy_pos_in(0, pos);
     B.evaluate(1);
+    engine->evaluated = true;
     const Topology& T = B.sys[0].prog->T;
     hip_check(hipMemcpy(deriv, B.sys[0].arena + T.nodes[0].sens, T.n_atom * 3 * sizeof(float), hipMemcpyDeviceToHost), "deriv");
     return 0;
@@ -1927,9 +1933,68 @@ int set_param(int n_param, const float* param, DerivEngine* engine, const char* 
     return 1;
 }
 
+// src/engine_c_library.cpp:94-110 and InteractionGraph::get_param_deriv
+// (src/interaction_graph.h:406-416): d(potential)/d(interaction table) at the
+// last evaluation, as sum over present edges of the edge sensitivity times the
+// edge value's coefficient derivative, each edge adding to its type pair's row
+// (and, in a symmetric graph, to the transposed row too, :544-547).  Provided
+// for the spline pair graphs the reference builds it for in ff_2.1: rotamer
+// pairs (src/bead_interaction.h:206-209) and hbond coverage
+// (src/hbond.cpp:286-291).  Other nodes return nonzero.
 int get_param_deriv(int n_param, float* deriv, DerivEngine* engine, const char* node_name) {
-    (void)n_param; (void)deriv; (void)engine; (void)node_name;
-    return -1;
+    ABI_TRY
+    auto& B = engine->batch;
+    if (!engine->evaluated) fail("get_param_deriv: evaluate the engine first");
+    const Topology& T = B.sys[0].prog->T;
+    const Node* n = B.node(0, node_name);
+    for (size_t g = 0; g < T.graphs.size(); ++g) {
+        if (&T.nodes[T.graph_node[g]] != n) continue;
+        const PairGraph& G = T.graphs[g];
+        if (G.kind != EDGE_ROTAMER && G.kind != EDGE_HBOND_COVERAGE)
+            fail(string("get_param_deriv: no parameter derivative for node ") + node_name);
+        const int spline_params = G.kind == EDGE_ROTAMER ? 2 * 15 + 2 * 12 : 2 * 15 + 2 * 10;
+        if (G.n_param != spline_params)
+            fail(string("get_param_deriv: ") + node_name + " has " + std::to_string(G.n_param) +
+                 " parameters per type pair, the spline needs " + std::to_string(spline_params));
+        const int rows = G.n_type1 * G.n_type2;
+        const int size = rows * G.n_param;
+        if (size != n_param)
+            fail("Wrong number of parameters, expected " + std::to_string(size) + " but got " + std::to_string(n_param));
+        const System& s = B.sys[0];
+        const Program& P = *s.prog;
+        ParamDerivArgs a;
+        a.x1 = s.arena + G.out1;
+        a.x2 = s.arena + G.out2;
+        a.w1 = G.w1;
+        a.w2 = G.w2;
+        a.loc1 = P.d_ci + G.loc1;
+        a.loc2 = P.d_ci + G.loc2;
+        a.type1 = P.d_ci + G.type1;
+        a.type2 = P.d_ci + G.type2;
+        a.row = s.iarena + G.row;
+        a.nbr = s.iarena + G.nbr;
+        a.act = s.iarena + G.act;
+        a.es = s.arena + G.esens;
+        int st[4];
+        hip_check(hipMemcpy(st, s.iarena + G.state, sizeof(st), hipMemcpyDeviceToHost), "state");
+        a.n_slot = st[3];
+        a.param = P.d_cf + G.param;
+        a.n_type2 = G.n_type2;
+        a.n_param = G.n_param;
+        a.sym = G.sym;
+        a.kind = G.kind;
+        float* d_out = nullptr;
+        hip_check(hipMalloc((void**)&d_out, (size_t)size * sizeof(float)), "param deriv");
+        a.out = d_out;
+        hipError_t e = upside_launch_param_deriv(a, rows, B.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(B.stream);
+        if (e == hipSuccess) e = hipMemcpy(deriv, d_out, (size_t)size * sizeof(float), hipMemcpyDeviceToHost);
+        (void)hipFree(d_out);
+        hip_check(e, "param deriv");
+        return 0;
+    }
+    fail(string("get_param_deriv: no parameters exposed for node ") + node_name);
+    ABI_CATCH(1)
 }
 
 int get_param(int n_param, float* param, DerivEngine* engine, const char* node_name) {
@@ -1940,12 +2005,7 @@ int get_param(int n_param, float* param, DerivEngine* engine, const char* node_n
     for (size_t g = 0; g < T.graphs.size(); ++g) {
         if (&T.nodes[T.graph_node[g]] != n) continue;
         const PairGraph& G = T.graphs[g];
-        int total = (int)((T.ci[G.type1] , 0));
-        (void)total;
-        // interaction table size = n_type1 * n_type2 * n_param; recover n_type1 from the layout
-        int n_type1 = 0;
-        for (int i = 0; i < G.n1; ++i) n_type1 = std::max(n_type1, T.ci[G.type1 + i] + 1);
-        int size = n_type1 * G.n_type2 * G.n_param;
+        int size = G.n_type1 * G.n_type2 * G.n_param;
         if (size != n_param) fail("Wrong number of parameters, expected " + std::to_string(size) + " but got " + std::to_string(n_param));
         std::copy(T.cf.begin() + G.param, T.cf.begin() + G.param + size, param);
         return 0;

@@ -1167,7 +1167,117 @@ __global__ void __launch_bounds__(NT) engine_eval_kernel(SysDesc* systems, RunAr
     engine_body(systems, args, red, ired, lds);
 }
 
+// ---- parameter derivatives of spline pair graphs ------------------------------
+// Weights of the four coefficients of a uniform cubic B-spline at x >= 1 and
+// the first coefficient they multiply: the derivative of the value with
+// respect to each coefficient (deBoor_coeff_deriv, src/spline.h:366-384).
+__device__ __forceinline__ int bspline_weights(float x, float* w) {
+    const int xb = (int)x;
+    const float y = x - (float)xb;
+    const float omy = 1.f - y, y2 = y * y, y3 = y2 * y;
+    w[0] = omy * omy * omy * (1.f / 6.f);
+    w[1] = (3.f * y3 - 6.f * y2 + 4.f) * (1.f / 6.f);
+    w[2] = (-3.f * y3 + 3.f * y2 + 3.f * y + 1.f) * (1.f / 6.f);
+    w[3] = y3 * (1.f / 6.f);
+    return xb - 1;
+}
+
+// Clamped variant, constant outside [1, n_knot-2] (clamped_deBoor_coeff_deriv,
+// src/spline.h:423-440); NaN takes the lower end.
+__device__ __forceinline__ int clamped_bspline_weights(float x, int n_knot, float* w) {
+    if (!(x > 1.f)) {
+        w[0] = 1.f / 6.f; w[1] = 2.f / 3.f; w[2] = 1.f / 6.f; w[3] = 0.f;
+        return 0;
+    }
+    if (x >= (float)(n_knot - 2)) {
+        w[0] = 0.f; w[1] = 1.f / 6.f; w[2] = 2.f / 3.f; w[3] = 1.f / 6.f;
+        return n_knot - 4;
+    }
+    return bspline_weights(x, w);
+}
+
+// scale * d(quadspline)/d(p) added into acc[0..2*NKA+2*NK): the wide radial
+// spline, the narrow one times both angular values, each angular spline times
+// the other and the narrow value (quadspline_param_deriv,
+// src/bead_interaction.h:87-131).  Geometry as in the value (quadspline).
+template <int NKA, int NK>
+__device__ void quadspline_param_deriv(const float* p, const float* x1, const float* x2, float inv_dtheta,
+                                       float inv_dx, float scale, float* acc) {
+    float dx = x2[0] - x1[0], dy = x2[1] - x1[1], dz = x2[2] - x1[2];
+    float dist2 = __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, dx * dx));
+    float inv = rsqrtf(dist2);
+    float dist_coord = dist2 * (inv * inv_dx);
+    float ux = inv * dx, uy = inv * dy, uz = inv * dz;
+    float c1 = x1[3] * ux + x1[4] * uy + x1[5] * uz;
+    float c2 = -(x2[3] * ux + x2[4] * uy + x2[5] * uz);
+    float xa1 = fminf(fmaxf((c1 + 1.f) * inv_dtheta + 1.f, 1.f), (float)(NKA - 2));
+    float xa2 = fminf(fmaxf((c2 + 1.f) * inv_dtheta + 1.f, 1.f), (float)(NKA - 2));
+    float a1, a2, nn, unused;
+    deboor(p, xa1, a1, unused);
+    deboor(p + NKA, xa2, a2, unused);
+    clamped_deboor(p + 2 * NKA + NK, dist_coord, NK, nn, unused);
+    float w[4];
+    int b = clamped_bspline_weights(dist_coord, NK, w);
+    for (int i = 0; i < 4; ++i) acc[2 * NKA + b + i] += scale * w[i];
+    for (int i = 0; i < 4; ++i) acc[2 * NKA + NK + b + i] += scale * (a1 * a2 * w[i]);
+    b = bspline_weights(xa1, w);
+    for (int i = 0; i < 4; ++i) acc[b + i] += scale * (a2 * nn * w[i]);
+    b = bspline_weights(xa2, w);
+    for (int i = 0; i < 4; ++i) acc[NKA + b + i] += scale * (a1 * nn * w[i]);
+}
+
+// One workgroup per interaction-table row t: every thread walks the slots in
+// a fixed stride, adds the derivative of each present edge whose type pair is
+// t (or, in a symmetric graph, whose transposed pair is t) into its own LDS
+// row, and the rows are summed in thread order, so the result is the same on
+// every run.  In a symmetric graph an edge is taken with its lower element
+// first, the orientation the reference's pair list gives it
+// (src/interaction_graph.h:134-145).
+__global__ void __launch_bounds__(NT) param_deriv_kernel(ParamDerivArgs a) {
+    extern __shared__ float lds[];
+    const int t = blockIdx.x;
+    const int np = a.n_param;
+    float* acc = lds + threadIdx.x * np;
+    for (int d = 0; d < np; ++d) acc[d] = 0.f;
+    for (int s = threadIdx.x; s < a.n_slot; s += NT) {
+        if (!a.act[s]) continue;
+        int i = a.row[s], j = a.nbr[s];
+        if (a.sym && j < i) { int k = i; i = j; j = k; }
+        const int t1 = a.type1[i], t2 = a.type2[j];
+        const int m = (t1 * a.n_type2 + t2 == t) + (a.sym && t2 * a.n_type2 + t1 == t);
+        if (!m) continue;
+        const float* x1 = a.x1 + a.loc1[i] * a.w1;
+        const float* x2 = a.x2 + a.loc2[j] * a.w2;
+        const float* p = a.param + (t1 * a.n_type2 + t2) * np;
+        for (int k = 0; k < m; ++k) {
+            if (a.kind == EDGE_ROTAMER) {
+                quadspline_param_deriv<15, 12>(p, x1, x2, 6.f, 1.f / 0.7f, a.es[s], acc);
+            } else {  // hbond coverage: (1 - hb)^2 times the spline (src/hbond.cpp:286-291)
+                const float om = 1.f - x1[6];
+                quadspline_param_deriv<15, 10>(p, x1, x2, 6.f, 1.f / 0.625f, a.es[s] * (om * om), acc);
+            }
+        }
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < np; d += NT) {
+        float sum = 0.f;
+        for (int k = 0; k < NT; ++k) sum += lds[k * np + d];
+        a.out[t * np + d] = sum;
+    }
+}
+
 }  // namespace upside
+
+hipError_t upside_launch_param_deriv(const upside::ParamDerivArgs& args, int n_rows, hipStream_t stream) {
+    const size_t lds_bytes = (size_t)upside::NT * args.n_param * sizeof(float);
+    if (args.n_param <= 0 || lds_bytes > upside::LDS_FLOATS * sizeof(float) || n_rows <= 0) return hipErrorInvalidValue;
+    (void)hipFuncSetAttribute((const void*)upside::param_deriv_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds_bytes);
+    upside::ParamDerivArgs a = args;
+    void* kargs[] = {&a};
+    return hipLaunchKernel((const void*)upside::param_deriv_kernel, dim3(n_rows), dim3(upside::NT), kargs, lds_bytes,
+                           stream);
+}
 
 // Host-side launcher (C++ linkage inside the library only).
 hipError_t upside_launch_engine(upside::SysDesc* d_systems, int n_sys, upside::RunArgs args, hipStream_t stream) {

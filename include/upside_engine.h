@@ -37,8 +37,10 @@ int evaluate_deriv(float* deriv, DerivEngine* engine, const float* pos);
  * interaction table of pair-graph nodes.  get_param_deriv (the reference's
  * PARAM_DERIV build, src/interaction_graph.h:406-416) returns d(potential)/d(table)
  * at the last evaluation for the spline pair graphs rotamer, hbond_coverage and
- * hbond_coverage_hydrophobe, and nonzero for other nodes.  set_param is not
- * supported and returns nonzero. */
+ * hbond_coverage_hydrophobe, and nonzero for other nodes.  set_param replaces
+ * the table of rotamer, hbond_coverage(_hydrophobe) and protein_hbond (fixed
+ * pair-list cutoffs; a symmetric table must match its transpose as
+ * src/interaction_graph.h:385-396 requires) and returns nonzero for others. */
 int set_param(int n_param, const float* param, DerivEngine* engine, const char* node_name);
 int get_param_deriv(int n_param, float* deriv, DerivEngine* engine, const char* node_name);
 int get_param(int n_param, float* param, DerivEngine* engine, const char* node_name);

@@ -332,6 +332,33 @@ def test_param_deriv_matches_reference(ue, chain_file, name):
     assert np.linalg.norm(want) > 0
     assert err < 1e-4, err
     np.testing.assert_array_equal(eng.get_param_deriv(shape, name), got)
+    # set_param: the same perturbed table in both engines gives the same
+    # energy and forces, and get_param returns it
+    lib.set_param.argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_char_p]
+    table = eng.get_param(shape, name)
+    noise = np.random.default_rng(4).standard_normal(shape)
+    if name == "rotamer":
+        # a symmetric table must stay compatible with its transpose (angular
+        # halves swapped, radial equal, src/bead_interaction.h:211-220): change
+        # the radial splines by a symmetric factor only
+        noise = 0.5 * (noise + noise.transpose(1, 0, 2))
+        noise[:, :, :30] = 0.0
+    new = (table * (1.0 + 0.05 * noise)).astype(np.float32)
+    eng.set_param(new, name)
+    assert lib.set_param(new.size, new.ctypes.data, ct.c_void_p(ref.h), name.encode()) == 0
+    np.testing.assert_array_equal(eng.get_param(shape, name), new)
+    e, d = eng.energy(pos), eng.deriv(pos)
+    er, dr = ref.energy(pos), ref.deriv(pos)
+    say("%s after set_param: E %.5f ref %.5f, force rel %.2e" % (name, e, er, rel_rms(d, dr)))
+    assert abs(e - er) / max(abs(er), 1.0) < E_TOL and rel_rms(d, dr) < F_TOL
+    with pytest.raises(RuntimeError):
+        eng.set_param(new.ravel()[:-1], name)
+    if name == "rotamer":
+        bad = new.copy()
+        bad[0, 1, 40] += 1.0          # radial part no longer equal to its transpose
+        with pytest.raises(RuntimeError):
+            eng.set_param(bad, name)
+        np.testing.assert_array_equal(eng.get_param(shape, name), new)
 
 
 def test_output_logger(ue, chain_file, tmp_path):

@@ -1158,6 +1158,7 @@ struct Program {
     RotamerDev* d_rots = nullptr;
     float* d_cf = nullptr;
     int* d_ci = nullptr;
+    std::map<int, vector<float>> param_set;  // interaction tables replaced by set_param, by graph
 
     void release() {
         (void)hipFree(d_ops); (void)hipFree(d_graphs); (void)hipFree(d_rots); (void)hipFree(d_cf); (void)hipFree(d_ci);
@@ -1170,6 +1171,8 @@ struct Program {
         if (n_atom <= 0) n_atom = (int)h5.shape("/input/pos")[0];
         T.cap_scale = cap_scale;
         load_topology(T, h5, n_atom);
+        for (auto& kv : param_set)  // tables set through set_param survive a reload
+            std::copy(kv.second.begin(), kv.second.end(), T.cf.begin() + T.graphs[kv.first].param);
         release();
         // potential nodes push their sensitivities in the value pass: their
         // deriv steps have nothing to do and stay out of the program
@@ -1927,10 +1930,55 @@ int evaluate_deriv(float* deriv, DerivEngine* engine, const float* pos) {
     ABI_CATCH(1)
 }
 
+// src/engine_c_library.cpp:68-78 and InteractionGraph::set_param
+// (src/interaction_graph.h:417-426): replace a pair graph's interaction table.
+// Supported where the pair-list cutoff does not depend on the table (the spline
+// graphs and protein_hbond, whose cutoffs are fixed, src/bead_interaction.h:193-195,
+// src/hbond.cpp:164-166, :258-260); the environment graphs, whose cutoff the
+// reference recomputes from the new table, return nonzero.
 int set_param(int n_param, const float* param, DerivEngine* engine, const char* node_name) {
-    (void)n_param; (void)param; (void)engine;
-    fprintf(stderr, "ERROR: set_param is not supported by this engine (node %s)\n", node_name);
-    return 1;
+    ABI_TRY
+    auto& B = engine->batch;
+    Program& P = *B.sys[0].prog;
+    const Topology& T = P.T;
+    const Node* n = B.node(0, node_name);
+    for (size_t g = 0; g < T.graphs.size(); ++g) {
+        if (&T.nodes[T.graph_node[g]] != n) continue;
+        const PairGraph& G = T.graphs[g];
+        if (G.kind != EDGE_ROTAMER && G.kind != EDGE_HBOND_COVERAGE && G.kind != EDGE_PROTEIN_HBOND)
+            fail(string("set_param: the pair-list cutoff of ") + node_name + " depends on its parameters; not supported");
+        const int size = G.n_type1 * G.n_type2 * G.n_param;
+        if (n_param != size)
+            fail("Bad param size, got " + std::to_string(n_param) + " params, but expected " + std::to_string(size) +
+                 " params of shape (" + std::to_string(G.n_type1) + ", " + std::to_string(G.n_type2) + ", " +
+                 std::to_string(G.n_param) + ")");
+        vector<float> v(param, param + n_param);
+        if (G.sym) {
+            // a symmetric table must equal its transpose with the angular
+            // halves swapped (PosQuadSplineInteraction::is_compatible,
+            // src/bead_interaction.h:211-220, checked by update_cutoffs,
+            // :385-396); checked here before anything is replaced
+            const int nka = 15, np = G.n_param, nt = G.n_type2;
+            for (int a = 0; a < G.n_type1; ++a)
+                for (int b = 0; b < nt; ++b) {
+                    const float* p1 = v.data() + (a * nt + b) * np;
+                    const float* p2 = v.data() + (b * nt + a) * np;
+                    for (int k = 0; k < nka; ++k)
+                        if (p1[k] != p2[k + nka] || p1[k + nka] != p2[k]) fail("bad angular match");
+                    for (int k = 2 * nka; k < np; ++k)
+                        if (p1[k] != p2[k]) fail("incompatible parameters");
+                }
+        }
+        hip_check(hipStreamSynchronize(B.stream), "sync");
+        hip_check(hipMemcpy(P.d_cf + G.param, v.data(), (size_t)size * sizeof(float), hipMemcpyHostToDevice),
+                  "set_param");
+        std::copy(v.begin(), v.end(), P.T.cf.begin() + G.param);
+        P.param_set[(int)g] = std::move(v);
+        engine->evaluated = false;
+        return 0;
+    }
+    fail(string("set_param: no parameters exposed for node ") + node_name);
+    ABI_CATCH(1)
 }
 
 // src/engine_c_library.cpp:94-110 and InteractionGraph::get_param_deriv

@@ -123,12 +123,28 @@ int upside_batch_bp_histogram(UpsideBatch* batch, int sys, long long* counts, in
  * file), else output_paths[s] is created afresh (as with -o).
  * upside_batch_run_logged advances n_round rounds and records a frame at
  * every round number (counted since the batch was created) that is a
- * multiple of frame_interval, before that round's thermostat and integration.
+ * multiple of frame_interval (> 0), before that round's thermostat and
+ * integration.
  * upside_batch_close_output flushes and closes the files (also done by
  * upside_batch_free).  0 on success. */
 int upside_batch_open_output(UpsideBatch* batch, const char* const* output_paths, int record_momentum);
 int upside_batch_run_logged(UpsideBatch* batch, int n_round, int frame_interval);
 int upside_batch_close_output(UpsideBatch* batch);
+/* Monte Carlo pivot moves (src/monte_carlo_sampler.cpp:19-137, :255-283, from
+ * each file's /input/pivot_moves, py/upside_config.py:2543-2552): with
+ * mc_interval > 0, upside_batch_run_logged makes one pivot move per system at
+ * every round number r > 0 that is a multiple of mc_interval, before that
+ * round's frame, thermostat and integration (src/main.cpp:941-944): the move
+ * is drawn from Threefry stream 2 with counter r and kept by the Metropolis
+ * test on the potentials evaluated before and after.  frame_interval 0 (or no
+ * output open) runs without frames; an open output then also records
+ * pivot_stats (accepted, attempted since the last frame) when the interval was
+ * set before upside_batch_open_output.  Fails when a file has no pivot moves.
+ * upside_batch_mc_stats: accepted and attempted moves since the interval was
+ * set.  No counterpart in the reference's C ABI (its CLI option
+ * --monte-carlo-interval, src/main.cpp:471, :570-571). */
+int upside_batch_set_monte_carlo(UpsideBatch* batch, int mc_interval);
+int upside_batch_mc_stats(UpsideBatch* batch, int sys, long long* accepted, long long* attempted);
 /* Checkpoint and resume (no counterpart in the reference, whose thermostat
  * counter lives only in memory, src/thermostat.h:24-29): the whole device state
  * of the batch (per system: thermostat seed and counter, temperature,

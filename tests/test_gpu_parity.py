@@ -361,6 +361,48 @@ def test_param_deriv_matches_reference(ue, chain_file, name):
         np.testing.assert_array_equal(eng.get_param(shape, name), new)
 
 
+def test_mc_pivot_matches_reference_cli(ue, chain_file, tmp_path):
+    """Pivot moves every 2 rounds of a 6-round run against the reference CLI
+    built from its sources (--monte-carlo-interval): the same moves are
+    accepted (pivot_stats), and every logged frame (positions, potential)
+    matches within 1e-4 relative."""
+    import shutil
+    import subprocess
+    import h5lite
+    from conftest import REF_BIN
+    if not os.path.exists(REF_BIN):
+        pytest.skip("reference CLI not built")
+    src = chain_file(40, 21)
+    ref_up, our_up = str(tmp_path / "ref.up"), str(tmp_path / "ours.up")
+    shutil.copy(src, ref_up)
+    shutil.copy(src, our_up)
+    say("case: MC pivots L=40, 6 rounds, pivots every 2, against the reference CLI")
+    r = subprocess.run([REF_BIN, "--duration", "0.162", "--frame-interval", "0.027", "--monte-carlo-interval",
+                        "0.0675", "--temperature", "0.85", "--seed", "7", "--log-level", "basic",
+                        "--disable-recentering", ref_up], env=dict(os.environ, OMP_NUM_THREADS="1"),
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    b = ue.Batch([our_up], [0.85], seed=7, dt=0.009)
+    b.set_monte_carlo(2)
+    b.open_output(None)
+    b.run_logged(6, 1)
+    b.close_output()
+    assert b.mc_stats(0)[1] == 2
+    with h5lite.File(ref_up) as f:
+        want = {k: f.read("/output/" + k) for k in ("pos", "potential", "pivot_stats", "time")}
+    with h5lite.File(our_up) as f:
+        got = {k: f.read("/output/" + k) for k in ("pos", "potential", "pivot_stats", "time")}
+    say("pivot_stats ours %s reference %s" % (got["pivot_stats"].tolist(), want["pivot_stats"].tolist()))
+    np.testing.assert_array_equal(got["pivot_stats"], want["pivot_stats"])
+    np.testing.assert_array_equal(got["time"], want["time"])
+    assert got["pos"].shape == want["pos"].shape == (6, 1, 120, 3)
+    for k in range(6):
+        e = rel_rms(got["pos"][k, 0], want["pos"][k, 0])
+        de = abs(got["potential"][k, 0] - want["potential"][k, 0]) / max(abs(want["potential"][k, 0]), 1.0)
+        say("frame %d: pos rel %.2e potential rel %.2e" % (k, e, de))
+        assert e < 1e-4 and de < 1e-4, (k, e, de)
+
+
 def test_output_logger(ue, chain_file, tmp_path):
     """/output datasets as the reference's basic log level writes them: frames
     at rounds 0, 3, 6 of a 7-round run hold the state before that round's

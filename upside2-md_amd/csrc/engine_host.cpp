@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -1138,6 +1139,133 @@ static void threefry_host(const uint32_t ctr[4], const uint32_t key[4], uint32_t
     for (int i = 0; i < 4; ++i) out[i] = x[i];
 }
 
+static inline float u01_open_closed(uint32_t x) {  // r123::u01<float>: (0, 1]
+    return (float)x * 2.3283064365386962890625e-10f + 1.16415321826934814453125e-10f;
+}
+
+// Monte Carlo pivot moves (PivotSampler, src/monte_carlo_sampler.cpp:19-137,
+// input group /input/pivot_moves written by py/upside_config.py:2543-2552):
+// a residue's (phi, psi) is redrawn from its Rama proposal map and everything
+// downstream of it is rotated rigidly.
+struct PivotMoves {
+    int n_layer = 0, n_bin = 0;
+    vector<int> atom;     // [n_loc][5] prevC N CA C nextN
+    vector<int> range;    // [n_loc][2] atoms rotated
+    vector<int> restype;  // [n_loc] proposal layer
+    vector<float> lprob;  // proposal_pot + log(sum exp(-proposal_pot)) per layer
+    vector<float> cdf;    // cumulative proposal probability per layer
+    int n_loc() const { return (int)restype.size(); }
+
+    void load(const H5& h5, int n_atom) {
+        const string g = "/input/pivot_moves";
+        if (!h5.exists(g)) return;
+        auto sh = h5.shape(g + "/proposal_pot");
+        if (sh.size() != 3 || sh[1] != sh[2]) fail("pivot_moves: proposal_pot must be (n_layer, n_bin, n_bin)");
+        n_layer = (int)sh[0];
+        n_bin = (int)sh[1];
+        atom = h5.ri(g + "/pivot_atom");
+        range = h5.ri(g + "/pivot_range");
+        restype = h5.ri(g + "/pivot_restype");
+        const int nl = n_loc();
+        if ((int)atom.size() != 5 * nl || (int)range.size() != 2 * nl) fail("pivot_moves: inconsistent sizes");
+        for (int k = 0; k < nl; ++k) {
+            if (restype[k] < 0 || restype[k] >= n_layer) fail("invalid pivot restype");
+            for (int a = 0; a < 5; ++a) {
+                if (atom[5 * k + a] < 0 || atom[5 * k + a] >= n_atom) fail("pivot_atom out of range");
+                if (range[2 * k] <= atom[5 * k + a] && atom[5 * k + a] < range[2 * k + 1])
+                    fail("pivot_range cannot contain any atoms in pivot_atom");
+            }
+            if (range[2 * k] < 0 || range[2 * k + 1] > n_atom) fail("pivot_range out of range");
+        }
+        auto pot = h5.rf(g + "/proposal_pot");
+        const int nn = n_bin * n_bin;
+        lprob = pot;
+        cdf.assign(pot.size(), 0.f);
+        for (int l = 0; l < n_layer; ++l) {  // normalisation in double (:70-83)
+            double sum = 0.0;
+            for (int i = 0; i < nn; ++i) {
+                sum += std::exp(-(double)pot[l * nn + i]);
+                cdf[l * nn + i] = (float)sum;
+            }
+            const double inv = 1.0 / sum, lsum = std::log(sum);
+            for (int i = 0; i < nn; ++i) {
+                cdf[l * nn + i] = (float)((double)cdf[l * nn + i] * inv);
+                lprob[l * nn + i] = (float)((double)lprob[l * nn + i] + lsum);
+            }
+            cdf[(l + 1) * nn - 1] = 1.f;
+        }
+    }
+
+    static void rot(float* U, float angle, const float* a) {  // axis_angle_to_rot, src/affine.h:49-64
+        const float c = std::cos(angle), s = std::sin(angle), C = 1.f - c;
+        const float x = a[0], y = a[1], z = a[2];
+        U[0] = x * x * C + c;     U[1] = x * y * C - z * s; U[2] = x * z * C + y * s;
+        U[3] = y * x * C + z * s; U[4] = y * y * C + c;     U[5] = y * z * C - x * s;
+        U[6] = z * x * C - y * s; U[7] = z * y * C + x * s; U[8] = z * z * C + c;
+    }
+    static float dihedral(const float* r1, const float* r2, const float* r3, const float* r4) {
+        // Blondel-Karplus form of dihedral_germ (src/vector_math.h:766-796)
+        float F[3], G[3], H[3], A[3], B[3], C[3];
+        for (int k = 0; k < 3; ++k) { F[k] = r1[k] - r2[k]; G[k] = r2[k] - r3[k]; H[k] = r4[k] - r3[k]; }
+        auto cross = [](const float* u, const float* v, float* w) {
+            w[0] = u[1] * v[2] - u[2] * v[1]; w[1] = u[2] * v[0] - u[0] * v[2]; w[2] = u[0] * v[1] - u[1] * v[0];
+        };
+        cross(F, G, A);
+        cross(H, G, B);
+        cross(B, A, C);
+        const float gmag = std::sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]);
+        return std::atan2(C[0] * G[0] + C[1] * G[1] + C[2] * G[2], (A[0] * B[0] + A[1] * B[1] + A[2] * B[2]) * gmag);
+    }
+
+    // Propose a move on pos (n_atom x 3) from four uniforms u (the first draw
+    // of the move's generator); returns log(proposal ratio) (:89-137).
+    float propose(float* pos, const float* u) const {
+        const float PI = 3.14159265358979323846f;
+        int loc = (int)(n_loc() * u[2]);
+        if (loc == n_loc()) loc--;
+        const int* at = &atom[5 * loc];
+        const int l = restype[loc], nn = n_bin * n_bin;
+        const int bin = (int)(std::lower_bound(cdf.begin() + l * nn, cdf.begin() + (l + 1) * nn, u[3]) -
+                              (cdf.begin() + l * nn));
+        const float new_lp = lprob[l * nn + bin];
+        const int phi_bin = bin / n_bin, psi_bin = bin % n_bin;
+        const float w = 2.f * PI / n_bin;
+        const float new_phi = w * (phi_bin + u[0] - 0.5f) - PI;
+        const float new_psi = w * (psi_bin + u[1] - 0.5f) - PI;
+        const float* prevC = pos + 3 * at[0];
+        const float* N = pos + 3 * at[1];
+        const float* CA = pos + 3 * at[2];
+        const float* Cp = pos + 3 * at[3];
+        const float* nextN = pos + 3 * at[4];
+        const float old_phi = dihedral(prevC, N, CA, Cp), old_psi = dihedral(N, CA, Cp, nextN);
+        int ob_phi = (int)((old_phi + PI) * (0.5f / PI) * n_bin + 0.5f);
+        int ob_psi = (int)((old_psi + PI) * (0.5f / PI) * n_bin + 0.5f);
+        ob_phi = ob_phi >= n_bin ? 0 : ob_phi;
+        ob_psi = ob_psi >= n_bin ? 0 : ob_psi;
+        const float old_lp = lprob[(l * n_bin + ob_phi) * n_bin + ob_psi];
+        float phi_o[3], psi_o[3], ax1[3], ax2[3];
+        for (int k = 0; k < 3; ++k) { phi_o[k] = CA[k]; psi_o[k] = Cp[k]; ax1[k] = CA[k] - N[k]; ax2[k] = Cp[k] - CA[k]; }
+        const float n1 = 1.f / std::sqrt(ax1[0] * ax1[0] + ax1[1] * ax1[1] + ax1[2] * ax1[2]);
+        const float n2 = 1.f / std::sqrt(ax2[0] * ax2[0] + ax2[1] * ax2[1] + ax2[2] * ax2[2]);
+        for (int k = 0; k < 3; ++k) { ax1[k] *= n1; ax2[k] *= n2; }
+        float Uphi[9], Upsi[9];
+        rot(Uphi, new_phi - old_phi, ax1);
+        rot(Upsi, new_psi - old_psi, ax2);
+        auto move = [&](int a) {
+            float* y = pos + 3 * a;
+            float d[3], t[3];
+            for (int k = 0; k < 3; ++k) d[k] = y[k] - psi_o[k];
+            for (int k = 0; k < 3; ++k) t[k] = psi_o[k] + (Upsi[3 * k] * d[0] + Upsi[3 * k + 1] * d[1] + Upsi[3 * k + 2] * d[2]);
+            for (int k = 0; k < 3; ++k) d[k] = t[k] - phi_o[k];
+            for (int k = 0; k < 3; ++k) y[k] = phi_o[k] + (Uphi[3 * k] * d[0] + Uphi[3 * k + 1] * d[1] + Uphi[3 * k + 2] * d[2]);
+        };
+        move(at[3]);
+        move(at[4]);
+        for (int a = range[2 * loc]; a < range[2 * loc + 1]; ++a) move(a);
+        return new_lp - old_lp;
+    }
+};
+
 // Op kinds with work in the sensitivity pass (the others push in their value pass).
 static bool has_deriv_step(int kind) {
     switch (kind) {
@@ -1159,6 +1287,7 @@ struct Program {
     float* d_cf = nullptr;
     int* d_ci = nullptr;
     std::map<int, vector<float>> param_set;  // interaction tables replaced by set_param, by graph
+    PivotMoves pivots;                       // /input/pivot_moves, if present
 
     void release() {
         (void)hipFree(d_ops); (void)hipFree(d_graphs); (void)hipFree(d_rots); (void)hipFree(d_cf); (void)hipFree(d_ci);
@@ -1171,6 +1300,8 @@ struct Program {
         if (n_atom <= 0) n_atom = (int)h5.shape("/input/pos")[0];
         T.cap_scale = cap_scale;
         load_topology(T, h5, n_atom);
+        pivots = PivotMoves();
+        pivots.load(h5, n_atom);
         for (auto& kv : param_set)  // tables set through set_param survive a reload
             std::copy(kv.second.begin(), kv.second.end(), T.cf.begin() + T.graphs[kv.first].param);
         release();
@@ -1773,6 +1904,7 @@ struct DerivEngine {
 struct OutputLog {
     hid_t file = -1, group = -1;
     hid_t pos = -1, mom = -1, kinetic = -1, potential = -1, time = -1, temperature = -1;
+    hid_t pivot_stats = -1;   // (n_frame, 2): accepted and attempted pivots since the last frame
     long long n = 0;
 
     static hid_t earray(hid_t g, const char* name, hid_t type, const vector<hsize_t>& row) {
@@ -1806,7 +1938,7 @@ struct OutputLog {
         H5Sclose(sp);
         if (e < 0) fail("output: write failed");
     }
-    void open(const string& path, bool into_input, int n_atom, bool record_mom) {
+    void open(const string& path, bool into_input, int n_atom, bool record_mom, bool mc) {
         if (into_input) {
             file = H5Fopen(path.c_str(), H5F_ACC_RDWR, H5P_DEFAULT);
             if (file >= 0 && H5Lexists(file, "output", H5P_DEFAULT) > 0) H5Ldelete(file, "output", H5P_DEFAULT);
@@ -1823,6 +1955,7 @@ struct OutputLog {
         potential = earray(group, "potential", H5T_NATIVE_DOUBLE, {1});
         time = earray(group, "time", H5T_NATIVE_DOUBLE, {});
         temperature = earray(group, "temperature", H5T_NATIVE_DOUBLE, {1});
+        if (mc) pivot_stats = earray(group, "pivot_stats", H5T_NATIVE_INT, {2});  // src/monte_carlo_sampler.h:32-36
         // the reference records its command line here (src/main.cpp:734-736)
         const char* inv = "libupside batch (upside_batch_run_logged)";
         hid_t st = H5Tcopy(H5T_C_S1);
@@ -1833,18 +1966,20 @@ struct OutputLog {
         H5Sclose(sp);
         H5Tclose(st);
     }
-    void frame(const float* p, const float* m, int n_atom, double kin, double pot, double t, double temp) {
+    void frame(const float* p, const float* m, int n_atom, double kin, double pot, double t, double temp,
+               const int* mc_stats) {
         append(pos, H5T_NATIVE_FLOAT, p, n);
         if (mom >= 0) append(mom, H5T_NATIVE_FLOAT, m, n);
         append(kinetic, H5T_NATIVE_DOUBLE, &kin, n);
         append(potential, H5T_NATIVE_DOUBLE, &pot, n);
         append(time, H5T_NATIVE_DOUBLE, &t, n);
         append(temperature, H5T_NATIVE_DOUBLE, &temp, n);
+        if (pivot_stats >= 0) append(pivot_stats, H5T_NATIVE_INT, mc_stats, n);
         ++n;
         (void)n_atom;
     }
     void close() {
-        for (hid_t* d : {&pos, &mom, &kinetic, &potential, &time, &temperature})
+        for (hid_t* d : {&pos, &mom, &kinetic, &potential, &time, &temperature, &pivot_stats})
             if (*d >= 0) { H5Dclose(*d); *d = -1; }
         if (group >= 0) { H5Gclose(group); group = -1; }
         if (file >= 0) { H5Fflush(file, H5F_SCOPE_LOCAL); H5Fclose(file); file = -1; }
@@ -1856,6 +1991,62 @@ struct UpsideBatch {
     Batch batch;
     std::vector<std::unique_ptr<OutputLog>> out;
     bool record_mom = false;
+    int mc_interval = 0;                    // rounds between pivot moves (0: none)
+    vector<std::array<long long, 2>> mc_total;  // accepted, attempted since set_monte_carlo
+    vector<std::array<int, 2>> mc_frame;        // the same since the last logged frame
+
+    // One pivot move per system at round `round` (MonteCarloSampler::
+    // monte_carlo_step, src/monte_carlo_sampler.cpp:255-283): potentials of
+    // every system before and after the proposals are evaluated together on
+    // the GPU; generator (seed, stream 2, 0, round), first draw for the
+    // proposal, second for the Metropolis test when it is needed.
+    void mc_step(uint64_t round) {
+        const int ns = (int)batch.sys.size();
+        batch.pull_desc();
+        batch.evaluate(1);
+        vector<float> e_old(ns), dl(ns);
+        vector<vector<float>> pos(ns), keep(ns);
+        for (int k = 0; k < ns; ++k) e_old[k] = batch.potential(k);
+        for (int k = 0; k < ns; ++k) {
+            const Topology& T = batch.sys[k].prog->T;
+            pos[k].resize(T.n_atom * 3);
+            hip_check(hipMemcpy(pos[k].data(), batch.sys[k].arena + T.nodes[0].out, pos[k].size() * 4,
+                                hipMemcpyDeviceToHost), "mc pos");
+            keep[k] = pos[k];
+            uint32_t key[4] = {batch.desc[k].seed, 2u, 0u, 0u};
+            uint32_t ctr[4] = {(uint32_t)(round & 0xffffffffull), (uint32_t)(round >> 32), 0u, 0u};
+            uint32_t bits[4];
+            threefry_host(ctr, key, bits);
+            float u[4] = {u01_open_closed(bits[0]), u01_open_closed(bits[1]), u01_open_closed(bits[2]),
+                          u01_open_closed(bits[3])};
+            dl[k] = batch.sys[k].prog->pivots.propose(pos[k].data(), u);
+            hip_check(hipMemcpy(batch.sys[k].arena + T.nodes[0].out, pos[k].data(), pos[k].size() * 4,
+                                hipMemcpyHostToDevice), "mc pos");
+        }
+        batch.evaluate(1);
+        for (int k = 0; k < ns; ++k) {
+            const float e_new = batch.potential(k);
+            const float lb = dl[k] - (1.f / batch.sys[k].temperature) * (e_new - e_old[k]);
+            bool accept = lb >= 0.f;
+            if (!accept) {
+                uint32_t key[4] = {batch.desc[k].seed, 2u, 0u, 0u};
+                uint32_t ctr[4] = {(uint32_t)(round & 0xffffffffull), (uint32_t)(round >> 32), 0u, 1u};
+                uint32_t bits[4];
+                threefry_host(ctr, key, bits);
+                accept = std::exp(lb) >= u01_open_closed(bits[0]);
+            }
+            mc_total[k][1]++;
+            mc_frame[k][1]++;
+            if (accept) {
+                mc_total[k][0]++;
+                mc_frame[k][0]++;
+            } else {
+                const Topology& T = batch.sys[k].prog->T;
+                hip_check(hipMemcpy(batch.sys[k].arena + T.nodes[0].out, keep[k].data(), keep[k].size() * 4,
+                                    hipMemcpyHostToDevice), "mc restore");
+            }
+        }
+    }
 
     // One frame of every system at the current state: the potential and
     // derivative are evaluated first, as src/main.cpp:946-949 does before
@@ -1876,8 +2067,10 @@ struct UpsideBatch {
             }
             kin *= 0.5 / T.n_atom;
             const float t = 3 * batch.dt * (float)batch.round_num;  // inner_step*dt*round_num (:830-831)
+            int st[2] = {0, 0};
+            if (k < mc_frame.size()) { st[0] = mc_frame[k][0]; st[1] = mc_frame[k][1]; mc_frame[k] = {0, 0}; }
             out[k]->frame(pos.data(), mom.data(), T.n_atom, kin, (double)batch.potential((int)k), (double)t,
-                          (double)batch.sys[k].temperature);
+                          (double)batch.sys[k].temperature, st);
         }
     }
 };
@@ -2313,8 +2506,30 @@ int upside_batch_open_output(UpsideBatch* b, const char* const* output_paths, in
     for (int k = 0; k < ns; ++k) {
         b->out.emplace_back(new OutputLog());
         const string path = output_paths ? string(output_paths[k]) : B.sys[k].prog->path;
-        b->out.back()->open(path, !output_paths, B.sys[k].prog->T.n_atom, b->record_mom);
+        b->out.back()->open(path, !output_paths, B.sys[k].prog->T.n_atom, b->record_mom, b->mc_interval > 0);
     }
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_set_monte_carlo(UpsideBatch* b, int mc_interval) {
+    ABI_TRY
+    if (mc_interval < 0) fail("set_monte_carlo: negative interval");
+    if (mc_interval > 0)
+        for (auto& s : b->batch.sys)
+            if (s.prog->pivots.n_loc() == 0) fail("set_monte_carlo: " + s.prog->path + " has no /input/pivot_moves");
+    b->mc_interval = mc_interval;
+    b->mc_total.assign(b->batch.sys.size(), {0, 0});
+    b->mc_frame.assign(b->batch.sys.size(), {0, 0});
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_mc_stats(UpsideBatch* b, int sys, long long* accepted, long long* attempted) {
+    ABI_TRY
+    if (sys < 0 || sys >= (int)b->mc_total.size()) fail("mc_stats: no Monte Carlo state for this system");
+    *accepted = b->mc_total[sys][0];
+    *attempted = b->mc_total[sys][1];
     return 0;
     ABI_CATCH(1)
 }
@@ -2322,15 +2537,19 @@ int upside_batch_open_output(UpsideBatch* b, const char* const* output_paths, in
 int upside_batch_run_logged(UpsideBatch* b, int n_round, int frame_interval) {
     ABI_TRY
     auto& B = b->batch;
-    if (b->out.empty()) fail("run_logged: no output open");
-    if (n_round < 0 || frame_interval < 1) fail("run_logged: need n_round >= 0 and frame_interval >= 1");
+    if (n_round < 0 || frame_interval < 0) fail("run_logged: need n_round >= 0 and frame_interval >= 0");
+    const bool frames = frame_interval > 0 && !b->out.empty();
+    const uint64_t mc = (uint64_t)b->mc_interval, fi = (uint64_t)frame_interval;
     int done = 0;
+    // the order of one round of src/main.cpp:941-964: pivot move (not at round
+    // 0), frame, then thermostat and integration in the launch
     while (done < n_round) {
         const uint64_t r = B.round_num;
-        if (r % (uint64_t)frame_interval == 0) b->log_frame();
-        // integrate up to the next frame round or the end
-        const int to_frame = (int)((uint64_t)frame_interval - r % (uint64_t)frame_interval);
-        const int n = std::min(to_frame, n_round - done);
+        if (mc && r > 0 && r % mc == 0) b->mc_step(r);
+        if (frames && r % fi == 0) b->log_frame();
+        int n = n_round - done;
+        if (mc) n = std::min<int64_t>(n, (int64_t)(mc - r % mc));
+        if (frames) n = std::min<int64_t>(n, (int64_t)(fi - r % fi));
         B.run(n, 0, nullptr, 0, (int)B.round_num);
         B.round_num += n;
         done += n;

@@ -484,7 +484,24 @@ def build_ff21(seq, pos, rama_pot=None, params=None, rama_seed=0, damping=0.4, m
     _environment(g, seq, params, sc, sc_resnum)
     _bb_environment(g, seq, params, donors, acceptors, sc_resnum)
     _tail_nodes(g, n_res, sc, pl)
+    _pivot_moves(g, n_res)
     return g
+
+
+def _pivot_moves(g, n_res):
+    """/input/pivot_moves for the Monte Carlo pivot sampler
+    (py/upside_config.py:2543-2552): every residue with a full
+    (prevC, N, CA, C, nextN) window, its Rama proposal layer, and the atoms
+    after nextN as the rotated range."""
+    n_atom = 3 * n_res
+    ids = g.dsets["/input/potential/rama_coord/id"]
+    keep = np.array([-1 not in tuple(x) for x in ids])
+    atom = ids[keep].astype(np.int32)
+    g.ds("/input/pivot_moves/proposal_pot", g.dsets["/input/potential/rama_map_pot/rama_pot"])
+    g.ds("/input/pivot_moves/pivot_atom", atom)
+    g.ds("/input/pivot_moves/pivot_restype", g.dsets["/input/potential/rama_map_pot/rama_map_id_all"][keep])
+    g.ds("/input/pivot_moves/pivot_range",
+         np.column_stack((atom[:, 4] + 1, np.zeros(len(atom), "i") + n_atom)).astype(np.int32))
 
 
 def write_up(path, g):

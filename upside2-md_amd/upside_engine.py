@@ -66,6 +66,8 @@ def load_upside_library(path=LIB_PATH):
         "upside_batch_open_output": (i, [vp, vp, i]),
         "upside_batch_run_logged": (i, [vp, i, i]),
         "upside_batch_close_output": (i, [vp]),
+        "upside_batch_set_monte_carlo": (i, [vp, i]),
+        "upside_batch_mc_stats": (i, [vp, i, vp, vp]),
         "upside_batch_state_size": (ct.c_longlong, [vp]),
         "upside_batch_save_state": (i, [vp, vp, ct.c_longlong]),
         "upside_batch_load_state": (i, [vp, vp, ct.c_longlong]),
@@ -260,6 +262,18 @@ class Batch:
         """Advance n_round rounds, logging a frame at rounds that are multiples of frame_interval."""
         if self.calc.upside_batch_run_logged(self.h, int(n_round), int(frame_interval)):
             raise RuntimeError("run_logged failed (see stderr)")
+
+    def set_monte_carlo(self, mc_interval):
+        """Pivot moves every mc_interval rounds in run_logged (0: none)."""
+        if self.calc.upside_batch_set_monte_carlo(self.h, int(mc_interval)):
+            raise RuntimeError("set_monte_carlo failed (see stderr)")
+
+    def mc_stats(self, s):
+        """(accepted, attempted) pivot moves of system s."""
+        a, t = ct.c_longlong(0), ct.c_longlong(0)
+        if self.calc.upside_batch_mc_stats(self.h, s, ct.byref(a), ct.byref(t)):
+            raise RuntimeError("mc_stats failed")
+        return a.value, t.value
 
     def close_output(self):
         if self.calc.upside_batch_close_output(self.h):

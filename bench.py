@@ -373,6 +373,7 @@ def main():
         "bp_iters_per_step": iters / max(solves, 1),
         "bp_passes_per_solve": passes / max(solves, 1),
         "bp_iters_per_solve_p50_p99": [percentile_from_hist(hist, 0.5), percentile_from_hist(hist, 0.99)],
+        "bp_solves_in_lds_fraction": (st1["lds"] - st0["lds"]) / max(solves, 1),
         "bp_edges_3x3_3x6_6x6": [bp0[3:6], bp1[3:6]],
         "force_rel_err_vs_reference_at_end": checks["force_rel_err_vs_reference"] if checks else None,
         "checks_at_end": checks,

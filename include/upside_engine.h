@@ -112,6 +112,9 @@ int upside_batch_bp_stats(UpsideBatch* batch, int sys, int* last_iters, long lon
  * least 256; returns the number of bins written, -1 on error.  No counterpart in the reference's C ABI (its solver keeps
  * only n_bad_solve, src/rotamer.cpp:845-846). */
 int upside_batch_bp_histogram(UpsideBatch* batch, int sys, long long* counts, int n_bins);
+/* Solves since creation whose BP state was held in LDS (the rest ran from
+ * memory); no counterpart in the reference. */
+int upside_batch_bp_lds_solves(UpsideBatch* batch, int sys, long long* n);
 /* /output logging as the reference's simulation loop does at its basic log
  * level (src/main.cpp:705-736, :807-831, :893, :946-949; datasets as
  * src/state_logger.h:21-39 creates them): per system, an `output` group with

@@ -431,25 +431,31 @@ __device__ void rot_node_pass(Ctx& c, const BpMem<F, I, J>& m, int n_node, F ncu
 #pragma unroll
             for (int r = 0; r < 6; ++r) b[r] *= inv;
         };
-        // two incidences per step: both messages are loaded before the first
-        // factor is applied (the factors stay in incidence order)
+        // four incidences per step: their indices, then all their messages,
+        // are loaded before the first factor is applied (the factors stay in
+        // incidence order); in memory each incidence is two dependent L2 trips
         int t = 0;
-        for (; t + 1 < n; t += 2) {
-            int inc0 = m.inc[t0 + t], inc1 = m.inc[t0 + t + 1];
-            int base0 = ((inc0 & 1) * 6) * m.ms + (inc0 >> 1);
-            int base1 = ((inc1 & 1) * 6) * m.ms + (inc1 >> 1);
-            float a0[6], a1[6];
+        for (; t + 3 < n; t += 4) {
+            int base[4];
 #pragma unroll
-            for (int r = 0; r < 6; ++r) { a0[r] = m.msg[base0 + r * m.ms]; a1[r] = m.msg[base1 + r * m.ms]; }
-            factor(a0);
-            factor(a1);
+            for (int q = 0; q < 4; ++q) {
+                const int code = m.inc[t0 + t + q];
+                base[q] = ((code & 1) * 6) * m.ms + (code >> 1);
+            }
+            float a[4][6];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < 6; ++r) a[q][r] = m.msg[base[q] + r * m.ms];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) factor(a[q]);
         }
-        if (t < n) {
-            int inc = m.inc[t0 + t];
-            int base = ((inc & 1) * 6) * m.ms + (inc >> 1);
+        for (; t < n; ++t) {
+            const int code = m.inc[t0 + t];
+            const int b0 = ((code & 1) * 6) * m.ms + (code >> 1);
             float a0[6];
 #pragma unroll
-            for (int r = 0; r < 6; ++r) a0[r] = m.msg[base + r * m.ms];
+            for (int r = 0; r < 6; ++r) a0[r] = m.msg[b0 + r * m.ms];
             factor(a0);
         }
         if (swap_first) {
@@ -574,6 +580,7 @@ __device__ int rot_solve_wg(Ctx& c, const RotamerDev& R) {
     const int cap = v.cap;
     unsigned long long t0 = ptick(c);
     if (bp_lds_floats(nbp, n) <= LDS_FLOATS && 2 * nbp < 65536 && 36 * nbp <= 12 * cap) {
+        if (c.tid == 0) v.stats[6] += 1;  // solves held in LDS
         lp<float> L = lmem(c.lds);
         BpMem<lp<float>, lp<int>, lp<unsigned short>> m;
         m.nrot = v.nrot; m.prob = (gp<const float>)v.nprob;

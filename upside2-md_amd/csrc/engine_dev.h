@@ -19,7 +19,7 @@ namespace upside {
 constexpr int LDS_FLOATS = 40704;
 // Rotamer statistics words: [0] iterations of the last solve, [1] solves at
 // the iteration limit, [2] total iterations, [3] solves, [4] BP edges, [5]
-// residue-pair candidates, then a histogram of iterations per solve
+// residue-pair candidates, [6] solves held in LDS, then a histogram of iterations per solve
 // (bin = iterations, the last bin collects the rest).
 constexpr int BP_HIST = 8;
 constexpr int BP_HIST_BINS = 256;

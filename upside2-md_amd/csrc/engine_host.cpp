@@ -2701,6 +2701,21 @@ int upside_batch_bp_stats(UpsideBatch* b, int sys, int* last_iters, long long* t
     ABI_CATCH(1)
 }
 
+int upside_batch_bp_lds_solves(UpsideBatch* b, int sys, long long* n) {
+    ABI_TRY
+    auto& B = b->batch;
+    if (sys < 0 || sys >= (int)B.sys.size()) fail("bp_lds_solves: system out of range");
+    long long t = 0;
+    for (auto& R : B.sys[sys].prog->T.rots) {
+        int v = 0;
+        hip_check(hipMemcpy(&v, B.sys[sys].iarena + R.stats + 6, sizeof(int), hipMemcpyDeviceToHost), "stats");
+        t += v;
+    }
+    *n = t;
+    return 0;
+    ABI_CATCH(1)
+}
+
 int upside_batch_bp_histogram(UpsideBatch* b, int sys, long long* counts, int n_bins) {
     ABI_TRY
     auto& B = b->batch;

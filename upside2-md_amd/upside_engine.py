@@ -63,6 +63,7 @@ def load_upside_library(path=LIB_PATH):
         "upside_batch_bp_stats": (i, [vp, i, vp, vp, vp, vp]),
         "upside_batch_potential_terms": (i, [vp, i, vp, i, ct.c_char_p, i]),
         "upside_batch_bp_histogram": (i, [vp, i, vp, i]),
+        "upside_batch_bp_lds_solves": (i, [vp, i, vp]),
         "upside_batch_open_output": (i, [vp, vp, i]),
         "upside_batch_run_logged": (i, [vp, i, i]),
         "upside_batch_close_output": (i, [vp]),
@@ -246,7 +247,10 @@ class Batch:
         tot, sol, bad = ct.c_longlong(0), ct.c_longlong(0), ct.c_longlong(0)
         if self.calc.upside_batch_bp_stats(self.h, s, ct.byref(last), ct.byref(tot), ct.byref(sol), ct.byref(bad)):
             raise RuntimeError("bp stats failed")
-        return dict(last=last.value, total=tot.value, solves=sol.value, bad=bad.value)
+        lds = ct.c_longlong(0)
+        if self.calc.upside_batch_bp_lds_solves(self.h, s, ct.byref(lds)):
+            raise RuntimeError("bp stats failed")
+        return dict(last=last.value, total=tot.value, solves=sol.value, bad=bad.value, lds=lds.value)
 
     def open_output(self, paths=None, record_momentum=False):
         """Start /output logging: into each system's input .up (paths None) or

@@ -96,6 +96,11 @@ int upside_batch_energies(UpsideBatch* batch, float* energies);
  * drawn from Threefry stream 1 with counter `round`.  accepted[n_pairs] out. */
 int upside_batch_replica_exchange(UpsideBatch* batch, int n_pairs, const int* pairs, uint64_t round,
                                   int* accepted);
+/* The same with the reference's --exchange-criterion (src/main.cpp:278-311):
+ * 0 as above; 1 compares the log Boltzmann weights at the current positions,
+ * exp(-b2 E2 + b1 E1), and swaps when the test passes. */
+int upside_batch_replica_exchange_criterion(UpsideBatch* batch, int n_pairs, const int* pairs, uint64_t round,
+                                            int criterion, int* accepted);
 /* Per-term potentials of system `sys` at the last evaluation (each potential
  * node's `potential`, summed into the total in src/deriv_engine.cpp:240-250): values[slot] for
  * slot < max_terms, node names '\n'-separated into names.  Returns the number

@@ -228,6 +228,28 @@ def test_replica_exchange(ue, chain_file):
         assert a == want, (s1, s2, diff)
         np.testing.assert_array_equal(b.pos(s1), pos_before[s2] if a else pos_before[s1])
         np.testing.assert_array_equal(b.pos(s2), pos_before[s1] if a else pos_before[s2])
+    # exchange criterion 1 (src/main.cpp:298-311): test on the current weights only
+    b.run(3)
+    pos_before = [b.pos(s) for s in range(8)]
+    e_before = b.energies()
+    pairs = np.array([[1, 2], [3, 4], [5, 6]])
+    acc = b.replica_exchange(pairs, 4, criterion=1)
+    ctr = 0
+    n_acc = 0
+    for (s1, s2), a in zip(pairs, acc):
+        diff = np.float32(-beta[s2] * e_before[s2]) - np.float32(-beta[s1] * e_before[s1])
+        want = 1
+        if diff < 0:
+            bits = O.threefry4x32_20(np.array([[4, 0, 0, ctr]], np.uint32), key)[0]
+            ctr += 1
+            u = np.float32(bits[0]) * np.float32(2.0 ** -32) + np.float32(2.0 ** -33)
+            want = 1 if np.exp(diff) >= u else 0
+        assert a == want, (s1, s2, diff)
+        n_acc += a
+        np.testing.assert_array_equal(b.pos(s1), pos_before[s2] if a else pos_before[s1])
+    say("criterion 1: accepted %d of %d" % (n_acc, len(pairs)))
+    with pytest.raises(RuntimeError):
+        b.replica_exchange(pairs, 5, criterion=2)
 
 
 def test_thermostat_temperature(ue, chig):

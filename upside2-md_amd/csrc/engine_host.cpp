@@ -2606,8 +2606,10 @@ int upside_batch_energies(UpsideBatch* b, float* energies) {
     ABI_CATCH(1)
 }
 
-int upside_batch_replica_exchange(UpsideBatch* b, int n_pairs, const int* pairs, uint64_t round, int* accepted) {
+int upside_batch_replica_exchange_criterion(UpsideBatch* b, int n_pairs, const int* pairs, uint64_t round,
+                                            int criterion, int* accepted) {
     ABI_TRY
+    if (criterion != 0 && criterion != 1) fail("replica exchange: exchange criterion must be 0 or 1");
     auto& B = b->batch;
     int ns = (int)B.sys.size();
     vector<float> beta(ns);
@@ -2632,28 +2634,44 @@ int upside_batch_replica_exchange(UpsideBatch* b, int n_pairs, const int* pairs,
     for (int p = 0; p < n_pairs; ++p)
         if (pairs[2 * p] < 0 || pairs[2 * p] >= ns || pairs[2 * p + 1] < 0 || pairs[2 * p + 1] >= ns)
             fail("swap pair out of range");
-    auto old_lb = lboltz();
-    for (int p = 0; p < n_pairs; ++p) swap_pos(pairs[2 * p], pairs[2 * p + 1]);
-    auto new_lb = lboltz();
-    // RandomGenerator(seed, REPLICA_EXCHANGE_RANDOM_STREAM, 0, round), one uniform per test
+    // RandomGenerator(seed, REPLICA_EXCHANGE_RANDOM_STREAM, 0, round), one uniform per test that needs it
     uint32_t key[4] = {B.base_seed, 1u, 0u, 0u};
     uint32_t ctr[4] = {(uint32_t)(round & 0xffffffffull), (uint32_t)(round >> 32), 0u, 0u};
-    for (int p = 0; p < n_pairs; ++p) {
-        int s1 = pairs[2 * p], s2 = pairs[2 * p + 1];
-        float diff = (new_lb[s1] + new_lb[s2]) - (old_lb[s1] + old_lb[s2]);
-        bool reject = false;
-        if (diff < 0.f) {
-            uint32_t bits[4];
-            threefry_host(ctr, key, bits);
-            ctr[3]++;
-            float u = (float)bits[0] * 2.3283064365386962890625e-10f + 1.16415321826934814453125e-10f;
-            reject = std::exp(diff) < u;
+    auto uniform = [&]() {
+        uint32_t bits[4];
+        threefry_host(ctr, key, bits);
+        ctr[3]++;
+        return u01_open_closed(bits[0]);
+    };
+    if (criterion == 0) {
+        // full change of log Boltzmann weight: swap all, re-evaluate, undo rejected (:278-297)
+        auto old_lb = lboltz();
+        for (int p = 0; p < n_pairs; ++p) swap_pos(pairs[2 * p], pairs[2 * p + 1]);
+        auto new_lb = lboltz();
+        for (int p = 0; p < n_pairs; ++p) {
+            int s1 = pairs[2 * p], s2 = pairs[2 * p + 1];
+            float diff = (new_lb[s1] + new_lb[s2]) - (old_lb[s1] + old_lb[s2]);
+            const bool reject = diff < 0.f && std::exp(diff) < uniform();
+            if (reject) swap_pos(s1, s2);
+            if (accepted) accepted[p] = reject ? 0 : 1;
         }
-        if (reject) swap_pos(s1, s2);
-        if (accepted) accepted[p] = reject ? 0 : 1;
+    } else {
+        // log Boltzmann weights of the current positions only (:298-311)
+        auto lb = lboltz();
+        for (int p = 0; p < n_pairs; ++p) {
+            int s1 = pairs[2 * p], s2 = pairs[2 * p + 1];
+            float diff = lb[s2] - lb[s1];
+            const bool accept = diff >= 0.f || std::exp(diff) >= uniform();
+            if (accept) swap_pos(s1, s2);
+            if (accepted) accepted[p] = accept ? 1 : 0;
+        }
     }
     return 0;
     ABI_CATCH(1)
+}
+
+int upside_batch_replica_exchange(UpsideBatch* b, int n_pairs, const int* pairs, uint64_t round, int* accepted) {
+    return upside_batch_replica_exchange_criterion(b, n_pairs, pairs, round, 0, accepted);
 }
 
 int upside_batch_potential_terms(UpsideBatch* b, int sys, float* values, int max_terms, char* names, int names_len) {

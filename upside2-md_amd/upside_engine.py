@@ -62,6 +62,7 @@ def load_upside_library(path=LIB_PATH):
         "upside_batch_replica_exchange": (i, [vp, i, vp, ct.c_uint64, vp]),
         "upside_batch_bp_stats": (i, [vp, i, vp, vp, vp, vp]),
         "upside_batch_potential_terms": (i, [vp, i, vp, i, ct.c_char_p, i]),
+        "upside_batch_replica_exchange_criterion": (i, [vp, i, vp, ct.c_uint64, i, vp]),
         "upside_batch_bp_histogram": (i, [vp, i, vp, i]),
         "upside_batch_bp_lds_solves": (i, [vp, i, vp]),
         "upside_batch_open_output": (i, [vp, vp, i]),
@@ -234,11 +235,12 @@ class Batch:
             raise RuntimeError("energies failed")
         return e
 
-    def replica_exchange(self, pairs, round_num):
+    def replica_exchange(self, pairs, round_num, criterion=0):
+        """One swap attempt per pair (exchange criterion 0 or 1); returns accepted flags."""
         pairs = np.require(np.asarray(pairs, dtype=np.intc).reshape(-1, 2), requirements="C")
         acc = np.zeros(len(pairs), np.intc)
-        if self.calc.upside_batch_replica_exchange(self.h, len(pairs), pairs.ctypes.data, int(round_num),
-                                                   acc.ctypes.data):
+        if self.calc.upside_batch_replica_exchange_criterion(self.h, len(pairs), pairs.ctypes.data, int(round_num),
+                                                             int(criterion), acc.ctypes.data):
             raise RuntimeError("replica exchange failed")
         return acc
 

@@ -37,7 +37,9 @@ int evaluate_deriv(float* deriv, DerivEngine* engine, const float* pos);
  * interaction table of pair-graph nodes.  get_param_deriv (the reference's
  * PARAM_DERIV build, src/interaction_graph.h:406-416) returns d(potential)/d(table)
  * at the last evaluation for the spline pair graphs rotamer, hbond_coverage and
- * hbond_coverage_hydrophobe, and nonzero for other nodes.  set_param replaces
+ * hbond_coverage_hydrophobe, zeros for the environment-coverage graphs (as the
+ * reference, src/environment.cpp:164-167, :222-225), and nonzero for nodes the
+ * reference gives no derivative (protein_hbond, backbone_pairs).  set_param replaces
  * the table of rotamer, hbond_coverage(_hydrophobe) and protein_hbond (fixed
  * pair-list cutoffs; a symmetric table must match its transpose as
  * src/interaction_graph.h:385-396 requires) and returns nonzero for others. */

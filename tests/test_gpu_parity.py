@@ -322,7 +322,8 @@ def test_capacity_regrowth_is_transparent(ue, chain_file, monkeypatch):
         np.testing.assert_array_equal(b1.pos(s), b0.pos(s))
 
 
-@pytest.mark.parametrize("name", ["rotamer", "hbond_coverage", "hbond_coverage_hydrophobe"])
+@pytest.mark.parametrize("name", ["rotamer", "hbond_coverage", "hbond_coverage_hydrophobe",
+                                  "environment_coverage_sc", "protein_hbond"])
 def test_param_deriv_matches_reference(ue, chain_file, name):
     """get_param_deriv of the spline pair graphs against the reference engine
     built from its sources with PARAM_DERIV (oracle/_ref), on the same
@@ -345,12 +346,21 @@ def test_param_deriv_matches_reference(ue, chain_file, name):
     ref = RefEngine(lib, path, len(pos0))
     eng.deriv(pos)
     ref.deriv(pos)
-    got = eng.get_param_deriv(shape, name)
     want = np.zeros(shape, np.float32)
-    assert lib.get_param_deriv(want.size, want.ctypes.data, ct.c_void_p(ref.h), name.encode()) == 0
-    err = float(np.linalg.norm(got - want) / np.linalg.norm(want))
+    rc = lib.get_param_deriv(want.size, want.ctypes.data, ct.c_void_p(ref.h), name.encode())
+    if name == "protein_hbond":          # no derivative hook in the reference: an error in both
+        assert rc != 0
+        with pytest.raises(RuntimeError):
+            eng.get_param_deriv(shape, name)
+        return
+    assert rc == 0
+    got = eng.get_param_deriv(shape, name)
+    err = float(np.linalg.norm(got - want) / max(np.linalg.norm(want), 1e-30))
     say("%s: |dE/dparam| %.4g, rel err %.2e, nonzero %d of %d" % (name, np.linalg.norm(want), err,
                                                                    np.count_nonzero(want), want.size))
+    if name.startswith("environment"):   # the reference has no derivative for these: zeros
+        np.testing.assert_array_equal(got, want)
+        return
     assert np.linalg.norm(want) > 0
     assert err < 1e-4, err
     np.testing.assert_array_equal(eng.get_param_deriv(shape, name), got)

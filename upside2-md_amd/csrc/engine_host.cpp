@@ -2191,16 +2191,25 @@ int get_param_deriv(int n_param, float* deriv, DerivEngine* engine, const char* 
     for (size_t g = 0; g < T.graphs.size(); ++g) {
         if (&T.nodes[T.graph_node[g]] != n) continue;
         const PairGraph& G = T.graphs[g];
-        if (G.kind != EDGE_ROTAMER && G.kind != EDGE_HBOND_COVERAGE)
+        // protein_hbond and backbone_pairs have no hook in the reference (its
+        // C ABI then reports a size mismatch): an error here too
+        if (G.kind != EDGE_ROTAMER && G.kind != EDGE_HBOND_COVERAGE && G.kind != EDGE_ENV_WEIGHTED &&
+            G.kind != EDGE_ENV_HBOND)
             fail(string("get_param_deriv: no parameter derivative for node ") + node_name);
-        const int spline_params = G.kind == EDGE_ROTAMER ? 2 * 15 + 2 * 12 : 2 * 15 + 2 * 10;
-        if (G.n_param != spline_params)
-            fail(string("get_param_deriv: ") + node_name + " has " + std::to_string(G.n_param) +
-                 " parameters per type pair, the spline needs " + std::to_string(spline_params));
         const int rows = G.n_type1 * G.n_type2;
         const int size = rows * G.n_param;
         if (size != n_param)
             fail("Wrong number of parameters, expected " + std::to_string(size) + " but got " + std::to_string(n_param));
+        if (G.kind == EDGE_ENV_WEIGHTED || G.kind == EDGE_ENV_HBOND) {
+            // the reference's environment-coverage interactions have no
+            // parameter derivative: zeros (src/environment.cpp:164-167, :222-225)
+            std::fill(deriv, deriv + size, 0.f);
+            return 0;
+        }
+        const int spline_params = G.kind == EDGE_ROTAMER ? 2 * 15 + 2 * 12 : 2 * 15 + 2 * 10;
+        if (G.n_param != spline_params)
+            fail(string("get_param_deriv: ") + node_name + " has " + std::to_string(G.n_param) +
+                 " parameters per type pair, the spline needs " + std::to_string(spline_params));
         const System& s = B.sys[0];
         const Program& P = *s.prog;
         ParamDerivArgs a;

@@ -38,8 +38,10 @@ int evaluate_deriv(float* deriv, DerivEngine* engine, const float* pos);
  * PARAM_DERIV build, src/interaction_graph.h:406-416) returns d(potential)/d(table)
  * at the last evaluation for the spline pair graphs rotamer, hbond_coverage and
  * hbond_coverage_hydrophobe, zeros for the environment-coverage graphs (as the
- * reference, src/environment.cpp:164-167, :222-225), and nonzero for nodes the
- * reference gives no derivative (protein_hbond, backbone_pairs).  set_param replaces
+ * reference, src/environment.cpp:164-167, :222-225), the scale, center,
+ * sharpness and weight derivatives of sigmoid_coupling and the scale
+ * derivative of bb_sigmoid_coupling (:958-997, :1118-1134), and nonzero for
+ * nodes the reference gives no derivative (protein_hbond, backbone_pairs).  set_param replaces
  * the table of rotamer, hbond_coverage(_hydrophobe) and protein_hbond (fixed
  * pair-list cutoffs; a symmetric table must match its transpose as
  * src/interaction_graph.h:385-396 requires) and returns nonzero for others. */

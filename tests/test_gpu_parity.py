@@ -435,6 +435,34 @@ def test_mc_pivot_matches_reference_cli(ue, chain_file, tmp_path):
         assert e < 1e-4 and de < 1e-4, (k, e, de)
 
 
+@pytest.mark.parametrize("name,size", [("sigmoid_coupling_environment", 3 * 20 + 400),
+                                       ("bb_sigmoid_coupling_environment", 4)])
+def test_coupling_param_deriv_matches_reference(ue, chain_file, name, size):
+    """get_param_deriv of the sigmoid coupling nodes (scale, center, sharpness
+    and weights) against the reference's PARAM_DERIV build."""
+    import ctypes as ct
+    lib = ref_lib()
+    if lib is None:
+        pytest.skip("reference engine not built")
+    lib.get_param_deriv.argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_char_p]
+    path = chain_file(60, 7)
+    pos0 = read_pos(path)
+    pos = (pos0 + np.random.default_rng(5).normal(scale=0.1, size=pos0.shape)).astype(np.float32)
+    say("case: coupling param deriv %s" % name)
+    eng = ue.Upside(path, quiet=True)
+    ref = RefEngine(lib, path, len(pos0))
+    eng.deriv(pos)
+    ref.deriv(pos)
+    want = np.zeros(size, np.float32)
+    assert lib.get_param_deriv(size, want.ctypes.data, ct.c_void_p(ref.h), name.encode()) == 0
+    got = eng.get_param_deriv((size,), name)
+    err = float(np.linalg.norm(got - want) / max(np.linalg.norm(want), 1e-30))
+    say("%s: |dE/dparam| %.4g rel err %.2e nonzero %d" % (name, np.linalg.norm(want), err, np.count_nonzero(want)))
+    assert np.linalg.norm(want) > 0 and err < 1e-4, err
+    with pytest.raises(RuntimeError):
+        eng.get_param_deriv((size + 1,), name)
+
+
 def test_output_logger(ue, chain_file, tmp_path):
     """/output datasets as the reference's basic log level writes them: frames
     at rounds 0, 3, 6 of a 7-round run hold the state before that round's

@@ -111,6 +111,23 @@ struct ParamDerivArgs {
     float* out;            // (n_type1*n_type2, n_param)
 };
 
+// Parameter derivative of a sigmoid coupling node after an evaluation
+// (one workgroup; one output entry per thread at a time).
+struct CouplingDerivArgs {
+    int kind;              // OP_SIGMOID_COUPLING or OP_BB_SIGMOID_COUPLING
+    int n_res, nt, n_indep;
+    const float* cov;      // coverage per residue and amino-acid type [n_res*nt]
+    const float* cov_hb;   // backbone coupling: hbond coverage per residue [n_res]
+    const int* ctype;      // sigmoid coupling: coupling type per residue
+    const float* weights;  // [nt*nt] (sigmoid) or [nt] (backbone)
+    const float* scale;    // [nt] (sigmoid)
+    const float* center;   // [nt] (sigmoid)
+    const float* sharp;    // [nt] (sigmoid)
+    float bb_center, bb_sharp, bb_hbond_weight;
+    int n_out;
+    float* out;
+};
+
 // Rotamer belief propagation over residue nodes (src/rotamer.cpp:628-1169).
 // Each residue node has one bead per rotamer state, listed in state order.
 struct RotamerDev {

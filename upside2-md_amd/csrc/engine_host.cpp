@@ -29,6 +29,7 @@
 
 hipError_t upside_launch_engine(upside::SysDesc* d_systems, int n_sys, upside::RunArgs args, hipStream_t stream);
 hipError_t upside_launch_param_deriv(const upside::ParamDerivArgs& args, int n_rows, hipStream_t stream);
+hipError_t upside_launch_coupling_param_deriv(const upside::CouplingDerivArgs& args, hipStream_t stream);
 int upside_max_resident_blocks();
 
 namespace upside {
@@ -589,6 +590,7 @@ struct Builder {
             o.ia[4] = T.cf_push(h5.rf(g + "/center"));
             o.ia[5] = T.cf_push(h5.rf(g + "/sharpness"));
             o.ia[6] = nt;
+            o.ia[7] = h5.attr_i(g, "number_independent_weights", 1, false);  // for get_param_deriv only
         } else if (t == "bb_sigmoid_coupling") {
             need_args(n, 2);
             n.potential = true;
@@ -2188,6 +2190,50 @@ int get_param_deriv(int n_param, float* deriv, DerivEngine* engine, const char* 
     if (!engine->evaluated) fail("get_param_deriv: evaluate the engine first");
     const Topology& T = B.sys[0].prog->T;
     const Node* n = B.node(0, node_name);
+    const int op_kind = n->op >= 0 ? T.ops[n->op].kind : -1;
+    if (op_kind == OP_SIGMOID_COUPLING || op_kind == OP_BB_SIGMOID_COUPLING) {
+        const OpRec& op = T.ops[n->op];
+        const System& s = B.sys[0];
+        const Program& P = *s.prog;
+        CouplingDerivArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.kind = op.kind;
+        const Node& p0 = T.nodes[n->parents[0]];
+        a.cov = s.arena + p0.out;
+        if (op.kind == OP_SIGMOID_COUPLING) {
+            a.nt = op.ia[6];
+            a.n_res = op.ia[0];
+            a.n_indep = op.ia[7];
+            if (a.n_indep != 1 && a.n_indep != 20 && a.n_indep != 400)
+                fail("the number of independent weights should be 1, 20 or 400");
+            a.ctype = P.d_ci + op.ia[1];
+            a.weights = P.d_cf + op.ia[2];
+            a.scale = P.d_cf + op.ia[3];
+            a.center = P.d_cf + op.ia[4];
+            a.sharp = P.d_cf + op.ia[5];
+            a.n_out = 3 * a.nt + a.nt * a.nt;
+        } else {
+            a.nt = op.ia[0];
+            a.n_res = op.ia[2];
+            a.weights = P.d_cf + op.ia[1];
+            a.cov_hb = s.arena + T.nodes[n->parents[1]].out;
+            a.bb_center = op.fa[1];
+            a.bb_sharp = op.fa[2];
+            a.bb_hbond_weight = op.fa[3];
+            a.n_out = 4;
+        }
+        if (n_param != a.n_out)
+            fail("Wrong number of parameters, expected " + std::to_string(a.n_out) + " but got " + std::to_string(n_param));
+        float* d_out = nullptr;
+        hip_check(hipMalloc((void**)&d_out, (size_t)a.n_out * sizeof(float)), "param deriv");
+        a.out = d_out;
+        hipError_t e = upside_launch_coupling_param_deriv(a, B.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(B.stream);
+        if (e == hipSuccess) e = hipMemcpy(deriv, d_out, (size_t)a.n_out * sizeof(float), hipMemcpyDeviceToHost);
+        (void)hipFree(d_out);
+        hip_check(e, "param deriv");
+        return 0;
+    }
     for (size_t g = 0; g < T.graphs.size(); ++g) {
         if (&T.nodes[T.graph_node[g]] != n) continue;
         const PairGraph& G = T.graphs[g];

@@ -1266,7 +1266,60 @@ __global__ void __launch_bounds__(NT) param_deriv_kernel(ParamDerivArgs a) {
     }
 }
 
+// Sigmoid coupling (SigmoidCoupling::get_param_deriv,
+// src/environment.cpp:958-997): entries [0,nt) d/dscale, [nt,2nt) d/dcenter,
+// [2nt,3nt) d/dsharpness per coupling type, then the weight block when the
+// node has independent weights (400: per coupling type and amino acid; 20: per
+// amino acid, repeated).  Backbone sigmoid coupling
+// (BackboneSigmoidCoupling::get_param_deriv, :1118-1134): only d/dscale.
+// Each thread owns output entries and sums over residues in residue order.
+__global__ void __launch_bounds__(NT) coupling_param_deriv_kernel(CouplingDerivArgs a) {
+    const int nt = a.nt;
+    for (int j = threadIdx.x; j < a.n_out; j += NT) {
+        float acc = 0.f;
+        if (a.kind == OP_BB_SIGMOID_COUPLING) {
+            if (j == 0)
+                for (int r = 0; r < a.n_res; ++r) {
+                    float bl = 0.f;
+                    for (int aa = 0; aa < nt; ++aa) bl += a.weights[aa] * a.cov[r * nt + aa];
+                    bl += a.bb_hbond_weight * a.cov_hb[r];
+                    float v, d;
+                    compact_sigmoid(bl - a.bb_center, a.bb_sharp, v, d);
+                    acc += v;
+                }
+        } else {
+            const int cs = 3 * nt;
+            const int q = j < cs ? j / nt : 3;
+            const int want_ct = j < cs ? j % nt : (a.n_indep == 400 ? (j - cs) / nt : -1);
+            const int want_aa = j < cs ? -1 : (j - cs) % nt;
+            if (j < cs || a.n_indep > 1)
+                for (int r = 0; r < a.n_res; ++r) {
+                    const int ct = a.ctype[r];
+                    if (want_ct >= 0 && ct != want_ct) continue;
+                    float w = 0.f;
+                    for (int aa = 0; aa < nt; ++aa) w += a.weights[ct * nt + aa] * a.cov[r * nt + aa];
+                    const float dc = w - a.center[ct];
+                    float v, d;
+                    compact_sigmoid(dc, a.sharp[ct], v, d);
+                    if (q == 0) acc += v;
+                    else if (q == 1) acc += -a.scale[ct] * d;
+                    else if (q == 2) acc += a.scale[ct] * dc * d / a.sharp[ct];
+                    else acc += (d * a.scale[ct]) * a.cov[r * nt + want_aa];
+                }
+        }
+        a.out[j] = acc;
+    }
+}
+
 }  // namespace upside
+
+hipError_t upside_launch_coupling_param_deriv(const upside::CouplingDerivArgs& args, hipStream_t stream) {
+    if (args.n_out <= 0 || args.nt <= 0) return hipErrorInvalidValue;
+    upside::CouplingDerivArgs a = args;
+    void* kargs[] = {&a};
+    return hipLaunchKernel((const void*)upside::coupling_param_deriv_kernel, dim3(1), dim3(upside::NT), kargs, 0,
+                           stream);
+}
 
 hipError_t upside_launch_param_deriv(const upside::ParamDerivArgs& args, int n_rows, hipStream_t stream) {
     const size_t lds_bytes = (size_t)upside::NT * args.n_param * sizeof(float);

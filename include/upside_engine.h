@@ -40,8 +40,9 @@ int evaluate_deriv(float* deriv, DerivEngine* engine, const float* pos);
  * hbond_coverage_hydrophobe, zeros for the environment-coverage graphs (as the
  * reference, src/environment.cpp:164-167, :222-225), the scale, center,
  * sharpness and weight derivatives of sigmoid_coupling and the scale
- * derivative of bb_sigmoid_coupling (:958-997, :1118-1134), and nonzero for
- * nodes the reference gives no derivative (protein_hbond, backbone_pairs).  set_param replaces
+ * derivative of bb_sigmoid_coupling (:958-997, :1118-1134), hbond_energy's
+ * (src/hbond.cpp:655-720), and nonzero for nodes the reference gives no
+ * derivative (protein_hbond, backbone_pairs).  set_param replaces
  * the table of rotamer, hbond_coverage(_hydrophobe) and protein_hbond (fixed
  * pair-list cutoffs; a symmetric table must match its transpose as
  * src/interaction_graph.h:385-396 requires) and returns nonzero for others. */

@@ -436,10 +436,11 @@ def test_mc_pivot_matches_reference_cli(ue, chain_file, tmp_path):
 
 
 @pytest.mark.parametrize("name,size", [("sigmoid_coupling_environment", 3 * 20 + 400),
-                                       ("bb_sigmoid_coupling_environment", 4)])
+                                       ("bb_sigmoid_coupling_environment", 4), ("hbond_energy", 12)])
 def test_coupling_param_deriv_matches_reference(ue, chain_file, name, size):
     """get_param_deriv of the sigmoid coupling nodes (scale, center, sharpness
-    and weights) against the reference's PARAM_DERIV build."""
+    and weights) and of hbond_energy against the reference's PARAM_DERIV
+    build."""
     import ctypes as ct
     lib = ref_lib()
     if lib is None:

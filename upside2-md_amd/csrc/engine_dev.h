@@ -124,6 +124,16 @@ struct CouplingDerivArgs {
     const float* center;   // [nt] (sigmoid)
     const float* sharp;    // [nt] (sigmoid)
     float bb_center, bb_sharp, bb_hbond_weight;
+    // hbond energy: protein_hbond output (hbond value in column 6) and Rama
+    // output rows, residue lists of hbond sites (CSR), the 12 parameters
+    const float* pp;
+    const float* rama;
+    int wp, wr;
+    const int* h1o;
+    const int* h1i;
+    const int* h2o;
+    const int* h2i;
+    float hp[12];
     int n_out;
     float* out;
 };

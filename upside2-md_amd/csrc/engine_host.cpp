@@ -2191,7 +2191,7 @@ int get_param_deriv(int n_param, float* deriv, DerivEngine* engine, const char* 
     const Topology& T = B.sys[0].prog->T;
     const Node* n = B.node(0, node_name);
     const int op_kind = n->op >= 0 ? T.ops[n->op].kind : -1;
-    if (op_kind == OP_SIGMOID_COUPLING || op_kind == OP_BB_SIGMOID_COUPLING) {
+    if (op_kind == OP_SIGMOID_COUPLING || op_kind == OP_BB_SIGMOID_COUPLING || op_kind == OP_HBOND_ENERGY) {
         const OpRec& op = T.ops[n->op];
         const System& s = B.sys[0];
         const Program& P = *s.prog;
@@ -2200,7 +2200,21 @@ int get_param_deriv(int n_param, float* deriv, DerivEngine* engine, const char* 
         a.kind = op.kind;
         const Node& p0 = T.nodes[n->parents[0]];
         a.cov = s.arena + p0.out;
-        if (op.kind == OP_SIGMOID_COUPLING) {
+        if (op.kind == OP_HBOND_ENERGY) {
+            const Node& p1 = T.nodes[n->parents[1]];
+            a.nt = 1;
+            a.n_res = op.ia[0];
+            a.pp = s.arena + p0.out;
+            a.wp = p0.width;
+            a.rama = s.arena + p1.out;
+            a.wr = p1.width;
+            a.h1o = P.d_ci + op.ia[1];
+            a.h1i = P.d_ci + op.ia[2];
+            a.h2o = P.d_ci + op.ia[3];
+            a.h2i = P.d_ci + op.ia[4];
+            for (int k = 0; k < 12; ++k) a.hp[k] = op.fa[k];
+            a.n_out = 12;
+        } else if (op.kind == OP_SIGMOID_COUPLING) {
             a.nt = op.ia[6];
             a.n_res = op.ia[0];
             a.n_indep = op.ia[7];

@@ -1277,7 +1277,37 @@ __global__ void __launch_bounds__(NT) coupling_param_deriv_kernel(CouplingDerivA
     const int nt = a.nt;
     for (int j = threadIdx.x; j < a.n_out; j += NT) {
         float acc = 0.f;
-        if (a.kind == OP_BB_SIGMOID_COUPLING) {
+        if (a.kind == OP_HBOND_ENERGY) {
+            // HBondEnergy::get_param_deriv (src/hbond.cpp:655-720): [0..2] hbonds of
+            // each residue times its helix, sheet and turn scores, [3] the bias
+            // term of each peptide; the shape parameters [4..11] stay zero
+            const float* P = a.hp;
+            if (j < 3) {
+                for (int i = 0; i < a.n_res; ++i) {
+                    const float phi = a.rama[i * a.wr], psi = a.rama[i * a.wr + 1];
+                    float p1, d1, p2, d2, s1, e1, s2, e2;
+                    compact_sigmoid(phi - P[4], P[5], p1, d1);
+                    compact_sigmoid(phi - P[6], P[7], p2, d2);
+                    compact_sigmoid(psi - P[8], P[9], s1, e1);
+                    compact_sigmoid(psi - P[10], P[11], s2, e2);
+                    const float turn = (float)((1. - (double)p1) * p2);
+                    const float psis = (float)((1. - (double)s1) * s2);
+                    const float nturn = (float)(1. - (double)turn), npsi = (float)(1. - (double)psis);
+                    const float score = j == 0 ? nturn * psis : (j == 1 ? nturn * npsi : turn);
+                    float hb = 0.f;
+                    for (int k = a.h1o[i]; k < a.h1o[i + 1]; ++k) hb += a.pp[a.h1i[k] * a.wp + 6];
+                    acc += hb * score;
+                }
+            } else if (j == 3) {
+                for (int i = 0; i + 1 < a.n_res; ++i) {
+                    float hb = 0.f;
+                    for (int k = a.h2o[i]; k < a.h2o[i + 1]; ++k) hb += a.pp[a.h2i[k] * a.wp + 6];
+                    float v, d;
+                    compact_sigmoid(1.5f - hb, 2.0f, v, d);
+                    acc += v;
+                }
+            }
+        } else if (a.kind == OP_BB_SIGMOID_COUPLING) {
             if (j == 0)
                 for (int r = 0; r < a.n_res; ++r) {
                     float bl = 0.f;

@@ -33,19 +33,22 @@ void free_deriv_engine(DerivEngine* engine);
 int evaluate_energy(float* energy, DerivEngine* engine, const float* pos);
 /* src/engine_c_library.cpp:49-65: dE/dx at pos, (n_atom,3). */
 int evaluate_deriv(float* deriv, DerivEngine* engine, const float* pos);
-/* src/engine_c_library.cpp:68-110: node parameter vectors.  get_param returns the
- * interaction table of pair-graph nodes.  get_param_deriv (the reference's
- * PARAM_DERIV build, src/interaction_graph.h:406-416) returns d(potential)/d(table)
- * at the last evaluation for the spline pair graphs rotamer, hbond_coverage and
+/* src/engine_c_library.cpp:68-110: node parameter vectors, in the reference's
+ * layouts: pair-graph interaction tables, sigmoid_coupling (scale, center,
+ * sharpness per type, then weights), bb_sigmoid_coupling (scale, center,
+ * sharpness, hbond weight) and hbond_energy (12 parameters)
+ * (src/environment.cpp:925-955, :1102-1115, src/hbond.cpp:653, :722-737).
+ * set_param accepts tables only where the pair-list cutoff does not depend on
+ * them (rotamer, hbond_coverage(_hydrophobe), protein_hbond; a symmetric table
+ * must match its transpose, src/interaction_graph.h:385-396).
+ * get_param_deriv (the reference's PARAM_DERIV build,
+ * src/interaction_graph.h:406-416) returns d(potential)/d(parameters) at the
+ * last evaluation for the spline pair graphs rotamer, hbond_coverage and
  * hbond_coverage_hydrophobe, zeros for the environment-coverage graphs (as the
- * reference, src/environment.cpp:164-167, :222-225), the scale, center,
- * sharpness and weight derivatives of sigmoid_coupling and the scale
- * derivative of bb_sigmoid_coupling (:958-997, :1118-1134), hbond_energy's
- * (src/hbond.cpp:655-720), and nonzero for nodes the reference gives no
- * derivative (protein_hbond, backbone_pairs).  set_param replaces
- * the table of rotamer, hbond_coverage(_hydrophobe) and protein_hbond (fixed
- * pair-list cutoffs; a symmetric table must match its transpose as
- * src/interaction_graph.h:385-396 requires) and returns nonzero for others. */
+ * reference, src/environment.cpp:164-167, :222-225), the derivatives of
+ * sigmoid_coupling, bb_sigmoid_coupling (:958-997, :1118-1134) and
+ * hbond_energy (src/hbond.cpp:655-720), and nonzero for nodes the reference
+ * gives no derivative (protein_hbond, backbone_pairs). */
 int set_param(int n_param, const float* param, DerivEngine* engine, const char* node_name);
 int get_param_deriv(int n_param, float* deriv, DerivEngine* engine, const char* node_name);
 int get_param(int n_param, float* param, DerivEngine* engine, const char* node_name);

@@ -462,6 +462,22 @@ def test_coupling_param_deriv_matches_reference(ue, chain_file, name, size):
     assert np.linalg.norm(want) > 0 and err < 1e-4, err
     with pytest.raises(RuntimeError):
         eng.get_param_deriv((size + 1,), name)
+    # get_param equals the reference's; after the same set_param in both
+    # engines, energies and forces agree
+    lib.get_param.argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_char_p]
+    lib.set_param.argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_char_p]
+    p_ref = np.zeros(size, np.float32)
+    assert lib.get_param(size, p_ref.ctypes.data, ct.c_void_p(ref.h), name.encode()) == 0
+    p_ours = eng.get_param((size,), name)
+    np.testing.assert_array_equal(p_ours, p_ref)
+    new = (p_ref * (1.0 + 0.05 * np.random.default_rng(6).standard_normal(size))).astype(np.float32)
+    eng.set_param(new, name)
+    assert lib.set_param(size, new.ctypes.data, ct.c_void_p(ref.h), name.encode()) == 0
+    np.testing.assert_array_equal(eng.get_param((size,), name), new)
+    e, d = eng.energy(pos), eng.deriv(pos)
+    er, dr = ref.energy(pos), ref.deriv(pos)
+    say("%s after set_param: E %.5f ref %.5f, force rel %.2e" % (name, e, er, rel_rms(d, dr)))
+    assert abs(e - er) / max(abs(er), 1.0) < E_TOL and rel_rms(d, dr) < F_TOL
 
 
 def test_output_logger(ue, chain_file, tmp_path):

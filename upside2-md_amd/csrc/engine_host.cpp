@@ -1288,8 +1288,58 @@ struct Program {
     RotamerDev* d_rots = nullptr;
     float* d_cf = nullptr;
     int* d_ci = nullptr;
-    std::map<int, vector<float>> param_set;  // interaction tables replaced by set_param, by graph
+    std::map<int, vector<float>> param_set;  // parameters replaced by set_param, by node
     PivotMoves pivots;                       // /input/pivot_moves, if present
+    vector<OpRec> prog;                      // the uploaded program (d_ops)
+    vector<int> prog_node;                   // node of each program entry
+
+    // Parameter vector of a node in the reference's get_param layout, and
+    // where it lives here: a pair graph's interaction table (constant floats);
+    // sigmoid coupling scale, center, sharpness per type then the weights
+    // (src/environment.cpp:925-939); backbone sigmoid coupling scale, center,
+    // sharpness, hbond weight (:1102-1109) and hbond energy's 12 parameters
+    // (src/hbond.cpp:653) in the op record.  Empty: none.
+    vector<float> get_params(int node) const {
+        const Node& n = T.nodes[node];
+        for (size_t g = 0; g < T.graphs.size(); ++g)
+            if (T.graph_node[g] == node) {
+                const PairGraph& G = T.graphs[g];
+                return vector<float>(T.cf.begin() + G.param,
+                                     T.cf.begin() + G.param + (size_t)G.n_type1 * G.n_type2 * G.n_param);
+            }
+        if (n.op < 0) return {};
+        const OpRec& o = T.ops[n.op];
+        vector<float> v;
+        if (o.kind == OP_SIGMOID_COUPLING) {
+            const int nt = o.ia[6];
+            for (int q : {3, 4, 5}) v.insert(v.end(), T.cf.begin() + o.ia[q], T.cf.begin() + o.ia[q] + nt);
+            v.insert(v.end(), T.cf.begin() + o.ia[2], T.cf.begin() + o.ia[2] + nt * nt);
+        } else if (o.kind == OP_BB_SIGMOID_COUPLING) {
+            v.assign(o.fa, o.fa + 4);
+        } else if (o.kind == OP_HBOND_ENERGY) {
+            v.assign(o.fa, o.fa + 12);
+        }
+        return v;
+    }
+    // Write a parameter vector of that layout back (sizes checked by the caller).
+    void put_params(int node, const vector<float>& v) {
+        const Node& n = T.nodes[node];
+        for (size_t g = 0; g < T.graphs.size(); ++g)
+            if (T.graph_node[g] == node) {
+                std::copy(v.begin(), v.end(), T.cf.begin() + T.graphs[g].param);
+                return;
+            }
+        OpRec& o = T.ops[n.op];
+        if (o.kind == OP_SIGMOID_COUPLING) {
+            const int nt = o.ia[6];
+            size_t k = 0;
+            for (int q : {3, 4, 5})
+                for (int i = 0; i < nt; ++i) T.cf[o.ia[q] + i] = v[k++];
+            for (int i = 0; i < nt * nt; ++i) T.cf[o.ia[2] + i] = v[k++];
+        } else {
+            for (size_t i = 0; i < v.size(); ++i) o.fa[i] = v[i];
+        }
+    }
 
     void release() {
         (void)hipFree(d_ops); (void)hipFree(d_graphs); (void)hipFree(d_rots); (void)hipFree(d_cf); (void)hipFree(d_ci);
@@ -1304,17 +1354,23 @@ struct Program {
         load_topology(T, h5, n_atom);
         pivots = PivotMoves();
         pivots.load(h5, n_atom);
-        for (auto& kv : param_set)  // tables set through set_param survive a reload
-            std::copy(kv.second.begin(), kv.second.end(), T.cf.begin() + T.graphs[kv.first].param);
+        for (auto& kv : param_set) put_params(kv.first, kv.second);  // set_param survives a reload
         release();
         // potential nodes push their sensitivities in the value pass: their
         // deriv steps have nothing to do and stay out of the program
         T.deriv.erase(std::remove_if(T.deriv.begin(), T.deriv.end(),
                                      [&](int i) { return !has_deriv_step(T.ops[T.nodes[i].op].kind); }),
                       T.deriv.end());
-        vector<OpRec> prog;
-        for (size_t k = 1; k < T.germ.size(); ++k) prog.push_back(T.ops[T.nodes[T.germ[k]].op]);
-        for (int i : T.deriv) prog.push_back(T.ops[T.nodes[i].op]);
+        prog.clear();
+        prog_node.clear();
+        for (size_t k = 1; k < T.germ.size(); ++k) {
+            prog.push_back(T.ops[T.nodes[T.germ[k]].op]);
+            prog_node.push_back(T.germ[k]);
+        }
+        for (int i : T.deriv) {
+            prog.push_back(T.ops[T.nodes[i].op]);
+            prog_node.push_back(i);
+        }
         // ops without a pair graph or rotamer solve are elementwise over at most
         // a few thousand elements: one workgroup runs them (UPSIDE_SOLO_OPS=0:
         // the whole team)
@@ -2125,29 +2181,33 @@ int evaluate_deriv(float* deriv, DerivEngine* engine, const float* pos) {
     ABI_CATCH(1)
 }
 
-// src/engine_c_library.cpp:68-78 and InteractionGraph::set_param
-// (src/interaction_graph.h:417-426): replace a pair graph's interaction table.
-// Supported where the pair-list cutoff does not depend on the table (the spline
-// graphs and protein_hbond, whose cutoffs are fixed, src/bead_interaction.h:193-195,
+// src/engine_c_library.cpp:68-78: replace a node's parameters.  Pair graphs
+// (InteractionGraph::set_param, src/interaction_graph.h:417-426) where the
+// pair-list cutoff does not depend on the table (the spline graphs and
+// protein_hbond, whose cutoffs are fixed, src/bead_interaction.h:193-195,
 // src/hbond.cpp:164-166, :258-260); the environment graphs, whose cutoff the
-// reference recomputes from the new table, return nonzero.
+// reference recomputes from a new table, return nonzero.  The sigmoid
+// couplings and hbond energy (src/environment.cpp:941-955, :1110-1115,
+// src/hbond.cpp:722-737) take the layout get_param returns.
 int set_param(int n_param, const float* param, DerivEngine* engine, const char* node_name) {
     ABI_TRY
     auto& B = engine->batch;
     Program& P = *B.sys[0].prog;
     const Topology& T = P.T;
     const Node* n = B.node(0, node_name);
+    const int node = (int)(n - T.nodes.data());
+    const vector<float> cur = P.get_params(node);
+    if (cur.empty()) fail(string("set_param: no parameters exposed for node ") + node_name);
+    vector<float> v(param, param + std::max(n_param, 0));
     for (size_t g = 0; g < T.graphs.size(); ++g) {
-        if (&T.nodes[T.graph_node[g]] != n) continue;
+        if (T.graph_node[g] != node) continue;
         const PairGraph& G = T.graphs[g];
         if (G.kind != EDGE_ROTAMER && G.kind != EDGE_HBOND_COVERAGE && G.kind != EDGE_PROTEIN_HBOND)
             fail(string("set_param: the pair-list cutoff of ") + node_name + " depends on its parameters; not supported");
-        const int size = G.n_type1 * G.n_type2 * G.n_param;
-        if (n_param != size)
-            fail("Bad param size, got " + std::to_string(n_param) + " params, but expected " + std::to_string(size) +
-                 " params of shape (" + std::to_string(G.n_type1) + ", " + std::to_string(G.n_type2) + ", " +
-                 std::to_string(G.n_param) + ")");
-        vector<float> v(param, param + n_param);
+        if (n_param != (int)cur.size())
+            fail("Bad param size, got " + std::to_string(n_param) + " params, but expected " +
+                 std::to_string(cur.size()) + " params of shape (" + std::to_string(G.n_type1) + ", " +
+                 std::to_string(G.n_type2) + ", " + std::to_string(G.n_param) + ")");
         if (G.sym) {
             // a symmetric table must equal its transpose with the angular
             // halves swapped (PosQuadSplineInteraction::is_compatible,
@@ -2164,15 +2224,22 @@ int set_param(int n_param, const float* param, DerivEngine* engine, const char* 
                         if (p1[k] != p2[k]) fail("incompatible parameters");
                 }
         }
-        hip_check(hipStreamSynchronize(B.stream), "sync");
-        hip_check(hipMemcpy(P.d_cf + G.param, v.data(), (size_t)size * sizeof(float), hipMemcpyHostToDevice),
-                  "set_param");
-        std::copy(v.begin(), v.end(), P.T.cf.begin() + G.param);
-        P.param_set[(int)g] = std::move(v);
-        engine->evaluated = false;
-        return 0;
     }
-    fail(string("set_param: no parameters exposed for node ") + node_name);
+    if (n_param != (int)cur.size())
+        fail(string("set_param: ") + node_name + " has " + std::to_string(cur.size()) + " parameters, got " +
+             std::to_string(n_param));
+    hip_check(hipStreamSynchronize(B.stream), "sync");
+    P.put_params(node, v);
+    P.param_set[node] = v;
+    hip_check(hipMemcpy(P.d_cf, P.T.cf.data(), P.T.cf.size() * sizeof(float), hipMemcpyHostToDevice), "set_param");
+    if (n->op >= 0)
+        for (size_t k = 0; k < P.prog.size(); ++k)
+            if (P.prog_node[k] == node) {
+                std::copy(P.T.ops[n->op].fa, P.T.ops[n->op].fa + 16, P.prog[k].fa);
+                hip_check(hipMemcpy(P.d_ops + k, &P.prog[k], sizeof(OpRec), hipMemcpyHostToDevice), "set_param");
+            }
+    engine->evaluated = false;
+    return 0;
     ABI_CATCH(1)
 }
 
@@ -2310,17 +2377,14 @@ int get_param_deriv(int n_param, float* deriv, DerivEngine* engine, const char* 
 int get_param(int n_param, float* param, DerivEngine* engine, const char* node_name) {
     ABI_TRY
     auto& B = engine->batch;
-    const Topology& T = B.sys[0].prog->T;
+    const Program& P = *B.sys[0].prog;
     const Node* n = B.node(0, node_name);
-    for (size_t g = 0; g < T.graphs.size(); ++g) {
-        if (&T.nodes[T.graph_node[g]] != n) continue;
-        const PairGraph& G = T.graphs[g];
-        int size = G.n_type1 * G.n_type2 * G.n_param;
-        if (size != n_param) fail("Wrong number of parameters, expected " + std::to_string(size) + " but got " + std::to_string(n_param));
-        std::copy(T.cf.begin() + G.param, T.cf.begin() + G.param + size, param);
-        return 0;
-    }
-    fail(string("no parameters exposed for node ") + node_name);
+    const vector<float> v = P.get_params((int)(n - P.T.nodes.data()));
+    if (v.empty()) fail(string("no parameters exposed for node ") + node_name);
+    if ((int)v.size() != n_param)
+        fail("Wrong number of parameters, expected " + std::to_string(v.size()) + " but got " + std::to_string(n_param));
+    std::copy(v.begin(), v.end(), param);
+    return 0;
     ABI_CATCH(1)
 }
 

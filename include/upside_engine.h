@@ -136,14 +136,19 @@ int upside_batch_bp_lds_solves(UpsideBatch* batch, int sys, long long* n);
  * (n_frame,) f8 and temperature (n_frame,1) f8, chunked by 100 frames.
  * output_paths NULL writes into each system's input .up (replacing an existing
  * /output, as the reference does without -o; systems must not share an input
- * file), else output_paths[s] is created afresh (as with -o).
+ * file), else output_paths[s] is created afresh (as with -o).  log_level 1
+ * (the reference's default, "detailed") adds hbond (n_frame, n_site) f4, the
+ * hbond value of every donor and acceptor (src/hbond.cpp:373-377), and
+ * rotamer_bad_solves_cumulative (n_frame, 1) i8 (src/rotamer.cpp:708-710); the
+ * reference's other detailed per-residue decompositions are not written.
  * upside_batch_run_logged advances n_round rounds and records a frame at
  * every round number (counted since the batch was created) that is a
  * multiple of frame_interval (> 0), before that round's thermostat and
  * integration.
  * upside_batch_close_output flushes and closes the files (also done by
  * upside_batch_free).  0 on success. */
-int upside_batch_open_output(UpsideBatch* batch, const char* const* output_paths, int record_momentum);
+int upside_batch_open_output(UpsideBatch* batch, const char* const* output_paths, int record_momentum,
+                             int log_level);
 int upside_batch_run_logged(UpsideBatch* batch, int n_round, int frame_interval);
 int upside_batch_close_output(UpsideBatch* batch);
 /* Monte Carlo pivot moves (src/monte_carlo_sampler.cpp:19-137, :255-283, from

@@ -395,9 +395,10 @@ def test_param_deriv_matches_reference(ue, chain_file, name):
 
 def test_mc_pivot_matches_reference_cli(ue, chain_file, tmp_path):
     """Pivot moves every 2 rounds of a 6-round run against the reference CLI
-    built from its sources (--monte-carlo-interval): the same moves are
-    accepted (pivot_stats), and every logged frame (positions, potential)
-    matches within 1e-4 relative."""
+    built from its sources (--monte-carlo-interval, detailed log level): the
+    same moves are accepted (pivot_stats), every logged frame (positions,
+    potential) matches within 1e-4 relative, and the detailed hbond and
+    bad-solve datasets agree."""
     import shutil
     import subprocess
     import h5lite
@@ -410,20 +411,24 @@ def test_mc_pivot_matches_reference_cli(ue, chain_file, tmp_path):
     shutil.copy(src, our_up)
     say("case: MC pivots L=40, 6 rounds, pivots every 2, against the reference CLI")
     r = subprocess.run([REF_BIN, "--duration", "0.162", "--frame-interval", "0.027", "--monte-carlo-interval",
-                        "0.0675", "--temperature", "0.85", "--seed", "7", "--log-level", "basic",
+                        "0.0675", "--temperature", "0.85", "--seed", "7", "--log-level", "detailed",
                         "--disable-recentering", ref_up], env=dict(os.environ, OMP_NUM_THREADS="1"),
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
     b = ue.Batch([our_up], [0.85], seed=7, dt=0.009)
     b.set_monte_carlo(2)
-    b.open_output(None)
+    b.open_output(None, log_level=1)
     b.run_logged(6, 1)
     b.close_output()
     assert b.mc_stats(0)[1] == 2
+    keys = ("pos", "potential", "pivot_stats", "time", "hbond", "rotamer_bad_solves_cumulative")
     with h5lite.File(ref_up) as f:
-        want = {k: f.read("/output/" + k) for k in ("pos", "potential", "pivot_stats", "time")}
+        want = {k: f.read("/output/" + k) for k in keys}
     with h5lite.File(our_up) as f:
-        got = {k: f.read("/output/" + k) for k in ("pos", "potential", "pivot_stats", "time")}
+        got = {k: f.read("/output/" + k) for k in keys}
+    np.testing.assert_array_equal(got["rotamer_bad_solves_cumulative"], want["rotamer_bad_solves_cumulative"])
+    assert got["hbond"].shape == want["hbond"].shape
+    np.testing.assert_allclose(got["hbond"], want["hbond"], rtol=1e-3, atol=1e-5)
     say("pivot_stats ours %s reference %s" % (got["pivot_stats"].tolist(), want["pivot_stats"].tolist()))
     np.testing.assert_array_equal(got["pivot_stats"], want["pivot_stats"])
     np.testing.assert_array_equal(got["time"], want["time"])
@@ -493,7 +498,7 @@ def test_output_logger(ue, chain_file, tmp_path):
     b = ue.Batch([path] * 2, temps, seed=8, dt=dt)
     with pytest.raises(RuntimeError):
         b.open_output(None)          # both systems read one input file
-    b.open_output(outs, record_momentum=True)
+    b.open_output(outs, record_momentum=True, log_level=1)
     b.run_logged(7, 3)
     b.close_output()
     ref = ue.Batch([path] * 2, temps, seed=8, dt=dt)
@@ -506,6 +511,7 @@ def test_output_logger(ue, chain_file, tmp_path):
             pos, mom = f.read("/output/pos"), f.read("/output/mom")
             kin, pot = f.read("/output/kinetic"), f.read("/output/potential")
             t, temp = f.read("/output/time"), f.read("/output/temperature")
+            hb, bad = f.read("/output/hbond"), f.read("/output/rotamer_bad_solves_cumulative")
         assert pos.shape == (3, 1, n_atom, 3) and mom.shape == (3, 1, n_atom, 3)
         assert kin.shape == (3, 1) and pot.shape == (3, 1) and temp.shape == (3, 1) and t.shape == (3,)
         assert pos.dtype == np.float32 and kin.dtype == np.float64 and t.dtype == np.float64
@@ -516,6 +522,14 @@ def test_output_logger(ue, chain_file, tmp_path):
         want_t = [float(np.float32(3 * np.float32(dt)) * np.float32(r)) for r in (0, 3, 6)]
         np.testing.assert_array_equal(t, want_t)
         np.testing.assert_array_equal(temp[:, 0], np.float32(temps[s]))
+        # detailed level: hbond of every donor and acceptor at the frame's
+        # positions, cumulative bad solves
+        eng = ue.Upside(path, quiet=True)
+        eng.energy(pos[0, 0])
+        want_hb = eng.get_output("protein_hbond")[:, 6]
+        assert hb.shape == (3, len(want_hb)) and hb.dtype == np.float32
+        np.testing.assert_allclose(hb[0], want_hb, rtol=1e-6, atol=1e-7)
+        assert bad.shape == (3, 1) and (bad == 0).all()
 
 
 def test_checkpoint_resume_is_bit_identical(ue, chain_file):

@@ -1963,6 +1963,9 @@ struct OutputLog {
     hid_t file = -1, group = -1;
     hid_t pos = -1, mom = -1, kinetic = -1, potential = -1, time = -1, temperature = -1;
     hid_t pivot_stats = -1;   // (n_frame, 2): accepted and attempted pivots since the last frame
+    // detailed level (src/hbond.cpp:373-377, src/rotamer.cpp:708-710)
+    hid_t hbond = -1, bad_solves = -1;
+    int hb_node = -1, hb_n = 0, rot_stats = -1;
     long long n = 0;
 
     static hid_t earray(hid_t g, const char* name, hid_t type, const vector<hsize_t>& row) {
@@ -1996,7 +1999,8 @@ struct OutputLog {
         H5Sclose(sp);
         if (e < 0) fail("output: write failed");
     }
-    void open(const string& path, bool into_input, int n_atom, bool record_mom, bool mc) {
+    void open(const string& path, bool into_input, int n_atom, bool record_mom, bool mc, const Topology& T,
+              int level) {
         if (into_input) {
             file = H5Fopen(path.c_str(), H5F_ACC_RDWR, H5P_DEFAULT);
             if (file >= 0 && H5Lexists(file, "output", H5P_DEFAULT) > 0) H5Ldelete(file, "output", H5P_DEFAULT);
@@ -2014,6 +2018,18 @@ struct OutputLog {
         time = earray(group, "time", H5T_NATIVE_DOUBLE, {});
         temperature = earray(group, "temperature", H5T_NATIVE_DOUBLE, {1});
         if (mc) pivot_stats = earray(group, "pivot_stats", H5T_NATIVE_INT, {2});  // src/monte_carlo_sampler.h:32-36
+        if (level >= 1) {
+            for (const Node& nd : T.nodes)
+                if (nd.op >= 0 && T.ops[nd.op].kind == OP_PROTEIN_HBOND && hb_node < 0) {
+                    hb_node = (int)(&nd - T.nodes.data());
+                    hb_n = nd.n_elem;
+                }
+            if (hb_node >= 0) hbond = earray(group, "hbond", H5T_NATIVE_FLOAT, {(hsize_t)hb_n});
+            if (!T.rots.empty()) {
+                rot_stats = T.rots[0].stats;
+                bad_solves = earray(group, "rotamer_bad_solves_cumulative", H5T_NATIVE_LLONG, {1});
+            }
+        }
         // the reference records its command line here (src/main.cpp:734-736)
         const char* inv = "libupside batch (upside_batch_run_logged)";
         hid_t st = H5Tcopy(H5T_C_S1);
@@ -2024,6 +2040,8 @@ struct OutputLog {
         H5Sclose(sp);
         H5Tclose(st);
     }
+    vector<float> hb_values;   // filled by the caller before frame() when hbond >= 0
+    long long n_bad = 0;
     void frame(const float* p, const float* m, int n_atom, double kin, double pot, double t, double temp,
                const int* mc_stats) {
         append(pos, H5T_NATIVE_FLOAT, p, n);
@@ -2033,11 +2051,13 @@ struct OutputLog {
         append(time, H5T_NATIVE_DOUBLE, &t, n);
         append(temperature, H5T_NATIVE_DOUBLE, &temp, n);
         if (pivot_stats >= 0) append(pivot_stats, H5T_NATIVE_INT, mc_stats, n);
+        if (hbond >= 0) append(hbond, H5T_NATIVE_FLOAT, hb_values.data(), n);
+        if (bad_solves >= 0) append(bad_solves, H5T_NATIVE_LLONG, &n_bad, n);
         ++n;
         (void)n_atom;
     }
     void close() {
-        for (hid_t* d : {&pos, &mom, &kinetic, &potential, &time, &temperature, &pivot_stats})
+        for (hid_t* d : {&pos, &mom, &kinetic, &potential, &time, &temperature, &pivot_stats, &hbond, &bad_solves})
             if (*d >= 0) { H5Dclose(*d); *d = -1; }
         if (group >= 0) { H5Gclose(group); group = -1; }
         if (file >= 0) { H5Fflush(file, H5F_SCOPE_LOCAL); H5Fclose(file); file = -1; }
@@ -2125,6 +2145,21 @@ struct UpsideBatch {
             }
             kin *= 0.5 / T.n_atom;
             const float t = 3 * batch.dt * (float)batch.round_num;  // inner_step*dt*round_num (:830-831)
+            OutputLog& o = *out[k];
+            if (o.hbond >= 0) {  // hbond value of every donor and acceptor site (column 6)
+                const Node& hn = T.nodes[o.hb_node];
+                vector<float> rows((size_t)hn.n_elem * hn.width);
+                hip_check(hipMemcpy(rows.data(), batch.sys[k].arena + hn.out, rows.size() * 4, hipMemcpyDeviceToHost),
+                          "hbond");
+                o.hb_values.resize(hn.n_elem);
+                for (int v = 0; v < hn.n_elem; ++v) o.hb_values[v] = rows[(size_t)v * hn.width + 6];
+            }
+            if (o.bad_solves >= 0) {
+                int st4[4];
+                hip_check(hipMemcpy(st4, batch.sys[k].iarena + o.rot_stats, sizeof(st4), hipMemcpyDeviceToHost),
+                          "stats");
+                o.n_bad = st4[1];
+            }
             int st[2] = {0, 0};
             if (k < mc_frame.size()) { st[0] = mc_frame[k][0]; st[1] = mc_frame[k][1]; mc_frame[k] = {0, 0}; }
             out[k]->frame(pos.data(), mom.data(), T.n_atom, kin, (double)batch.potential((int)k), (double)t,
@@ -2624,7 +2659,7 @@ int upside_batch_get_pos(UpsideBatch* b, int sys, float* pos) { return batch_cop
 int upside_batch_set_mom(UpsideBatch* b, int sys, const float* mom) { return batch_copy(b, sys, (float*)mom, true, true); }
 int upside_batch_get_mom(UpsideBatch* b, int sys, float* mom) { return batch_copy(b, sys, mom, false, true); }
 
-int upside_batch_open_output(UpsideBatch* b, const char* const* output_paths, int record_momentum) {
+int upside_batch_open_output(UpsideBatch* b, const char* const* output_paths, int record_momentum, int log_level) {
     ABI_TRY
     auto& B = b->batch;
     const int ns = (int)B.sys.size();
@@ -2639,7 +2674,8 @@ int upside_batch_open_output(UpsideBatch* b, const char* const* output_paths, in
     for (int k = 0; k < ns; ++k) {
         b->out.emplace_back(new OutputLog());
         const string path = output_paths ? string(output_paths[k]) : B.sys[k].prog->path;
-        b->out.back()->open(path, !output_paths, B.sys[k].prog->T.n_atom, b->record_mom, b->mc_interval > 0);
+        b->out.back()->open(path, !output_paths, B.sys[k].prog->T.n_atom, b->record_mom, b->mc_interval > 0,
+                            B.sys[k].prog->T, log_level);
     }
     return 0;
     ABI_CATCH(1)

@@ -65,7 +65,7 @@ def load_upside_library(path=LIB_PATH):
         "upside_batch_replica_exchange_criterion": (i, [vp, i, vp, ct.c_uint64, i, vp]),
         "upside_batch_bp_histogram": (i, [vp, i, vp, i]),
         "upside_batch_bp_lds_solves": (i, [vp, i, vp]),
-        "upside_batch_open_output": (i, [vp, vp, i]),
+        "upside_batch_open_output": (i, [vp, vp, i, i]),
         "upside_batch_run_logged": (i, [vp, i, i]),
         "upside_batch_close_output": (i, [vp]),
         "upside_batch_set_monte_carlo": (i, [vp, i]),
@@ -254,14 +254,14 @@ class Batch:
             raise RuntimeError("bp stats failed")
         return dict(last=last.value, total=tot.value, solves=sol.value, bad=bad.value, lds=lds.value)
 
-    def open_output(self, paths=None, record_momentum=False):
+    def open_output(self, paths=None, record_momentum=False, log_level=0):
         """Start /output logging: into each system's input .up (paths None) or
-        into new files paths[s]."""
+        into new files paths[s]; log_level 1 adds the detailed datasets."""
         arr = None
         if paths is not None:
             enc = [p.encode() for p in paths]
             arr = (ct.c_char_p * len(enc))(*enc)
-        if self.calc.upside_batch_open_output(self.h, arr, int(bool(record_momentum))):
+        if self.calc.upside_batch_open_output(self.h, arr, int(bool(record_momentum)), int(log_level)):
             raise RuntimeError("open_output failed (see stderr)")
 
     def run_logged(self, n_round, frame_interval):

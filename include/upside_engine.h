@@ -165,6 +165,20 @@ int upside_batch_close_output(UpsideBatch* batch);
  * set.  No counterpart in the reference's C ABI (its CLI option
  * --monte-carlo-interval, src/main.cpp:471, :570-571). */
 int upside_batch_set_monte_carlo(UpsideBatch* batch, int mc_interval);
+/* Replica exchange inside upside_batch_run_logged, as the reference's system
+ * loop does it with --replica-interval, --swap-set and --exchange-criterion
+ * (src/main.cpp:159-316, :915-999): n_sets swap sets of set_sizes[s] pairs
+ * (pairs flattened), tried in turn; a swap is attempted when the batch's round
+ * counter reaches the end of a segment of the reference loop (the first
+ * multiple of replica_interval at least two rounds after the previous swap),
+ * with Threefry stream 1 and counter = that round.  An open output then also
+ * records replica_index, replica_swap_partner and replica_cumulative_swaps
+ * (:210-232) when this was set before upside_batch_open_output.
+ * replica_interval 0 disables.  upside_batch_replica_stats: the replica index
+ * held by each system and successes/attempts of every pair (sets in order). */
+int upside_batch_set_replica_exchange(UpsideBatch* batch, int replica_interval, int criterion, int n_sets,
+                                      const int* set_sizes, const int* pairs);
+int upside_batch_replica_stats(UpsideBatch* batch, int* replica_index, long long* successes, long long* attempts);
 int upside_batch_mc_stats(UpsideBatch* batch, int sys, long long* accepted, long long* attempted);
 /* Checkpoint and resume (no counterpart in the reference, whose thermostat
  * counter lives only in memory, src/thermostat.h:24-29): the whole device state

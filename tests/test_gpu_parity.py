@@ -485,6 +485,56 @@ def test_coupling_param_deriv_matches_reference(ue, chain_file, name, size):
     assert abs(e - er) / max(abs(er), 1.0) < E_TOL and rel_rms(d, dr) < F_TOL
 
 
+def test_replica_exchange_loop_matches_reference_cli(ue, chain_file, tmp_path):
+    """Four replicas, swap sets 0-1,2-3 and 1-2 tried in turn every 2 rounds,
+    6 rounds with a frame every round, against the reference CLI
+    (--replica-interval, --swap-set): the same replica indices and cumulative
+    swap counts in every frame, the same swap partners, and positions and
+    potentials within 1e-4 relative."""
+    import shutil
+    import subprocess
+    import h5lite
+    from conftest import REF_BIN
+    if not os.path.exists(REF_BIN):
+        pytest.skip("reference CLI not built")
+    src = chain_file(40, 21)
+    temps = [0.85, 0.88, 0.91, 0.95]
+    refs = [str(tmp_path / ("ref%d.up" % k)) for k in range(4)]
+    ours = [str(tmp_path / ("ours%d.up" % k)) for k in range(4)]
+    for a, c in zip(refs, ours):
+        shutil.copy(src, a)
+        shutil.copy(src, c)
+    say("case: replica exchange loop, 4 x L=40, 6 rounds, against the reference CLI")
+    r = subprocess.run([REF_BIN, "--duration", "0.162", "--frame-interval", "0.027", "--replica-interval", "0.0675",
+                        "--swap-set", "0-1,2-3", "--swap-set", "1-2", "--temperature", ",".join(map(str, temps)),
+                        "--seed", "7", "--log-level", "basic", "--disable-recentering"] + refs,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    b = ue.Batch(ours, temps, seed=7, dt=0.009)
+    b.set_replica_exchange(2, [[(0, 1), (2, 3)], [(1, 2)]])
+    b.open_output(None)
+    b.run_logged(6, 1)
+    b.close_output()
+    idx, suc, att = b.replica_stats()
+    say("after 6 rounds: replica index %s, swaps %s of %s" % (idx.tolist(), suc.tolist(), att.tolist()))
+    np.testing.assert_array_equal(att, [2, 2, 1])
+    keys = ("pos", "potential", "replica_index", "replica_cumulative_swaps", "replica_swap_partner")
+    for k in range(4):
+        with h5lite.File(refs[k]) as f:
+            want = {q: f.read("/output/" + q) for q in keys}
+        with h5lite.File(ours[k]) as f:
+            got = {q: f.read("/output/" + q) for q in keys}
+        say("system %d replica_index %s reference %s" % (k, got["replica_index"].ravel().tolist(),
+                                                         want["replica_index"].ravel().tolist()))
+        for q in ("replica_index", "replica_cumulative_swaps", "replica_swap_partner"):
+            np.testing.assert_array_equal(got[q], want[q], err_msg=q)
+        assert got["pos"].shape == want["pos"].shape
+        for fr in range(got["pos"].shape[0]):
+            e = rel_rms(got["pos"][fr, 0], want["pos"][fr, 0])
+            de = abs(got["potential"][fr, 0] - want["potential"][fr, 0]) / max(abs(want["potential"][fr, 0]), 1.0)
+            assert e < 1e-4 and de < 1e-4, (k, fr, e, de)
+
+
 def test_output_logger(ue, chain_file, tmp_path):
     """/output datasets as the reference's basic log level writes them: frames
     at rounds 0, 3, 6 of a 7-round run hold the state before that round's

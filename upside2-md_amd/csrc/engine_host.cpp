@@ -1965,6 +1965,8 @@ struct OutputLog {
     hid_t pivot_stats = -1;   // (n_frame, 2): accepted and attempted pivots since the last frame
     // detailed level (src/hbond.cpp:373-377, src/rotamer.cpp:708-710)
     hid_t hbond = -1, bad_solves = -1;
+    hid_t replica_index = -1, replica_swaps = -1;   // src/main.cpp:210-232
+    int n_partner = 0;
     int hb_node = -1, hb_n = 0, rot_stats = -1;
     long long n = 0;
 
@@ -2000,7 +2002,7 @@ struct OutputLog {
         if (e < 0) fail("output: write failed");
     }
     void open(const string& path, bool into_input, int n_atom, bool record_mom, bool mc, const Topology& T,
-              int level) {
+              int level, const vector<int>* partners) {
         if (into_input) {
             file = H5Fopen(path.c_str(), H5F_ACC_RDWR, H5P_DEFAULT);
             if (file >= 0 && H5Lexists(file, "output", H5P_DEFAULT) > 0) H5Ldelete(file, "output", H5P_DEFAULT);
@@ -2018,6 +2020,18 @@ struct OutputLog {
         time = earray(group, "time", H5T_NATIVE_DOUBLE, {});
         temperature = earray(group, "temperature", H5T_NATIVE_DOUBLE, {1});
         if (mc) pivot_stats = earray(group, "pivot_stats", H5T_NATIVE_INT, {2});  // src/monte_carlo_sampler.h:32-36
+        if (partners) {  // replica exchange: index, partners once, cumulative swaps per partner
+            replica_index = earray(group, "replica_index", H5T_NATIVE_INT, {1});
+            n_partner = (int)partners->size();
+            if (n_partner > 0) {
+                hid_t d = earray(group, "replica_swap_partner", H5T_NATIVE_INT, {});
+                hsize_t dims[1] = {(hsize_t)n_partner};
+                H5Dset_extent(d, dims);
+                H5Dwrite(d, H5T_NATIVE_INT, H5S_ALL, H5S_ALL, H5P_DEFAULT, partners->data());
+                H5Dclose(d);
+                replica_swaps = earray(group, "replica_cumulative_swaps", H5T_NATIVE_INT, {(hsize_t)n_partner, 2});
+            }
+        }
         if (level >= 1) {
             for (const Node& nd : T.nodes)
                 if (nd.op >= 0 && T.ops[nd.op].kind == OP_PROTEIN_HBOND && hb_node < 0) {
@@ -2042,6 +2056,8 @@ struct OutputLog {
     }
     vector<float> hb_values;   // filled by the caller before frame() when hbond >= 0
     long long n_bad = 0;
+    int rep_index = 0;
+    vector<int> rep_swaps;     // [n_partner][2] successes, attempts
     void frame(const float* p, const float* m, int n_atom, double kin, double pot, double t, double temp,
                const int* mc_stats) {
         append(pos, H5T_NATIVE_FLOAT, p, n);
@@ -2053,11 +2069,14 @@ struct OutputLog {
         if (pivot_stats >= 0) append(pivot_stats, H5T_NATIVE_INT, mc_stats, n);
         if (hbond >= 0) append(hbond, H5T_NATIVE_FLOAT, hb_values.data(), n);
         if (bad_solves >= 0) append(bad_solves, H5T_NATIVE_LLONG, &n_bad, n);
+        if (replica_index >= 0) append(replica_index, H5T_NATIVE_INT, &rep_index, n);
+        if (replica_swaps >= 0) append(replica_swaps, H5T_NATIVE_INT, rep_swaps.data(), n);
         ++n;
         (void)n_atom;
     }
     void close() {
-        for (hid_t* d : {&pos, &mom, &kinetic, &potential, &time, &temperature, &pivot_stats, &hbond, &bad_solves})
+        for (hid_t* d : {&pos, &mom, &kinetic, &potential, &time, &temperature, &pivot_stats, &hbond, &bad_solves,
+                         &replica_index, &replica_swaps})
             if (*d >= 0) { H5Dclose(*d); *d = -1; }
         if (group >= 0) { H5Gclose(group); group = -1; }
         if (file >= 0) { H5Fflush(file, H5F_SCOPE_LOCAL); H5Fclose(file); file = -1; }
@@ -2070,6 +2089,20 @@ struct UpsideBatch {
     std::vector<std::unique_ptr<OutputLog>> out;
     bool record_mom = false;
     int mc_interval = 0;                    // rounds between pivot moves (0: none)
+    // replica exchange inside run_logged (ReplicaExchange, src/main.cpp:159-316)
+    int rex_interval = 0, rex_criterion = 0;
+    vector<vector<std::array<int, 2>>> rex_sets;        // swap sets, pairs in order
+    vector<vector<std::array<long long, 2>>> rex_stats; // per pair: successes, attempts
+    vector<vector<std::array<int, 2>>> rex_partner;     // per system: (set, pair) it takes part in
+    vector<int> replica_of;                            // replica index held by each system
+    long long rex_attempts = 0;
+    uint64_t rex_next = 0;
+    static uint64_t next_swap_round(uint64_t after, int ri) {
+        // a segment of the reference loop ends at the first multiple of the
+        // interval at least two rounds past its start (src/main.cpp:918-990)
+        uint64_t e = after + 2;
+        return (e + ri - 1) / ri * ri;
+    }
     vector<std::array<long long, 2>> mc_total;  // accepted, attempted since set_monte_carlo
     vector<std::array<int, 2>> mc_frame;        // the same since the last logged frame
 
@@ -2159,6 +2192,14 @@ struct UpsideBatch {
                 hip_check(hipMemcpy(st4, batch.sys[k].iarena + o.rot_stats, sizeof(st4), hipMemcpyDeviceToHost),
                           "stats");
                 o.n_bad = st4[1];
+            }
+            if (o.replica_index >= 0) {
+                o.rep_index = replica_of[k];
+                o.rep_swaps.clear();
+                for (auto& sp : rex_partner[k]) {
+                    o.rep_swaps.push_back((int)rex_stats[sp[0]][sp[1]][0]);
+                    o.rep_swaps.push_back((int)rex_stats[sp[0]][sp[1]][1]);
+                }
             }
             int st[2] = {0, 0};
             if (k < mc_frame.size()) { st[0] = mc_frame[k][0]; st[1] = mc_frame[k][1]; mc_frame[k] = {0, 0}; }
@@ -2674,8 +2715,13 @@ int upside_batch_open_output(UpsideBatch* b, const char* const* output_paths, in
     for (int k = 0; k < ns; ++k) {
         b->out.emplace_back(new OutputLog());
         const string path = output_paths ? string(output_paths[k]) : B.sys[k].prog->path;
+        vector<int> partners;  // the other system of each swap this one takes part in (:219-225)
+        for (auto& sp : b->rex_partner.size() > (size_t)k ? b->rex_partner[k] : vector<std::array<int, 2>>()) {
+            const auto& pr = b->rex_sets[sp[0]][sp[1]];
+            partners.push_back(pr[0] != k ? pr[0] : pr[1]);
+        }
         b->out.back()->open(path, !output_paths, B.sys[k].prog->T.n_atom, b->record_mom, b->mc_interval > 0,
-                            B.sys[k].prog->T, log_level);
+                            B.sys[k].prog->T, log_level, b->rex_interval > 0 ? &partners : nullptr);
     }
     return 0;
     ABI_CATCH(1)
@@ -2703,6 +2749,66 @@ int upside_batch_mc_stats(UpsideBatch* b, int sys, long long* accepted, long lon
     ABI_CATCH(1)
 }
 
+int upside_batch_set_replica_exchange(UpsideBatch* b, int replica_interval, int criterion, int n_sets,
+                                      const int* set_sizes, const int* pairs) {
+    ABI_TRY
+    auto& B = b->batch;
+    const int ns = (int)B.sys.size();
+    if (replica_interval < 0) fail("set_replica_exchange: negative interval");
+    if (criterion != 0 && criterion != 1) fail("replica exchange: exchange criterion must be 0 or 1");
+    vector<vector<std::array<int, 2>>> sets;
+    int at = 0;
+    for (int s = 0; s < n_sets; ++s) {
+        if (set_sizes[s] < 1) fail("set_replica_exchange: empty swap set");
+        std::set<int> seen;
+        sets.emplace_back();
+        for (int p = 0; p < set_sizes[s]; ++p, ++at) {
+            const int a = pairs[2 * at], c = pairs[2 * at + 1];
+            if (a < 0 || a >= ns || c < 0 || c >= ns) fail("invalid system");
+            if (a == c || seen.count(a) || seen.count(c))
+                fail("Overlapping indices in swap set.  No replica index can appear more than once in a swap set.");
+            seen.insert(a);
+            seen.insert(c);
+            sets.back().push_back({a, c});
+        }
+    }
+    if (replica_interval > 0) {
+        if (sets.empty()) fail("replica exchange requested but no swap sets proposed");
+        for (int k = 1; k < ns; ++k)
+            if (B.sys[k].prog->T.n_atom != B.sys[0].prog->T.n_atom)
+                fail("Replica exchange requires all systems have the same number of atoms");
+    }
+    b->rex_interval = replica_interval;
+    b->rex_criterion = criterion;
+    b->rex_sets = sets;
+    b->rex_stats.clear();
+    for (auto& st : sets) b->rex_stats.emplace_back(st.size(), std::array<long long, 2>{0, 0});
+    b->rex_partner.assign(ns, {});
+    for (size_t s = 0; s < sets.size(); ++s)
+        for (size_t p = 0; p < sets[s].size(); ++p)
+            for (int x : {sets[s][p][0], sets[s][p][1]}) b->rex_partner[x].push_back({(int)s, (int)p});
+    b->replica_of.resize(ns);
+    for (int k = 0; k < ns; ++k) b->replica_of[k] = k;
+    b->rex_attempts = 0;
+    b->rex_next = replica_interval > 0 ? UpsideBatch::next_swap_round(B.round_num, replica_interval) : 0;
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_replica_stats(UpsideBatch* b, int* replica_index, long long* successes, long long* attempts) {
+    ABI_TRY
+    for (size_t k = 0; k < b->replica_of.size(); ++k) replica_index[k] = b->replica_of[k];
+    size_t i = 0;
+    for (auto& st : b->rex_stats)
+        for (auto& x : st) {
+            successes[i] = x[0];
+            attempts[i] = x[1];
+            ++i;
+        }
+    return 0;
+    ABI_CATCH(1)
+}
+
 int upside_batch_run_logged(UpsideBatch* b, int n_round, int frame_interval) {
     ABI_TRY
     auto& B = b->batch;
@@ -2711,7 +2817,9 @@ int upside_batch_run_logged(UpsideBatch* b, int n_round, int frame_interval) {
     const uint64_t mc = (uint64_t)b->mc_interval, fi = (uint64_t)frame_interval;
     int done = 0;
     // the order of one round of src/main.cpp:941-964: pivot move (not at round
-    // 0), frame, then thermostat and integration in the launch
+    // 0), frame, then thermostat and integration in the launch; replica
+    // exchange where a segment of the reference loop ends (:991-995), with
+    // the swap sets taken in turn (:260-263)
     while (done < n_round) {
         const uint64_t r = B.round_num;
         if (mc && r > 0 && r % mc == 0) b->mc_step(r);
@@ -2719,9 +2827,28 @@ int upside_batch_run_logged(UpsideBatch* b, int n_round, int frame_interval) {
         int n = n_round - done;
         if (mc) n = std::min<int64_t>(n, (int64_t)(mc - r % mc));
         if (frames) n = std::min<int64_t>(n, (int64_t)(fi - r % fi));
+        if (b->rex_interval > 0 && b->rex_next > r) n = std::min<int64_t>(n, (int64_t)(b->rex_next - r));
         B.run(n, 0, nullptr, 0, (int)B.round_num);
         B.round_num += n;
         done += n;
+        if (b->rex_interval > 0 && B.round_num == b->rex_next) {
+            const int set = (int)(b->rex_attempts % (long long)b->rex_sets.size());
+            const auto& S = b->rex_sets[set];
+            vector<int> pr, acc(S.size());
+            for (auto& p : S) { pr.push_back(p[0]); pr.push_back(p[1]); }
+            if (upside_batch_replica_exchange_criterion(b, (int)S.size(), pr.data(), B.round_num, b->rex_criterion,
+                                                        acc.data()))
+                fail("replica exchange failed");
+            for (size_t p = 0; p < S.size(); ++p) {
+                b->rex_stats[set][p][1]++;
+                if (acc[p]) {
+                    b->rex_stats[set][p][0]++;
+                    std::swap(b->replica_of[S[p][0]], b->replica_of[S[p][1]]);
+                }
+            }
+            b->rex_attempts++;
+            b->rex_next = UpsideBatch::next_swap_round(B.round_num, b->rex_interval);
+        }
     }
     return 0;
     ABI_CATCH(1)

@@ -69,6 +69,8 @@ def load_upside_library(path=LIB_PATH):
         "upside_batch_run_logged": (i, [vp, i, i]),
         "upside_batch_close_output": (i, [vp]),
         "upside_batch_set_monte_carlo": (i, [vp, i]),
+        "upside_batch_set_replica_exchange": (i, [vp, i, i, i, vp, vp]),
+        "upside_batch_replica_stats": (i, [vp, vp, vp, vp]),
         "upside_batch_mc_stats": (i, [vp, i, vp, vp]),
         "upside_batch_state_size": (ct.c_longlong, [vp]),
         "upside_batch_save_state": (i, [vp, vp, ct.c_longlong]),
@@ -273,6 +275,25 @@ class Batch:
         """Pivot moves every mc_interval rounds in run_logged (0: none)."""
         if self.calc.upside_batch_set_monte_carlo(self.h, int(mc_interval)):
             raise RuntimeError("set_monte_carlo failed (see stderr)")
+
+    def set_replica_exchange(self, replica_interval, swap_sets, criterion=0):
+        """Replica exchange in run_logged every replica_interval rounds over
+        swap_sets (a list of lists of (i, j) pairs, tried in turn)."""
+        sizes = np.array([len(s) for s in swap_sets], np.intc)
+        pairs = np.ascontiguousarray(np.array([p for s in swap_sets for p in s], np.intc).reshape(-1, 2))
+        if self.calc.upside_batch_set_replica_exchange(self.h, int(replica_interval), int(criterion), len(sizes),
+                                                       sizes.ctypes.data, pairs.ctypes.data):
+            raise RuntimeError("set_replica_exchange failed (see stderr)")
+        self._n_pairs = int(sizes.sum())
+
+    def replica_stats(self):
+        """(replica index of each system, successes and attempts of every pair)."""
+        n = getattr(self, "_n_pairs", 0)
+        idx = np.zeros(self.n_sys, np.intc)
+        suc, att = np.zeros(max(n, 1), np.int64), np.zeros(max(n, 1), np.int64)
+        if self.calc.upside_batch_replica_stats(self.h, idx.ctypes.data, suc.ctypes.data, att.ctypes.data):
+            raise RuntimeError("replica_stats failed")
+        return idx, suc[:n], att[:n]
 
     def mc_stats(self, s):
         """(accepted, attempted) pivot moves of system s."""

@@ -165,6 +165,11 @@ int upside_batch_close_output(UpsideBatch* batch);
  * set.  No counterpart in the reference's C ABI (its CLI option
  * --monte-carlo-interval, src/main.cpp:471, :570-571). */
 int upside_batch_set_monte_carlo(UpsideBatch* batch, int mc_interval);
+/* Recentering before every logged frame, as the reference CLI does unless
+ * --disable-recentering (src/main.cpp:576-577, :947; src/deriv_engine.cpp:39-49):
+ * mode 0 none (the default here), 1 subtract the centroid, 2 its x and y only
+ * (--disable-z-recentering). */
+int upside_batch_set_recentering(UpsideBatch* batch, int mode);
 /* Replica exchange inside upside_batch_run_logged, as the reference's system
  * loop does it with --replica-interval, --swap-set and --exchange-criterion
  * (src/main.cpp:159-316, :915-999): n_sets swap sets of set_sizes[s] pairs

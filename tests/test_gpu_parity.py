@@ -535,6 +535,60 @@ def test_replica_exchange_loop_matches_reference_cli(ue, chain_file, tmp_path):
             assert e < 1e-4 and de < 1e-4, (k, fr, e, de)
 
 
+def test_recentering_matches_reference_cli(ue, chain_file, tmp_path):
+    """Recentering before every frame (the reference CLI's default): frame 0
+    equals the CLI's recentered start; every frame has its centroid at the
+    origin, a potential equal to the oracle's at the logged positions, and
+    positions equal to an unrecentered run's minus their centroid.  (After
+    frame 0 the CLI's own trajectory is not a reference: its affine alignment
+    of residue 0 of this chain changes under translation, which moves that
+    residue's side chain, so its energy takes two values under rigid shifts.)"""
+    import shutil
+    import subprocess
+    import h5lite
+    from conftest import REF_BIN
+    if not os.path.exists(REF_BIN):
+        pytest.skip("reference CLI not built")
+    src = chain_file(40, 21)
+    ref_up, our_up, plain_up = str(tmp_path / "ref.up"), str(tmp_path / "ours.up"), str(tmp_path / "plain.up")
+    for q in (ref_up, our_up, plain_up):
+        shutil.copy(src, q)
+    say("case: recentering, L=40, 6 rounds")
+    r = subprocess.run([REF_BIN, "--duration", "0.162", "--frame-interval", "0.027", "--temperature", "0.85",
+                        "--seed", "7", "--log-level", "basic", ref_up], env=dict(os.environ, OMP_NUM_THREADS="1"),
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    b = ue.Batch([our_up], [0.85], seed=7, dt=0.009)
+    b.set_recentering(1)
+    b.open_output(None)
+    b.run_logged(6, 1)
+    b.close_output()
+    plain = ue.Batch([plain_up], [0.85], seed=7, dt=0.009)
+    plain.open_output(None)
+    plain.run_logged(6, 1)
+    plain.close_output()
+    with h5lite.File(ref_up) as f:
+        ref_pos = f.read("/output/pos")
+    with h5lite.File(our_up) as f:
+        got_pos, got_pot = f.read("/output/pos"), f.read("/output/potential")
+    with h5lite.File(plain_up) as f:
+        plain_pos = f.read("/output/pos")
+    assert got_pos.shape == ref_pos.shape == plain_pos.shape
+    assert rel_rms(got_pos[0, 0], ref_pos[0, 0]) < 1e-6
+    ora = O.Engine(src)
+    for fr in range(got_pos.shape[0]):
+        p = got_pos[fr, 0]
+        np.testing.assert_allclose(p.mean(axis=0), 0.0, atol=1e-4)
+        e_ora = float(ora.energy(p))
+        de = abs(got_pot[fr, 0] - e_ora) / max(abs(e_ora), 1.0)
+        q = plain_pos[fr, 0] - plain_pos[fr, 0].mean(axis=0)
+        e = rel_rms(p, q)
+        say("frame %d: potential vs oracle %.2e, pos vs unrecentered run %.2e" % (fr, de, e))
+        assert de < 1e-4 and e < 1e-4, (fr, de, e)
+    with pytest.raises(RuntimeError):
+        b.set_recentering(3)
+
+
 def test_output_logger(ue, chain_file, tmp_path):
     """/output datasets as the reference's basic log level writes them: frames
     at rounds 0, 3, 6 of a 7-round run hold the state before that round's

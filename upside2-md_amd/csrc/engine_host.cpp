@@ -30,6 +30,7 @@
 hipError_t upside_launch_engine(upside::SysDesc* d_systems, int n_sys, upside::RunArgs args, hipStream_t stream);
 hipError_t upside_launch_param_deriv(const upside::ParamDerivArgs& args, int n_rows, hipStream_t stream);
 hipError_t upside_launch_coupling_param_deriv(const upside::CouplingDerivArgs& args, hipStream_t stream);
+hipError_t upside_launch_recenter(float* pos, int n_atom, int xy_only, hipStream_t stream);
 int upside_max_resident_blocks();
 
 namespace upside {
@@ -2089,6 +2090,7 @@ struct UpsideBatch {
     std::vector<std::unique_ptr<OutputLog>> out;
     bool record_mom = false;
     int mc_interval = 0;                    // rounds between pivot moves (0: none)
+    int recenter = 0;                       // before each frame: 0 none, 1 xyz, 2 xy (src/main.cpp:576-577, :947)
     // replica exchange inside run_logged (ReplicaExchange, src/main.cpp:159-316)
     int rex_interval = 0, rex_criterion = 0;
     vector<vector<std::array<int, 2>>> rex_sets;        // swap sets, pairs in order
@@ -2163,6 +2165,12 @@ struct UpsideBatch {
     // derivative are evaluated first, as src/main.cpp:946-949 does before
     // collect_samples.
     void log_frame() {
+        if (recenter)
+            for (auto& s : batch.sys) {
+                const Topology& T = s.prog->T;
+                hip_check(upside_launch_recenter(s.arena + T.nodes[0].out, T.n_atom, recenter == 2, batch.stream),
+                          "recenter");
+            }
         batch.evaluate(1);
         for (size_t k = 0; k < batch.sys.size(); ++k) {
             const Topology& T = batch.sys[k].prog->T;
@@ -2736,6 +2744,14 @@ int upside_batch_set_monte_carlo(UpsideBatch* b, int mc_interval) {
     b->mc_interval = mc_interval;
     b->mc_total.assign(b->batch.sys.size(), {0, 0});
     b->mc_frame.assign(b->batch.sys.size(), {0, 0});
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_set_recentering(UpsideBatch* b, int mode) {
+    ABI_TRY
+    if (mode < 0 || mode > 2) fail("set_recentering: mode must be 0 (none), 1 (x, y, z) or 2 (x, y)");
+    b->recenter = mode;
     return 0;
     ABI_CATCH(1)
 }

@@ -1341,7 +1341,30 @@ __global__ void __launch_bounds__(NT) coupling_param_deriv_kernel(CouplingDerivA
     }
 }
 
+// Subtract the centroid of the positions (x, y and z, or x and y only), as
+// recenter does before a logged frame (src/deriv_engine.cpp:39-49): the sum
+// runs over atoms in order on one thread, so it rounds as the reference's.
+__global__ void __launch_bounds__(NT) recenter_kernel(float* pos, int n_atom, int xy_only) {
+    __shared__ float cen[3];
+    if (threadIdx.x == 0) {
+        float cx = 0.f, cy = 0.f, cz = 0.f;
+        for (int a = 0; a < n_atom; ++a) { cx += pos[3 * a]; cy += pos[3 * a + 1]; cz += pos[3 * a + 2]; }
+        const float inv = (float)n_atom;
+        cen[0] = cx / inv;
+        cen[1] = cy / inv;
+        cen[2] = xy_only ? 0.f : cz / inv;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * n_atom; i += NT) pos[i] -= cen[i % 3];
+}
+
 }  // namespace upside
+
+hipError_t upside_launch_recenter(float* pos, int n_atom, int xy_only, hipStream_t stream) {
+    if (n_atom <= 0) return hipErrorInvalidValue;
+    void* kargs[] = {&pos, &n_atom, &xy_only};
+    return hipLaunchKernel((const void*)upside::recenter_kernel, dim3(1), dim3(upside::NT), kargs, 0, stream);
+}
 
 hipError_t upside_launch_coupling_param_deriv(const upside::CouplingDerivArgs& args, hipStream_t stream) {
     if (args.n_out <= 0 || args.nt <= 0) return hipErrorInvalidValue;

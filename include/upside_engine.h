@@ -191,7 +191,8 @@ int upside_batch_mc_stats(UpsideBatch* batch, int sys, long long* accepted, long
  * positions, momenta, Verlet caches, pair lists and BP state) into a caller
  * buffer of upside_batch_state_size() bytes.  Loading it into a batch made
  * from the same .up files (any seed, any team size) continues bit for bit as
- * the saved batch would have; a state whose system count or arena sizes do not
+ * the saved batch would have, including the run loop's state (Monte Carlo
+ * interval and counts, replica-exchange sets, indices and counts, recentering); a state whose system count or arena sizes do not
  * match the batch is refused.  0 on success. */
 long long upside_batch_state_size(UpsideBatch* batch);
 int upside_batch_save_state(UpsideBatch* batch, void* buf, long long size);

@@ -659,6 +659,21 @@ def test_checkpoint_resume_is_bit_identical(ue, chain_file):
         np.testing.assert_array_equal(b2.pos(s), b1.pos(s))
         np.testing.assert_array_equal(b2.mom(s), b1.mom(s))
     np.testing.assert_array_equal(b2.energies(), b1.energies())
+    # the loop state travels too: pivot moves and replica exchange resume
+    c1 = ue.Batch([path] * 2, [0.85, 0.9], seed=6)
+    c1.set_monte_carlo(3)
+    c1.set_replica_exchange(2, [[(0, 1)]])
+    c1.run_logged(3, 0)
+    loop_state = c1.save_state()
+    c1.run_logged(5, 0)
+    c2 = ue.Batch([path] * 2, [0.85, 0.9], seed=17)
+    c2.load_state(loop_state)
+    c2.run_logged(5, 0)
+    for s in range(2):
+        np.testing.assert_array_equal(c2.pos(s), c1.pos(s))
+        assert c2.mc_stats(s) == c1.mc_stats(s)
+    for x, y in zip(c2.replica_stats(1), c1.replica_stats()):
+        np.testing.assert_array_equal(x, y)
     other = ue.Batch([chain_file(40, 21)] * 2, [0.85, 0.9], seed=6)
     with pytest.raises(RuntimeError):
         other.load_state(state)

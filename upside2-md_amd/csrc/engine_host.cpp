@@ -2878,16 +2878,43 @@ int upside_batch_close_output(UpsideBatch* b) {
     ABI_CATCH(1)
 }
 
+// Loop state of the wrapper after the device state: Monte Carlo interval and
+// counts, replica-exchange configuration, replica indices, pair statistics and
+// next swap round, recentering mode (ints and 64-bit counts in one stream).
+static vector<long long> loop_state(const UpsideBatch* b) {
+    vector<long long> v{0x55505354415445ll, b->mc_interval, b->recenter, b->rex_interval, b->rex_criterion,
+                        b->rex_attempts, (long long)b->rex_next, (long long)b->mc_total.size()};
+    for (auto& t : b->mc_total) { v.push_back(t[0]); v.push_back(t[1]); }
+    v.push_back((long long)b->rex_sets.size());
+    for (size_t s = 0; s < b->rex_sets.size(); ++s) {
+        v.push_back((long long)b->rex_sets[s].size());
+        for (size_t p = 0; p < b->rex_sets[s].size(); ++p) {
+            v.push_back(b->rex_sets[s][p][0]);
+            v.push_back(b->rex_sets[s][p][1]);
+            v.push_back(b->rex_stats[s][p][0]);
+            v.push_back(b->rex_stats[s][p][1]);
+        }
+    }
+    v.push_back((long long)b->replica_of.size());
+    for (int x : b->replica_of) v.push_back(x);
+    return v;
+}
+
 long long upside_batch_state_size(UpsideBatch* b) {
     ABI_TRY
-    return (long long)b->batch.state_size();
+    return (long long)(b->batch.state_size() + sizeof(long long) * (1 + loop_state(b).size()));
     ABI_CATCH(-1)
 }
 
 int upside_batch_save_state(UpsideBatch* b, void* buf, long long size) {
     ABI_TRY
-    if (!buf || size < 0) fail("save_state: no buffer");
+    if (!buf || size < upside_batch_state_size(b)) fail("save_state: buffer too small");
     b->batch.save_state((char*)buf, (size_t)size);
+    const vector<long long> v = loop_state(b);
+    const long long n = (long long)v.size();
+    char* p = (char*)buf + b->batch.state_size();
+    std::memcpy(p, &n, sizeof(n));
+    std::memcpy(p + sizeof(n), v.data(), v.size() * sizeof(long long));
     return 0;
     ABI_CATCH(1)
 }
@@ -2896,6 +2923,48 @@ int upside_batch_load_state(UpsideBatch* b, const void* buf, long long size) {
     ABI_TRY
     if (!buf || size < 0) fail("load_state: no buffer");
     b->batch.load_state((const char*)buf, (size_t)size);
+    // the loop state follows the device state (whose size is now this batch's)
+    const size_t at = b->batch.state_size();
+    long long n = 0;
+    if ((size_t)size < at + sizeof(n)) fail("load_state: truncated state");
+    std::memcpy(&n, (const char*)buf + at, sizeof(n));
+    if (n < 8 || (size_t)size < at + sizeof(n) * (1 + (size_t)n)) fail("load_state: truncated state");
+    vector<long long> v(n);
+    std::memcpy(v.data(), (const char*)buf + at + sizeof(n), (size_t)n * sizeof(long long));
+    if (v[0] != 0x55505354415445ll) fail("load_state: not a saved batch state");
+    size_t i = 1;
+    auto next = [&]() {
+        if (i >= v.size()) fail("load_state: truncated loop state");
+        return v[i++];
+    };
+    b->mc_interval = (int)next();
+    b->recenter = (int)next();
+    b->rex_interval = (int)next();
+    b->rex_criterion = (int)next();
+    b->rex_attempts = next();
+    b->rex_next = (uint64_t)next();
+    b->mc_total.assign((size_t)next(), {0, 0});
+    for (auto& t : b->mc_total) { t[0] = next(); t[1] = next(); }
+    b->mc_frame.assign(b->mc_total.size(), {0, 0});
+    b->rex_sets.assign((size_t)next(), {});
+    b->rex_stats.assign(b->rex_sets.size(), {});
+    for (size_t s = 0; s < b->rex_sets.size(); ++s) {
+        const size_t np = (size_t)next();
+        for (size_t p = 0; p < np; ++p) {
+            const int x = (int)next(), y = (int)next();
+            const long long su = next(), at2 = next();
+            b->rex_sets[s].push_back({x, y});
+            b->rex_stats[s].push_back({su, at2});
+        }
+    }
+    b->replica_of.assign((size_t)next(), 0);
+    for (auto& x : b->replica_of) x = (int)next();
+    const int ns = (int)b->batch.sys.size();
+    b->rex_partner.assign(ns, {});
+    for (size_t s = 0; s < b->rex_sets.size(); ++s)
+        for (size_t p = 0; p < b->rex_sets[s].size(); ++p)
+            for (int x : {b->rex_sets[s][p][0], b->rex_sets[s][p][1]})
+                if (x >= 0 && x < ns) b->rex_partner[x].push_back({(int)s, (int)p});
     return 0;
     ABI_CATCH(1)
 }

@@ -287,9 +287,10 @@ class Batch:
             raise RuntimeError("set_replica_exchange failed (see stderr)")
         self._n_pairs = int(sizes.sum())
 
-    def replica_stats(self):
-        """(replica index of each system, successes and attempts of every pair)."""
-        n = getattr(self, "_n_pairs", 0)
+    def replica_stats(self, n_pairs=None):
+        """(replica index of each system, successes and attempts of every pair);
+        n_pairs defaults to the pairs given to set_replica_exchange."""
+        n = getattr(self, "_n_pairs", 0) if n_pairs is None else int(n_pairs)
         idx = np.zeros(self.n_sys, np.intc)
         suc, att = np.zeros(max(n, 1), np.int64), np.zeros(max(n, 1), np.int64)
         if self.calc.upside_batch_replica_stats(self.h, idx.ctypes.data, suc.ctypes.data, att.ctypes.data):

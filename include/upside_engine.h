@@ -165,6 +165,10 @@ int upside_batch_close_output(UpsideBatch* batch);
  * set.  No counterpart in the reference's C ABI (its CLI option
  * --monte-carlo-interval, src/main.cpp:471, :570-571). */
 int upside_batch_set_monte_carlo(UpsideBatch* batch, int mc_interval);
+/* Thermostat timescale (--thermostat-timescale, src/main.cpp:477, :775-781;
+ * 5 by default): only before the first round; the fresh-start
+ * thermalisation is redone with the new parameters. */
+int upside_batch_set_thermostat_timescale(UpsideBatch* batch, float timescale);
 /* Recentering before every logged frame, as the reference CLI does unless
  * --disable-recentering (src/main.cpp:576-577, :947; src/deriv_engine.cpp:39-49):
  * mode 0 none (the default here), 1 subtract the centroid, 2 its x and y only

@@ -589,6 +589,43 @@ def test_recentering_matches_reference_cli(ue, chain_file, tmp_path):
         b.set_recentering(3)
 
 
+def test_thermostat_timescale_matches_reference_cli(ue, chain_file, tmp_path):
+    """--thermostat-timescale 2: 6 rounds with frames every round against the
+    reference CLI, positions and potentials within 1e-4 relative."""
+    import shutil
+    import subprocess
+    import h5lite
+    from conftest import REF_BIN
+    if not os.path.exists(REF_BIN):
+        pytest.skip("reference CLI not built")
+    src = chain_file(40, 21)
+    ref_up, our_up = str(tmp_path / "ref.up"), str(tmp_path / "ours.up")
+    shutil.copy(src, ref_up)
+    shutil.copy(src, our_up)
+    say("case: thermostat timescale 2, L=40, 6 rounds, against the reference CLI")
+    r = subprocess.run([REF_BIN, "--duration", "0.162", "--frame-interval", "0.027", "--temperature", "0.85",
+                        "--thermostat-timescale", "2", "--seed", "7", "--log-level", "basic",
+                        "--disable-recentering", ref_up], env=dict(os.environ, OMP_NUM_THREADS="1"),
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    b = ue.Batch([our_up], [0.85], seed=7, dt=0.009)
+    b.set_thermostat_timescale(2.0)
+    b.open_output(None)
+    b.run_logged(6, 1)
+    b.close_output()
+    with h5lite.File(ref_up) as f:
+        want_pos, want_pot = f.read("/output/pos"), f.read("/output/potential")
+    with h5lite.File(our_up) as f:
+        got_pos, got_pot = f.read("/output/pos"), f.read("/output/potential")
+    for fr in range(got_pos.shape[0]):
+        e = rel_rms(got_pos[fr, 0], want_pos[fr, 0])
+        de = abs(got_pot[fr, 0] - want_pot[fr, 0]) / max(abs(want_pot[fr, 0]), 1.0)
+        say("frame %d: pos rel %.2e potential rel %.2e" % (fr, e, de))
+        assert e < 1e-4 and de < 1e-4, (fr, e, de)
+    with pytest.raises(RuntimeError):
+        b.set_thermostat_timescale(3.0)       # only before the first round
+
+
 def test_output_logger(ue, chain_file, tmp_path):
     """/output datasets as the reference's basic log level writes them: frames
     at rounds 0, 3, 6 of a 7-round run hold the state before that round's

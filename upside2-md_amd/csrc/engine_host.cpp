@@ -1448,6 +1448,7 @@ struct Batch {
     bool have_mom = false;
     double cap_scale = 1.0;
     uint64_t round_num = 0;      // rounds integrated since creation (src/main.cpp:920 round_num)
+    float thermostat_timescale = 5.f;  // --thermostat-timescale (src/main.cpp:477, :775-781)
     bool profile = false;
     vector<unsigned long long*> prof;
     int prof_len(int k) const { return sys[k].prog->n_germ() + sys[k].prog->n_deriv() + PROF_EXTRA; }
@@ -1559,8 +1560,7 @@ struct Batch {
         d.seed = base_seed + (uint32_t)k;
         // OrnsteinUhlenbeckThermostat parameters (src/thermostat.h:9-12, src/main.cpp:775-781)
         float delta_t = (float)thermostat_interval * 3.f * dt;
-        float timescale = 5.f;
-        float ms = std::exp(-delta_t / timescale);
+        float ms = std::exp(-delta_t / thermostat_timescale);
         d.mom_scale = ms;
         d.noise_scale = std::sqrt(s.temperature * (1.f - ms * ms));
         d.n_invocations = 0;
@@ -1802,7 +1802,7 @@ struct Batch {
         uint32_t base_seed;
         double cap_scale;
         uint64_t round_num;
-        float dt, pad2;
+        float dt, timescale;
     };
     struct SysState {
         long long nf, ni;
@@ -1831,6 +1831,7 @@ struct Batch {
         h.cap_scale = cap_scale;
         h.round_num = round_num;
         h.dt = dt;
+        h.timescale = thermostat_timescale;
         std::memcpy(out, &h, sizeof(h));
         out += sizeof(h);
         for (size_t k = 0; k < sys.size(); ++k) {
@@ -1894,6 +1895,7 @@ struct Batch {
         thermostat_interval = h.thermostat_interval;
         base_seed = h.base_seed;
         round_num = h.round_num;
+        thermostat_timescale = h.timescale;
         have_mom = true;
         push_desc();
     }
@@ -2752,6 +2754,25 @@ int upside_batch_set_recentering(UpsideBatch* b, int mode) {
     ABI_TRY
     if (mode < 0 || mode > 2) fail("set_recentering: mode must be 0 (none), 1 (x, y, z) or 2 (x, y)");
     b->recenter = mode;
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_set_thermostat_timescale(UpsideBatch* b, float timescale) {
+    ABI_TRY
+    auto& B = b->batch;
+    if (!(timescale > 0.f)) fail("set_thermostat_timescale: timescale must be positive");
+    if (B.round_num != 0) fail("set_thermostat_timescale: only before the first round");
+    B.thermostat_timescale = timescale;
+    B.pull_desc();
+    for (size_t k = 0; k < B.sys.size(); ++k) {
+        const SysDesc d = B.make_desc((int)k);
+        B.desc[k].mom_scale = d.mom_scale;
+        B.desc[k].noise_scale = d.noise_scale;
+        B.desc[k].n_invocations = 0;
+    }
+    B.push_desc();
+    B.thermalize();  // the fresh-start thermalisation again, with the new parameters (src/main.cpp:800-803)
     return 0;
     ABI_CATCH(1)
 }

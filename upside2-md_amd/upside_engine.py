@@ -70,6 +70,7 @@ def load_upside_library(path=LIB_PATH):
         "upside_batch_close_output": (i, [vp]),
         "upside_batch_set_monte_carlo": (i, [vp, i]),
         "upside_batch_set_recentering": (i, [vp, i]),
+        "upside_batch_set_thermostat_timescale": (i, [vp, f]),
         "upside_batch_set_replica_exchange": (i, [vp, i, i, i, vp, vp]),
         "upside_batch_replica_stats": (i, [vp, vp, vp, vp]),
         "upside_batch_mc_stats": (i, [vp, i, vp, vp]),
@@ -296,6 +297,11 @@ class Batch:
         if self.calc.upside_batch_replica_stats(self.h, idx.ctypes.data, suc.ctypes.data, att.ctypes.data):
             raise RuntimeError("replica_stats failed")
         return idx, suc[:n], att[:n]
+
+    def set_thermostat_timescale(self, timescale):
+        """Thermostat timescale (before the first round; re-thermalises)."""
+        if self.calc.upside_batch_set_thermostat_timescale(self.h, float(timescale)):
+            raise RuntimeError("set_thermostat_timescale failed (see stderr)")
 
     def set_recentering(self, mode):
         """Recenter before each logged frame: 0 none, 1 x/y/z, 2 x/y only."""

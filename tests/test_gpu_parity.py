@@ -3,9 +3,11 @@
 Bars (written per test): total energy within 1e-4 relative (|dE|/max(|E|,1):
 ff_2.1 totals cancel to near zero on folded frames) and dE/dx within 1e-4
 relative RMS (||F-F_ref||/||F_ref||) of the float32 oracle, of the FP64
-oracle and, where oracle/_ref is present, of the compiled reference; pair
-lists and BP edge sets bit-exact; integer statistics (edge counts by type,
-BP iterations) equal.
+oracle and, where oracle/_ref is present, of the compiled reference (against
+FP64 and the reference: 1e-4 or 1.25 times the reference's own distance from
+FP64, whichever is larger, since float32 rounding grows with chain length);
+pair lists and BP edge sets bit-exact; integer statistics (edge counts by
+type, BP iterations) equal.
 """
 import os
 
@@ -45,12 +47,18 @@ def check_point(ue_eng, ora, ora64, pos, label, ref=None):
     e64, d64 = float(ora64.energy(pos.astype(np.float64))), ora64.deriv(pos.astype(np.float64))
     err = dict(e32=abs(e - e32) / max(abs(e32), 1.0), f32=rel_rms(d, d32),
                e64=abs(e - e64) / max(abs(e64), 1.0), f64=rel_rms(d, d64))
+    # float32 rounding grows with the chain: where the compiled reference is
+    # itself further than F_TOL from float64 (about 2e-4 at 1000 residues),
+    # the bar against float64 and the reference is 1.25 times its own distance
+    f_wide = F_TOL
     if ref is not None:
         er, dr = ref.energy(pos), ref.deriv(pos)
         err.update(eref=abs(e - er) / max(abs(er), 1.0), fref=rel_rms(d, dr))
-    say(label, " ".join("%s=%.2e" % kv for kv in sorted(err.items())))
+        f_wide = max(F_TOL, 1.25 * rel_rms(dr, d64))
+    say(label, " ".join("%s=%.2e" % kv for kv in sorted(err.items())), "(force bar vs f64/ref %.2e)" % f_wide)
     for k, v in err.items():
-        assert v < (E_TOL if k.startswith("e") else F_TOL), (label, k, v)
+        bar = E_TOL if k.startswith("e") else (F_TOL if k == "f32" else f_wide)
+        assert v < bar, (label, k, v, bar)
     return e, d
 
 
@@ -71,7 +79,7 @@ def test_chig_frames(ue, chig):
     assert dev.max() < 2e-3
 
 
-@pytest.mark.parametrize("n_res,seed", [(40, 1), (100, 2), (500, 3)])
+@pytest.mark.parametrize("n_res,seed", [(40, 1), (100, 2), (500, 3), (1000, 4)])
 def test_random_chain_energy_forces(ue, chain_file, n_res, seed):
     path = chain_file(n_res, seed)
     say("case: random chain L=%d seed=%d" % (n_res, seed))

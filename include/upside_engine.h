@@ -169,6 +169,13 @@ int upside_batch_set_monte_carlo(UpsideBatch* batch, int mc_interval);
  * 5 by default): only before the first round; the fresh-start
  * thermalisation is redone with the new parameters. */
 int upside_batch_set_thermostat_timescale(UpsideBatch* batch, float timescale);
+/* Simulated annealing as --anneal-factor, --anneal-start, --anneal-end and
+ * --anneal-duration (src/main.cpp:593-612, :975-977), times in the CLI's
+ * units: each thermostat application at round r uses the schedule's
+ * temperature at time 3 dt (r+1), which replica exchange and the logged
+ * temperature then see.  The CLI's default duration (the whole run) must be
+ * given as `duration` or through an end after the start; factor 1 disables. */
+int upside_batch_set_annealing(UpsideBatch* batch, double factor, double start, double end, double duration);
 /* Recentering before every logged frame, as the reference CLI does unless
  * --disable-recentering (src/main.cpp:576-577, :947; src/deriv_engine.cpp:39-49):
  * mode 0 none (the default here), 1 subtract the centroid, 2 its x and y only

@@ -597,6 +597,47 @@ def test_recentering_matches_reference_cli(ue, chain_file, tmp_path):
         b.set_recentering(3)
 
 
+def test_annealing_matches_reference_cli(ue, chain_file, tmp_path):
+    """--anneal-factor 0.5 --anneal-start 0.03 --anneal-end 0.13 over 6 rounds
+    (the schedule holds T0, then blends, then reaches T1 inside the run)
+    against the reference CLI: logged temperatures equal, positions and
+    potentials within 1e-4 relative."""
+    import shutil
+    import subprocess
+    import h5lite
+    from conftest import REF_BIN
+    if not os.path.exists(REF_BIN):
+        pytest.skip("reference CLI not built")
+    src = chain_file(40, 21)
+    ref_up, our_up = str(tmp_path / "ref.up"), str(tmp_path / "ours.up")
+    shutil.copy(src, ref_up)
+    shutil.copy(src, our_up)
+    say("case: annealing, L=40, 6 rounds, against the reference CLI")
+    r = subprocess.run([REF_BIN, "--duration", "0.162", "--frame-interval", "0.027", "--temperature", "0.85",
+                        "--anneal-factor", "0.5", "--anneal-start", "0.03", "--anneal-end", "0.13",
+                        "--seed", "7", "--log-level", "basic", "--disable-recentering", ref_up],
+                       env=dict(os.environ, OMP_NUM_THREADS="1"), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    b = ue.Batch([our_up], [0.85], seed=7, dt=0.009)
+    b.set_annealing(0.5, 0.03, 0.13)
+    b.open_output(None)
+    b.run_logged(6, 1)
+    b.close_output()
+    keys = ("pos", "potential", "temperature")
+    with h5lite.File(ref_up) as f:
+        want = {k: f.read("/output/" + k) for k in keys}
+    with h5lite.File(our_up) as f:
+        got = {k: f.read("/output/" + k) for k in keys}
+    say("temperatures ours %s reference %s" % (got["temperature"].ravel().tolist(),
+                                               want["temperature"].ravel().tolist()))
+    np.testing.assert_array_equal(got["temperature"], want["temperature"])
+    for fr in range(got["pos"].shape[0]):
+        e = rel_rms(got["pos"][fr, 0], want["pos"][fr, 0])
+        de = abs(got["potential"][fr, 0] - want["potential"][fr, 0]) / max(abs(want["potential"][fr, 0]), 1.0)
+        say("frame %d: pos rel %.2e potential rel %.2e" % (fr, e, de))
+        assert e < 1e-4 and de < 1e-4, (fr, e, de)
+
+
 def test_thermostat_timescale_matches_reference_cli(ue, chain_file, tmp_path):
     """--thermostat-timescale 2: 6 rounds with frames every round against the
     reference CLI, positions and potentials within 1e-4 relative."""

@@ -211,6 +211,7 @@ struct SysDesc {
     uint32_t seed;
     float mom_scale, noise_scale;
     uint64_t n_invocations;
+    float temp0;           // initial temperature (annealing schedule)
     int error;             // int offset of error word
     int bar;               // int offset (16-byte aligned) of the team barrier words: [0..1] 64-bit
                            // arrival counter, [2] spin-timeout flag, [3] placement report, [4..4+MAX_TEAM)
@@ -240,6 +241,11 @@ struct RunArgs {
     int n_frames;
     int team;              // workgroups per system (1: no grid barriers)
     int n_sys;             // systems in the launch
+    // simulated annealing (src/main.cpp:593-612, :975-977): when anneal is set
+    // the thermostat temperature at round r is the schedule at time
+    // 3 dt (r+1) from each system's initial temperature
+    int anneal;
+    double anneal_factor, anneal_start, anneal_end, anneal_duration;
 };
 
 constexpr int MAX_TEAM = 64;

@@ -1430,7 +1430,8 @@ struct System {
     std::shared_ptr<Program> prog;
     float* arena = nullptr;
     int* iarena = nullptr;
-    float temperature = 1.f;
+    float temperature = 1.f;   // current (annealed) temperature
+    float temp0 = 1.f;         // initial temperature
     void release() { (void)hipFree(arena); (void)hipFree(iarena); arena = nullptr; iarena = nullptr; }
 };
 
@@ -1449,6 +1450,23 @@ struct Batch {
     double cap_scale = 1.0;
     uint64_t round_num = 0;      // rounds integrated since creation (src/main.cpp:920 round_num)
     float thermostat_timescale = 5.f;  // --thermostat-timescale (src/main.cpp:477, :775-781)
+    // simulated annealing (--anneal-factor/-start/-end/-duration, src/main.cpp:593-612)
+    int anneal = 0;
+    double anneal_factor = 1.0, anneal_start = 0.0, anneal_end = -1.0, anneal_duration = 1.0;
+    double anneal_temp(double T0, double time) const {  // the reference's anneal_temp, in double
+        const double T1 = T0 * anneal_factor;
+        if (time <= anneal_start) return T0;
+        if (time > anneal_end) return T1;
+        const double f = std::max(0.0, (time - anneal_start) / anneal_duration);
+        const double s = std::sqrt(T0) * (1.0 - f) + std::sqrt(T1) * f;
+        return s * s;
+    }
+    // temperature in effect after the thermostat of the round before round_num
+    void sync_temperature() {
+        if (!anneal || round_num == 0) return;
+        const float time = 3.f * dt * (float)round_num;
+        for (auto& s : sys) s.temperature = (float)anneal_temp((double)s.temp0, (double)time);
+    }
     bool profile = false;
     vector<unsigned long long*> prof;
     int prof_len(int k) const { return sys[k].prog->n_germ() + sys[k].prog->n_deriv() + PROF_EXTRA; }
@@ -1564,6 +1582,7 @@ struct Batch {
         d.mom_scale = ms;
         d.noise_scale = std::sqrt(s.temperature * (1.f - ms * ms));
         d.n_invocations = 0;
+        d.temp0 = s.temp0;
         d.prof = (profile && k < (int)prof.size()) ? prof[k] : nullptr;
         return d;
     }
@@ -1591,6 +1610,7 @@ struct Batch {
             System s;
             s.prog = it->second;
             s.temperature = k < temps.size() ? temps[k] : 1.f;
+            s.temp0 = s.temperature;
             sys.push_back(s);
         }
         for (auto& s : sys) alloc_system(s);
@@ -1771,6 +1791,11 @@ struct Batch {
         a.frame_interval = frame_interval;
         a.frame_pot = d_frames;
         a.n_frames = n_frames;
+        a.anneal = anneal;
+        a.anneal_factor = anneal_factor;
+        a.anneal_start = anneal_start;
+        a.anneal_end = anneal_end;
+        a.anneal_duration = anneal_duration;
         for (;;) {
             launch(a);
             if (!overflowed()) break;
@@ -1809,6 +1834,7 @@ struct Batch {
         uint64_t n_invocations;
         float temperature, mom_scale, noise_scale;
         uint32_t seed;
+        float temp0, pad;
     };
     size_t state_size() const {
         size_t n = sizeof(StateHeader);
@@ -1845,6 +1871,7 @@ struct Batch {
             st.mom_scale = desc[k].mom_scale;
             st.noise_scale = desc[k].noise_scale;
             st.seed = desc[k].seed;
+            st.temp0 = sys[k].temp0;
             std::memcpy(out, &st, sizeof(st));
             out += sizeof(st);
             hip_check(hipMemcpy(out, sys[k].arena, st.nf * 4, hipMemcpyDeviceToHost), "save arena");
@@ -1889,6 +1916,8 @@ struct Batch {
             desc[k].mom_scale = st[k].mom_scale;
             desc[k].noise_scale = st[k].noise_scale;
             desc[k].seed = st[k].seed;
+            sys[k].temp0 = st[k].temp0;
+            desc[k].temp0 = st[k].temp0;
             sys[k].temperature = st[k].temperature;
         }
         dt = h.dt;
@@ -2758,6 +2787,25 @@ int upside_batch_set_recentering(UpsideBatch* b, int mode) {
     ABI_CATCH(1)
 }
 
+int upside_batch_set_annealing(UpsideBatch* b, double factor, double start, double end, double duration) {
+    ABI_TRY
+    auto& B = b->batch;
+    if (!(factor > 0.0)) fail("set_annealing: factor must be positive");
+    // the reference's option handling (src/main.cpp:593-600); the run's total
+    // duration, the CLI's default for the annealing duration, is the caller's
+    if (start < 0.0) start = 0.0;
+    if (end > start) duration = end - start;
+    if (factor != 1.0 && !(duration > 0.0)) fail("set_annealing: give an end after the start or a positive duration");
+    B.anneal = factor != 1.0;
+    B.anneal_factor = factor;
+    B.anneal_start = start;
+    B.anneal_end = end;
+    B.anneal_duration = duration;
+    B.sync_temperature();
+    return 0;
+    ABI_CATCH(1)
+}
+
 int upside_batch_set_thermostat_timescale(UpsideBatch* b, float timescale) {
     ABI_TRY
     auto& B = b->batch;
@@ -2867,6 +2915,7 @@ int upside_batch_run_logged(UpsideBatch* b, int n_round, int frame_interval) {
         if (b->rex_interval > 0 && b->rex_next > r) n = std::min<int64_t>(n, (int64_t)(b->rex_next - r));
         B.run(n, 0, nullptr, 0, (int)B.round_num);
         B.round_num += n;
+        B.sync_temperature();
         done += n;
         if (b->rex_interval > 0 && B.round_num == b->rex_next) {
             const int set = (int)(b->rex_attempts % (long long)b->rex_sets.size());
@@ -2903,8 +2952,11 @@ int upside_batch_close_output(UpsideBatch* b) {
 // counts, replica-exchange configuration, replica indices, pair statistics and
 // next swap round, recentering mode (ints and 64-bit counts in one stream).
 static vector<long long> loop_state(const UpsideBatch* b) {
+    auto bits = [](double x) { long long r; std::memcpy(&r, &x, sizeof(r)); return r; };
     vector<long long> v{0x55505354415445ll, b->mc_interval, b->recenter, b->rex_interval, b->rex_criterion,
-                        b->rex_attempts, (long long)b->rex_next, (long long)b->mc_total.size()};
+                        b->rex_attempts, (long long)b->rex_next, b->batch.anneal, bits(b->batch.anneal_factor),
+                        bits(b->batch.anneal_start), bits(b->batch.anneal_end), bits(b->batch.anneal_duration),
+                        (long long)b->mc_total.size()};
     for (auto& t : b->mc_total) { v.push_back(t[0]); v.push_back(t[1]); }
     v.push_back((long long)b->rex_sets.size());
     for (size_t s = 0; s < b->rex_sets.size(); ++s) {
@@ -2949,7 +3001,7 @@ int upside_batch_load_state(UpsideBatch* b, const void* buf, long long size) {
     long long n = 0;
     if ((size_t)size < at + sizeof(n)) fail("load_state: truncated state");
     std::memcpy(&n, (const char*)buf + at, sizeof(n));
-    if (n < 8 || (size_t)size < at + sizeof(n) * (1 + (size_t)n)) fail("load_state: truncated state");
+    if (n < 13 || (size_t)size < at + sizeof(n) * (1 + (size_t)n)) fail("load_state: truncated state");
     vector<long long> v(n);
     std::memcpy(v.data(), (const char*)buf + at + sizeof(n), (size_t)n * sizeof(long long));
     if (v[0] != 0x55505354415445ll) fail("load_state: not a saved batch state");
@@ -2964,6 +3016,13 @@ int upside_batch_load_state(UpsideBatch* b, const void* buf, long long size) {
     b->rex_criterion = (int)next();
     b->rex_attempts = next();
     b->rex_next = (uint64_t)next();
+    auto dbl = [](long long x) { double r; std::memcpy(&r, &x, sizeof(r)); return r; };
+    b->batch.anneal = (int)next();
+    b->batch.anneal_factor = dbl(next());
+    b->batch.anneal_start = dbl(next());
+    b->batch.anneal_end = dbl(next());
+    b->batch.anneal_duration = dbl(next());
+    b->batch.sync_temperature();
     b->mc_total.assign((size_t)next(), {0, 0});
     for (auto& t : b->mc_total) { t[0] = next(); t[1] = next(); }
     b->mc_frame.assign(b->mc_total.size(), {0, 0});
@@ -2996,6 +3055,7 @@ int upside_batch_run(UpsideBatch* b, int n_round, int frame_interval, float* fra
     if (n_round < 0) fail("run: negative round count");
     b->batch.run(n_round, frame_interval, frame_pot, n_frames, (int)b->batch.round_num);
     b->batch.round_num += n_round;
+    b->batch.sync_temperature();
     return 0;
     ABI_CATCH(1)
 }

@@ -1061,12 +1061,25 @@ __device__ void normal3(uint32_t seed, uint32_t stream, uint32_t atom, uint64_t 
     o[2] = sinf(t3) * r3;
 }
 
-__device__ void thermostat(Ctx& c, SysDesc& S) {
+// Temperature of the annealing schedule at `time` (the anneal_temp lambda of
+// src/main.cpp:603-612, in double): T0 until the start, T0*factor after the
+// end, between them the square of a linear blend of the square roots.
+__device__ double anneal_temperature(const RunArgs& a, double T0, double time) {
+    const double T1 = T0 * a.anneal_factor;
+    if (time <= a.anneal_start) return T0;
+    if (time > a.anneal_end) return T1;
+    double f = (time - a.anneal_start) / a.anneal_duration;
+    f = f > 0.0 ? f : 0.0;
+    const double s = sqrt(T0) * (1.0 - f) + sqrt(T1) * f;
+    return s * s;
+}
+
+__device__ void thermostat(Ctx& c, SysDesc& S, float noise_scale) {
     auto mom = gmem(c.A + S.mom);
     for (int a = c.gtid; a < S.n_atom; a += c.gsize) {
         float n[3];
         normal3(S.seed, 0u, (uint32_t)a, S.n_invocations, n);
-        for (int k = 0; k < 3; ++k) mom[a * 3 + k] = S.mom_scale * mom[a * 3 + k] + S.noise_scale * n[k];
+        for (int k = 0; k < 3; ++k) mom[a * 3 + k] = S.mom_scale * mom[a * 3 + k] + noise_scale * n[k];
     }
     team_sync(c);
     if (c.gtid == 0) S.n_invocations += 1;
@@ -1116,7 +1129,7 @@ __device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& arg
         return;
     }
     if (args.n_round < 0) {  // thermostat application only (fresh-start thermalisation)
-        thermostat(c, S);
+        thermostat(c, S, S.noise_scale);
         return;
     }
     auto pos = gmem(c.A + S.pos_out);
@@ -1134,7 +1147,15 @@ __device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& arg
             team_sync(c);
         }
         unsigned long long t0 = ptick(c);
-        if (args.thermostat_interval > 0 && round % args.thermostat_interval == 0) thermostat(c, S);
+        if (args.thermostat_interval > 0 && round % args.thermostat_interval == 0) {
+            float ns = S.noise_scale;
+            if (args.anneal) {  // set_temperature then the update_parameters of src/thermostat.h:9-12
+                const float time = 3.f * args.dt * (float)(round + 1);
+                const float temp = (float)anneal_temperature(args, (double)S.temp0, (double)time);
+                ns = sqrtf(temp * (1.f - S.mom_scale * S.mom_scale));
+            }
+            thermostat(c, S, ns);
+        }
         pacc(c, PROF_THERMO, t0);
         for (int stage = 0; stage < 3; ++stage) {
             compute(c, S, 0);

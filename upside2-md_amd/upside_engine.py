@@ -71,6 +71,7 @@ def load_upside_library(path=LIB_PATH):
         "upside_batch_set_monte_carlo": (i, [vp, i]),
         "upside_batch_set_recentering": (i, [vp, i]),
         "upside_batch_set_thermostat_timescale": (i, [vp, f]),
+        "upside_batch_set_annealing": (i, [vp, ct.c_double, ct.c_double, ct.c_double, ct.c_double]),
         "upside_batch_set_replica_exchange": (i, [vp, i, i, i, vp, vp]),
         "upside_batch_replica_stats": (i, [vp, vp, vp, vp]),
         "upside_batch_mc_stats": (i, [vp, i, vp, vp]),
@@ -302,6 +303,11 @@ class Batch:
         """Thermostat timescale (before the first round; re-thermalises)."""
         if self.calc.upside_batch_set_thermostat_timescale(self.h, float(timescale)):
             raise RuntimeError("set_thermostat_timescale failed (see stderr)")
+
+    def set_annealing(self, factor, start=-1.0, end=-1.0, duration=-1.0):
+        """Simulated annealing with the CLI's --anneal-* meanings (times in time units)."""
+        if self.calc.upside_batch_set_annealing(self.h, float(factor), float(start), float(end), float(duration)):
+            raise RuntimeError("set_annealing failed (see stderr)")
 
     def set_recentering(self, mode):
         """Recenter before each logged frame: 0 none, 1 x/y/z, 2 x/y only."""

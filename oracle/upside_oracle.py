@@ -51,6 +51,11 @@ def ru(n):
     return n if n == 1 else (n + 3) // 4 * 4
 
 
+def _scalar(v):
+    """A numeric attribute stored either as a scalar or as a 1-element array."""
+    return np.asarray(v).reshape(-1)[0]
+
+
 def compact_sigmoid(x, sharpness):
     """src/vector_math.h:701-720: value and derivative, 1 below -1/s, 0 above 1/s."""
     y = x * sharpness
@@ -310,9 +315,9 @@ class Spring(PotentialNode):
         self.id = f.read(g + "/id").astype(np.int64)
         self.eq = f.read(g + "/equil_dist").astype(self.T)
         self.k = f.read(g + "/spring_const").astype(self.T)
-        self.dim1 = int(f.attr(g, "dim1"))
-        self.pbc = int(f.attr(g, "pbc"))
-        self.box = self.T(f.attr(g, "box_len"))
+        self.dim1 = int(_scalar(f.attr(g, "dim1")))
+        self.pbc = int(_scalar(f.attr(g, "pbc")))
+        self.box = self.T(_scalar(f.attr(g, "box_len")))
 
     def compute(self, mode):
         x = self.arg(0).output[self.id, self.dim1]
@@ -348,7 +353,7 @@ class CatPos(CoordNode):
     def load(self, f, g):
         self.i1 = f.read(g + "/index_pos1").astype(np.int64)
         self.i2 = f.read(g + "/index_pos2").astype(np.int64)
-        self.w = int(f.attr(g, "n_dim"))
+        self.w = int(_scalar(f.attr(g, "n_dim")))
         self.alloc(len(self.i1) + len(self.i2), self.w)
 
     def compute(self, mode):
@@ -951,7 +956,7 @@ class EnvironmentCoverage(CoordNode):
     """environment_coverage (src/environment.cpp:288-348): binned by aa type."""
 
     def load(self, f, g):
-        self.n_aa = int(f.attr(g, "num_aa_types"))
+        self.n_aa = int(_scalar(f.attr(g, "num_aa_types")))
         self.ig = IGraph(f, g, self.T, self.arg(0), self.arg(1), EnvCoverage(True))
         self.aa = f.read(g + "/aa_types").astype(np.int64)
         self.alloc(len(self.ig.loc1) * self.n_aa, 1)
@@ -1023,12 +1028,12 @@ class BBSigmoidCoupling(PotentialNode):
     """bb_sigmoid_coupling (src/environment.cpp:1054-1138); args (env_sc, env_hb)."""
 
     def load(self, f, g):
-        self.nt = int(f.attr(g, "num_aa_types"))
+        self.nt = int(_scalar(f.attr(g, "num_aa_types")))
         T = self.T
-        self.scale = T(np.float32(f.attr(g, "scale")))
-        self.center = T(np.float32(f.attr(g, "center")))
-        self.sharp = T(np.float32(f.attr(g, "sharpness")))
-        self.hw = T(np.float32(f.attr(g, "hbond_weight")))
+        self.scale = T(np.float32(_scalar(f.attr(g, "scale"))))
+        self.center = T(np.float32(_scalar(f.attr(g, "center"))))
+        self.sharp = T(np.float32(_scalar(f.attr(g, "sharpness"))))
+        self.hw = T(np.float32(_scalar(f.attr(g, "hbond_weight"))))
         self.w = f.read(g + "/weights").astype(np.float32).astype(T)[:self.nt]
 
     def compute(self, mode):
@@ -1158,10 +1163,10 @@ class Rotamer(PotentialNode):
     def load(self, f, g):
         self.ig = IGraph(f, g + "/pair_interaction", self.T, self.arg(0), None, RotamerPair())
         self.prob_nodes = [self.eng.nodes[a] for a in self.args[1:]]
-        self.damping = self.T(np.float32(f.attr(g, "damping")))
-        self.max_iter = int(f.attr(g, "max_iter"))
-        self.tol = np.float32(f.attr(g, "tol"))
-        self.chunk = int(f.attr(g, "iteration_chunk_size"))
+        self.damping = self.T(np.float32(_scalar(f.attr(g, "damping"))))
+        self.max_iter = int(_scalar(f.attr(g, "max_iter")))
+        self.tol = np.float32(_scalar(f.attr(g, "tol")))
+        self.chunk = int(_scalar(f.attr(g, "iteration_chunk_size")))
         ids = self.ig.id1
         self.rot = ids & 15
         self.nrot = (ids >> 4) & 15

@@ -59,9 +59,12 @@ int get_output_dims(int* n_elem, int* elem_width, DerivEngine* engine, const cha
 int get_output(int n_output, float* output, DerivEngine* engine, const char* node_name);
 int get_sens(int n_output, float* output, DerivEngine* engine, const char* node_name);
 /* src/engine_c_library.cpp:181-195 and the node hooks of src/rotamer.cpp:736-834,
- * src/hbond.cpp:476-482.  Supported names: rotamer: "rotamer_free_energy",
+ * src/hbond.cpp:476-482.  Supported names: rotamer: "rotamer_free_energy"
+ * (n_node), "rotamer_1body_energy" (n_node x energy inputs), "node_energy"
+ * (n_node x 6), "edge_energy" and "edge_marginal_in_graph_order" (n_node x
+ * n_node x 6 x 6, nodes numbered by class 1, 3, 6 rotamers as the reference),
  * "count_edges_by_type", "n_node", "read n_bad_solve", "read n_bad_solve and
- * reset"; any pair-graph node: "count_edges_by_type", "n_edge", "edge_pairs"
+ * reset" (values of the last evaluation); any pair-graph node: "count_edges_by_type", "n_edge", "edge_pairs"
  * (2*n_edge element indices, sorted), "edge_values" (n_edge, same order).
  * Measurement hooks with no reference counterpart: pair graphs "graph_shape"
  * (n1, dim1, n2, dim2); rotamer "iterations" (of the last solve),

@@ -168,6 +168,59 @@ def test_rotamer_bp_statistics(ue, chain_file):
     assert eng.get_value_by_name((1,), "rotamer", "read n_bad_solve and reset")[0] == 0
 
 
+def test_rotamer_values_match_reference(ue, chain_file):
+    """The rotamer node's value hooks (src/rotamer.cpp:736-834) equal the
+    compiled reference's at the same coordinates: per-residue free energies
+    and 1-body energies, node energies, edge energies and edge marginals in
+    the reference's graph order.  Tolerances: 2e-4 relative to the largest
+    magnitude for values fixed before BP (node and edge energies, float32
+    exp/log of the same sums); 2e-3 absolute for BP-dependent values (BP stops
+    at a belief change of 1e-3, and a float32 rounding difference can end it
+    one chunk earlier or later)."""
+    import h5lite
+    lib = ref_lib()
+    if lib is None:
+        pytest.skip("reference engine not built")
+    path = chain_file(50, 21)
+    pos0 = read_pos(path)
+    say("case: rotamer value hooks L=50")
+    eng = ue.Upside(path, quiet=True)
+    ref = RefEngine(lib, path, len(pos0))
+    rng = np.random.default_rng(8)
+    for k in range(2):
+        pos = (pos0 + rng.normal(scale=0.1, size=pos0.shape)).astype(np.float32)
+        eng.energy(pos)
+        ref.energy(pos)
+        nn = int(ref.value((1,), "rotamer", "n_node")[0])
+        assert int(eng.get_value_by_name((1,), "rotamer", "n_node")[0]) == nn
+        with h5lite.File(path) as f:
+            n_prob = len(f.attr("/input/potential/rotamer", "arguments")) - 1
+        for name, shape, bp in (("rotamer_free_energy", (nn,), True),
+                                ("rotamer_1body_energy", (nn, n_prob), True),
+                                ("node_energy", (nn, 6), False),
+                                ("edge_energy", (nn, nn, 6, 6), False),
+                                ("edge_marginal_in_graph_order", (nn, nn, 6, 6), True)):
+            got = eng.get_value_by_name(shape, "rotamer", name)
+            want = ref.value(shape, "rotamer", name)
+            if name == "node_energy":   # 1e5 marks states past a residue's rotamer count
+                pad = want == np.float32(1e5)
+                assert pad.sum() > 0 and np.array_equal(got[pad], want[pad])
+                got, want = got[~pad], want[~pad]
+            err = float(np.abs(got - want).max())
+            scale = max(1.0, float(np.abs(want).max()))
+            say("jitter %d %s: max abs diff %.3g (max |value| %.3g)" % (k, name, err, scale))
+            if bp:
+                assert err <= 2e-3, (name, err)
+            else:
+                assert err <= 2e-4 * scale, (name, err)
+            if name == "rotamer_free_energy":   # the split sums to the node's potential
+                total = float(eng.get_output("rotamer").reshape(-1)[0])
+                assert abs(float(got.sum()) - total) <= 1e-4 * max(1.0, abs(total)), (got.sum(), total)
+            if name == "edge_marginal_in_graph_order":   # each residue pair's block is a distribution
+                blk = got.sum(axis=(2, 3))
+                assert np.abs(blk - 1).max() <= 1e-4, np.abs(blk - 1).max()
+
+
 def test_batch_matches_single_engine(ue, chain_file):
     """Systems of uneven length in one launch: each system's potential equals
     the single-engine value at its positions."""

@@ -1759,7 +1759,7 @@ struct Batch {
         RunArgs a;
         std::memset(&a, 0, sizeof(a));
         a.mode = mode;
-        for (int attempt = 0;; ++attempt) {
+        for (;;) {  // grow() fails once a capacity reaches its limit
             launch(a);
             if (!overflowed()) break;
             grow();
@@ -2535,6 +2535,151 @@ int get_sens(int n_output, float* output, DerivEngine* engine, const char* node_
     return copy_node(n_output, output, engine, node_name, true);
 }
 
+// Rotamer BP state read back after an evaluation, in the reference's layouts
+// (src/rotamer.cpp:736-834): per-residue values in bead order (arrange_energies,
+// :1023-1055), graph-ordered values indexed by node class (1, 3, then 6
+// rotamers) and the residue's index within its class, which is the bead id's
+// bits above the rotamer fields (py/upside_config.py:1171-1178).
+static vector<float> rotamer_values(Batch& B, const Topology& T, int op, const string& ln) {
+    const RotamerDev& R = T.rots[T.ops[op].rot];
+    const PairGraph& G = T.graphs[T.ops[op].graph];
+    const float* A = B.sys[0].arena;
+    const int* I = B.sys[0].iarena;
+    const int nn = R.n_node, cap = R.ccap;
+    const int* nrot = T.ci.data() + R.node_nrot;
+    const int* bead0 = T.ci.data() + R.node_bead0;
+    const int* brot = T.ci.data() + R.bead_rot;
+    const int* bnode = T.ci.data() + R.bead_node;
+    const int* ids = T.ci.data() + G.id1;
+    auto fetch = [](const float* src, size_t n) {
+        vector<float> v(n);
+        if (n) hip_check(hipMemcpy(v.data(), src, n * 4, hipMemcpyDeviceToHost), "rotamer state");
+        return v;
+    };
+    auto ifetch = [](const int* src, size_t n) {
+        vector<int> v(n);
+        if (n) hip_check(hipMemcpy(v.data(), src, n * 4, hipMemcpyDeviceToHost), "rotamer state");
+        return v;
+    };
+    vector<float> prob = fetch(A + R.nprob, (size_t)nn * 6), cur = fetch(A + R.ncur, (size_t)nn * 6);
+    vector<float> off = fetch(A + R.noff, nn);
+    int st[8];
+    hip_check(hipMemcpy(st, I + R.stats, sizeof(st), hipMemcpyDeviceToHost), "stats");
+    const int nbp = st[4];
+    vector<int> bpf = ifetch(I + R.bpf, nbp), bps = ifetch(I + R.bps, nbp);
+    vector<float> P((size_t)36 * nbp), M((size_t)36 * nbp);
+    for (int q = 0; q < 36; ++q) {  // SoA rows q = r1*6 + r2 over the BP edges
+        vector<float> p = fetch(A + R.P + (size_t)q * cap, nbp), m = fetch(A + R.marg + (size_t)q * cap, nbp);
+        for (int e = 0; e < nbp; ++e) { P[(size_t)e * 36 + q] = p[e]; M[(size_t)e * 36 + q] = m[e]; }
+    }
+    // 1-rotamer residue pairs: one bead pair each, probability exp(-E)
+    int gst[4];
+    hip_check(hipMemcpy(gst, I + G.state, sizeof(gst), hipMemcpyDeviceToHost), "state");
+    const int E = gst[3];
+    vector<int> row = ifetch(I + G.row, E), nbr = ifetch(I + G.nbr, E), act = ifetch(I + G.act, E);
+    vector<float> val = fetch(A + G.val, E);
+    vector<std::array<int, 2>> e11;
+    vector<float> p11;
+    for (int s = 0; s < E; ++s) {
+        if (!act[s]) continue;
+        int a = bnode[row[s]], b = bnode[nbr[s]];
+        if (nrot[a] != 1 || nrot[b] != 1) continue;
+        e11.push_back({a, b});
+        p11.push_back(expf(-val[s]));
+    }
+
+    if (ln == "rotamer_free_energy") {  // residue_free_energies (:929-963)
+        vector<float> en(nn, 0.f);
+        for (int x = 0; x < nn; ++x) {  // node_free_energy (:292-302)
+            float s = 0.f, e = off[x];
+            for (int r = 0; r < nrot[x]; ++r) s += cur[x * 6 + r];
+            for (int r = 0; r < nrot[x]; ++r) {
+                float b = cur[x * 6 + r] / s;
+                e += b * logf((1e-10f + b) / (1e-10f + prob[x * 6 + r]));
+            }
+            en[x] += e;
+        }
+        for (size_t k = 0; k < e11.size(); ++k) {
+            float e = -logf(p11[k]);
+            en[e11[k][0]] += 0.5f * e;
+            en[e11[k][1]] += 0.5f * e;
+        }
+        for (int e = 0; e < nbp; ++e) {  // edge_free_energy (:451-471)
+            int f = bpf[e], s2 = bps[e];
+            float fe = 0.f;
+            for (int i = 0; i < nrot[f]; ++i)
+                for (int j = 0; j < nrot[s2]; ++j) {
+                    float p = M[(size_t)e * 36 + i * 6 + j];
+                    float q = P[(size_t)e * 36 + i * 6 + j] * cur[f * 6 + i] * cur[s2 * 6 + j];
+                    fe += p * logf((1e-10f + p) / (1e-10f + q));
+                }
+            en[f] += 0.5f * fe;
+            en[s2] += 0.5f * fe;
+        }
+        return en;
+    }
+    if (ln == "rotamer_1body_energy") {  // rotamer_1body_energy (:1005-1021), [residue][energy node]
+        vector<float> res((size_t)nn * R.n_prob, 0.f);
+        const vector<int> loc1(T.ci.data() + G.loc1, T.ci.data() + G.loc1 + G.n1);
+        for (int q = 0; q < R.n_prob; ++q) {
+            int maxloc = 0;
+            for (int b = 0; b < G.n1; ++b) maxloc = std::max(maxloc, loc1[b]);
+            vector<float> e1 = fetch(A + R.prob_out[q], (size_t)(maxloc + 1) * R.prob_w[q]);
+            for (int b = 0; b < G.n1; ++b) {
+                int x = bnode[b];
+                res[(size_t)x * R.n_prob + q] += cur[x * 6 + brot[b]] * e1[(size_t)loc1[b] * R.prob_w[q]];
+            }
+        }
+        return res;
+    }
+    // graph order: index of each node among all nodes as the reference numbers them
+    int n_class[7] = {0};
+    for (int x = 0; x < nn; ++x) n_class[nrot[x]] += 1;
+    const int start[7] = {0, 0, 0, n_class[1], 0, 0, n_class[1] + n_class[3]};
+    vector<int> gi(nn);
+    for (int x = 0; x < nn; ++x) {
+        int local = ids[bead0[x]] >> 8;
+        if (local < 0 || local >= n_class[nrot[x]])
+            fail("rotamer: residue index within its rotamer class out of range for graph-order values");
+        gi[x] = start[nrot[x]] + local;
+    }
+    if (ln == "node_energy") {  // -log of the node probability, 1e5 past the rotamer count
+        vector<float> res((size_t)nn * 6);
+        for (int x = 0; x < nn; ++x)
+            for (int r = 0; r < 6; ++r) res[(size_t)gi[x] * 6 + r] = r < nrot[x] ? -logf(prob[x * 6 + r]) : 1e5f;
+        return res;
+    }
+    // edge_energy / edge_marginal_in_graph_order: [n_node][n_node][6][6]
+    const bool marginal = ln == "edge_marginal_in_graph_order";
+    vector<float> res((size_t)nn * nn * 36, 0.f);
+    auto at = [&](int i1, int i2, int r1, int r2) -> float& {
+        return res[(((size_t)i1 * nn + i2) * 6 + r1) * 6 + r2];
+    };
+    if (marginal) {  // residue pairs without a BP edge are uncorrelated
+        vector<float> nm((size_t)nn * 6, 0.f);
+        for (int x = 0; x < nn; ++x)
+            for (int r = 0; r < nrot[x]; ++r) nm[(size_t)gi[x] * 6 + r] = cur[x * 6 + r];
+        for (int i1 = 0; i1 < nn; ++i1)
+            for (int i2 = 0; i2 < nn; ++i2)
+                for (int r1 = 0; r1 < 6; ++r1)
+                    for (int r2 = 0; r2 < 6; ++r2)
+                        at(i1, i2, r1, r2) = i1 == i2 ? nm[i1 * 6 + r1] * (r1 == r2) : nm[i1 * 6 + r1] * nm[i2 * 6 + r2];
+    }
+    for (size_t k = 0; k < e11.size(); ++k) {  // the single state pair: marginal 1
+        int i1 = gi[e11[k][0]], i2 = gi[e11[k][1]];
+        at(i1, i2, 0, 0) = at(i2, i1, 0, 0) = marginal ? 1.f : -logf(p11[k]);
+    }
+    for (int e = 0; e < nbp; ++e) {
+        int f = bpf[e], s2 = bps[e], i1 = gi[f], i2 = gi[s2];
+        for (int r1 = 0; r1 < nrot[f]; ++r1)
+            for (int r2 = 0; r2 < nrot[s2]; ++r2) {
+                float v = marginal ? M[(size_t)e * 36 + r1 * 6 + r2] : -logf(P[(size_t)e * 36 + r1 * 6 + r2]);
+                at(i1, i2, r1, r2) = at(i2, i1, r2, r1) = v;
+            }
+    }
+    return res;
+}
+
 int get_value_by_name(int n_output, float* output, DerivEngine* engine, const char* node_name, const char* log_name) {
     ABI_TRY
     auto& B = engine->batch;
@@ -2624,6 +2769,9 @@ int get_value_by_name(int n_output, float* output, DerivEngine* engine, const ch
             st[1] = 0;
             hip_check(hipMemcpy(B.sys[0].iarena + R.stats, st, sizeof(st), hipMemcpyHostToDevice), "stats");
         } else if (ln == "iterations") out = {(float)st[0]};
+        else if (ln == "rotamer_free_energy" || ln == "rotamer_1body_energy" || ln == "node_energy" ||
+                 ln == "edge_energy" || ln == "edge_marginal_in_graph_order")
+            out = rotamer_values(B, T, n->op, ln);
         else fail("Value " + ln + " not implemented");
     } else {
         fail("Value " + ln + " not implemented");

@@ -64,7 +64,10 @@ int get_sens(int n_output, float* output, DerivEngine* engine, const char* node_
  * (n_node x 6), "edge_energy" and "edge_marginal_in_graph_order" (n_node x
  * n_node x 6 x 6, nodes numbered by class 1, 3, 6 rotamers as the reference),
  * "count_edges_by_type", "n_node", "read n_bad_solve", "read n_bad_solve and
- * reset" (values of the last evaluation); any pair-graph node: "count_edges_by_type", "n_edge", "edge_pairs"
+ * reset" (values of the last evaluation), and "rotamer_potential" and
+ * "rotamer_entropy" (n_node), the per-residue parts of the free energy that
+ * the reference writes only to its detailed log (src/rotamer.cpp:716-724);
+ * any pair-graph node: "count_edges_by_type", "n_edge", "edge_pairs"
  * (2*n_edge element indices, sorted), "edge_values" (n_edge, same order).
  * Measurement hooks with no reference counterpart: pair graphs "graph_shape"
  * (n1, dim1, n2, dim2); rotamer "iterations" (of the last solve),

@@ -70,13 +70,29 @@ def test_chig_frames(ue, chig):
     eng = ue.Upside(path, quiet=True)
     ora, ora64 = O.Engine(path), O.Engine(path, dtype=np.float64)
     got = []
+    # per-residue rotamer values of the reference's detailed log
+    # (src/rotamer.cpp:708-731): 2e-3 absolute, the BP-dependent bar of the
+    # logged potential
+    logged = ["rotamer_free_energy", "rotamer_potential", "rotamer_entropy"]
+    res_dev = {k: 0.0 for k in logged + ["rotamer_1body_energy"]}
     for f in range(len(z["frame_pos"])):
         pos = np.ascontiguousarray(z["frame_pos"][f], np.float32)
         e, _ = check_point(eng, ora, ora64, pos, "chig frame %d" % f)
         got.append(e)
+        nn = z["frame_rotamer_free_energy"].shape[1]
+        for name in logged:
+            v = eng.get_value_by_name((nn,), "rotamer", name)
+            res_dev[name] = max(res_dev[name], float(np.abs(v - z["frame_" + name][f]).max()))
+        one = eng.get_value_by_name((nn, 3), "rotamer", "rotamer_1body_energy")
+        for q in range(3):
+            res_dev["rotamer_1body_energy"] = max(res_dev["rotamer_1body_energy"],
+                                                  float(np.abs(one[:, q] - z["frame_rotamer_1body_energy%d" % q][f]).max()))
     dev = np.abs(np.array(got) - z["frame_potential"].ravel())
     say("chig max |E - E_logged| = %.2e" % dev.max())
+    say("chig per-residue max |value - logged|:", " ".join("%s=%.2e" % kv for kv in sorted(res_dev.items())))
     assert dev.max() < 2e-3
+    for name, d in res_dev.items():
+        assert d < 2e-3, (name, d)
 
 
 @pytest.mark.parametrize("n_res,seed", [(40, 1), (100, 2), (500, 3), (1000, 4)])

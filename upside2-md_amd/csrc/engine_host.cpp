@@ -2540,11 +2540,12 @@ int get_sens(int n_output, float* output, DerivEngine* engine, const char* node_
 // :1023-1055), graph-ordered values indexed by node class (1, 3, then 6
 // rotamers) and the residue's index within its class, which is the bead id's
 // bits above the rotamer fields (py/upside_config.py:1171-1178).
-static vector<float> rotamer_values(Batch& B, const Topology& T, int op, const string& ln) {
+static vector<float> rotamer_values(Batch& B, int sys, int op, const string& ln) {
+    const Topology& T = B.sys[sys].prog->T;
     const RotamerDev& R = T.rots[T.ops[op].rot];
     const PairGraph& G = T.graphs[T.ops[op].graph];
-    const float* A = B.sys[0].arena;
-    const int* I = B.sys[0].iarena;
+    const float* A = B.sys[sys].arena;
+    const int* I = B.sys[sys].iarena;
     const int nn = R.n_node, cap = R.ccap;
     const int* nrot = T.ci.data() + R.node_nrot;
     const int* bead0 = T.ci.data() + R.node_bead0;
@@ -2588,30 +2589,39 @@ static vector<float> rotamer_values(Batch& B, const Topology& T, int op, const s
         p11.push_back(expf(-val[s]));
     }
 
-    if (ln == "rotamer_free_energy") {  // residue_free_energies (:929-963)
+    // per-residue split of the Bethe free energy: residue_free_energies
+    // (:929-963) and its average-energy and entropy parts residue_entropy<0>
+    // and <1> (:965-1003, node_entropy :303-321, edge_entropy :473-497)
+    const int split = ln == "rotamer_free_energy" ? 2 : ln == "rotamer_potential" ? 0 : ln == "rotamer_entropy" ? 1 : -1;
+    if (split >= 0) {
         vector<float> en(nn, 0.f);
-        for (int x = 0; x < nn; ++x) {  // node_free_energy (:292-302)
-            float s = 0.f, e = off[x];
+        for (int x = 0; x < nn; ++x) {
+            float s = 0.f, e = split == 1 ? 0.f : off[x];
             for (int r = 0; r < nrot[x]; ++r) s += cur[x * 6 + r];
             for (int r = 0; r < nrot[x]; ++r) {
-                float b = cur[x * 6 + r] / s;
-                e += b * logf((1e-10f + b) / (1e-10f + prob[x * 6 + r]));
+                float b = cur[x * 6 + r] / s, pr = prob[x * 6 + r];
+                if (split == 2) e += b * logf((1e-10f + b) / (1e-10f + pr));
+                else if (split == 0) e -= b * logf(1e-10f + pr);
+                else e += b * logf(1e-10f + b);
             }
             en[x] += e;
         }
-        for (size_t k = 0; k < e11.size(); ++k) {
-            float e = -logf(p11[k]);
-            en[e11[k][0]] += 0.5f * e;
-            en[e11[k][1]] += 0.5f * e;
-        }
-        for (int e = 0; e < nbp; ++e) {  // edge_free_energy (:451-471)
+        if (split != 1)
+            for (size_t k = 0; k < e11.size(); ++k) {
+                float e = -logf(p11[k]);
+                en[e11[k][0]] += 0.5f * e;
+                en[e11[k][1]] += 0.5f * e;
+            }
+        for (int e = 0; e < nbp; ++e) {
             int f = bpf[e], s2 = bps[e];
             float fe = 0.f;
             for (int i = 0; i < nrot[f]; ++i)
                 for (int j = 0; j < nrot[s2]; ++j) {
-                    float p = M[(size_t)e * 36 + i * 6 + j];
-                    float q = P[(size_t)e * 36 + i * 6 + j] * cur[f * 6 + i] * cur[s2 * 6 + j];
-                    fe += p * logf((1e-10f + p) / (1e-10f + q));
+                    float p = M[(size_t)e * 36 + i * 6 + j], pr = P[(size_t)e * 36 + i * 6 + j];
+                    float bb = cur[f * 6 + i] * cur[s2 * 6 + j];
+                    if (split == 2) fe += p * logf((1e-10f + p) / (1e-10f + pr * bb));
+                    else if (split == 0) fe -= p * logf(1e-10f + pr);
+                    else fe += p * logf((1e-10f + p) / (1e-10f + bb));
                 }
             en[f] += 0.5f * fe;
             en[s2] += 0.5f * fe;
@@ -2770,8 +2780,9 @@ int get_value_by_name(int n_output, float* output, DerivEngine* engine, const ch
             hip_check(hipMemcpy(B.sys[0].iarena + R.stats, st, sizeof(st), hipMemcpyHostToDevice), "stats");
         } else if (ln == "iterations") out = {(float)st[0]};
         else if (ln == "rotamer_free_energy" || ln == "rotamer_1body_energy" || ln == "node_energy" ||
-                 ln == "edge_energy" || ln == "edge_marginal_in_graph_order")
-            out = rotamer_values(B, T, n->op, ln);
+                 ln == "edge_energy" || ln == "edge_marginal_in_graph_order" || ln == "rotamer_potential" ||
+                 ln == "rotamer_entropy")
+            out = rotamer_values(B, 0, n->op, ln);
         else fail("Value " + ln + " not implemented");
     } else {
         fail("Value " + ln + " not implemented");

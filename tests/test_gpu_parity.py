@@ -791,6 +791,20 @@ def test_output_logger(ue, chain_file, tmp_path):
         assert hb.shape == (3, len(want_hb)) and hb.dtype == np.float32
         np.testing.assert_allclose(hb[0], want_hb, rtol=1e-6, atol=1e-7)
         assert bad.shape == (3, 1) and (bad == 0).all()
+        # per-residue rotamer values (src/rotamer.cpp:711-731) of every frame
+        # equal the single engine's at the frame's positions
+        nn = int(eng.get_value_by_name((1,), "rotamer", "n_node")[0])
+        names = ["rotamer_free_energy", "rotamer_potential", "rotamer_entropy"]
+        with h5lite.File(outs[s]) as f:
+            logged = {k: f.read("/output/" + k) for k in names + ["rotamer_1body_energy%d" % q for q in range(3)]}
+        for fr in range(3):
+            eng.energy(pos[fr, 0])
+            one = eng.get_value_by_name((nn, 3), "rotamer", "rotamer_1body_energy")
+            want = {k: eng.get_value_by_name((nn,), "rotamer", k) for k in names}
+            want.update({"rotamer_1body_energy%d" % q: one[:, q] for q in range(3)})
+            for k, v in want.items():
+                assert logged[k].shape == (3, nn) and logged[k].dtype == np.float32, k
+                np.testing.assert_allclose(logged[k][fr], v, rtol=1e-5, atol=1e-5, err_msg=k)
 
 
 def test_checkpoint_resume_is_bit_identical(ue, chain_file):

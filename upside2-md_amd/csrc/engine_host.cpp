@@ -1991,6 +1991,8 @@ struct DerivEngine {
 // :893 temperature) as extendable arrays with the first dimension unlimited
 // and chunks of 100 frames (src/state_logger.h:21-39), appended one frame at a
 // time and flushed on close.
+static vector<float> rotamer_values(Batch& B, int sys, int op, const string& ln);
+
 struct OutputLog {
     hid_t file = -1, group = -1;
     hid_t pos = -1, mom = -1, kinetic = -1, potential = -1, time = -1, temperature = -1;
@@ -2000,6 +2002,11 @@ struct OutputLog {
     hid_t replica_index = -1, replica_swaps = -1;   // src/main.cpp:210-232
     int n_partner = 0;
     int hb_node = -1, hb_n = 0, rot_stats = -1;
+    // per-residue rotamer free energy, its parts and the 1-body energies
+    // (src/rotamer.cpp:711-731), each (n_frame, n_node)
+    int rot_op = -1, rot_nn = 0, rot_nprob = 0;
+    vector<hid_t> rot_sets;
+    vector<vector<float>> rot_rows;   // filled by the caller before frame()
     long long n = 0;
 
     static hid_t earray(hid_t g, const char* name, hid_t type, const vector<hsize_t>& row) {
@@ -2074,6 +2081,15 @@ struct OutputLog {
             if (!T.rots.empty()) {
                 rot_stats = T.rots[0].stats;
                 bad_solves = earray(group, "rotamer_bad_solves_cumulative", H5T_NATIVE_LLONG, {1});
+                for (size_t o = 0; o < T.ops.size(); ++o)
+                    if (T.ops[o].kind == OP_ROTAMER && T.ops[o].rot == 0) rot_op = (int)o;
+                rot_nn = T.rots[0].n_node;
+                rot_nprob = T.rots[0].n_prob;
+                vector<string> names{"rotamer_free_energy", "rotamer_potential", "rotamer_entropy"};
+                for (int q = 0; q < rot_nprob; ++q) names.push_back("rotamer_1body_energy" + std::to_string(q));
+                if (rot_op >= 0)
+                    for (auto& nm : names) rot_sets.push_back(earray(group, nm.c_str(), H5T_NATIVE_FLOAT, {(hsize_t)rot_nn}));
+                rot_rows.resize(rot_sets.size());
             }
         }
         // the reference records its command line here (src/main.cpp:734-736)
@@ -2101,6 +2117,7 @@ struct OutputLog {
         if (pivot_stats >= 0) append(pivot_stats, H5T_NATIVE_INT, mc_stats, n);
         if (hbond >= 0) append(hbond, H5T_NATIVE_FLOAT, hb_values.data(), n);
         if (bad_solves >= 0) append(bad_solves, H5T_NATIVE_LLONG, &n_bad, n);
+        for (size_t q = 0; q < rot_sets.size(); ++q) append(rot_sets[q], H5T_NATIVE_FLOAT, rot_rows[q].data(), n);
         if (replica_index >= 0) append(replica_index, H5T_NATIVE_INT, &rep_index, n);
         if (replica_swaps >= 0) append(replica_swaps, H5T_NATIVE_INT, rep_swaps.data(), n);
         ++n;
@@ -2110,6 +2127,8 @@ struct OutputLog {
         for (hid_t* d : {&pos, &mom, &kinetic, &potential, &time, &temperature, &pivot_stats, &hbond, &bad_solves,
                          &replica_index, &replica_swaps})
             if (*d >= 0) { H5Dclose(*d); *d = -1; }
+        for (hid_t& d : rot_sets)
+            if (d >= 0) { H5Dclose(d); d = -1; }
         if (group >= 0) { H5Gclose(group); group = -1; }
         if (file >= 0) { H5Fflush(file, H5F_SCOPE_LOCAL); H5Fclose(file); file = -1; }
     }
@@ -2231,6 +2250,15 @@ struct UpsideBatch {
                 hip_check(hipMemcpy(st4, batch.sys[k].iarena + o.rot_stats, sizeof(st4), hipMemcpyDeviceToHost),
                           "stats");
                 o.n_bad = st4[1];
+            }
+            if (!o.rot_sets.empty()) {  // BP state of the evaluation above
+                const char* split[3] = {"rotamer_free_energy", "rotamer_potential", "rotamer_entropy"};
+                for (int q = 0; q < 3; ++q) o.rot_rows[q] = rotamer_values(batch, (int)k, o.rot_op, split[q]);
+                vector<float> one = rotamer_values(batch, (int)k, o.rot_op, "rotamer_1body_energy");
+                for (int q = 0; q < o.rot_nprob; ++q) {
+                    o.rot_rows[3 + q].resize(o.rot_nn);
+                    for (int x = 0; x < o.rot_nn; ++x) o.rot_rows[3 + q][x] = one[(size_t)x * o.rot_nprob + q];
+                }
             }
             if (o.replica_index >= 0) {
                 o.rep_index = replica_of[k];

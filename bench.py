@@ -159,15 +159,20 @@ def percentile_from_hist(h, q):
 
 
 # ------------------------------------------------------- algorithmic bytes
+# pair-graph nodes of the ff_2.1 graph (InteractionGraph users, src/interaction_graph.h)
+PAIR_NODES = ("rotamer", "protein_hbond", "hbond_coverage", "hbond_coverage_hydrophobe",
+              "environment_coverage_sc", "environment_coverage_hb", "hb_environment_coverage_hn",
+              "hb_environment_coverage_oc", "backbone_pairs")
+
+
 def graph_counts(eng, node_names):
     """Edge counts of every interaction graph and BP class counts at the
     engine's last evaluation."""
     out = {}
     for name in node_names:
-        try:
-            shape = eng.get_value_by_name((4,), name, "graph_shape")
-        except RuntimeError:
+        if name not in PAIR_NODES:
             continue
+        shape = eng.get_value_by_name((4,), name, "graph_shape")
         n_edge = float(eng.get_value_by_name((1,), name, "n_edge")[0])
         out[name] = dict(shape=[float(x) for x in shape], n_edge=n_edge)
     bp = eng.get_value_by_name((6,), "rotamer", "bp_edge_classes")
@@ -217,42 +222,145 @@ def traffic_per_step():
 
 
 # ------------------------------------------------------ batched workloads
-def batched_workloads(work, rounds=60):
-    """Two batch workloads reported beside the single-chain line (not `value`):
+def host_info():
+    """Host cores this process may use and the CPU model (for every CPU baseline)."""
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count() or 1
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return nproc, model
+
+
+def cpu_reference_batch(paths, temps, rounds, seed, extra=()):
+    """The reference CLI (oracle/_ref/upside_ref) over copies of the systems'
+    .up files in one invocation, OMP_NUM_THREADS = min(systems, host cores):
+    the reference parallelises across systems only (src/main.cpp:917)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "upside_ref")
+    if not os.path.exists(exe):
+        return None
+    nproc, model = host_info()
+    threads = max(1, min(len(paths), nproc))
+    d = tempfile.mkdtemp()
+    try:
+        files = []
+        for k, q in enumerate(paths):
+            f = os.path.join(d, "sys%03d.up" % k)
+            shutil.copy(q, f)
+            files.append(f)
+        dur = rounds * 3 * DT
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        cmd = [exe, "--duration", "%.6f" % dur, "--frame-interval", "%.6f" % dur, "--temperature",
+               ",".join("%.6f" % t for t in temps), "--seed", str(seed), "--log-level", "basic",
+               "--disable-recentering"] + list(extra) + files
+        t0 = time.perf_counter()
+        r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            raise RuntimeError("reference CLI failed: " + r.stderr[-500:])
+        m = re.search(r"\(([0-9.eE+-]+) us/systems/step", r.stdout)
+        n_res = sum(h5lite_n_res(q) for q in paths)
+        us = float(m.group(1)) if m else wall * 1e6 / (rounds * 3 * len(paths))
+        value = n_res / (us * 1e-6 * len(paths))
+        return dict(value=value, unit="residue-steps/s", cores=threads, kind="reference", nproc=nproc,
+                    cpu_model=model,
+                    sample="reference upside CLI built from its sources (oracle/_ref), one invocation over the "
+                           "%d systems, OMP_NUM_THREADS=%d, %d rounds = %d steps; %.2f us/systems/step as it "
+                           "reports, %.1f s wall" % (len(paths), threads, rounds, rounds * 3, us, wall))
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def h5lite_n_res(path):
+    with h5lite.File(path) as f:
+        return f.read("/input/pos").shape[0] // 3
+
+
+def batch_roofline(b, paths, st0, st1, steps, kernel_ms):
+    """Algorithmic bytes of a batch launch (sum over systems of the per-step
+    and per-BP-pass bytes at the end state, section `algorithmic_bytes`) over
+    the launch's kernel time."""
+    import upside_engine
+    probes = {}
+    total = 0.0
+    for s, q in enumerate(paths):
+        if q not in probes:
+            with h5lite.File(q) as f:
+                names = f.children("/input/potential")
+            probes[q] = (upside_engine.Upside(q, quiet=True), names)
+        eng, names = probes[q]
+        eng.energy(b.pos(s))
+        g, bp, nodes = graph_counts(eng, names)
+        small = small_node_bytes(eng, names, set(g))
+        passes = (st1[s]["total"] - st0[s]["total"]) + (st1[s]["solves"] - st0[s]["solves"])
+        n_atom = b.pos(s).shape[0]
+        alg, _, _ = algorithmic_bytes(g, bp, nodes, n_atom, small, steps, passes)
+        total += alg
+    achieved = total / (kernel_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": total,
+            "kernel_ms": kernel_ms, "kernel": "upside::engine_step_kernel"}
+
+
+def bp_iters(b, st0, st1):
+    it = sum(a["total"] - c["total"] for a, c in zip(st1, st0))
+    so = sum(a["solves"] - c["solves"] for a, c in zip(st1, st0))
+    return it / max(so, 1)
+
+
+def batched_workloads(work, rounds=60, cpu=True):
+    """The batch workloads reported beside the single-chain line (not
+    `value`), each with its own roofline and reference CPU baseline:
     16 replicas of one 150-residue chain at temperatures geometric over
     0.80-0.95 with alternating neighbour swap sets every 7 rounds (21 steps),
-    and 64 independent chains with lengths uniform in [50, 400].  Each is
-    autotuned, warmed up, then timed over `rounds` rounds."""
+    exchanged inside the persistent launch; 64 independent chains with
+    lengths uniform in [50, 400].  Each is autotuned (team size and rotamer
+    solve layout), warmed up, then timed over `rounds` rounds in one launch.
+    Returns the lines and the 64-chain batch's paths (training chains of the
+    warm start)."""
     import upside_engine
     out = []
 
-    def timed(b, n_res_total, segment=None, sets=None):
+    def tune(b):
         cand = [t for t in (1, 2, 4, 8, 16) if t <= b.max_team()]
         b.run(3)
-        b.autotune(3, cand)
-        hip_sync()
-        t0 = time.perf_counter()
-        done, k = 0, 0
-        while done < rounds:
-            n = min(segment or rounds, rounds - done)
-            b.run(n)
-            done += n
-            if sets:
-                b.replica_exchange(sets[k % 2], done)
-                k += 1
-        hip_sync()
-        wall = time.perf_counter() - t0
-        return n_res_total * 3 * rounds / wall, wall, b.team()
+        b.autotune_layout(3, cand, (0, 1))
 
     p = os.path.join(work, "rex150.up")
     upconfig.make_chain(p, 150, seed=2)
     temps = [0.80 * (0.95 / 0.80) ** (i / 15.0) for i in range(16)]
+    sets = [[(i, i + 1) for i in range(0, 15, 2)], [(i, i + 1) for i in range(1, 14, 2)]]
     b = upside_engine.Batch([p] * 16, temps, seed=2, dt=DT)
-    sets = [[[i, i + 1] for i in range(0, 15, 2)], [[i, i + 1] for i in range(1, 14, 2)]]
-    v, wall, team = timed(b, 16 * 150, segment=7, sets=sets)
-    out.append({"workload": "16 replicas x 150 residues, T geometric 0.80-0.95, neighbour swaps every 21 steps",
-                "value": v, "unit": "residue-steps/s", "steps": 3 * rounds, "seconds": wall,
-                "workgroups_per_system": team})
+    tune(b)
+    b.set_replica_exchange(7, sets)
+    st0 = [b.bp_stats(s) for s in range(16)]
+    hip_sync()
+    t0 = time.perf_counter()
+    b.run_logged(rounds, 0)          # one launch: the swaps happen inside it
+    hip_sync()
+    wall = time.perf_counter() - t0
+    st1 = [b.bp_stats(s) for s in range(16)]
+    _, suc, att = b.replica_stats()
+    line = {"workload": "16 replicas x 150 residues, T geometric 0.80-0.95, neighbour swap sets alternating "
+                        "every 21 steps, exchanged inside the persistent launch",
+            "value": 16 * 150 * 3 * rounds / wall, "unit": "residue-steps/s", "steps": 3 * rounds,
+            "seconds": wall, "workgroups_per_system": b.team(), "bp_layout": b.bp_layout(),
+            "swaps_accepted_attempted": [int(suc.sum()), int(att.sum())],
+            "bp_iters_per_solve": bp_iters(b, st0, st1),
+            "roofline": batch_roofline(b, [p] * 16, st0, st1, 3 * rounds, b.last_run_ms())}
+    if cpu:
+        line["cpu_baseline"] = cpu_reference_batch(
+            [p] * 16, temps, rounds, 2,
+            ["--replica-interval", "%.4f" % (7 * 3 * DT + 1e-4)] +
+            sum((["--swap-set", ",".join("%d-%d" % q for q in st)] for st in sets), []))
+    out.append(line)
     del b
     rng = np.random.default_rng(3)
     lens = [int(x) for x in rng.integers(50, 401, 64)]
@@ -262,11 +370,100 @@ def batched_workloads(work, rounds=60):
         upconfig.make_chain(q, L, seed=100 + i)
         paths.append(q)
     b = upside_engine.Batch(paths, [TEMP] * 64, seed=3, dt=DT)
-    v, wall, team = timed(b, sum(lens))
-    out.append({"workload": "64 independent chains, lengths uniform in [50, 400] (%d residues)" % sum(lens),
-                "value": v, "unit": "residue-steps/s", "steps": 3 * rounds, "seconds": wall,
-                "workgroups_per_system": team})
-    return out
+    tune(b)
+    st0 = [b.bp_stats(s) for s in range(64)]
+    hip_sync()
+    t0 = time.perf_counter()
+    b.run(rounds)
+    hip_sync()
+    wall = time.perf_counter() - t0
+    st1 = [b.bp_stats(s) for s in range(64)]
+    line = {"workload": "64 independent chains, lengths uniform in [50, 400] (%d residues)" % sum(lens),
+            "value": sum(lens) * 3 * rounds / wall, "unit": "residue-steps/s", "steps": 3 * rounds,
+            "seconds": wall, "workgroups_per_system": b.team(), "bp_layout": b.bp_layout(),
+            "bp_iters_per_solve": bp_iters(b, st0, st1),
+            "roofline": batch_roofline(b, paths, st0, st1, 3 * rounds, b.last_run_ms())}
+    if cpu:
+        line["cpu_baseline"] = cpu_reference_batch(paths, [TEMP] * 64, rounds, 3)
+    out.append(line)
+    return out, paths
+
+
+def warm_start_workloads(work, train_paths, rounds=20, train_steps=2000):
+    """The learned BP warm start: rows harvested from converged plain solves
+    (schedule 1, residual 1e-5) of the 64-chain batch, the network trained on
+    the GPU, then a 1000-residue chain and 32 replicas of a 150-residue chain,
+    each run from the same state under the reference's schedule (0), the
+    plain start with the residual rule (1) and the warm start with the
+    residual rule (2)."""
+    import upside_engine
+    b = upside_engine.Batch(train_paths, [TEMP] * len(train_paths), seed=5, dt=DT)
+    b.set_team(min(2, b.max_team()))
+    b.set_bp_schedule(1, 1e-5)
+    F, T = [], []
+    t0 = time.perf_counter()
+    for frame in range(8):
+        b.run(2)
+        for s in range(len(train_paths)):
+            f, t = b.bp_training_rows(s)
+            F.append(f)
+            T.append(t)
+    del b
+    F, T = np.concatenate(F), np.concatenate(T)
+    net = upside_engine.WarmStartNet(seed=1)
+    net.set_data(F, T)
+    l0, _ = net.loss_grad()
+    hip_sync()
+    t1 = time.perf_counter()
+    net.train(train_steps, batch_rows=8192, lr=1e-3)
+    hip_sync()
+    t_train = time.perf_counter() - t1
+    l1, _ = net.loss_grad()
+    params = net.params()
+    info = {"training_rows": int(len(F)), "harvest_seconds": t1 - t0, "train_steps": train_steps,
+            "train_seconds": t_train, "loss_before_after": [l0, l1]}
+    p1000 = os.path.join(work, "warm1000.up")
+    upconfig.make_chain(p1000, 1000, seed=4)
+    p150 = os.path.join(work, "warm150.up")
+    upconfig.make_chain(p150, 150, seed=6)
+    lines = []
+    for label, paths in (("one 1000-residue chain", [p1000]), ("32 replicas x 150 residues", [p150] * 32)):
+        n_res = sum(h5lite_n_res(q) for q in paths)
+        res = {}
+        for sched in (0, 1, 2):
+            bb = upside_engine.Batch(paths, [TEMP] * len(paths), seed=6, dt=DT)
+            bb.set_team(min(16 if len(paths) == 1 else 2, bb.max_team()))
+            bb.run(2)
+            if sched == 2:
+                bb.set_warm_start(params)
+            if sched:
+                bb.set_bp_schedule(sched, 1e-5)
+            st0 = [bb.bp_stats(s) for s in range(len(paths))]
+            hip_sync()
+            t0 = time.perf_counter()
+            bb.run(rounds)
+            hip_sync()
+            wall = time.perf_counter() - t0
+            st1 = [bb.bp_stats(s) for s in range(len(paths))]
+            ws = [bb.bp_warm_stats(s) for s in range(len(paths))]
+            res[sched] = {"value": n_res * 3 * rounds / wall, "ms_per_step": wall * 1e3 / (3 * rounds),
+                          "bp_iters_per_solve": bp_iters(bb, st0, st1),
+                          "fallbacks": int(sum(w["fallbacks"] for w in ws))}
+            if sched == 2:
+                # the converged result does not depend on the start: evaluate the
+                # warm run's end state with the plain start too
+                bb.energies()
+                d_warm = [bb.deriv(s) for s in range(min(len(paths), 4))]
+                bb.set_bp_schedule(1, 1e-5)
+                bb.energies()
+                d_plain = [bb.deriv(s) for s in range(min(len(paths), 4))]
+                res[sched]["force_rel_diff_vs_plain_start"] = max(
+                    float(np.sqrt(((a - c) ** 2).sum() / (c ** 2).sum())) for a, c in zip(d_warm, d_plain))
+        lines.append({"workload": label + ", warm start vs plain", "unit": "residue-steps/s", "steps": 3 * rounds,
+                      "reference_schedule": res[0], "plain_start_residual_1e-5": res[1],
+                      "warm_start_residual_1e-5": res[2],
+                      "warm_vs_plain_iters": res[2]["bp_iters_per_solve"] / res[1]["bp_iters_per_solve"]})
+    return info, lines
 
 
 # ----------------------------------------------------------------- main
@@ -277,7 +474,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--cpu-rounds", type=int, default=400, help="rounds of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-batched", action="store_true", help="skip the two batch workloads")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batch and warm-start workloads")
+    ap.add_argument("--no-warm-start", action="store_true", help="skip the warm-start workloads")
     ap.add_argument("--team", type=int, default=0,
                     help="workgroups per system (0: time 1..32 on the warmed state and keep the fastest)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -322,8 +520,8 @@ def main():
         b.set_team(args.team)
     else:
         cand = [t for t in (1, 2, 4, 8, 16, 32) if t <= b.max_team()]
-        tune = b.autotune(6, cand)       # restores the warmed state after each candidate
-        log("team autotune (ms per 6 rounds): %s -> team %d" % (tune, b.team()))
+        tune = b.autotune_layout(6, cand, (0, 1, 2))  # restores the warmed state after each candidate
+        log("team / BP layout autotune (ms per 6 rounds): %s -> team %d layout %d" % (tune, b.team(), b.bp_layout()))
     probe.energy(b.pos(0))
     g0, bp0, nodes = graph_counts(probe, node_names)
     small = small_node_bytes(probe, node_names, set(g0))
@@ -369,7 +567,7 @@ def main():
         "config": {"workload": "one %d-residue seeded random-sequence chain (seed %d), random-Rama start, "
                                "ff_2.1, FP32, T=%.2f, dt=%.3f, thermostat every round" % (N_RES, SEED, TEMP, DT),
                    "n_res": N_RES, "n_atom": n_atom, "launch": "one persistent launch of all steps",
-                   "workgroups_per_system": b.team()},
+                   "workgroups_per_system": b.team(), "bp_layout": b.bp_layout()},
         "bp_iters_per_step": iters / max(solves, 1),
         "bp_passes_per_solve": passes / max(solves, 1),
         "bp_iters_per_solve_p50_p99": [percentile_from_hist(hist, 0.5), percentile_from_hist(hist, 0.99)],
@@ -387,7 +585,12 @@ def main():
     if cpu:
         line["speedup_vs_cpu_baseline"] = value / cpu["value"]
     if not args.no_batched:
-        line["batched"] = batched_workloads(work)
+        line["batched"], uneven = batched_workloads(work, cpu=not args.no_cpu_baseline)
+        if not args.no_warm_start:
+            info, wl = warm_start_workloads(work, uneven)
+            line["warm_start"] = dict(info, workloads=wl)
+    if cpu:
+        cpu["nproc"], cpu["cpu_model"] = host_info()
     shutil.rmtree(work, ignore_errors=True)
     print(json.dumps(line), flush=True)
 

@@ -105,6 +105,10 @@ int upside_batch_set_mom(UpsideBatch* batch, int sys, const float* mom);
 int upside_batch_run(UpsideBatch* batch, int n_round, int frame_interval, float* frame_pot, int n_frames);
 /* Potential of every system at its current positions: energies[n_sys]. */
 int upside_batch_energies(UpsideBatch* batch, float* energies);
+/* dE/dx (n_atom, 3) of system sys at the last evaluation: upside_batch_energies,
+ * or the last force evaluation of a run (which precedes that run's last
+ * position update). */
+int upside_batch_get_deriv(UpsideBatch* batch, int sys, float* deriv);
 /* Replica exchange step (src/main.cpp:243-316, criterion 0): swap positions
  * of the pairs of swap set `set_index`, keep a swap with the Metropolis test
  * drawn from Threefry stream 1 with counter `round`.  accepted[n_pairs] out. */
@@ -144,9 +148,18 @@ int upside_batch_bp_lds_solves(UpsideBatch* batch, int sys, long long* n);
  * /output, as the reference does without -o; systems must not share an input
  * file), else output_paths[s] is created afresh (as with -o).  log_level 1
  * (the reference's default, "detailed") adds hbond (n_frame, n_site) f4, the
- * hbond value of every donor and acceptor (src/hbond.cpp:373-377), and
- * rotamer_bad_solves_cumulative (n_frame, 1) i8 (src/rotamer.cpp:708-710); the
- * reference's other detailed per-residue decompositions are not written.
+ * hbond value of every donor and acceptor (src/hbond.cpp:373-377),
+ * rotamer_bad_solves_cumulative (n_frame, 1) i8 (src/rotamer.cpp:708-710), the
+ * per-residue rotamer_free_energy, rotamer_potential, rotamer_entropy and
+ * rotamer_1body_energy<k> (src/rotamer.cpp:711-731), rama_map_potential
+ * (n_frame, n_res) of a rama_map_pot node with log_pot set
+ * (src/rama_map_pot.cpp:49-55) and sigmoid_coupling (n_frame, n_res)
+ * (src/environment.cpp:892-900); log_level 2 ("extensive") also
+ * environment_coverage and hbond_environment_coverage (n_frame, n_elem), the
+ * coverage node outputs (src/environment.cpp:304-310, :355-360), and virtual
+ * (n_frame, n_site, 3), the inferred H and O positions (src/hbond.cpp:53-58).
+ * Where two nodes would log under one name (the reference would fail to
+ * create the second dataset) the first in load order is logged.
  * upside_batch_run_logged advances n_round rounds and records a frame at
  * every round number (counted since the batch was created) that is a
  * multiple of frame_interval (> 0), before that round's thermostat and
@@ -225,11 +238,77 @@ int upside_batch_max_team(UpsideBatch* batch);
 int upside_batch_set_team(UpsideBatch* batch, int team);
 int upside_batch_get_team(UpsideBatch* batch);
 int upside_batch_autotune(UpsideBatch* batch, int n_round, const int* candidates, int n_candidates, float* ms_out);
+/* Where each rotamer BP solve runs (no reference counterpart; the reference
+ * solves on its one thread, src/rotamer.cpp:1106-1162): 0 (default) on one
+ * workgroup with the BP state in its LDS when it fits, else on the system's
+ * whole team with the state in memory; 1 always on the whole team; 2 always
+ * on one workgroup (LDS when it fits, else memory).  Every layout computes the
+ * same arithmetic in the same order: results are identical.
+ * upside_batch_autotune_layout times every (team, layout) pair from the same
+ * state as upside_batch_autotune does and keeps the fastest;
+ * ms_out[n_teams * n_layouts] (may be NULL), row-major by team. */
+int upside_batch_set_bp_layout(UpsideBatch* batch, int layout);
+int upside_batch_get_bp_layout(UpsideBatch* batch);
+int upside_batch_autotune_layout(UpsideBatch* batch, int n_round, const int* teams, int n_teams, const int* layouts,
+                                 int n_layouts, float* ms_out);
 /* Per-node wall-clock accumulation inside the kernel (the analogue of the
  * reference's -DCOLLECT_PROFILE timers, src/timing.cpp:11-54).  The report is
  * a text table; returns the needed size if buflen is too small. */
 int upside_batch_enable_profile(UpsideBatch* batch, int on);
 int upside_batch_profile_report(UpsideBatch* batch, int sys, char* buf, int buflen);
+
+/* ---- BP warm-start network (no reference counterpart) ----
+ * The reference starts every rotamer BP solve from the node probabilities
+ * with all messages 1 (src/rotamer.cpp:1110-1133).  The warm start replaces
+ * the beliefs after the first (undamped) pass by the prediction of a small
+ * network: 32 features per residue -> 128 -> 128 -> 16 (ReLU), bf16 weights
+ * and activations, float32 accumulation on the matrix cores.  Parameters are
+ * a float32 vector of upside_mlp_n_param() entries: W1 [128][32], b1 [128],
+ * W2 [128][128], b2 [128], W3 [16][128], b3 [16] (weights [out][in]).
+ * Training rows are features [n][32] and targets [n][8] (the converged BP
+ * marginal of each state, 0 beyond the residue's states), as
+ * upside_batch_bp_training_rows returns them; the loss is the mean over rows
+ * of -sum_r t_r log softmax(f[6+r] + out_r) over the residue's states.
+ * upside_mlp_create: seeded He-normal hidden layers, zero output layer (the
+ * untrained network leaves the first-pass beliefs unchanged).
+ * upside_mlp_loss_grad: loss and gradient over all rows at the current
+ * parameters (no update).  upside_mlp_train: n_steps Adam steps (lr; beta
+ * 0.9, 0.999; eps 1e-8) on consecutive windows of batch_rows rows (wrapping),
+ * losses[n_steps] (may be NULL) the loss of each step's batch before the
+ * step.  upside_mlp_forward: the outputs [n][16] of given rows.  0 on
+ * success. */
+/* Rotamer BP schedule of a batch: mode 0 (default) the reference's; 1 start
+ * as the reference, stop when the largest two-sided change of a node belief
+ * over an iteration chunk is at most `residual`; 2 warm start: after the
+ * first (undamped) pass the network's beliefs replace the pass's, klocal
+ * local updates per edge make the messages consistent with them, the
+ * residual rule stops the solve, and a solve that has not met it after
+ * `budget` iterations restarts from the plain start as mode 1 (the fallback;
+ * its iterations count too).  Mode 2 needs upside_batch_set_warm_start
+ * (parameters as upside_mlp_get_params gives them).
+ * upside_batch_bp_training_rows: the features of the first pass and the
+ * converged node marginals of system sys's last solve in mode 1 or 2, one
+ * row per residue with more than one rotamer state; with features or targets
+ * NULL it returns the row count, else fills up to max_rows rows (more rows:
+ * error) and returns the count; -1 on error.
+ * upside_batch_bp_warm_stats: warm-started solves, fallbacks and solves that
+ * met the residual rule since the batch was created. */
+int upside_batch_set_bp_schedule(UpsideBatch* batch, int mode, float residual, int budget, int klocal);
+int upside_batch_set_warm_start(UpsideBatch* batch, const float* params, int n_param);
+int upside_batch_bp_training_rows(UpsideBatch* batch, int sys, float* features, float* targets, int max_rows);
+int upside_batch_bp_warm_stats(UpsideBatch* batch, int sys, long long* warm, long long* fallbacks,
+                               long long* residual_solves);
+
+typedef struct UpsideMlp UpsideMlp;
+UpsideMlp* upside_mlp_create(uint32_t seed);
+void upside_mlp_free(UpsideMlp* net);
+int upside_mlp_n_param(void);
+int upside_mlp_get_params(UpsideMlp* net, float* params, int n_param);
+int upside_mlp_set_params(UpsideMlp* net, const float* params, int n_param);
+int upside_mlp_set_data(UpsideMlp* net, const float* features, const float* targets, int n_rows);
+int upside_mlp_loss_grad(UpsideMlp* net, float* loss, float* grad);
+int upside_mlp_train(UpsideMlp* net, int n_steps, int batch_rows, float lr, float* losses);
+int upside_mlp_forward(UpsideMlp* net, const float* features, int n_rows, float* out);
 
 #ifdef __cplusplus
 }

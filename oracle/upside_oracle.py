@@ -1158,7 +1158,17 @@ class BackbonePairs(PotentialNode):
 
 class Rotamer(PotentialNode):
     """Rotamer side-chain free energy by loopy belief propagation
-    (src/rotamer.cpp:628-1169)."""
+    (src/rotamer.cpp:628-1169).  With `residual` set (Engine(...,
+    bp_residual=x)) the solve stops when the largest two-sided change of a
+    node belief over a chunk is at most x instead of the reference's
+    one-sided test against tol: the converged fixed point that the engine's
+    warm-started schedule is checked against (same start and updates)."""
+    residual = None
+    # with force_iters set, the solve runs exactly that many iterations (the
+    # reference's loop with tol 0 and max_iter = force_iters): comparisons with
+    # an engine that stopped after that many iterations then see the
+    # arithmetic alone, not where the one-sided test at tol happened to stop
+    force_iters = None
 
     def load(self, f, g):
         self.ig = IGraph(f, g + "/pair_interaction", self.T, self.arg(0), None, RotamerPair())
@@ -1257,7 +1267,11 @@ class Rotamer(PotentialNode):
                 cur[k] = cur[k] / cur[k].max(1, keepdims=True)
         dev, it = np.float32(1e10), 0
         d = self.damping
-        while dev > self.tol and it < self.max_iter:
+        tol = self.tol if self.residual is None else self.T(self.residual)
+        max_iter = self.max_iter
+        if self.force_iters is not None:  # exactly that many iterations (the reference's loop with tol 0)
+            tol, max_iter = self.T(-1), self.force_iters
+        while dev > tol and it < max_iter:
             for _ in range(self.chunk):
                 old = cur
                 cur = {k: prob[k].copy() for k in (3, 6)}
@@ -1265,7 +1279,10 @@ class Rotamer(PotentialNode):
                 for k in (3, 6):
                     if len(cur[k]):
                         cur[k] = (1 - d) * cur[k] / cur[k].max(1, keepdims=True) + d * old[k]
-            dev = max([np.float32(0)] + [np.max(cur[k] - old[k]) for k in (3, 6) if len(cur[k])])
+            if self.residual is None:
+                dev = max([np.float32(0)] + [np.max(cur[k] - old[k]) for k in (3, 6) if len(cur[k])])
+            else:  # the engine's converged schedule: two-sided change over the chunk
+                dev = max([np.float32(0)] + [np.max(np.abs(cur[k] - old[k])) for k in (3, 6) if len(cur[k])])
             it += self.chunk
         self.node_marg = {1: np.ones((self.n_elem[1], 1), T)}
         for k in (3, 6):
@@ -1344,7 +1361,7 @@ def node_type_for(name):
 class Engine:
     """Oracle DerivEngine over one `.up` file."""
 
-    def __init__(self, path, dtype=np.float32, n_atom=None):
+    def __init__(self, path, dtype=np.float32, n_atom=None, bp_residual=None):
         self.T = np.dtype(dtype).type
         f = h5lite.File(path)
         if n_atom is None:
@@ -1373,6 +1390,8 @@ class Engine:
             node = NODE_TYPES[t](self, nm, deps[nm])
             self.nodes[nm] = node
             node.load(f, "/input/potential/" + nm)
+            if bp_residual is not None and isinstance(node, Rotamer):
+                node.residual = bp_residual
         f.close()
         self._levels(deps)
 

@@ -95,6 +95,52 @@ def test_chig_frames(ue, chig):
         assert d < 2e-3, (name, d)
 
 
+@pytest.mark.parametrize("level", [1, 2])
+def test_detailed_and_extensive_node_loggers(ue, chig, tmp_path, level):
+    """Frames logged at the chignolin golden positions: rama_map_potential and
+    sigmoid_coupling (detailed level, src/rama_map_pot.cpp:49-55,
+    src/environment.cpp:892-900) against the values the reference logged for
+    those frames (1e-4 absolute on values of order 1-10); at the extensive
+    level environment_coverage, hbond_environment_coverage and virtual
+    (src/environment.cpp:304-310, :355-360, src/hbond.cpp:53-58) equal to
+    the node outputs of an evaluation at the same positions."""
+    import h5lite
+    path, z = chig
+    frames = [0, 9, 19]
+    say("case: node loggers at level %d on chignolin frames" % level)
+    b = ue.Batch([path], [0.8], seed=1)
+    out = str(tmp_path / ("log%d.up" % level))
+    b.open_output([out], log_level=level)
+    for f in frames:
+        b.set_pos(0, np.ascontiguousarray(z["frame_pos"][f], np.float32))
+        b.run_logged(1, 1)
+    b.close_output()
+    with h5lite.File(out) as h:
+        names = h.children("/output")
+        got = {k: h.read("/output/" + k) for k in ("rama_map_potential", "sigmoid_coupling")}
+        ext = {k: h.read("/output/" + k) for k in ("environment_coverage", "hbond_environment_coverage", "virtual")
+               if k in names}
+    for k, v in got.items():
+        want = z["frame_" + k][frames]
+        say("%s max |logged - reference| %.2e" % (k, np.abs(v - want).max()))
+        assert v.shape == want.shape and np.abs(v - want).max() < 1e-4, k
+    if level < 2:
+        assert not ext
+        return
+    assert set(ext) == {"environment_coverage", "hbond_environment_coverage", "virtual"}
+    eng = ue.Upside(path, quiet=True)
+    with h5lite.File(path) as h:
+        nodes = h.children("/input/potential")
+    env = [n for n in nodes if n.startswith("environment_coverage")][0]
+    hbe = [n for n in nodes if n.startswith("hb_environment_coverage")][0]
+    for k, f in enumerate(frames):
+        eng.energy(np.ascontiguousarray(z["frame_pos"][f], np.float32))
+        np.testing.assert_allclose(ext["environment_coverage"][k], eng.get_output(env).ravel(), rtol=1e-6, atol=1e-7)
+        np.testing.assert_allclose(ext["hbond_environment_coverage"][k], eng.get_output(hbe).ravel(), rtol=1e-6,
+                                   atol=1e-7)
+        np.testing.assert_allclose(ext["virtual"][k], eng.get_output("infer_H_O")[:, :3], rtol=1e-6, atol=1e-6)
+
+
 @pytest.mark.parametrize("n_res,seed", [(40, 1), (100, 2), (500, 3), (1000, 4)])
 def test_random_chain_energy_forces(ue, chain_file, n_res, seed):
     path = chain_file(n_res, seed)
@@ -396,6 +442,13 @@ def test_capacity_regrowth_is_transparent(ue, chain_file, monkeypatch):
     assert small.energy(pos) == e0
     np.testing.assert_array_equal(small.deriv(pos), d0)
     b1 = ue.Batch([path] * 2, [0.85, 0.9], seed=4)
+    early = b1.save_state()       # saved at the small capacities, before any overflow
+    b1.run(5)
+    for s in range(2):
+        np.testing.assert_array_equal(b1.pos(s), b0.pos(s))
+    # a batch that grew rolls back to its own earlier (smaller) checkpoint and
+    # repeats the run bit for bit
+    b1.load_state(early)
     b1.run(5)
     for s in range(2):
         np.testing.assert_array_equal(b1.pos(s), b0.pos(s))
@@ -668,11 +721,14 @@ def test_recentering_matches_reference_cli(ue, chain_file, tmp_path):
         b.set_recentering(3)
 
 
-def test_annealing_matches_reference_cli(ue, chain_file, tmp_path):
+@pytest.mark.parametrize("interval", [1, 2])
+def test_annealing_matches_reference_cli(ue, chain_file, tmp_path, interval):
     """--anneal-factor 0.5 --anneal-start 0.03 --anneal-end 0.13 over 6 rounds
     (the schedule holds T0, then blends, then reaches T1 inside the run)
     against the reference CLI: logged temperatures equal, positions and
-    potentials within 1e-4 relative."""
+    potentials within 1e-4 relative.  With a thermostat interval of 2 rounds
+    the temperature changes only at thermostat rounds (src/main.cpp:974-977),
+    so frames between them log the last one set."""
     import shutil
     import subprocess
     import h5lite
@@ -683,13 +739,14 @@ def test_annealing_matches_reference_cli(ue, chain_file, tmp_path):
     ref_up, our_up = str(tmp_path / "ref.up"), str(tmp_path / "ours.up")
     shutil.copy(src, ref_up)
     shutil.copy(src, our_up)
-    say("case: annealing, L=40, 6 rounds, against the reference CLI")
+    say("case: annealing, L=40, 6 rounds, thermostat interval %d, against the reference CLI" % interval)
     r = subprocess.run([REF_BIN, "--duration", "0.162", "--frame-interval", "0.027", "--temperature", "0.85",
                         "--anneal-factor", "0.5", "--anneal-start", "0.03", "--anneal-end", "0.13",
+                        "--thermostat-interval", "%.3f" % (0.027 * interval),
                         "--seed", "7", "--log-level", "basic", "--disable-recentering", ref_up],
                        env=dict(os.environ, OMP_NUM_THREADS="1"), capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
-    b = ue.Batch([our_up], [0.85], seed=7, dt=0.009)
+    b = ue.Batch([our_up], [0.85], seed=7, dt=0.009, thermostat_interval=interval)
     b.set_annealing(0.5, 0.03, 0.13)
     b.open_output(None)
     b.run_logged(6, 1)
@@ -871,6 +928,30 @@ def test_team_execution_matches_single_workgroup(ue, chain_file, team):
     np.testing.assert_allclose(bt.energies(), b1.energies(), rtol=1e-5)
 
 
+@pytest.mark.parametrize("n_res,layout", [(120, 1), (400, 0), (400, 2)])
+def test_rotamer_solve_layouts_match(ue, chain_file, n_res, layout):
+    """The rotamer BP solve on the whole team (state in memory) or on one
+    workgroup (state in LDS, or in memory when it does not fit) computes the
+    same arithmetic in the same order: the trajectory is bit for bit the
+    single-workgroup one.  At 400 residues the random start has more BP edges
+    than one workgroup's LDS holds, so layout 0 solves on the team there."""
+    path = chain_file(n_res, 73)
+    b1 = ue.Batch([path], [0.85], seed=6)
+    b1.run(3)
+    bt = ue.Batch([path], [0.85], seed=6)
+    team = min(8, bt.max_team())
+    if team < 2:
+        pytest.skip("no team of two workgroups fits")
+    bt.set_team(team)
+    bt.set_bp_layout(layout)
+    say("case: L=%d team %d bp layout %d vs single workgroup" % (n_res, team, layout))
+    bt.run(3)
+    st1, stt = b1.bp_stats(0), bt.bp_stats(0)
+    assert st1["total"] == stt["total"] and st1["solves"] == stt["solves"]
+    np.testing.assert_array_equal(bt.pos(0), b1.pos(0))
+    np.testing.assert_array_equal(bt.mom(0), b1.mom(0))
+
+
 def test_team_evaluations_are_fresh(ue, chain_file):
     """Every evaluation launch of a team recomputes every term, whatever ran
     before it: right after a trajectory launch, after positions set by the
@@ -907,3 +988,163 @@ def test_team_evaluations_are_fresh(ue, chain_file):
     for s in range(2):
         bf.set_pos(s, b1.pos(s))
     check(bf, "fresh batch, positions from the host")
+
+
+def _threefry_uniform(seed, done, k):
+    """Draw k of RandomGenerator(seed, 1, 0, done).uniform_open_closed()
+    (src/main.cpp:266, src/random.h): Threefry-4x32-20 of counter
+    (done lo, done hi, 0, k), first word to (0, 1] in float32."""
+    bits = O.threefry4x32_20(np.array([[done & 0xFFFFFFFF, done >> 32, 0, k]], np.uint32), [seed, 1, 0, 0])[0]
+    return np.float32(bits[0]) * np.float32(2.0 ** -32) + np.float32(2.0 ** -33)
+
+
+# Force bar where the rotamer solves of two engines stopped after different
+# iteration counts.  The reference stops its BP when the one-sided change
+# over a chunk is at most tol = 1e-3 (src/rotamer.cpp:1139-1152), which leaves
+# the marginals up to ~1e-3 from the fixed point; two float32 engines whose
+# roundings differ can stop a chunk apart, and the forces then differ by up
+# to a few 1e-4 (measured: 1.0e-4 at worst over 64 chains after 20 rounds,
+# 3e-4 at the ideal start of a 150-residue chain).  Compared with the
+# iteration count matched, the bar is F_TOL.
+F_TOL_STOP = 5e-4
+
+
+def _check_pairs_and_edges(ue, path, positions, label, ref_lib_obj, f64_every=1):
+    """Energy and forces of every position set against the float32 oracle,
+    the compiled reference and (every f64_every-th) the float64 oracle; every
+    pair graph's edge set against the float32 oracle's (XOR 0).  The oracles
+    also run their BP for exactly the engine's iteration count: those forces
+    are held to F_TOL; the plain comparisons, whose BP may stop elsewhere, to
+    F_TOL when the counts agree and F_TOL_STOP otherwise (the compiled
+    reference's count is not readable: F_TOL_STOP)."""
+    n_atom = len(positions[0])
+    eng = ue.Upside(path, quiet=True)
+    ora = O.Engine(path)
+    ora64 = O.Engine(path, dtype=np.float64)
+    ref = RefEngine(ref_lib_obj, path, n_atom) if ref_lib_obj is not None else None
+    worst = {}
+    for k, pos in enumerate(positions):
+        pos = np.ascontiguousarray(pos, np.float32)
+        e, d = float(eng.energy(pos)), eng.deriv(pos)
+        it = int(eng.get_value_by_name((1,), "rotamer", "iterations")[0])
+        e32, d32 = float(ora.energy(pos)), ora.deriv(pos)
+        same32 = ora.nodes["rotamer"].iterations == it
+        err = dict(e32=abs(e - e32) / max(abs(e32), 1.0), f32=rel_rms(d, d32))
+        bars = dict(f32=F_TOL if same32 else F_TOL_STOP)
+        ora.nodes["rotamer"].force_iters = it
+        err["f32_matched"] = rel_rms(d, ora.deriv(pos))
+        ora.nodes["rotamer"].force_iters = None
+        if k % f64_every == 0:
+            p64 = pos.astype(np.float64)
+            e64, d64 = float(ora64.energy(p64)), ora64.deriv(p64)
+            bars["f64"] = F_TOL if ora64.nodes["rotamer"].iterations == it else F_TOL_STOP
+            err.update(e64=abs(e - e64) / max(abs(e64), 1.0), f64=rel_rms(d, d64))
+            ora64.nodes["rotamer"].force_iters = it
+            err["f64_matched"] = rel_rms(d, ora64.deriv(p64))
+            ora64.nodes["rotamer"].force_iters = None
+        if ref is not None:
+            er, dr = ref.energy(pos), ref.deriv(pos)
+            err.update(eref=abs(e - er) / max(abs(er), 1.0), fref=rel_rms(d, dr))
+            bars["fref"] = F_TOL_STOP
+        for key, v in err.items():
+            worst[key] = max(worst.get(key, 0.0), v)
+            bar = E_TOL if key.startswith("e") else bars.get(key, F_TOL)
+            assert v < bar, (label, k, key, v, bar)
+        for name in PAIR_NODES:
+            want = oracle_edges(ora, name)
+            got = set(map(tuple, eng.edge_pairs(name).tolist()))
+            assert got == want, (label, k, name, len(got ^ want))
+    say(label, "worst over %d:" % len(positions), " ".join("%s=%.2e" % kv for kv in sorted(worst.items())))
+    return worst
+
+
+def test_replica_exchange_16x150_in_one_launch(ue, chain_file):
+    """Sixteen replicas of one 150-residue chain at temperatures geometric
+    over 0.80-0.95, neighbour swap sets alternating every 7 rounds, the
+    exchange inside the persistent launch (no host round trip per swap).
+    Every attempt's decisions replayed from Threefry stream 1 on the
+    energies the replicas held (src/main.cpp:243-316, criterion 0); replica
+    indices permuted by exactly the accepted swaps; after 105 rounds every
+    replica's energy and forces against the oracles and the compiled
+    reference, and every pair graph's edge set against the oracle's."""
+    path = chain_file(150, 2)
+    temps = [0.80 * (0.95 / 0.80) ** (i / 15.0) for i in range(16)]
+    beta = (1.0 / np.array(temps, np.float32)).astype(np.float32)
+    sets = [[(i, i + 1) for i in range(0, 15, 2)], [(i, i + 1) for i in range(1, 14, 2)]]
+    b = ue.Batch([path] * 16, temps, seed=2)
+    b.set_team(min(4, b.max_team()))
+    b.set_replica_exchange(7, sets, criterion=0)
+    say("case: 16x150 replica exchange in the launch, team %d" % b.team())
+    offs = [0, len(sets[0]), len(sets[0]) + len(sets[1])]
+    rep_prev = np.arange(16)
+    suc_prev = np.zeros(offs[-1], np.int64)
+    att_prev = np.zeros(offs[-1], np.int64)
+    n_acc = near = 0
+    for seg in range(15):
+        b.run_logged(7, 0)
+        done = 7 * (seg + 1)
+        rep, suc, att = b.replica_stats()
+        k = seg % 2
+        lo, hi = offs[k], offs[k + 1]
+        assert np.all(att[lo:hi] - att_prev[lo:hi] == 1) and np.all(np.delete(att - att_prev, np.s_[lo:hi]) == 0)
+        acc = (suc - suc_prev)[lo:hi]
+        e_now = b.energies()
+        want_rep = rep_prev.copy()
+        draw = 0
+        for (s1, s2), a in zip(sets[k], acc):
+            # energies the pair held at the attempt (an accepted pair traded positions)
+            e1, e2 = (e_now[s2], e_now[s1]) if a else (e_now[s1], e_now[s2])
+            diff = np.float32((np.float32(-beta[s1] * e2) + np.float32(-beta[s2] * e1))
+                              - (np.float32(-beta[s1] * e1) + np.float32(-beta[s2] * e2)))
+            want = 1
+            if diff < 0:
+                u = _threefry_uniform(2, done, draw)
+                draw += 1
+                want = 0 if np.exp(diff) < u else 1
+                if abs(float(np.log(u)) - float(diff)) < 1e-4:
+                    near += 1
+                    continue
+            assert a == want, (seg, s1, s2, diff)
+            if a:
+                want_rep[[s1, s2]] = want_rep[[s2, s1]]
+            n_acc += a
+        np.testing.assert_array_equal(rep, want_rep)
+        rep_prev, suc_prev, att_prev = rep, suc, att
+    say("accepted %d swaps over 15 attempts; %d decisions within 1e-4 of the threshold" % (n_acc, near))
+    assert 0 < n_acc and near == 0
+    _check_pairs_and_edges(ue, path, [b.pos(s) for s in range(16)], "16x150 after 105 rounds", ref_lib())
+
+
+def test_uneven_batch_64_chains(ue, chain_file):
+    """Sixty-four independent chains of lengths uniform in [50, 400] in one
+    launch (team autotuned as the bench does): after 20 rounds every chain's
+    energy and forces against the float32 oracle and the compiled reference
+    (float64 for every fourth), and every pair graph's edge set against the
+    oracle's."""
+    lens = [int(x) for x in np.random.default_rng(3).integers(50, 401, 64)]
+    paths = [chain_file(L, 100 + i) for i, L in enumerate(lens)]
+    b = ue.Batch(paths, [0.85] * 64, seed=3)
+    cand = [t for t in (1, 2, 4) if t <= b.max_team()]
+    b.autotune(2, cand)
+    say("case: 64 uneven chains (%d residues), team %d" % (sum(lens), b.team()))
+    b.run(20)
+    lib = ref_lib()
+    for s, p in enumerate(paths):
+        _check_pairs_and_edges(ue, p, [b.pos(s)], "chain %d (L=%d)" % (s, lens[s]), lib, f64_every=1 if s % 4 == 0 else 2)
+
+
+def test_long_run_500_residues_sampled(ue, chain_file):
+    """The plain-method larger workload over a long run: 3,333 rounds (9,999
+    steps) of the 500-residue chain in 10 launches; at the end of each, the
+    engine's energy and forces against the float32 and float64 oracles and
+    the compiled reference, and edge sets against the oracle's."""
+    path = chain_file(500, 1)
+    b = ue.Batch([path], [0.85], seed=1)
+    b.set_team(min(16, b.max_team()))
+    samples = []
+    for k in range(10):
+        say("case: 500-residue long run, segment %d" % k)
+        b.run(333 + (3 if k == 9 else 0))
+        samples.append(b.pos(0))
+    assert np.all(np.isfinite(samples[-1]))
+    _check_pairs_and_edges(ue, path, samples, "500 residues, 10 samples over 3333 rounds", ref_lib())

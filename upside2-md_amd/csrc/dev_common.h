@@ -45,6 +45,8 @@ struct Ctx {
     int tpar2;                // the sub-team's next reduction buffer parity
     float* tred;       // team reduction scratch, 2 buffers of G floats
     int tpar;          // buffer parity of the next team reduction
+    int bp_layout;     // where rotamer solves run (see bp_on_one_wg)
+    const RunArgs* args;  // the launch's arguments (rotamer solve schedule, warm-start network)
 };
 
 __device__ __forceinline__ unsigned long long ptick(const Ctx& c) { return c.prof ? wall_clock64() : 0ull; }

@@ -1,6 +1,7 @@
 // Rotamer side-chain free energy by loopy belief propagation on the device
 // (src/rotamer.cpp:628-1169).  Graph work is spread over the system's team of
-// workgroups; the iterative solve runs on one workgroup.
+// workgroups; the iterative solve runs on one workgroup when its state fits
+// that workgroup's LDS, else on the whole team.
 //
 // Residue nodes carry 1, 3 or 6 rotamer states, one bead per state.  Residue
 // pairs that may hold a bead pair of the rotamer pair graph's cache are
@@ -27,7 +28,7 @@ namespace upside {
 struct RotView {
     gp<const int> nrot, bead0, brot, bnode;
     gp<int> coff, ca, cb, scand, ioff, iinc, eidx, bpc, bpf, bps, ninc, binc, stats;
-    gp<float> P, msg, marg, e1x, nprob, ncur, nold, noff;
+    gp<float> P, msg, marg, e1x, nprob, ncur, nold, noff, feat, nprev;
     int cap;
 };
 
@@ -42,6 +43,8 @@ __device__ RotView rot_view(Ctx& c, const RotamerDev& R) {
     v.P = gmem(c.A + R.P); v.msg = gmem(c.A + R.msg); v.marg = gmem(c.A + R.marg); v.e1x = gmem(c.A + R.e1x);
     v.nprob = gmem(c.A + R.nprob); v.ncur = gmem(c.A + R.ncur); v.nold = gmem(c.A + R.nold);
     v.noff = gmem(c.A + R.noff);
+    v.feat = gmem(c.A + R.feat);
+    v.nprev = gmem(c.A + R.nprev);
     v.cap = R.ccap;
     return v;
 }
@@ -328,6 +331,20 @@ struct BpMem {
     gp<const int> pos; // position of BP edge e in this order (null: the order of the BP edges)
 };
 
+// Who runs a BP pass: one workgroup (TEAM = false: threads c.tid of NT,
+// workgroup barriers) or the system's whole team (TEAM = true: threads c.gtid
+// of c.gsize, team barriers; state in memory).  The arithmetic of every edge
+// and node is the same either way, so both give identical results.
+template <bool TEAM> __device__ __forceinline__ int bp_t(const Ctx& c) { return TEAM ? c.gtid : c.tid; }
+template <bool TEAM> __device__ __forceinline__ int bp_nt(const Ctx& c) { return TEAM ? c.gsize : NT; }
+template <bool TEAM> __device__ __forceinline__ void bp_sync(Ctx& c) {
+    if (TEAM) team_sync(c);
+    else __syncthreads();
+}
+template <bool TEAM> __device__ __forceinline__ float bp_max(Ctx& c, float v) {
+    return TEAM ? team_max(c, v) : block_max(c, v);
+}
+
 // New messages of edges e[0..W) (W = 1 or 2) of one class A x B (the first
 // node has A <= B states; padded states of an edge never enter: their matrix
 // entries and beliefs are zero) from old node beliefs `nold_` and the edges'
@@ -388,15 +405,16 @@ __device__ __forceinline__ void rot_edge_cls(const BpMem<F, I, J>& m, int cls, c
     else rot_edge_w<W, 6, 6>(m, e, nold_);
 }
 
-// One synchronous message pass over all BP edges.  One workgroup; a thread
-// takes edges e and e+NT together when they are of one class (edges sorted by
-// class keep most waves on one code path).
-template <class F, class I, class J>
+// One synchronous message pass over all BP edges.  A thread takes edges e and
+// e+nt together when they are of one class (edges sorted by class keep most
+// waves on one code path).
+template <bool TEAM, class F, class I, class J>
 __device__ void rot_edge_pass(Ctx& c, const BpMem<F, I, J>& m, int nbp, F nold_) {
     auto cls = [&](int k) { return k < m.c1 ? 0 : (k < m.c2 ? 1 : 2); };
-    int e = c.tid;
-    for (; e + NT < nbp; e += 2 * NT) {
-        const int pair[2] = {e, e + NT};
+    const int nt = bp_nt<TEAM>(c);
+    int e = bp_t<TEAM>(c);
+    for (; e + nt < nbp; e += 2 * nt) {
+        const int pair[2] = {e, e + nt};
         const int ca = cls(e), cb = cls(e + NT);
         if (ca == cb) {
             rot_edge_cls<2>(m, ca, pair, nold_);
@@ -406,34 +424,100 @@ __device__ void rot_edge_pass(Ctx& c, const BpMem<F, I, J>& m, int nbp, F nold_)
         }
     }
     if (e < nbp) rot_edge_cls<1>(m, cls(e), &e, nold_);
-    __syncthreads();
+    bp_sync<TEAM>(c);
 }
 
 // Node pass: cur = prob * prod(new messages), renormalised after every factor;
 // then cur = (1-d) cur/max(cur) + d*old (standardize_belief_update, :258-273).
 // With swap_first, the product goes to `nold_` and cur restarts from prob
 // (the first, undamped pass of solve_for_marginals).  Padding states carry
-// zeros throughout.  One workgroup.
-template <class F, class I, class J>
+// zeros throughout.
+//   On one workgroup a thread takes a node (incidences and messages in LDS
+// are cheap to walk).  On the team, with the state in memory, a group of
+// NODE_LANES lanes takes a node: each lane loads one incidence and its
+// message, and every lane of the group then applies the factors in incidence
+// order from the group's registers, so the arithmetic (and the result) is the
+// same as a single thread's while the dependent memory trips are paid once
+// per NODE_LANES incidences.
+constexpr int NODE_LANES = 16;
+
+// The roundings are pinned (no contraction of a product into a later sum) so
+// that every code path computing a node product gives the same bits.
+template <class F>
+__device__ __forceinline__ void bp_factor(float* b, const float* a) {
+    float sum = 0.f;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) { b[r] = __fmul_rn(b[r], a[r]); sum = __fadd_rn(sum, b[r]); }
+    float inv = __builtin_amdgcn_rcpf(sum);  // renormalisation only: 1 ulp is enough
+#pragma unroll
+    for (int r = 0; r < 6; ++r) b[r] = __fmul_rn(b[r], inv);
+}
+
+template <class F>
+__device__ __forceinline__ void bp_node_store(int x, const float* pr, const float* b, F ncur_, F nold_,
+                                              float damping, bool swap_first) {
+    if (swap_first) {
+        float mx = 0.f;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) mx = fmaxf(mx, pr[r]);
+        float im = 1.f / mx;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            nold_[x * 6 + r] = b[r];
+            ncur_[x * 6 + r] = pr[r] * im;
+        }
+    } else {
+        float mx = 0.f;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) mx = fmaxf(mx, b[r]);
+        float im = 1.f / mx;
+        const float w = __fmul_rn(1.f - damping, im);
+#pragma unroll
+        for (int r = 0; r < 6; ++r) ncur_[x * 6 + r] = __fmaf_rn(damping, nold_[x * 6 + r], __fmul_rn(w, b[r]));
+    }
+}
+
+template <bool TEAM, class F, class I, class J>
 __device__ void rot_node_pass(Ctx& c, const BpMem<F, I, J>& m, int n_node, F ncur_, F nold_, float damping,
                               bool swap_first) {
-    for (int x = c.tid; x < n_node; x += NT) {
+    if (TEAM) {
+        const int t = bp_t<TEAM>(c), gl = t % NODE_LANES;
+        for (int x = t / NODE_LANES; x < n_node; x += bp_nt<TEAM>(c) / NODE_LANES) {  // uniform in a group
+            if (m.nrot[x] == 1) continue;
+            float pr[6], b[6];
+#pragma unroll
+            for (int r = 0; r < 6; ++r) { pr[r] = m.prob[x * 6 + r]; b[r] = pr[r]; }
+            const int t0 = m.ioff[x], n = m.ninc[x];
+            for (int tb = 0; tb < n; tb += NODE_LANES) {
+                float a[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                if (tb + gl < n) {
+                    const int code = m.inc[t0 + tb + gl];
+                    const int b0 = ((code & 1) * 6) * m.ms + (code >> 1);
+#pragma unroll
+                    for (int r = 0; r < 6; ++r) a[r] = m.msg[b0 + r * m.ms];
+                }
+                const int cnt = min(NODE_LANES, n - tb);
+                for (int q = 0; q < cnt; ++q) {
+                    float f[6];
+#pragma unroll
+                    for (int r = 0; r < 6; ++r) f[r] = __shfl(a[r], q, NODE_LANES);
+                    bp_factor<F>(b, f);
+                }
+            }
+            if (gl == 0) bp_node_store(x, pr, b, ncur_, nold_, damping, swap_first);
+        }
+        bp_sync<TEAM>(c);
+        return;
+    }
+    for (int x = bp_t<TEAM>(c); x < n_node; x += bp_nt<TEAM>(c)) {
         if (m.nrot[x] == 1) continue;
         float pr[6], b[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r) { pr[r] = m.prob[x * 6 + r]; b[r] = pr[r]; }
         int t0 = m.ioff[x], n = m.ninc[x];
-        auto factor = [&](const float* a) {
-            float sum = 0.f;
-#pragma unroll
-            for (int r = 0; r < 6; ++r) { b[r] *= a[r]; sum += b[r]; }
-            float inv = __builtin_amdgcn_rcpf(sum);  // renormalisation only: 1 ulp is enough
-#pragma unroll
-            for (int r = 0; r < 6; ++r) b[r] *= inv;
-        };
         // four incidences per step: their indices, then all their messages,
         // are loaded before the first factor is applied (the factors stay in
-        // incidence order); in memory each incidence is two dependent L2 trips
+        // incidence order)
         int t = 0;
         for (; t + 3 < n; t += 4) {
             int base[4];
@@ -448,7 +532,7 @@ __device__ void rot_node_pass(Ctx& c, const BpMem<F, I, J>& m, int n_node, F ncu
 #pragma unroll
                 for (int r = 0; r < 6; ++r) a[q][r] = m.msg[base[q] + r * m.ms];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) factor(a[q]);
+            for (int q = 0; q < 4; ++q) bp_factor<F>(b, a[q]);
         }
         for (; t < n; ++t) {
             const int code = m.inc[t0 + t];
@@ -456,68 +540,247 @@ __device__ void rot_node_pass(Ctx& c, const BpMem<F, I, J>& m, int n_node, F ncu
             float a0[6];
 #pragma unroll
             for (int r = 0; r < 6; ++r) a0[r] = m.msg[b0 + r * m.ms];
-            factor(a0);
+            bp_factor<F>(b, a0);
         }
-        if (swap_first) {
-            float mx = 0.f;
+        bp_node_store(x, pr, b, ncur_, nold_, damping, swap_first);
+    }
+    bp_sync<TEAM>(c);
+}
+
+// ---- warm start (bp_mode 1 and 2; no counterpart in the reference) ----------
+// Features of node x after the first pass (b0 = its undamped product of the
+// first messages, pr = its probabilities, prev = its marginal at the end of
+// the previous solve of this system, all zero before the first), MLP_IN floats:
+//   [0,6)   log pr_r - max (clamped at -30), 0 off the node's states
+//   [6,12)  log b0_r - max (clamped at -30): the first pass's log beliefs
+//   [12,18) log prev_r - max (clamped at -30), 0 without a previous solve
+//   [18,24) state mask
+//   [24]    BP incidences / 8 (edge degree), [25] six states, [26] entropy
+//   and [27] 1 - max of softmax([6,12)), [28] a previous solve exists, the
+//   rest 0.
+// Nodes with one state get a row of zeros.
+template <bool TEAM, class F, class I, class J>
+__device__ void bp_features(Ctx& c, const BpMem<F, I, J>& m, int n_node, F b0, gp<const float> prev,
+                            gp<float> feat) {
+    for (int x = bp_t<TEAM>(c); x < n_node; x += bp_nt<TEAM>(c)) {
+        float f[MLP_IN];
 #pragma unroll
-            for (int r = 0; r < 6; ++r) mx = fmaxf(mx, pr[r]);
-            float im = 1.f / mx;
+        for (int k = 0; k < MLP_IN; ++k) f[k] = 0.f;
+        const int nr = m.nrot[x];
+        if (nr > 1) {
+            float mp = -3.0e38f, mb = -3.0e38f;
 #pragma unroll
-            for (int r = 0; r < 6; ++r) {
-                nold_[x * 6 + r] = b[r];
-                ncur_[x * 6 + r] = pr[r] * im;
+            for (int r = 0; r < 6; ++r)
+                if (r < nr) {
+                    f[r] = logf(fmaxf(m.prob[x * 6 + r], 1e-30f));
+                    f[6 + r] = logf(fmaxf(b0[x * 6 + r], 1e-30f));
+                    mp = fmaxf(mp, f[r]);
+                    mb = fmaxf(mb, f[6 + r]);
+                }
+            float se = 0.f, sle = 0.f;
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+                if (r < nr) {
+                    f[r] = fmaxf(f[r] - mp, -30.f);
+                    f[6 + r] = fmaxf(f[6 + r] - mb, -30.f);
+                    f[18 + r] = 1.f;
+                    const float e = expf(f[6 + r]);
+                    se += e;
+                    sle += e * f[6 + r];
+                }
+            f[24] = (float)m.ninc[x] * 0.125f;
+            f[25] = nr == 6 ? 1.f : 0.f;
+            f[26] = logf(se) - sle / se;   // entropy of softmax(log b0)
+            f[27] = 1.f - 1.f / se;        // 1 - its largest probability (max log b0 is 0)
+            float pv[6], mv = 0.f;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) { pv[r] = prev[x * 6 + r]; mv = fmaxf(mv, pv[r]); }
+            if (mv > 0.f) {
+                const float lm = logf(mv);
+#pragma unroll
+                for (int r = 0; r < 6; ++r)
+                    if (r < nr) f[12 + r] = fmaxf(logf(fmaxf(pv[r], 1e-30f)) - lm, -30.f);
+                f[28] = 1.f;
             }
-        } else {
-            float mx = 0.f;
+        }
 #pragma unroll
-            for (int r = 0; r < 6; ++r) mx = fmaxf(mx, b[r]);
-            float im = 1.f / mx;
+        for (int k = 0; k < MLP_IN; ++k) feat[x * MLP_IN + k] = f[k];
+    }
+    bp_sync<TEAM>(c);
+}
+
+// The network over all node rows (16-row tiles, one wave each): the warm
+// beliefs of node x are exp(z_r - max z), z_r = feat[6 + r] + out_r over its
+// states (max-normalised, as the damped update leaves beliefs), into cur.
+template <bool TEAM, class F, class I, class J>
+__device__ void bp_warm_beliefs(Ctx& c, const BpMem<F, I, J>& m, const MlpDev& w, int n_node,
+                                gp<const float> feat, F cur) {
+    const int t = bp_t<TEAM>(c), nw = bp_nt<TEAM>(c) / 64;
+    const int l = t & 63, g = l >> 4, cl = l & 15;
+    for (int tile = t / 64; tile * 16 < n_node; tile += nw) {  // uniform over the wave
+        f32x4 y;
+        mlp_tile_infer(w, feat, n_node, tile * 16, y);
+        float yo[8];
 #pragma unroll
-            for (int r = 0; r < 6; ++r) ncur_[x * 6 + r] = (1.f - damping) * im * b[r] + damping * nold_[x * 6 + r];
+        for (int i = 0; i < 4; ++i) {
+            yo[i] = __shfl(y[i], cl, 64);
+            yo[4 + i] = __shfl(y[i], 16 + cl, 64);
+        }
+        const int x = tile * 16 + cl;
+        if (g == 0 && x < n_node) {
+            const int nr = m.nrot[x];
+            if (nr > 1) {
+                float z[6], mx = -3.0e38f;
+#pragma unroll
+                for (int r = 0; r < 6; ++r) {
+                    z[r] = feat[x * MLP_IN + 6 + r] + yo[r];
+                    if (r < nr) mx = fmaxf(mx, z[r]);
+                }
+#pragma unroll
+                for (int r = 0; r < 6; ++r) cur[x * 6 + r] = r < nr ? expf(z[r] - mx) : 0.f;
+            }
         }
     }
-    __syncthreads();
+    bp_sync<TEAM>(c);
+}
+
+// Messages consistent with given node beliefs b: per edge, K updates of its
+// two messages in registers with the beliefs held fixed (each new message
+// from the other side's cavity belief b / message, scaled to unit L1 norm,
+// as in rot_edge_w).  Every edge on the 6x6 path (padding is zero).
+template <bool TEAM, class F, class I, class J>
+__device__ void bp_local_messages(Ctx& c, const BpMem<F, I, J>& m, int nbp, F b, int K) {
+    for (int e = bp_t<TEAM>(c); e < nbp; e += bp_nt<TEAM>(c)) {
+        const int ends = m.ends[e];
+        const int f = ends & 0xFFFF, s = ends >> 16;
+        float p[36], m1[6], m2[6], bf[6], bs[6];
+#pragma unroll
+        for (int q = 0; q < 36; ++q) p[q] = m.P[q * m.ps + e];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            m1[r] = m.msg[r * m.ms + e];
+            m2[r] = m.msg[(6 + r) * m.ms + e];
+            bf[r] = b[f * 6 + r];
+            bs[r] = b[s * 6 + r];
+        }
+        for (int k = 0; k < K; ++k) {
+            float v1[6], v2[6], n1[6], n2[6];
+#pragma unroll
+            for (int r = 0; r < 6; ++r) {
+                v1[r] = bf[r] * __builtin_amdgcn_rcpf(1e-10f + m1[r]);
+                v2[r] = bs[r] * __builtin_amdgcn_rcpf(1e-10f + m2[r]);
+                n1[r] = 0.f;
+                n2[r] = 0.f;
+            }
+#pragma unroll
+            for (int i = 0; i < 6; ++i)
+#pragma unroll
+                for (int j = 0; j < 6; ++j) {
+                    n1[i] += p[i * 6 + j] * v2[j];
+                    n2[j] += p[i * 6 + j] * v1[i];
+                }
+            float s1 = 0.f, s2 = 0.f;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) { s1 += n1[r]; s2 += n2[r]; }
+            const float i1 = 1.f / s1, i2 = 1.f / s2;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) { m1[r] = n1[r] * i1; m2[r] = n2[r] * i2; }
+        }
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            m.msg[r * m.ms + e] = m1[r];
+            m.msg[(6 + r) * m.ms + e] = m2[r];
+        }
+    }
+    bp_sync<TEAM>(c);
+}
+
+// The plain start: messages 1, old beliefs = probabilities (:1110-1133).
+template <bool TEAM, class F, class I, class J>
+__device__ void bp_plain_start(Ctx& c, const BpMem<F, I, J>& m, int nbp, int n_node, F old) {
+    for (int e = bp_t<TEAM>(c); e < nbp; e += bp_nt<TEAM>(c))
+#pragma unroll
+        for (int r = 0; r < 12; ++r) m.msg[r * m.ms + e] = 1.f;
+    for (int i = bp_t<TEAM>(c); i < 6 * n_node; i += bp_nt<TEAM>(c)) old[i] = m.prob[i];
+    bp_sync<TEAM>(c);
 }
 
 // Iterations, node marginals and edge marginals of one solve whose state is
 // set up in m (messages all 1, `old` = prob); returns the iteration count.
-template <class F, class I, class J>
+//   Schedules (RunArgs::bp_mode): 0 is the reference's; 1 stops instead
+// when the largest two-sided change of a belief over a chunk is at most
+// bp_residual; 2 replaces the beliefs after the first pass by the
+// warm-start network's and derives consistent messages (bp_local_messages),
+// stops by the residual rule, and when bp_budget iterations do not meet it
+// restarts from the plain start as schedule 1 (the iteration count returned
+// includes both).  Schedules 1 and 2 also leave the features of the first
+// pass in R.feat (training rows for the network).
+template <bool TEAM, class F, class I, class J>
 __device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const BpMem<F, I, J>& m, int nbp, F cur,
                           F old) {
     const int n = R.n_node;
     const int cap = v.cap;
-    unsigned long long t0 = ptick(c);
-    rot_edge_pass(c, m, nbp, old);
-    pacc(c, PROF_BP_EDGE, t0);
-    t0 = ptick(c);
-    rot_node_pass(c, m, n, cur, old, 0.f, true);
-    pacc(c, PROF_BP_NODE, t0);
-    float dev = 1e10f;
-    int it = 0;
-    for (; dev > R.tol && it < R.max_iter; it += R.chunk) {
-        for (int j = 0; j < R.chunk; ++j) {
-            F t = cur; cur = old; old = t;          // swap node beliefs
-            t0 = ptick(c);
-            rot_edge_pass(c, m, nbp, old);
-            pacc(c, PROF_BP_EDGE, t0);
-            t0 = ptick(c);
-            rot_node_pass(c, m, n, cur, old, R.damping, false);
-            pacc(c, PROF_BP_NODE, t0);
+    const int t = bp_t<TEAM>(c), nt = bp_nt<TEAM>(c);
+    const int mode = c.args->bp_mode;
+    bool warm = mode == 2;
+    int total = 0;
+    for (;;) {
+        unsigned long long t0 = ptick(c);
+        rot_edge_pass<TEAM>(c, m, nbp, old);
+        pacc(c, PROF_BP_EDGE, t0);
+        t0 = ptick(c);
+        rot_node_pass<TEAM>(c, m, n, cur, old, 0.f, true);
+        pacc(c, PROF_BP_NODE, t0);
+        if (mode > 0 && total == 0)  // old: the first pass's product
+            bp_features<TEAM>(c, m, n, old, (gp<const float>)v.nprev, v.feat);
+        if (warm) {
+            bp_warm_beliefs<TEAM>(c, m, c.args->mlp, n, (gp<const float>)v.feat, cur);
+            bp_local_messages<TEAM>(c, m, nbp, cur, c.args->bp_klocal);
         }
-        float d = 0.f;
-        for (int x = c.tid; x < n; x += NT) {
-            if (m.nrot[x] == 1) continue;
+        const float tol = mode > 0 ? c.args->bp_residual : R.tol;
+        const int limit = warm ? min(c.args->bp_budget, R.max_iter) : R.max_iter;
+        float dev = 1e10f;
+        int it = 0;
+        for (; dev > tol && it < limit; it += R.chunk) {
+            for (int j = 0; j < R.chunk; ++j) {
+                F tt = cur; cur = old; old = tt;          // swap node beliefs
+                t0 = ptick(c);
+                rot_edge_pass<TEAM>(c, m, nbp, old);
+                pacc(c, PROF_BP_EDGE, t0);
+                t0 = ptick(c);
+                rot_node_pass<TEAM>(c, m, n, cur, old, R.damping, false);
+                pacc(c, PROF_BP_NODE, t0);
+            }
+            float d = 0.f;
+            for (int x = t; x < n; x += nt) {
+                if (m.nrot[x] == 1) continue;
+                if (mode > 0) {
 #pragma unroll
-            for (int r = 0; r < 6; ++r) d = fmaxf(d, cur[x * 6 + r] - old[x * 6 + r]);
+                    for (int r = 0; r < 6; ++r) d = fmaxf(d, fabsf(cur[x * 6 + r] - old[x * 6 + r]));
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 6; ++r) d = fmaxf(d, cur[x * 6 + r] - old[x * 6 + r]);
+                }
+            }
+            dev = bp_max<TEAM>(c, d);
         }
-        dev = block_max(c, d);
+        total += it;
+        if (c.gtid == 0) {
+            if (warm) v.stats[7] += 1;
+            if (mode > 0 && dev <= tol) v.stats[9] += 1;
+        }
+        if (!warm || dev <= tol) break;
+        // the warm start missed the residual within its budget: plain start
+        if (c.gtid == 0) v.stats[8] += 1;
+        warm = false;
+        bp_plain_start<TEAM>(c, m, nbp, n, old);
     }
-    t0 = ptick(c);
+    const int it = total;
+    unsigned long long t0 = ptick(c);
     // node marginals (:283-290) into memory and into `old` (free now; cur and
     // old are always distinct buffers)
     gp<float> ncur = gmem(v.ncur);
-    for (int x = c.tid; x < n; x += NT) {
+    for (int x = t; x < n; x += nt) {
         int nr = m.nrot[x];
         float b[6];
         float s = 0.f;
@@ -527,12 +790,12 @@ __device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const B
 #pragma unroll
         for (int r = 0; r < 6; ++r) b[r] = nr == 1 ? (r == 0 ? 1.f : 0.f) : b[r] * is;
 #pragma unroll
-        for (int r = 0; r < 6; ++r) { ncur[x * 6 + r] = b[r]; old[x * 6 + r] = b[r]; }
+        for (int r = 0; r < 6; ++r) { ncur[x * 6 + r] = b[r]; old[x * 6 + r] = b[r]; v.nprev[x * 6 + r] = b[r]; }
     }
-    __syncthreads();
+    bp_sync<TEAM>(c);
     // edge marginals (:425-449), written in the order of the BP edges
     gp<float> marg = gmem(v.marg);
-    for (int e = c.tid; e < nbp; e += NT) {
+    for (int e = t; e < nbp; e += nt) {
         const int k = m.pos ? m.pos[e] : e;
         const int ends = m.ends[k];
         const int f = ends & 0xFFFF, s = ends >> 16;
@@ -566,6 +829,13 @@ __device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const B
 // scratch for sorting the edges by class while staging (5 per edge).
 __device__ __forceinline__ int bp_lds_floats(int nbp, int n) { return 14 * nbp + 14 * n + 2; }
 
+// Whether a solve of nbp BP edges over n nodes runs on one workgroup with
+// its state in LDS (else on the whole team from memory, or, with
+// bp_layout 2, on one workgroup from memory).
+__device__ __forceinline__ bool bp_fits_lds(int nbp, int n, int cap) {
+    return bp_lds_floats(nbp, n) <= LDS_FLOATS && 2 * nbp < 65536 && 36 * nbp <= 12 * cap;
+}
+
 // The solve proper, on one workgroup: a pass has a few thousand flops per
 // node, too little to pay for team barriers, and a single workgroup can keep
 // messages, beliefs and incidences in LDS (then every dependent load of the
@@ -579,7 +849,7 @@ __device__ int rot_solve_wg(Ctx& c, const RotamerDev& R) {
     const int n = R.n_node;
     const int cap = v.cap;
     unsigned long long t0 = ptick(c);
-    if (bp_lds_floats(nbp, n) <= LDS_FLOATS && 2 * nbp < 65536 && 36 * nbp <= 12 * cap) {
+    if (bp_fits_lds(nbp, n, cap)) {
         if (c.tid == 0) v.stats[6] += 1;  // solves held in LDS
         lp<float> L = lmem(c.lds);
         BpMem<lp<float>, lp<int>, lp<unsigned short>> m;
@@ -646,7 +916,7 @@ __device__ int rot_solve_wg(Ctx& c, const RotamerDev& R) {
             for (int r = 0; r < 12; ++r) m.msg[r * m.ms + e] = 1.f;
         __syncthreads();
         pacc(c, PROF_BP_SETUP, t0);
-        return bp_iterate(c, R, v, m, nbp, cur, old);
+        return bp_iterate<false>(c, R, v, m, nbp, cur, old);
     }
     BpMem<gp<float>, gp<int>, gp<int>> m;
     m.P = (gp<const float>)v.P; m.ps = cap; m.nrot = v.nrot; m.prob = (gp<const float>)v.nprob;
@@ -660,11 +930,50 @@ __device__ int rot_solve_wg(Ctx& c, const RotamerDev& R) {
         for (int r = 0; r < 12; ++r) m.msg[r * m.ms + e] = 1.f;
     __syncthreads();
     pacc(c, PROF_BP_SETUP, t0);
-    return bp_iterate(c, R, v, m, nbp, v.ncur, v.nold);
+    return bp_iterate<false>(c, R, v, m, nbp, v.ncur, v.nold);
 }
 
-// Solve by rank 0 of the team while the others wait at the barrier.
-__device__ int rot_solve(Ctx& c, const RotamerDev& R) {
+// The solve on the whole team with its state in memory (matrices, messages,
+// beliefs, incidences): every pass is spread over all the team's threads and
+// separated by team barriers.  Every thread of the team returns the
+// iteration count.
+__device__ int rot_solve_team(Ctx& c, const RotamerDev& R) {
+    RotView v = rot_view(c, R);
+    const int nbp = v.stats[4];
+    const int n = R.n_node;
+    const int cap = v.cap;
+    unsigned long long t0 = ptick(c);
+    BpMem<gp<float>, gp<int>, gp<int>> m;
+    m.P = (gp<const float>)v.P; m.ps = cap; m.nrot = v.nrot; m.prob = (gp<const float>)v.nprob;
+    m.msg = v.msg; m.ms = cap;
+    m.ends = v.bpc; m.ioff = v.ioff; m.ninc = v.ninc; m.inc = v.binc;
+    m.c1 = m.c2 = 0;
+    m.pos = nullptr;
+    for (int i = c.gtid; i < n * 6; i += c.gsize) v.nold[i] = v.nprob[i];
+    for (int e = c.gtid; e < nbp; e += c.gsize)
+#pragma unroll
+        for (int r = 0; r < 12; ++r) m.msg[r * m.ms + e] = 1.f;
+    team_sync(c);
+    pacc(c, PROF_BP_SETUP, t0);
+    return bp_iterate<true>(c, R, v, m, nbp, v.ncur, v.nold);
+}
+
+// Where a solve runs: true = rank 0 alone (rot_solve_wg), false = the whole
+// team (rot_solve_team).  bp_layout 0: one workgroup when the state fits LDS
+// or the team has one workgroup, else the team; 1: always the team; 2:
+// always one workgroup.  Uniform over the team (nbp is read after a team
+// barrier).
+__device__ __forceinline__ bool bp_on_one_wg(const Ctx& c, const RotamerDev& R, int nbp, int bp_layout) {
+    if (c.G == 1 || bp_layout == 2) return true;
+    if (bp_layout == 1) return false;
+    return bp_fits_lds(nbp, R.n_node, R.ccap);
+}
+
+// Solve by rank 0 of the team while the others wait at the barrier, or by
+// the whole team (see bp_on_one_wg).
+__device__ int rot_solve(Ctx& c, const RotamerDev& R, int bp_layout) {
+    const int nbp = c.I[R.stats + 4];
+    if (!bp_on_one_wg(c, R, nbp, bp_layout)) return rot_solve_team(c, R);
     int it = 0;
     if (c.rank == 0) it = rot_solve_wg(c, R);
     team_sync(c);

@@ -9,6 +9,7 @@
 // offsets, so replicas that share a topology share one program.
 #pragma once
 #include <cstdint>
+#include "dev_mlp.h"
 
 namespace upside {
 
@@ -19,9 +20,11 @@ namespace upside {
 constexpr int LDS_FLOATS = 40704;
 // Rotamer statistics words: [0] iterations of the last solve, [1] solves at
 // the iteration limit, [2] total iterations, [3] solves, [4] BP edges, [5]
-// residue-pair candidates, [6] solves held in LDS, then a histogram of iterations per solve
-// (bin = iterations, the last bin collects the rest).
-constexpr int BP_HIST = 8;
+// residue-pair candidates, [6] solves held in LDS, [7] warm-started solves,
+// [8] warm starts that fell back to the plain start, [9] solves stopped by
+// the residual rule (bp_mode 1 and 2), then a histogram of iterations per
+// solve (bin = iterations, the last bin collects the rest).
+constexpr int BP_HIST = 12;
 constexpr int BP_HIST_BINS = 256;
 constexpr int NT = 512;            // threads per workgroup (8 wave64)
 constexpr int NWAVE = NT / 64;
@@ -169,6 +172,8 @@ struct RotamerDev {
     int ncen;              // float n_node*4 bead-sphere centre and radius of each residue
     int nprob, ncur, nold; // float n_node*6
     int noff;              // float n_node   energy offset
+    int feat;              // float n_node*MLP_IN  warm-start features of the last solve (bp_mode 1, 2)
+    int nprev;             // float n_node*6  node marginals of the previous solve (0 before the first)
     int stats;             // int [0] last iterations [1] bad solves [2] total iterations [3] solves
                            //     [4] BP edges [5] candidates
 };
@@ -230,6 +235,34 @@ enum {
     PROF_BP_SETUP = 16, PROF_EXTRA = 17
 };
 
+// Replica exchange inside a persistent launch (the swap step of
+// ReplicaExchange::attempt_swaps, src/main.cpp:243-316, between rounds): all
+// systems share one potential and differ in temperature.  When the number of
+// completed rounds reaches state[1], every team evaluates its potential and
+// stages its positions, the launch meets at a grid barrier, one thread takes
+// the Metropolis decisions of the current swap set with Threefry stream 1
+// (counter = that round count), and after a second grid barrier every system
+// of an accepted pair takes its partner's staged positions.
+struct RexDev {
+    int interval;          // rounds between attempts (0: no exchange in the launch)
+    int criterion;         // --exchange-criterion 0 or 1 (src/main.cpp:278-311)
+    int n_sets;
+    const int* set_off;    // [n_sets+1] first pair of each set
+    const int* pairs;      // [2 * pairs] system indices
+    long long* state;      // [0] attempts so far, [1] round count of the next attempt
+    long long* stats;      // [2 * pairs] successes, attempts of every pair
+    int* replica_of;       // [n_sys] replica index held by each system
+    const float* beta;     // [n_sys] 1 / temperature
+    float* energy;         // [n_sys] potentials at the attempt
+    float* stage;          // [n_sys * stage_stride] staged positions
+    int stage_stride;
+    int* partner;          // [n_sys] system whose positions to take (-1: keep)
+    unsigned long long* grid_bar;  // grid barrier arrival counter (multiple of n_wg between launches)
+    unsigned* grid_flag;           // its spin-timeout flag
+    uint32_t seed;
+    int n_wg;              // workgroups that take part (systems x team)
+};
+
 struct RunArgs {
     int mode;              // 1: potential and derivative, 0: derivative only
     int n_round;           // rounds to integrate (0: one evaluation only)
@@ -246,6 +279,21 @@ struct RunArgs {
     // 3 dt (r+1) from each system's initial temperature
     int anneal;
     double anneal_factor, anneal_start, anneal_end, anneal_duration;
+    // rotamer solves: 0 on one workgroup when the BP state fits its LDS, else
+    // on the whole team; 1 always the team; 2 always one workgroup
+    int bp_layout;
+    // rotamer solve schedule: 0 the reference's (start from the node
+    // probabilities, stop when the one-sided change is at most the node's
+    // tol); 1 the same start, stop when the two-sided change over a chunk is
+    // at most bp_residual; 2 warm start (first pass, then the network's
+    // beliefs and bp_klocal local message updates), the residual rule, and
+    // after bp_budget iterations without meeting it a restart as schedule 1
+    int bp_mode;
+    float bp_residual;
+    int bp_budget;
+    int bp_klocal;
+    MlpDev mlp;            // warm-start network (bp_mode 2)
+    RexDev rex;            // replica exchange between rounds of this launch
 };
 
 constexpr int MAX_TEAM = 64;

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <set>
@@ -25,6 +26,7 @@
 #include <vector>
 
 #include "engine_dev.h"
+#include "warmstart.h"
 #include "../../include/upside_engine.h"
 
 hipError_t upside_launch_engine(upside::SysDesc* d_systems, int n_sys, upside::RunArgs args, hipStream_t stream);
@@ -423,6 +425,7 @@ struct Builder {
             o.ia[4] = (int)res.size();
             o.fa[0] = (float)sh[1] * (0.5f / 3.141592653589793f - 1e-7f);
             o.fa[1] = 3.141592653589793f;
+            o.ia[5] = h5.attr_i(g, "log_pot", 1, false);  // per-residue potential logged (src/rama_map_pot.cpp:34,50)
             o.scratch = T.falloc(res.size() * 3);
             vector<std::pair<int, int>> con;
             for (size_t e = 0; e < res.size(); ++e) con.push_back({res[e], (int)e});
@@ -1004,6 +1007,8 @@ struct Builder {
         R.ncur = T.falloc(nn * 6);
         R.nold = T.falloc(nn * 6);
         R.noff = T.falloc(nn);
+        R.feat = T.falloc(nn * MLP_IN);
+        R.nprev = T.falloc(nn * 6);
         R.stats = T.ialloc(BP_HIST + BP_HIST_BINS);
         o.ia[0] = T.ci_push(node_of);
         T.rots.push_back(R);
@@ -1462,9 +1467,14 @@ struct Batch {
         return s * s;
     }
     // temperature in effect after the thermostat of the round before round_num
+    // The temperature the reference's loop holds after round_num rounds: the
+    // one set at its last thermostat round nr (nr % thermostat_interval == 0),
+    // the schedule at time 3 dt (nr + 1) (src/main.cpp:974-977).
     void sync_temperature() {
-        if (!anneal || round_num == 0) return;
-        const float time = 3.f * dt * (float)round_num;
+        if (!anneal || round_num == 0 || thermostat_interval <= 0) return;
+        const uint64_t last = round_num - 1;
+        const uint64_t nr = last / (uint64_t)thermostat_interval * (uint64_t)thermostat_interval;
+        const float time = 3.f * dt * (float)(nr + 1);
         for (auto& s : sys) s.temperature = (float)anneal_temp((double)s.temp0, (double)time);
     }
     bool profile = false;
@@ -1647,8 +1657,15 @@ struct Batch {
     // Re-lower every program with larger list capacities, keeping positions,
     // momenta and thermostat counters.
     void grow() {
-        cap_scale *= 2.0;
-        if (cap_scale > 128.0) fail("neighbour list capacity growth limit reached");
+        if (cap_scale * 2.0 > 128.0) fail("neighbour list capacity growth limit reached");
+        relower(cap_scale * 2.0);
+    }
+
+    // Lower every program again with list capacities scaled by `scale` (larger
+    // or smaller) and reallocate the systems, keeping positions, momenta and
+    // thermostat counters.
+    void relower(double scale) {
+        cap_scale = scale;
         vector<vector<float>> pos(sys.size()), mom(sys.size());
         pull_desc();
         for (size_t k = 0; k < sys.size(); ++k) {
@@ -1673,6 +1690,19 @@ struct Batch {
     }
 
     int team = 1;   // workgroups per system (set_team / autotune)
+    int bp_layout = 0;  // where rotamer solves run (RunArgs::bp_layout)
+    // rotamer solve schedule (RunArgs::bp_mode and its parameters) and the
+    // warm-start network's device copy (schedule 2)
+    int bp_mode = 0;
+    float bp_residual = 1e-5f;
+    int bp_budget = 200;
+    int bp_klocal = 4;
+    std::unique_ptr<MlpParams> mlp;
+    // replica exchange inside the next run's launch (interval 0: none); the
+    // owner restores the exchange state before every launch attempt of the
+    // run (a retry after a list overflow repeats the attempts)
+    RexDev rex_dev{};
+    std::function<void()> rex_restore;
     bool team_local = false;  // the last team launch ran every team on one XCD
 
     // Largest team every system of the batch can have with all workgroups
@@ -1693,11 +1723,18 @@ struct Batch {
         team = t;
     }
 
-    // Pick the team size with the shortest run of n_round rounds: every
-    // candidate starts from the same snapshot (positions, momenta, thermostat
-    // counters), which is restored afterwards, so tuning leaves the state as
-    // it was.  ms_out[k] receives candidate k's kernel time (-1 if skipped).
-    void autotune(int n_round, const int* cand, int n_cand, float* ms_out) {
+    // Pick the team size and rotamer-solve layout with the shortest run of
+    // n_round rounds: every candidate pair starts from the same snapshot
+    // (positions, momenta, thermostat counters), which is restored
+    // afterwards, so tuning leaves the state as it was.
+    // ms_out[k * n_lay + l] receives the kernel time of team cand[k] with
+    // layout lay[l] (-1 if skipped).  lay == nullptr tunes the team alone.
+    void autotune(int n_round, const int* cand, int n_cand, float* ms_out, const int* lay = nullptr,
+                  int n_lay = 0) {
+        const int cur_layout = bp_layout;
+        if (!lay) { lay = &cur_layout; n_lay = 1; }
+        for (int l = 0; l < n_lay; ++l)
+            if (lay[l] < 0 || lay[l] > 2) fail("rotamer solve layout must be 0, 1 or 2");
         vector<vector<float>> pos(sys.size()), mom(sys.size());
         pull_desc();
         vector<SysDesc> saved = desc;
@@ -1717,24 +1754,34 @@ struct Batch {
             }
             push_desc();
         };
-        int best = team;
+        int best = team, best_lay = bp_layout;
         float best_ms = 3.0e38f;
         int limit = max_team();
-        for (int k = 0; k < n_cand; ++k) {
-            if (ms_out) ms_out[k] = -1.f;
-            if (cand[k] < 1 || cand[k] > limit) continue;
-            set_team(cand[k]);
-            run(n_round, 0, nullptr, 0, (int)round_num);
-            if (ms_out) ms_out[k] = last_ms;
-            if (last_ms < best_ms) { best_ms = last_ms; best = cand[k]; }
-            restore();
-        }
+        for (int k = 0; k < n_cand; ++k)
+            for (int l = 0; l < n_lay; ++l) {
+                if (ms_out) ms_out[k * n_lay + l] = -1.f;
+                if (cand[k] < 1 || cand[k] > limit) continue;
+                if (cand[k] == 1 && l > 0) continue;  // a team of one solves on its workgroup anyway
+                set_team(cand[k]);
+                bp_layout = lay[l];
+                run(n_round, 0, nullptr, 0, (int)round_num);
+                if (ms_out) ms_out[k * n_lay + l] = last_ms;
+                if (last_ms < best_ms) { best_ms = last_ms; best = cand[k]; best_lay = lay[l]; }
+                restore();
+            }
         set_team(best);
+        bp_layout = best_lay;
     }
 
     void launch(const RunArgs& a0) {
         RunArgs a = a0;
         a.team = team;
+        a.bp_layout = bp_layout;
+        a.bp_mode = bp_mode;
+        a.bp_residual = bp_residual;
+        a.bp_budget = bp_budget;
+        a.bp_klocal = bp_klocal;
+        if (mlp) a.mlp = mlp->dev;
         hip_check(hipEventRecord(ev0, stream), "event");
         hip_check(upside_launch_engine(d_desc, (int)sys.size(), a, stream), "engine launch");
         hip_check(hipEventRecord(ev1, stream), "event");
@@ -1796,8 +1843,19 @@ struct Batch {
         a.anneal_start = anneal_start;
         a.anneal_end = anneal_end;
         a.anneal_duration = anneal_duration;
+        a.rex = rex_dev;
+        a.rex.n_wg = (int)sys.size() * team;  // the workgroups that meet at its grid barriers
         for (;;) {
+            if (a.rex.interval > 0) {
+                rex_restore();
+                hip_check(hipMemset(a.rex.grid_bar, 0, 2 * sizeof(unsigned long long)), "grid barrier");
+            }
             launch(a);
+            if (a.rex.interval > 0) {
+                unsigned flag = 0;
+                hip_check(hipMemcpy(&flag, a.rex.grid_flag, sizeof(flag), hipMemcpyDeviceToHost), "grid barrier");
+                if (flag && !overflowed()) fail("replica exchange grid barrier timed out: results of this run are void");
+            }
             if (!overflowed()) break;
             grow();
             for (size_t k = 0; k < sys.size(); ++k) {
@@ -1888,19 +1946,36 @@ struct Batch {
         if (h.n_sys != (int)sys.size())
             fail("load_state: state of " + std::to_string(h.n_sys) + " systems, batch of " +
                  std::to_string(sys.size()));
-        if (h.cap_scale < cap_scale) fail("load_state: state has smaller list capacities than this batch");
+        if (!(h.cap_scale > 0.0 && h.cap_scale <= 128.0)) fail("load_state: list capacity scale out of range");
         hip_check(hipStreamSynchronize(stream), "sync");
-        while (cap_scale < h.cap_scale) grow();
-        if (size < state_size()) fail("load_state: truncated state");
-        const char* p = in + sizeof(h);
-        vector<SysState> st(sys.size());
-        for (size_t k = 0; k < sys.size(); ++k) {  // validate everything before touching the device
-            std::memcpy(&st[k], p, sizeof(SysState));
-            const Topology& T = sys[k].prog->T;
-            if (st[k].nf != std::max(T.nf, 4) || st[k].ni != std::max(T.ni, 4))
-                fail("load_state: system " + std::to_string(k) + " was saved from a different .up file");
-            p += sizeof(SysState) + st[k].nf * 4 + st[k].ni * 4;
+        // validate everything before touching the batch: the arena sizes the
+        // programs have when lowered at the saved capacity scale
+        std::map<const Program*, std::array<int, 2>> sizes;
+        for (auto& P : progs) {
+            if (h.cap_scale == cap_scale) {
+                sizes[P.get()] = {std::max(P->T.nf, 4), std::max(P->T.ni, 4)};
+            } else {
+                Topology t;
+                t.cap_scale = h.cap_scale;
+                H5 h5(P->path);
+                load_topology(t, h5, P->T.n_atom);
+                sizes[P.get()] = {std::max(t.nf, 4), std::max(t.ni, 4)};
+            }
         }
+        vector<SysState> st(sys.size());
+        const char* p = in + sizeof(h);
+        size_t need = sizeof(h);
+        for (size_t k = 0; k < sys.size(); ++k) {
+            if (size < need + sizeof(SysState)) fail("load_state: truncated state");
+            std::memcpy(&st[k], p, sizeof(SysState));
+            const auto& z = sizes[sys[k].prog.get()];
+            if (st[k].nf != z[0] || st[k].ni != z[1])
+                fail("load_state: system " + std::to_string(k) + " was saved from a different .up file");
+            need += sizeof(SysState) + (size_t)st[k].nf * 4 + (size_t)st[k].ni * 4;
+            p += sizeof(SysState) + (size_t)st[k].nf * 4 + (size_t)st[k].ni * 4;
+        }
+        if (size < need) fail("load_state: truncated state");
+        if (h.cap_scale != cap_scale) relower(h.cap_scale);  // larger or smaller, as saved
         p = in + sizeof(h);
         pull_desc();
         for (size_t k = 0; k < sys.size(); ++k) {
@@ -2007,6 +2082,19 @@ struct OutputLog {
     int rot_op = -1, rot_nn = 0, rot_nprob = 0;
     vector<hid_t> rot_sets;
     vector<vector<float>> rot_rows;   // filled by the caller before frame()
+    // per-node values of the detailed and extensive levels, one dataset each:
+    // NL_RAMA rama_map_potential (src/rama_map_pot.cpp:49-55), NL_SIGMOID
+    // sigmoid_coupling (src/environment.cpp:892-900), NL_OUTPUT the node's
+    // output (environment_coverage, hbond_environment_coverage,
+    // src/environment.cpp:304-310, :355-360), NL_VIRTUAL the first three
+    // columns of infer_H_O's rows (virtual, src/hbond.cpp:53-58)
+    enum { NL_RAMA, NL_SIGMOID, NL_OUTPUT, NL_VIRTUAL };
+    struct NodeLog {
+        hid_t d;
+        int kind, node, n;
+        vector<float> row;   // filled by the caller before frame()
+    };
+    vector<NodeLog> node_logs;
     long long n = 0;
 
     static hid_t earray(hid_t g, const char* name, hid_t type, const vector<hsize_t>& row) {
@@ -2091,6 +2179,31 @@ struct OutputLog {
                     for (auto& nm : names) rot_sets.push_back(earray(group, nm.c_str(), H5T_NATIVE_FLOAT, {(hsize_t)rot_nn}));
                 rot_rows.resize(rot_sets.size());
             }
+            // a dataset name is created once: the reference would fail on a
+            // second node logging under the same name, here the first in
+            // load order is logged
+            std::set<string> made;
+            auto add = [&](const char* name, int kind, int node, int n, vector<hsize_t> row) {
+                if (made.count(name)) return;
+                made.insert(name);
+                node_logs.push_back({earray(group, name, H5T_NATIVE_FLOAT, row), kind, node, n, {}});
+            };
+            for (size_t k = 0; k < T.nodes.size(); ++k) {
+                const Node& nd = T.nodes[k];
+                if (nd.op < 0) continue;
+                const OpRec& o = T.ops[nd.op];
+                if (o.kind == OP_RAMA_MAP_POT && o.ia[5])
+                    add("rama_map_potential", NL_RAMA, (int)k, o.ia[4], {(hsize_t)o.ia[4]});
+                if (o.kind == OP_SIGMOID_COUPLING) add("sigmoid_coupling", NL_SIGMOID, (int)k, o.ia[0], {(hsize_t)o.ia[0]});
+                if (level >= 2) {
+                    if (o.kind == OP_ENV_COVERAGE)
+                        add("environment_coverage", NL_OUTPUT, (int)k, nd.n_elem * nd.width, {(hsize_t)(nd.n_elem * nd.width)});
+                    if (o.kind == OP_HB_ENV_COVERAGE)
+                        add("hbond_environment_coverage", NL_OUTPUT, (int)k, nd.n_elem * nd.width,
+                            {(hsize_t)(nd.n_elem * nd.width)});
+                    if (o.kind == OP_INFER_HO) add("virtual", NL_VIRTUAL, (int)k, nd.n_elem * 3, {(hsize_t)nd.n_elem, 3});
+                }
+            }
         }
         // the reference records its command line here (src/main.cpp:734-736)
         const char* inv = "libupside batch (upside_batch_run_logged)";
@@ -2118,6 +2231,7 @@ struct OutputLog {
         if (hbond >= 0) append(hbond, H5T_NATIVE_FLOAT, hb_values.data(), n);
         if (bad_solves >= 0) append(bad_solves, H5T_NATIVE_LLONG, &n_bad, n);
         for (size_t q = 0; q < rot_sets.size(); ++q) append(rot_sets[q], H5T_NATIVE_FLOAT, rot_rows[q].data(), n);
+        for (auto& nl : node_logs) append(nl.d, H5T_NATIVE_FLOAT, nl.row.data(), n);
         if (replica_index >= 0) append(replica_index, H5T_NATIVE_INT, &rep_index, n);
         if (replica_swaps >= 0) append(replica_swaps, H5T_NATIVE_INT, rep_swaps.data(), n);
         ++n;
@@ -2155,6 +2269,101 @@ struct UpsideBatch {
         uint64_t e = after + 2;
         return (e + ri - 1) / ri * ri;
     }
+    // device copies of the exchange configuration and state for runs that
+    // exchange inside the launch (one allocation, rebuilt with the sets)
+    char* rex_mem = nullptr;
+    size_t rex_mem_size = 0;
+    RexDev rex_dev{};
+    ~UpsideBatch() { (void)hipFree(rex_mem); }
+
+    // The exchange can run inside the launch when every system evaluates the
+    // same potential (one lowered program), temperatures are fixed (no
+    // annealing) and every workgroup of the launch can be resident at once.
+    bool device_rex_ok() const {
+        if (rex_interval <= 0 || batch.anneal) return false;
+        for (auto& sy : batch.sys)
+            if (sy.prog != batch.sys[0].prog) return false;
+        return (long long)batch.sys.size() * batch.team <= (long long)upside_max_resident_blocks();
+    }
+
+    void rex_alloc() {
+        const int ns = (int)batch.sys.size();
+        int np = 0;
+        for (auto& st : rex_sets) np += (int)st.size();
+        const int stride = (batch.sys[0].prog->T.n_atom * 3 + 63) / 64 * 64;
+        auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+        const size_t sizes[] = {(rex_sets.size() + 1) * 4, (size_t)2 * np * 4, 2 * 8, (size_t)2 * np * 8,
+                                (size_t)ns * 4, (size_t)ns * 4, (size_t)ns * 4, (size_t)ns * stride * 4,
+                                (size_t)ns * 4, 4 * 8};
+        size_t total = 0;
+        for (size_t b : sizes) total += up(b);
+        if (total != rex_mem_size) {
+            (void)hipFree(rex_mem);
+            rex_mem = nullptr;
+            hip_check(hipMalloc((void**)&rex_mem, total), "replica exchange state");
+            rex_mem_size = total;
+        }
+        char* p = rex_mem;
+        auto take = [&](size_t b) { char* q = p; p += up(b); return q; };
+        rex_dev.set_off = (const int*)take(sizes[0]);
+        rex_dev.pairs = (const int*)take(sizes[1]);
+        rex_dev.state = (long long*)take(sizes[2]);
+        rex_dev.stats = (long long*)take(sizes[3]);
+        rex_dev.replica_of = (int*)take(sizes[4]);
+        rex_dev.beta = (const float*)take(sizes[5]);
+        rex_dev.energy = (float*)take(sizes[6]);
+        rex_dev.stage = (float*)take(sizes[7]);
+        rex_dev.partner = (int*)take(sizes[8]);
+        char* g = take(sizes[9]);
+        rex_dev.grid_bar = (unsigned long long*)g;
+        rex_dev.grid_flag = (unsigned*)(g + 16);
+        rex_dev.stage_stride = stride;
+        rex_dev.interval = rex_interval;
+        rex_dev.criterion = rex_criterion;
+        rex_dev.n_sets = (int)rex_sets.size();
+        rex_dev.seed = batch.base_seed;
+        vector<int> off{0}, pr;
+        for (auto& st : rex_sets) {
+            for (auto& q : st) { pr.push_back(q[0]); pr.push_back(q[1]); }
+            off.push_back((int)pr.size() / 2);
+        }
+        hip_check(hipMemcpy((void*)rex_dev.set_off, off.data(), off.size() * 4, hipMemcpyHostToDevice), "rex");
+        if (!pr.empty()) hip_check(hipMemcpy((void*)rex_dev.pairs, pr.data(), pr.size() * 4, hipMemcpyHostToDevice), "rex");
+    }
+
+    // host state -> device (before every launch attempt of a run)
+    void rex_upload() {
+        const int ns = (int)batch.sys.size();
+        long long st[2] = {rex_attempts, (long long)rex_next};
+        vector<long long> stats;
+        for (auto& v : rex_stats)
+            for (auto& q : v) { stats.push_back(q[0]); stats.push_back(q[1]); }
+        vector<float> beta(ns);
+        for (int k = 0; k < ns; ++k) beta[k] = 1.f / batch.sys[k].temperature;
+        hip_check(hipMemcpy(rex_dev.state, st, sizeof(st), hipMemcpyHostToDevice), "rex");
+        if (!stats.empty()) hip_check(hipMemcpy(rex_dev.stats, stats.data(), stats.size() * 8, hipMemcpyHostToDevice), "rex");
+        hip_check(hipMemcpy(rex_dev.replica_of, replica_of.data(), ns * 4, hipMemcpyHostToDevice), "rex");
+        hip_check(hipMemcpy((void*)rex_dev.beta, beta.data(), ns * 4, hipMemcpyHostToDevice), "rex");
+        hip_check(hipMemset(rex_dev.grid_flag, 0, sizeof(unsigned)), "rex");
+    }
+
+    // device state -> host (after a run)
+    void rex_download() {
+        const int ns = (int)batch.sys.size();
+        long long st[2];
+        hip_check(hipMemcpy(st, rex_dev.state, sizeof(st), hipMemcpyDeviceToHost), "rex");
+        rex_attempts = st[0];
+        rex_next = (uint64_t)st[1];
+        size_t np = 0;
+        for (auto& v : rex_stats) np += v.size();
+        vector<long long> stats(2 * np);
+        if (np) hip_check(hipMemcpy(stats.data(), rex_dev.stats, stats.size() * 8, hipMemcpyDeviceToHost), "rex");
+        size_t k = 0;
+        for (auto& v : rex_stats)
+            for (auto& q : v) { q[0] = stats[k]; q[1] = stats[k + 1]; k += 2; }
+        hip_check(hipMemcpy(replica_of.data(), rex_dev.replica_of, ns * 4, hipMemcpyDeviceToHost), "rex");
+    }
+
     vector<std::array<long long, 2>> mc_total;  // accepted, attempted since set_monte_carlo
     vector<std::array<int, 2>> mc_frame;        // the same since the last logged frame
 
@@ -2258,6 +2467,41 @@ struct UpsideBatch {
                 for (int q = 0; q < o.rot_nprob; ++q) {
                     o.rot_rows[3 + q].resize(o.rot_nn);
                     for (int x = 0; x < o.rot_nn; ++x) o.rot_rows[3 + q][x] = one[(size_t)x * o.rot_nprob + q];
+                }
+            }
+            for (auto& nl : o.node_logs) {
+                const Node& nd = T.nodes[nl.node];
+                const OpRec& op = T.ops[nd.op];
+                nl.row.assign(nl.n, 0.f);
+                float* A = batch.sys[k].arena;
+                if (nl.kind == OutputLog::NL_RAMA) {  // per-residue potentials the kernel left after its values
+                    hip_check(hipMemcpy(nl.row.data(), A + op.scratch + 2 * nl.n, nl.n * 4, hipMemcpyDeviceToHost),
+                              "rama log");
+                } else if (nl.kind == OutputLog::NL_SIGMOID) {
+                    // scale * compact_sigmoid(sum_a w[t][a] coverage[r][a] - center[t], sharpness[t])
+                    const int nr = op.ia[0], nt = op.ia[6];
+                    vector<float> x((size_t)nr * nt);
+                    hip_check(hipMemcpy(x.data(), A + op.in_out[0], x.size() * 4, hipMemcpyDeviceToHost), "coupling log");
+                    const int* ct = T.ci.data() + op.ia[1];
+                    const float *W = T.cf.data() + op.ia[2], *sc = T.cf.data() + op.ia[3];
+                    const float *ce = T.cf.data() + op.ia[4], *sh = T.cf.data() + op.ia[5];
+                    for (int r = 0; r < nr; ++r) {
+                        const int t = ct[r];
+                        float wn = 0.f;
+                        for (int a = 0; a < nt; ++a) wn += W[t * nt + a] * x[(size_t)r * nt + a];
+                        const float y = (wn - ce[t]) * sh[t];
+                        float v = 0.25f * (y + 2.f) * (y - 1.f) * (y - 1.f);
+                        if (y < -1.f) v = 1.f;
+                        if (y > 1.f) v = 0.f;
+                        nl.row[r] = v * sc[t];
+                    }
+                } else if (nl.kind == OutputLog::NL_OUTPUT) {
+                    hip_check(hipMemcpy(nl.row.data(), A + nd.out, nl.n * 4, hipMemcpyDeviceToHost), "node log");
+                } else {
+                    vector<float> rows((size_t)nd.n_elem * nd.width);
+                    hip_check(hipMemcpy(rows.data(), A + nd.out, rows.size() * 4, hipMemcpyDeviceToHost), "virtual log");
+                    for (int v = 0; v < nd.n_elem; ++v)
+                        for (int d = 0; d < 3; ++d) nl.row[(size_t)v * 3 + d] = rows[(size_t)v * nd.width + d];
                 }
             }
             if (o.replica_index >= 0) {
@@ -3092,6 +3336,18 @@ int upside_batch_run_logged(UpsideBatch* b, int n_round, int frame_interval) {
     // 0), frame, then thermostat and integration in the launch; replica
     // exchange where a segment of the reference loop ends (:991-995), with
     // the swap sets taken in turn (:260-263)
+    // with one shared potential and fixed temperatures the exchange runs
+    // inside the launch: a span between frames and pivot moves is one launch
+    const bool dev_rex = b->device_rex_ok();
+    if (dev_rex) {
+        b->rex_alloc();
+        B.rex_dev = b->rex_dev;
+        B.rex_restore = [b]() { b->rex_upload(); };
+    }
+    struct Clear {
+        Batch& B;
+        ~Clear() { B.rex_dev = RexDev{}; B.rex_restore = nullptr; }
+    } clear{B};
     while (done < n_round) {
         const uint64_t r = B.round_num;
         if (mc && r > 0 && r % mc == 0) b->mc_step(r);
@@ -3099,12 +3355,13 @@ int upside_batch_run_logged(UpsideBatch* b, int n_round, int frame_interval) {
         int n = n_round - done;
         if (mc) n = std::min<int64_t>(n, (int64_t)(mc - r % mc));
         if (frames) n = std::min<int64_t>(n, (int64_t)(fi - r % fi));
-        if (b->rex_interval > 0 && b->rex_next > r) n = std::min<int64_t>(n, (int64_t)(b->rex_next - r));
+        if (!dev_rex && b->rex_interval > 0 && b->rex_next > r) n = std::min<int64_t>(n, (int64_t)(b->rex_next - r));
         B.run(n, 0, nullptr, 0, (int)B.round_num);
+        if (dev_rex) b->rex_download();
         B.round_num += n;
         B.sync_temperature();
         done += n;
-        if (b->rex_interval > 0 && B.round_num == b->rex_next) {
+        if (!dev_rex && b->rex_interval > 0 && B.round_num == b->rex_next) {
             const int set = (int)(b->rex_attempts % (long long)b->rex_sets.size());
             const auto& S = b->rex_sets[set];
             vector<int> pr, acc(S.size());
@@ -3251,6 +3508,17 @@ int upside_batch_energies(UpsideBatch* b, float* energies) {
     ABI_TRY
     b->batch.evaluate(1);
     for (size_t k = 0; k < b->batch.sys.size(); ++k) energies[k] = b->batch.potential((int)k);
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_get_deriv(UpsideBatch* b, int sys, float* deriv) {
+    ABI_TRY
+    auto& B = b->batch;
+    if (sys < 0 || sys >= (int)B.sys.size()) fail("get_deriv: system out of range");
+    const Topology& T = B.sys[sys].prog->T;
+    hip_check(hipMemcpy(deriv, B.sys[sys].arena + T.nodes[0].sens, (size_t)T.n_atom * 3 * sizeof(float),
+                        hipMemcpyDeviceToHost), "deriv");
     return 0;
     ABI_CATCH(1)
 }
@@ -3420,6 +3688,96 @@ int upside_batch_get_team(UpsideBatch* b) { return b ? b->batch.team : -1; }
 int upside_batch_autotune(UpsideBatch* b, int n_round, const int* candidates, int n_candidates, float* ms_out) {
     ABI_TRY
     b->batch.autotune(n_round, candidates, n_candidates, ms_out);
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_autotune_layout(UpsideBatch* b, int n_round, const int* teams, int n_teams, const int* layouts,
+                                 int n_layouts, float* ms_out) {
+    ABI_TRY
+    if (!layouts || n_layouts < 1) throw std::runtime_error("no rotamer solve layouts to tune");
+    b->batch.autotune(n_round, teams, n_teams, ms_out, layouts, n_layouts);
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_set_bp_layout(UpsideBatch* b, int layout) {
+    ABI_TRY
+    if (layout < 0 || layout > 2) throw std::runtime_error("rotamer solve layout must be 0, 1 or 2");
+    b->batch.bp_layout = layout;
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_get_bp_layout(UpsideBatch* b) { return b ? b->batch.bp_layout : -1; }
+
+int upside_batch_set_bp_schedule(UpsideBatch* b, int mode, float residual, int budget, int klocal) {
+    ABI_TRY
+    auto& B = b->batch;
+    if (mode < 0 || mode > 2) fail("rotamer solve schedule must be 0, 1 or 2");
+    if (mode > 0 && !(residual > 0.f && residual < 1.f)) fail("residual must be in (0, 1)");
+    if (mode == 2 && !B.mlp) fail("schedule 2 needs the warm-start network (upside_batch_set_warm_start)");
+    if (budget < 2 || klocal < 0 || klocal > 64) fail("budget must be at least 2 iterations and klocal in [0, 64]");
+    B.bp_mode = mode;
+    B.bp_residual = residual;
+    B.bp_budget = budget;
+    B.bp_klocal = klocal;
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_set_warm_start(UpsideBatch* b, const float* params, int n_param) {
+    ABI_TRY
+    auto& B = b->batch;
+    if (n_param != MLP_NPARAM) fail("warm-start parameters: " + std::to_string(MLP_NPARAM) + " floats expected");
+    if (!B.mlp) B.mlp.reset(new MlpParams());
+    B.mlp->set(params, B.stream);
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_bp_training_rows(UpsideBatch* b, int sys, float* features, float* targets, int max_rows) {
+    ABI_TRY
+    auto& B = b->batch;
+    if (sys < 0 || sys >= (int)B.sys.size()) fail("bp_training_rows: system out of range");
+    const Topology& T = B.sys[sys].prog->T;
+    int n = 0;
+    for (auto& R : T.rots) {
+        const int nn = R.n_node;
+        const int* nrot = T.ci.data() + R.node_nrot;
+        vector<float> f((size_t)nn * MLP_IN), cur((size_t)nn * 6);
+        hip_check(hipMemcpy(f.data(), B.sys[sys].arena + R.feat, f.size() * 4, hipMemcpyDeviceToHost), "rows");
+        hip_check(hipMemcpy(cur.data(), B.sys[sys].arena + R.ncur, cur.size() * 4, hipMemcpyDeviceToHost), "rows");
+        for (int x = 0; x < nn; ++x) {
+            if (nrot[x] < 2) continue;
+            if (features && targets) {
+                if (n >= max_rows) fail("bp_training_rows: more rows than max_rows");
+                std::memcpy(features + (size_t)n * MLP_IN, f.data() + (size_t)x * MLP_IN, MLP_IN * 4);
+                for (int r = 0; r < 8; ++r) targets[(size_t)n * 8 + r] = r < 6 ? cur[(size_t)x * 6 + r] : 0.f;
+            }
+            ++n;
+        }
+    }
+    return n;
+    ABI_CATCH(-1)
+}
+
+int upside_batch_bp_warm_stats(UpsideBatch* b, int sys, long long* warm, long long* fallbacks,
+                               long long* residual_solves) {
+    ABI_TRY
+    auto& B = b->batch;
+    if (sys < 0 || sys >= (int)B.sys.size()) fail("bp_warm_stats: system out of range");
+    long long w = 0, f = 0, r = 0;
+    for (auto& R : B.sys[sys].prog->T.rots) {
+        int st[10];
+        hip_check(hipMemcpy(st, B.sys[sys].iarena + R.stats, sizeof(st), hipMemcpyDeviceToHost), "stats");
+        w += st[7];
+        f += st[8];
+        r += st[9];
+    }
+    if (warm) *warm = w;
+    if (fallbacks) *fallbacks = f;
+    if (residual_solves) *residual_solves = r;
     return 0;
     ABI_CATCH(1)
 }

@@ -853,7 +853,7 @@ __device__ void rotamer_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const Pai
                             const RotamerDev& R, int mode) {
     if (!rotamer_pre(c, S, op, G, R)) return;
     unsigned long long t0 = ptick(c);
-    int it = rot_solve(c, R);
+    int it = rot_solve(c, R, c.bp_layout);
     pacc(c, PROF_BP, t0);
     rotamer_post(c, S, op, G, R, mode, it);
 }
@@ -959,6 +959,18 @@ __device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mod
             const RotamerDev& R = S.rots[op.rot];
             if (!rotamer_pre(c, S, op, G, R)) return false;
             int it = 0;
+            if (!bp_on_one_wg(c, R, c.I[R.stats + 4], c.bp_layout)) {
+                // the state does not fit one workgroup's LDS: the whole team
+                // solves, and the independent ops follow in program order
+                unsigned long long t1 = ptick(c);
+                it = rot_solve_team(c, R);
+                pacc(c, PROF_BP, t1);
+                rotamer_post(c, S, op, G, R, mode, it);
+                prof_op(c, k, t0);
+                if (c.I[S.error]) return false;
+                ++k;
+                continue;
+            }
             if (c.rank == 0) {
                 unsigned long long t1 = ptick(c);
                 it = rot_solve_wg(c, R);
@@ -1092,6 +1104,98 @@ __device__ float total_potential(Ctx& c, const SysDesc& S) {
     return t;
 }
 
+// Barrier over every workgroup of the launch that takes part in replica
+// exchange (teams on different XCDs: device-scope fences on both sides), on a
+// monotonic arrival counter as team_sync.  Bounded: a timeout sets the flag,
+// which the host turns into an error, and no later wait blocks.
+__device__ void grid_sync(Ctx& c, const RexDev& x) {
+    __syncthreads();
+    if (c.tid == 0) {
+        __threadfence();
+        unsigned long long old = __hip_atomic_fetch_add(x.grid_bar, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned long long target = (old / (unsigned long long)x.n_wg + 1ull) * (unsigned long long)x.n_wg;
+        unsigned spins = 0;
+        while (__hip_atomic_load(x.grid_bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++spins > BAR_SPIN_LIMIT ||
+                __hip_atomic_load(x.grid_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                __hip_atomic_store(x.grid_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+// One replica-exchange attempt after `done` completed rounds (see RexDev):
+// all teams of the launch.  The decisions follow attempt_swaps
+// (src/main.cpp:243-316): criterion 0 keeps a swap unless the change of the
+// summed log Boltzmann weights is negative and exp(change) falls below a
+// uniform draw; criterion 1 swaps when the change lb[s2] - lb[s1] is
+// non-negative or exp(change) reaches the draw; draws (0, 1] from Threefry
+// stream 1, counter (done, 0, k), one per test that needs it.  With one
+// shared potential the weights after a swap are those of the partner's
+// positions, -beta_s E(x_partner), evaluated once per system.
+__device__ void rex_attempt(Ctx& c, SysDesc& S, const RexDev& x, int sys, int n_sys, long long done) {
+    compute(c, S, 1);
+    if (c.gtid == 0) x.energy[sys] = total_potential(c, S);
+    {
+        auto pos = gmem(c.A + S.pos_out);
+        auto st = gmem(x.stage + (size_t)sys * x.stage_stride);
+        for (int i = c.gtid; i < S.n_atom * 3; i += c.gsize) st[i] = pos[i];
+    }
+    grid_sync(c, x);
+    if (sys == 0 && c.rank == 0 && c.tid == 0) {
+        const long long att = x.state[0];
+        const int set = (int)(att % (long long)x.n_sets);
+        for (int k = 0; k < n_sys; ++k) x.partner[k] = -1;
+        uint32_t key[4] = {x.seed, 1u, 0u, 0u};
+        uint32_t ctr[4] = {(uint32_t)((unsigned long long)done & 0xffffffffull), (uint32_t)((unsigned long long)done >> 32), 0u, 0u};
+        for (int p = x.set_off[set]; p < x.set_off[set + 1]; ++p) {
+            const int s1 = x.pairs[2 * p], s2 = x.pairs[2 * p + 1];
+            const float e1 = x.energy[s1], e2 = x.energy[s2];
+            const float b1 = x.beta[s1], b2 = x.beta[s2];
+            bool swap;
+            auto uniform = [&]() {
+                uint32_t bits[4];
+                threefry4x32_20(ctr, key, bits);
+                ctr[3]++;
+                return (float)bits[0] * 2.3283064365386962890625e-10f + 1.16415321826934814453125e-10f;
+            };
+            if (x.criterion == 0) {
+                const float diff = ((-b1 * e2) + (-b2 * e1)) - ((-b1 * e1) + (-b2 * e2));
+                swap = !(diff < 0.f && expf(diff) < uniform());
+            } else {
+                const float diff = (-b2 * e2) - (-b1 * e1);
+                swap = diff >= 0.f || expf(diff) >= uniform();
+            }
+            x.stats[2 * p + 1] += 1;
+            if (swap) {
+                x.stats[2 * p] += 1;
+                x.partner[s1] = s2;
+                x.partner[s2] = s1;
+                const int t = x.replica_of[s1];
+                x.replica_of[s1] = x.replica_of[s2];
+                x.replica_of[s2] = t;
+            }
+        }
+        x.state[0] = att + 1;
+        // the next attempt ends the next segment of the reference's loop: the
+        // first multiple of the interval at least two rounds on (:918-990)
+        const long long e = done + 2;
+        x.state[1] = (e + x.interval - 1) / x.interval * x.interval;
+    }
+    grid_sync(c, x);
+    const int q = x.partner[sys];
+    if (q >= 0) {
+        auto pos = gmem(c.A + S.pos_out);
+        auto st = gmem((const float*)x.stage + (size_t)q * x.stage_stride);
+        for (int i = c.gtid; i < S.n_atom * 3; i += c.gsize) pos[i] = st[i];
+    }
+    team_sync(c);
+}
+
 // One launch evaluates or integrates every system of a batch; system s is
 // handled by the team of workgroups [s*team, (s+1)*team).  The body is shared
 // by two kernel symbols so that the integration launches (the hot path) and
@@ -1123,6 +1227,8 @@ __device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& arg
     c.bar2 = reinterpret_cast<unsigned long long*>(S.iarena + S.bar + 4 + MAX_TEAM);
     c.tred2 = c.tred + 2 * MAX_TEAM;
     c.tpar2 = 0;
+    c.bp_layout = args.bp_layout;
+    c.args = &args;
     team_place(c, S.iarena + S.bar + 4);
     if (args.n_round == 0) {
         compute(c, S, args.mode);
@@ -1168,6 +1274,13 @@ __device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& arg
             }
             team_sync(c);
             pacc(c, PROF_INTEGRATE, t0);
+        }
+        // replica exchange where a segment of the reference's loop ends
+        // (src/main.cpp:991-997): after this round the count of completed
+        // rounds is round + 1
+        if (args.rex.interval > 0 && (long long)round + 1 == args.rex.state[1]) {
+            rex_attempt(c, S, args.rex, sys, args.n_sys, (long long)round + 1);
+            if (c.I[S.error]) return;
         }
     }
 }
@@ -1422,8 +1535,11 @@ hipError_t upside_launch_engine(upside::SysDesc* d_systems, int n_sys, upside::R
     if (args.team <= 1) args.team = 1;
     args.n_sys = n_sys;
     void* kargs[] = {&d_systems, &args};
-    if (args.team == 1)
+    if (args.team == 1) {
+        if (args.rex.interval > 0)  // the grid barriers need every workgroup resident
+            return hipLaunchCooperativeKernel(fn, dim3(n_sys), dim3(upside::NT), kargs, (unsigned)lds_bytes, stream);
         return hipLaunchKernel(fn, dim3(n_sys), dim3(upside::NT), kargs, lds_bytes, stream);
+    }
     // teams wait for each other: every workgroup must be resident at once,
     // which a cooperative launch guarantees (or refuses).  The grid covers
     // N_XCD slots per team rank; slots past the last system exit at once.

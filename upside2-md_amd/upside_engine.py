@@ -59,6 +59,7 @@ def load_upside_library(path=LIB_PATH):
         "upside_batch_get_mom": (i, [vp, i, vp]),
         "upside_batch_run": (i, [vp, i, i, vp, i]),
         "upside_batch_energies": (i, [vp, vp]),
+        "upside_batch_get_deriv": (i, [vp, i, vp]),
         "upside_batch_replica_exchange": (i, [vp, i, vp, ct.c_uint64, vp]),
         "upside_batch_bp_stats": (i, [vp, i, vp, vp, vp, vp]),
         "upside_batch_potential_terms": (i, [vp, i, vp, i, ct.c_char_p, i]),
@@ -84,7 +85,23 @@ def load_upside_library(path=LIB_PATH):
         "upside_batch_set_team": (i, [vp, i]),
         "upside_batch_get_team": (i, [vp]),
         "upside_batch_autotune": (i, [vp, i, vp, i, vp]),
+        "upside_batch_autotune_layout": (i, [vp, i, vp, i, vp, i, vp]),
+        "upside_batch_set_bp_layout": (i, [vp, i]),
+        "upside_batch_get_bp_layout": (i, [vp]),
         "upside_batch_profile_report": (i, [vp, i, ct.c_char_p, i]),
+        "upside_batch_set_bp_schedule": (i, [vp, i, f, i, i]),
+        "upside_batch_set_warm_start": (i, [vp, vp, i]),
+        "upside_batch_bp_training_rows": (i, [vp, i, vp, vp, i]),
+        "upside_batch_bp_warm_stats": (i, [vp, i, vp, vp, vp]),
+        "upside_mlp_create": (vp, [ct.c_uint32]),
+        "upside_mlp_free": (None, [vp]),
+        "upside_mlp_n_param": (i, []),
+        "upside_mlp_get_params": (i, [vp, vp, i]),
+        "upside_mlp_set_params": (i, [vp, vp, i]),
+        "upside_mlp_set_data": (i, [vp, vp, vp, i]),
+        "upside_mlp_loss_grad": (i, [vp, vp, vp]),
+        "upside_mlp_train": (i, [vp, i, i, f, vp]),
+        "upside_mlp_forward": (i, [vp, vp, i, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(c, name)
@@ -234,6 +251,13 @@ class Batch:
             raise RuntimeError("batch run failed")
         return frames
 
+    def deriv(self, s):
+        """dE/dx of system s at the last evaluation (call energies() first)."""
+        d = np.zeros((self.calc.upside_batch_n_atom(self.h, s), 3), "f4")
+        if self.calc.upside_batch_get_deriv(self.h, s, d.ctypes.data):
+            raise RuntimeError("get_deriv failed")
+        return d
+
     def energies(self):
         e = np.zeros(self.n_sys, "f4")
         if self.calc.upside_batch_energies(self.h, e.ctypes.data):
@@ -379,6 +403,55 @@ class Batch:
             raise RuntimeError("autotune failed")
         return {int(t): float(m) for t, m in zip(cand, ms) if m >= 0}
 
+    def autotune_layout(self, n_round=3, teams=(1, 2, 4, 8, 16, 32), layouts=(0, 1)):
+        """Time n_round rounds per (team size, rotamer-solve layout) pair from
+        the current state, keep the fastest; returns {(team, layout): ms}."""
+        t = np.asarray(teams, dtype=np.intc)
+        lay = np.asarray(layouts, dtype=np.intc)
+        ms = np.zeros(len(t) * len(lay), dtype="f4")
+        if self.calc.upside_batch_autotune_layout(self.h, int(n_round), t.ctypes.data, len(t), lay.ctypes.data,
+                                                  len(lay), ms.ctypes.data):
+            raise RuntimeError("autotune failed")
+        return {(int(a), int(b)): float(ms[k * len(lay) + l]) for k, a in enumerate(t) for l, b in enumerate(lay)
+                if ms[k * len(lay) + l] >= 0}
+
+    def bp_layout(self):
+        return int(self.calc.upside_batch_get_bp_layout(self.h))
+
+    def set_bp_layout(self, layout):
+        if self.calc.upside_batch_set_bp_layout(self.h, int(layout)):
+            raise RuntimeError("rotamer solve layout %d not possible" % layout)
+
+    def set_bp_schedule(self, mode, residual=1e-5, budget=200, klocal=4):
+        """Rotamer BP schedule: 0 the reference's, 1 plain start with the
+        residual stopping rule, 2 warm start (needs set_warm_start) with the
+        residual rule and a plain restart after `budget` iterations."""
+        if self.calc.upside_batch_set_bp_schedule(self.h, int(mode), float(residual), int(budget), int(klocal)):
+            raise RuntimeError("set_bp_schedule failed")
+
+    def set_warm_start(self, params):
+        p = _f4(params)
+        if self.calc.upside_batch_set_warm_start(self.h, p.ctypes.data, len(p)):
+            raise RuntimeError("set_warm_start failed")
+
+    def bp_training_rows(self, s):
+        """Features and converged marginals of system s's last solve (schedule
+        1 or 2): (features (n, 32), targets (n, 8))."""
+        n = self.calc.upside_batch_bp_training_rows(self.h, s, None, None, 0)
+        if n < 0:
+            raise RuntimeError("bp_training_rows failed")
+        f = np.zeros((n, 32), "f4")
+        t = np.zeros((n, 8), "f4")
+        if self.calc.upside_batch_bp_training_rows(self.h, s, f.ctypes.data, t.ctypes.data, n) != n:
+            raise RuntimeError("bp_training_rows failed")
+        return f, t
+
+    def bp_warm_stats(self, s):
+        w, fb, r = ct.c_longlong(0), ct.c_longlong(0), ct.c_longlong(0)
+        if self.calc.upside_batch_bp_warm_stats(self.h, s, ct.byref(w), ct.byref(fb), ct.byref(r)):
+            raise RuntimeError("bp_warm_stats failed")
+        return dict(warm=w.value, fallbacks=fb.value, residual=r.value)
+
     def enable_profile(self, on=True):
         if self.calc.upside_batch_enable_profile(self.h, int(on)):
             raise RuntimeError("enable_profile failed")
@@ -393,4 +466,63 @@ class Batch:
     def __del__(self):
         if getattr(self, "h", None):
             self.calc.upside_batch_free(self.h)
+            self.h = None
+
+
+class WarmStartNet:
+    """The BP warm-start network (no reference counterpart): parameters,
+    training data and Adam state on the GPU; every product on the matrix
+    cores (bf16 operands, float32 accumulation)."""
+
+    N_IN, N_OUT = 32, 16
+
+    def __init__(self, seed=1):
+        self.calc = load_upside_library()
+        self.h = self.calc.upside_mlp_create(int(seed))
+        if not self.h:
+            raise RuntimeError("upside_mlp_create failed")
+        self.n_param = int(self.calc.upside_mlp_n_param())
+
+    def params(self):
+        p = np.zeros(self.n_param, "f4")
+        if self.calc.upside_mlp_get_params(self.h, p.ctypes.data, self.n_param):
+            raise RuntimeError("get_params failed")
+        return p
+
+    def set_params(self, p):
+        p = _f4(p, (self.n_param,))
+        if self.calc.upside_mlp_set_params(self.h, p.ctypes.data, self.n_param):
+            raise RuntimeError("set_params failed")
+
+    def set_data(self, feat, target):
+        feat = _f4(feat)
+        target = _f4(target)
+        if feat.ndim != 2 or feat.shape[1] != self.N_IN or target.shape != (len(feat), 8):
+            raise ValueError("features (n, 32) and targets (n, 8) expected")
+        if self.calc.upside_mlp_set_data(self.h, feat.ctypes.data, target.ctypes.data, len(feat)):
+            raise RuntimeError("set_data failed")
+
+    def loss_grad(self):
+        loss = np.zeros(1, "f4")
+        g = np.zeros(self.n_param, "f4")
+        if self.calc.upside_mlp_loss_grad(self.h, loss.ctypes.data, g.ctypes.data):
+            raise RuntimeError("loss_grad failed")
+        return float(loss[0]), g
+
+    def train(self, n_steps, batch_rows=4096, lr=1e-3):
+        losses = np.zeros(n_steps, "f4")
+        if self.calc.upside_mlp_train(self.h, int(n_steps), int(batch_rows), float(lr), losses.ctypes.data):
+            raise RuntimeError("train failed")
+        return losses
+
+    def forward(self, feat):
+        feat = _f4(feat)
+        out = np.zeros((len(feat), self.N_OUT), "f4")
+        if self.calc.upside_mlp_forward(self.h, feat.ctypes.data, len(feat), out.ctypes.data):
+            raise RuntimeError("forward failed")
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.calc.upside_mlp_free(self.h)
             self.h = None

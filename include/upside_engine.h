@@ -309,6 +309,18 @@ int upside_mlp_set_data(UpsideMlp* net, const float* features, const float* targ
 int upside_mlp_loss_grad(UpsideMlp* net, float* loss, float* grad);
 int upside_mlp_train(UpsideMlp* net, int n_steps, int batch_rows, float lr, float* losses);
 int upside_mlp_forward(UpsideMlp* net, const float* features, int n_rows, float* out);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * allocation).  The warm-start network's loss and gradient as
+ * upside_mlp_loss_grad computes them (same kernels and order, so the same
+ * bits), on buffers the caller owns in device memory: params[n_param],
+ * features[n_rows][32], targets[n_rows][8], out[n_param + 1] (gradient, then
+ * the loss).  Every intermediate lives in a static device array of the
+ * library (about 3 MB per GPU, held until the process ends); n_rows must be
+ * in [1, 2^24].  Calls that overlap on one GPU share that array and race.
+ * Runs on the null stream and returns when out is written; 0 on success. */
+int upside_mlp_loss_grad_noalloc(const float* params, const float* features, const float* targets, int n_rows,
+                                 float* out);
 
 #ifdef __cplusplus
 }

@@ -10,6 +10,7 @@
 // marginal; loss = -sum_r t_r log p_r, averaged over the rows of the batch.
 // d loss / d delta_r = (p_r - t_r) / n_rows.
 #include <hip/hip_runtime.h>
+#include <cstdio>
 #include "dev_mlp.h"
 
 namespace upside {
@@ -341,3 +342,93 @@ hipError_t upside_mlp_launch_forward(const MlpDev& w, const float* feat, int n, 
 }
 
 int upside_mlp_partial_size() { return TR_PARTIAL; }
+
+// ---- loss and gradient without allocation ---------------------------------------
+// Callers that cannot allocate (code under graph capture, serving loops) get
+// the network's loss and gradient on device buffers they own, with every
+// intermediate in a file-scope device array: the packed bf16 weights and the
+// per-workgroup partial gradients.  Each GPU holds its own copy of the array
+// from the library's first use on it until the process ends; its device
+// address is looked up once per device.  Calls that overlap on one GPU share
+// the array and race: callers serialise them.
+namespace {
+constexpr size_t NOALLOC_WEIGHTS = 96 * 1024;                               // bf16 copies and biases
+constexpr size_t NOALLOC_BYTES = NOALLOC_WEIGHTS + (size_t)MLP_PARTS * TR_PARTIAL * sizeof(float);
+constexpr int NOALLOC_MAX_ROWS = 1 << 24;                                    // row * MLP_IN stays in int range
+}  // namespace
+
+static __device__ __align__(256) char scratch[NOALLOC_BYTES];
+
+namespace {
+char* scratch_ptr[64];
+bool scratch_ready[64];
+
+// Device address of `scratch` on the current device (hipGetSymbolAddress once per device).
+hipError_t scratch_address(char** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    if (!scratch_ready[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        scratch_ptr[dev] = (char*)p;
+        scratch_ready[dev] = true;
+    }
+    *out = scratch_ptr[dev];
+    return hipSuccess;
+}
+}  // namespace
+
+// Loss and gradient of the mean loss over n_rows rows, as upside_mlp_loss_grad
+// computes them (same kernels, grid and order, so the same bits):
+// params[MLP_NPARAM], features[n_rows][MLP_IN] and targets[n_rows][8] in
+// device memory; out[MLP_NPARAM + 1] (gradient, then the loss) in device
+// memory.  Runs on the null stream and returns when the result is written.
+extern "C" int upside_mlp_loss_grad_noalloc(const float* params, const float* features, const float* targets,
+                                            int n_rows, float* out) {
+    if (n_rows <= 0 || n_rows > NOALLOC_MAX_ROWS) {
+        std::fprintf(stderr, "ERROR: upside_mlp_loss_grad_noalloc: n_rows must be in [1, %d], got %d\n",
+                     NOALLOC_MAX_ROWS, n_rows);
+        return 1;
+    }
+    if (!params || !features || !targets || !out) {
+        std::fprintf(stderr, "ERROR: upside_mlp_loss_grad_noalloc: null buffer\n");
+        return 1;
+    }
+    char* base = nullptr;
+    hipError_t e = scratch_address(&base);
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "ERROR: upside_mlp_loss_grad_noalloc: scratch address: %s\n", hipGetErrorString(e));
+        return 1;
+    }
+    // carve the weight copies (256-byte aligned pieces) and the partials
+    char* p = base;
+    auto take = [&](size_t b) { char* q = p; p += (b + 255) & ~(size_t)255; return q; };
+    __bf16* w1 = (__bf16*)take(MLP_H * MLP_IN * 2);
+    __bf16* w2p = (__bf16*)take(MLP_H * MLP_H * 2);
+    __bf16* w3p = (__bf16*)take(MLP_OUT * MLP_H * 2);
+    __bf16* w3tp = (__bf16*)take(MLP_H * 32 * 2);
+    __bf16* w2tp = (__bf16*)take(MLP_H * MLP_H * 2);
+    float* b1 = (float*)take(MLP_H * 4);
+    float* b2 = (float*)take(MLP_H * 4);
+    float* b3 = (float*)take(MLP_OUT * 4);
+    float* partial = (float*)(base + NOALLOC_WEIGHTS);
+    static_assert(((MLP_H * MLP_IN * 2 + 255) / 256 + (MLP_H * MLP_H * 2 + 255) / 256 * 2 +
+                   (MLP_OUT * MLP_H * 2 + 255) / 256 + (MLP_H * 32 * 2 + 255) / 256 +
+                   (MLP_H * 4 + 255) / 256 * 2 + (MLP_OUT * 4 + 255) / 256) * 256 <= NOALLOC_WEIGHTS,
+                  "weight copies exceed their part of the scratch array");
+    MlpDev w{w1, w2p, w3p, w3tp, w2tp, b1, b2, b3};
+    const hipStream_t s = 0;
+    if ((e = upside_mlp_launch_pack(params, w1, w2p, w3p, w3tp, w2tp, b1, b2, b3, s)) != hipSuccess ||
+        (e = upside_mlp_launch_partial(w, features, targets, n_rows, 0, n_rows, 1.f / (float)n_rows, partial,
+                                       MLP_PARTS, s)) != hipSuccess ||
+        (e = upside_mlp_launch_reduce(partial, MLP_PARTS, nullptr, nullptr, nullptr, out, 0, 0.f, 0.f, 0.f, 0.f,
+                                      1.f, 1.f, s)) != hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess) {
+        std::fprintf(stderr, "ERROR: upside_mlp_loss_grad_noalloc: %s\n", hipGetErrorString(e));
+        return 1;
+    }
+    return 0;
+}

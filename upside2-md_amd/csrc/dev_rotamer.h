@@ -382,14 +382,14 @@ __device__ __forceinline__ void rot_edge_w(const BpMem<F, I, J>& m, const int* e
         for (int i = 0; i < A; ++i)
 #pragma unroll
             for (int j = 0; j < B; ++j) {
-                m1[i] += p[w][i * B + j] * v2[w][j];
-                m2[j] += p[w][i * B + j] * v1[w][i];
+                m1[i] = __fmaf_rn(p[w][i * B + j], v2[w][j], m1[i]);
+                m2[j] = __fmaf_rn(p[w][i * B + j], v1[w][i], m2[j]);
             }
         float s1 = 0.f, s2 = 0.f;
 #pragma unroll
-        for (int i = 0; i < A; ++i) s1 += m1[i];
+        for (int i = 0; i < A; ++i) s1 = __fadd_rn(s1, m1[i]);
 #pragma unroll
-        for (int j = 0; j < B; ++j) s2 += m2[j];
+        for (int j = 0; j < B; ++j) s2 = __fadd_rn(s2, m2[j]);
         float i1 = 1.f / s1, i2 = 1.f / s2;
 #pragma unroll
         for (int i = 0; i < A; ++i) m.msg[i * m.ms + e[w]] = m1[i] * i1;
@@ -545,6 +545,77 @@ __device__ void rot_node_pass(Ctx& c, const BpMem<F, I, J>& m, int n_node, F ncu
         bp_node_store(x, pr, b, ncur_, nold_, damping, swap_first);
     }
     bp_sync<TEAM>(c);
+}
+
+// One synchronous iteration of the team solve in a single phase: a group of
+// NODE_LANES lanes takes a node, each lane one of its incidences, and computes
+// that edge's two new messages from the old messages (min) and the old node
+// beliefs exactly as rot_edge_w does; the group multiplies the messages into
+// its node in incidence order as rot_node_pass does, and the edge's first
+// node stores both messages into mout.  Every edge is thus computed by both
+// of its nodes (the same bits), and the message and node passes need one
+// team barrier instead of two.  Results equal rot_edge_pass + rot_node_pass.
+template <class F, class I, class J>
+__device__ void rot_fused_pass(Ctx& c, const BpMem<F, I, J>& m, gp<const float> min_, gp<float> mout, int n_node,
+                               F ncur_, F nold_, float damping, bool swap_first) {
+    const int t = c.gtid, gl = t % NODE_LANES;
+    for (int x = t / NODE_LANES; x < n_node; x += c.gsize / NODE_LANES) {  // uniform in a group
+        if (m.nrot[x] == 1) continue;
+        float pr[6], b[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) { pr[r] = m.prob[x * 6 + r]; b[r] = pr[r]; }
+        const int t0 = m.ioff[x], n = m.ninc[x];
+        for (int tb = 0; tb < n; tb += NODE_LANES) {
+            float a[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            if (tb + gl < n) {
+                const int code = m.inc[t0 + tb + gl];
+                const int e = code >> 1, side = code & 1;
+                const int ends = m.ends[e];
+                const int f = ends & 0xFFFF, s = ends >> 16;
+                float p[36], v1[6], v2[6], m1[6], m2[6];
+#pragma unroll
+                for (int q = 0; q < 36; ++q) p[q] = m.P[q * m.ps + e];
+#pragma unroll
+                for (int r = 0; r < 6; ++r) {
+                    v1[r] = nold_[f * 6 + r] * __builtin_amdgcn_rcpf(1e-10f + min_[r * m.ms + e]);
+                    v2[r] = nold_[s * 6 + r] * __builtin_amdgcn_rcpf(1e-10f + min_[(6 + r) * m.ms + e]);
+                    m1[r] = 0.f;
+                    m2[r] = 0.f;
+                }
+#pragma unroll
+                for (int i = 0; i < 6; ++i)
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) {
+                        m1[i] = __fmaf_rn(p[i * 6 + j], v2[j], m1[i]);
+                        m2[j] = __fmaf_rn(p[i * 6 + j], v1[i], m2[j]);
+                    }
+                float s1 = 0.f, s2 = 0.f;
+#pragma unroll
+                for (int r = 0; r < 6; ++r) { s1 = __fadd_rn(s1, m1[r]); s2 = __fadd_rn(s2, m2[r]); }
+                const float i1 = 1.f / s1, i2 = 1.f / s2;
+#pragma unroll
+                for (int r = 0; r < 6; ++r) { m1[r] = m1[r] * i1; m2[r] = m2[r] * i2; }
+                if (side == 0) {
+#pragma unroll
+                    for (int r = 0; r < 6; ++r) {
+                        mout[r * m.ms + e] = m1[r];
+                        mout[(6 + r) * m.ms + e] = m2[r];
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < 6; ++r) a[r] = side ? m2[r] : m1[r];
+            }
+            const int cnt = min(NODE_LANES, n - tb);
+            for (int q = 0; q < cnt; ++q) {
+                float fa[6];
+#pragma unroll
+                for (int r = 0; r < 6; ++r) fa[r] = __shfl(a[r], q, NODE_LANES);
+                bp_factor<F>(b, fa);
+            }
+        }
+        if (gl == 0) bp_node_store(x, pr, b, ncur_, nold_, damping, swap_first);
+    }
+    team_sync(c);
 }
 
 // ---- warm start (bp_mode 1 and 2; no counterpart in the reference) ----------
@@ -724,18 +795,37 @@ __device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const B
     const int mode = c.args->bp_mode;
     bool warm = mode == 2;
     int total = 0;
-    for (;;) {
+    // On the team the message and node passes of an iteration are fused
+    // (rot_fused_pass) over two message buffers that alternate; mm.msg is the
+    // buffer holding the current messages.
+    BpMem<F, I, J> mm = m;
+    F other{};
+    if constexpr (TEAM) other = gmem(c.A + R.msg2);
+    // The first (undamped) pass writes the product into `old`, which the
+    // fused pass's message updates are still reading: it keeps two phases.
+    auto iterate = [&](F cur_, F old_, float damping, bool swap_first) {
         unsigned long long t0 = ptick(c);
-        rot_edge_pass<TEAM>(c, m, nbp, old);
+        if constexpr (TEAM) {
+            if (!swap_first) {
+                rot_fused_pass(c, mm, (gp<const float>)mm.msg, (gp<float>)other, n, cur_, old_, damping, false);
+                F tt = mm.msg; mm.msg = other; other = tt;
+                pacc(c, PROF_BP_NODE, t0);
+                return;
+            }
+        }
+        rot_edge_pass<TEAM>(c, mm, nbp, old_);
         pacc(c, PROF_BP_EDGE, t0);
         t0 = ptick(c);
-        rot_node_pass<TEAM>(c, m, n, cur, old, 0.f, true);
+        rot_node_pass<TEAM>(c, mm, n, cur_, old_, damping, swap_first);
         pacc(c, PROF_BP_NODE, t0);
+    };
+    for (;;) {
+        iterate(cur, old, 0.f, true);
         if (mode > 0 && total == 0)  // old: the first pass's product
-            bp_features<TEAM>(c, m, n, old, (gp<const float>)v.nprev, v.feat);
+            bp_features<TEAM>(c, mm, n, old, (gp<const float>)v.nprev, v.feat);
         if (warm) {
-            bp_warm_beliefs<TEAM>(c, m, c.args->mlp, n, (gp<const float>)v.feat, cur);
-            bp_local_messages<TEAM>(c, m, nbp, cur, c.args->bp_klocal);
+            bp_warm_beliefs<TEAM>(c, mm, c.args->mlp, n, (gp<const float>)v.feat, cur);
+            bp_local_messages<TEAM>(c, mm, nbp, cur, c.args->bp_klocal);
         }
         const float tol = mode > 0 ? c.args->bp_residual : R.tol;
         const int limit = warm ? min(c.args->bp_budget, R.max_iter) : R.max_iter;
@@ -744,12 +834,7 @@ __device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const B
         for (; dev > tol && it < limit; it += R.chunk) {
             for (int j = 0; j < R.chunk; ++j) {
                 F tt = cur; cur = old; old = tt;          // swap node beliefs
-                t0 = ptick(c);
-                rot_edge_pass<TEAM>(c, m, nbp, old);
-                pacc(c, PROF_BP_EDGE, t0);
-                t0 = ptick(c);
-                rot_node_pass<TEAM>(c, m, n, cur, old, R.damping, false);
-                pacc(c, PROF_BP_NODE, t0);
+                iterate(cur, old, R.damping, false);
             }
             float d = 0.f;
             for (int x = t; x < n; x += nt) {
@@ -773,7 +858,7 @@ __device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const B
         // the warm start missed the residual within its budget: plain start
         if (c.gtid == 0) v.stats[8] += 1;
         warm = false;
-        bp_plain_start<TEAM>(c, m, nbp, n, old);
+        bp_plain_start<TEAM>(c, mm, nbp, n, old);
     }
     const int it = total;
     unsigned long long t0 = ptick(c);
@@ -802,8 +887,8 @@ __device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const B
         float b1[6], b2[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            b1[r] = old[f * 6 + r] / (1e-10f + m.msg[r * m.ms + k]);
-            b2[r] = old[s * 6 + r] / (1e-10f + m.msg[(6 + r) * m.ms + k]);
+            b1[r] = old[f * 6 + r] / (1e-10f + mm.msg[r * m.ms + k]);
+            b2[r] = old[s * 6 + r] / (1e-10f + mm.msg[(6 + r) * m.ms + k]);
         }
         float p[36];
 #pragma unroll

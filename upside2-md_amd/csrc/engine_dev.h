@@ -167,6 +167,7 @@ struct RotamerDev {
     int binc;              // int 2*ccap     BP incidences (edge*2 + side), stored from ioff[x]
     int P;                 // float 36*ccap  edge probability matrix, SoA over BP edges
     int msg;               // float 12*ccap  [m1(6) m2(6)] per BP edge, SoA, updated in place
+    int msg2;              // float 12*ccap  second message buffer of the team solve (passes alternate)
     int marg;              // float 36*ccap  edge marginals, SoA
     int e1x;               // float n_bead   energy of edges to 1-rotamer residues
     int ncen;              // float n_node*4 bead-sphere centre and radius of each residue

@@ -1000,6 +1000,7 @@ struct Builder {
         R.binc = T.ialloc(2 * cc);
         R.P = T.falloc(cc * 36);
         R.msg = T.falloc(cc * 12);
+        R.msg2 = T.falloc(cc * 12);
         R.marg = T.falloc(cc * 36);
         R.e1x = T.falloc(nb);
         R.ncen = T.falloc(nn * 4);

@@ -1137,7 +1137,7 @@ __device__ void grid_sync(Ctx& c, const RexDev& x) {
 // stream 1, counter (done, 0, k), one per test that needs it.  With one
 // shared potential the weights after a swap are those of the partner's
 // positions, -beta_s E(x_partner), evaluated once per system.
-__device__ void rex_attempt(Ctx& c, SysDesc& S, const RexDev& x, int sys, int n_sys, long long done) {
+__device__ __noinline__ void rex_attempt(Ctx& c, SysDesc& S, const RexDev& x, int sys, int n_sys, long long done) {
     compute(c, S, 1);
     if (c.gtid == 0) x.energy[sys] = total_potential(c, S);
     {

@@ -240,10 +240,14 @@ int upside_batch_get_team(UpsideBatch* batch);
 int upside_batch_autotune(UpsideBatch* batch, int n_round, const int* candidates, int n_candidates, float* ms_out);
 /* Where each rotamer BP solve runs (no reference counterpart; the reference
  * solves on its one thread, src/rotamer.cpp:1106-1162): 0 (default) on one
- * workgroup with the BP state in its LDS when it fits, else on the system's
- * whole team with the state in memory; 1 always on the whole team; 2 always
- * on one workgroup (LDS when it fits, else memory).  Every layout computes the
- * same arithmetic in the same order: results are identical.
+ * workgroup with the BP state in its LDS when it fits (the rest of the team
+ * meanwhile evaluates the ops that do not depend on the solve), else on
+ * several workgroups with the state in memory (with four or more workgroups
+ * per system the first half of the team solves while the second half
+ * evaluates those ops); 1 always on the whole team; 2 always on one workgroup
+ * (LDS when it fits, else memory); 3 always on several workgroups as in 0.
+ * Every layout computes the same arithmetic in the same order: results are
+ * identical.
  * upside_batch_autotune_layout times every (team, layout) pair from the same
  * state as upside_batch_autotune does and keeps the fastest;
  * ms_out[n_teams * n_layouts] (may be NULL), row-major by team. */

@@ -968,7 +968,7 @@ def test_team_execution_matches_single_workgroup(ue, chain_file, team):
     np.testing.assert_allclose(bt.energies(), b1.energies(), rtol=1e-5)
 
 
-@pytest.mark.parametrize("n_res,layout", [(120, 1), (400, 0), (400, 2)])
+@pytest.mark.parametrize("n_res,layout", [(120, 1), (120, 3), (400, 0), (400, 2)])
 def test_rotamer_solve_layouts_match(ue, chain_file, n_res, layout):
     """The rotamer BP solve on the whole team (state in memory) or on one
     workgroup (state in LDS, or in memory when it does not fit) computes the

@@ -43,6 +43,12 @@ struct Ctx {
     unsigned long long* bar2; // arrival counter of the sub-team of ranks 1..G-1
     float* tred2;             // the sub-team's reduction scratch
     int tpar2;                // the sub-team's next reduction buffer parity
+    unsigned long long* bar_lo;  // sub-team of ranks 0..G/2-1: counter, scratch, parity
+    float* tred_lo;
+    int tpar_lo;
+    unsigned long long* bar_hi;  // sub-team of ranks G/2..G-1
+    float* tred_hi;
+    int tpar_hi;
     float* tred;       // team reduction scratch, 2 buffers of G floats
     int tpar;          // buffer parity of the next team reduction
     int bp_layout;     // where rotamer solves run (see bp_on_one_wg)
@@ -104,20 +110,25 @@ __device__ void team_sync(Ctx& c) {
     }
 }
 
-// The sub-team of ranks 1..G-1 (G >= 2), with its own barrier counter and
-// reduction scratch, for work that runs beside rank 0.
-__device__ __forceinline__ Ctx sub_team(const Ctx& c) {
+// The sub-team of ranks [first, first + size) with its own barrier counter,
+// reduction scratch and buffer parity, for work that runs beside the rest of
+// the team.  Only rank 0 keeps the profile accumulators.
+__device__ __forceinline__ Ctx sub_team(const Ctx& c, int first, int size, unsigned long long* bar, float* tred,
+                                        int tpar) {
     Ctx s = c;
-    s.G = c.G - 1;
-    s.rank = c.rank - 1;
+    s.G = size;
+    s.rank = c.rank - first;
     s.gtid = s.rank * NT + c.tid;
-    s.gsize = s.G * NT;
-    s.bar = c.bar2;
-    s.tred = c.tred2;
-    s.tpar = c.tpar2;
-    s.prof = nullptr;
+    s.gsize = size * NT;
+    s.bar = bar;
+    s.tred = tred;
+    s.tpar = tpar;
+    if (c.rank != 0) s.prof = nullptr;
     return s;
 }
+
+// The sub-team of ranks 1..G-1 (G >= 2), for work that runs beside rank 0.
+__device__ __forceinline__ Ctx sub_team(const Ctx& c) { return sub_team(c, 1, c.G - 1, c.bar2, c.tred2, c.tpar2); }
 
 // XCD of the calling workgroup (hardware register XCC_ID).
 __device__ __forceinline__ unsigned xcd_id() { return __builtin_amdgcn_s_getreg((15 << 11) | (0 << 6) | 20); }

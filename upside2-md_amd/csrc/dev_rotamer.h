@@ -1043,14 +1043,15 @@ __device__ int rot_solve_team(Ctx& c, const RotamerDev& R) {
     return bp_iterate<true>(c, R, v, m, nbp, v.ncur, v.nold);
 }
 
-// Where a solve runs: true = rank 0 alone (rot_solve_wg), false = the whole
-// team (rot_solve_team).  bp_layout 0: one workgroup when the state fits LDS
-// or the team has one workgroup, else the team; 1: always the team; 2:
-// always one workgroup.  Uniform over the team (nbp is read after a team
-// barrier).
+// Where a solve runs: true = rank 0 alone (rot_solve_wg), false = several
+// workgroups (rot_solve_team: the whole team, or its first half beside the
+// independent ops).  bp_layout 0: one workgroup when the state fits LDS or
+// the team has one workgroup, else several; 1: always the whole team; 2:
+// always one workgroup; 3: always several.  Uniform over the team (nbp is
+// read after a team barrier).
 __device__ __forceinline__ bool bp_on_one_wg(const Ctx& c, const RotamerDev& R, int nbp, int bp_layout) {
     if (c.G == 1 || bp_layout == 2) return true;
-    if (bp_layout == 1) return false;
+    if (bp_layout == 1 || bp_layout == 3) return false;
     return bp_fits_lds(nbp, R.n_node, R.ccap);
 }
 

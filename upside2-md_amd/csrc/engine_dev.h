@@ -298,7 +298,12 @@ struct RunArgs {
 };
 
 constexpr int MAX_TEAM = 64;
-constexpr int BAR_WORDS = 4 + MAX_TEAM + 2;
+// Team words: [0..1] the team's 64-bit arrival counter, [2] spin-timeout flag,
+// [3] placement report, [4..4+MAX_TEAM) XCD of each workgroup, then the
+// 64-bit arrival counters of the sub-teams: ranks 1..G-1, ranks 0..G/2-1 and
+// ranks G/2..G-1 (each stays a multiple of its own size).
+constexpr int BAR_WORDS = 4 + MAX_TEAM + 6;
+constexpr int BAR_SUB1 = 4 + MAX_TEAM, BAR_LO = 4 + MAX_TEAM + 2, BAR_HI = 4 + MAX_TEAM + 4;
 // Workgroups of a launch are dispatched round-robin over this many XCDs
 // (MI355X: 8 XCDs of 32 CUs, each with its own L2).  Team launches place every
 // team on one XCD (see engine_kernels.hip), which the kernel verifies.

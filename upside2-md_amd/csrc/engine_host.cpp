@@ -1111,7 +1111,7 @@ static void load_topology(Topology& T, H5& h5, int n_atom) {
     T.mom = T.falloc(n_atom * 3);
     T.err = T.ialloc(4);
     T.bar = T.ialloc(BAR_WORDS);
-    T.tred = T.falloc(4 * MAX_TEAM);  // the team's two buffers, then the sub-team's
+    T.tred = T.falloc(8 * MAX_TEAM);  // two buffers each: the team, sub-teams 1..G-1, 0..G/2-1, G/2..G-1
     vector<int> z;
     for (auto& r : T.sens_zero) { z.push_back(r.first); z.push_back(r.second); }
     T.szero = T.ialloc(z.size());
@@ -1735,7 +1735,7 @@ struct Batch {
         const int cur_layout = bp_layout;
         if (!lay) { lay = &cur_layout; n_lay = 1; }
         for (int l = 0; l < n_lay; ++l)
-            if (lay[l] < 0 || lay[l] > 2) fail("rotamer solve layout must be 0, 1 or 2");
+            if (lay[l] < 0 || lay[l] > 3) fail("rotamer solve layout must be 0, 1, 2 or 3");
         vector<vector<float>> pos(sys.size()), mom(sys.size());
         pull_desc();
         vector<SysDesc> saved = desc;
@@ -1792,11 +1792,11 @@ struct Batch {
             for (auto& sy : sys) {
                 unsigned w[BAR_WORDS];
                 hip_check(hipMemcpy(w, sy.iarena + sy.prog->T.bar, sizeof(w), hipMemcpyDeviceToHost), "barrier");
-                unsigned long long count = (unsigned long long)w[0] | ((unsigned long long)w[1] << 32);
-                const int s2 = 4 + MAX_TEAM;
-                unsigned long long count2 = (unsigned long long)w[s2] | ((unsigned long long)w[s2 + 1] << 32);
+                auto word64 = [&](int k) { return (unsigned long long)w[k] | ((unsigned long long)w[k + 1] << 32); };
+                const unsigned long long g = (unsigned long long)team, lo = g / 2, hi = g - g / 2;
                 if (w[2]) fail("team barrier wait timed out: results of this run are void");
-                if (count % (unsigned long long)team || (team > 2 && count2 % (unsigned long long)(team - 1)))
+                if (word64(0) % g || (g > 2 && word64(BAR_SUB1) % (g - 1)) || (g >= 4 && word64(BAR_LO) % lo) ||
+                    (g >= 4 && word64(BAR_HI) % hi))
                     fail("team barrier arrivals unbalanced at the end of a launch: results of this run are void");
                 team_local = w[3] != 0;
             }
@@ -3704,7 +3704,7 @@ int upside_batch_autotune_layout(UpsideBatch* b, int n_round, const int* teams, 
 
 int upside_batch_set_bp_layout(UpsideBatch* b, int layout) {
     ABI_TRY
-    if (layout < 0 || layout > 2) throw std::runtime_error("rotamer solve layout must be 0, 1 or 2");
+    if (layout < 0 || layout > 3) throw std::runtime_error("rotamer solve layout must be 0, 1, 2 or 3");
     b->batch.bp_layout = layout;
     return 0;
     ABI_CATCH(1)

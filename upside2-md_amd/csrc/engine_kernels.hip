@@ -960,8 +960,30 @@ __device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mod
             if (!rotamer_pre(c, S, op, G, R)) return false;
             int it = 0;
             if (!bp_on_one_wg(c, R, c.I[R.stats + 4], c.bp_layout)) {
-                // the state does not fit one workgroup's LDS: the whole team
-                // solves, and the independent ops follow in program order
+                // the state does not fit one workgroup's LDS: with four or
+                // more workgroups the first half solves while the second half
+                // evaluates the independent ops (bp_layout 0 and 3); else
+                // the whole team solves and the ops follow in program order
+                if (c.bp_layout != 1 && c.G >= 4) {
+                    const int gb = c.G / 2;
+                    unsigned long long t1 = ptick(c);
+                    if (c.rank < gb) {
+                        Ctx b = sub_team(c, 0, gb, c.bar_lo, c.tred_lo, c.tpar_lo);
+                        it = rot_solve_team(b, R);
+                        c.tpar_lo = b.tpar;
+                    } else {
+                        Ctx o = sub_team(c, gb, c.G - gb, c.bar_hi, c.tred_hi, c.tpar_hi);
+                        run_germ_range<false>(o, S, k + 1, op.overlap_end, mode);
+                        c.tpar_hi = o.tpar;
+                    }
+                    pacc(c, PROF_BP, t1);
+                    team_sync(c);  // join; an error in the overlapped ops is read after the rotamer
+                    rotamer_post(c, S, op, G, R, mode, it);
+                    prof_op(c, k, t0);
+                    if (c.I[S.error]) return false;
+                    k = op.overlap_end;
+                    continue;
+                }
                 unsigned long long t1 = ptick(c);
                 it = rot_solve_team(c, R);
                 pacc(c, PROF_BP, t1);
@@ -1224,9 +1246,15 @@ __device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& arg
     c.bar_flag = reinterpret_cast<unsigned*>(S.iarena + S.bar + 2);
     c.tred = S.arena + S.tred;
     c.tpar = 0;
-    c.bar2 = reinterpret_cast<unsigned long long*>(S.iarena + S.bar + 4 + MAX_TEAM);
+    c.bar2 = reinterpret_cast<unsigned long long*>(S.iarena + S.bar + BAR_SUB1);
     c.tred2 = c.tred + 2 * MAX_TEAM;
     c.tpar2 = 0;
+    c.bar_lo = reinterpret_cast<unsigned long long*>(S.iarena + S.bar + BAR_LO);
+    c.tred_lo = c.tred + 4 * MAX_TEAM;
+    c.tpar_lo = 0;
+    c.bar_hi = reinterpret_cast<unsigned long long*>(S.iarena + S.bar + BAR_HI);
+    c.tred_hi = c.tred + 6 * MAX_TEAM;
+    c.tpar_hi = 0;
     c.bp_layout = args.bp_layout;
     c.args = &args;
     team_place(c, S.iarena + S.bar + 4);

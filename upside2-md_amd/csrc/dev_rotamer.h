@@ -205,7 +205,6 @@ __device__ void rot_fill(Ctx& c, const PairGraph& G, const RotamerDev& R) {
     const int* loc1 = c.CI + G.loc1;
     const int E = c.I[G.state + 3];
     const int C = v.coff[R.n_node];
-    const int cap = v.cap;
     // energies of each bead's edges to 1-rotamer residues (folded into the node)
     {
         int grp = c.gtid / GW, gl = c.gtid % GW;
@@ -263,7 +262,7 @@ __device__ void rot_fill(Ctx& c, const PairGraph& G, const RotamerDev& R) {
         v.bpc[e] = f | (s2 << 16); v.bpf[e] = f; v.bps[e] = s2;
         int n1 = v.nrot[f], n2 = v.nrot[s2];
 #pragma unroll
-        for (int q = 0; q < 36; ++q) v.P[q * cap + e] = ((q / 6) < n1 && (q % 6) < n2) ? 1.f : 0.f;
+        for (int q = 0; q < 36; ++q) v.P[e * 36 + q] = ((q / 6) < n1 && (q % 6) < n2) ? 1.f : 0.f;
     }
     if (c.gtid == 0) v.stats[4] = nbp;
     team_sync(c);
@@ -277,7 +276,7 @@ __device__ void rot_fill(Ctx& c, const PairGraph& G, const RotamerDev& R) {
         bool sw = v.nrot[a] > v.nrot[b];
         int ra = v.brot[i], rb = v.brot[j];
         int q = sw ? rb * 6 + ra : ra * 6 + rb;
-        v.P[q * cap + e] = expf(-val[s]);
+        v.P[e * 36 + q] = expf(-val[s]);
     }
     // compact BP incidences of every node, in incidence order: a 16-lane
     // group per node tests 16 incidences at a time and keeps the BP ones
@@ -318,10 +317,21 @@ __device__ void rot_fill(Ctx& c, const PairGraph& G, const RotamerDev& R) {
 // pointers), else in memory.
 template <class F, class I, class J>
 struct BpMem {
-    F msg;             // component (side*6 + r) of edge e at msg[(side*6 + r)*ms + e]
+    // With the messages in memory (the team solve, and a workgroup solve too
+    // large for LDS) every edge's 12 message components and 36 matrix entries
+    // are contiguous (array of structures): a node gathers its incidences'
+    // edges from scattered places, and so reads 1-2 cache lines per edge
+    // instead of 48.  In LDS the edges are swept in order by consecutive
+    // threads, and a structure of arrays (stride ms, ps) keeps those sweeps
+    // conflict-free and coalesced.
+    static constexpr bool AOS = !__is_same(F, lp<float>);
+    F msg;             // component (side*6 + r) of edge e at msg[mi(side*6 + r, e)]
     int ms;
-    gp<const float> P; // edge matrices (memory), P[(i*6+j)*ps + e]
+    gp<const float> P; // edge matrices (memory), entry (i, j) of edge e at P[pi(i*6 + j, e)]
     int ps;
+    __device__ __forceinline__ int mi(int r, int e) const { return AOS ? e * 12 + r : r * ms + e; }
+    __device__ __forceinline__ int pi(int q, int e) const { return AOS ? e * 36 + q : q * ps + e; }
+    __device__ __forceinline__ int cs() const { return AOS ? 1 : ms; }  // stride of a message's components
     gp<const float> prob;  // node probabilities [n*6]
     I ends;            // edge endpoints: first | second << 16 (rows, columns of P)
     I ioff, ninc;      // node incidences inc[ioff[x] + t], t < ninc[x]
@@ -361,15 +371,15 @@ __device__ __forceinline__ void rot_edge_w(const BpMem<F, I, J>& m, const int* e
 #pragma unroll
         for (int i = 0; i < A; ++i)
 #pragma unroll
-            for (int j = 0; j < B; ++j) p[w][i * B + j] = m.P[(i * 6 + j) * m.ps + e[w]];
+            for (int j = 0; j < B; ++j) p[w][i * B + j] = m.P[m.pi(i * 6 + j, e[w])];
         // belief / old message by the hardware reciprocal (1 ulp) rather
         // than a correctly rounded division: these dominate the pass
 #pragma unroll
         for (int r = 0; r < A; ++r)
-            v1[w][r] = nold_[f * 6 + r] * __builtin_amdgcn_rcpf(1e-10f + m.msg[r * m.ms + e[w]]);
+            v1[w][r] = nold_[f * 6 + r] * __builtin_amdgcn_rcpf(1e-10f + m.msg[m.mi(r, e[w])]);
 #pragma unroll
         for (int r = 0; r < B; ++r)
-            v2[w][r] = nold_[s * 6 + r] * __builtin_amdgcn_rcpf(1e-10f + m.msg[(6 + r) * m.ms + e[w]]);
+            v2[w][r] = nold_[s * 6 + r] * __builtin_amdgcn_rcpf(1e-10f + m.msg[m.mi(6 + r, e[w])]);
     }
 #pragma unroll
     for (int w = 0; w < W; ++w) {
@@ -392,9 +402,9 @@ __device__ __forceinline__ void rot_edge_w(const BpMem<F, I, J>& m, const int* e
         for (int j = 0; j < B; ++j) s2 = __fadd_rn(s2, m2[j]);
         float i1 = 1.f / s1, i2 = 1.f / s2;
 #pragma unroll
-        for (int i = 0; i < A; ++i) m.msg[i * m.ms + e[w]] = m1[i] * i1;
+        for (int i = 0; i < A; ++i) m.msg[m.mi(i, e[w])] = m1[i] * i1;
 #pragma unroll
-        for (int j = 0; j < B; ++j) m.msg[(6 + j) * m.ms + e[w]] = m2[j] * i2;
+        for (int j = 0; j < B; ++j) m.msg[m.mi(6 + j, e[w])] = m2[j] * i2;
     }
 }
 
@@ -492,9 +502,9 @@ __device__ void rot_node_pass(Ctx& c, const BpMem<F, I, J>& m, int n_node, F ncu
                 float a[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 if (tb + gl < n) {
                     const int code = m.inc[t0 + tb + gl];
-                    const int b0 = ((code & 1) * 6) * m.ms + (code >> 1);
+                    const int b0 = m.mi((code & 1) * 6, code >> 1);
 #pragma unroll
-                    for (int r = 0; r < 6; ++r) a[r] = m.msg[b0 + r * m.ms];
+                    for (int r = 0; r < 6; ++r) a[r] = m.msg[b0 + r * m.cs()];
                 }
                 const int cnt = min(NODE_LANES, n - tb);
                 for (int q = 0; q < cnt; ++q) {
@@ -524,22 +534,22 @@ __device__ void rot_node_pass(Ctx& c, const BpMem<F, I, J>& m, int n_node, F ncu
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int code = m.inc[t0 + t + q];
-                base[q] = ((code & 1) * 6) * m.ms + (code >> 1);
+                base[q] = m.mi((code & 1) * 6, code >> 1);
             }
             float a[4][6];
 #pragma unroll
             for (int q = 0; q < 4; ++q)
 #pragma unroll
-                for (int r = 0; r < 6; ++r) a[q][r] = m.msg[base[q] + r * m.ms];
+                for (int r = 0; r < 6; ++r) a[q][r] = m.msg[base[q] + r * m.cs()];
 #pragma unroll
             for (int q = 0; q < 4; ++q) bp_factor<F>(b, a[q]);
         }
         for (; t < n; ++t) {
             const int code = m.inc[t0 + t];
-            const int b0 = ((code & 1) * 6) * m.ms + (code >> 1);
+            const int b0 = m.mi((code & 1) * 6, code >> 1);
             float a0[6];
 #pragma unroll
-            for (int r = 0; r < 6; ++r) a0[r] = m.msg[b0 + r * m.ms];
+            for (int r = 0; r < 6; ++r) a0[r] = m.msg[b0 + r * m.cs()];
             bp_factor<F>(b, a0);
         }
         bp_node_store(x, pr, b, ncur_, nold_, damping, swap_first);
@@ -574,11 +584,11 @@ __device__ void rot_fused_pass(Ctx& c, const BpMem<F, I, J>& m, gp<const float> 
                 const int f = ends & 0xFFFF, s = ends >> 16;
                 float p[36], v1[6], v2[6], m1[6], m2[6];
 #pragma unroll
-                for (int q = 0; q < 36; ++q) p[q] = m.P[q * m.ps + e];
+                for (int q = 0; q < 36; ++q) p[q] = m.P[m.pi(q, e)];
 #pragma unroll
                 for (int r = 0; r < 6; ++r) {
-                    v1[r] = nold_[f * 6 + r] * __builtin_amdgcn_rcpf(1e-10f + min_[r * m.ms + e]);
-                    v2[r] = nold_[s * 6 + r] * __builtin_amdgcn_rcpf(1e-10f + min_[(6 + r) * m.ms + e]);
+                    v1[r] = nold_[f * 6 + r] * __builtin_amdgcn_rcpf(1e-10f + min_[m.mi(r, e)]);
+                    v2[r] = nold_[s * 6 + r] * __builtin_amdgcn_rcpf(1e-10f + min_[m.mi(6 + r, e)]);
                     m1[r] = 0.f;
                     m2[r] = 0.f;
                 }
@@ -598,8 +608,8 @@ __device__ void rot_fused_pass(Ctx& c, const BpMem<F, I, J>& m, gp<const float> 
                 if (side == 0) {
 #pragma unroll
                     for (int r = 0; r < 6; ++r) {
-                        mout[r * m.ms + e] = m1[r];
-                        mout[(6 + r) * m.ms + e] = m2[r];
+                        mout[m.mi(r, e)] = m1[r];
+                        mout[m.mi(6 + r, e)] = m2[r];
                     }
                 }
 #pragma unroll
@@ -726,11 +736,11 @@ __device__ void bp_local_messages(Ctx& c, const BpMem<F, I, J>& m, int nbp, F b,
         const int f = ends & 0xFFFF, s = ends >> 16;
         float p[36], m1[6], m2[6], bf[6], bs[6];
 #pragma unroll
-        for (int q = 0; q < 36; ++q) p[q] = m.P[q * m.ps + e];
+        for (int q = 0; q < 36; ++q) p[q] = m.P[m.pi(q, e)];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            m1[r] = m.msg[r * m.ms + e];
-            m2[r] = m.msg[(6 + r) * m.ms + e];
+            m1[r] = m.msg[m.mi(r, e)];
+            m2[r] = m.msg[m.mi(6 + r, e)];
             bf[r] = b[f * 6 + r];
             bs[r] = b[s * 6 + r];
         }
@@ -759,8 +769,8 @@ __device__ void bp_local_messages(Ctx& c, const BpMem<F, I, J>& m, int nbp, F b,
         }
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            m.msg[r * m.ms + e] = m1[r];
-            m.msg[(6 + r) * m.ms + e] = m2[r];
+            m.msg[m.mi(r, e)] = m1[r];
+            m.msg[m.mi(6 + r, e)] = m2[r];
         }
     }
     bp_sync<TEAM>(c);
@@ -771,7 +781,7 @@ template <bool TEAM, class F, class I, class J>
 __device__ void bp_plain_start(Ctx& c, const BpMem<F, I, J>& m, int nbp, int n_node, F old) {
     for (int e = bp_t<TEAM>(c); e < nbp; e += bp_nt<TEAM>(c))
 #pragma unroll
-        for (int r = 0; r < 12; ++r) m.msg[r * m.ms + e] = 1.f;
+        for (int r = 0; r < 12; ++r) m.msg[m.mi(r, e)] = 1.f;
     for (int i = bp_t<TEAM>(c); i < 6 * n_node; i += bp_nt<TEAM>(c)) old[i] = m.prob[i];
     bp_sync<TEAM>(c);
 }
@@ -887,12 +897,12 @@ __device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const B
         float b1[6], b2[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            b1[r] = old[f * 6 + r] / (1e-10f + mm.msg[r * m.ms + k]);
-            b2[r] = old[s * 6 + r] / (1e-10f + mm.msg[(6 + r) * m.ms + k]);
+            b1[r] = old[f * 6 + r] / (1e-10f + mm.msg[m.mi(r, k)]);
+            b2[r] = old[s * 6 + r] / (1e-10f + mm.msg[m.mi(6 + r, k)]);
         }
         float p[36];
 #pragma unroll
-        for (int q = 0; q < 36; ++q) p[q] = m.P[q * m.ps + k];
+        for (int q = 0; q < 36; ++q) p[q] = m.P[m.pi(q, k)];
         float sum = 0.f;
 #pragma unroll
         for (int i = 0; i < 6; ++i)
@@ -982,7 +992,7 @@ __device__ int rot_solve_wg(Ctx& c, const RotamerDev& R) {
             posl[e] = k;
             gpos[e] = k;
 #pragma unroll 4
-            for (int q = 0; q < 36; ++q) Ps[q * nbp + k] = v.P[q * cap + e];
+            for (int q = 0; q < 36; ++q) Ps[q * nbp + k] = v.P[e * 36 + q];
         }
         __syncthreads();
         for (int x = c.tid; x < n; x += NT) {
@@ -998,21 +1008,21 @@ __device__ int rot_solve_wg(Ctx& c, const RotamerDev& R) {
         // initial messages = 1, old beliefs = prob (:1110-1133)
         for (int e = c.tid; e < nbp; e += NT)
 #pragma unroll
-            for (int r = 0; r < 12; ++r) m.msg[r * m.ms + e] = 1.f;
+            for (int r = 0; r < 12; ++r) m.msg[m.mi(r, e)] = 1.f;
         __syncthreads();
         pacc(c, PROF_BP_SETUP, t0);
         return bp_iterate<false>(c, R, v, m, nbp, cur, old);
     }
     BpMem<gp<float>, gp<int>, gp<int>> m;
-    m.P = (gp<const float>)v.P; m.ps = cap; m.nrot = v.nrot; m.prob = (gp<const float>)v.nprob;
-    m.msg = v.msg; m.ms = cap;
+    m.P = (gp<const float>)v.P; m.ps = 0; m.nrot = v.nrot; m.prob = (gp<const float>)v.nprob;
+    m.msg = v.msg; m.ms = 0;
     m.ends = v.bpc; m.ioff = v.ioff; m.ninc = v.ninc; m.inc = v.binc;
     m.c1 = m.c2 = 0;
     m.pos = nullptr;
     for (int i = c.tid; i < n * 6; i += NT) v.nold[i] = v.nprob[i];
     for (int e = c.tid; e < nbp; e += NT)
 #pragma unroll
-        for (int r = 0; r < 12; ++r) m.msg[r * m.ms + e] = 1.f;
+        for (int r = 0; r < 12; ++r) m.msg[m.mi(r, e)] = 1.f;
     __syncthreads();
     pacc(c, PROF_BP_SETUP, t0);
     return bp_iterate<false>(c, R, v, m, nbp, v.ncur, v.nold);
@@ -1026,18 +1036,17 @@ __device__ int rot_solve_team(Ctx& c, const RotamerDev& R) {
     RotView v = rot_view(c, R);
     const int nbp = v.stats[4];
     const int n = R.n_node;
-    const int cap = v.cap;
     unsigned long long t0 = ptick(c);
     BpMem<gp<float>, gp<int>, gp<int>> m;
-    m.P = (gp<const float>)v.P; m.ps = cap; m.nrot = v.nrot; m.prob = (gp<const float>)v.nprob;
-    m.msg = v.msg; m.ms = cap;
+    m.P = (gp<const float>)v.P; m.ps = 0; m.nrot = v.nrot; m.prob = (gp<const float>)v.nprob;
+    m.msg = v.msg; m.ms = 0;
     m.ends = v.bpc; m.ioff = v.ioff; m.ninc = v.ninc; m.inc = v.binc;
     m.c1 = m.c2 = 0;
     m.pos = nullptr;
     for (int i = c.gtid; i < n * 6; i += c.gsize) v.nold[i] = v.nprob[i];
     for (int e = c.gtid; e < nbp; e += c.gsize)
 #pragma unroll
-        for (int r = 0; r < 12; ++r) m.msg[r * m.ms + e] = 1.f;
+        for (int r = 0; r < 12; ++r) m.msg[m.mi(r, e)] = 1.f;
     team_sync(c);
     pacc(c, PROF_BP_SETUP, t0);
     return bp_iterate<true>(c, R, v, m, nbp, v.ncur, v.nold);
@@ -1096,7 +1105,7 @@ __device__ float rot_free_energy(Ctx& c, const PairGraph& G, const RotamerDev& R
 #pragma unroll
             for (int j = 0; j < 6; ++j) {
                 float p = v.marg[(i * 6 + j) * cap + e];
-                float q = v.P[(i * 6 + j) * cap + e] * v.ncur[f * 6 + i] * v.ncur[s * 6 + j];
+                float q = v.P[e * 36 + i * 6 + j] * v.ncur[f * 6 + i] * v.ncur[s * 6 + j];
                 en += p * logf((1e-10f + p) / (1e-10f + q));
             }
     }

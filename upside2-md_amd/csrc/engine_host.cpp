@@ -2841,10 +2841,12 @@ static vector<float> rotamer_values(Batch& B, int sys, int op, const string& ln)
     hip_check(hipMemcpy(st, I + R.stats, sizeof(st), hipMemcpyDeviceToHost), "stats");
     const int nbp = st[4];
     vector<int> bpf = ifetch(I + R.bpf, nbp), bps = ifetch(I + R.bps, nbp);
-    vector<float> P((size_t)36 * nbp), M((size_t)36 * nbp);
-    for (int q = 0; q < 36; ++q) {  // SoA rows q = r1*6 + r2 over the BP edges
-        vector<float> p = fetch(A + R.P + (size_t)q * cap, nbp), m = fetch(A + R.marg + (size_t)q * cap, nbp);
-        for (int e = 0; e < nbp; ++e) { P[(size_t)e * 36 + q] = p[e]; M[(size_t)e * 36 + q] = m[e]; }
+    // matrices: 36 contiguous entries r1*6 + r2 per BP edge; marginals: rows
+    // q = r1*6 + r2 of stride cap over the edges
+    vector<float> P = fetch(A + R.P, (size_t)36 * nbp), M((size_t)36 * nbp);
+    for (int q = 0; q < 36; ++q) {
+        vector<float> m = fetch(A + R.marg + (size_t)q * cap, nbp);
+        for (int e = 0; e < nbp; ++e) M[(size_t)e * 36 + q] = m[e];
     }
     // 1-rotamer residue pairs: one bead pair each, probability exp(-E)
     int gst[4];

@@ -331,7 +331,7 @@ def batched_workloads(work, rounds=60, cpu=True):
     def tune(b):
         cand = [t for t in (1, 2, 4, 8, 16) if t <= b.max_team()]
         b.run(3)
-        b.autotune_layout(3, cand, (0, 1))
+        b.autotune_layout(3, cand, (0, 1, 3))
 
     p = os.path.join(work, "rex150.up")
     upconfig.make_chain(p, 150, seed=2)
@@ -478,6 +478,9 @@ def main():
     ap.add_argument("--no-warm-start", action="store_true", help="skip the warm-start workloads")
     ap.add_argument("--team", type=int, default=0,
                     help="workgroups per system (0: time 1..32 on the warmed state and keep the fastest)")
+    ap.add_argument("--bp-layout", type=int, default=-1,
+                    help="rotamer solve layout with --team (0 auto, 1 team, 2 one workgroup, 3 split; "
+                         "default: autotuned with the team size, or 0 with --team)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only an elementwise copy of 1 GiB (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -518,9 +521,10 @@ def main():
     tune = None
     if args.team > 0:
         b.set_team(args.team)
+        b.set_bp_layout(max(args.bp_layout, 0))
     else:
         cand = [t for t in (1, 2, 4, 8, 16, 32) if t <= b.max_team()]
-        tune = b.autotune_layout(6, cand, (0, 1, 2))  # restores the warmed state after each candidate
+        tune = b.autotune_layout(6, cand, (0, 1, 2, 3))  # restores the warmed state after each candidate
         log("team / BP layout autotune (ms per 6 rounds): %s -> team %d layout %d" % (tune, b.team(), b.bp_layout()))
     probe.energy(b.pos(0))
     g0, bp0, nodes = graph_counts(probe, node_names)

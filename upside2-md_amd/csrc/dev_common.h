@@ -35,9 +35,11 @@ struct Ctx {
     int tid;           // thread in the workgroup
     unsigned long long* prof;  // wall-clock accumulators (null: profiling off)
     int pbase;                 // first fixed profile slot
+    int cur_op;                // program op being run (team barriers are counted per op while profiling)
     int rank, G;       // workgroup rank in the team, team size
     int gtid, gsize;   // thread in the team, team threads
     unsigned long long* bar;  // team barrier arrival counter
+    unsigned long long bar_next;  // count that opens this workgroup's next barrier on `bar` (thread 0)
     unsigned* bar_flag;       // team barrier spin-timeout flag
     int local;                // the whole team runs on one XCD (shares one L2)
     unsigned long long* bar2; // arrival counter of the sub-team of ranks 1..G-1
@@ -64,10 +66,12 @@ __device__ __forceinline__ void pacc(const Ctx& c, int slot, unsigned long long 
 
 // Team barrier on a monotonic 64-bit arrival counter: arrival n of a team of
 // G falls in barrier n / G, which opens when the counter reaches the next
-// multiple of G.  No generation is read at launch and nothing is reset
-// between barriers, so a stale read can only prolong a wait.  The counter is a
+// multiple of G.  Nothing is reset between barriers, and the counter is a
 // multiple of G between launches (the host zeroes it when the team size
-// changes).
+// changes).  Every workgroup keeps the count that opens its next barrier
+// (bar_base at the start, then + G per barrier), so its arrival is a
+// no-return atomic: it does not wait for the counter's old value before it
+// starts polling.  A stale poll can only prolong a wait.
 //   Memory ordering: a team spread over XCDs (separate L2s) needs
 // device-scope fences (write back L2 before arriving, invalidate after the
 // wait).  A team on one XCD shares its L2: every wave waits for its stores to
@@ -83,6 +87,15 @@ constexpr unsigned BAR_SPIN_LIMIT = 1u << 24;
 // s_waitcnt immediate: vmcnt(0), expcnt and lgkmcnt not waited on.
 constexpr int WAIT_VMEM = 0x0F70;
 
+// Count that opens the first barrier of a team of G on counter `bar`, read
+// before this workgroup's first arrival: fewer than G arrivals of that
+// barrier can precede it (it cannot open without this workgroup), so the
+// count rounded down to a multiple of G is the base.  Thread 0.
+__device__ __forceinline__ unsigned long long bar_first(unsigned long long* bar, int G) {
+    const unsigned long long v = __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return (v / (unsigned long long)G + 1ull) * (unsigned long long)G;
+}
+
 __device__ void team_sync(Ctx& c) {
     // every wave's stores acknowledged by L2 before the team can observe the arrival
     if (c.G > 1 && c.local) __builtin_amdgcn_s_waitcnt(WAIT_VMEM);
@@ -90,8 +103,8 @@ __device__ void team_sync(Ctx& c) {
     if (c.G > 1) {
         if (c.tid == 0) {
             if (!c.local) __threadfence();
-            unsigned long long old = __hip_atomic_fetch_add(c.bar, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            unsigned long long target = (old / (unsigned long long)c.G + 1ull) * (unsigned long long)c.G;
+            (void)__hip_atomic_fetch_add(c.bar, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long target = c.bar_next;
             unsigned spins = 0;
             while (__hip_atomic_load(c.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
                 __builtin_amdgcn_s_sleep(2);
@@ -103,10 +116,15 @@ __device__ void team_sync(Ctx& c) {
                     break;
                 }
             }
+            c.bar_next = target + (unsigned long long)c.G;
             if (c.local) asm volatile("buffer_inv sc1" ::: "memory");  // L1 (no L2 write-back)
             else __threadfence();
         }
         __syncthreads();
+        // profiling: team barriers per op, after the fixed slots
+        if (c.prof && c.tid == 0)
+            __hip_atomic_fetch_add(c.prof + c.pbase + PROF_EXTRA + c.cur_op, 1ull, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -121,6 +139,7 @@ __device__ __forceinline__ Ctx sub_team(const Ctx& c, int first, int size, unsig
     s.gtid = s.rank * NT + c.tid;
     s.gsize = size * NT;
     s.bar = bar;
+    if (size > 1 && c.tid == 0) s.bar_next = bar_first(bar, size);
     s.tred = tred;
     s.tpar = tpar;
     if (c.rank != 0) s.prof = nullptr;
@@ -360,7 +379,29 @@ __device__ __forceinline__ void quat_to_rot(P q, float* U) {
 }
 
 // Fixed-order gather (see engine_dev.h): each target row adds the sum of its
-// contributions, in plan order.  Called by all threads.
+// contributions, in plan order.  A thread loads each contribution's index
+// once and all its components with it (per component the terms are summed in
+// plan order, as one component at a time would).  Called by all threads.
+template <int CW>
+__device__ __forceinline__ void plan_gather_w(Ctx& c, gp<const int> off, gp<const int> idx, gp<float> sens,
+                                              gp<const float> con, int nt, int ss, int col, int cs) {
+    for (int t = c.gtid; t < nt; t += c.gsize) {
+        const int b = off[t], e = off[t + 1];
+        if (b == e) continue;
+        float acc[CW];
+#pragma unroll
+        for (int k = 0; k < CW; ++k) acc[k] = 0.f;
+#pragma unroll 2
+        for (int j = b; j < e; ++j) {
+            gp<const float> q = con + idx[j] * cs;
+#pragma unroll
+            for (int k = 0; k < CW; ++k) acc[k] += q[k];
+        }
+#pragma unroll
+        for (int k = 0; k < CW; ++k) sens[t * ss + col + k] += acc[k];
+    }
+}
+
 __device__ void plan_gather(Ctx& c, int plan) {
     if (plan < 0) return;
     gp<const int> h = gmem(c.CI + plan);
@@ -369,13 +410,20 @@ __device__ void plan_gather(Ctx& c, int plan) {
     gp<const int> idx = off + nt + 1;
     gp<float> sens = gmem(c.A + so);
     gp<const float> con = gmem((const float*)c.A + cb);
-    for (int t = c.gtid; t < nt; t += c.gsize) {
-        int b = off[t], e = off[t + 1];
-        if (b == e) continue;
-        for (int k = 0; k < cw; ++k) {
-            float acc = 0.f;
-            for (int j = b; j < e; ++j) acc += con[idx[j] * cs + k];
-            sens[t * ss + col + k] += acc;
+    switch (cw) {
+    case 1: plan_gather_w<1>(c, off, idx, sens, con, nt, ss, col, cs); break;
+    case 2: plan_gather_w<2>(c, off, idx, sens, con, nt, ss, col, cs); break;
+    case 3: plan_gather_w<3>(c, off, idx, sens, con, nt, ss, col, cs); break;
+    case 6: plan_gather_w<6>(c, off, idx, sens, con, nt, ss, col, cs); break;
+    default:
+        for (int t = c.gtid; t < nt; t += c.gsize) {
+            int b = off[t], e = off[t + 1];
+            if (b == e) continue;
+            for (int k = 0; k < cw; ++k) {
+                float acc = 0.f;
+                for (int j = b; j < e; ++j) acc += con[idx[j] * cs + k];
+                sens[t * ss + col + k] += acc;
+            }
         }
     }
     team_sync(c);

@@ -1480,7 +1480,8 @@ struct Batch {
     }
     bool profile = false;
     vector<unsigned long long*> prof;
-    int prof_len(int k) const { return sys[k].prog->n_germ() + sys[k].prog->n_deriv() + PROF_EXTRA; }
+    // wall-clock ticks per op, fixed slots, then team barriers per op (+1: outside the ops)
+    int prof_len(int k) const { return 2 * (sys[k].prog->n_germ() + sys[k].prog->n_deriv()) + PROF_EXTRA + 1; }
 
     void enable_profile(bool on) {
         for (auto p : prof) (void)hipFree(p);
@@ -1511,7 +1512,10 @@ struct Batch {
         if (team > 1) out += string("team of ") + std::to_string(team) + " workgroups, " +
                              (team_local ? "on one XCD (L2-local barriers)\n" : "spread over XCDs (device-scope barriers)\n");
         int ng = sys[k].prog->n_germ();
-        for (int i = 0; i < n; ++i) {
+        const int nops = ng + (int)T.deriv.size(), nb0 = nops + PROF_EXTRA;
+        unsigned long long nbar = 0;
+        for (int i = nb0; i < n; ++i) nbar += t[i];
+        for (int i = 0; i < nb0; ++i) {
             string label;
             if (i < ng) label = "value  " + T.nodes[T.germ[i + 1]].name;
             else if (i < ng + (int)T.deriv.size()) label = "deriv  " + T.nodes[T.deriv[i - ng]].name;
@@ -1528,9 +1532,16 @@ struct Batch {
                                            "BP: message passes", "BP: node passes", "BP: staging and marginals"};
                 label = ex[i - ng - (int)T.deriv.size()];
             }
-            snprintf(line, sizeof(line), "%-48s %12.6f s\n", label.c_str(), t[i] * tick);
+            if (i < nops)
+                snprintf(line, sizeof(line), "%-48s %12.6f s %8llu team barriers\n", label.c_str(), t[i] * tick,
+                         t[nb0 + i]);
+            else
+                snprintf(line, sizeof(line), "%-48s %12.6f s\n", label.c_str(), t[i] * tick);
             out += line;
         }
+        snprintf(line, sizeof(line), "%-48s %12llu\n%-48s %12llu\n", "team barriers outside the ops", t[n - 1],
+                 "team barriers in all (rank 0)", nbar);
+        out += line;
         for (size_t g = 0; g < T.graphs.size(); ++g) {
             int st[4];
             hip_check(hipMemcpy(st, sys[k].iarena + T.graphs[g].state, sizeof(st), hipMemcpyDeviceToHost), "state");

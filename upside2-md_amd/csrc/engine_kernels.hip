@@ -937,6 +937,7 @@ template <bool SPLIT>
 __device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mode) {
     for (int k = k0; k < k1;) {
         const OpRec& op = S.ops[k];
+        c.cur_op = k;
         if (c.G > 1 && op.solo) {
             const int k2 = solo_run_end(S, k, k1);
             if (c.rank == 0) {
@@ -1020,6 +1021,7 @@ __device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mod
 
 // One evaluation: value ops in germ order, then sensitivity pushes.
 __device__ void compute(Ctx& c, const SysDesc& S, int mode) {
+    c.cur_op = S.n_germ + S.n_deriv;  // outside the ops
     const int* zr = c.I + S.sens_zero;
     for (int r = 0; r < S.sens_zero_cnt; ++r) {
         float* p = c.A + zr[2 * r];
@@ -1030,6 +1032,7 @@ __device__ void compute(Ctx& c, const SysDesc& S, int mode) {
     if (!run_germ_range<true>(c, S, 0, S.n_germ, mode)) return;
     const int nd = S.n_germ + S.n_deriv;
     for (int k = S.n_germ; k < nd;) {
+        c.cur_op = k;
         if (c.G > 1 && S.ops[k].solo) {
             const int k2 = solo_run_end(S, k, nd);
             if (c.rank == 0) {
@@ -1240,10 +1243,12 @@ __device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& arg
     c.tid = (int)threadIdx.x;
     c.prof = rank == 0 ? S.prof : nullptr;
     c.pbase = S.n_germ + S.n_deriv;
+    c.cur_op = c.pbase;
     c.rank = rank; c.G = G;
     c.gtid = rank * NT + c.tid; c.gsize = G * NT;
     c.bar = reinterpret_cast<unsigned long long*>(S.iarena + S.bar);  // 16-byte aligned words
     c.bar_flag = reinterpret_cast<unsigned*>(S.iarena + S.bar + 2);
+    c.bar_next = (G > 1 && c.tid == 0) ? bar_first(c.bar, G) : 0ull;
     c.tred = S.arena + S.tred;
     c.tpar = 0;
     c.bar2 = reinterpret_cast<unsigned long long*>(S.iarena + S.bar + BAR_SUB1);
@@ -1295,6 +1300,7 @@ __device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& arg
             compute(c, S, 0);
             if (c.I[S.error]) return;  // capacity overflow: the host repeats the run
             t0 = ptick(c);
+            c.cur_op = S.n_germ + S.n_deriv;
             for (int i = c.gtid; i < S.n_atom * 3; i += c.gsize) {
                 float p = mom[i] - args.dt * der[i];
                 mom[i] = p;

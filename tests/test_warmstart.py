@@ -233,13 +233,16 @@ def test_warm_start_trained_on_gpu_reduces_iterations(ue, chain_file):
             hb.set_warm_start(net.params() if name == "trained" else ue.WarmStartNet(seed=1).params())
             hb.set_bp_schedule(2, 1e-5)
         st0 = [hb.bp_stats(s) for s in range(len(held))]
+        h0 = sum(hb.bp_histogram(s) for s in range(len(held)))
         hb.run(6)
         st1 = [hb.bp_stats(s) for s in range(len(held))]
+        hist = sum(hb.bp_histogram(s) for s in range(len(held))) - h0
         it = sum(a["total"] - c["total"] for a, c in zip(st1, st0))
         so = sum(a["solves"] - c["solves"] for a, c in zip(st1, st0))
         ws = [hb.bp_warm_stats(s) for s in range(len(held))]
-        runs[name] = (it / so, sum(w["fallbacks"] for w in ws))
-        say("%-9s iterations per solve %.2f, fallbacks %d" % (name, *runs[name]))
+        p99 = int(np.searchsorted(np.cumsum(hist), 0.99 * hist.sum()))
+        runs[name] = (it / so, sum(w["fallbacks"] for w in ws), p99)
+        say("%-9s iterations per solve %.2f, fallbacks %d, p99 %d" % (name, *runs[name]))
     assert runs["trained"][1] == 0
     assert runs["trained"][0] < runs["plain"][0]
     assert runs["trained"][0] <= runs["untrained"][0]

@@ -36,6 +36,7 @@ struct Ctx {
     unsigned long long* prof;  // wall-clock accumulators (null: profiling off)
     int pbase;                 // first fixed profile slot
     int cur_op;                // program op being run (team barriers are counted per op while profiling)
+    int err;                   // int arena offset of the error word this (sub-)team sets and reads
     int rank, G;       // workgroup rank in the team, team size
     int gtid, gsize;   // thread in the team, team threads
     unsigned long long* bar;  // team barrier arrival counter

@@ -195,6 +195,9 @@ struct OpRec {
     int rot;               // index into RotamerDev table (-1 if none)
     int solo;              // run by one workgroup of the team (small elementwise op)
     int overlap_end;       // rotamer: germ ops [this+1, overlap_end) run beside the BP solve (0: none)
+    int lane_end;          // first op of a two-lane section: germ ops [this, lane_end) run on two
+                           // half-teams by lane (0: not a section start)
+    int lane;              // in a section: 0 runs on ranks [0, G/2), 1 on [G/2, G)
 };
 
 struct SysDesc {

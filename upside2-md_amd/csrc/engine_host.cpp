@@ -1354,6 +1354,100 @@ struct Program {
     }
     ~Program() { release(); }
 
+    // Value-pass order for two-lane sections (no counterpart in the reference,
+    // whose value pass runs its exec levels in order, src/deriv_engine.cpp:127-178).
+    // Any topological order computes the same values; only the order in which
+    // potential nodes add into their parents' sensitivities changes, which
+    // moves those sums by rounding.
+    //  1. Elementwise ops whose ancestors are all elementwise come first (a
+    //     stable partition, which keeps both parts in topological order): one
+    //     run of them, then the pair-graph ops.
+    //  2. From the first pair-graph op up to the first rotamer op, ops are
+    //     dealt to two lanes: an op that descends from an op of one lane, adds
+    //     into the same sensitivities as a potential of that lane, or shares
+    //     its pair graph, joins that lane; an op tied to both lanes ends the
+    //     section; a free op joins the lane with less work (a pair-graph op
+    //     counts 10, an elementwise op 1).  The two halves of a team run the
+    //     two lanes at once.
+    vector<int> lane_of;                        // node -> lane (0 outside sections)
+    vector<std::pair<int, int>> lane_sections;  // [first op, end op) in program order
+    void order_germ_for_lanes() {
+        const int nn = (int)T.nodes.size();
+        lane_of.assign(nn, 0);
+        lane_sections.clear();
+        auto elementwise = [&](int i) {
+            const OpRec& o = T.ops[T.nodes[i].op];
+            return o.graph < 0 && o.rot < 0;
+        };
+        // anc[i][q]: node i descends from node q (pos, node 0, left out)
+        vector<vector<char>> anc(nn, vector<char>(nn, 0));
+        for (size_t k = 1; k < T.germ.size(); ++k) {
+            const int i = T.germ[k];
+            for (int p : T.nodes[i].parents) {
+                if (p == 0) continue;
+                anc[i][p] = 1;
+                for (int q = 0; q < nn; ++q) anc[i][q] |= anc[p][q];
+            }
+        }
+        vector<int> first, rest;
+        for (size_t k = 1; k < T.germ.size(); ++k) {
+            const int i = T.germ[k];
+            bool pure = elementwise(i);
+            for (int q = 0; q < nn && pure; ++q)
+                if (anc[i][q] && !elementwise(q)) pure = false;
+            (pure ? first : rest).push_back(i);
+        }
+        vector<int> in_lane[2], deferred;
+        int work[2] = {0, 0};
+        auto tied = [&](int i, const vector<int>& lane) {
+            const Node& ni = T.nodes[i];
+            const OpRec& oi = T.ops[ni.op];
+            for (int j : lane) {
+                if (anc[i][j]) return true;
+                if (oi.graph >= 0 && oi.graph == T.ops[T.nodes[j].op].graph) return true;
+                if (ni.potential && T.nodes[j].potential)
+                    for (int p : ni.parents)
+                        for (int q : T.nodes[j].parents)
+                            if (p == q) return true;
+            }
+            return false;
+        };
+        size_t r = 0;
+        for (; r < rest.size(); ++r) {
+            const int i = rest[r];
+            if (T.ops[T.nodes[i].op].kind == OP_ROTAMER) break;
+            // tied to both lanes, or descending from such an op: after the section
+            bool defer = tied(i, in_lane[0]) && tied(i, in_lane[1]);
+            for (int j : deferred) defer = defer || anc[i][j];
+            if (defer) { deferred.push_back(i); continue; }
+            const bool t0 = tied(i, in_lane[0]), t1 = tied(i, in_lane[1]);
+            const int lane = t0 ? 0 : t1 ? 1 : (work[1] < work[0] ? 1 : 0);
+            in_lane[lane].push_back(i);
+            work[lane] += elementwise(i) ? 1 : 10;
+            lane_of[i] = lane;
+        }
+        // a section needs pair-graph work in both lanes
+        auto has_pair = [&](const vector<int>& lane) {
+            for (int i : lane)
+                if (!elementwise(i)) return true;
+            return false;
+        };
+        T.germ.assign(1, 0);
+        T.germ.insert(T.germ.end(), first.begin(), first.end());
+        if (!(has_pair(in_lane[0]) && has_pair(in_lane[1]))) {
+            lane_of.assign(nn, 0);
+            T.germ.insert(T.germ.end(), rest.begin(), rest.end());
+            return;
+        }
+        // the section's ops in their program order, then the deferred ops, then the rest
+        const int k0 = (int)first.size();
+        for (size_t q = 0; q < r; ++q)
+            if (std::find(deferred.begin(), deferred.end(), rest[q]) == deferred.end()) T.germ.push_back(rest[q]);
+        lane_sections.push_back({k0, (int)T.germ.size() - 1});
+        T.germ.insert(T.germ.end(), deferred.begin(), deferred.end());
+        T.germ.insert(T.germ.end(), rest.begin() + r, rest.end());
+    }
+
     void load(int n_atom, double cap_scale) {
         H5 h5(path);
         if (n_atom <= 0) n_atom = (int)h5.shape("/input/pos")[0];
@@ -1368,6 +1462,11 @@ struct Program {
         T.deriv.erase(std::remove_if(T.deriv.begin(), T.deriv.end(),
                                      [&](int i) { return !has_deriv_step(T.ops[T.nodes[i].op].kind); }),
                       T.deriv.end());
+        const char* so = std::getenv("UPSIDE_SOLO_OPS");
+        const bool solo_ok = !(so && so[0] == '0');
+        const char* ln = std::getenv("UPSIDE_LANES");
+        if (solo_ok && !(ln && ln[0] == '0')) order_germ_for_lanes();
+        else lane_of.assign(T.nodes.size(), 0), lane_sections.clear();
         prog.clear();
         prog_node.clear();
         for (size_t k = 1; k < T.germ.size(); ++k) {
@@ -1381,9 +1480,12 @@ struct Program {
         // ops without a pair graph or rotamer solve are elementwise over at most
         // a few thousand elements: one workgroup runs them (UPSIDE_SOLO_OPS=0:
         // the whole team)
-        const char* so = std::getenv("UPSIDE_SOLO_OPS");
-        const bool solo_ok = !(so && so[0] == '0');
         for (auto& o : prog) o.solo = (solo_ok && o.graph < 0 && o.rot < 0) ? 1 : 0;
+        for (size_t k = 0; k + 1 < T.germ.size(); ++k) {
+            prog[k].lane = lane_of[T.germ[k + 1]];
+            prog[k].lane_end = 0;
+        }
+        for (auto& sec : lane_sections) prog[sec.first].lane_end = sec.second;
         // A rotamer node has no coordinate output, so the germ ops after it
         // usually do not depend on it: those that neither descend from it nor
         // push sensitivities into its inputs run beside its BP solve

@@ -532,7 +532,7 @@ __device__ void infer_bwd(Ctx& c, const OpRec& op) {
 // word is set, the remaining ops are skipped and the host grows the
 // capacities and repeats the run.
 __device__ void graph_overflow(Ctx& c, const SysDesc& S) {
-    if (c.gtid == 0) c.I[S.error] = 1;
+    if (c.gtid == 0) c.I[c.err] = 1;
     team_sync(c);
 }
 
@@ -926,6 +926,43 @@ __device__ __forceinline__ int solo_run_end(const SysDesc& S, int k, int end) {
     return k;
 }
 
+// The ops of one lane of a two-lane section [k0, k1) on half-team s: ops of
+// the other lane are skipped (they run on the other half, and the host made
+// the two lanes independent: neither reads what the other writes).  A lane's
+// solo ops up to its next team op run on the half's rank 0.  Returns false
+// when an op set the half's error word.
+__device__ bool run_lane(Ctx& s, const SysDesc& S, int k0, int k1, int lane, int mode) {
+    for (int k = k0; k < k1;) {
+        const OpRec& op = S.ops[k];
+        if (op.lane != lane) { ++k; continue; }
+        s.cur_op = k;
+        if (s.G > 1 && op.solo) {
+            int k2 = k;
+            while (k2 < k1 && (S.ops[k2].lane != lane || S.ops[k2].solo)) ++k2;
+            if (s.rank == 0) {
+                Ctx l = solo_ctx(s);
+                for (int q = k; q < k2; ++q) {
+                    if (S.ops[q].lane != lane) continue;
+                    unsigned long long t0 = ptick(l);
+                    run_value(l, S, S.ops[q], mode);
+                    prof_op(l, q, t0);
+                    if (l.I[l.err]) break;
+                }
+            }
+            team_sync(s);
+            if (s.I[s.err]) return false;
+            k = k2;
+            continue;
+        }
+        unsigned long long t0 = ptick(s);
+        run_value(s, S, op, mode);
+        prof_op(s, k, t0);
+        if (s.I[s.err]) return false;
+        ++k;
+    }
+    return true;
+}
+
 // Value ops [k0, k1) in germ order on the team c.  Ops marked solo
 // (elementwise work over at most a few thousand elements) run on rank 0 of
 // the team alone: a run of them goes back to back with workgroup barriers and
@@ -938,6 +975,31 @@ __device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mod
     for (int k = k0; k < k1;) {
         const OpRec& op = S.ops[k];
         c.cur_op = k;
+        if (SPLIT && op.lane_end > k && op.lane_end <= k1 && c.G >= 4) {
+            // two-lane section: each half-team runs its lane with its own
+            // barrier counter, reduction scratch and error word; the join
+            // barrier makes both halves' error words readable by everyone
+            const int gb = c.G / 2;
+            if (c.rank < gb) {
+                Ctx b = sub_team(c, 0, gb, c.bar_lo, c.tred_lo, c.tpar_lo);
+                b.err = S.error + 1;
+                run_lane(b, S, k, op.lane_end, 0, mode);
+                c.tpar_lo = b.tpar;
+            } else {
+                Ctx o = sub_team(c, gb, c.G - gb, c.bar_hi, c.tred_hi, c.tpar_hi);
+                o.err = S.error + 2;
+                run_lane(o, S, k, op.lane_end, 1, mode);
+                c.tpar_hi = o.tpar;
+            }
+            team_sync(c);
+            if (c.I[S.error + 1] | c.I[S.error + 2]) {
+                if (c.gtid == 0) c.I[c.err] = 1;
+                team_sync(c);
+                return false;
+            }
+            k = op.lane_end;
+            continue;
+        }
         if (c.G > 1 && op.solo) {
             const int k2 = solo_run_end(S, k, k1);
             if (c.rank == 0) {
@@ -946,11 +1008,11 @@ __device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mod
                     unsigned long long t0 = ptick(l);
                     run_value(l, S, S.ops[q], mode);
                     prof_op(l, q, t0);
-                    if (l.I[S.error]) break;
+                    if (l.I[l.err]) break;
                 }
             }
             team_sync(c);
-            if (c.I[S.error]) return false;
+            if (c.I[c.err]) return false;
             k = k2;
             continue;
         }
@@ -981,7 +1043,7 @@ __device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mod
                     team_sync(c);  // join; an error in the overlapped ops is read after the rotamer
                     rotamer_post(c, S, op, G, R, mode, it);
                     prof_op(c, k, t0);
-                    if (c.I[S.error]) return false;
+                    if (c.I[c.err]) return false;
                     k = op.overlap_end;
                     continue;
                 }
@@ -990,7 +1052,7 @@ __device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mod
                 pacc(c, PROF_BP, t1);
                 rotamer_post(c, S, op, G, R, mode, it);
                 prof_op(c, k, t0);
-                if (c.I[S.error]) return false;
+                if (c.I[c.err]) return false;
                 ++k;
                 continue;
             }
@@ -1006,14 +1068,14 @@ __device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mod
             team_sync(c);  // join; an error in the overlapped ops is read after the rotamer
             rotamer_post(c, S, op, G, R, mode, it);
             prof_op(c, k, t0);
-            if (c.I[S.error]) return false;
+            if (c.I[c.err]) return false;
             k = op.overlap_end;
             continue;
         }
         unsigned long long t0 = ptick(c);
         run_value(c, S, op, mode);
         prof_op(c, k, t0);
-        if (c.I[S.error]) return false;  // uniform: read after the op's final barrier
+        if (c.I[c.err]) return false;  // uniform: read after the op's final barrier
         ++k;
     }
     return true;
@@ -1244,6 +1306,7 @@ __device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& arg
     c.prof = rank == 0 ? S.prof : nullptr;
     c.pbase = S.n_germ + S.n_deriv;
     c.cur_op = c.pbase;
+    c.err = S.error;
     c.rank = rank; c.G = G;
     c.gtid = rank * NT + c.tid; c.gsize = G * NT;
     c.bar = reinterpret_cast<unsigned long long*>(S.iarena + S.bar);  // 16-byte aligned words
@@ -1279,7 +1342,7 @@ __device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& arg
         int round = args.first_round + rr;
         if (args.frame_interval > 0 && round % args.frame_interval == 0) {
             compute(c, S, 1);
-            if (c.I[S.error]) return;
+            if (c.I[c.err]) return;
             if (args.frame_pot && c.gtid == 0 && nf < args.n_frames)
                 args.frame_pot[sys * args.n_frames + nf] = total_potential(c, S);
             ++nf;
@@ -1298,7 +1361,7 @@ __device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& arg
         pacc(c, PROF_THERMO, t0);
         for (int stage = 0; stage < 3; ++stage) {
             compute(c, S, 0);
-            if (c.I[S.error]) return;  // capacity overflow: the host repeats the run
+            if (c.I[c.err]) return;  // capacity overflow: the host repeats the run
             t0 = ptick(c);
             c.cur_op = S.n_germ + S.n_deriv;
             for (int i = c.gtid; i < S.n_atom * 3; i += c.gsize) {
@@ -1314,7 +1377,7 @@ __device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& arg
         // rounds is round + 1
         if (args.rex.interval > 0 && (long long)round + 1 == args.rex.state[1]) {
             rex_attempt(c, S, args.rex, sys, args.n_sys, (long long)round + 1);
-            if (c.I[S.error]) return;
+            if (c.I[c.err]) return;
         }
     }
 }

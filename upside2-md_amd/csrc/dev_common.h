@@ -37,6 +37,7 @@ struct Ctx {
     int pbase;                 // first fixed profile slot
     int cur_op;                // program op being run (team barriers are counted per op while profiling)
     int err;                   // int arena offset of the error word this (sub-)team sets and reads
+    int early_done;            // this evaluation ran the early deriv ops beside the rotamer solve
     int rank, G;       // workgroup rank in the team, team size
     int gtid, gsize;   // thread in the team, team threads
     unsigned long long* bar;  // team barrier arrival counter

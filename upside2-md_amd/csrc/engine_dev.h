@@ -198,6 +198,7 @@ struct OpRec {
     int lane_end;          // first op of a two-lane section: germ ops [this, lane_end) run on two
                            // half-teams by lane (0: not a section start)
     int lane;              // in a section: 0 runs on ranks [0, G/2), 1 on [G/2, G)
+    int early;             // deriv op that may run beside the rotamer solve, after its overlap ops
 };
 
 struct SysDesc {

@@ -1520,6 +1520,38 @@ struct Program {
             }
             prog[k].overlap_end = j > k + 1 ? j : 0;
         }
+        // Deriv ops whose sensitivities are complete once the ops before the
+        // rotamer solve and its overlap ops have run (every consumer is a
+        // potential among those, other than the rotamer, or itself such a
+        // deriv op) run beside the solve, after the overlap ops, in deriv
+        // order (UPSIDE_EARLY_DERIV=0: never).  The solve writes no
+        // sensitivities, and the rotamer's own push into its inputs comes
+        // after the join, so nothing they touch is touched at the same time.
+        for (auto& o : prog) o.early = 0;
+        const char* ed = std::getenv("UPSIDE_EARLY_DERIV");
+        const bool early_ok = overlap_ok && !(ed && ed[0] == '0');
+        for (int k = 0; k < ng && early_ok; ++k) {
+            if (prog[k].kind != OP_ROTAMER || prog[k].overlap_end <= k + 1) continue;
+            const int e = prog[k].overlap_end;
+            vector<int> germ_pos(T.nodes.size(), -1);
+            for (int q = 0; q < ng; ++q) germ_pos[T.germ[q + 1]] = q;
+            vector<char> early(T.nodes.size(), 0);
+            for (size_t d = 0; d < T.deriv.size(); ++d) {
+                const int i = T.deriv[d];
+                bool ok = true;
+                for (int ch : T.nodes[i].children) {
+                    if (T.nodes[ch].potential)
+                        ok = ok && germ_pos[ch] >= 0 && germ_pos[ch] < e && germ_pos[ch] != k;
+                    else
+                        ok = ok && early[ch];
+                }
+                if (ok) {
+                    early[i] = 1;
+                    prog[ng + d].early = 1;
+                }
+            }
+            break;  // beside the first rotamer solve only
+        }
         auto up = [](void** dst, const void* src, size_t bytes, const char* what) {
             hip_check(hipMalloc(dst, std::max(bytes, (size_t)16)), what);
             if (bytes) hip_check(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), what);

@@ -926,6 +926,39 @@ __device__ __forceinline__ int solo_run_end(const SysDesc& S, int k, int end) {
     return k;
 }
 
+// The deriv ops marked early, in deriv order, on the (sub-)team s that ran the
+// rotamer's overlap ops: their sensitivities are complete once those ops are
+// done, and nothing the solve or the rotamer's sensitivity push touches is
+// read or written by them.  Runs of solo ops go to s's rank 0.
+__device__ void run_early_derivs(Ctx& s, const SysDesc& S) {
+    const int nd = S.n_germ + S.n_deriv;
+    for (int k = S.n_germ; k < nd;) {
+        const OpRec& op = S.ops[k];
+        if (!op.early) { ++k; continue; }
+        s.cur_op = k;
+        if (s.G > 1 && op.solo) {
+            int k2 = k;
+            while (k2 < nd && (!S.ops[k2].early || S.ops[k2].solo)) ++k2;
+            if (s.rank == 0) {
+                Ctx l = solo_ctx(s);
+                for (int q = k; q < k2; ++q) {
+                    if (!S.ops[q].early) continue;
+                    unsigned long long t0 = ptick(l);
+                    run_deriv(l, S, S.ops[q]);
+                    prof_op(l, q, t0);
+                }
+            }
+            team_sync(s);
+            k = k2;
+            continue;
+        }
+        unsigned long long t0 = ptick(s);
+        run_deriv(s, S, op);
+        prof_op(s, k, t0);
+        ++k;
+    }
+}
+
 // The ops of one lane of a two-lane section [k0, k1) on half-team s: ops of
 // the other lane are skipped (they run on the other half, and the host made
 // the two lanes independent: neither reads what the other writes).  A lane's
@@ -1016,56 +1049,52 @@ __device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mod
             k = k2;
             continue;
         }
-        if (SPLIT && op.kind == OP_ROTAMER && c.G > 1 && op.overlap_end > k + 1 && op.overlap_end <= k1) {
+        if (SPLIT && op.kind == OP_ROTAMER && op.overlap_end > k + 1 && op.overlap_end <= k1) {
+            // A rotamer op with an overlap range: the ops of the range and the
+            // early deriv ops run beside the solve where the team can split,
+            // else between the solve and the rotamer's sensitivity push.
+            // Either way they come after the solve's start and before the push,
+            // so every sum sees its terms in the same order: same bits on
+            // every team size and layout.
             unsigned long long t0 = ptick(c);
             const PairGraph& G = S.graphs[op.graph];
             const RotamerDev& R = S.rots[op.rot];
             if (!rotamer_pre(c, S, op, G, R)) return false;
             int it = 0;
-            if (!bp_on_one_wg(c, R, c.I[R.stats + 4], c.bp_layout)) {
-                // the state does not fit one workgroup's LDS: with four or
-                // more workgroups the first half solves while the second half
-                // evaluates the independent ops (bp_layout 0 and 3); else
-                // the whole team solves and the ops follow in program order
-                if (c.bp_layout != 1 && c.G >= 4) {
-                    const int gb = c.G / 2;
-                    unsigned long long t1 = ptick(c);
-                    if (c.rank < gb) {
-                        Ctx b = sub_team(c, 0, gb, c.bar_lo, c.tred_lo, c.tpar_lo);
-                        it = rot_solve_team(b, R);
-                        c.tpar_lo = b.tpar;
-                    } else {
-                        Ctx o = sub_team(c, gb, c.G - gb, c.bar_hi, c.tred_hi, c.tpar_hi);
-                        run_germ_range<false>(o, S, k + 1, op.overlap_end, mode);
-                        c.tpar_hi = o.tpar;
-                    }
-                    pacc(c, PROF_BP, t1);
-                    team_sync(c);  // join; an error in the overlapped ops is read after the rotamer
-                    rotamer_post(c, S, op, G, R, mode, it);
-                    prof_op(c, k, t0);
-                    if (c.I[c.err]) return false;
-                    k = op.overlap_end;
-                    continue;
+            const bool one_wg = bp_on_one_wg(c, R, c.I[R.stats + 4], c.bp_layout);
+            unsigned long long t1 = ptick(c);
+            if (c.G > 1 && !one_wg && c.bp_layout != 1 && c.G >= 4) {
+                // the state does not fit one workgroup's LDS: the first half
+                // solves while the second half runs the range
+                const int gb = c.G / 2;
+                if (c.rank < gb) {
+                    Ctx b = sub_team(c, 0, gb, c.bar_lo, c.tred_lo, c.tpar_lo);
+                    it = rot_solve_team(b, R);
+                    c.tpar_lo = b.tpar;
+                } else {
+                    Ctx o = sub_team(c, gb, c.G - gb, c.bar_hi, c.tred_hi, c.tpar_hi);
+                    if (run_germ_range<false>(o, S, k + 1, op.overlap_end, mode)) run_early_derivs(o, S);
+                    c.tpar_hi = o.tpar;
                 }
-                unsigned long long t1 = ptick(c);
-                it = rot_solve_team(c, R);
-                pacc(c, PROF_BP, t1);
-                rotamer_post(c, S, op, G, R, mode, it);
-                prof_op(c, k, t0);
-                if (c.I[c.err]) return false;
-                ++k;
-                continue;
-            }
-            if (c.rank == 0) {
-                unsigned long long t1 = ptick(c);
-                it = rot_solve_wg(c, R);
-                pacc(c, PROF_BP, t1);
+                team_sync(c);  // join; an error in the range is read after the rotamer
+            } else if (c.G > 1 && one_wg) {
+                // rank 0 solves in its LDS while ranks 1..G-1 run the range
+                if (c.rank == 0) {
+                    it = rot_solve_wg(c, R);
+                } else {
+                    Ctx s = sub_team(c);
+                    if (run_germ_range<false>(s, S, k + 1, op.overlap_end, mode)) run_early_derivs(s, S);
+                    c.tpar2 = s.tpar;
+                }
+                team_sync(c);  // join; an error in the range is read after the rotamer
             } else {
-                Ctx s = sub_team(c);
-                run_germ_range<false>(s, S, k + 1, op.overlap_end, mode);
-                c.tpar2 = s.tpar;
+                // one workgroup, or the whole team solves: the range follows
+                it = rot_solve(c, R, c.bp_layout);
+                if (!run_germ_range<false>(c, S, k + 1, op.overlap_end, mode)) return false;
+                run_early_derivs(c, S);
             }
-            team_sync(c);  // join; an error in the overlapped ops is read after the rotamer
+            pacc(c, PROF_BP, t1);
+            c.early_done = 1;
             rotamer_post(c, S, op, G, R, mode, it);
             prof_op(c, k, t0);
             if (c.I[c.err]) return false;
@@ -1084,6 +1113,7 @@ __device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mod
 // One evaluation: value ops in germ order, then sensitivity pushes.
 __device__ void compute(Ctx& c, const SysDesc& S, int mode) {
     c.cur_op = S.n_germ + S.n_deriv;  // outside the ops
+    c.early_done = 0;
     const int* zr = c.I + S.sens_zero;
     for (int r = 0; r < S.sens_zero_cnt; ++r) {
         float* p = c.A + zr[2 * r];
@@ -1095,11 +1125,13 @@ __device__ void compute(Ctx& c, const SysDesc& S, int mode) {
     const int nd = S.n_germ + S.n_deriv;
     for (int k = S.n_germ; k < nd;) {
         c.cur_op = k;
+        if (c.early_done && S.ops[k].early) { ++k; continue; }  // ran beside the rotamer solve
         if (c.G > 1 && S.ops[k].solo) {
             const int k2 = solo_run_end(S, k, nd);
             if (c.rank == 0) {
                 Ctx l = solo_ctx(c);
                 for (int q = k; q < k2; ++q) {
+                    if (c.early_done && S.ops[q].early) continue;
                     unsigned long long t0 = ptick(l);
                     run_deriv(l, S, S.ops[q]);
                     prof_op(l, q, t0);

@@ -237,7 +237,7 @@ enum {
     PROF_BP = 0, PROF_REBUILD = 1, PROF_THERMO = 2, PROF_INTEGRATE = 3, PROF_PG_REBUILD = 4, PROF_PG_EDGES = 5,
     PROF_PG_GATHER = 6, PROF_ROT_FILL = 7, PROF_ROT_REBUILD = 8, PROF_RB_COUNT = 9, PROF_RB_FILL = 10,
     PROF_RB_MAP = 11, PROF_RB_GRID = 12, PROF_RB_SCAN = 13, PROF_BP_EDGE = 14, PROF_BP_NODE = 15,
-    PROF_BP_SETUP = 16, PROF_EXTRA = 17
+    PROF_BP_SETUP = 16, PROF_BESIDE = 17, PROF_EXTRA = 18
 };
 
 // Replica exchange inside a persistent launch (the swap step of

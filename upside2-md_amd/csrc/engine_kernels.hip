@@ -1020,6 +1020,7 @@ __device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mod
                 c.tpar_lo = b.tpar;
             } else {
                 Ctx o = sub_team(c, gb, c.G - gb, c.bar_hi, c.tred_hi, c.tpar_hi);
+                if (o.rank == 0) o.prof = S.prof;  // profiling: lane 1's ops are timed by its rank 0
                 o.err = S.error + 2;
                 run_lane(o, S, k, op.lane_end, 1, mode);
                 c.tpar_hi = o.tpar;
@@ -1049,7 +1050,7 @@ __device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mod
             k = k2;
             continue;
         }
-        if (SPLIT && op.kind == OP_ROTAMER && op.overlap_end > k + 1 && op.overlap_end <= k1) {
+        if (SPLIT && op.kind == OP_ROTAMER && op.overlap_end > k && op.overlap_end <= k1) {
             // A rotamer op with an overlap range: the ops of the range and the
             // early deriv ops run beside the solve where the team can split,
             // else between the solve and the rotamer's sensitivity push.
@@ -1073,8 +1074,14 @@ __device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mod
                     c.tpar_lo = b.tpar;
                 } else {
                     Ctx o = sub_team(c, gb, c.G - gb, c.bar_hi, c.tred_hi, c.tpar_hi);
+                    if (o.rank == 0) o.prof = S.prof;  // profiling: the ops of this half are timed by its rank 0
+                    const unsigned long long t2 = S.prof ? wall_clock64() : 0ull;
                     if (run_germ_range<false>(o, S, k + 1, op.overlap_end, mode)) run_early_derivs(o, S);
                     c.tpar_hi = o.tpar;
+                    // profiling: the half's own time, by its first thread (rank 0 keeps the others)
+                    if (S.prof && o.gtid == 0)
+                        __hip_atomic_fetch_add(S.prof + c.pbase + PROF_BESIDE, wall_clock64() - t2, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
                 }
                 team_sync(c);  // join; an error in the range is read after the rotamer
             } else if (c.G > 1 && one_wg) {
@@ -1083,8 +1090,13 @@ __device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mod
                     it = rot_solve_wg(c, R);
                 } else {
                     Ctx s = sub_team(c);
+                    if (s.rank == 0) s.prof = S.prof;
+                    const unsigned long long t2 = S.prof ? wall_clock64() : 0ull;
                     if (run_germ_range<false>(s, S, k + 1, op.overlap_end, mode)) run_early_derivs(s, S);
                     c.tpar2 = s.tpar;
+                    if (S.prof && s.gtid == 0)
+                        __hip_atomic_fetch_add(S.prof + c.pbase + PROF_BESIDE, wall_clock64() - t2, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
                 }
                 team_sync(c);  // join; an error in the range is read after the rotamer
             } else {

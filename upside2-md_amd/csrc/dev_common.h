@@ -38,6 +38,7 @@ struct Ctx {
     int cur_op;                // program op being run (team barriers are counted per op while profiling)
     int err;                   // int arena offset of the error word this (sub-)team sets and reads
     int early_done;            // this evaluation ran the early deriv ops beside the rotamer solve
+    int rot_graph_done;        // this evaluation refreshed the rotamer pair graph in a section lane
     int rank, G;       // workgroup rank in the team, team size
     int gtid, gsize;   // thread in the team, team threads
     unsigned long long* bar;  // team barrier arrival counter

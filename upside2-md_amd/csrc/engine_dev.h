@@ -199,6 +199,8 @@ struct OpRec {
                            // half-teams by lane (0: not a section start)
     int lane;              // in a section: 0 runs on ranks [0, G/2), 1 on [G/2, G)
     int early;             // deriv op that may run beside the rotamer solve, after its overlap ops
+    int rot_graph_op;      // first op of a section: the rotamer op whose pair-list refresh and edge
+    int rot_graph_lane;    // evaluation lane rot_graph_lane runs first (-1: none)
 };
 
 struct SysDesc {

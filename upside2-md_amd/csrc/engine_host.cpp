@@ -1371,12 +1371,14 @@ struct Program {
     //     every descendant of these are deferred past the section, in their
     //     order.  The two halves of a team run the two lanes at once.
     vector<int> lane_of;                        // node -> lane (0 outside sections)
+    int rot_graph_node = -1;                    // rotamer whose pair graph lane 0 of the section refreshes
     vector<std::pair<int, int>> lane_sections;  // [first op, end op) in program order
     // rough relative cost of a pair-graph op by its edge kind (a 500-residue
     // chain: environment graphs ~140 us per step, coverage ~90, backbone and
     // hbond ~50, hbond environment ~35)
     int lane_weight(int g) const {
         switch (T.graphs[g].kind) {
+        case EDGE_ROTAMER: return 14;
         case EDGE_ENV_WEIGHTED: return 14;
         case EDGE_HBOND_COVERAGE: return 9;
         case EDGE_ENV_HBOND: return 4;
@@ -1411,6 +1413,24 @@ struct Program {
         }
         vector<int> in_lane[2], deferred;
         int work[2] = {0, 0};
+        // the first rotamer's pair-list refresh and edge evaluation read only
+        // its bead placements: when those run before the section, lane 0
+        // starts with them
+        rot_graph_node = -1;
+        for (int i : rest) {
+            const OpRec& o = T.ops[T.nodes[i].op];
+            if (o.kind != OP_ROTAMER) continue;
+            const PairGraph& G = T.graphs[o.graph];
+            bool ready = true;
+            for (int p : T.nodes[i].parents)
+                if (T.nodes[p].out == G.out1 || T.nodes[p].out == G.out2)
+                    ready = ready && std::find(first.begin(), first.end(), p) != first.end();
+            if (ready) {
+                rot_graph_node = i;
+                work[0] += lane_weight(o.graph);
+            }
+            break;
+        }
         auto tied = [&](int i, const vector<int>& lane) {
             const Node& ni = T.nodes[i];
             const OpRec& oi = T.ops[ni.op];
@@ -1456,6 +1476,7 @@ struct Program {
         T.germ.insert(T.germ.end(), first.begin(), first.end());
         if (!(has_pair(in_lane[0]) && has_pair(in_lane[1]))) {
             lane_of.assign(nn, 0);
+            rot_graph_node = -1;
             T.germ.insert(T.germ.end(), rest.begin(), rest.end());
             return;
         }
@@ -1486,7 +1507,7 @@ struct Program {
         const bool solo_ok = !(so && so[0] == '0');
         const char* ln = std::getenv("UPSIDE_LANES");
         if (solo_ok && !(ln && ln[0] == '0')) order_germ_for_lanes();
-        else lane_of.assign(T.nodes.size(), 0), lane_sections.clear();
+        else lane_of.assign(T.nodes.size(), 0), lane_sections.clear(), rot_graph_node = -1;
         prog.clear();
         prog_node.clear();
         for (size_t k = 1; k < T.germ.size(); ++k) {
@@ -1505,7 +1526,16 @@ struct Program {
             prog[k].lane = lane_of[T.germ[k + 1]];
             prog[k].lane_end = 0;
         }
-        for (auto& sec : lane_sections) prog[sec.first].lane_end = sec.second;
+        for (auto& o : prog) o.rot_graph_op = o.rot_graph_lane = -1;
+        for (auto& sec : lane_sections) {
+            prog[sec.first].lane_end = sec.second;
+            for (int q = sec.second; q < (int)T.germ.size() - 1 && rot_graph_node >= 0; ++q)
+                if (prog_node[q] == rot_graph_node) {
+                    prog[sec.first].rot_graph_op = q;
+                    prog[sec.first].rot_graph_lane = 0;
+                    break;
+                }
+        }
         // A rotamer node has no coordinate output, so the germ ops after it
         // usually do not depend on it: those that neither descend from it nor
         // push sensitivities into its inputs run beside its BP solve

@@ -810,7 +810,10 @@ __device__ void backbone_fwd(Ctx& c, const SysDesc& S, const OpRec& op, const Pa
 // ---- rotamer (src/rotamer.cpp:840-850) ----------------------------------------------
 // Rotamer node, before the solve: cache check and rebuild, edge energies,
 // BP fill.  Returns false on capacity overflow (error word set, team synced).
-__device__ bool rotamer_pre(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G, const RotamerDev& R) {
+// The rotamer's pair list (refreshed when the beads moved past half the
+// buffer, with the residue-pair candidates) and its edge energies.  Reads only
+// the bead placements; false on a capacity overflow.
+__device__ bool rotamer_graph(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G, const RotamerDev& R) {
     unsigned long long t0 = ptick(c);
     bool rebuild = pg_pack_and_check(c, G);
     if (rebuild) {
@@ -824,7 +827,12 @@ __device__ bool rotamer_pre(Ctx& c, const SysDesc& S, const OpRec& op, const Pai
     t0 = ptick(c);
     pg_edges(c, G, op);
     pacc(c, PROF_PG_EDGES, t0);
-    t0 = ptick(c);
+    return true;
+}
+
+__device__ bool rotamer_pre(Ctx& c, const SysDesc& S, const OpRec& op, const PairGraph& G, const RotamerDev& R) {
+    if (!c.rot_graph_done && !rotamer_graph(c, S, op, G, R)) return false;
+    unsigned long long t0 = ptick(c);
     rot_fill(c, G, R);
     pacc(c, PROF_ROT_FILL, t0);
     return true;
@@ -965,6 +973,12 @@ __device__ void run_early_derivs(Ctx& s, const SysDesc& S) {
 // solo ops up to its next team op run on the half's rank 0.  Returns false
 // when an op set the half's error word.
 __device__ bool run_lane(Ctx& s, const SysDesc& S, int k0, int k1, int lane, int mode) {
+    const OpRec& first = S.ops[k0];
+    if (first.rot_graph_op >= 0 && first.rot_graph_lane == lane) {
+        const OpRec& ro = S.ops[first.rot_graph_op];
+        s.cur_op = first.rot_graph_op;
+        if (!rotamer_graph(s, S, ro, S.graphs[ro.graph], S.rots[ro.rot])) return false;
+    }
     for (int k = k0; k < k1;) {
         const OpRec& op = S.ops[k];
         if (op.lane != lane) { ++k; continue; }
@@ -1031,6 +1045,7 @@ __device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mod
                 team_sync(c);
                 return false;
             }
+            if (op.rot_graph_op >= 0) c.rot_graph_done = 1;
             k = op.lane_end;
             continue;
         }
@@ -1126,6 +1141,7 @@ __device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mod
 __device__ void compute(Ctx& c, const SysDesc& S, int mode) {
     c.cur_op = S.n_germ + S.n_deriv;  // outside the ops
     c.early_done = 0;
+    c.rot_graph_done = 0;
     const int* zr = c.I + S.sens_zero;
     for (int r = 0; r < S.sens_zero_cnt; ++r) {
         float* p = c.A + zr[2 * r];

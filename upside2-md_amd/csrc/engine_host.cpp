@@ -1444,6 +1444,35 @@ struct Program {
             }
             return false;
         };
+        // the elementwise ops themselves form a first section: one workgroup
+        // of each half runs its lane.  The affine alignment, whose eigen-solve
+        // costs about eight of the others, starts lane 1; the rest are dealt
+        // in order as above, one unit each.
+        vector<int> fl[2], fdef;
+        {
+            int fw[2] = {0, 0};
+            for (int i : first)
+                if (T.ops[T.nodes[i].op].kind == OP_AFFINE && fl[1].empty()) {
+                    fl[1].push_back(i);
+                    fw[1] = 8;
+                    lane_of[i] = 1;
+                }
+            for (int i : first) {
+                if (!fl[1].empty() && i == fl[1][0]) continue;
+                bool defer = tied(i, fl[0]) && tied(i, fl[1]);
+                for (int j : fdef) defer = defer || anc[i][j];
+                if (defer) { fdef.push_back(i); continue; }
+                const bool t0 = tied(i, fl[0]), t1 = tied(i, fl[1]);
+                const int lane = t0 ? 0 : t1 ? 1 : (fw[1] < fw[0] ? 1 : 0);
+                fl[lane].push_back(i);
+                fw[lane] += 1;
+                lane_of[i] = lane;
+            }
+            if (fl[0].size() < 2 || fl[1].size() < 2) {
+                for (int l = 0; l < 2; ++l) for (int i : fl[l]) lane_of[i] = 0;
+                fl[0].clear(); fl[1].clear(); fdef.clear();
+            }
+        }
         // pair-graph ops after the first rotamer op that do not depend on it
         // join the section too; elementwise ops after it stay behind, beside
         // the solve (measured on a 500-residue chain: 2 % faster than moving
@@ -1473,9 +1502,16 @@ struct Program {
             return false;
         };
         T.germ.assign(1, 0);
-        T.germ.insert(T.germ.end(), first.begin(), first.end());
+        if (!fl[0].empty()) {  // the elementwise section, then its deferred ops
+            for (int i : first)
+                if (std::find(fdef.begin(), fdef.end(), i) == fdef.end()) T.germ.push_back(i);
+            lane_sections.push_back({0, (int)T.germ.size() - 1});
+            T.germ.insert(T.germ.end(), fdef.begin(), fdef.end());
+        } else {
+            T.germ.insert(T.germ.end(), first.begin(), first.end());
+        }
         if (!(has_pair(in_lane[0]) && has_pair(in_lane[1]))) {
-            lane_of.assign(nn, 0);
+            for (int l = 0; l < 2; ++l) for (int i : in_lane[l]) lane_of[i] = 0;
             rot_graph_node = -1;
             T.germ.insert(T.germ.end(), rest.begin(), rest.end());
             return;
@@ -1527,9 +1563,10 @@ struct Program {
             prog[k].lane_end = 0;
         }
         for (auto& o : prog) o.rot_graph_op = o.rot_graph_lane = -1;
-        for (auto& sec : lane_sections) {
-            prog[sec.first].lane_end = sec.second;
-            for (int q = sec.second; q < (int)T.germ.size() - 1 && rot_graph_node >= 0; ++q)
+        for (auto& sec : lane_sections) prog[sec.first].lane_end = sec.second;
+        if (rot_graph_node >= 0) {  // lane 0 of the pair-graph section (the last) refreshes the rotamer graph
+            const auto& sec = lane_sections.back();
+            for (int q = sec.second; q < (int)T.germ.size() - 1; ++q)
                 if (prog_node[q] == rot_graph_node) {
                     prog[sec.first].rot_graph_op = q;
                     prog[sec.first].rot_graph_lane = 0;

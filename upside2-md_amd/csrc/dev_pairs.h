@@ -375,8 +375,13 @@ __device__ void pg_rebuild(Ctx& c, const PairGraph& G) {
     for (int j = c.gtid; j <= n2; j += c.gsize) toff[j] = 0;
     team_sync(c);
     unsigned long long t0 = ptick(c);
-    // the grid (when it fits) stays in LDS from the count scan to the fill scan
-    const bool use_grid = lgrid_fits(n2);
+    // Small graphs scan every row against the whole other side staged in LDS
+    // (n1 * n2 distance tests spread over the team cost less than the grid
+    // build's serial phases; measured: 4-5 % of a step at 150-500 residues);
+    // larger ones use the cell grid, which stays in LDS from the count scan
+    // to the fill scan.  The choice depends on the sizes only, so every team
+    // size builds the same rows.
+    const bool use_grid = lgrid_fits(n2) && (long long)n1 * n2 > (1ll << 21);
     LGrid g{};
     if (use_grid) {
         g = lgrid_build(c, G, p2, id2, n2);

@@ -579,41 +579,34 @@ __device__ void rot_fused_pass(Ctx& c, const BpMem<F, I, J>& m, gp<const float> 
             float a[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
             if (tb + gl < n) {
                 const int code = m.inc[t0 + tb + gl];
+                // the lane computes only the message into its own node: side 0
+                // (the edge's first node f) m1 = P v2 from s, side 1 m2 = P^T v1
+                // from f, each summed in the order rot_edge_w uses, and stores it
                 const int e = code >> 1, side = code & 1;
                 const int ends = m.ends[e];
-                const int f = ends & 0xFFFF, s = ends >> 16;
-                float p[36], v1[6], v2[6], m1[6], m2[6];
+                const int from = side ? (ends & 0xFFFF) : (ends >> 16);
+                float p[36], v[6], mm[6];
 #pragma unroll
                 for (int q = 0; q < 36; ++q) p[q] = m.P[m.pi(q, e)];
 #pragma unroll
                 for (int r = 0; r < 6; ++r) {
-                    v1[r] = nold_[f * 6 + r] * __builtin_amdgcn_rcpf(1e-10f + min_[m.mi(r, e)]);
-                    v2[r] = nold_[s * 6 + r] * __builtin_amdgcn_rcpf(1e-10f + min_[m.mi(6 + r, e)]);
-                    m1[r] = 0.f;
-                    m2[r] = 0.f;
+                    v[r] = nold_[from * 6 + r] * __builtin_amdgcn_rcpf(1e-10f + min_[m.mi((side ? 0 : 6) + r, e)]);
+                    mm[r] = 0.f;
                 }
 #pragma unroll
                 for (int i = 0; i < 6; ++i)
 #pragma unroll
-                    for (int j = 0; j < 6; ++j) {
-                        m1[i] = __fmaf_rn(p[i * 6 + j], v2[j], m1[i]);
-                        m2[j] = __fmaf_rn(p[i * 6 + j], v1[i], m2[j]);
-                    }
-                float s1 = 0.f, s2 = 0.f;
+                    for (int j = 0; j < 6; ++j)
+                        mm[i] = __fmaf_rn(side ? p[j * 6 + i] : p[i * 6 + j], v[j], mm[i]);
+                float sm = 0.f;
 #pragma unroll
-                for (int r = 0; r < 6; ++r) { s1 = __fadd_rn(s1, m1[r]); s2 = __fadd_rn(s2, m2[r]); }
-                const float i1 = 1.f / s1, i2 = 1.f / s2;
+                for (int r = 0; r < 6; ++r) sm = __fadd_rn(sm, mm[r]);
+                const float im = 1.f / sm;
 #pragma unroll
-                for (int r = 0; r < 6; ++r) { m1[r] = m1[r] * i1; m2[r] = m2[r] * i2; }
-                if (side == 0) {
-#pragma unroll
-                    for (int r = 0; r < 6; ++r) {
-                        mout[m.mi(r, e)] = m1[r];
-                        mout[m.mi(6 + r, e)] = m2[r];
-                    }
+                for (int r = 0; r < 6; ++r) {
+                    a[r] = mm[r] * im;
+                    mout[m.mi((side ? 6 : 0) + r, e)] = a[r];
                 }
-#pragma unroll
-                for (int r = 0; r < 6; ++r) a[r] = side ? m2[r] : m1[r];
             }
             const int cnt = min(NODE_LANES, n - tb);
             for (int q = 0; q < cnt; ++q) {

@@ -2307,7 +2307,19 @@ struct DerivEngine {
 // :893 temperature) as extendable arrays with the first dimension unlimited
 // and chunks of 100 frames (src/state_logger.h:21-39), appended one frame at a
 // time and flushed on close.
-static vector<float> rotamer_values(Batch& B, int sys, int op, const string& ln);
+// The BP state of the last evaluation on the host: node probabilities and
+// marginals, BP edges with matrices and marginals, 1-rotamer pairs.  One
+// fetch per array; a caller asking for several values of one evaluation
+// passes the same RotFetch to every rotamer_values call.
+struct RotFetch {
+    bool filled = false;
+    vector<float> prob, cur, off, P, M, p11;
+    vector<int> bpf, bps;
+    vector<std::array<int, 2>> e11;
+    int nbp = 0;
+};
+
+static vector<float> rotamer_values(Batch& B, int sys, int op, const string& ln, RotFetch* cache = nullptr);
 
 struct OutputLog {
     hid_t file = -1, group = -1;
@@ -2703,8 +2715,9 @@ struct UpsideBatch {
             }
             if (!o.rot_sets.empty()) {  // BP state of the evaluation above
                 const char* split[3] = {"rotamer_free_energy", "rotamer_potential", "rotamer_entropy"};
-                for (int q = 0; q < 3; ++q) o.rot_rows[q] = rotamer_values(batch, (int)k, o.rot_op, split[q]);
-                vector<float> one = rotamer_values(batch, (int)k, o.rot_op, "rotamer_1body_energy");
+                RotFetch fetched;  // one download of the BP state for the four values
+                for (int q = 0; q < 3; ++q) o.rot_rows[q] = rotamer_values(batch, (int)k, o.rot_op, split[q], &fetched);
+                vector<float> one = rotamer_values(batch, (int)k, o.rot_op, "rotamer_1body_energy", &fetched);
                 for (int q = 0; q < o.rot_nprob; ++q) {
                     o.rot_rows[3 + q].resize(o.rot_nn);
                     for (int x = 0; x < o.rot_nn; ++x) o.rot_rows[3 + q][x] = one[(size_t)x * o.rot_nprob + q];
@@ -3053,7 +3066,7 @@ int get_sens(int n_output, float* output, DerivEngine* engine, const char* node_
 // :1023-1055), graph-ordered values indexed by node class (1, 3, then 6
 // rotamers) and the residue's index within its class, which is the bead id's
 // bits above the rotamer fields (py/upside_config.py:1171-1178).
-static vector<float> rotamer_values(Batch& B, int sys, int op, const string& ln) {
+static vector<float> rotamer_values(Batch& B, int sys, int op, const string& ln, RotFetch* cache) {
     const Topology& T = B.sys[sys].prog->T;
     const RotamerDev& R = T.rots[T.ops[op].rot];
     const PairGraph& G = T.graphs[T.ops[op].graph];
@@ -3075,34 +3088,49 @@ static vector<float> rotamer_values(Batch& B, int sys, int op, const string& ln)
         if (n) hip_check(hipMemcpy(v.data(), src, n * 4, hipMemcpyDeviceToHost), "rotamer state");
         return v;
     };
-    vector<float> prob = fetch(A + R.nprob, (size_t)nn * 6), cur = fetch(A + R.ncur, (size_t)nn * 6);
-    vector<float> off = fetch(A + R.noff, nn);
-    int st[8];
-    hip_check(hipMemcpy(st, I + R.stats, sizeof(st), hipMemcpyDeviceToHost), "stats");
-    const int nbp = st[4];
-    vector<int> bpf = ifetch(I + R.bpf, nbp), bps = ifetch(I + R.bps, nbp);
-    // matrices: 36 contiguous entries r1*6 + r2 per BP edge; marginals: rows
-    // q = r1*6 + r2 of stride cap over the edges
-    vector<float> P = fetch(A + R.P, (size_t)36 * nbp), M((size_t)36 * nbp);
-    for (int q = 0; q < 36; ++q) {
-        vector<float> m = fetch(A + R.marg + (size_t)q * cap, nbp);
-        for (int e = 0; e < nbp; ++e) M[(size_t)e * 36 + q] = m[e];
+    RotFetch local;
+    RotFetch& F = cache ? *cache : local;
+    if (!F.filled) {
+        F.prob = fetch(A + R.nprob, (size_t)nn * 6);
+        F.cur = fetch(A + R.ncur, (size_t)nn * 6);
+        F.off = fetch(A + R.noff, nn);
+        int st[8];
+        hip_check(hipMemcpy(st, I + R.stats, sizeof(st), hipMemcpyDeviceToHost), "stats");
+        F.nbp = st[4];
+        F.bpf = ifetch(I + R.bpf, F.nbp);
+        F.bps = ifetch(I + R.bps, F.nbp);
+        // matrices: 36 contiguous entries r1*6 + r2 per BP edge; marginals: rows
+        // q = r1*6 + r2 of stride cap over the edges (one copy of all rows)
+        F.P = fetch(A + R.P, (size_t)36 * F.nbp);
+        F.M.assign((size_t)36 * F.nbp, 0.f);
+        const vector<float> rows = fetch(A + R.marg, (size_t)36 * cap);
+        for (int q = 0; q < 36; ++q)
+            for (int e = 0; e < F.nbp; ++e) F.M[(size_t)e * 36 + q] = rows[(size_t)q * cap + e];
+        // 1-rotamer residue pairs: one bead pair each, probability exp(-E)
+        int gst[4];
+        hip_check(hipMemcpy(gst, I + G.state, sizeof(gst), hipMemcpyDeviceToHost), "state");
+        const int E = gst[3];
+        vector<int> row = ifetch(I + G.row, E), nbr = ifetch(I + G.nbr, E), act = ifetch(I + G.act, E);
+        vector<float> val = fetch(A + G.val, E);
+        for (int s = 0; s < E; ++s) {
+            if (!act[s]) continue;
+            int a = bnode[row[s]], b = bnode[nbr[s]];
+            if (nrot[a] != 1 || nrot[b] != 1) continue;
+            F.e11.push_back({a, b});
+            F.p11.push_back(expf(-val[s]));
+        }
+        F.filled = true;
     }
-    // 1-rotamer residue pairs: one bead pair each, probability exp(-E)
-    int gst[4];
-    hip_check(hipMemcpy(gst, I + G.state, sizeof(gst), hipMemcpyDeviceToHost), "state");
-    const int E = gst[3];
-    vector<int> row = ifetch(I + G.row, E), nbr = ifetch(I + G.nbr, E), act = ifetch(I + G.act, E);
-    vector<float> val = fetch(A + G.val, E);
-    vector<std::array<int, 2>> e11;
-    vector<float> p11;
-    for (int s = 0; s < E; ++s) {
-        if (!act[s]) continue;
-        int a = bnode[row[s]], b = bnode[nbr[s]];
-        if (nrot[a] != 1 || nrot[b] != 1) continue;
-        e11.push_back({a, b});
-        p11.push_back(expf(-val[s]));
-    }
+    const vector<float>& prob = F.prob;
+    const vector<float>& cur = F.cur;
+    const vector<float>& off = F.off;
+    const vector<float>& P = F.P;
+    const vector<float>& M = F.M;
+    const vector<float>& p11 = F.p11;
+    const vector<int>& bpf = F.bpf;
+    const vector<int>& bps = F.bps;
+    const vector<std::array<int, 2>>& e11 = F.e11;
+    const int nbp = F.nbp;
 
     // per-residue split of the Bethe free energy: residue_free_energies
     // (:929-963) and its average-energy and entropy parts residue_entropy<0>

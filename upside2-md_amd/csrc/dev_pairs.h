@@ -691,6 +691,7 @@ __device__ void pg_edges(Ctx& c, const PairGraph& G, const OpRec& op) {
 // per element, each lane striding the element's slots, then a fixed-order
 // group reduction.
 constexpr int GW = 16;
+static_assert(GW <= 32 && (GW & (GW - 1)) == 0, "a lane group is a power of two within half a wave");
 
 template <int D1N, int D2N>
 __device__ void pg_gather_t(Ctx& c, const PairGraph& G) {

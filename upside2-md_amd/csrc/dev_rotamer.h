@@ -63,7 +63,7 @@ __device__ bool rot_candidate_lists(Ctx& c, const PairGraph& G, const RotamerDev
         return dx * dx + dy * dy + dz * dz < lim * lim;
     };
     const int lane = c.tid & 63;
-    const unsigned long long gmask = 0xFFFFull << (lane & ~(GW - 1));
+    const unsigned long long gmask = ((1ull << GW) - 1ull) << (lane & ~(GW - 1));
     const unsigned long long below = (1ull << lane) - 1ull;
     const int grp = c.gtid / GW, gl = c.gtid % GW, NG = c.gsize / GW;
     for (int base = 0; base < n; base += NG) {
@@ -283,7 +283,7 @@ __device__ void rot_fill(Ctx& c, const PairGraph& G, const RotamerDev& R) {
     // through a ballot
     {
         const int lane = c.tid & 63;
-        const unsigned long long gmask = 0xFFFFull << (lane & ~(GW - 1));
+        const unsigned long long gmask = ((1ull << GW) - 1ull) << (lane & ~(GW - 1));
         const unsigned long long below = (1ull << lane) - 1ull;
         const int grp = c.gtid / GW, gl = c.gtid % GW, NG = c.gsize / GW;
         for (int base = 0; base < R.n_node; base += NG) {

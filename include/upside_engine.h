@@ -288,8 +288,12 @@ int upside_batch_profile_report(UpsideBatch* batch, int sys, char* buf, int bufl
  * local updates per edge make the messages consistent with them, the
  * residual rule stops the solve, and a solve that has not met it after
  * `budget` iterations restarts from the plain start as mode 1 (the fallback;
- * its iterations count too).  Mode 2 needs upside_batch_set_warm_start
- * (parameters as upside_mlp_get_params gives them).
+ * its iterations count too); 3 the warm start of mode 2 stopped by the
+ * reference's own rule (the largest one-sided change of a belief at most the
+ * rotamer node's tolerance as in mode 0, or at most `residual` when that is
+ * positive), with the same budget and plain restart.  Modes 2 and 3 need
+ * upside_batch_set_warm_start (parameters as upside_mlp_get_params gives
+ * them).
  * upside_batch_bp_training_rows: the features of the first pass and the
  * converged node marginals of system sys's last solve in mode 1 or 2, one
  * row per residue with more than one rotamer state; with features or targets

@@ -787,8 +787,11 @@ __device__ void bp_plain_start(Ctx& c, const BpMem<F, I, J>& m, int nbp, int n_n
 // warm-start network's and derives consistent messages (bp_local_messages),
 // stops by the residual rule, and when bp_budget iterations do not meet it
 // restarts from the plain start as schedule 1 (the iteration count returned
-// includes both).  Schedules 1 and 2 also leave the features of the first
-// pass in R.feat (training rows for the network).
+// includes both).  Schedule 3 is the warm start of schedule 2 stopped by the
+// reference's rule (the largest one-sided change at most R.tol, or at most
+// bp_residual when that is positive), with the same budget and plain
+// restart.  Schedules 1-3 also leave the features of the
+// first pass in R.feat (training rows for the network).
 template <bool TEAM, class F, class I, class J>
 __device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const BpMem<F, I, J>& m, int nbp, F cur,
                           F old) {
@@ -796,7 +799,8 @@ __device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const B
     const int cap = v.cap;
     const int t = bp_t<TEAM>(c), nt = bp_nt<TEAM>(c);
     const int mode = c.args->bp_mode;
-    bool warm = mode == 2;
+    bool warm = mode == 2 || mode == 3;
+    const bool resid = mode == 1 || mode == 2;  // residual rule; else the reference's
     int total = 0;
     // On the team the message and node passes of an iteration are fused
     // (rot_fused_pass) over two message buffers that alternate; mm.msg is the
@@ -830,7 +834,7 @@ __device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const B
             bp_warm_beliefs<TEAM>(c, mm, c.args->mlp, n, (gp<const float>)v.feat, cur);
             bp_local_messages<TEAM>(c, mm, nbp, cur, c.args->bp_klocal);
         }
-        const float tol = mode > 0 ? c.args->bp_residual : R.tol;
+        const float tol = resid || (mode == 3 && c.args->bp_residual > 0.f) ? c.args->bp_residual : R.tol;
         const int limit = warm ? min(c.args->bp_budget, R.max_iter) : R.max_iter;
         float dev = 1e10f;
         int it = 0;
@@ -842,7 +846,7 @@ __device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const B
             float d = 0.f;
             for (int x = t; x < n; x += nt) {
                 if (m.nrot[x] == 1) continue;
-                if (mode > 0) {
+                if (resid) {
 #pragma unroll
                     for (int r = 0; r < 6; ++r) d = fmaxf(d, fabsf(cur[x * 6 + r] - old[x * 6 + r]));
                 } else {
@@ -855,7 +859,7 @@ __device__ int bp_iterate(Ctx& c, const RotamerDev& R, const RotView& v, const B
         total += it;
         if (c.gtid == 0) {
             if (warm) v.stats[7] += 1;
-            if (mode > 0 && dev <= tol) v.stats[9] += 1;
+            if (resid && dev <= tol) v.stats[9] += 1;
         }
         if (!warm || dev <= tol) break;
         // the warm start missed the residual within its budget: plain start

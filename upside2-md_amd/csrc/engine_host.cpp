@@ -3985,9 +3985,10 @@ int upside_batch_get_bp_layout(UpsideBatch* b) { return b ? b->batch.bp_layout :
 int upside_batch_set_bp_schedule(UpsideBatch* b, int mode, float residual, int budget, int klocal) {
     ABI_TRY
     auto& B = b->batch;
-    if (mode < 0 || mode > 2) fail("rotamer solve schedule must be 0, 1 or 2");
-    if (mode > 0 && !(residual > 0.f && residual < 1.f)) fail("residual must be in (0, 1)");
-    if (mode == 2 && !B.mlp) fail("schedule 2 needs the warm-start network (upside_batch_set_warm_start)");
+    if (mode < 0 || mode > 3) fail("rotamer solve schedule must be 0, 1, 2 or 3");
+    if ((mode == 1 || mode == 2) && !(residual > 0.f && residual < 1.f)) fail("residual must be in (0, 1)");
+    if (mode == 3 && !(residual >= 0.f && residual < 1.f)) fail("schedule 3 tolerance must be in [0, 1)");
+    if (mode >= 2 && !B.mlp) fail("schedules 2 and 3 need the warm-start network (upside_batch_set_warm_start)");
     if (budget < 2 || klocal < 0 || klocal > 64) fail("budget must be at least 2 iterations and klocal in [0, 64]");
     B.bp_mode = mode;
     B.bp_residual = residual;

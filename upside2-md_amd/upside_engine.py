@@ -422,10 +422,15 @@ class Batch:
         if self.calc.upside_batch_set_bp_layout(self.h, int(layout)):
             raise RuntimeError("rotamer solve layout %d not possible" % layout)
 
-    def set_bp_schedule(self, mode, residual=1e-5, budget=200, klocal=4):
+    def set_bp_schedule(self, mode, residual=None, budget=200, klocal=4):
         """Rotamer BP schedule: 0 the reference's, 1 plain start with the
         residual stopping rule, 2 warm start (needs set_warm_start) with the
-        residual rule and a plain restart after `budget` iterations."""
+        residual rule and a plain restart after `budget` iterations, 3 the
+        warm start stopped by the reference's rule.  `residual` is the
+        residual rule's bound (default 1e-5) for 1 and 2, and for 3 the
+        one-sided tolerance (default the rotamer node's own)."""
+        if residual is None:
+            residual = 0.0 if mode == 3 else 1e-5
         if self.calc.upside_batch_set_bp_schedule(self.h, int(mode), float(residual), int(budget), int(klocal)):
             raise RuntimeError("set_bp_schedule failed")
 

@@ -329,9 +329,17 @@ def batched_workloads(work, rounds=60, cpu=True):
     out = []
 
     def tune(b):
-        cand = [t for t in (1, 2, 4, 8, 16) if t <= b.max_team()]
+        # team sizes shared by every system, and (0) per-system teams in
+        # proportion to the systems' sizes
+        cand = [t for t in (1, 2, 4, 8, 16) if t <= b.max_team()] + [0]
         b.run(3)
-        b.autotune_layout(3, cand, (0, 1, 3))
+        ms = b.autotune_layout(3, cand, (0, 1, 3))
+        print("batch of %d: team / BP layout autotune (ms per 3 rounds; team 0: per-system teams %s): %s"
+              % (b.n_sys, list(map(int, b.balanced_teams())), ms), file=sys.stderr)
+
+    def teams(b):
+        t = [int(x) for x in b.teams()]
+        return t[0] if len(set(t)) == 1 else t
 
     p = os.path.join(work, "rex150.up")
     upconfig.make_chain(p, 150, seed=2)
@@ -351,7 +359,7 @@ def batched_workloads(work, rounds=60, cpu=True):
     line = {"workload": "16 replicas x 150 residues, T geometric 0.80-0.95, neighbour swap sets alternating "
                         "every 21 steps, exchanged inside the persistent launch",
             "value": 16 * 150 * 3 * rounds / wall, "unit": "residue-steps/s", "steps": 3 * rounds,
-            "seconds": wall, "workgroups_per_system": b.team(), "bp_layout": b.bp_layout(),
+            "seconds": wall, "workgroups_per_system": teams(b), "bp_layout": b.bp_layout(),
             "swaps_accepted_attempted": [int(suc.sum()), int(att.sum())],
             "bp_iters_per_solve": bp_iters(b, st0, st1),
             "roofline": batch_roofline(b, [p] * 16, st0, st1, 3 * rounds, b.last_run_ms())}
@@ -380,7 +388,7 @@ def batched_workloads(work, rounds=60, cpu=True):
     st1 = [b.bp_stats(s) for s in range(64)]
     line = {"workload": "64 independent chains, lengths uniform in [50, 400] (%d residues)" % sum(lens),
             "value": sum(lens) * 3 * rounds / wall, "unit": "residue-steps/s", "steps": 3 * rounds,
-            "seconds": wall, "workgroups_per_system": b.team(), "bp_layout": b.bp_layout(),
+            "seconds": wall, "workgroups_per_system": teams(b), "bp_layout": b.bp_layout(),
             "bp_iters_per_solve": bp_iters(b, st0, st1),
             "roofline": batch_roofline(b, paths, st0, st1, 3 * rounds, b.last_run_ms())}
     if cpu:

@@ -233,11 +233,23 @@ float upside_batch_last_run_ms(UpsideBatch* batch);
  * system, no grid barriers).  max_team is the largest team every system can
  * have with all workgroups resident at once.  autotune runs n_round rounds per
  * candidate team size from the same state, restores the state and keeps the
- * fastest; ms_out[n_candidates] (may be NULL) receives the kernel times. */
+ * fastest; ms_out[n_candidates] (may be NULL) receives the kernel times.  A
+ * candidate 0 stands for the balanced per-system teams below.
+ * No reference counterpart (the reference runs each system on one thread,
+ * src/main.cpp:917): set_teams gives every system its own team size
+ * (teams[n_sys]; each team stays on one XCD, and all must be resident at
+ * once, else an error); get_team then reports the largest.  get_teams fills
+ * teams[n_sys] with the current sizes.  balanced_teams proposes sizes in
+ * proportion to the systems' atom counts within `budget` workgroups (<= 0:
+ * the resident maximum), without applying them.  Results do not depend on
+ * the team sizes.  0 on success. */
 int upside_batch_max_team(UpsideBatch* batch);
 int upside_batch_set_team(UpsideBatch* batch, int team);
 int upside_batch_get_team(UpsideBatch* batch);
 int upside_batch_autotune(UpsideBatch* batch, int n_round, const int* candidates, int n_candidates, float* ms_out);
+int upside_batch_set_teams(UpsideBatch* batch, const int* teams, int n_sys);
+int upside_batch_get_teams(UpsideBatch* batch, int* teams, int n_sys);
+int upside_batch_balanced_teams(UpsideBatch* batch, int budget, int* teams, int n_sys);
 /* Where each rotamer BP solve runs (no reference counterpart; the reference
  * solves on its one thread, src/rotamer.cpp:1106-1162): 0 (default) on one
  * workgroup with the BP state in its LDS when it fits (the rest of the team

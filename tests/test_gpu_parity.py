@@ -968,6 +968,42 @@ def test_team_execution_matches_single_workgroup(ue, chain_file, team):
     np.testing.assert_allclose(bt.energies(), b1.energies(), rtol=1e-5)
 
 
+def test_per_system_teams_match_single_workgroup(ue, chain_file):
+    """Every system with its own team size (sizes 1, 3, 6, 5 and 2, so odd
+    and uneven splits into halves too; each team placed on one XCD) follows
+    the single-workgroup trajectories bit for bit, and so do the balanced
+    teams (sizes in proportion to the atom counts, within the resident
+    workgroups) for the rounds after."""
+    lens = [40, 150, 90, 120, 60]
+    paths = [chain_file(L, 80 + k) for k, L in enumerate(lens)]
+    temps = [0.8, 0.85, 0.9, 0.85, 0.8]
+    b1 = ue.Batch(paths, temps, seed=9)
+    b1.run(4)
+    bt = ue.Batch(paths, temps, seed=9)
+    teams = [1, 3, 6, 5, 2]
+    bt.set_teams(teams)
+    assert list(bt.teams()) == teams and bt.team() == 6
+    say("case: per-system teams %s, L=%s" % (teams, lens))
+    bt.run(4)
+    for s in range(len(lens)):
+        np.testing.assert_array_equal(bt.pos(s), b1.pos(s))
+        np.testing.assert_array_equal(bt.mom(s), b1.mom(s))
+    bal = bt.balanced_teams()
+    say("balanced teams %s (atoms %s)" % (list(bal), bt.n_atom))
+    order = np.argsort(bt.n_atom, kind="stable")
+    assert np.all(np.diff(bal[order]) >= 0) and bal.min() >= 1
+    bt.set_teams(bal)
+    bt.run(2)
+    b1.run(2)
+    for s in range(len(lens)):
+        np.testing.assert_array_equal(bt.pos(s), b1.pos(s))
+    np.testing.assert_array_equal(bt.energies(), b1.energies())
+    with pytest.raises(RuntimeError):
+        bt.set_teams([1, 2])
+    bt.set_team(1)
+    assert list(bt.teams()) == [1] * len(lens)
+
+
 @pytest.mark.parametrize("n_res,layout", [(120, 1), (120, 3), (400, 0), (400, 2)])
 def test_rotamer_solve_layouts_match(ue, chain_file, n_res, layout):
     """The rotamer BP solve on the whole team (state in memory) or on one

@@ -281,6 +281,12 @@ struct RunArgs {
     int n_frames;
     int team;              // workgroups per system (1: no grid barriers)
     int n_sys;             // systems in the launch
+    // per-system teams (null: every system has `team`): entry b of the
+    // n_blocks-long map is (sys << 14) | (G << 7) | rank for workgroup b, -1
+    // for a padding workgroup; workgroup b runs on XCD b % N_XCD, and the host
+    // packs every team onto one XCD
+    const int* blockmap;
+    int n_blocks;
     // simulated annealing (src/main.cpp:593-612, :975-977): when anneal is set
     // the thermostat temperature at round r is the schedule at time
     // 3 dt (r+1) from each system's initial temperature

@@ -1737,7 +1737,7 @@ struct Batch {
         const Topology& T = sys[k].prog->T;
         string out;
         char line[256];
-        if (team > 1) out += string("team of ") + std::to_string(team) + " workgroups, " +
+        if (team_of(k) > 1) out += string("team of ") + std::to_string(team_of(k)) + " workgroups, " +
                              (team_local ? "on one XCD (L2-local barriers)\n" : "spread over XCDs (device-scope barriers)\n");
         int ng = sys[k].prog->n_germ();
         const int nops = ng + (int)T.deriv.size(), nb0 = nops + PROF_EXTRA;
@@ -1782,6 +1782,7 @@ struct Batch {
 
     ~Batch() {
         for (auto& s : sys) s.release();
+        (void)hipFree(d_blockmap);
         for (auto p : prof) (void)hipFree(p);
         (void)hipFree(d_desc);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -1930,7 +1931,18 @@ struct Batch {
         push_desc();
     }
 
-    int team = 1;   // workgroups per system (set_team / autotune)
+    int team = 1;   // workgroups per system (set_team / autotune); with per-system teams their largest
+    // per-system team sizes (set_teams; empty: every system has `team`) and the
+    // device block map of their launches (RunArgs::blockmap)
+    vector<int> teams;
+    int* d_blockmap = nullptr;
+    int n_blocks = 0, map_cap = 0;
+    int team_of(size_t s) const { return teams.empty() ? team : teams[s]; }
+    long long team_total() const {
+        long long t = 0;
+        for (size_t s = 0; s < sys.size(); ++s) t += team_of(s);
+        return t;
+    }
     int bp_layout = 0;  // where rotamer solves run (RunArgs::bp_layout)
     // rotamer solve schedule (RunArgs::bp_mode and its parameters) and the
     // warm-start network's device copy (schedule 2)
@@ -1955,13 +1967,94 @@ struct Batch {
         return std::max(1, std::min(t, MAX_TEAM));
     }
 
+    void reset_barriers() {  // the barrier counters must be multiples of a new team size
+        for (auto& sy : sys) hip_check(hipMemset(sy.iarena + sy.prog->T.bar, 0, BAR_WORDS * sizeof(int)), "barrier reset");
+    }
+
     void set_team(int t) {
         if (t < 1 || t > max_team()) fail("team size " + std::to_string(t) + " not possible (max " +
                                           std::to_string(max_team()) + " for this batch)");
-        if (t != team)  // the barrier counters must be multiples of the new size
-            for (auto& sy : sys)
-                hip_check(hipMemset(sy.iarena + sy.prog->T.bar, 0, BAR_WORDS * sizeof(int)), "barrier reset");
+        if (t != team || !teams.empty()) reset_barriers();
         team = t;
+        teams.clear();
+    }
+
+    // Place per-system teams t on the XCDs (a team never spans two): largest
+    // team first (ties: system order) onto the least loaded XCD, each XCD
+    // holding at most its share of the resident workgroups.  False if they do
+    // not fit; else, when map is given, the block map: workgroup b runs on XCD
+    // b % N_XCD, each XCD's teams in system order.
+    bool pack_teams(const vector<int>& t, vector<int>* map) const {
+        const int cap = upside_max_resident_blocks() / N_XCD;
+        vector<int> order(t.size());
+        for (size_t s = 0; s < t.size(); ++s) order[s] = (int)s;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return t[a] > t[b]; });
+        vector<int> load(N_XCD, 0), where(t.size(), 0);
+        for (int s : order) {
+            int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            if (load[x] + t[s] > cap) return false;
+            where[s] = x;
+            load[x] += t[s];
+        }
+        if (map) {
+            const int slots = *std::max_element(load.begin(), load.end());
+            map->assign((size_t)N_XCD * slots, -1);
+            vector<int> fill(N_XCD, 0);
+            for (size_t s = 0; s < t.size(); ++s) {
+                const int x = where[s];
+                for (int r = 0; r < t[s]; ++r) (*map)[(size_t)(fill[x] + r) * N_XCD + x] = ((int)s << 14) | (t[s] << 7) | r;
+                fill[x] += t[s];
+            }
+        }
+        return true;
+    }
+
+    void set_teams(const vector<int>& v) {
+        if (v.size() != sys.size()) fail("set_teams: one team size per system expected");
+        if (sys.size() >= (1u << 17)) fail("set_teams: too many systems for per-system teams");
+        for (int g : v)
+            if (g < 1 || g > MAX_TEAM) fail("set_teams: team size " + std::to_string(g) + " out of range");
+        vector<int> map;
+        if (!pack_teams(v, &map))
+            fail("set_teams: the teams do not fit on the device at once (" +
+                 std::to_string(upside_max_resident_blocks() / N_XCD) + " workgroups per XCD)");
+        if ((int)map.size() > map_cap) {
+            (void)hipFree(d_blockmap);
+            d_blockmap = nullptr;
+            map_cap = 0;
+            hip_check(hipMalloc(&d_blockmap, map.size() * sizeof(int)), "block map");
+            map_cap = (int)map.size();
+        }
+        hip_check(hipMemcpy(d_blockmap, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice), "block map");
+        n_blocks = (int)map.size();
+        if (v != teams) reset_barriers();
+        teams = v;
+        team = *std::max_element(v.begin(), v.end());
+    }
+
+    // Team sizes for the systems in proportion to their sizes: starting from
+    // one workgroup each, the system with the most atoms per workgroup (ties:
+    // the lower index) gains a workgroup while the total stays within budget
+    // (<= 0: the device's resident workgroups) and the teams still fit
+    // (pack_teams).  Depends on the systems only.
+    vector<int> balanced_teams(int budget) const {
+        const int n = (int)sys.size();
+        if (budget <= 0) budget = upside_max_resident_blocks();
+        vector<int> t(n, 1);
+        if (n > budget || !pack_teams(t, nullptr)) fail("balanced teams: more systems than resident workgroups");
+        for (long long total = n; total < budget; ++total) {
+            int best = -1;
+            double bw = -1.0;
+            for (int s = 0; s < n; ++s) {
+                if (t[s] >= MAX_TEAM) continue;
+                const double w = (double)sys[s].prog->T.n_atom / t[s];
+                if (w > bw) { bw = w; best = s; }
+            }
+            if (best < 0) break;
+            t[best] += 1;
+            if (!pack_teams(t, nullptr)) { t[best] -= 1; break; }
+        }
+        return t;
     }
 
     // Pick the team size and rotamer-solve layout with the shortest run of
@@ -1995,28 +2088,37 @@ struct Batch {
             }
             push_desc();
         };
-        int best = team, best_lay = bp_layout;
+        int best = teams.empty() ? team : 0, best_lay = bp_layout;
         float best_ms = 3.0e38f;
         int limit = max_team();
+        vector<int> balanced;
+        for (int k = 0; k < n_cand; ++k)
+            if (cand[k] == 0 && balanced.empty()) balanced = balanced_teams(0);
         for (int k = 0; k < n_cand; ++k)
             for (int l = 0; l < n_lay; ++l) {
                 if (ms_out) ms_out[k * n_lay + l] = -1.f;
-                if (cand[k] < 1 || cand[k] > limit) continue;
+                if (cand[k] < 0 || cand[k] > limit) continue;
                 if (cand[k] == 1 && l > 0) continue;  // a team of one solves on its workgroup anyway
-                set_team(cand[k]);
+                if (cand[k] == 0) set_teams(balanced);
+                else set_team(cand[k]);
                 bp_layout = lay[l];
                 run(n_round, 0, nullptr, 0, (int)round_num);
                 if (ms_out) ms_out[k * n_lay + l] = last_ms;
                 if (last_ms < best_ms) { best_ms = last_ms; best = cand[k]; best_lay = lay[l]; }
                 restore();
             }
-        set_team(best);
+        if (best == 0) set_teams(balanced);
+        else set_team(best);
         bp_layout = best_lay;
     }
 
     void launch(const RunArgs& a0) {
         RunArgs a = a0;
         a.team = team;
+        if (!teams.empty()) {
+            a.blockmap = d_blockmap;
+            a.n_blocks = n_blocks;
+        }
         a.bp_layout = bp_layout;
         a.bp_mode = bp_mode;
         a.bp_residual = bp_residual;
@@ -2028,17 +2130,20 @@ struct Batch {
         hip_check(hipEventRecord(ev1, stream), "event");
         hip_check(hipStreamSynchronize(stream), "engine run");
         (void)hipEventElapsedTime(&last_ms, ev0, ev1);
+        if (team > 1) team_local = true;
         if (team > 1)
-            for (auto& sy : sys) {
+            for (size_t s = 0; s < sys.size(); ++s) {
+                const System& sy = sys[s];
+                if (team_of(s) < 2) continue;
                 unsigned w[BAR_WORDS];
                 hip_check(hipMemcpy(w, sy.iarena + sy.prog->T.bar, sizeof(w), hipMemcpyDeviceToHost), "barrier");
                 auto word64 = [&](int k) { return (unsigned long long)w[k] | ((unsigned long long)w[k + 1] << 32); };
-                const unsigned long long g = (unsigned long long)team, lo = g / 2, hi = g - g / 2;
+                const unsigned long long g = (unsigned long long)team_of(s), lo = g / 2, hi = g - g / 2;
                 if (w[2]) fail("team barrier wait timed out: results of this run are void");
                 if (word64(0) % g || (g > 2 && word64(BAR_SUB1) % (g - 1)) || (g >= 4 && word64(BAR_LO) % lo) ||
                     (g >= 4 && word64(BAR_HI) % hi))
                     fail("team barrier arrivals unbalanced at the end of a launch: results of this run are void");
-                team_local = w[3] != 0;
+                team_local = team_local && w[3] != 0;
             }
     }
 
@@ -2085,7 +2190,7 @@ struct Batch {
         a.anneal_end = anneal_end;
         a.anneal_duration = anneal_duration;
         a.rex = rex_dev;
-        a.rex.n_wg = (int)sys.size() * team;  // the workgroups that meet at its grid barriers
+        a.rex.n_wg = (int)team_total();  // the workgroups that meet at its grid barriers
         for (;;) {
             if (a.rex.interval > 0) {
                 rex_restore();
@@ -2536,7 +2641,7 @@ struct UpsideBatch {
         if (rex_interval <= 0 || batch.anneal) return false;
         for (auto& sy : batch.sys)
             if (sy.prog != batch.sys[0].prog) return false;
-        return (long long)batch.sys.size() * batch.team <= (long long)upside_max_resident_blocks();
+        return batch.team_total() <= (long long)upside_max_resident_blocks();
     }
 
     void rex_alloc() {
@@ -3955,6 +4060,33 @@ int upside_batch_set_team(UpsideBatch* b, int team) {
 }
 
 int upside_batch_get_team(UpsideBatch* b) { return b ? b->batch.team : -1; }
+
+int upside_batch_set_teams(UpsideBatch* b, const int* teams, int n_sys) {
+    ABI_TRY
+    if (!teams || n_sys < 0) fail("set_teams: no team sizes");
+    b->batch.set_teams(vector<int>(teams, teams + n_sys));
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_get_teams(UpsideBatch* b, int* teams, int n_sys) {
+    ABI_TRY
+    auto& B = b->batch;
+    if (!teams || n_sys != (int)B.sys.size()) fail("get_teams: one entry per system expected");
+    for (int s = 0; s < n_sys; ++s) teams[s] = B.team_of(s);
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_balanced_teams(UpsideBatch* b, int budget, int* teams, int n_sys) {
+    ABI_TRY
+    auto& B = b->batch;
+    if (!teams || n_sys != (int)B.sys.size()) fail("balanced_teams: one entry per system expected");
+    vector<int> t = B.balanced_teams(budget);
+    std::copy(t.begin(), t.end(), teams);
+    return 0;
+    ABI_CATCH(1)
+}
 
 int upside_batch_autotune(UpsideBatch* b, int n_round, const int* candidates, int n_candidates, float* ms_out) {
     ABI_TRY

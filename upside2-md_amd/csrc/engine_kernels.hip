@@ -1350,9 +1350,15 @@ __device__ __noinline__ void rex_attempt(Ctx& c, SysDesc& S, const RexDev& x, in
 // kernel trace.
 __device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& args, float* red, int* ired,
                                             float* lds) {
-    const int G = args.team;
+    int G = args.team;
     int sys = blockIdx.x, rank = 0;
-    if (G > 1) {  // team launches: workgroup b runs on XCD b % N_XCD; a team stays on one XCD
+    if (args.blockmap) {  // per-system teams, placed by the host
+        const int e = args.blockmap[blockIdx.x];
+        if (e < 0) return;  // padding workgroup
+        sys = e >> 14;
+        G = (e >> 7) & 127;
+        rank = e & 127;
+    } else if (G > 1) {  // team launches: workgroup b runs on XCD b % N_XCD; a team stays on one XCD
         const int xcd = blockIdx.x % N_XCD, slot = blockIdx.x / N_XCD;
         sys = (slot / G) * N_XCD + xcd;
         rank = slot % G;
@@ -1692,6 +1698,9 @@ hipError_t upside_launch_engine(upside::SysDesc* d_systems, int n_sys, upside::R
     if (args.team <= 1) args.team = 1;
     args.n_sys = n_sys;
     void* kargs[] = {&d_systems, &args};
+    if (args.blockmap)  // per-system teams: every workgroup resident at once
+        return hipLaunchCooperativeKernel(fn, dim3(args.n_blocks), dim3(upside::NT), kargs, (unsigned)lds_bytes,
+                                          stream);
     if (args.team == 1) {
         if (args.rex.interval > 0)  // the grid barriers need every workgroup resident
             return hipLaunchCooperativeKernel(fn, dim3(n_sys), dim3(upside::NT), kargs, (unsigned)lds_bytes, stream);

@@ -83,6 +83,9 @@ def load_upside_library(path=LIB_PATH):
         "upside_batch_enable_profile": (i, [vp, i]),
         "upside_batch_max_team": (i, [vp]),
         "upside_batch_set_team": (i, [vp, i]),
+        "upside_batch_set_teams": (i, [vp, vp, i]),
+        "upside_batch_get_teams": (i, [vp, vp, i]),
+        "upside_batch_balanced_teams": (i, [vp, i, vp, i]),
         "upside_batch_get_team": (i, [vp]),
         "upside_batch_autotune": (i, [vp, i, vp, i, vp]),
         "upside_batch_autotune_layout": (i, [vp, i, vp, i, vp, i, vp]),
@@ -394,9 +397,29 @@ class Batch:
         if self.calc.upside_batch_set_team(self.h, int(team)):
             raise RuntimeError("team size %d not possible" % team)
 
+    def set_teams(self, teams):
+        """One team size per system (each team on one XCD, all resident)."""
+        t = np.ascontiguousarray(teams, dtype=np.intc)
+        if self.calc.upside_batch_set_teams(self.h, t.ctypes.data, len(t)):
+            raise RuntimeError("per-system teams %s not possible" % list(t))
+
+    def teams(self):
+        t = np.zeros(self.n_sys, dtype=np.intc)
+        if self.calc.upside_batch_get_teams(self.h, t.ctypes.data, len(t)):
+            raise RuntimeError("get_teams failed")
+        return t
+
+    def balanced_teams(self, budget=0):
+        """Team sizes in proportion to the systems' atom counts (not applied)."""
+        t = np.zeros(self.n_sys, dtype=np.intc)
+        if self.calc.upside_batch_balanced_teams(self.h, int(budget), t.ctypes.data, len(t)):
+            raise RuntimeError("balanced_teams failed")
+        return t
+
     def autotune(self, n_round=3, candidates=(1, 2, 4, 8, 16, 32)):
         """Time n_round rounds per team size from the current state, keep the
-        fastest; returns {team: kernel ms} of the candidates that ran."""
+        fastest; returns {team: kernel ms} of the candidates that ran (0: the
+        balanced per-system teams)."""
         cand = np.asarray(candidates, dtype=np.intc)
         ms = np.zeros(len(cand), dtype="f4")
         if self.calc.upside_batch_autotune(self.h, int(n_round), cand.ctypes.data, len(cand), ms.ctypes.data):

@@ -122,11 +122,47 @@ def reference_errors(path, pos, energy_gpu, deriv_gpu):
             float(np.sqrt(((deriv_gpu - d) ** 2).sum() / (d ** 2).sum())))
 
 
+def oracle_edges_on(probe, ora, name):
+    """The float32 oracle's edge set of a pair graph refined from the engine's
+    own coordinates of the graph's producers (probe's last evaluation), so
+    that the refine is compared on identical inputs; and the same set from the
+    oracle's own producer coordinates."""
+    import upside_oracle
+    node = ora.nodes[name]
+    own = None
+    if hasattr(node, "ig"):
+        ig = node.ig
+        own = set(zip(ig.i1.tolist(), ig.i2.tolist()))
+        prods = {ig.node1.name: ig.node1, ig.node2.name: ig.node2}
+        saved = {k: n.output.copy() for k, n in prods.items()}
+        for k, n in prods.items():
+            n.output[:] = probe.get_output(k)
+        try:
+            i1, i2 = ig.edges()
+        finally:
+            for k, n in prods.items():
+                n.output[:] = saved[k]
+            ig.edges()
+        return set(zip(i1.tolist(), i2.tolist())), own
+    if not (hasattr(node, "i1") and hasattr(node, "res") and hasattr(node, "dist_cutoff")):
+        return None, None
+    own = set(zip(node.i1.tolist(), node.i2.tolist()))
+    # backbone pairs: residue pairs of the alignment frames' centres
+    t = probe.get_output(node.args[0])[node.res, :3].astype(np.float32)
+    d2 = upside_oracle.refine_dist2(t[:, None] - t[None, :], np.float32)
+    cut2 = np.float32(node.dist_cutoff) * np.float32(node.dist_cutoff)
+    i1, i2 = np.nonzero(np.triu((d2 < cut2) & (np.abs(node.res[:, None] - node.res[None, :]) > 1), 1))
+    return set(zip(i1.tolist(), i2.tolist())), own
+
+
 def oracle_checks(path, pos, energy_gpu, deriv_gpu, probe, graph_names):
     """At the final positions: relative energy and force errors against the
-    float64 oracle, and the XOR count of every pair graph's edge set (GPU list
-    against the float32 oracle's refined list; must be 0).  Part of the CPU
-    checker leg."""
+    float64 oracle, and the XOR count of every pair graph's edge set against
+    the float32 oracle's refine of the engine's own producer coordinates
+    (must be 0).  The XOR against the oracle's refine of its own producer
+    coordinates is reported beside it: a pair whose distance lies within the
+    two engines' rounding of the bead positions from the cutoff can differ
+    there.  Part of the CPU checker leg."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import upside_oracle
     ora = upside_oracle.Engine(path, dtype=np.float64)
@@ -135,19 +171,18 @@ def oracle_checks(path, pos, energy_gpu, deriv_gpu, probe, graph_names):
     d64 = ora.deriv(p64)
     ora32 = upside_oracle.Engine(path)      # edge sets are defined in float32
     ora32.energy(np.ascontiguousarray(pos, np.float32))
-    xor = {}
+    xor, xor_own = {}, {}
     for name in graph_names:
-        node = ora32.nodes.get(name)
-        if node is None:
+        if name not in ora32.nodes:
             continue
-        obj = node.ig if hasattr(node, "ig") else node
-        if not hasattr(obj, "i1"):
+        want, own = oracle_edges_on(probe, ora32, name)
+        if want is None:
             continue
-        want = set(zip(obj.i1.tolist(), obj.i2.tolist()))
         got = set(map(tuple, probe.edge_pairs(name).tolist()))
         xor[name] = len(got ^ want)
+        xor_own[name] = len(got ^ own)
     return (abs(energy_gpu - e64) / max(abs(e64), 1.0),
-            float(np.sqrt(((deriv_gpu - d64) ** 2).sum() / (d64 ** 2).sum())), xor)
+            float(np.sqrt(((deriv_gpu - d64) ** 2).sum() / (d64 ** 2).sum())), xor, xor_own)
 
 
 def percentile_from_hist(h, q):
@@ -443,7 +478,7 @@ def warm_start_workloads(work, train_paths, rounds=20, train_steps=2000):
         res = {}
         for sched in (0, 1, 2, 3):
             bb = upside_engine.Batch(paths, [TEMP] * len(paths), seed=6, dt=DT)
-            bb.set_team(min(16 if len(paths) == 1 else 2, bb.max_team()))
+            bb.set_team(bb.max_team())  # 32 for the single chain, 8 for the replicas
             bb.run(2)
             if sched >= 2:
                 bb.set_warm_start(params)
@@ -536,7 +571,7 @@ def main():
         b.set_team(args.team)
         b.set_bp_layout(max(args.bp_layout, 0))
     else:
-        cand = [t for t in (1, 2, 4, 8, 16, 32) if t <= b.max_team()]
+        cand = [t for t in (1, 2, 4, 8, 16, 24, 28, 32) if t <= b.max_team()]
         tune = b.autotune_layout(6, cand, (0, 1, 2, 3))  # restores the warmed state after each candidate
         log("team / BP layout autotune (ms per 6 rounds): %s -> team %d layout %d" % (tune, b.team(), b.bp_layout()))
     probe.energy(b.pos(0))
@@ -572,10 +607,10 @@ def main():
     if not args.no_cpu_baseline:
         e_end, d_end = float(probe.energy(pos_end)), probe.deriv(pos_end)
         er, fr = reference_errors(path, pos_end, e_end, d_end)
-        eo, fo, xor = oracle_checks(path, pos_end, e_end, d_end, probe, list(g1))
+        eo, fo, xor, xor_own = oracle_checks(path, pos_end, e_end, d_end, probe, list(g1))
         checks = {"energy_rel_err_vs_reference": er, "force_rel_err_vs_reference": fr,
                   "energy_rel_err_vs_oracle_f64": eo, "force_rel_err_vs_oracle_f64": fo,
-                  "edge_set_xor_vs_oracle": xor}
+                  "edge_set_xor_vs_oracle": xor, "edge_set_xor_vs_oracle_own_bead_coordinates": xor_own}
 
     line = {
         "metric": METRIC, "value": value, "unit": "residue-steps/s", "n_gpus": 1, "steps": steps,

@@ -559,12 +559,13 @@ __device__ void rot_node_pass(Ctx& c, const BpMem<F, I, J>& m, int n_node, F ncu
 
 // One synchronous iteration of the team solve in a single phase: a group of
 // NODE_LANES lanes takes a node, each lane one of its incidences, and computes
-// that edge's two new messages from the old messages (min) and the old node
-// beliefs exactly as rot_edge_w does; the group multiplies the messages into
-// its node in incidence order as rot_node_pass does, and the edge's first
-// node stores both messages into mout.  Every edge is thus computed by both
-// of its nodes (the same bits), and the message and node passes need one
-// team barrier instead of two.  Results equal rot_edge_pass + rot_node_pass.
+// the edge's new message into that node from the old messages (min) and the
+// old belief of the edge's other node, in the summation order rot_edge_w
+// uses, and stores it into mout; the group multiplies the messages into its
+// node in incidence order as rot_node_pass does.  Each message is computed
+// once, by the group of the node it goes to, and the message and node passes
+// need one team barrier instead of two.  Results equal rot_edge_pass +
+// rot_node_pass.
 template <class F, class I, class J>
 __device__ void rot_fused_pass(Ctx& c, const BpMem<F, I, J>& m, gp<const float> min_, gp<float> mout, int n_node,
                                F ncur_, F nold_, float damping, bool swap_first) {

@@ -1745,7 +1745,13 @@ struct Batch {
         for (int i = nb0; i < n; ++i) nbar += t[i];
         for (int i = 0; i < nb0; ++i) {
             string label;
-            if (i < ng) label = "value  " + T.nodes[T.germ[i + 1]].name;
+            if (i < ng) {
+                label = "value  " + T.nodes[T.germ[i + 1]].name;
+                const Program& P = *sys[k].prog;
+                for (auto& sec : P.lane_sections)  // ops run by one half of the team
+                    if (i >= sec.first && i < sec.second && team_of(k) >= 4)
+                        label += " [lane " + std::to_string(P.lane_of[T.germ[i + 1]]) + "]";
+            }
             else if (i < ng + (int)T.deriv.size()) label = "deriv  " + T.nodes[T.deriv[i - ng]].name;
             else {
                 static const char* ex[] = {"rotamer BP solve (in value)", "rotamer graph rebuild (in value)",

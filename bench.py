@@ -428,6 +428,28 @@ def batched_workloads(work, rounds=60, cpu=True):
             "roofline": batch_roofline(b, paths, st0, st1, 3 * rounds, b.last_run_ms())}
     if cpu:
         line["cpu_baseline"] = cpu_reference_batch(paths, [TEMP] * 64, rounds, 3)
+    del b
+    # the same engine one system at a time: every 8th chain alone on the GPU
+    # (largest team, autotuned layout, the same warmup), timed over the same
+    # rounds; the rate is the sample's residue-steps over its summed times
+    res_steps, secs = 0, 0.0
+    for q, L in list(zip(paths, lens))[::8]:
+        one = upside_engine.Batch([q], [TEMP], seed=3, dt=DT)
+        one.set_team(one.max_team())
+        one.run(3)
+        one.autotune_layout(3, [one.max_team()], (0, 1, 3))
+        hip_sync()
+        t0 = time.perf_counter()
+        one.run(rounds)
+        hip_sync()
+        secs += time.perf_counter() - t0
+        res_steps += L * 3 * rounds
+        del one
+    line["one_system_at_a_time"] = {
+        "value": res_steps / secs, "unit": "residue-steps/s",
+        "sample": "every 8th chain of the batch (8 chains, %d residues) alone on the GPU with a team of 32, "
+                  "%d rounds each; residue-steps over the summed times" % (sum(lens[::8]), rounds)}
+    line["batched_vs_one_at_a_time"] = line["value"] / (res_steps / secs)
     out.append(line)
     return out, paths
 

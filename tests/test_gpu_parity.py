@@ -1002,6 +1002,24 @@ def test_per_system_teams_match_single_workgroup(ue, chain_file):
         bt.set_teams([1, 2])
     bt.set_team(1)
     assert list(bt.teams()) == [1] * len(lens)
+    # replica exchange inside the launch (grid barriers over every team) with
+    # per-system teams: the same swaps and trajectories as one workgroup each
+    path = chain_file(90, 88)
+    temps = [0.80, 0.85, 0.90, 0.95]
+    sets = [[(0, 1), (2, 3)], [(1, 2)]]
+    runs = []
+    for teams in ([1, 1, 1, 1], [2, 5, 1, 3]):
+        b = ue.Batch([path] * 4, temps, seed=5)
+        b.set_teams(teams)
+        b.set_replica_exchange(3, sets)
+        say("case: in-launch replica exchange with teams %s" % teams)
+        b.run_logged(24, 0)
+        runs.append(([b.pos(s) for s in range(4)], b.replica_stats()))
+    for s in range(4):
+        np.testing.assert_array_equal(runs[1][0][s], runs[0][0][s])
+    for x, y in zip(runs[1][1], runs[0][1]):
+        np.testing.assert_array_equal(x, y)
+    assert runs[0][1][2].sum() > 0  # swaps were attempted
 
 
 @pytest.mark.parametrize("n_res,layout", [(120, 1), (120, 3), (400, 0), (400, 2)])

@@ -250,6 +250,16 @@ int upside_batch_autotune(UpsideBatch* batch, int n_round, const int* candidates
 int upside_batch_set_teams(UpsideBatch* batch, const int* teams, int n_sys);
 int upside_batch_get_teams(UpsideBatch* batch, int* teams, int n_sys);
 int upside_batch_balanced_teams(UpsideBatch* batch, int budget, int* teams, int n_sys);
+/* The host-side plans behind set_teams and balanced_teams, for a device with
+ * per_xcd resident workgroups on each of its 8 XCDs (no reference
+ * counterpart; no device calls).  upside_plan_team_map writes the block map,
+ * one word per workgroup b (which runs on XCD b % 8): (system << 14) |
+ * (team size << 7) | rank, or -1 for padding; it returns the map's length,
+ * or -1 if the teams do not fit or map_len is too small (map may be NULL to
+ * ask for the length).  upside_plan_balanced_teams fills teams[n_sys] from
+ * the atom counts within budget workgroups; 0 on success. */
+int upside_plan_team_map(const int* teams, int n_sys, int per_xcd, int* map, int map_len);
+int upside_plan_balanced_teams(const int* n_atom, int n_sys, int budget, int per_xcd, int* teams);
 /* Where each rotamer BP solve runs (no reference counterpart; the reference
  * solves on its one thread, src/rotamer.cpp:1106-1162): 0 (default) on one
  * workgroup with the BP state in its LDS when it fits (the rest of the team

@@ -199,3 +199,90 @@ def test_config_writer_matches_reference_chig_up(chig):
                 np.testing.assert_allclose(b, a, rtol=2e-6, atol=2e-6, err_msg=p)
             else:
                 np.testing.assert_array_equal(b, a, err_msg=p)
+
+
+# ---------------------------------------------------- per-system team plans
+N_XCD = 8
+
+
+def _team_map(lib, teams, per_xcd):
+    t = np.ascontiguousarray(teams, dtype=np.intc)
+    fn = lib.upside_plan_team_map
+    fn.restype = ct.c_int
+    n = fn(t.ctypes.data_as(ct.c_void_p), ct.c_int(len(t)), ct.c_int(per_xcd), None, ct.c_int(0))
+    if n < 0:
+        return None
+    m = np.zeros(n, dtype=np.intc)
+    assert fn(t.ctypes.data_as(ct.c_void_p), ct.c_int(len(t)), ct.c_int(per_xcd),
+              m.ctypes.data_as(ct.c_void_p), ct.c_int(n)) == n
+    if n > 0:  # a buffer one short is refused
+        assert fn(t.ctypes.data_as(ct.c_void_p), ct.c_int(len(t)), ct.c_int(per_xcd),
+                  m.ctypes.data_as(ct.c_void_p), ct.c_int(n - 1)) == -1
+    return m
+
+
+def _check_map(m, teams, per_xcd):
+    """Every system's ranks 0..G-1 appear once, all on one XCD (b % 8), every
+    XCD holds at most per_xcd workgroups, and the rest is padding."""
+    seen = {}
+    for b, e in enumerate(m):
+        if e < 0:
+            assert e == -1
+            continue
+        s, g, r = e >> 14, (e >> 7) & 127, e & 127
+        assert g == teams[s] and 0 <= r < g
+        assert (s, r) not in seen
+        seen[(s, r)] = b % N_XCD
+    assert len(seen) == sum(teams)
+    for s, g in enumerate(teams):
+        assert len({seen[(s, r)] for r in range(g)}) == 1, s
+    load = np.bincount([x for x in seen.values()], minlength=N_XCD)
+    assert load.max() <= per_xcd
+    assert len(m) == N_XCD * load.max()
+
+
+def test_team_map_places_every_team_on_one_xcd(lib):
+    rng = np.random.default_rng(4)
+    for trial in range(200):
+        n = int(rng.integers(1, 80))
+        teams = [int(x) for x in rng.integers(1, 9, n)]
+        m = _team_map(lib, teams, 32)
+        if m is None:  # must really not fit: more than the device or a failed greedy near the limit
+            assert sum(teams) > 32 * N_XCD - 7 * N_XCD
+            continue
+        _check_map(m, teams, 32)
+    # uniform teams fill the XCDs evenly
+    m = _team_map(lib, [4] * 64, 32)
+    _check_map(m, [4] * 64, 32)
+    assert len(m) == 256 and np.all(m >= 0)
+    # a team larger than an XCD, or more workgroups than the device, does not fit
+    assert _team_map(lib, [33], 32) is None
+    assert _team_map(lib, [32] * 9, 32) is None
+    assert _team_map(lib, [0, 1], 32) is None
+
+
+def test_balanced_teams_follow_the_atom_counts(lib):
+    fn = lib.upside_plan_balanced_teams
+    fn.restype = ct.c_int
+
+    def plan(atoms, budget, per_xcd=32):
+        a = np.ascontiguousarray(atoms, dtype=np.intc)
+        t = np.zeros(len(a), dtype=np.intc)
+        rc = fn(a.ctypes.data_as(ct.c_void_p), ct.c_int(len(a)), ct.c_int(budget), ct.c_int(per_xcd),
+                t.ctypes.data_as(ct.c_void_p))
+        return None if rc else t
+
+    rng = np.random.default_rng(3)
+    atoms = 3 * rng.integers(50, 401, 64)
+    t = plan(atoms, 256)
+    assert t is not None and t.sum() <= 256 and t.min() >= 1
+    order = np.argsort(atoms, kind="stable")
+    assert np.all(np.diff(t[order]) >= 0)            # more atoms, no smaller team
+    assert (atoms / t).max() <= 2.0 * (atoms / t).min() + 1e-9 or t.min() == 1
+    _check_map(_team_map(lib, t, 32), list(t), 32)
+    # equal systems get equal teams; one system gets one XCD
+    assert np.all(plan([450] * 16, 256) == 16)
+    assert list(plan([1500], 256)) == [32]
+    # the budget bounds the total; too many systems is an error
+    assert plan(atoms, 100).sum() <= 100
+    assert plan([10] * 300, 256) is None

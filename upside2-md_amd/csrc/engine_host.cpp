@@ -1669,6 +1669,59 @@ struct System {
     void release() { (void)hipFree(arena); (void)hipFree(iarena); arena = nullptr; iarena = nullptr; }
 };
 
+// Place per-system teams t on the N_XCD XCDs (a team never spans two):
+// largest team first (ties: system order) onto the least loaded XCD, each XCD
+// holding at most per_xcd workgroups.  False if they do not fit; else, when
+// map is given, the block map (RunArgs::blockmap): workgroup b runs on XCD
+// b % N_XCD, each XCD's teams in system order, -1 for padding.
+static bool plan_team_map(const vector<int>& t, int per_xcd, vector<int>* map) {
+    vector<int> order(t.size());
+    for (size_t s = 0; s < t.size(); ++s) order[s] = (int)s;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return t[a] > t[b]; });
+    vector<int> load(N_XCD, 0), where(t.size(), 0);
+    for (int s : order) {
+        int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        if (t[s] < 1 || load[x] + t[s] > per_xcd) return false;
+        where[s] = x;
+        load[x] += t[s];
+    }
+    if (map) {
+        const int slots = *std::max_element(load.begin(), load.end());
+        map->assign((size_t)N_XCD * slots, -1);
+        vector<int> fill(N_XCD, 0);
+        for (size_t s = 0; s < t.size(); ++s) {
+            const int x = where[s];
+            for (int r = 0; r < t[s]; ++r) (*map)[(size_t)(fill[x] + r) * N_XCD + x] = ((int)s << 14) | (t[s] << 7) | r;
+            fill[x] += t[s];
+        }
+    }
+    return true;
+}
+
+// Team sizes in proportion to the systems' sizes (atom counts): starting from
+// one workgroup each, the system with the most atoms per workgroup (ties: the
+// lower index) gains a workgroup while the total stays within budget, no team
+// exceeds MAX_TEAM and the teams still fit (plan_team_map).  Empty if even
+// one workgroup per system does not fit.
+static vector<int> plan_balanced_teams(const vector<int>& atoms, int budget, int per_xcd) {
+    const int n = (int)atoms.size();
+    vector<int> t(n, 1);
+    if (n > budget || !plan_team_map(t, per_xcd, nullptr)) return {};
+    for (long long total = n; total < budget; ++total) {
+        int best = -1;
+        double bw = -1.0;
+        for (int s = 0; s < n; ++s) {
+            if (t[s] >= MAX_TEAM) continue;
+            const double w = (double)atoms[s] / t[s];
+            if (w > bw) { bw = w; best = s; }
+        }
+        if (best < 0) break;
+        t[best] += 1;
+        if (!plan_team_map(t, per_xcd, nullptr)) { t[best] -= 1; break; }
+    }
+    return t;
+}
+
 struct Batch {
     vector<std::shared_ptr<Program>> progs;
     vector<System> sys;
@@ -1985,34 +2038,8 @@ struct Batch {
         teams.clear();
     }
 
-    // Place per-system teams t on the XCDs (a team never spans two): largest
-    // team first (ties: system order) onto the least loaded XCD, each XCD
-    // holding at most its share of the resident workgroups.  False if they do
-    // not fit; else, when map is given, the block map: workgroup b runs on XCD
-    // b % N_XCD, each XCD's teams in system order.
     bool pack_teams(const vector<int>& t, vector<int>* map) const {
-        const int cap = upside_max_resident_blocks() / N_XCD;
-        vector<int> order(t.size());
-        for (size_t s = 0; s < t.size(); ++s) order[s] = (int)s;
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return t[a] > t[b]; });
-        vector<int> load(N_XCD, 0), where(t.size(), 0);
-        for (int s : order) {
-            int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-            if (load[x] + t[s] > cap) return false;
-            where[s] = x;
-            load[x] += t[s];
-        }
-        if (map) {
-            const int slots = *std::max_element(load.begin(), load.end());
-            map->assign((size_t)N_XCD * slots, -1);
-            vector<int> fill(N_XCD, 0);
-            for (size_t s = 0; s < t.size(); ++s) {
-                const int x = where[s];
-                for (int r = 0; r < t[s]; ++r) (*map)[(size_t)(fill[x] + r) * N_XCD + x] = ((int)s << 14) | (t[s] << 7) | r;
-                fill[x] += t[s];
-            }
-        }
-        return true;
+        return plan_team_map(t, upside_max_resident_blocks() / N_XCD, map);
     }
 
     void set_teams(const vector<int>& v) {
@@ -2038,28 +2065,15 @@ struct Batch {
         team = *std::max_element(v.begin(), v.end());
     }
 
-    // Team sizes for the systems in proportion to their sizes: starting from
-    // one workgroup each, the system with the most atoms per workgroup (ties:
-    // the lower index) gains a workgroup while the total stays within budget
-    // (<= 0: the device's resident workgroups) and the teams still fit
-    // (pack_teams).  Depends on the systems only.
+    // Balanced per-system teams (plan_balanced_teams) within budget
+    // workgroups (<= 0: the device's resident workgroups).
     vector<int> balanced_teams(int budget) const {
-        const int n = (int)sys.size();
-        if (budget <= 0) budget = upside_max_resident_blocks();
-        vector<int> t(n, 1);
-        if (n > budget || !pack_teams(t, nullptr)) fail("balanced teams: more systems than resident workgroups");
-        for (long long total = n; total < budget; ++total) {
-            int best = -1;
-            double bw = -1.0;
-            for (int s = 0; s < n; ++s) {
-                if (t[s] >= MAX_TEAM) continue;
-                const double w = (double)sys[s].prog->T.n_atom / t[s];
-                if (w > bw) { bw = w; best = s; }
-            }
-            if (best < 0) break;
-            t[best] += 1;
-            if (!pack_teams(t, nullptr)) { t[best] -= 1; break; }
-        }
+        const int resident = upside_max_resident_blocks();
+        if (budget <= 0) budget = resident;
+        vector<int> atoms(sys.size());
+        for (size_t s = 0; s < sys.size(); ++s) atoms[s] = sys[s].prog->T.n_atom;
+        vector<int> t = plan_balanced_teams(atoms, budget, resident / N_XCD);
+        if (t.empty()) fail("balanced teams: more systems than resident workgroups");
         return t;
     }
 
@@ -4066,6 +4080,33 @@ int upside_batch_set_team(UpsideBatch* b, int team) {
 }
 
 int upside_batch_get_team(UpsideBatch* b) { return b ? b->batch.team : -1; }
+
+int upside_plan_team_map(const int* teams, int n_sys, int per_xcd, int* map, int map_len) {
+    try {
+        if (!teams || n_sys < 1 || per_xcd < 1) return -1;
+        vector<int> m;
+        if (!plan_team_map(vector<int>(teams, teams + n_sys), per_xcd, &m)) return -1;
+        if (map) {
+            if ((int)m.size() > map_len) return -1;
+            std::copy(m.begin(), m.end(), map);
+        }
+        return (int)m.size();
+    } catch (...) {
+        return -1;
+    }
+}
+
+int upside_plan_balanced_teams(const int* n_atom, int n_sys, int budget, int per_xcd, int* teams) {
+    try {
+        if (!n_atom || !teams || n_sys < 1 || per_xcd < 1) return 1;
+        vector<int> t = plan_balanced_teams(vector<int>(n_atom, n_atom + n_sys), budget, per_xcd);
+        if (t.empty()) return 1;
+        std::copy(t.begin(), t.end(), teams);
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}
 
 int upside_batch_set_teams(UpsideBatch* b, const int* teams, int n_sys) {
     ABI_TRY

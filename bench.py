@@ -394,7 +394,7 @@ def batched_workloads(work, rounds=60, cpu=True):
     line = {"workload": "16 replicas x 150 residues, T geometric 0.80-0.95, neighbour swap sets alternating "
                         "every 21 steps, exchanged inside the persistent launch",
             "value": 16 * 150 * 3 * rounds / wall, "unit": "residue-steps/s", "steps": 3 * rounds,
-            "seconds": wall, "workgroups_per_system": teams(b), "bp_layout": b.bp_layout(),
+            "seconds": wall, "workgroups_per_system": teams(b), "bp_layout": b.bp_layout(), "lane_split": b.lane_split(),
             "swaps_accepted_attempted": [int(suc.sum()), int(att.sum())],
             "bp_iters_per_solve": bp_iters(b, st0, st1),
             "roofline": batch_roofline(b, [p] * 16, st0, st1, 3 * rounds, b.last_run_ms())}
@@ -423,7 +423,7 @@ def batched_workloads(work, rounds=60, cpu=True):
     st1 = [b.bp_stats(s) for s in range(64)]
     line = {"workload": "64 independent chains, lengths uniform in [50, 400] (%d residues)" % sum(lens),
             "value": sum(lens) * 3 * rounds / wall, "unit": "residue-steps/s", "steps": 3 * rounds,
-            "seconds": wall, "workgroups_per_system": teams(b), "bp_layout": b.bp_layout(),
+            "seconds": wall, "workgroups_per_system": teams(b), "bp_layout": b.bp_layout(), "lane_split": b.lane_split(),
             "bp_iters_per_solve": bp_iters(b, st0, st1),
             "roofline": batch_roofline(b, paths, st0, st1, 3 * rounds, b.last_run_ms())}
     if cpu:
@@ -548,6 +548,8 @@ def main():
     ap.add_argument("--no-warm-start", action="store_true", help="skip the warm-start workloads")
     ap.add_argument("--team", type=int, default=0,
                     help="workgroups per system (0: time 1..32 on the warmed state and keep the fastest)")
+    ap.add_argument("--lane-split", type=int, default=2, choices=(2, 4),
+                    help="sections on quarters (4) or halves (2) of the team, with --team (default: autotuned)")
     ap.add_argument("--bp-layout", type=int, default=-1,
                     help="rotamer solve layout with --team (0 auto, 1 team, 2 one workgroup, 3 split; "
                          "default: autotuned with the team size, or 0 with --team)")
@@ -592,10 +594,12 @@ def main():
     if args.team > 0:
         b.set_team(args.team)
         b.set_bp_layout(max(args.bp_layout, 0))
+        b.set_lane_split(args.lane_split)
     else:
         cand = [t for t in (1, 2, 4, 8, 16, 24, 28, 32) if t <= b.max_team()]
         tune = b.autotune_layout(6, cand, (0, 1, 2, 3))  # restores the warmed state after each candidate
-        log("team / BP layout autotune (ms per 6 rounds): %s -> team %d layout %d" % (tune, b.team(), b.bp_layout()))
+        log("team / BP layout autotune (ms per 6 rounds): %s -> team %d layout %d lane split %d"
+            % (tune, b.team(), b.bp_layout(), b.lane_split()))
     probe.energy(b.pos(0))
     g0, bp0, nodes = graph_counts(probe, node_names)
     small = small_node_bytes(probe, node_names, set(g0))
@@ -641,7 +645,8 @@ def main():
         "config": {"workload": "one %d-residue seeded random-sequence chain (seed %d), random-Rama start, "
                                "ff_2.1, FP32, T=%.2f, dt=%.3f, thermostat every round" % (N_RES, SEED, TEMP, DT),
                    "n_res": N_RES, "n_atom": n_atom, "launch": "one persistent launch of all steps",
-                   "workgroups_per_system": b.team(), "bp_layout": b.bp_layout()},
+                   "workgroups_per_system": b.team(), "bp_layout": b.bp_layout(),
+                   "lane_split": b.lane_split()},
         "bp_iters_per_step": iters / max(solves, 1),
         "bp_passes_per_solve": passes / max(solves, 1),
         "bp_iters_per_solve_p50_p99": [percentile_from_hist(hist, 0.5), percentile_from_hist(hist, 0.99)],

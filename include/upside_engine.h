@@ -260,6 +260,14 @@ int upside_batch_balanced_teams(UpsideBatch* batch, int budget, int* teams, int 
  * the atom counts within budget workgroups; 0 on success. */
 int upside_plan_team_map(const int* teams, int n_sys, int per_xcd, int* map, int map_len);
 int upside_plan_balanced_teams(const int* n_atom, int n_sys, int budget, int per_xcd, int* teams);
+/* Independent ops of a step run side by side in sections of four lanes (no
+ * reference counterpart).  On a team of eight or more workgroups, split 4
+ * runs each lane on a quarter of the team and split 2 (default) runs two
+ * lanes on each half, one after the other; the results are the same.  The
+ * autotune calls try both after choosing the team and layout.  0 on success;
+ * get returns the split. */
+int upside_batch_set_lane_split(UpsideBatch* batch, int split);
+int upside_batch_get_lane_split(UpsideBatch* batch);
 /* Where each rotamer BP solve runs (no reference counterpart; the reference
  * solves on its one thread, src/rotamer.cpp:1106-1162): 0 (default) on one
  * workgroup with the BP state in its LDS when it fits (the rest of the team

@@ -968,6 +968,29 @@ def test_team_execution_matches_single_workgroup(ue, chain_file, team):
     np.testing.assert_allclose(bt.energies(), b1.energies(), rtol=1e-5)
 
 
+def test_section_lane_split_matches_single_workgroup(ue, chain_file):
+    """The four lanes of a section run on quarters of a team of eight
+    (split 4) or two by two on its halves (split 2): both follow the
+    single-workgroup trajectory bit for bit, the lanes being independent."""
+    path = chain_file(150, 72)
+    b1 = ue.Batch([path] * 2, [0.85, 0.9], seed=7)
+    b1.run(4)
+    for split in (2, 4):
+        bt = ue.Batch([path] * 2, [0.85, 0.9], seed=7)
+        if bt.max_team() < 8:
+            pytest.skip("no team of eight fits")
+        bt.set_team(8)
+        bt.set_lane_split(split)
+        assert bt.lane_split() == split
+        say("case: team 8, lane split %d, L=150" % split)
+        bt.run(4)
+        for s in range(2):
+            np.testing.assert_array_equal(bt.pos(s), b1.pos(s))
+            np.testing.assert_array_equal(bt.mom(s), b1.mom(s))
+    with pytest.raises(RuntimeError):
+        bt.set_lane_split(3)
+
+
 def test_per_system_teams_match_single_workgroup(ue, chain_file):
     """Every system with its own team size (sizes 1, 3, 6, 5 and 2, so odd
     and uneven splits into halves too; each team placed on one XCD) follows

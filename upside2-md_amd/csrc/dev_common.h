@@ -54,6 +54,9 @@ struct Ctx {
     unsigned long long* bar_hi;  // sub-team of ranks G/2..G-1
     float* tred_hi;
     int tpar_hi;
+    unsigned long long* bar_q;   // the four quarter sub-teams: counters bar_q[j], scratch tred_q + 2*MAX_TEAM*j,
+    float* tred_q;               // and the parity of this workgroup's own quarter
+    int tpar_q;
     float* tred;       // team reduction scratch, 2 buffers of G floats
     int tpar;          // buffer parity of the next team reduction
     int bp_layout;     // where rotamer solves run (see bp_on_one_wg)

@@ -224,7 +224,8 @@ struct SysDesc {
     float mom_scale, noise_scale;
     uint64_t n_invocations;
     float temp0;           // initial temperature (annealing schedule)
-    int error;             // int offset of error word
+    int error;             // int offset of error word (then one per section sub-team)
+    int n_lanes;           // lanes of the program's sections (2 or 4)
     int bar;               // int offset (16-byte aligned) of the team barrier words: [0..1] 64-bit
                            // arrival counter, [2] spin-timeout flag, [3] placement report, [4..4+MAX_TEAM)
                            // XCD of each workgroup, [4+MAX_TEAM..+2) 64-bit counter of the sub-team
@@ -287,6 +288,10 @@ struct RunArgs {
     // packs every team onto one XCD
     const int* blockmap;
     int n_blocks;
+    // sections of four-lane programs on teams of eight or more workgroups:
+    // 4 runs each lane on a quarter of the team, else two halves run two
+    // lanes each (the same results either way: the lanes are independent)
+    int lane_split;
     // simulated annealing (src/main.cpp:593-612, :975-977): when anneal is set
     // the thermostat temperature at round r is the schedule at time
     // 3 dt (r+1) from each system's initial temperature
@@ -312,10 +317,15 @@ struct RunArgs {
 constexpr int MAX_TEAM = 64;
 // Team words: [0..1] the team's 64-bit arrival counter, [2] spin-timeout flag,
 // [3] placement report, [4..4+MAX_TEAM) XCD of each workgroup, then the
-// 64-bit arrival counters of the sub-teams: ranks 1..G-1, ranks 0..G/2-1 and
-// ranks G/2..G-1 (each stays a multiple of its own size).
-constexpr int BAR_WORDS = 4 + MAX_TEAM + 6;
+// 64-bit arrival counters of the sub-teams: ranks 1..G-1, ranks 0..G/2-1,
+// ranks G/2..G-1, and the four quarters [j*G/4, (j+1)*G/4) (each stays a
+// multiple of its own size).
+constexpr int BAR_WORDS = 4 + MAX_TEAM + 14;
 constexpr int BAR_SUB1 = 4 + MAX_TEAM, BAR_LO = 4 + MAX_TEAM + 2, BAR_HI = 4 + MAX_TEAM + 4;
+constexpr int BAR_Q = 4 + MAX_TEAM + 6;
+// Lanes of a program's sections (OpRec::lane < MAX_LANES); error words
+// error+1 .. error+MAX_LANES belong to the sub-teams of a section.
+constexpr int MAX_LANES = 4;
 // Workgroups of a launch are dispatched round-robin over this many XCDs
 // (MI355X: 8 XCDs of 32 CUs, each with its own L2).  Team launches place every
 // team on one XCD (see engine_kernels.hip), which the kernel verifies.

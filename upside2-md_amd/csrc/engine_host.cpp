@@ -1109,9 +1109,10 @@ static void load_topology(Topology& T, H5& h5, int n_atom) {
     }
     T.pot = T.falloc(std::max(T.n_pot, 1));
     T.mom = T.falloc(n_atom * 3);
-    T.err = T.ialloc(4);
+    T.err = T.ialloc(2 + MAX_LANES);  // the team's word, then one per section sub-team
     T.bar = T.ialloc(BAR_WORDS);
-    T.tred = T.falloc(8 * MAX_TEAM);  // two buffers each: the team, sub-teams 1..G-1, 0..G/2-1, G/2..G-1
+    // two buffers each: the team, sub-teams 1..G-1, 0..G/2-1, G/2..G-1, the four quarters
+    T.tred = T.falloc(16 * MAX_TEAM);
     vector<int> z;
     for (auto& r : T.sens_zero) { z.push_back(r.first); z.push_back(r.second); }
     T.szero = T.ialloc(z.size());
@@ -1371,6 +1372,7 @@ struct Program {
     //     every descendant of these are deferred past the section, in their
     //     order.  The two halves of a team run the two lanes at once.
     vector<int> lane_of;                        // node -> lane (0 outside sections)
+    int n_lanes = MAX_LANES;                    // lanes per section (UPSIDE_LANES=2: two)
     int rot_graph_node = -1;                    // rotamer whose pair graph lane 0 of the section refreshes
     vector<std::pair<int, int>> lane_sections;  // [first op, end op) in program order
     // rough relative cost of a pair-graph op by its edge kind (a 500-residue
@@ -1411,8 +1413,9 @@ struct Program {
                 if (anc[i][q] && !elementwise(q)) pure = false;
             (pure ? first : rest).push_back(i);
         }
-        vector<int> in_lane[2], deferred;
-        int work[2] = {0, 0};
+        const int NL = n_lanes;
+        vector<vector<int>> in_lane(NL);
+        vector<int> deferred, work(NL, 0);
         // the first rotamer's pair-list refresh and edge evaluation read only
         // its bead placements: when those run before the section, lane 0
         // starts with them
@@ -1444,13 +1447,28 @@ struct Program {
             }
             return false;
         };
+        // lane of op i: the one lane it is tied to, the least loaded lane
+        // (lowest index on ties) when it is tied to none, -1 (deferred past
+        // the section) when tied to two or more
+        auto pick = [&](int i, const vector<vector<int>>& lanes, const vector<int>& w) {
+            int tl = -1, nt = 0;
+            for (int l = 0; l < NL; ++l)
+                if (tied(i, lanes[l])) { tl = l; ++nt; }
+            if (nt > 1) return -1;
+            if (nt == 1) return tl;
+            int best = 0;
+            for (int l = 1; l < NL; ++l)
+                if (w[l] < w[best]) best = l;
+            return best;
+        };
         // the elementwise ops themselves form a first section: one workgroup
-        // of each half runs its lane.  The affine alignment, whose eigen-solve
-        // costs about eight of the others, starts lane 1; the rest are dealt
-        // in order as above, one unit each.
-        vector<int> fl[2], fdef;
+        // of each sub-team runs its lane.  The affine alignment, whose
+        // eigen-solve costs about eight of the others, starts lane 1; the rest
+        // are dealt in order as above, one unit each.
+        vector<vector<int>> fl(NL);
+        vector<int> fdef;
         {
-            int fw[2] = {0, 0};
+            vector<int> fw(NL, 0);
             for (int i : first)
                 if (T.ops[T.nodes[i].op].kind == OP_AFFINE && fl[1].empty()) {
                     fl[1].push_back(i);
@@ -1459,18 +1477,19 @@ struct Program {
                 }
             for (int i : first) {
                 if (!fl[1].empty() && i == fl[1][0]) continue;
-                bool defer = tied(i, fl[0]) && tied(i, fl[1]);
-                for (int j : fdef) defer = defer || anc[i][j];
-                if (defer) { fdef.push_back(i); continue; }
-                const bool t0 = tied(i, fl[0]), t1 = tied(i, fl[1]);
-                const int lane = t0 ? 0 : t1 ? 1 : (fw[1] < fw[0] ? 1 : 0);
+                int lane = pick(i, fl, fw);
+                for (int j : fdef) lane = anc[i][j] ? -1 : lane;
+                if (lane < 0) { fdef.push_back(i); continue; }
                 fl[lane].push_back(i);
                 fw[lane] += 1;
                 lane_of[i] = lane;
             }
-            if (fl[0].size() < 2 || fl[1].size() < 2) {
-                for (int l = 0; l < 2; ++l) for (int i : fl[l]) lane_of[i] = 0;
-                fl[0].clear(); fl[1].clear(); fdef.clear();
+            int busy = 0;
+            for (int l = 0; l < NL; ++l) busy += fl[l].size() >= 2;
+            if (busy < 2) {
+                for (int l = 0; l < NL; ++l) for (int i : fl[l]) lane_of[i] = 0;
+                for (auto& v : fl) v.clear();
+                fdef.clear();
             }
         }
         // pair-graph ops after the first rotamer op that do not depend on it
@@ -1483,26 +1502,29 @@ struct Program {
             const int i = rest[r];
             const bool rot = T.ops[T.nodes[i].op].kind == OP_ROTAMER;
             if (rot && after_rot) break;  // one rotamer per section
-            // tied to both lanes, a rotamer, or descending from a deferred op:
+            // tied to two lanes, a rotamer, or descending from a deferred op:
             // after the section
-            bool defer = rot || (tied(i, in_lane[0]) && tied(i, in_lane[1])) || (after_rot && elementwise(i));
+            const int lane = rot ? -1 : pick(i, in_lane, work);
+            bool defer = rot || lane < 0 || (after_rot && elementwise(i));
             for (int j : deferred) defer = defer || anc[i][j];
             if (rot) after_rot = true;
             if (defer) { deferred.push_back(i); continue; }
-            const bool t0 = tied(i, in_lane[0]), t1 = tied(i, in_lane[1]);
-            const int lane = t0 ? 0 : t1 ? 1 : (work[1] < work[0] ? 1 : 0);
             in_lane[lane].push_back(i);
             work[lane] += elementwise(i) ? 1 : lane_weight(T.ops[T.nodes[i].op].graph);
             lane_of[i] = lane;
         }
-        // a section needs pair-graph work in both lanes
+        // a section needs pair-graph work in two lanes at least
         auto has_pair = [&](const vector<int>& lane) {
             for (int i : lane)
                 if (!elementwise(i)) return true;
             return false;
         };
+        int pair_lanes = 0;
+        for (int l = 0; l < NL; ++l) pair_lanes += has_pair(in_lane[l]);
         T.germ.assign(1, 0);
-        if (!fl[0].empty()) {  // the elementwise section, then its deferred ops
+        bool any_fl = false;
+        for (auto& v : fl) any_fl = any_fl || !v.empty();
+        if (any_fl) {  // the elementwise section, then its deferred ops
             for (int i : first)
                 if (std::find(fdef.begin(), fdef.end(), i) == fdef.end()) T.germ.push_back(i);
             lane_sections.push_back({0, (int)T.germ.size() - 1});
@@ -1510,8 +1532,8 @@ struct Program {
         } else {
             T.germ.insert(T.germ.end(), first.begin(), first.end());
         }
-        if (!(has_pair(in_lane[0]) && has_pair(in_lane[1]))) {
-            for (int l = 0; l < 2; ++l) for (int i : in_lane[l]) lane_of[i] = 0;
+        if (pair_lanes < 2) {
+            for (int l = 0; l < NL; ++l) for (int i : in_lane[l]) lane_of[i] = 0;
             rot_graph_node = -1;
             T.germ.insert(T.germ.end(), rest.begin(), rest.end());
             return;
@@ -1542,6 +1564,7 @@ struct Program {
         const char* so = std::getenv("UPSIDE_SOLO_OPS");
         const bool solo_ok = !(so && so[0] == '0');
         const char* ln = std::getenv("UPSIDE_LANES");
+        n_lanes = (ln && ln[0] == '2') ? 2 : MAX_LANES;
         if (solo_ok && !(ln && ln[0] == '0')) order_germ_for_lanes();
         else lane_of.assign(T.nodes.size(), 0), lane_sections.clear(), rot_graph_node = -1;
         prog.clear();
@@ -1884,6 +1907,7 @@ struct Batch {
         d.sens_zero = T.szero;
         d.mom = T.mom;
         d.error = T.err;
+        d.n_lanes = P.n_lanes;
         d.bar = T.bar;
         d.tred = T.tred;
         d.seed = base_seed + (uint32_t)k;
@@ -1990,6 +2014,7 @@ struct Batch {
         push_desc();
     }
 
+    int lane_split = 2;  // sections of four-lane programs: 4 quarters or 2 halves (RunArgs::lane_split)
     int team = 1;   // workgroups per system (set_team / autotune); with per-system teams their largest
     // per-system team sizes (set_teams; empty: every system has `team`) and the
     // device block map of their launches (RunArgs::blockmap)
@@ -2130,11 +2155,27 @@ struct Batch {
         if (best == 0) set_teams(balanced);
         else set_team(best);
         bp_layout = best_lay;
+        // then the section split of four-lane programs (quarters or halves;
+        // the same results either way) where a team has eight workgroups
+        bool four = false;
+        for (size_t s = 0; s < sys.size(); ++s) four = four || (sys[s].prog->n_lanes == 4 && team_of(s) >= 8);
+        if (four) {
+            int best_split = lane_split;
+            float split_ms = 3.0e38f;
+            for (int split : {2, 4}) {
+                lane_split = split;
+                run(n_round, 0, nullptr, 0, (int)round_num);
+                if (last_ms < split_ms) { split_ms = last_ms; best_split = split; }
+                restore();
+            }
+            lane_split = best_split;
+        }
     }
 
     void launch(const RunArgs& a0) {
         RunArgs a = a0;
         a.team = team;
+        a.lane_split = lane_split;
         if (!teams.empty()) {
             a.blockmap = d_blockmap;
             a.n_blocks = n_blocks;
@@ -2160,8 +2201,12 @@ struct Batch {
                 auto word64 = [&](int k) { return (unsigned long long)w[k] | ((unsigned long long)w[k + 1] << 32); };
                 const unsigned long long g = (unsigned long long)team_of(s), lo = g / 2, hi = g - g / 2;
                 if (w[2]) fail("team barrier wait timed out: results of this run are void");
+                bool quarters_ok = true;
+                if (sy.prog->n_lanes == 4 && g >= 8)
+                    for (unsigned long long j = 0; j < 4; ++j)
+                        quarters_ok = quarters_ok && word64(BAR_Q + 2 * (int)j) % ((j + 1) * g / 4 - j * g / 4) == 0;
                 if (word64(0) % g || (g > 2 && word64(BAR_SUB1) % (g - 1)) || (g >= 4 && word64(BAR_LO) % lo) ||
-                    (g >= 4 && word64(BAR_HI) % hi))
+                    (g >= 4 && word64(BAR_HI) % hi) || !quarters_ok)
                     fail("team barrier arrivals unbalanced at the end of a launch: results of this run are void");
                 team_local = team_local && w[3] != 0;
             }
@@ -4080,6 +4125,16 @@ int upside_batch_set_team(UpsideBatch* b, int team) {
 }
 
 int upside_batch_get_team(UpsideBatch* b) { return b ? b->batch.team : -1; }
+
+int upside_batch_set_lane_split(UpsideBatch* b, int split) {
+    ABI_TRY
+    if (split != 2 && split != 4) fail("lane split must be 2 or 4");
+    b->batch.lane_split = split;
+    return 0;
+    ABI_CATCH(1)
+}
+
+int upside_batch_get_lane_split(UpsideBatch* b) { return b ? b->batch.lane_split : -1; }
 
 int upside_plan_team_map(const int* teams, int n_sys, int per_xcd, int* map, int map_len) {
     try {

@@ -972,24 +972,25 @@ __device__ void run_early_derivs(Ctx& s, const SysDesc& S) {
 // the two lanes independent: neither reads what the other writes).  A lane's
 // solo ops up to its next team op run on the half's rank 0.  Returns false
 // when an op set the half's error word.
-__device__ bool run_lane(Ctx& s, const SysDesc& S, int k0, int k1, int lane, int mode) {
+__device__ bool run_lane(Ctx& s, const SysDesc& S, int k0, int k1, unsigned lanes, int mode) {
     const OpRec& first = S.ops[k0];
-    if (first.rot_graph_op >= 0 && first.rot_graph_lane == lane) {
+    auto mine = [&](const OpRec& o) { return ((lanes >> o.lane) & 1u) != 0u; };
+    if (first.rot_graph_op >= 0 && ((lanes >> first.rot_graph_lane) & 1u)) {
         const OpRec& ro = S.ops[first.rot_graph_op];
         s.cur_op = first.rot_graph_op;
         if (!rotamer_graph(s, S, ro, S.graphs[ro.graph], S.rots[ro.rot])) return false;
     }
     for (int k = k0; k < k1;) {
         const OpRec& op = S.ops[k];
-        if (op.lane != lane) { ++k; continue; }
+        if (!mine(op)) { ++k; continue; }
         s.cur_op = k;
         if (s.G > 1 && op.solo) {
             int k2 = k;
-            while (k2 < k1 && (S.ops[k2].lane != lane || S.ops[k2].solo)) ++k2;
+            while (k2 < k1 && (!mine(S.ops[k2]) || S.ops[k2].solo)) ++k2;
             if (s.rank == 0) {
                 Ctx l = solo_ctx(s);
                 for (int q = k; q < k2; ++q) {
-                    if (S.ops[q].lane != lane) continue;
+                    if (!mine(S.ops[q])) continue;
                     unsigned long long t0 = ptick(l);
                     run_value(l, S, S.ops[q], mode);
                     prof_op(l, q, t0);
@@ -1023,24 +1024,36 @@ __device__ bool run_germ_range(Ctx& c, const SysDesc& S, int k0, int k1, int mod
         const OpRec& op = S.ops[k];
         c.cur_op = k;
         if (SPLIT && op.lane_end > k && op.lane_end <= k1 && c.G >= 4) {
-            // two-lane section: each half-team runs its lane with its own
-            // barrier counter, reduction scratch and error word; the join
-            // barrier makes both halves' error words readable by everyone
-            const int gb = c.G / 2;
-            if (c.rank < gb) {
-                Ctx b = sub_team(c, 0, gb, c.bar_lo, c.tred_lo, c.tpar_lo);
-                b.err = S.error + 1;
-                run_lane(b, S, k, op.lane_end, 0, mode);
-                c.tpar_lo = b.tpar;
+            // section: the team splits into nq sub-teams (four quarters when
+            // the program has four lanes, the team eight or more workgroups
+            // and the launch asks for them, else two halves), each with its own barrier
+            // counter, reduction scratch and error word; sub-team j runs
+            // lane j, or lanes j and j+2 (four lanes on two halves) in program
+            // order.  The join barrier makes every error word readable.
+            const int nl = S.n_lanes;
+            const int nq = (nl == 4 && c.G >= 8 && c.args->lane_split == 4) ? 4 : 2;
+            int j = 0;
+            while (j + 1 < nq && c.rank >= (j + 1) * c.G / nq) ++j;
+            const int r0 = j * c.G / nq, r1 = (j + 1) * c.G / nq;
+            const unsigned mask = (nq == nl) ? (1u << j) : ((1u << j) | (1u << (j + 2)));
+            if (nq == 2) {
+                Ctx s = sub_team(c, r0, r1 - r0, j ? c.bar_hi : c.bar_lo, j ? c.tred_hi : c.tred_lo,
+                                 j ? c.tpar_hi : c.tpar_lo);
+                if (s.rank == 0) s.prof = S.prof;  // profiling: each sub-team's ops are timed by its rank 0
+                s.err = S.error + 1 + j;
+                run_lane(s, S, k, op.lane_end, mask, mode);
+                (j ? c.tpar_hi : c.tpar_lo) = s.tpar;
             } else {
-                Ctx o = sub_team(c, gb, c.G - gb, c.bar_hi, c.tred_hi, c.tpar_hi);
-                if (o.rank == 0) o.prof = S.prof;  // profiling: lane 1's ops are timed by its rank 0
-                o.err = S.error + 2;
-                run_lane(o, S, k, op.lane_end, 1, mode);
-                c.tpar_hi = o.tpar;
+                Ctx s = sub_team(c, r0, r1 - r0, c.bar_q + j, c.tred_q + 2 * MAX_TEAM * j, c.tpar_q);
+                if (s.rank == 0) s.prof = S.prof;
+                s.err = S.error + 1 + j;
+                run_lane(s, S, k, op.lane_end, mask, mode);
+                c.tpar_q = s.tpar;
             }
             team_sync(c);
-            if (c.I[S.error + 1] | c.I[S.error + 2]) {
+            int lane_err = 0;
+            for (int q = 1; q <= nq; ++q) lane_err |= c.I[S.error + q];
+            if (lane_err) {
                 if (c.gtid == 0) c.I[c.err] = 1;
                 team_sync(c);
                 return false;
@@ -1389,6 +1402,9 @@ __device__ __forceinline__ void engine_body(SysDesc* systems, const RunArgs& arg
     c.bar_hi = reinterpret_cast<unsigned long long*>(S.iarena + S.bar + BAR_HI);
     c.tred_hi = c.tred + 6 * MAX_TEAM;
     c.tpar_hi = 0;
+    c.bar_q = reinterpret_cast<unsigned long long*>(S.iarena + S.bar + BAR_Q);
+    c.tred_q = c.tred + 8 * MAX_TEAM;
+    c.tpar_q = 0;
     c.bp_layout = args.bp_layout;
     c.args = &args;
     team_place(c, S.iarena + S.bar + 4);

@@ -84,6 +84,8 @@ def load_upside_library(path=LIB_PATH):
         "upside_batch_max_team": (i, [vp]),
         "upside_batch_set_team": (i, [vp, i]),
         "upside_batch_set_teams": (i, [vp, vp, i]),
+        "upside_batch_set_lane_split": (i, [vp, i]),
+        "upside_batch_get_lane_split": (i, [vp]),
         "upside_batch_get_teams": (i, [vp, vp, i]),
         "upside_batch_balanced_teams": (i, [vp, i, vp, i]),
         "upside_batch_get_team": (i, [vp]),
@@ -396,6 +398,14 @@ class Batch:
     def set_team(self, team):
         if self.calc.upside_batch_set_team(self.h, int(team)):
             raise RuntimeError("team size %d not possible" % team)
+
+    def set_lane_split(self, split):
+        """Sections on teams of eight or more: 4 quarters or 2 halves."""
+        if self.calc.upside_batch_set_lane_split(self.h, int(split)):
+            raise RuntimeError("lane split %d not possible" % split)
+
+    def lane_split(self):
+        return int(self.calc.upside_batch_get_lane_split(self.h))
 
     def set_teams(self, teams):
         """One team size per system (each team on one XCD, all resident)."""

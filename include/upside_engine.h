@@ -258,6 +258,16 @@ int upside_batch_balanced_teams(UpsideBatch* batch, int budget, int* teams, int 
  * or -1 if the teams do not fit or map_len is too small (map may be NULL to
  * ask for the length).  upside_plan_balanced_teams fills teams[n_sys] from
  * the atom counts within budget workgroups; 0 on success. */
+/* Lowers the .up file's node graph on the host (no device calls) and checks
+ * the invariants the kernel relies on: the value pass is a topological order,
+ * ops in different lanes of a section are independent (no dependency, no
+ * shared pair graph, no potentials pushing into a shared input), the
+ * rotamer's pair-list refresh and overlap ops read nothing they must wait
+ * for, and the deriv pass runs each node after its consumers.  0 when all
+ * hold (msg: a summary), 1 on a violation (msg: the first one), -1 when the
+ * file cannot be lowered.  No reference counterpart (the scheduling it
+ * checks replaces src/deriv_engine.cpp:127-178). */
+int upside_check_program(const char* up_path, char* msg, int msg_len);
 int upside_plan_team_map(const int* teams, int n_sys, int per_xcd, int* map, int map_len);
 int upside_plan_balanced_teams(const int* n_atom, int n_sys, int budget, int per_xcd, int* teams);
 /* Independent ops of a step run side by side in sections of four lanes (no

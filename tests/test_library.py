@@ -286,3 +286,40 @@ def test_balanced_teams_follow_the_atom_counts(lib):
     # the budget bounds the total; too many systems is an error
     assert plan(atoms, 100).sum() <= 100
     assert plan([10] * 300, 256) is None
+
+
+# ------------------------------------------------ lowered program invariants
+@pytest.mark.parametrize("lanes", ["4", "2", "0"])
+def test_lowered_program_invariants(lib, tmp_path, monkeypatch, lanes):
+    """The host lowering of the node graph (value-pass order, lane sections,
+    the rotamer's overlap range and early deriv ops) keeps the invariants the
+    kernel relies on, for chains of several sizes, the chignolin fixture, and
+    with sections of four lanes, of two, or none (upside_check_program runs
+    on the host only)."""
+    import upconfig
+    monkeypatch.setenv("UPSIDE_LANES", lanes)
+    paths = []
+    for L in (20, 150, 500):
+        p = str(tmp_path / ("c%d.up" % L))
+        upconfig.make_chain(p, L, seed=5)
+        paths.append(p)
+    z = np.load(os.path.join(ROOT, "tests", "golden", "chig_frames.npz"))
+    g = upconfig.build_ff21([s.decode() for s in z["sequence"]], z["initial_pos"], rama_pot=z["rama_pot"],
+                            normalize_rama=False)
+    chig = str(tmp_path / "chig.up")
+    upconfig.write_up(chig, g)
+    paths.append(chig)
+    fn = lib.upside_check_program
+    fn.restype = ct.c_int
+    for p in paths:
+        buf = ct.create_string_buffer(1024)
+        rc = fn(p.encode(), buf, len(buf))
+        msg = buf.value.decode()
+        assert rc == 0, (p, msg)
+        n_sec = int(re.search(r"(\d+) sections", msg).group(1))
+        if lanes == "0":
+            assert n_sec == 0, msg
+        else:
+            assert n_sec >= 1 and ("of %s lanes" % lanes) in msg, msg
+    buf = ct.create_string_buffer(256)
+    assert fn(str(tmp_path / "missing.up").encode(), buf, len(buf)) == -1

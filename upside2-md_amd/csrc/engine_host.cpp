@@ -1350,7 +1350,8 @@ struct Program {
     }
 
     void release() {
-        (void)hipFree(d_ops); (void)hipFree(d_graphs); (void)hipFree(d_rots); (void)hipFree(d_cf); (void)hipFree(d_ci);
+        for (void* p : {(void*)d_ops, (void*)d_graphs, (void*)d_rots, (void*)d_cf, (void*)d_ci})
+            if (p) (void)hipFree(p);
         d_ops = nullptr; d_graphs = nullptr; d_rots = nullptr; d_cf = nullptr; d_ci = nullptr;
     }
     ~Program() { release(); }
@@ -1547,7 +1548,7 @@ struct Program {
         T.germ.insert(T.germ.end(), rest.begin() + r, rest.end());
     }
 
-    void load(int n_atom, double cap_scale) {
+    void load(int n_atom, double cap_scale, bool upload = true) {
         H5 h5(path);
         if (n_atom <= 0) n_atom = (int)h5.shape("/input/pos")[0];
         T.cap_scale = cap_scale;
@@ -1669,6 +1670,7 @@ struct Program {
             for (size_t d = 0; d < T.deriv.size(); ++d) any = any || prog[ng + d].early;
             if (!any) prog[k].overlap_end = 0;
         }
+        if (!upload) return;  // lowering only (check_program)
         auto up = [](void** dst, const void* src, size_t bytes, const char* what) {
             hip_check(hipMalloc(dst, std::max(bytes, (size_t)16)), what);
             if (bytes) hip_check(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), what);
@@ -1681,6 +1683,105 @@ struct Program {
     }
     int n_germ() const { return (int)T.germ.size() - 1; }
     int n_deriv() const { return (int)T.deriv.size(); }
+
+    // Invariants of the lowered program that the kernel relies on; "" when
+    // they all hold, else the first violation.
+    //  1. the value pass (germ) is a topological order of every node with an
+    //     op, each once, after node 0;
+    //  2. in a section, ops of different lanes are independent: neither
+    //     descends from the other, they share no pair graph, and no two
+    //     potentials push into a shared input;
+    //  3. the section's rotamer pair-list refresh reads only nodes placed
+    //     before the section;
+    //  4. the ops of a rotamer's overlap range neither descend from the
+    //     rotamer nor (potentials) push into its inputs;
+    //  5. the deriv pass runs a node after every non-potential child's deriv
+    //     op, and an early deriv op's consumers are potentials run before
+    //     the solve (or early deriv ops themselves).
+    string check_program() const {
+        const int nn = (int)T.nodes.size(), ng = n_germ();
+        if (T.germ.empty() || T.germ[0] != 0) return "value pass does not start at node 0";
+        vector<int> pos(nn, -1);
+        for (int q = 0; q <= ng; ++q) {
+            const int i = T.germ[q];
+            if (i < 0 || i >= nn || pos[i] >= 0) return "value pass repeats or leaves the nodes";
+            pos[i] = q;
+        }
+        for (int i = 1; i < nn; ++i) {
+            if (T.nodes[i].op >= 0 && pos[i] < 0) return "node " + T.nodes[i].name + " missing from the value pass";
+            for (int p : T.nodes[i].parents)
+                if (pos[i] >= 0 && (pos[p] < 0 || pos[p] >= pos[i]))
+                    return "node " + T.nodes[i].name + " runs before its input " + T.nodes[p].name;
+        }
+        // transitive ancestry
+        vector<vector<char>> anc(nn, vector<char>(nn, 0));
+        for (int q = 1; q <= ng; ++q) {
+            const int i = T.germ[q];
+            for (int p : T.nodes[i].parents) {
+                anc[i][p] = 1;
+                for (int z = 0; z < nn; ++z) anc[i][z] |= anc[p][z];
+            }
+        }
+        auto share_input = [&](int a, int b) {
+            for (int p : T.nodes[a].parents)
+                for (int q : T.nodes[b].parents)
+                    if (p == q && p != 0) return true;
+            return false;
+        };
+        for (auto& sec : lane_sections) {
+            for (int x = sec.first; x < sec.second; ++x)
+                for (int y = x + 1; y < sec.second; ++y) {
+                    if (prog[x].lane == prog[y].lane) continue;
+                    const int a = prog_node[x], b = prog_node[y];
+                    const string pair = T.nodes[a].name + " / " + T.nodes[b].name;
+                    if (anc[a][b] || anc[b][a]) return "lanes of a section depend on each other: " + pair;
+                    if (prog[x].graph >= 0 && prog[x].graph == prog[y].graph) return "lanes share a pair graph: " + pair;
+                    if (T.nodes[a].potential && T.nodes[b].potential && share_input(a, b))
+                        return "lanes push into a shared input: " + pair;
+                }
+            for (int x = sec.first; x < sec.second; ++x)
+                if (prog[x].lane < 0 || prog[x].lane >= n_lanes) return "lane out of range";
+            const int rgo = prog[sec.first].rot_graph_op;
+            if (rgo >= 0) {
+                const PairGraph& G = T.graphs[prog[rgo].graph];
+                for (int p : T.nodes[prog_node[rgo]].parents)
+                    if ((T.nodes[p].out == G.out1 || T.nodes[p].out == G.out2) && pos[p] - 1 >= sec.first)
+                        return "rotamer pair-list refresh reads " + T.nodes[p].name + " of its own section";
+            }
+        }
+        for (int k = 0; k < ng; ++k) {
+            if (prog[k].kind != OP_ROTAMER || prog[k].overlap_end <= k + 1) continue;
+            const int rn = prog_node[k];
+            for (int j = k + 1; j < prog[k].overlap_end; ++j) {
+                const int x = prog_node[j];
+                if (anc[x][rn]) return "overlap op " + T.nodes[x].name + " descends from the rotamer";
+                if (T.nodes[x].potential)
+                    for (int p : T.nodes[x].parents)
+                        for (int q : T.nodes[rn].parents)
+                            if (p == q) return "overlap potential " + T.nodes[x].name + " pushes into a rotamer input";
+            }
+        }
+        vector<int> dpos(nn, -1);
+        for (size_t d = 0; d < T.deriv.size(); ++d) dpos[T.deriv[d]] = (int)d;
+        int rot_k = -1, rot_end = -1;
+        for (int k = 0; k < ng && rot_k < 0; ++k)
+            if (prog[k].kind == OP_ROTAMER && prog[k].overlap_end > k) { rot_k = k; rot_end = prog[k].overlap_end; }
+        for (size_t d = 0; d < T.deriv.size(); ++d) {
+            const int i = T.deriv[d];
+            for (int ch : T.nodes[i].children) {
+                if (T.nodes[ch].potential) {
+                    if (prog[ng + d].early && !(pos[ch] - 1 < rot_end && pos[ch] - 1 != rot_k))
+                        return "early deriv op " + T.nodes[i].name + " before its consumer " + T.nodes[ch].name;
+                    continue;
+                }
+                if (dpos[ch] < 0 || dpos[ch] >= (int)d)
+                    return "deriv op " + T.nodes[i].name + " runs before its consumer " + T.nodes[ch].name;
+                if (prog[ng + d].early && !prog[ng + dpos[ch]].early)
+                    return "early deriv op " + T.nodes[i].name + " waits on a late one";
+            }
+        }
+        return "";
+    }
 };
 
 struct System {
@@ -4135,6 +4236,35 @@ int upside_batch_set_lane_split(UpsideBatch* b, int split) {
 }
 
 int upside_batch_get_lane_split(UpsideBatch* b) { return b ? b->batch.lane_split : -1; }
+
+int upside_check_program(const char* up_path, char* msg, int msg_len) {
+    string out;
+    int rc = 0;
+    try {
+        if (!up_path) return -1;
+        Program P;
+        P.path = up_path;
+        P.load(0, 1.0, false);
+        out = P.check_program();
+        if (!out.empty()) {
+            rc = 1;
+        } else {
+            int in_sections = 0;
+            for (auto& sec : P.lane_sections) in_sections += sec.second - sec.first;
+            out = std::to_string(P.n_germ()) + " value ops, " + std::to_string(P.n_deriv()) + " deriv ops, " +
+                  std::to_string(P.lane_sections.size()) + " sections of " + std::to_string(P.n_lanes) +
+                  " lanes holding " + std::to_string(in_sections) + " ops";
+        }
+    } catch (std::exception& e) {
+        out = e.what();
+        rc = -1;
+    }
+    if (msg && msg_len > 0) {
+        std::strncpy(msg, out.c_str(), (size_t)msg_len - 1);
+        msg[msg_len - 1] = 0;
+    }
+    return rc;
+}
 
 int upside_plan_team_map(const int* teams, int n_sys, int per_xcd, int* map, int map_len) {
     try {

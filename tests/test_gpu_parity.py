@@ -74,10 +74,10 @@ def oracle_edges_at(ue_eng, ora, name):
 
 
 def dump_case(tag, **arrays):
-    """Save the inputs of a failing comparison for study on the CPU (into
-    gpurun_out/ when that folder exists)."""
-    d = os.path.join(os.environ.get("GRAFT_REPO_ROOT", os.getcwd()), "gpurun_out")
-    if os.path.isdir(d):
+    """Save the inputs of a failing comparison for study on the CPU (into the
+    folder UPSIDE_DUMP_DIR names, when it is set and exists)."""
+    d = os.environ.get("UPSIDE_DUMP_DIR", "")
+    if d and os.path.isdir(d):
         np.savez(os.path.join(d, "case_" + "".join(ch if ch.isalnum() else "_" for ch in tag) + ".npz"), **arrays)
 
 

@@ -1,4 +1,4 @@
-"""Pinning the CPU oracle (oracle/upside_oracle.py) before it judges the GPU.
+"""Pinning the CPU oracle (oracle/upside_oracle.py) before it checks the GPU.
 
 Each check is chosen so that a plausible mistake in the restatement (a dropped
 term, a wrong sign or index, a transposed operand) fails it:

@@ -553,6 +553,8 @@ def main():
     ap.add_argument("--bp-layout", type=int, default=-1,
                     help="rotamer solve layout with --team (0 auto, 1 team, 2 one workgroup, 3 split; "
                          "default: autotuned with the team size, or 0 with --team)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batch workloads and print their lines (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only an elementwise copy of 1 GiB (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -574,6 +576,11 @@ def main():
     rounds = max(1, math.ceil(args.steps / 3))
     warm_rounds = max(1, math.ceil(args.warmup / 3)) if args.warmup > 0 else 0
     work = tempfile.mkdtemp()
+    if args.batched_only:
+        lines, _ = batched_workloads(work, cpu=not args.no_cpu_baseline)
+        shutil.rmtree(work, ignore_errors=True)
+        print(json.dumps({"batched": lines}), flush=True)
+        return
     path = os.path.join(work, "chain%d_seed%d.up" % (N_RES, SEED))
     upconfig.make_chain(path, N_RES, seed=SEED)
     with h5lite.File(path) as f:
